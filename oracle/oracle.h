@@ -1,0 +1,110 @@
+/* Oracle: CPU restatement of the reference's UVS2D Newton hot path.
+ *
+ * TEST INFRASTRUCTURE ONLY.  Only tests/, __graft_entry__.smoke() and bench.py's
+ * cpu_baseline leg may load liboracle.so, and only as the checker of the GPU library
+ * (rismo2d_amd/csrc), never as the thing measured or shipped.
+ *
+ * Every function restates one reference routine (file:line given at each definition in
+ * rismo_oracle.c) with the same floating-point operation order, so that its results can
+ * be compared bit for bit with the reference program's own outputs (see
+ * tests/golden/make_golden.py).  The REALPR-dependent routines exist twice: *_f32 has the
+ * reference's stock semantics (REALPR = float matrix storage, Defs.h:92) and *_f64 the
+ * FP64 semantics (REALPR = double).
+ *
+ * Indices are 0-based.  CSR arrays: rowptr[n+1], cols[nnz] with each row's diagonal first
+ * and the remaining columns in insertion order (IndexMat.cpp:91).
+ */
+#ifndef RMX_ORACLE_H
+#define RMX_ORACLE_H
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+typedef struct orc_model orc_model;
+
+typedef struct {
+  double maxAbs[3];      /* max |dU|, |dV|, |dh| of the Newton correction (Update.cpp:42) */
+  double avAbs[3];
+  double relaxUs, relaxS;
+  int    iters;          /* Krylov iterations */
+  double accuracy;       /* Krylov relative accuracy reached */
+  int    converged;      /* Krylov converged flag */
+  int    nrconv;         /* Newton convergence test of this step */
+  double scale;          /* ||B||_2 before scaling */
+} orc_step_stats;
+
+orc_model* orc_create(int np, const double* x, const double* y, const double* z,
+                      int ne, const int* elems, const int* nnd,
+                      const double* manning, const double* vt,
+                      int n_in, const int* in_segs, int n_out, const int* out_segs,
+                      double Q, double S_in, double S_out);
+void   orc_destroy(orc_model* m);
+/* relaxMethod (0|2), dry method (0|2|3 marsh, else clip), dryLimit, maxDeltaUV, maxDeltaS,
+ * convUV, convS, maxUs */
+void   orc_set_params(orc_model* m, int relaxMethod, int dryMethod, double dryLimit,
+                      double maxDeltaUV, double maxDeltaS, double convUV, double convS,
+                      double maxUs);
+void   orc_set_state(orc_model* m, const double* U, const double* V, const double* S);
+void   orc_get_state(const orc_model* m, double* U, double* V, double* S, double* z);
+void   orc_get_nodal(const orc_model* m, double* cf, int* bckind, int* rot,
+                     double* rotm /* np*4: r00 r01 r10 r11 */, double* bcU, double* bcS);
+
+/* Newton-step preparation (EqsUVS2D.cpp:300-360): friction, boundary conditions, normals,
+ * rotation, and (once) numbering + CSR pattern.  Returns neq. */
+int    orc_prepare(orc_model* m);
+int    orc_neq(const orc_model* m);
+long   orc_nnz(const orc_model* m);
+void   orc_get_numbering(const orc_model* m, int* eqno /* 3*np, -1 = none */);
+void   orc_get_pattern(const orc_model* m, int* rowptr, int* cols);
+int    orc_n_model_elems(const orc_model* m);
+/* model element k: returns nnd; nodes[] gets the node indices; *is_bound set */
+int    orc_model_elem(const orc_model* m, int k, int* nodes, int* is_bound);
+/* element coefficients of model element k (CoefsUVS2D.cpp:49): estifm[24*24], force[24];
+ * returns 0 for a skipped (dry) element */
+int    orc_coefs(orc_model* m, int k, double* estifm, double* force);
+
+void   orc_assemble_f64(orc_model* m, double* vals, double* B);
+void   orc_assemble_f32(orc_model* m, float* vals, double* B);
+
+double orc_scale_f64(int n, const int* rowptr, double* vals, double* B);
+double orc_scale_f32(int n, const int* rowptr, float* vals, double* B);
+int    orc_ilu0_factor_f64(int n, const int* rowptr, const int* cols, const double* A, double* ILU);
+int    orc_ilu0_factor_f32(int n, const int* rowptr, const int* cols, const float* A, float* ILU);
+void   orc_ilu0_solve_f64(int n, const int* rowptr, const int* cols, const double* ILU,
+                          const double* B, double* X);
+void   orc_ilu0_solve_f32(int n, const int* rowptr, const int* cols, const float* ILU,
+                          const double* B, double* X);
+void   orc_mulvec_f64(int n, const int* rowptr, const int* cols, const double* A,
+                      const double* x, double* r);
+void   orc_mulvec_f32(int n, const int* rowptr, const int* cols, const float* A,
+                      const double* x, double* r);
+int    orc_bicgstab_f64(int n, const int* rowptr, const int* cols, const double* A,
+                        const double* ILU, const double* B, double* X, int maxIter, double eps,
+                        int* iters, double* accuracy);
+int    orc_bicgstab_f32(int n, const int* rowptr, const int* cols, const float* A,
+                        const float* ILU, const double* B, double* X, int maxIter, double eps,
+                        int* iters, double* accuracy);
+int    orc_fgmres_f64(int n, const int* rowptr, const int* cols, const double* A,
+                      const double* ILU, const double* B, double* X, int mkyrl, int maxIter,
+                      double eps, int* iters, double* accuracy);
+int    orc_fgmres_f32(int n, const int* rowptr, const int* cols, const float* A,
+                      const float* ILU, const double* B, double* X, int mkyrl, int maxIter,
+                      double eps, int* iters, double* accuracy);
+
+/* One Newton step (EqsUVS2D.cpp:189-760): prepare, assemble, scale, ILU(0), Krylov,
+ * statistics, relaxation, correction, midside interpolation.  prec: 32 | 64.
+ * solver: 1 = P_BCGSTABD, 2 = P_FGMRESD (with the Hessenberg assignment the serial
+ * reference omits).  X_out (nullable) receives the Newton correction. */
+int    orc_newton_step(orc_model* m, int prec, int solver, int maxIter, double eps, int mkyrl,
+                       orc_step_stats* st, double* X_out);
+
+/* shape tables at GPdeg 5 (Shape.cpp:138-395): kind 0 line3, 1 tri6, 2 quad8.
+ * f, dfdx, dfdy: [ngp*nnd]; lf, ldfdx, ldfdy: linear shape [ngp*ncn]; w[ngp] */
+int    orc_shape(int kind, double* f, double* dfdx, double* dfdy, double* lf, double* ldfdx,
+                 double* ldfdy, double* w);
+
+#ifdef __cplusplus
+}
+#endif
+#endif

@@ -1,0 +1,344 @@
+"""Synthetic Taylor-Hood meshes, boundary lines and flow parameters for the UVS2D solve.
+
+A *deck* is the input of one steady Newton solve of the depth-averaged shallow-water
+system: a P2/P1 (Taylor-Hood) triangle or serendipity-quad mesh, the control lines that
+carry the inlet discharge (QINLET) and the outlet water level (OUTLET), per-element
+roughness and eddy viscosity, and the initial state.  The same deck feeds the CPU oracle,
+the GPU library and, in the oracle container, the reference program itself
+(``write_reference_deck`` writes the reference's keyword files: region file
+``Grid.cpp:137``, control file ``Grid.cpp:378``, initial file ``Grid.cpp:510``, project
+file ``Project.cpp:1296``, time-step file ``Timeint.cpp:1466``, material file
+``Type.cpp:130``).
+
+Node and element conventions follow the reference (``Shape.cpp``/``Triangle.cpp``):
+triangles list corners counter-clockwise then the midsides 01, 12, 20; quads list
+corners then midsides 01, 12, 23, 30.  All indices here are 0-based; the reference's
+files are 1-based.
+"""
+from __future__ import annotations
+
+import dataclasses
+import math
+import os
+
+import numpy as np
+
+INLET_LINE = 1
+OUTLET_LINE = 2
+
+
+@dataclasses.dataclass
+class Deck:
+    x: np.ndarray            # [np] float64
+    y: np.ndarray
+    z: np.ndarray            # bed elevation (zor)
+    elems: np.ndarray        # [ne, 8] int32, -1 padded
+    nnd: np.ndarray          # [ne] int32 (6 triangle, 8 quad)
+    manning: np.ndarray      # [ne] float64, Manning n of each element
+    vt: np.ndarray           # [ne] float64, constant eddy viscosity of each element
+    lines: dict              # line no -> int32 [nseg, 3] (corner, corner, midside)
+    U0: np.ndarray           # initial state [np]
+    V0: np.ndarray
+    S0: np.ndarray
+    Q_inlet: float = 100.0   # QINLET discharge (m3/s)
+    S_inlet: float = 2.1     # QINLET reference water level used for the q distribution
+    S_outlet: float = 2.0    # OUTLET water level
+    name: str = "deck"
+
+    @property
+    def n_nodes(self) -> int:
+        return int(self.x.shape[0])
+
+    @property
+    def n_elems(self) -> int:
+        return int(self.elems.shape[0])
+
+
+# ---------------------------------------------------------------------------------------
+# bathymetry noise
+
+def gaussian_random_field(xs: np.ndarray, ys: np.ndarray, sigma: float, corr: float,
+                          seed: int) -> np.ndarray:
+    """Seeded stationary Gaussian field with covariance sigma^2 exp(-r^2 / (2 corr^2)).
+
+    Spectral synthesis on a regular grid (xs along axis 1, ys along axis 0) that is
+    zero-padded by 4 correlation lengths on every side so that the periodic wrap does not
+    correlate opposite boundaries.  Returns an array of shape (len(ys), len(xs)).
+    """
+    if sigma == 0.0:
+        return np.zeros((len(ys), len(xs)))
+    dx = float(xs[1] - xs[0]) if len(xs) > 1 else 1.0
+    dy = float(ys[1] - ys[0]) if len(ys) > 1 else 1.0
+    pad_x = int(math.ceil(4.0 * corr / dx))
+    pad_y = int(math.ceil(4.0 * corr / dy))
+    nx = len(xs) + 2 * pad_x
+    ny = len(ys) + 2 * pad_y
+    rng = np.random.default_rng(seed)
+    white = rng.standard_normal((ny, nx))
+    kx = 2.0 * np.pi * np.fft.fftfreq(nx, d=dx)
+    ky = 2.0 * np.pi * np.fft.fftfreq(ny, d=dy)
+    k2 = ky[:, None] ** 2 + kx[None, :] ** 2
+    # amplitude = sqrt(spectral density) of the Gaussian covariance, on the discrete grid
+    amp = np.sqrt(2.0 * np.pi * corr * corr * np.exp(-0.5 * corr * corr * k2) / (dx * dy))
+    field = np.real(np.fft.ifft2(np.fft.fft2(white) * amp))
+    field = field[pad_y:pad_y + len(ys), pad_x:pad_x + len(xs)]
+    std = field.std()
+    if std > 0:
+        field *= sigma / std
+    return field
+
+
+# ---------------------------------------------------------------------------------------
+# structured channel (configs: straight channel 1000 m x 100 m, 200 x 20 cells)
+
+def channel(nx: int = 200, ny: int = 20, length: float = 1000.0, width: float = 100.0,
+            slope: float = 1.0e-4, z_out: float = 0.0, noise_sigma: float = 0.05,
+            noise_corr: float = 20.0, seed: int = 0, manning: float = 0.03,
+            vt: float = 0.1, Q: float = 100.0, S_out: float = 2.0, S_in: float = 2.1,
+            U_init: float = 0.5, quads: bool = False, name: str | None = None) -> Deck:
+    """Straight rectangular channel on a (2nx+1) x (2ny+1) lattice.
+
+    Each cell becomes two six-node triangles split along its (i,j)->(i+2,j+2) diagonal,
+    or, with ``quads``, one eight-node serendipity quad (cell centres are then unused and
+    dropped).  Elements are listed column by column along x with the front across the
+    width (the order that a front-width minimisation gives for a channel).  Bed:
+    ``z = z_out + slope*(length - x) + GRF``; midside nodes carry the mean of their
+    corners so the bed is piecewise linear.  Inlet line at x=0, outlet line at x=length.
+    """
+    NX, NY = 2 * nx + 1, 2 * ny + 1
+    xs = np.linspace(0.0, length, NX)
+    ys = np.linspace(0.0, width, NY)
+    X, Y = np.meshgrid(xs, ys)                        # [NY, NX]
+    Zc = z_out + slope * (length - X) + gaussian_random_field(xs, ys, noise_sigma,
+                                                              noise_corr, seed)
+    # piecewise-linear bed: odd lattice lines take the mean of their neighbours
+    Z = Zc.copy()
+    Z[:, 1::2] = 0.5 * (Zc[:, 0:-1:2] + Zc[:, 2::2])
+    Z[1::2, :] = 0.5 * (Z[0:-1:2, :] + Z[2::2, :])
+    if not quads:
+        # the diagonal midside (cell centre) is the mean of the diagonal corners
+        Z[1::2, 1::2] = 0.5 * (Zc[0:-1:2, 0:-1:2] + Zc[2::2, 2::2])
+
+    def nid(i, j):
+        return j * NX + i
+
+    elems = []
+    for i in range(nx):
+        for j in range(ny):
+            i0, j0 = 2 * i, 2 * j
+            a, b = nid(i0, j0), nid(i0 + 2, j0)
+            c, d = nid(i0 + 2, j0 + 2), nid(i0, j0 + 2)
+            if quads:
+                elems.append([a, b, c, d, nid(i0 + 1, j0), nid(i0 + 2, j0 + 1),
+                              nid(i0 + 1, j0 + 2), nid(i0, j0 + 1)])
+            else:
+                m = nid(i0 + 1, j0 + 1)
+                elems.append([a, b, c, nid(i0 + 1, j0), nid(i0 + 2, j0 + 1), m, -1, -1])
+                elems.append([a, c, d, m, nid(i0 + 1, j0 + 2), nid(i0, j0 + 1), -1, -1])
+    elems = np.asarray(elems, dtype=np.int64)
+    x, y, z = X.ravel(), Y.ravel(), Z.ravel()
+    inlet = [[nid(0, 2 * j), nid(0, 2 * j + 2), nid(0, 2 * j + 1)] for j in range(ny)]
+    outlet = [[nid(2 * nx, 2 * j), nid(2 * nx, 2 * j + 2), nid(2 * nx, 2 * j + 1)]
+              for j in range(ny)]
+    lines = {INLET_LINE: np.asarray(inlet, np.int64), OUTLET_LINE: np.asarray(outlet, np.int64)}
+    if quads:
+        used = np.zeros(x.shape[0], bool)
+        used[elems.ravel()] = True
+        remap = -np.ones(x.shape[0], np.int64)
+        remap[used] = np.arange(int(used.sum()))
+        elems = remap[elems]
+        lines = {k: remap[v] for k, v in lines.items()}
+        x, y, z = x[used], y[used], z[used]
+    nnd = np.full(elems.shape[0], 8 if quads else 6, np.int32)
+    ne = elems.shape[0]
+    npn = x.shape[0]
+    return Deck(x=x.astype(np.float64), y=y.astype(np.float64), z=z.astype(np.float64),
+                elems=elems.astype(np.int32), nnd=nnd,
+                manning=np.full(ne, manning), vt=np.full(ne, vt),
+                lines={k: v.astype(np.int32) for k, v in lines.items()},
+                U0=np.full(npn, U_init), V0=np.zeros(npn), S0=np.full(npn, S_out),
+                Q_inlet=Q, S_inlet=S_in, S_outlet=S_out,
+                name=name or f"channel_{nx}x{ny}{'_quad' if quads else ''}")
+
+
+# ---------------------------------------------------------------------------------------
+# meandering channel with floodplains (GPU-scale workload)
+
+def meander(n_along: int = 3200, n_across: int = 320, reach: float = 10000.0,
+            amplitude: float = 300.0, wavelength: float = 2000.0, width: float = 900.0,
+            main_width: float = 120.0, main_depth: float = 3.0, bank_slope: float = 0.2,
+            slope: float = 2.0e-4, noise_sigma: float = 0.5, noise_corr: float = 50.0,
+            jitter: float = 0.2, seed: int = 0, n_range=(0.025, 0.045), vt: float = 0.1,
+            Q: float = 600.0, S_out: float = 4.2, U_init: float = 0.4,
+            name: str | None = None) -> Deck:
+    """Meandering channel with floodplains on a curvilinear, jittered Taylor-Hood mesh.
+
+    Centreline ``y_c = amplitude*sin(2*pi*s/wavelength)`` over ``reach`` metres.  The
+    cross-section is a trapezoidal main channel (``main_width`` wide at the bed,
+    ``main_depth`` deep, banks of slope ``bank_slope``) cut into floodplains that span
+    ``width`` in total.  The lattice is mapped along the centreline (normal offsets) and
+    interior corner nodes are jittered by ``jitter`` of the local spacing, so element
+    shapes and sizes vary (unstructured connectivity is not needed for the solver's cost
+    model: row widths and the frontal ordering depend on the element adjacency, which is
+    that of a triangulated lattice).  The diagonal of every cell alternates, giving
+    vertices of degree 4, 6 and 8.  Bathymetry adds a seeded Gaussian random field;
+    Manning n is drawn per element from ``U[n_range]``.  ``n_along*n_across*2`` triangles.
+    """
+    nx, ny = n_along, n_across
+    NX, NY = 2 * nx + 1, 2 * ny + 1
+    rng = np.random.default_rng(seed + 1)
+    s = np.linspace(0.0, reach, NX)
+    t = np.linspace(-0.5 * width, 0.5 * width, NY)
+    yc = amplitude * np.sin(2.0 * np.pi * s / wavelength)
+    dyc = amplitude * 2.0 * np.pi / wavelength * np.cos(2.0 * np.pi * s / wavelength)
+    nrm = np.sqrt(1.0 + dyc * dyc)
+    tx, ty = 1.0 / nrm, dyc / nrm                  # unit tangent
+    ox, oy = -ty, tx                               # unit normal (left)
+    S_, T_ = np.meshgrid(np.arange(NX), np.arange(NY))   # lattice indices [NY, NX]
+    # jitter interior corner nodes in lattice units (corners are even lattice indices)
+    js = np.zeros((NY, NX))
+    jt = np.zeros((NY, NX))
+    corner = (S_ % 2 == 0) & (T_ % 2 == 0)
+    interior = (S_ > 0) & (S_ < NX - 1) & (T_ > 0) & (T_ < NY - 1)
+    sel = corner & interior
+    js[sel] = rng.uniform(-jitter, jitter, int(sel.sum())) * 2.0
+    jt[sel] = rng.uniform(-jitter, jitter, int(sel.sum())) * 2.0
+    ds = s[1] - s[0]
+    dt_ = t[1] - t[0]
+    sc = S_ * ds + js * ds
+    tc = (T_ - (NY - 1) / 2.0) * dt_ + jt * dt_
+    # corner positions
+    cx = np.interp(sc, s, s) + 0.0
+    xc_s = np.interp(sc, s, s)
+    yc_s = np.interp(sc, s, yc)
+    oxs = np.interp(sc, s, ox)
+    oys = np.interp(sc, s, oy)
+    Xc = xc_s + tc * oxs
+    Yc = yc_s + tc * oys
+    del cx
+    # bed: valley slope + trapezoidal section + GRF, evaluated at corner positions
+    half_bed = 0.5 * main_width
+    dist = np.abs(tc)
+    bank = np.clip((dist - half_bed) * bank_slope, 0.0, main_depth)
+    zsec = -main_depth + bank
+    grf = gaussian_random_field(np.arange(NX) * ds, np.arange(NY) * dt_, noise_sigma,
+                                noise_corr, seed)
+    Zc = slope * (reach - sc) + zsec + grf
+    X = Xc.copy()
+    Y = Yc.copy()
+    Z = Zc.copy()
+
+    # midside nodes are the straight-edge midpoints of their corners
+    def mid(A, i0, j0, i1, j1):
+        return 0.5 * (A[j0, i0] + A[j1, i1])
+
+    # horizontal edge midsides (odd i, even j)
+    for A in (X, Y, Z):
+        A[0::2, 1::2] = 0.5 * (A[0::2, 0:-1:2] + A[0::2, 2::2])
+        A[1::2, 0::2] = 0.5 * (A[0:-1:2, 0::2] + A[2::2, 0::2])
+    # cell centres are diagonal midsides; the diagonal alternates per cell
+    ci = np.arange(nx)
+    cj = np.arange(ny)
+    CI, CJ = np.meshgrid(ci, cj)                  # [ny, nx]
+    flip = ((CI + CJ) % 2) == 1                   # diagonal (i,j+2)->(i+2,j) when flipped
+    for A in (X, Y, Z):
+        a = A[0:-1:2, 0:-1:2]
+        b = A[0:-1:2, 2::2]
+        c = A[2::2, 2::2]
+        d = A[2::2, 0:-1:2]
+        A[1::2, 1::2] = np.where(flip, 0.5 * (b + d), 0.5 * (a + c))
+    del mid
+
+    base = np.arange(NX * NY).reshape(NY, NX)
+    i0 = 2 * CI.T.ravel()                          # column-major: along x outer
+    j0 = 2 * CJ.T.ravel()
+    fl = flip.T.ravel()
+    a = base[j0, i0]
+    b = base[j0, i0 + 2]
+    c = base[j0 + 2, i0 + 2]
+    d = base[j0 + 2, i0]
+    mab = base[j0, i0 + 1]
+    mbc = base[j0 + 1, i0 + 2]
+    mcd = base[j0 + 2, i0 + 1]
+    mda = base[j0 + 1, i0]
+    m = base[j0 + 1, i0 + 1]
+    neg = -np.ones_like(a)
+    # unflipped: (a,b,c) + (a,c,d); flipped: (a,b,d) + (b,c,d)
+    t1 = np.where(fl[:, None], np.stack([a, b, d, mab, m, mda, neg, neg], 1),
+                  np.stack([a, b, c, mab, mbc, m, neg, neg], 1))
+    t2 = np.where(fl[:, None], np.stack([b, c, d, mbc, mcd, m, neg, neg], 1),
+                  np.stack([a, c, d, m, mcd, mda, neg, neg], 1))
+    elems = np.empty((2 * t1.shape[0], 8), np.int64)
+    elems[0::2] = t1
+    elems[1::2] = t2
+    ne = elems.shape[0]
+    inlet = np.stack([base[0:-1:2, 0], base[2::2, 0], base[1::2, 0]], 1)
+    outlet = np.stack([base[0:-1:2, -1], base[2::2, -1], base[1::2, -1]], 1)
+    npn = NX * NY
+    lo, hi = n_range
+    man = np.random.default_rng(seed + 2).uniform(lo, hi, ne)
+    return Deck(x=X.ravel().astype(np.float64), y=Y.ravel().astype(np.float64),
+                z=Z.ravel().astype(np.float64), elems=elems.astype(np.int32),
+                nnd=np.full(ne, 6, np.int32), manning=man, vt=np.full(ne, vt),
+                lines={INLET_LINE: inlet.astype(np.int32), OUTLET_LINE: outlet.astype(np.int32)},
+                U0=np.full(npn, U_init), V0=np.zeros(npn), S0=np.full(npn, S_out),
+                Q_inlet=Q, S_inlet=S_out + slope * reach, S_outlet=S_out,
+                name=name or f"meander_{nx}x{ny}")
+
+
+# ---------------------------------------------------------------------------------------
+# reference-format writer (used only by the oracle harness in the oracle container)
+
+def write_reference_deck(deck: Deck, outdir: str, solver: str = "1 6 1 0 60 1000 1.0e-10",
+                         fgmres_solver: str = "2 7 1 0 60 30 1000 1.0e-10",
+                         tm_solver: int = 1, maxiter: int = 12,
+                         convergence: str = "1.0e-7 1.0e-7 1.0e-4 1.0e-4",
+                         relax: str = "2 0.001 0.1 10.0 10.0 0.1",
+                         dryrew: str = "2 1000 0.001 0.005 100 5") -> None:
+    """Write the reference's keyword input files for ``deck`` into ``outdir``.
+
+    One material per distinct (Manning n, vt) pair.  Values are written with %.17g so
+    that the reference reads back the exact doubles used by the oracle.
+    """
+    os.makedirs(outdir, exist_ok=True)
+    pairs = sorted(set(zip(deck.manning.tolist(), deck.vt.tolist())))
+    mat_of = {p: i + 1 for i, p in enumerate(pairs)}
+    with open(os.path.join(outdir, "region.dat"), "w") as f:
+        f.write(f"{deck.n_nodes} {deck.n_elems} 0\n")
+        for i in range(deck.n_nodes):
+            f.write(f"{i + 1} {deck.x[i]:.17g} {deck.y[i]:.17g} {deck.z[i]:.17g}\n")
+        for e in range(deck.n_elems):
+            k = int(deck.nnd[e])
+            mat = mat_of[(float(deck.manning[e]), float(deck.vt[e]))]
+            shape = "tri" if k == 6 else "quad"
+            ids = " ".join(str(int(v) + 1) for v in deck.elems[e, :k])
+            f.write(f"{e + 1} {mat} {shape} {ids}\n")
+    segs = [(no, s) for no in sorted(deck.lines) for s in deck.lines[no]]
+    with open(os.path.join(outdir, "control.dat"), "w") as f:
+        f.write(f"0 {len(segs)}\n")
+        for k, (no, s) in enumerate(segs):
+            f.write(f"{k + 1} {no} line {int(s[0]) + 1} {int(s[1]) + 1} {int(s[2]) + 1}\n")
+    with open(os.path.join(outdir, "init.dat"), "w") as f:
+        f.write("# 0.0 x 40000 U V S\n")
+        f.write(f"{deck.n_nodes} 0\n")
+        for i in range(deck.n_nodes):
+            f.write(f"{i + 1} {deck.U0[i]:.17g} {deck.V0[i]:.17g} {deck.S0[i]:.17g}\n")
+    with open(os.path.join(outdir, "mat.dat"), "w") as f:
+        f.write("$RISMO2D 40000\n")
+        for (n, vt), no in mat_of.items():
+            f.write(f"$ROUGHNESS {no} 5 {n:.17g} 0 0.0 0.0 0.0\n")
+        for (n, vt), no in mat_of.items():
+            f.write(f"$TURBULENCE {no} {vt:.17g} 0.0 0.0 0.0 0.0 0.0 0.0\n")
+    with open(os.path.join(outdir, "tm.dat"), "w") as f:
+        f.write("$RISMO2D 40100\n$TM_STEPS 1 1 0\n$TM_INTERVAL 1000000.0 0.0 0.0\n"
+                "$TM_WEIGHT 1.0 1.0 1.0\n$TM_STEP_NO 1\n$TM_CYCLE 1\n$TM_STATIONARY 1\n"
+                f"$TM_TURBULENCE 1\n$TM_MAXITER {maxiter}\n$TM_SOLVER {tm_solver}\n"
+                f"$TM_LINE QINLET {INLET_LINE} {deck.Q_inlet:.17g} {deck.S_inlet:.17g}\n"
+                f"$TM_LINE OUTLET {OUTLET_LINE} {deck.S_outlet:.17g} 1000.0 0.0 0 0.0\n")
+    with open(os.path.join(outdir, "input.dat"), "w") as f:
+        f.write("$RISMO2D 40700\n$REPORTLEVEL 3\n$REPORTFILE report.txt\n"
+                "$TIMESTEPFILE tm.dat\n$MATERIALFILE mat.dat\n$REGIONFILE region.dat\n"
+                "$CONTROLFILE control.dat\n$ASC_INITFILE init.dat\n"
+                f"$SOLVER {solver}\n$SOLVER {fgmres_solver}\n"
+                f"$CONVERGENCE {convergence}\n$RELAX {relax}\n$DRYREW {dryrew}\n"
+                "$GPDEGREE 5\n")

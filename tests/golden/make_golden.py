@@ -1,0 +1,129 @@
+"""Generate the golden vectors of tests/golden/*.npz from the reference program.
+
+Runs only in the oracle container (the reference sources must be present under
+/root/reference/sources).  It builds oracle/_ref/rismo_harness (oracle/Makefile: the
+reference's own objects, unmodified, plus oracle/ref_harness.cpp), writes each deck in
+the reference's input format, runs the reference and stores what the harness recorded:
+
+* ``tri20x4.npz``   6-node triangles, 20 x 4 cells: every Newton step's node state,
+                    numbering, CSR pattern, assembled FP32 matrix and RHS, scale and
+                    correction; step 0's element matrices, scaled system, ILU(0) factors,
+                    ILU apply and MulVec of the scaled RHS; the final state.
+* ``quad10x3.npz``  8-node quads, 10 x 3 cells, same records.
+* ``channel200x20.npz``  the straight-channel workload (200 x 20 cells, 8,000 triangles,
+                    bed noise sigma 0.05 m / 20 m, seed 0): numbering, per-step scale and
+                    correction, BiCGStab iteration counts from the report, final state,
+                    and a SHA-256 of the step-0 pattern.
+
+The reference ships no test vectors of its own; these are the only golden data.
+Usage:  python tests/golden/make_golden.py
+"""
+from __future__ import annotations
+
+import hashlib
+import os
+import re
+import subprocess
+import sys
+import tempfile
+
+import numpy as np
+
+HERE = os.path.dirname(os.path.abspath(__file__))
+ROOT = os.path.dirname(os.path.dirname(HERE))
+sys.path.insert(0, ROOT)
+sys.path.insert(0, HERE)
+
+from rismo2d_amd import deck as dk  # noqa: E402
+from refdump import read_dump  # noqa: E402
+
+REF = "/root/reference/sources"
+
+DECKS = {
+    "tri20x4": dict(nx=20, ny=4, length=100.0, width=20.0, noise_sigma=0.05, noise_corr=5.0,
+                    Q=10.0),
+    "quad10x3": dict(nx=10, ny=3, length=100.0, width=30.0, noise_sigma=0.05, noise_corr=5.0,
+                     Q=10.0, quads=True),
+    "channel200x20": dict(nx=200, ny=20),
+}
+
+
+def run_reference(deck, workdir):
+    dk.write_reference_deck(deck, workdir)
+    env = dict(os.environ, RMX_DUMP=os.path.join(workdir, "dump.bin"), RMX_DUMP_ELEM_STEP="0",
+               RMX_DUMP_ILU_STEP="0")
+    exe = os.path.join(ROOT, "oracle", "_ref", "rismo_harness")
+    subprocess.run([exe, "input.dat"], cwd=workdir, env=env, check=True,
+                   stdout=subprocess.DEVNULL, stderr=subprocess.DEVNULL, timeout=600)
+    report = open(os.path.join(workdir, "report.txt")).read()
+    iters = [int(v) for v in re.findall(r"P_BCGSTABD::Iterate\)\s+(\d+)\. Iteration", report)]
+    return read_dump(os.path.join(workdir, "dump.bin")), iters
+
+
+def deck_arrays(d):
+    return dict(x=d.x, y=d.y, z=d.z, elems=d.elems, nnd=d.nnd, manning=d.manning, vt=d.vt,
+                inlet=d.lines[dk.INLET_LINE], outlet=d.lines[dk.OUTLET_LINE], U0=d.U0, V0=d.V0,
+                S0=d.S0, Q=d.Q_inlet, S_in=d.S_inlet, S_out=d.S_outlet)
+
+
+def state_arrays(prefix, st):
+    out = {}
+    for k in ("U", "V", "S", "z", "cf", "kind", "rot", "bcU", "bcS", "r00", "r01", "r10", "r11",
+              "eqU", "eqV", "eqS", "nx", "ny", "niox", "nioy"):
+        out[f"{prefix}{k}"] = st[k]
+    return out
+
+
+def main():
+    subprocess.run(["make", "-s", "-C", os.path.join(ROOT, "oracle"), "ref", f"REF={REF}"],
+                   check=True)
+    for name, kw in DECKS.items():
+        d = dk.channel(**kw)
+        with tempfile.TemporaryDirectory() as tmp:
+            (steps, final), iters = run_reference(d, tmp)
+        out = deck_arrays(d)
+        out["nsteps"] = len(steps)
+        out["bicgstab_iters"] = np.asarray(iters, np.int64)
+        full = name != "channel200x20"
+        for k, st in enumerate(steps):
+            p = f"s{k}_"
+            out[p + "scale"] = st["scale"]
+            out[p + "xsol"] = st["xsol"]
+            if full:
+                out.update(state_arrays(p, st["state"]))
+                out[p + "width"] = st["width"]
+                out[p + "cols"] = st["cols"]
+                out[p + "aval"] = st["aval"]
+                out[p + "bvec"] = st["bvec"]
+            elif k == 0:
+                out.update(state_arrays(p, st["state"]))
+                h = hashlib.sha256(st["width"].tobytes() + st["cols"].tobytes()).hexdigest()
+                out["s0_pattern_sha256"] = h
+                out["s0_nnz"] = int(st["nnz"])
+                out["s0_bvec"] = st["bvec"]
+        s0 = steps[0]
+        if full:
+            ne = len(s0["elems"])
+            out["el_names"] = np.stack([np.pad(e["names"], (0, 8 - e["nnd"]), constant_values=0)
+                                        for e in s0["elems"]])
+            out["el_nnd"] = np.asarray([e["nnd"] for e in s0["elems"]])
+            out["el_bound"] = np.asarray([e["bound"] for e in s0["elems"]])
+            K = np.zeros((ne, 24, 24))
+            F = np.zeros((ne, 24))
+            for i, e in enumerate(s0["elems"]):
+                nd = 3 * e["nnd"]
+                K[i, :nd, :nd] = e["estifm"]
+                F[i, :nd] = e["force"]
+            out["el_estifm"] = K
+            out["el_force"] = F
+            for k in ("ascaled", "bscaled", "ilu", "iluapply", "mulvec"):
+                out["s0_" + k] = s0[k]
+        out.update(state_arrays("final_", final))
+        path = os.path.join(HERE, f"{name}.npz")
+        np.savez_compressed(path, **out)
+        print(f"{name}: {len(steps)} Newton steps, BiCGStab iterations {iters}, "
+              f"{os.path.getsize(path) / 1e6:.2f} MB")
+
+
+if __name__ == "__main__":
+    main()
