@@ -1,0 +1,190 @@
+/* rmx: MI355X (gfx950) library for the Rismo2D UVS2D flow-solve hot path.
+ *
+ * C ABI, plain pointers and sizes.  Every entry point returns an rmx_status (0 = ok,
+ * < 0 = error; rmx_last_error() gives the message of the calling thread's last error).
+ * A context owns one HIP stream and all device state of one equation system; calls on
+ * one context must not overlap (the reference is single-threaded as well).  Host
+ * pointers are caller-owned and only read or written during the call.
+ *
+ * Two groups of entry points:
+ *  - the drop-in group replaces the reference's EQS::Coefs + CRSMAT assembly, scaling
+ *    and MulVec, PRECON::Factor / Solve and SOLVER::Iterate (cited at each function);
+ *  - the model group is the host-side set-up the reference does in EQS_UVS2D::Execute
+ *    around EQS::Solve (friction, boundary conditions, numbering, pattern, Newton update),
+ *    used by callers that do not keep the reference's own host code.
+ */
+#ifndef RMX_H
+#define RMX_H
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+typedef struct rmx_ctx rmx_ctx;
+typedef int rmx_status;
+
+enum {
+  RMX_OK = 0,
+  RMX_ERR_ARG = -1,       /* invalid argument or call order */
+  RMX_ERR_HIP = -2,       /* HIP runtime error */
+  RMX_ERR_PIVOT = -3,     /* ILU(0) met a pivot below 1e-20 (Preco_ilu0.cpp:95) */
+  RMX_ERR_NODEVICE = -4,  /* no gfx950 device */
+  RMX_ERR_RANGE = -5      /* size outside what the kernels support (row width > 128) */
+};
+
+enum { RMX_F64 = 64, RMX_F32 = 32 };           /* matrix / factor storage precision */
+enum { RMX_BICGSTAB = 1, RMX_FGMRES = 2 };      /* Krylov method (SOLVER types) */
+enum { RMX_PREC_ILU0 = 1, RMX_PREC_NONE = 0, RMX_PREC_MPNN = 2 };
+
+const char* rmx_last_error(void);
+
+/* Create a context on HIP device `device` (must be gfx950). */
+rmx_status rmx_create(int device, rmx_ctx** out);
+rmx_status rmx_destroy(rmx_ctx* ctx);
+
+/* ------------------------------------------------------------------------------------ */
+/* model group                                                                           */
+
+typedef struct {
+  int np;                     /* region nodes */
+  const double *x, *y, *z;    /* coordinates and bed elevation (zor) */
+  int ne;                     /* region elements */
+  const int* elems;           /* [ne*8], 0-based, -1 padded; tri: c0 c1 c2 m01 m12 m20 */
+  const int* nnd;             /* [ne] 6 (triangle) or 8 (quad) */
+  const double* manning;      /* [ne] Manning n */
+  const double* vt;           /* [ne] constant eddy viscosity */
+  int n_inlet;                /* QINLET control-line segments */
+  const int* inlet_segs;      /* [n_inlet*3] corner, corner, midside */
+  int n_outlet;               /* OUTLET control-line segments */
+  const int* outlet_segs;
+  double Q, S_in, S_out;      /* inlet discharge, inlet reference level, outlet level */
+} rmx_mesh_desc;
+
+typedef struct {
+  int relax_method;           /* 0: none, 2: cap by max_delta_uv / max_delta_s */
+  int dry_method;             /* 0,2,3: marsh bed lowering; else depth clip */
+  double dry_limit, max_delta_uv, max_delta_s, conv_uv, conv_s, max_us;
+} rmx_params;
+
+/* Mesh, boundary lines and material data (MODEL::Input, MODEL::Initialize,
+ * Model.cpp:121-215; GRID::AreaFactors, ArFact.cpp:41).  Builds the boundary line
+ * elements and the model element list on the host and uploads node/element arrays. */
+rmx_status rmx_mesh_upload(rmx_ctx* ctx, const rmx_mesh_desc* mesh);
+rmx_status rmx_set_params(rmx_ctx* ctx, const rmx_params* p);
+rmx_status rmx_state_upload(rmx_ctx* ctx, const double* U, const double* V, const double* S);
+rmx_status rmx_state_download(rmx_ctx* ctx, double* U, double* V, double* S, double* z);
+
+/* Per Newton step set-up (EqsUVS2D.cpp:300-360): nodal friction (MODEL::DoFriction) and
+ * inlet discharge distribution (BCONSET::InitBcon) on the device; boundary kinds,
+ * normals, rotations (MODEL::SetLocation/SetNormal/SetRotation), equation numbering
+ * (EQS::SetEqno), CSR pattern (EQS::SetIndexMat), element colouring and ILU level sets on
+ * the host on the first call.  Returns the number of equations and non-zeros. */
+rmx_status rmx_prepare(rmx_ctx* ctx, int* neq, long long* nnz);
+rmx_status rmx_get_numbering(rmx_ctx* ctx, int* eqno /* [3*np] */);
+rmx_status rmx_get_pattern(rmx_ctx* ctx, int* rowptr /* [neq+1] */, int* cols /* [nnz] */);
+rmx_status rmx_get_nodal(rmx_ctx* ctx, double* cf, double* bcU, int* kind, double* rotm);
+rmx_status rmx_n_model_elems(rmx_ctx* ctx, int* n);
+
+/* ------------------------------------------------------------------------------------ */
+/* drop-in group                                                                         */
+
+/* Element coefficients of model elements [first, first+count): replaces EQS::Coefs /
+ * EQS_UVS2D::Region + Bound + Rotate2D + inlet rows (CoefsUVS2D.cpp:49-1181).
+ * estifm: [count*24*24] row-major per element, force: [count*24]; layout of the
+ * reference (row i + j*nnd, j = 0 U, 1 V, 2 H). */
+rmx_status rmx_element_coefs(rmx_ctx* ctx, int first, int count, double* estifm, double* force);
+
+/* Assemble the Newton system on the device: replaces CRSMAT::AssembleEqs_im
+ * (Assemble.cpp:195) with EQS_UVS2D::Coefs.  prec RMX_F64: FP64 values; RMX_F32: FP32
+ * values with each element entry rounded to FP32 before it is added (the reference's
+ * stock REALPR = float semantics).  The RHS B is FP64 either way. */
+rmx_status rmx_assemble(rmx_ctx* ctx, int prec);
+
+/* B /= ||B||_2 and A /= ||B||_2: replaces CRSMAT::ScaleL2Norm (CRSMat.cpp:319). */
+rmx_status rmx_scale_l2(rmx_ctx* ctx, double* scale_out);
+
+/* Copy the FP32 matrix to or from the FP64 matrix (rounding to nearest). */
+rmx_status rmx_convert_matrix(rmx_ctx* ctx, int to_prec);
+
+/* ILU(0) on the matrix pattern: replaces PRECO_ILU0::Factor (Preco_ilu0.cpp:64).
+ * prec selects which matrix (and factor) storage to use. */
+rmx_status rmx_ilu0_factor(rmx_ctx* ctx, int prec);
+
+/* x = U^-1 L^-1 b with the ILU(0) factors: replaces PRECO_ILU0::Solve
+ * (Preco_ilu0.cpp:497).  Host pointers, length neq. */
+rmx_status rmx_ilu0_apply(rmx_ctx* ctx, int prec, const double* b, double* x);
+
+/* r = A x: replaces CRSMAT::MulVec (CRSMat.cpp:466).  Host pointers. */
+rmx_status rmx_spmv(rmx_ctx* ctx, int prec, const double* x, double* r);
+
+typedef struct {
+  int method;                 /* RMX_BICGSTAB | RMX_FGMRES */
+  int prec;                   /* storage of matrix + factors used by the Krylov loop */
+  int precond;                /* RMX_PREC_ILU0 | RMX_PREC_NONE | RMX_PREC_MPNN */
+  int max_iter;               /* SOLVER::maxIter */
+  int mkyrl;                  /* FGMRES restart length (SOLVER::mkyrl) */
+  double eps;                 /* SOLVER::maxDiff: relative residual tolerance */
+  int refine;                 /* 1: FP64 iterative refinement around the solve */
+  double refine_tol;          /* true FP64 relative residual to reach when refining */
+  int refine_max;             /* maximum refinement sweeps */
+} rmx_krylov_opts;
+
+typedef struct {
+  int iters;                  /* Krylov iterations (summed over refinement sweeps) */
+  int sweeps;                 /* refinement sweeps (0 without refinement) */
+  int converged;
+  double accuracy;            /* last Krylov relative residual estimate */
+  double true_resid;          /* ||B - A64 X||_2 / ||B||_2 in FP64 */
+  double seconds;             /* wall time of the solve on the device stream */
+} rmx_krylov_out;
+
+/* Solve A X = B for the assembled system: replaces SOLVER::Iterate with P_BCGSTABD
+ * (P_bcgstabd.cpp:98) or P_FGMRESD (P_fgmresd.cpp:90) and the ILU(0) preconditioner.
+ * X starts at zero.  With refine = 1 the solve runs in the requested storage precision
+ * inside an FP64 iterative refinement on the FP64 matrix. */
+rmx_status rmx_solve(rmx_ctx* ctx, const rmx_krylov_opts* opt, rmx_krylov_out* out);
+
+/* Host access to the system (tests, drop-in callers that keep host arrays). */
+rmx_status rmx_get_matrix(rmx_ctx* ctx, int prec, void* vals /* nnz */, double* B /* neq */);
+rmx_status rmx_get_factors(rmx_ctx* ctx, int prec, void* vals /* nnz */);
+rmx_status rmx_get_solution(rmx_ctx* ctx, double* X /* neq */);
+rmx_status rmx_set_rhs(rmx_ctx* ctx, const double* B);
+/* Load an arbitrary CSR system (diagonal first in every row): lets a caller that keeps
+ * the reference's own EQS/CRSMAT host code hand its pattern and values over. */
+rmx_status rmx_csr_upload(rmx_ctx* ctx, int n, const int* rowptr, const int* cols,
+                          const double* vals, const double* B);
+
+/* ------------------------------------------------------------------------------------ */
+/* Newton step                                                                           */
+
+typedef struct {
+  double maxAbs[3];           /* max |dU|, |dV|, |dh| of the correction (EQS::Update) */
+  double avAbs[3];
+  double relaxUs, relaxS;
+  int nrconv;                 /* max corrections below conv_uv / conv_s */
+  double scale;               /* ||B||_2 before scaling */
+  rmx_krylov_out solve;
+  double t_prepare, t_assemble, t_factor, t_solve, t_update;   /* seconds */
+} rmx_step_stats;
+
+/* Relaxation, correction (rotated back for rotated nodes), velocity cap, depth / marsh
+ * handling and midside interpolation: EqsUVS2D.cpp:398-760, EQS::Update (Update.cpp:42). */
+rmx_status rmx_newton_update(rmx_ctx* ctx, rmx_step_stats* st);
+
+/* One full Newton step on the device: prepare, assemble (FP64), scale, factor, solve,
+ * update. */
+rmx_status rmx_newton_step(rmx_ctx* ctx, const rmx_krylov_opts* opt, rmx_step_stats* st);
+
+/* Timing of the kernels of the last rmx_solve (HIP events on the context stream):
+ * average duration in seconds of one SpMV, one ILU apply (forward + backward sweeps) and
+ * the number of each that ran. */
+typedef struct {
+  double spmv_s, ilu_apply_s, ilu_fwd_s, ilu_bwd_s;
+  long long n_spmv, n_ilu;
+} rmx_kernel_times;
+rmx_status rmx_profile_kernels(rmx_ctx* ctx, int prec, int reps, rmx_kernel_times* out);
+
+#ifdef __cplusplus
+}
+#endif
+#endif

@@ -1,0 +1,493 @@
+// C ABI of the rmx library (declared in include/rmx.h).
+#include <chrono>
+#include <cstring>
+
+#include "rmx_internal.h"
+
+namespace rmx {
+
+static thread_local std::string g_err;
+
+[[noreturn]] void fail(rmx_status code, const std::string& msg) { throw Error{ code, msg }; }
+
+void ensure_work(Ctx& c);
+
+}  // namespace rmx
+
+using namespace rmx;
+
+#define API_BEGIN try {
+#define API_END                                                                          \
+  }                                                                                      \
+  catch (const rmx::Error& e) {                                                          \
+    rmx::g_err = e.msg;                                                                  \
+    return e.code;                                                                       \
+  }                                                                                      \
+  catch (const std::exception& e) {                                                      \
+    rmx::g_err = e.what();                                                               \
+    return RMX_ERR_ARG;                                                                  \
+  }                                                                                      \
+  return RMX_OK;
+
+static Ctx* need(rmx_ctx* p)
+{
+  if (!p) fail(RMX_ERR_ARG, "null context");
+  Ctx* c = reinterpret_cast<Ctx*>(p);
+  HIPCHECK(hipSetDevice(c->device));
+  return c;
+}
+
+extern "C" {
+
+const char* rmx_last_error(void) { return rmx::g_err.c_str(); }
+
+rmx_status rmx_create(int device, rmx_ctx** out)
+{
+  API_BEGIN
+  if (!out) fail(RMX_ERR_ARG, "null output pointer");
+  int count = 0;
+  if (hipGetDeviceCount(&count) != hipSuccess || count <= device || device < 0)
+    fail(RMX_ERR_NODEVICE, "no HIP device " + std::to_string(device));
+  hipDeviceProp_t prop;
+  HIPCHECK(hipGetDeviceProperties(&prop, device));
+  if (std::strncmp(prop.gcnArchName, "gfx950", 6) != 0)
+    fail(RMX_ERR_NODEVICE, std::string("device is ") + prop.gcnArchName + ", not gfx950");
+  HIPCHECK(hipSetDevice(device));
+  Ctx* c = new Ctx();
+  c->device = device;
+  HIPCHECK(hipStreamCreateWithFlags(&c->stream, hipStreamNonBlocking));
+  *out = reinterpret_cast<rmx_ctx*>(c);
+  API_END
+}
+
+rmx_status rmx_destroy(rmx_ctx* p)
+{
+  API_BEGIN
+  if (!p) return RMX_OK;
+  Ctx* c = need(p);
+  (void)hipStreamSynchronize(c->stream);
+  if (c->h_red) (void)hipHostFree(c->h_red);
+  hipStream_t s = c->stream;
+  delete c;
+  (void)hipStreamDestroy(s);
+  API_END
+}
+
+rmx_status rmx_mesh_upload(rmx_ctx* p, const rmx_mesh_desc* d)
+{
+  API_BEGIN
+  Ctx& c = *need(p);
+  if (!d || d->np <= 0 || d->ne <= 0 || !d->x || !d->y || !d->z || !d->elems || !d->nnd ||
+      !d->manning || !d->vt)
+    fail(RMX_ERR_ARG, "mesh: missing arrays");
+  if ((d->n_inlet && !d->inlet_segs) || (d->n_outlet && !d->outlet_segs))
+    fail(RMX_ERR_ARG, "mesh: missing control-line segments");
+  host_mesh(c.hm, *d);
+  const HostModel& hm = c.hm;
+  const int np = hm.np;
+  hipStream_t s = c.stream;
+  c.x.upload(hm.x, s); c.y.upload(hm.y, s); c.zor.upload(hm.zor, s); c.z.upload(hm.zor, s);
+  c.U.alloc(np); c.V.alloc(np); c.S.upload(hm.zor, s); c.cf.alloc(np);
+  HIPCHECK(hipMemsetAsync(c.U.p, 0, sizeof(double) * np, s));
+  HIPCHECK(hipMemsetAsync(c.V.p, 0, sizeof(double) * np, s));
+  c.d_elem.upload(hm.elem, s); c.d_nnd.upload(hm.nnd, s);
+  c.d_manning.upload(hm.manning, s); c.d_areaFact.upload(hm.areaFact, s);
+  // node -> region elements (ascending) for the friction gather
+  std::vector<int> ptr(np + 1, 0), list;
+  for (int e = 0; e < hm.ne; e++)
+    for (int i = 0; i < hm.nnd[e]; i++) ptr[hm.elem[8LL * e + i] + 1]++;
+  for (int i = 0; i < np; i++) ptr[i + 1] += ptr[i];
+  list.resize(ptr[np]);
+  std::vector<int> fill(ptr.begin(), ptr.end() - 1);
+  for (int e = 0; e < hm.ne; e++)
+    for (int i = 0; i < hm.nnd[e]; i++) list[fill[hm.elem[8LL * e + i]]++] = e;
+  c.n2e_ptr.upload(ptr, s); c.n2e.upload(list, s);
+  // model element list: nodes, type, eddy viscosity
+  const int nme = (int)hm.me.size();
+  std::vector<int> men(8LL * nme, -1), met(nme);
+  std::vector<double> mvt(nme, 0.0);
+  for (int k = 0; k < nme; k++) {
+    const int v = hm.me[k];
+    if (v >= 0) {
+      for (int i = 0; i < 8; i++) men[8LL * k + i] = hm.elem[8LL * v + i];
+      met[k] = hm.nnd[v] == 6 ? ET_TRI : ET_QUAD;
+      mvt[k] = hm.vt[v];
+    } else {
+      const int b = -v - 1;
+      for (int i = 0; i < 3; i++) men[8LL * k + i] = hm.bnd[3LL * b + i];
+      met[k] = ET_LINE;
+    }
+  }
+  c.nme = nme;
+  c.me_nodes.upload(men, s); c.me_type.upload(met, s); c.me_vt.upload(mvt, s);
+  c.d_in_segs.upload(hm.in_segs, s);
+  c.mesh_ready = true;
+  c.prepared = false;
+  c.csr_only = false;
+  HIPCHECK(hipStreamSynchronize(s));
+  API_END
+}
+
+rmx_status rmx_set_params(rmx_ctx* p, const rmx_params* q)
+{
+  API_BEGIN
+  Ctx& c = *need(p);
+  if (!q) fail(RMX_ERR_ARG, "null params");
+  c.par.relax_method = q->relax_method; c.par.dry_method = q->dry_method;
+  c.par.dry_limit = q->dry_limit; c.par.max_delta_uv = q->max_delta_uv;
+  c.par.max_delta_s = q->max_delta_s; c.par.conv_uv = q->conv_uv; c.par.conv_s = q->conv_s;
+  c.par.max_us = q->max_us;
+  API_END
+}
+
+rmx_status rmx_state_upload(rmx_ctx* p, const double* U, const double* V, const double* S)
+{
+  API_BEGIN
+  Ctx& c = *need(p);
+  if (!c.mesh_ready) fail(RMX_ERR_ARG, "state: upload a mesh first");
+  const int np = c.hm.np;
+  if (U) c.U.upload(U, np, c.stream);
+  if (V) c.V.upload(V, np, c.stream);
+  if (S) c.S.upload(S, np, c.stream);
+  HIPCHECK(hipStreamSynchronize(c.stream));
+  API_END
+}
+
+rmx_status rmx_state_download(rmx_ctx* p, double* U, double* V, double* S, double* z)
+{
+  API_BEGIN
+  Ctx& c = *need(p);
+  if (!c.mesh_ready) fail(RMX_ERR_ARG, "state: no mesh");
+  const int np = c.hm.np;
+  if (U) c.U.download(U, np, c.stream);
+  if (V) c.V.download(V, np, c.stream);
+  if (S) c.S.download(S, np, c.stream);
+  if (z) c.z.download(z, np, c.stream);
+  HIPCHECK(hipStreamSynchronize(c.stream));
+  API_END
+}
+
+static void first_prepare(Ctx& c)
+{
+  HostModel& hm = c.hm;
+  host_boundary_conditions(hm);
+  host_numbering(hm);
+  host_pattern(hm);
+  hipStream_t s = c.stream;
+  const int np = hm.np;
+  // device node BC data
+  std::vector<uint32_t> df(np, 0);
+  std::vector<double> rotm(4LL * np, 0.0);
+  for (int n = 0; n < np; n++) {
+    if (hm.flag[n] & nRotat) df[n] |= dRotat;
+    if (hm.kind[n] & kInlet) df[n] |= dInlet;
+    if (hm.kind[n] & kOutlet) df[n] |= dOutlet;
+    double* R = &rotm[4LL * n];
+    if (hm.rot[n] == rotTangent) { R[0] = hm.ny[n]; R[1] = hm.nx[n]; R[2] = -hm.nx[n]; R[3] = hm.ny[n]; }
+    else if (hm.rot[n] == rotNormal) { R[0] = hm.niox[n]; R[1] = hm.nioy[n]; R[2] = hm.nioy[n]; R[3] = -hm.niox[n]; }
+  }
+  c.dflag.upload(df, s); c.rotm.upload(rotm, s);
+  c.niox.upload(hm.niox, s); c.nioy.upload(hm.nioy, s); c.valS.upload(hm.valS, s);
+  c.valU.alloc(np);
+  HIPCHECK(hipMemsetAsync(c.valU.p, 0, sizeof(double) * np, s));
+  c.d_inlet_nodes.upload(hm.inlet_nodes, s);
+  c.eqno.upload(hm.eqno, s);
+  // CSR pattern
+  c.n = hm.neq;
+  c.nnz = hm.nnz;
+  c.rowptr.upload(hm.rowptr, s);
+  c.cols.upload(hm.cols, s);
+  c.B.alloc(c.n); c.X.alloc(c.n);
+  // assembly schedule
+  std::vector<int> ce, cp, ct;
+  host_colouring(hm, ce, cp, ct);
+  c.col_ptr = cp; c.col_type = ct;
+  c.col_elems.upload(ce, s);
+  std::vector<long long> off;
+  std::vector<int> slots, type;
+  host_slots(hm, off, slots, type);
+  c.me_slot_off.upload(off, s);
+  c.slots.upload(slots, s);
+  HIPCHECK(hipStreamSynchronize(s));
+  ensure_work(c);
+  setup_ilu_schedule(c);
+  c.have32 = c.have64 = c.fac32 = c.fac64 = false;
+  c.prepared = true;
+}
+
+rmx_status rmx_prepare(rmx_ctx* p, int* neq, long long* nnz)
+{
+  API_BEGIN
+  Ctx& c = *need(p);
+  if (!c.mesh_ready) fail(RMX_ERR_ARG, "prepare: upload a mesh first");
+  if (!c.prepared) first_prepare(c);
+  dev_prepare_step(c);
+  HIPCHECK(hipStreamSynchronize(c.stream));
+  if (neq) *neq = c.n;
+  if (nnz) *nnz = c.nnz;
+  API_END
+}
+
+rmx_status rmx_get_numbering(rmx_ctx* p, int* eqno)
+{
+  API_BEGIN
+  Ctx& c = *need(p);
+  if (!c.prepared || c.csr_only) fail(RMX_ERR_ARG, "numbering: call rmx_prepare first");
+  std::memcpy(eqno, c.hm.eqno.data(), sizeof(int) * c.hm.eqno.size());
+  API_END
+}
+
+rmx_status rmx_get_pattern(rmx_ctx* p, int* rowptr, int* cols)
+{
+  API_BEGIN
+  Ctx& c = *need(p);
+  if (!c.prepared) fail(RMX_ERR_ARG, "pattern: call rmx_prepare or rmx_csr_upload first");
+  c.rowptr.download(rowptr, c.n + 1, c.stream);
+  c.cols.download(cols, c.nnz, c.stream);
+  HIPCHECK(hipStreamSynchronize(c.stream));
+  API_END
+}
+
+rmx_status rmx_get_nodal(rmx_ctx* p, double* cf, double* bcU, int* kind, double* rotm)
+{
+  API_BEGIN
+  Ctx& c = *need(p);
+  if (!c.prepared || c.csr_only) fail(RMX_ERR_ARG, "nodal: call rmx_prepare first");
+  const int np = c.hm.np;
+  if (cf) c.cf.download(cf, np, c.stream);
+  if (bcU) c.valU.download(bcU, np, c.stream);
+  if (rotm) c.rotm.download(rotm, 4LL * np, c.stream);
+  HIPCHECK(hipStreamSynchronize(c.stream));
+  if (kind) for (int n = 0; n < np; n++) kind[n] = (int)c.hm.kind[n];
+  API_END
+}
+
+rmx_status rmx_n_model_elems(rmx_ctx* p, int* n)
+{
+  API_BEGIN
+  Ctx& c = *need(p);
+  if (!c.mesh_ready) fail(RMX_ERR_ARG, "no mesh");
+  *n = c.nme;
+  API_END
+}
+
+rmx_status rmx_element_coefs(rmx_ctx* p, int first, int count, double* K, double* F)
+{
+  API_BEGIN
+  Ctx& c = *need(p);
+  if (!c.prepared || c.csr_only) fail(RMX_ERR_ARG, "coefs: call rmx_prepare first");
+  if (first < 0 || count <= 0 || first + count > c.nme) fail(RMX_ERR_ARG, "coefs: range");
+  dev_element_coefs(c, first, count, K, F);
+  API_END
+}
+
+rmx_status rmx_assemble(rmx_ctx* p, int prec)
+{
+  API_BEGIN
+  Ctx& c = *need(p);
+  if (!c.prepared || c.csr_only) fail(RMX_ERR_ARG, "assemble: call rmx_prepare first");
+  dev_assemble(c, prec == RMX_F32 ? RMX_F32 : RMX_F64, true);
+  HIPCHECK(hipStreamSynchronize(c.stream));
+  API_END
+}
+
+rmx_status rmx_scale_l2(rmx_ctx* p, double* scale)
+{
+  API_BEGIN
+  Ctx& c = *need(p);
+  if (!c.have64 && !c.have32) fail(RMX_ERR_ARG, "scale: no matrix");
+  const double s = dev_scale_l2(c);
+  if (scale) *scale = s;
+  c.fac32 = c.fac64 = false;
+  API_END
+}
+
+rmx_status rmx_convert_matrix(rmx_ctx* p, int to_prec)
+{
+  API_BEGIN
+  Ctx& c = *need(p);
+  dev_convert(c, to_prec == RMX_F32 ? RMX_F32 : RMX_F64);
+  HIPCHECK(hipStreamSynchronize(c.stream));
+  API_END
+}
+
+rmx_status rmx_ilu0_factor(rmx_ctx* p, int prec)
+{
+  API_BEGIN
+  Ctx& c = *need(p);
+  dev_ilu0_factor(c, prec == RMX_F32 ? RMX_F32 : RMX_F64);
+  API_END
+}
+
+rmx_status rmx_ilu0_apply(rmx_ctx* p, int prec, const double* b, double* x)
+{
+  API_BEGIN
+  Ctx& c = *need(p);
+  if (!c.prepared) fail(RMX_ERR_ARG, "no system");
+  c.w[0].upload(b, c.n, c.stream);
+  dev_ilu0_apply(c, prec == RMX_F32 ? RMX_F32 : RMX_F64, c.w[0].p, c.w[1].p);
+  c.w[1].download(x, c.n, c.stream);
+  HIPCHECK(hipStreamSynchronize(c.stream));
+  API_END
+}
+
+rmx_status rmx_spmv(rmx_ctx* p, int prec, const double* x, double* r)
+{
+  API_BEGIN
+  Ctx& c = *need(p);
+  if (!c.prepared) fail(RMX_ERR_ARG, "no system");
+  const int pr = prec == RMX_F32 ? RMX_F32 : RMX_F64;
+  if ((pr == RMX_F32 && !c.have32) || (pr == RMX_F64 && !c.have64)) fail(RMX_ERR_ARG, "spmv: no matrix");
+  c.w[0].upload(x, c.n, c.stream);
+  dev_spmv(c, pr, c.w[0].p, c.w[1].p);
+  c.w[1].download(r, c.n, c.stream);
+  HIPCHECK(hipStreamSynchronize(c.stream));
+  API_END
+}
+
+rmx_status rmx_solve(rmx_ctx* p, const rmx_krylov_opts* o, rmx_krylov_out* out)
+{
+  API_BEGIN
+  Ctx& c = *need(p);
+  if (!o || !out) fail(RMX_ERR_ARG, "solve: null argument");
+  if (!c.prepared) fail(RMX_ERR_ARG, "solve: no system");
+  if (o->method == RMX_FGMRES && (o->mkyrl < 1 || o->mkyrl > 30))
+    fail(RMX_ERR_ARG, "solve: FGMRES restart length must be 1..30");
+  dev_solve(c, *o, *out);
+  API_END
+}
+
+rmx_status rmx_get_matrix(rmx_ctx* p, int prec, void* vals, double* B)
+{
+  API_BEGIN
+  Ctx& c = *need(p);
+  if (prec == RMX_F32) {
+    if (!c.have32) fail(RMX_ERR_ARG, "no FP32 matrix");
+    if (vals) c.A32.download(static_cast<float*>(vals), c.nnz, c.stream);
+  } else {
+    if (!c.have64) fail(RMX_ERR_ARG, "no FP64 matrix");
+    if (vals) c.A64.download(static_cast<double*>(vals), c.nnz, c.stream);
+  }
+  if (B) c.B.download(B, c.n, c.stream);
+  HIPCHECK(hipStreamSynchronize(c.stream));
+  API_END
+}
+
+rmx_status rmx_get_factors(rmx_ctx* p, int prec, void* vals)
+{
+  API_BEGIN
+  Ctx& c = *need(p);
+  if (prec == RMX_F32) {
+    if (!c.fac32) fail(RMX_ERR_ARG, "no FP32 factors");
+    c.L32.download(static_cast<float*>(vals), c.nnz, c.stream);
+  } else {
+    if (!c.fac64) fail(RMX_ERR_ARG, "no FP64 factors");
+    c.L64.download(static_cast<double*>(vals), c.nnz, c.stream);
+  }
+  HIPCHECK(hipStreamSynchronize(c.stream));
+  API_END
+}
+
+rmx_status rmx_get_solution(rmx_ctx* p, double* X)
+{
+  API_BEGIN
+  Ctx& c = *need(p);
+  c.X.download(X, c.n, c.stream);
+  HIPCHECK(hipStreamSynchronize(c.stream));
+  API_END
+}
+
+rmx_status rmx_set_rhs(rmx_ctx* p, const double* B)
+{
+  API_BEGIN
+  Ctx& c = *need(p);
+  if (!c.prepared) fail(RMX_ERR_ARG, "no system");
+  c.B.upload(B, c.n, c.stream);
+  HIPCHECK(hipStreamSynchronize(c.stream));
+  API_END
+}
+
+rmx_status rmx_csr_upload(rmx_ctx* p, int n, const int* rowptr, const int* cols,
+                          const double* vals, const double* B)
+{
+  API_BEGIN
+  Ctx& c = *need(p);
+  if (n <= 0 || !rowptr || !cols || !vals || !B) fail(RMX_ERR_ARG, "csr: missing arrays");
+  if (rowptr[0] != 0) fail(RMX_ERR_ARG, "csr: rowptr[0] must be 0");
+  for (int i = 0; i < n; i++) {
+    if (rowptr[i + 1] <= rowptr[i]) fail(RMX_ERR_ARG, "csr: empty row");
+    if (cols[rowptr[i]] != i) fail(RMX_ERR_ARG, "csr: the diagonal must come first in each row");
+    for (int q = rowptr[i]; q < rowptr[i + 1]; q++)
+      if (cols[q] < 0 || cols[q] >= n) fail(RMX_ERR_ARG, "csr: column out of range");
+  }
+  c.csr_only = true;
+  c.mesh_ready = false;
+  c.n = n;
+  c.nnz = rowptr[n];
+  c.rowptr.upload(rowptr, n + 1, c.stream);
+  c.cols.upload(cols, c.nnz, c.stream);
+  c.A64.upload(vals, c.nnz, c.stream);
+  c.B.upload(B, n, c.stream);
+  c.X.alloc(n);
+  HIPCHECK(hipStreamSynchronize(c.stream));
+  ensure_work(c);
+  setup_ilu_schedule(c);
+  c.have64 = true; c.have32 = false; c.fac32 = c.fac64 = false;
+  c.prepared = true;
+  API_END
+}
+
+rmx_status rmx_newton_update(rmx_ctx* p, rmx_step_stats* st)
+{
+  API_BEGIN
+  Ctx& c = *need(p);
+  if (!c.prepared || c.csr_only) fail(RMX_ERR_ARG, "update: call rmx_prepare first");
+  rmx_step_stats tmp{};
+  dev_newton_update(c, st ? *st : tmp);
+  API_END
+}
+
+rmx_status rmx_newton_step(rmx_ctx* p, const rmx_krylov_opts* o, rmx_step_stats* st)
+{
+  API_BEGIN
+  Ctx& c = *need(p);
+  if (!o || !st) fail(RMX_ERR_ARG, "newton: null argument");
+  if (!c.mesh_ready) fail(RMX_ERR_ARG, "newton: upload a mesh first");
+  *st = rmx_step_stats{};
+  auto now = [] { return std::chrono::steady_clock::now(); };
+  auto sec = [](auto a, auto b) { return std::chrono::duration<double>(b - a).count(); };
+  auto t0 = now();
+  if (!c.prepared) first_prepare(c);
+  dev_prepare_step(c);
+  HIPCHECK(hipStreamSynchronize(c.stream));
+  auto t1 = now();
+  dev_assemble(c, RMX_F64, true);
+  st->scale = dev_scale_l2(c);
+  auto t2 = now();
+  const int prec = o->prec == RMX_F32 ? RMX_F32 : RMX_F64;
+  if (prec == RMX_F32) dev_convert(c, RMX_F32);
+  if (o->precond == RMX_PREC_ILU0) dev_ilu0_factor(c, prec);
+  HIPCHECK(hipStreamSynchronize(c.stream));
+  auto t3 = now();
+  dev_solve(c, *o, st->solve);
+  auto t4 = now();
+  dev_newton_update(c, *st);
+  auto t5 = now();
+  st->t_prepare = sec(t0, t1); st->t_assemble = sec(t1, t2); st->t_factor = sec(t2, t3);
+  st->t_solve = sec(t3, t4); st->t_update = sec(t4, t5);
+  API_END
+}
+
+rmx_status rmx_profile_kernels(rmx_ctx* p, int prec, int reps, rmx_kernel_times* out)
+{
+  API_BEGIN
+  Ctx& c = *need(p);
+  if (!out || reps <= 0) fail(RMX_ERR_ARG, "profile: bad argument");
+  if (!c.prepared) fail(RMX_ERR_ARG, "profile: no system");
+  const int pr = prec == RMX_F32 ? RMX_F32 : RMX_F64;
+  if ((pr == RMX_F32 && !c.have32) || (pr == RMX_F64 && !c.have64)) fail(RMX_ERR_ARG, "profile: no matrix");
+  dev_profile(c, pr, reps, *out);
+  API_END
+}
+
+}  // extern "C"

@@ -1,0 +1,417 @@
+// Krylov loops on the device: right-preconditioned BiCGStab (P_BCGSTABD::Iterate,
+// P_bcgstabd.cpp:98-426) and flexible GMRES(m) with classical Gram-Schmidt and Givens
+// rotations (P_FGMRESD::Iterate, P_fgmresd.cpp:90-389, with the Hessenberg entries
+// assigned from the Gram-Schmidt dot products).  Vectors are FP64 and stay on the
+// device; inner products are FP64 with a fixed-order two-stage reduction; only the
+// handful of scalars per iteration (and FGMRES's small Hessenberg system) cross to the
+// host.  Around either loop an optional FP64 iterative refinement solves with the
+// FP32 matrix and factors and corrects with the FP64 residual.
+#include <chrono>
+#include <cmath>
+
+#include "rmx_internal.h"
+
+namespace rmx {
+
+void mpnn_apply(Ctx& c, const double* r, double* z);   // mpnn.hip
+bool mpnn_ready(Ctx& c);
+
+enum { S_RHO = 0, S_RV = 2, S_TS = 3, S_TT = 4, S_RR = 5, S_B2 = 6 };
+
+__global__ void k_copy3(long long n, const double* __restrict__ b, double* __restrict__ r,
+                        double* __restrict__ r0, double* __restrict__ p, double* __restrict__ x)
+{
+  for (long long i = (long long)blockIdx.x * blockDim.x + threadIdx.x; i < n;
+       i += (long long)gridDim.x * blockDim.x) {
+    const double v = b[i];
+    r[i] = v; r0[i] = v; p[i] = v; x[i] = 0.0;
+  }
+}
+
+__global__ void k_bicg_s(long long n, const double* __restrict__ sc, int rho, const double* __restrict__ r,
+                         const double* __restrict__ v, double* __restrict__ s)
+{
+  const double alpha = sc[rho] / sc[S_RV];
+  for (long long i = (long long)blockIdx.x * blockDim.x + threadIdx.x; i < n;
+       i += (long long)gridDim.x * blockDim.x)
+    s[i] = r[i] + (-alpha) * v[i];
+}
+
+__global__ void k_bicg_xr(long long n, const double* __restrict__ sc, int rho,
+                          const double* __restrict__ p, const double* __restrict__ s,
+                          const double* __restrict__ t, double* __restrict__ x,
+                          double* __restrict__ r)
+{
+  const double alpha = sc[rho] / sc[S_RV];
+  const double omega = sc[S_TS] / sc[S_TT];
+  for (long long i = (long long)blockIdx.x * blockDim.x + threadIdx.x; i < n;
+       i += (long long)gridDim.x * blockDim.x) {
+    double xi = x[i] + alpha * p[i];
+    xi += omega * s[i];
+    x[i] = xi;
+    r[i] = s[i] + (-omega) * t[i];
+  }
+}
+
+__global__ void k_bicg_p(long long n, const double* __restrict__ sc, int rho_old, int rho_new,
+                         const double* __restrict__ r, const double* __restrict__ v,
+                         double* __restrict__ p)
+{
+  const double alpha = sc[rho_old] / sc[S_RV];
+  const double omega = sc[S_TS] / sc[S_TT];
+  const double beta = sc[rho_new] * alpha / (sc[rho_old] * omega);
+  const double bo = -beta * omega;
+  for (long long i = (long long)blockIdx.x * blockDim.x + threadIdx.x; i < n;
+       i += (long long)gridDim.x * blockDim.x) {
+    double pi = beta * p[i];
+    pi += r[i];
+    pi += bo * v[i];
+    p[i] = pi;
+  }
+}
+
+__global__ void k_copy(long long n, const double* __restrict__ a, double* __restrict__ b)
+{
+  for (long long i = (long long)blockIdx.x * blockDim.x + threadIdx.x; i < n;
+       i += (long long)gridDim.x * blockDim.x)
+    b[i] = a[i];
+}
+
+__global__ void k_axpy(long long n, double a, const double* __restrict__ x, double* __restrict__ y)
+{
+  for (long long i = (long long)blockIdx.x * blockDim.x + threadIdx.x; i < n;
+       i += (long long)gridDim.x * blockDim.x)
+    y[i] += a * x[i];
+}
+
+__global__ void k_scal(long long n, double a, double* __restrict__ x)
+{
+  for (long long i = (long long)blockIdx.x * blockDim.x + threadIdx.x; i < n;
+       i += (long long)gridDim.x * blockDim.x)
+    x[i] *= a;
+}
+
+__global__ void k_scal_dev(long long n, const double* __restrict__ sc, int slot, double* __restrict__ x)
+{
+  const double nr = sqrt(sc[slot]);
+  const double t = 1.0 / nr;
+  for (long long i = (long long)blockIdx.x * blockDim.x + threadIdx.x; i < n;
+       i += (long long)gridDim.x * blockDim.x)
+    x[i] *= t;
+}
+
+// y -= sum_j h[j] * V[j]   (classical Gram-Schmidt update, h on the device)
+__global__ void k_gs_update(long long n, int K, const double* __restrict__ h,
+                            const double* __restrict__ Vb, long long ld, double* __restrict__ y)
+{
+  for (long long i = (long long)blockIdx.x * blockDim.x + threadIdx.x; i < n;
+       i += (long long)gridDim.x * blockDim.x) {
+    double v = y[i];
+    for (int j = 0; j < K; j++) v += (-h[j]) * Vb[j * ld + i];
+    y[i] = v;
+  }
+}
+
+// x += sum_j rs[j] * Z[j]
+__global__ void k_comb(long long n, int K, const double* __restrict__ rs,
+                       const double* __restrict__ Zb, long long ld, double* __restrict__ x)
+{
+  for (long long i = (long long)blockIdx.x * blockDim.x + threadIdx.x; i < n;
+       i += (long long)gridDim.x * blockDim.x) {
+    double v = x[i];
+    for (int j = 0; j < K; j++) v += rs[j] * Zb[j * ld + i];
+    x[i] = v;
+  }
+}
+
+constexpr int RED_BLOCKS = 512;
+
+__device__ __forceinline__ double bsum(double v, double* sh)
+{
+  for (int o = 32; o > 0; o >>= 1) v += __shfl_xor(v, o, 64);
+  const int w = threadIdx.x >> 6, l = threadIdx.x & 63;
+  if (l == 0) sh[w] = v;
+  __syncthreads();
+  double s = 0.0;
+  if (threadIdx.x == 0)
+    for (int i = 0; i < (int)(blockDim.x >> 6); i++) s += sh[i];
+  __syncthreads();
+  return s;
+}
+
+// partial[j*RED_BLOCKS + block] = <V[j], y> for j < K
+__global__ void __launch_bounds__(256) k_mdot(long long n, int K, const double* __restrict__ Vb,
+                                              long long ld, const double* __restrict__ y,
+                                              double* __restrict__ partial)
+{
+  __shared__ double sh[4];
+  for (int j = 0; j < K; j++) {
+    double acc = 0.0;
+    const double* vj = Vb + j * ld;
+    for (long long i = (long long)blockIdx.x * blockDim.x + threadIdx.x; i < n;
+         i += (long long)gridDim.x * blockDim.x)
+      acc += vj[i] * y[i];
+    const double s = bsum(acc, sh);
+    if (threadIdx.x == 0) partial[j * RED_BLOCKS + blockIdx.x] = s;
+  }
+}
+
+__global__ void k_mfinish(int K, const double* __restrict__ partial, double* __restrict__ out)
+{
+  __shared__ double sh[4];
+  for (int k = 0; k < K; k++) {
+    double v = 0.0;
+    for (int i = threadIdx.x; i < RED_BLOCKS; i += blockDim.x) v += partial[k * RED_BLOCKS + i];
+    const double s = bsum(v, sh);
+    if (threadIdx.x == 0) out[k] = s;
+  }
+}
+
+static int vblocks(long long n) { return (int)std::min<long long>(4096, (n + 255) / 256); }
+
+static void precond(Ctx& c, int prec, int kind, const double* in, double* out)
+{
+  if (kind == RMX_PREC_ILU0) {
+    dev_ilu0_apply(c, prec, in, out);
+  } else if (kind == RMX_PREC_MPNN) {
+    mpnn_apply(c, in, out);
+  } else {
+    k_copy<<<vblocks(c.n), 256, 0, c.stream>>>(c.n, in, out);
+  }
+}
+
+void ensure_work(Ctx& c)
+{
+  for (int k = 0; k < 10; k++) c.w[k].alloc(c.n);
+  c.red.alloc(64 + 40 * RED_BLOCKS);
+  if (!c.h_red) HIPCHECK(hipHostMalloc(&c.h_red, 64 * sizeof(double), hipHostMallocDefault));
+}
+
+static double fetch(Ctx& c, const double* dev, int k = 1)
+{
+  HIPCHECK(hipMemcpyAsync(c.h_red, dev, sizeof(double) * k, hipMemcpyDeviceToHost, c.stream));
+  HIPCHECK(hipStreamSynchronize(c.stream));
+  return c.h_red[0];
+}
+
+// BiCGStab on A(prec) x = b, x0 = 0.  Returns convergence; x gets the solution.
+static bool bicgstab(Ctx& c, int prec, int pk, const double* b, double* x, int maxIter, double eps,
+                     int& iters, double& accuracy)
+{
+  const long long n = c.n;
+  double *r0 = c.w[0].p, *r = c.w[1].p, *s = c.w[2].p, *t = c.w[3].p, *v = c.w[4].p,
+         *p = c.w[5].p, *tmp = c.w[6].p;
+  double* sc = c.red.p;
+  const int vb = vblocks(n);
+  k_copy3<<<vb, 256, 0, c.stream>>>(n, b, r, r0, p, x);
+  {
+    const double* xs[1] = { r0 };
+    const double* ys[1] = { r };
+    dev_dots(c, 1, xs, ys, sc + S_RHO);
+  }
+  const double resid = std::sqrt(fetch(c, sc + S_RHO));
+  accuracy = 1.0;
+  iters = 0;
+  bool conv = false;
+  if (!(resid > 0.0)) { iters = 0; accuracy = 0.0; return true; }
+  int cur = 0;
+  while (!conv) {
+    iters++;
+    precond(c, prec, pk, p, tmp);
+    dev_spmv(c, prec, tmp, v);
+    {
+      const double* xs[1] = { r0 };
+      const double* ys[1] = { v };
+      dev_dots(c, 1, xs, ys, sc + S_RV);
+    }
+    k_bicg_s<<<vb, 256, 0, c.stream>>>(n, sc, S_RHO + cur, r, v, s);
+    precond(c, prec, pk, s, tmp);
+    dev_spmv(c, prec, tmp, t);
+    {
+      const double* xs[2] = { t, t };
+      const double* ys[2] = { s, t };
+      dev_dots(c, 2, xs, ys, sc + S_TS);
+    }
+    k_bicg_xr<<<vb, 256, 0, c.stream>>>(n, sc, S_RHO + cur, p, s, t, x, r);
+    {
+      // rho_new into the other rho slot, (r,r) into S_RR
+      const double* xs[2] = { r0, r };
+      const double* ys[2] = { r, r };
+      dev_dots(c, 1, xs, ys, sc + S_RHO + (cur ^ 1));
+      dev_dots(c, 1, xs + 1, ys + 1, sc + S_RR);
+    }
+    k_bicg_p<<<vb, 256, 0, c.stream>>>(n, sc, S_RHO + cur, S_RHO + (cur ^ 1), r, v, p);
+    HIPCHECK(hipGetLastError());
+    cur ^= 1;
+    const double rr = fetch(c, sc + S_RR);
+    const double relacc = std::sqrt(rr) / resid;
+    accuracy = relacc;
+    if (relacc < eps) conv = true;
+    else if (iters >= maxIter || !std::isfinite(relacc)) break;
+  }
+  // x = M^-1 x
+  k_copy<<<vb, 256, 0, c.stream>>>(n, x, tmp);
+  precond(c, prec, pk, tmp, x);
+  return conv;
+}
+
+// Flexible GMRES(m) on A(prec) x = b, x0 = 0.
+static bool fgmres(Ctx& c, int prec, int pk, const double* b, double* x, int m, int maxIter,
+                   double eps, int& iters, double& accuracy)
+{
+  const long long n = c.n;
+  if (c.basis_m != m || c.V_basis.n != (size_t)(m + 1) * n) {
+    c.V_basis.alloc((size_t)(m + 1) * n);
+    c.Z_basis.alloc((size_t)m * n);
+    c.basis_m = m;
+  }
+  double* Vb = c.V_basis.p;
+  double* Zb = c.Z_basis.p;
+  double* sc = c.red.p;
+  double* partial = c.red.p + 64;
+  const int vb = vblocks(n);
+  HIPCHECK(hipMemsetAsync(x, 0, sizeof(double) * n, c.stream));
+  std::vector<std::vector<double>> hh(m, std::vector<double>(m + 1, 0.0));
+  std::vector<double> cs(m + 1), sn(m + 1), rs(m + 1);
+  const double EPSILON = 1.0e-20, EPSMAC = 1.0e-16;
+  double eps1 = 0.0, resid = 0.0, ro = 0.0;
+  accuracy = 1.0;
+  iters = 0;
+  bool conv = false;
+  DBuf<double> d_rs;
+  d_rs.alloc(m + 1);
+  while (true) {
+    // vv[0] = (b - A x) / ||b - A x||
+    dev_resid_p(c, prec, x, b, Vb);
+    {
+      const double* xs[1] = { Vb };
+      dev_dots(c, 1, xs, xs, sc + S_B2);
+    }
+    ro = std::sqrt(fetch(c, sc + S_B2));
+    if (std::fabs(ro) <= EPSILON) { conv = true; break; }
+    k_scal<<<vb, 256, 0, c.stream>>>(n, 1.0 / ro, Vb);
+    if (iters == 0) { resid = ro; eps1 = eps * ro; }
+    rs[0] = ro;
+    int i = -1;
+    while (true) {
+      i++;
+      iters++;
+      const int i1 = i + 1;
+      double* vi = Vb + (size_t)i * n;
+      double* vi1 = Vb + (size_t)i1 * n;
+      double* zi = Zb + (size_t)i * n;
+      precond(c, prec, pk, vi, zi);
+      dev_spmv(c, prec, zi, vi1);
+      // classical Gram-Schmidt: all dots, then all updates
+      k_mdot<<<RED_BLOCKS, 256, 0, c.stream>>>(n, i1, Vb, n, vi1, partial);
+      k_mfinish<<<1, 256, 0, c.stream>>>(i1, partial, sc + 16);
+      k_gs_update<<<vb, 256, 0, c.stream>>>(n, i1, sc + 16, Vb, n, vi1);
+      {
+        const double* xs[1] = { vi1 };
+        dev_dots(c, 1, xs, xs, sc + S_B2);
+      }
+      HIPCHECK(hipMemcpyAsync(c.h_red, sc, sizeof(double) * (16 + i1), hipMemcpyDeviceToHost, c.stream));
+      HIPCHECK(hipStreamSynchronize(c.stream));
+      for (int j = 0; j <= i; j++) hh[i][j] = c.h_red[16 + j];
+      double tn = std::sqrt(c.h_red[S_B2]);
+      hh[i][i1] = tn;
+      if (std::fabs(tn) > EPSILON) k_scal_dev<<<vb, 256, 0, c.stream>>>(n, sc, S_B2, vi1);
+      for (int k = 1; k <= i; k++) {
+        const int k1 = k - 1;
+        const double tt = hh[i][k1];
+        hh[i][k1] = cs[k1] * tt + sn[k1] * hh[i][k];
+        hh[i][k] = -sn[k1] * tt + cs[k1] * hh[i][k];
+      }
+      double gam = std::sqrt(hh[i][i] * hh[i][i] + hh[i][i1] * hh[i][i1]);
+      if (std::fabs(gam) <= EPSILON) gam = EPSMAC;
+      cs[i] = hh[i][i] / gam;
+      sn[i] = hh[i][i1] / gam;
+      rs[i1] = -sn[i] * rs[i];
+      rs[i] = cs[i] * rs[i];
+      hh[i][i] = cs[i] * hh[i][i] + sn[i] * hh[i][i1];
+      ro = std::fabs(rs[i1]);
+      accuracy = ro / resid;
+      if (i + 1 >= m || ro <= eps1 || iters >= maxIter || !std::isfinite(ro)) break;
+    }
+    // back substitution
+    rs[i] = rs[i] / hh[i][i];
+    for (int ii = 2; ii <= i + 1; ii++) {
+      const int k = i - ii + 1, k1 = k + 1;
+      double tt = rs[k];
+      for (int j = k1; j <= i; j++) tt -= hh[j][k] * rs[j];
+      rs[k] = tt / hh[k][k];
+    }
+    HIPCHECK(hipMemcpyAsync(d_rs.p, rs.data(), sizeof(double) * (i + 1), hipMemcpyHostToDevice, c.stream));
+    k_comb<<<vb, 256, 0, c.stream>>>(n, i + 1, d_rs.p, Zb, n, x);
+    HIPCHECK(hipStreamSynchronize(c.stream));
+    if (ro <= eps1) { conv = true; break; }
+    if (iters >= maxIter || !std::isfinite(ro)) { conv = false; break; }
+  }
+  return conv;
+}
+
+static bool krylov(Ctx& c, int method, int prec, int pk, const double* b, double* x, int maxIter,
+                   double eps, int mkyrl, int& iters, double& acc)
+{
+  if (method == RMX_FGMRES) return fgmres(c, prec, pk, b, x, mkyrl, maxIter, eps, iters, acc);
+  return bicgstab(c, prec, pk, b, x, maxIter, eps, iters, acc);
+}
+
+static double true_resid(Ctx& c, const double* x, double* r)
+{
+  dev_resid(c, x, c.B.p, r);
+  const double* xs[2] = { r, c.B.p };
+  dev_dots(c, 2, xs, xs, c.red.p + 8);
+  HIPCHECK(hipMemcpyAsync(c.h_red, c.red.p + 8, sizeof(double) * 2, hipMemcpyDeviceToHost, c.stream));
+  HIPCHECK(hipStreamSynchronize(c.stream));
+  const double nb = std::sqrt(c.h_red[1]);
+  return nb > 0 ? std::sqrt(c.h_red[0]) / nb : 0.0;
+}
+
+void dev_solve(Ctx& c, const rmx_krylov_opts& o, rmx_krylov_out& out)
+{
+  ensure_work(c);
+  c.X.alloc(c.n);
+  const auto t0 = std::chrono::steady_clock::now();
+  const int prec = o.prec == RMX_F32 ? RMX_F32 : RMX_F64;
+  const int pk = o.precond;
+  if (pk == RMX_PREC_ILU0 && !((prec == RMX_F32) ? c.fac32 : c.fac64))
+    fail(RMX_ERR_ARG, "solve: ILU(0) factors of the requested precision are missing");
+  if (pk == RMX_PREC_MPNN && !mpnn_ready(c)) fail(RMX_ERR_ARG, "solve: no learned preconditioner loaded");
+  if ((prec == RMX_F32 && !c.have32) || (prec == RMX_F64 && !c.have64))
+    fail(RMX_ERR_ARG, "solve: matrix of the requested precision is missing");
+  out = rmx_krylov_out{};
+  if (!o.refine) {
+    int it = 0; double acc = 1.0;
+    const bool conv = krylov(c, o.method, prec, pk, c.B.p, c.X.p, o.max_iter, o.eps, o.mkyrl, it, acc);
+    out.iters = it; out.accuracy = acc; out.converged = conv;
+    out.true_resid = c.have64 ? true_resid(c, c.X.p, c.w[7].p) : -1.0;
+  } else {
+    if (!c.have64) fail(RMX_ERR_ARG, "refinement needs the FP64 matrix");
+    // FP64 iterative refinement: x = 0, r = B; repeat: solve A_p d = r; x += d; r = B - A64 x
+    double* xr = c.X.p;
+    double* r = c.w[7].p;
+    double* d = c.w[8].p;
+    HIPCHECK(hipMemsetAsync(xr, 0, sizeof(double) * c.n, c.stream));
+    k_copy<<<vblocks(c.n), 256, 0, c.stream>>>(c.n, c.B.p, r);
+    double tr = 1.0;
+    int total = 0;
+    const int sweeps_max = o.refine_max > 0 ? o.refine_max : 10;
+    int sw = 0;
+    for (; sw < sweeps_max; sw++) {
+      int it = 0; double acc = 1.0;
+      krylov(c, o.method, prec, pk, r, d, o.max_iter, o.eps, o.mkyrl, it, acc);
+      total += it;
+      out.accuracy = acc;
+      k_axpy<<<vblocks(c.n), 256, 0, c.stream>>>(c.n, 1.0, d, xr);
+      tr = true_resid(c, xr, r);
+      if (tr <= o.refine_tol || !std::isfinite(tr)) { sw++; break; }
+    }
+    out.iters = total; out.sweeps = sw; out.true_resid = tr;
+    out.converged = tr <= o.refine_tol;
+  }
+  HIPCHECK(hipStreamSynchronize(c.stream));
+  out.seconds = std::chrono::duration<double>(std::chrono::steady_clock::now() - t0).count();
+}
+
+}  // namespace rmx

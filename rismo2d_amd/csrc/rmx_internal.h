@@ -1,0 +1,198 @@
+// Internal definitions of the rmx library (context, device buffers, host model).
+#pragma once
+
+#include <hip/hip_runtime.h>
+
+#include <cstdint>
+#include <functional>
+#include <string>
+#include <vector>
+
+#include "rmx.h"
+
+namespace rmx {
+
+// Boundary-condition kind bits: the reference's BCON flag values (Bcon.h:160-190).
+enum : uint32_t {
+  kInlet = 1, kOutlet = 2, kOpenBnd = 4, kQInlet = 8, kSlip = 128, kSetUV = 512,
+  kSetS = 1024, kFixU = 32768, kFixV = 65536, kAutoSlip = 4194304
+};
+enum { rotTangent = 1, rotNormal = 2 };
+// node flags of this library
+enum : uint32_t { nCorn = 1, nMids = 2, nBound = 4, nRotat = 64, nNoMoment = 128 };
+// device node flag bits (packed for the kernels)
+enum : uint32_t { dRotat = 1, dInlet = 2, dOutlet = 4 };
+
+enum ElemType : int { ET_TRI = 0, ET_QUAD = 1, ET_LINE = 2 };
+
+struct Error {
+  rmx_status code;
+  std::string msg;
+};
+
+[[noreturn]] void fail(rmx_status code, const std::string& msg);
+
+#define HIPCHECK(x)                                                                     \
+  do {                                                                                  \
+    hipError_t e_ = (x);                                                                \
+    if (e_ != hipSuccess)                                                               \
+      ::rmx::fail(RMX_ERR_HIP, std::string(#x) + ": " + hipGetErrorString(e_));        \
+  } while (0)
+
+// A device buffer that frees itself.
+template <class T>
+struct DBuf {
+  T* p = nullptr;
+  size_t n = 0;
+  DBuf() = default;
+  DBuf(const DBuf&) = delete;
+  DBuf& operator=(const DBuf&) = delete;
+  ~DBuf() { release(); }
+  void release() {
+    if (p) (void)hipFree(p);
+    p = nullptr;
+    n = 0;
+  }
+  void alloc(size_t count) {
+    if (count == n && p) return;
+    release();
+    if (count) HIPCHECK(hipMalloc(&p, count * sizeof(T)));
+    n = count;
+  }
+  void upload(const T* h, size_t count, hipStream_t s) {
+    alloc(count);
+    if (count) HIPCHECK(hipMemcpyAsync(p, h, count * sizeof(T), hipMemcpyHostToDevice, s));
+  }
+  void upload(const std::vector<T>& v, hipStream_t s) { upload(v.data(), v.size(), s); }
+  void download(T* h, size_t count, hipStream_t s) const {
+    if (count) HIPCHECK(hipMemcpyAsync(h, p, count * sizeof(T), hipMemcpyDeviceToHost, s));
+  }
+};
+
+// Host-side model: mesh, boundary line elements, boundary conditions, numbering, pattern.
+struct HostModel {
+  int np = 0, ne = 0;
+  std::vector<double> x, y, zor, areaFact, manning, vt;
+  std::vector<int> elem, nnd;                 // [ne*8], [ne]
+  std::vector<int> in_segs, out_segs;
+  double Q = 0, S_in = 0, S_out = 0;
+  // topology
+  std::vector<int> noel;
+  std::vector<uint32_t> flag;
+  int nb = 0;
+  std::vector<int> bnd;                       // [nb*3]
+  std::vector<uint32_t> bflag;                // 1 inlet, 2 outlet, 4 open
+  std::vector<int> me;                        // model element list (>=0 region, <0 -(b+1))
+  // boundary conditions (static part)
+  std::vector<uint32_t> kind;
+  std::vector<int> rot;
+  std::vector<double> nx, ny, niox, nioy, valS;
+  std::vector<int> inlet_nodes;               // nodes of the QINLET line, ascending
+  // numbering and pattern
+  int neq = 0;
+  std::vector<int> eqno;                      // [3*np]
+  std::vector<int> rowptr, cols;              // CSR, diagonal first, insertion order
+  long long nnz = 0;
+};
+
+struct Params {
+  int relax_method = 2, dry_method = 2;
+  double dry_limit = 1e-3, max_delta_uv = 10.0, max_delta_s = 10.0, conv_uv = 1e-7,
+         conv_s = 1e-7, max_us = 100.0;
+  double g = 9.81;
+  double vk = 1.78e-6 / (1.0 + 0.0337 * 10.0 + 0.00022 * 10.0 * 10.0);
+  double hmin() const { return dry_limit / 2.0; }
+};
+
+// Level-set schedule of a triangular sweep or of the ILU(0) factorisation.
+struct Levels {
+  int nlev = 0;
+  std::vector<int> ptr, rows;                 // host copy
+  DBuf<int> d_ptr, d_rows;
+};
+
+struct Ctx {
+  int device = 0;
+  hipStream_t stream = nullptr;
+  Params par;
+  HostModel hm;
+  bool mesh_ready = false, prepared = false, csr_only = false;
+
+  // ---- device node data (SoA) ----
+  DBuf<double> x, y, zor, z, U, V, S, cf, niox, nioy, valU, valS, rotm;  // rotm [np*4]
+  DBuf<uint32_t> dflag;
+  DBuf<int> eqno;                             // [3*np]
+  // friction gather: node -> incident region elements (ascending), areaFact*cf summed
+  DBuf<int> n2e_ptr, n2e;
+  DBuf<int> d_elem, d_nnd;                    // region elements [ne*8], [ne]
+  DBuf<double> d_manning, d_areaFact;
+  DBuf<int> d_in_segs, d_inlet_nodes;
+  // ---- model elements for assembly ----
+  int nme = 0;
+  DBuf<int> me_nodes;                         // [nme*8]
+  DBuf<double> me_vt;                         // [nme]
+  DBuf<int> me_type;                          // [nme]
+  DBuf<long long> me_slot_off;                // offset of each element's slot block
+  DBuf<int> slots;                            // per element [nrow_ldof * ncol_ldof]
+  std::vector<int> col_ptr;                   // colour classes: launches
+  std::vector<int> col_type;
+  DBuf<int> col_elems;
+  // ---- CSR system ----
+  int n = 0;
+  long long nnz = 0;
+  DBuf<int> rowptr, cols;
+  DBuf<double> A64, L64, B, X;
+  DBuf<float> A32, L32;
+  bool have64 = false, have32 = false, fac64 = false, fac32 = false;
+  // ILU schedule
+  Levels lev_lo, lev_up;                      // forward / backward sweep levels
+  DBuf<int> lrp, lcol, urp, ucol;             // strictly lower / upper patterns
+  long long nlo = 0, nup = 0;
+  DBuf<double> lval64, uval64, diag64;
+  DBuf<float> lval32, uval32, diag32;
+  DBuf<int> srt_cols;                         // per-row sorted columns (factor lookups)
+  DBuf<unsigned char> srt_pos;                // original position of each sorted column
+  int max_width = 0;
+  // Krylov work vectors
+  DBuf<double> w[10];
+  DBuf<double> red;                           // reduction scratch
+  double* h_red = nullptr;                    // pinned host mirror of red
+  // FGMRES basis
+  DBuf<double> V_basis, Z_basis;
+  int basis_m = 0;
+  // kernel timing of the last profiled solve
+  rmx_kernel_times kt{};
+};
+
+// host model set-up (model.cpp)
+void host_mesh(HostModel& hm, const rmx_mesh_desc& d);
+void host_boundary_conditions(HostModel& hm);
+void host_numbering(HostModel& hm);
+void host_pattern(HostModel& hm);
+void host_colouring(const HostModel& hm, std::vector<int>& col_elems, std::vector<int>& col_ptr,
+                    std::vector<int>& col_type);
+void host_levels(int n, const std::vector<int>& rowptr, const std::vector<int>& cols, bool lower,
+                 std::vector<int>& ptr, std::vector<int>& rows);
+void host_slots(const HostModel& hm, std::vector<long long>& off, std::vector<int>& slots,
+                std::vector<int>& type);
+void parallel_for(long long n, const std::function<void(long long, long long)>& f);
+
+// device entry points implemented in the .hip files
+void dev_prepare_step(Ctx& c);                 // friction + inlet distribution
+void dev_assemble(Ctx& c, int prec, bool zero);
+double dev_scale_l2(Ctx& c);
+void dev_convert(Ctx& c, int to_prec);
+void dev_ilu0_factor(Ctx& c, int prec);
+void dev_ilu0_apply(Ctx& c, int prec, const double* b, double* x);   // device pointers
+void dev_spmv(Ctx& c, int prec, const double* x, double* r);        // device pointers
+void dev_solve(Ctx& c, const rmx_krylov_opts& o, rmx_krylov_out& out);
+void dev_newton_update(Ctx& c, rmx_step_stats& st);
+void dev_element_coefs(Ctx& c, int first, int count, double* K, double* F);
+void dev_profile(Ctx& c, int prec, int reps, rmx_kernel_times& out);
+void dev_resid(Ctx& c, const double* x, const double* b, double* r);
+void dev_resid_p(Ctx& c, int prec, const double* x, const double* b, double* r);
+void dev_dots(Ctx& c, int K, const double* const* x, const double* const* y, double* dst);
+void setup_ilu_schedule(Ctx& c);
+void upload_shapes(hipStream_t s);
+
+}  // namespace rmx

@@ -1,0 +1,299 @@
+"""ctypes binding of librmx.so (the C ABI declared in include/rmx.h).
+
+This is the only way Python reaches the GPU path: there is no fallback.  Loading fails
+with an ImportError-like ``RmxError`` if librmx.so has not been built, and every call
+that fails in the library raises ``RmxError`` with the library's message.
+"""
+from __future__ import annotations
+
+import ctypes as C
+import os
+import re
+
+import numpy as np
+
+HERE = os.path.dirname(os.path.abspath(__file__))
+LIB_PATH = os.path.join(HERE, "librmx.so")
+HEADER = os.path.join(os.path.dirname(HERE), "include", "rmx.h")
+
+F64, F32 = 64, 32
+BICGSTAB, FGMRES = 1, 2
+PREC_NONE, PREC_ILU0, PREC_MPNN = 0, 1, 2
+ERR_NODEVICE = -4
+
+_d = C.POINTER(C.c_double)
+_f = C.POINTER(C.c_float)
+_i = C.POINTER(C.c_int)
+_v = C.c_void_p
+
+
+class RmxError(RuntimeError):
+    def __init__(self, code, msg):
+        super().__init__(f"rmx error {code}: {msg}")
+        self.code = code
+
+
+class MeshDesc(C.Structure):
+    _fields_ = [("np", C.c_int), ("x", _d), ("y", _d), ("z", _d), ("ne", C.c_int),
+                ("elems", _i), ("nnd", _i), ("manning", _d), ("vt", _d),
+                ("n_inlet", C.c_int), ("inlet_segs", _i), ("n_outlet", C.c_int),
+                ("outlet_segs", _i), ("Q", C.c_double), ("S_in", C.c_double),
+                ("S_out", C.c_double)]
+
+
+class Params(C.Structure):
+    _fields_ = [("relax_method", C.c_int), ("dry_method", C.c_int), ("dry_limit", C.c_double),
+                ("max_delta_uv", C.c_double), ("max_delta_s", C.c_double),
+                ("conv_uv", C.c_double), ("conv_s", C.c_double), ("max_us", C.c_double)]
+
+
+class KrylovOpts(C.Structure):
+    _fields_ = [("method", C.c_int), ("prec", C.c_int), ("precond", C.c_int),
+                ("max_iter", C.c_int), ("mkyrl", C.c_int), ("eps", C.c_double),
+                ("refine", C.c_int), ("refine_tol", C.c_double), ("refine_max", C.c_int)]
+
+
+class KrylovOut(C.Structure):
+    _fields_ = [("iters", C.c_int), ("sweeps", C.c_int), ("converged", C.c_int),
+                ("accuracy", C.c_double), ("true_resid", C.c_double), ("seconds", C.c_double)]
+
+
+class StepStats(C.Structure):
+    _fields_ = [("maxAbs", C.c_double * 3), ("avAbs", C.c_double * 3), ("relaxUs", C.c_double),
+                ("relaxS", C.c_double), ("nrconv", C.c_int), ("scale", C.c_double),
+                ("solve", KrylovOut), ("t_prepare", C.c_double), ("t_assemble", C.c_double),
+                ("t_factor", C.c_double), ("t_solve", C.c_double), ("t_update", C.c_double)]
+
+
+class KernelTimes(C.Structure):
+    _fields_ = [("spmv_s", C.c_double), ("ilu_apply_s", C.c_double), ("ilu_fwd_s", C.c_double),
+                ("ilu_bwd_s", C.c_double), ("n_spmv", C.c_longlong), ("n_ilu", C.c_longlong)]
+
+
+_LIB = None
+
+
+def header_functions():
+    """Names of the entry points declared in include/rmx.h."""
+    txt = open(HEADER).read()
+    return sorted(set(re.findall(r"^\s*(?:rmx_status|const char\*)\s+(rmx_\w+)\s*\(", txt, re.M)))
+
+
+def lib():
+    global _LIB
+    if _LIB is None:
+        if not os.path.exists(LIB_PATH):
+            raise RmxError(-100, f"{LIB_PATH} is not built (run __graft_entry__.build())")
+        L = C.CDLL(LIB_PATH)
+        L.rmx_last_error.restype = C.c_char_p
+        sig = {
+            "rmx_create": [C.c_int, C.POINTER(_v)], "rmx_destroy": [_v],
+            "rmx_mesh_upload": [_v, C.POINTER(MeshDesc)], "rmx_set_params": [_v, C.POINTER(Params)],
+            "rmx_state_upload": [_v, _d, _d, _d], "rmx_state_download": [_v, _d, _d, _d, _d],
+            "rmx_prepare": [_v, _i, C.POINTER(C.c_longlong)], "rmx_get_numbering": [_v, _i],
+            "rmx_get_pattern": [_v, _i, _i], "rmx_get_nodal": [_v, _d, _d, _i, _d],
+            "rmx_n_model_elems": [_v, _i], "rmx_element_coefs": [_v, C.c_int, C.c_int, _d, _d],
+            "rmx_assemble": [_v, C.c_int], "rmx_scale_l2": [_v, _d],
+            "rmx_convert_matrix": [_v, C.c_int], "rmx_ilu0_factor": [_v, C.c_int],
+            "rmx_ilu0_apply": [_v, C.c_int, _d, _d], "rmx_spmv": [_v, C.c_int, _d, _d],
+            "rmx_solve": [_v, C.POINTER(KrylovOpts), C.POINTER(KrylovOut)],
+            "rmx_get_matrix": [_v, C.c_int, _v, _d], "rmx_get_factors": [_v, C.c_int, _v],
+            "rmx_get_solution": [_v, _d], "rmx_set_rhs": [_v, _d],
+            "rmx_csr_upload": [_v, C.c_int, _i, _i, _d, _d],
+            "rmx_newton_update": [_v, C.POINTER(StepStats)],
+            "rmx_newton_step": [_v, C.POINTER(KrylovOpts), C.POINTER(StepStats)],
+            "rmx_profile_kernels": [_v, C.c_int, C.c_int, C.POINTER(KernelTimes)],
+        }
+        for name, args in sig.items():
+            fn = getattr(L, name)
+            fn.argtypes = args
+            fn.restype = C.c_int
+        _LIB = L
+    return _LIB
+
+
+def _check(code):
+    if code != 0:
+        raise RmxError(code, lib().rmx_last_error().decode())
+
+
+def _p(a, t):
+    return a.ctypes.data_as(t)
+
+
+def _c(a, dt):
+    return np.ascontiguousarray(a, dtype=dt)
+
+
+class Context:
+    """One equation system on one GPU (an rmx_ctx)."""
+
+    def __init__(self, device: int = 0):
+        self.h = _v()
+        _check(lib().rmx_create(device, C.byref(self.h)))
+        self.np = 0
+        self.n = 0
+        self.nnz = 0
+        self._keep = {}
+
+    def close(self):
+        if self.h:
+            lib().rmx_destroy(self.h)
+            self.h = _v()
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+    # ---- model group -------------------------------------------------------------------
+    def upload_mesh(self, deck):
+        k = dict(x=_c(deck.x, np.float64), y=_c(deck.y, np.float64), z=_c(deck.z, np.float64),
+                 el=_c(deck.elems, np.int32), nnd=_c(deck.nnd, np.int32),
+                 man=_c(deck.manning, np.float64), vt=_c(deck.vt, np.float64),
+                 inl=_c(deck.lines[1], np.int32), out=_c(deck.lines[2], np.int32))
+        self._keep = k
+        d = MeshDesc(deck.n_nodes, _p(k["x"], _d), _p(k["y"], _d), _p(k["z"], _d), deck.n_elems,
+                     _p(k["el"], _i), _p(k["nnd"], _i), _p(k["man"], _d), _p(k["vt"], _d),
+                     len(k["inl"]), _p(k["inl"], _i), len(k["out"]), _p(k["out"], _i),
+                     deck.Q_inlet, deck.S_inlet, deck.S_outlet)
+        _check(lib().rmx_mesh_upload(self.h, C.byref(d)))
+        self.np = deck.n_nodes
+        self.state_upload(deck.U0, deck.V0, deck.S0)
+
+    def set_params(self, relax_method=2, dry_method=2, dry_limit=1e-3, max_delta_uv=10.0,
+                   max_delta_s=10.0, conv_uv=1e-7, conv_s=1e-7, max_us=100.0):
+        p = Params(relax_method, dry_method, dry_limit, max_delta_uv, max_delta_s, conv_uv,
+                   conv_s, max_us)
+        _check(lib().rmx_set_params(self.h, C.byref(p)))
+
+    def state_upload(self, U, V, S):
+        U, V, S = _c(U, np.float64), _c(V, np.float64), _c(S, np.float64)
+        _check(lib().rmx_state_upload(self.h, _p(U, _d), _p(V, _d), _p(S, _d)))
+
+    def state(self):
+        U, V, S, z = (np.zeros(self.np) for _ in range(4))
+        _check(lib().rmx_state_download(self.h, _p(U, _d), _p(V, _d), _p(S, _d), _p(z, _d)))
+        return U, V, S, z
+
+    def prepare(self):
+        n = C.c_int(0)
+        nnz = C.c_longlong(0)
+        _check(lib().rmx_prepare(self.h, C.byref(n), C.byref(nnz)))
+        self.n, self.nnz = n.value, nnz.value
+        return self.n
+
+    def numbering(self):
+        eq = np.zeros(3 * self.np, np.int32)
+        _check(lib().rmx_get_numbering(self.h, _p(eq, _i)))
+        return eq.reshape(self.np, 3)
+
+    def pattern(self):
+        rp = np.zeros(self.n + 1, np.int32)
+        cols = np.zeros(self.nnz, np.int32)
+        _check(lib().rmx_get_pattern(self.h, _p(rp, _i), _p(cols, _i)))
+        return rp, cols
+
+    def nodal(self):
+        cf, bcU = np.zeros(self.np), np.zeros(self.np)
+        kind = np.zeros(self.np, np.int32)
+        rotm = np.zeros((self.np, 4))
+        _check(lib().rmx_get_nodal(self.h, _p(cf, _d), _p(bcU, _d), _p(kind, _i), _p(rotm, _d)))
+        return dict(cf=cf, bcU=bcU, kind=kind, rotm=rotm)
+
+    def n_model_elems(self):
+        n = C.c_int(0)
+        _check(lib().rmx_n_model_elems(self.h, C.byref(n)))
+        return n.value
+
+    def element_coefs(self, first, count):
+        K = np.zeros((count, 24, 24))
+        F = np.zeros((count, 24))
+        _check(lib().rmx_element_coefs(self.h, first, count, _p(K, _d), _p(F, _d)))
+        return K, F
+
+    # ---- drop-in group -----------------------------------------------------------------
+    def assemble(self, prec=F64):
+        _check(lib().rmx_assemble(self.h, prec))
+
+    def matrix(self, prec=F64):
+        A = np.zeros(self.nnz, np.float32 if prec == F32 else np.float64)
+        B = np.zeros(self.n)
+        _check(lib().rmx_get_matrix(self.h, prec, A.ctypes.data_as(_v), _p(B, _d)))
+        return A, B
+
+    def factors(self, prec=F64):
+        L = np.zeros(self.nnz, np.float32 if prec == F32 else np.float64)
+        _check(lib().rmx_get_factors(self.h, prec, L.ctypes.data_as(_v)))
+        return L
+
+    def scale_l2(self):
+        s = C.c_double(0)
+        _check(lib().rmx_scale_l2(self.h, C.byref(s)))
+        return s.value
+
+    def convert(self, to_prec):
+        _check(lib().rmx_convert_matrix(self.h, to_prec))
+
+    def ilu0_factor(self, prec=F64):
+        _check(lib().rmx_ilu0_factor(self.h, prec))
+
+    def ilu0_apply(self, b, prec=F64):
+        b = _c(b, np.float64)
+        x = np.zeros(self.n)
+        _check(lib().rmx_ilu0_apply(self.h, prec, _p(b, _d), _p(x, _d)))
+        return x
+
+    def spmv(self, x, prec=F64):
+        x = _c(x, np.float64)
+        r = np.zeros(self.n)
+        _check(lib().rmx_spmv(self.h, prec, _p(x, _d), _p(r, _d)))
+        return r
+
+    def solve(self, method=BICGSTAB, prec=F64, precond=PREC_ILU0, max_iter=1000, mkyrl=30,
+              eps=1e-10, refine=False, refine_tol=1e-8, refine_max=10):
+        o = KrylovOpts(method, prec, precond, max_iter, mkyrl, eps, int(refine), refine_tol,
+                       refine_max)
+        out = KrylovOut()
+        _check(lib().rmx_solve(self.h, C.byref(o), C.byref(out)))
+        return out
+
+    def solution(self):
+        X = np.zeros(self.n)
+        _check(lib().rmx_get_solution(self.h, _p(X, _d)))
+        return X
+
+    def set_rhs(self, B):
+        B = _c(B, np.float64)
+        _check(lib().rmx_set_rhs(self.h, _p(B, _d)))
+
+    def csr_upload(self, rowptr, cols, vals, B):
+        rowptr, cols = _c(rowptr, np.int32), _c(cols, np.int32)
+        vals, B = _c(vals, np.float64), _c(B, np.float64)
+        _check(lib().rmx_csr_upload(self.h, len(B), _p(rowptr, _i), _p(cols, _i), _p(vals, _d),
+                                    _p(B, _d)))
+        self.n, self.nnz = len(B), int(rowptr[-1])
+
+    def newton_update(self):
+        st = StepStats()
+        _check(lib().rmx_newton_update(self.h, C.byref(st)))
+        return st
+
+    def newton_step(self, method=BICGSTAB, prec=F64, precond=PREC_ILU0, max_iter=1000, mkyrl=30,
+                    eps=1e-10, refine=False, refine_tol=1e-8, refine_max=10):
+        o = KrylovOpts(method, prec, precond, max_iter, mkyrl, eps, int(refine), refine_tol,
+                       refine_max)
+        st = StepStats()
+        _check(lib().rmx_newton_step(self.h, C.byref(o), C.byref(st)))
+        if not self.n:
+            self.prepare_sizes()
+        return st
+
+    def prepare_sizes(self):
+        return self.prepare()
+
+    def profile_kernels(self, prec=F64, reps=10):
+        kt = KernelTimes()
+        _check(lib().rmx_profile_kernels(self.h, prec, reps, C.byref(kt)))
+        return kt

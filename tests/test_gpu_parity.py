@@ -1,0 +1,236 @@
+"""GPU parity of the HIP path (through the C ABI) against the oracle and the reference's
+golden vectors.
+
+Tolerances (floating point):
+* integer work (numbering, CSR pattern, colouring-independent results): bit-exact;
+* FP32 ILU(0) factors computed from the reference's own scaled FP32 matrix: bit-exact
+  (the factor kernel keeps the reference's per-entry update order, FMA contraction off);
+* element matrices, assembled FP64 entries, RHS: |gpu - oracle| <= 1e-12 * max|row| (the
+  GPU contracts to FMA and sums element contributions in colour order);
+* nodal friction / inlet discharge: 1e-13 relative;
+* SpMV, ILU apply (FP64 accumulation in another order): 1e-12 relative to max|result|;
+* Newton-converged u, v, h: within 1e-8 relative of the reference / oracle.
+"""
+import hashlib
+
+import numpy as np
+import pytest
+
+from helpers import golden_deck, load_golden
+from oracle import pyoracle as po
+from rismo2d_amd import deck as dk
+from rismo2d_amd import rmx
+
+pytestmark = pytest.mark.gpu
+
+
+@pytest.fixture(scope="module")
+def ctx():
+    c = rmx.Context(0)
+    yield c
+    c.close()
+
+
+def _rowmax(rp, A):
+    w = np.diff(rp)
+    rows = np.repeat(np.arange(len(w)), w)
+    m = np.zeros(len(w))
+    np.maximum.at(m, rows, np.abs(A))
+    return m[rows], rows
+
+
+@pytest.mark.parametrize("name", ["tri20x4", "quad10x3"])
+def test_setup_and_element_kernel_match_oracle(ctx, name):
+    print(f"[gpu] setup/element parity on {name}", flush=True)
+    g = load_golden(name)
+    deck = golden_deck(g)
+    ctx.upload_mesh(deck)
+    n = ctx.prepare()
+    om = po.OracleModel(deck)
+    assert om.prepare() == n
+    np.testing.assert_array_equal(ctx.numbering(), om.numbering())
+    rp, cols = ctx.pattern()
+    orp, ocols = om.pattern()
+    np.testing.assert_array_equal(rp, orp)
+    np.testing.assert_array_equal(cols, ocols)
+    nod, onod = ctx.nodal(), om.nodal()
+    np.testing.assert_array_equal(nod["kind"], onod["kind"])
+    np.testing.assert_allclose(nod["cf"], onod["cf"], rtol=1e-13, atol=0)
+    np.testing.assert_allclose(nod["bcU"], onod["bcU"], rtol=1e-13, atol=0)
+    np.testing.assert_array_equal(nod["rotm"], onod["rotm"])
+    ne = ctx.n_model_elems()
+    K, F = ctx.element_coefs(0, ne)
+    for k in range(ne):
+        Ko, Fo = om.coefs(k)
+        sk = max(np.abs(Ko).max(), 1e-300)
+        assert np.abs(K[k] - Ko).max() <= 1e-12 * sk, k
+        sf = max(np.abs(Fo).max(), np.abs(Ko).max() * 1e-3)
+        assert np.abs(F[k] - Fo).max() <= 1e-12 * sf, k
+    # assembled FP64 system
+    ctx.assemble(rmx.F64)
+    A, B = ctx.matrix(rmx.F64)
+    Ao, Bo = om.assemble(64)
+    rm, _ = _rowmax(rp, Ao)
+    assert (np.abs(A - Ao) <= 1e-12 * rm).all()
+    assert np.abs(B - Bo).max() <= 1e-12 * np.abs(Bo).max()
+    # FP32 stock semantics against the reference's own assembled values
+    ctx.assemble(rmx.F32)
+    A32, B32 = ctx.matrix(rmx.F32)
+    ref = g["s0_aval"]
+    rm32, _ = _rowmax(rp, ref)
+    assert (np.abs(A32.astype(np.float64) - ref) <= 2.4e-7 * rm32).all()
+    np.testing.assert_allclose(B32, g["s0_bvec"], rtol=0, atol=1e-12 * np.abs(g["s0_bvec"]).max())
+
+
+@pytest.mark.parametrize("name", ["tri20x4", "quad10x3"])
+def test_ilu_spmv_match_reference_records(ctx, name):
+    """Upload the reference's own scaled FP32 system, factor it on the GPU and compare
+    with the reference's ILU(0) factors, ILU apply and MulVec."""
+    print(f"[gpu] ILU/SpMV parity on {name}", flush=True)
+    g = load_golden(name)
+    rp = np.concatenate([[0], np.cumsum(g["s0_width"])]).astype(np.int32)
+    cols = g["s0_cols"].astype(np.int32)
+    ctx.csr_upload(rp, cols, g["s0_ascaled"], g["s0_bscaled"])
+    ctx.convert(rmx.F32)
+    ctx.ilu0_factor(rmx.F32)
+    L = ctx.factors(rmx.F32)
+    np.testing.assert_array_equal(L.astype(np.float64), g["s0_ilu"])
+    y = ctx.ilu0_apply(g["s0_bscaled"], rmx.F32)
+    ref = g["s0_iluapply"]
+    assert np.abs(y - ref).max() <= 1e-12 * np.abs(ref).max()
+    r = ctx.spmv(g["s0_bscaled"], rmx.F32)
+    ref = g["s0_mulvec"]
+    assert np.abs(r - ref).max() <= 1e-12 * np.abs(ref).max()
+    # FP64 factorisation of the same matrix against the oracle's FP64 ILU(0)
+    ctx.ilu0_factor(rmx.F64)
+    L64 = ctx.factors(rmx.F64)
+    Lo, _ = po.ilu0_factor(rp, cols, g["s0_ascaled"].astype(np.float64))
+    rm, _ = _rowmax(rp, Lo)
+    assert (np.abs(L64 - Lo) <= 1e-12 * rm).all()
+
+
+def _relerr(a, b, scale):
+    return np.abs(a - b).max() / scale
+
+
+@pytest.mark.parametrize("name", ["tri20x4", "quad10x3", "channel200x20"])
+def test_newton_converges_to_reference_solution(ctx, name):
+    """Full Newton iteration on the GPU (FP64 assembly, FP64 ILU0-BiCGStab) reaches the
+    reference program's converged u, v, h within 1e-8 relative."""
+    print(f"[gpu] Newton FP64 on {name}", flush=True)
+    g = load_golden(name)
+    deck = golden_deck(g)
+    ctx.upload_mesh(deck)
+    n = ctx.prepare()
+    if name == "channel200x20":
+        rp, cols = ctx.pattern()
+        h = hashlib.sha256(np.diff(rp).astype(np.int32).tobytes() + cols.tobytes()).hexdigest()
+        assert h == str(g["s0_pattern_sha256"])
+        eq = ctx.numbering()
+        np.testing.assert_array_equal(eq, np.stack([g["s0_eqU"], g["s0_eqV"], g["s0_eqS"]], 1))
+    for k in range(int(g["nsteps"]) + 1):
+        st = ctx.newton_step(method=rmx.BICGSTAB, prec=rmx.F64, eps=1e-12)
+        assert st.solve.converged, (k, st.solve.accuracy)
+        assert st.solve.true_resid < 1e-10
+    U, V, S, z = ctx.state()
+    Hr = g["final_S"] - g["final_z"]
+    us = np.abs(g["final_U"]).max()
+    assert _relerr(U, g["final_U"], us) <= 1e-8
+    assert _relerr(V, g["final_V"], us) <= 1e-8
+    assert _relerr(S - z, Hr, np.abs(Hr).max()) <= 1e-8
+    assert n == len(g["s0_xsol"])
+
+
+def test_first_newton_correction_matches_reference(ctx):
+    """Stock semantics (FP32 assembly, FP32 ILU, BiCGStab to 1e-10): the first Newton
+    correction agrees with the reference's to the solver tolerance."""
+    g = load_golden("tri20x4")
+    ctx.upload_mesh(golden_deck(g))
+    ctx.prepare()
+    ctx.assemble(rmx.F32)
+    s = ctx.scale_l2()
+    assert abs(s - float(g["s0_scale"])) <= 1e-13 * s
+    ctx.ilu0_factor(rmx.F32)
+    out = ctx.solve(rmx.BICGSTAB, rmx.F32, rmx.PREC_ILU0, eps=1e-10)
+    assert out.converged and abs(out.iters - int(g["bicgstab_iters"][0])) <= 2
+    X = ctx.solution()
+    ref = g["s0_xsol"]
+    assert np.abs(X - ref).max() <= 1e-7 * np.abs(ref).max()
+
+
+@pytest.mark.parametrize("method", [rmx.BICGSTAB, rmx.FGMRES])
+def test_mixed_precision_refinement_reaches_fp64_residual(ctx, method):
+    """FP32 matrix + FP32 ILU(0) inside FP64 iterative refinement reaches true FP64
+    relative residual 1e-8 and the FP64 solution."""
+    print(f"[gpu] mixed precision, method {method}", flush=True)
+    d = dk.channel(nx=60, ny=8, length=300.0, width=40.0, noise_corr=10.0, Q=40.0)
+    ctx.upload_mesh(d)
+    ctx.prepare()
+    ctx.assemble(rmx.F64)
+    ctx.scale_l2()
+    ctx.ilu0_factor(rmx.F64)
+    ref = ctx.solve(method, rmx.F64, rmx.PREC_ILU0, eps=1e-12)
+    X64 = ctx.solution()
+    assert ref.converged
+    ctx.convert(rmx.F32)
+    ctx.ilu0_factor(rmx.F32)
+    out = ctx.solve(method, rmx.F32, rmx.PREC_ILU0, eps=1e-5, refine=True, refine_tol=1e-8)
+    assert out.converged and out.true_resid <= 1e-8
+    X = ctx.solution()
+    assert np.abs(X - X64).max() <= 1e-6 * np.abs(X64).max()
+
+
+def _random_csr(n, seed, density):
+    rng = np.random.default_rng(seed)
+    M = rng.standard_normal((n, n)) * (rng.random((n, n)) < density)
+    M = M + M.T * 0.3
+    np.fill_diagonal(M, 0.0)
+    M += np.diag(np.abs(M).sum(1) + 1.0)
+    rp, cols, vals = [0], [], []
+    for i in range(n):
+        js = [i] + [j for j in rng.permutation(n) if j != i and M[i, j] != 0.0]
+        cols += js
+        vals += [M[i, j] for j in js]
+        rp.append(len(cols))
+    return M, np.asarray(rp, np.int32), np.asarray(cols, np.int32), np.asarray(vals)
+
+
+def test_csr_kernels_against_dense(ctx):
+    M, rp, cols, vals = _random_csr(300, 7, 0.03)
+    b = np.random.default_rng(8).standard_normal(300)
+    ctx.csr_upload(rp, cols, vals, b)
+    np.testing.assert_allclose(ctx.spmv(b), M @ b, rtol=1e-12, atol=1e-12)
+    ctx.ilu0_factor(rmx.F64)
+    Lo, _ = po.ilu0_factor(rp, cols, vals)
+    np.testing.assert_allclose(ctx.ilu0_apply(b), po.ilu0_solve(rp, cols, Lo, b), rtol=1e-11, atol=1e-12)
+    for method in (rmx.BICGSTAB, rmx.FGMRES):
+        out = ctx.solve(method, rmx.F64, rmx.PREC_ILU0, eps=1e-13)
+        assert out.converged
+        np.testing.assert_allclose(ctx.solution(), np.linalg.solve(M, b), rtol=1e-9, atol=1e-10)
+
+
+def test_full_pattern_ilu_is_exact_lu(ctx):
+    M, rp, cols, vals = _random_csr(40, 3, 1.0)
+    b = np.random.default_rng(4).standard_normal(40)
+    ctx.csr_upload(rp, cols, vals, b)
+    ctx.ilu0_factor(rmx.F64)
+    np.testing.assert_allclose(ctx.ilu0_apply(b), np.linalg.solve(M, b), rtol=1e-10)
+
+
+def test_error_paths(ctx):
+    M, rp, cols, vals = _random_csr(20, 5, 0.2)
+    bad = cols.copy()
+    bad[rp[3]] = 4                      # diagonal not first in row 3
+    with pytest.raises(rmx.RmxError):
+        ctx.csr_upload(rp, bad, vals, np.ones(20))
+    ctx.csr_upload(rp, cols, vals, np.ones(20))
+    with pytest.raises(rmx.RmxError):
+        ctx.solve(rmx.BICGSTAB, rmx.F32, rmx.PREC_ILU0)   # no FP32 matrix / factors
+    with pytest.raises(rmx.RmxError):
+        ctx.solve(rmx.FGMRES, rmx.F64, rmx.PREC_NONE, mkyrl=64)
+    z = vals.copy()
+    z[rp[:-1]] = 0.0                    # zero diagonal everywhere: pivot failure
+    ctx.csr_upload(rp, cols, z, np.ones(20))
+    with pytest.raises(rmx.RmxError) as e:
+        ctx.ilu0_factor(rmx.F64)
+    assert e.value.code == -3

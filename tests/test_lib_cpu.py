@@ -1,0 +1,51 @@
+"""CPU-side checks of the HIP library: it loads, exports every entry point that
+include/rmx.h declares, and reports the missing device cleanly (no compute on CPU)."""
+import ctypes
+
+import numpy as np
+import pytest
+
+from rismo2d_amd import deck as dk
+from rismo2d_amd import rmx
+
+
+def test_library_exports_every_declared_symbol():
+    names = rmx.header_functions()
+    assert len(names) >= 25
+    L = rmx.lib()
+    for n in names:
+        assert hasattr(L, n), n
+
+
+def test_create_without_gpu_reports_no_device():
+    import torch
+    if torch.cuda.is_available():
+        pytest.skip("a GPU is present")
+    h = ctypes.c_void_p()
+    rc = rmx.lib().rmx_create(0, ctypes.byref(h))
+    assert rc == rmx.ERR_NODEVICE
+    assert b"device" in rmx.lib().rmx_last_error()
+
+
+def test_null_context_is_an_argument_error():
+    rc = rmx.lib().rmx_assemble(None, 64)
+    assert rc == -1
+    assert b"null context" in rmx.lib().rmx_last_error()
+
+
+def test_deck_generators_shapes():
+    d = dk.channel(nx=200, ny=20)
+    assert d.n_elems == 8000 and d.n_nodes == 16441
+    q = dk.channel(nx=10, ny=3, quads=True)
+    assert q.n_elems == 30 and (q.nnd == 8).all()
+    m = dk.meander(n_along=40, n_across=8)
+    assert m.n_elems == 2 * 40 * 8
+    # every triangle is counter-clockwise
+    e = m.elems[:, :3]
+    x, y = m.x[e], m.y[e]
+    area2 = (x[:, 1] - x[:, 0]) * (y[:, 2] - y[:, 0]) - (x[:, 2] - x[:, 0]) * (y[:, 1] - y[:, 0])
+    assert (area2 > 0).all()
+    # midside nodes sit on their edges' midpoints
+    for i, (a, b) in enumerate([(0, 1), (1, 2), (2, 0)]):
+        mid = m.elems[:, 3 + i]
+        np.testing.assert_allclose(m.x[mid], 0.5 * (m.x[m.elems[:, a]] + m.x[m.elems[:, b]]), atol=1e-9)
