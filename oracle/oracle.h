@@ -66,6 +66,8 @@ int    orc_coefs(orc_model* m, int k, double* estifm, double* force);
 
 void   orc_assemble_f64(orc_model* m, double* vals, double* B);
 void   orc_assemble_f32(orc_model* m, float* vals, double* B);
+/* sum over elements of |contribution| for every CSR entry (FP64 instance) */
+void   orc_assemble_abs_f64(orc_model* m, double* vals);
 
 double orc_scale_f64(int n, const int* rowptr, double* vals, double* B);
 double orc_scale_f32(int n, const int* rowptr, float* vals, double* B);

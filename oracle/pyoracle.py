@@ -53,6 +53,7 @@ def lib():
         L.orc_coefs.argtypes = [C.c_void_p, C.c_int, _d, _d]
         L.orc_assemble_f64.argtypes = [C.c_void_p, _d, _d]
         L.orc_assemble_f32.argtypes = [C.c_void_p, _f, _d]
+        L.orc_assemble_abs_f64.argtypes = [C.c_void_p, _d]
         L.orc_scale_f64.argtypes = [C.c_int, _i, _d, _d]
         L.orc_scale_f64.restype = C.c_double
         L.orc_scale_f32.argtypes = [C.c_int, _i, _f, _d]
@@ -178,6 +179,12 @@ class OracleModel:
             A = np.zeros(nnz, np.float64)
             lib().orc_assemble_f64(self.h, _p(A, _d), _p(B, _d))
         return A, B
+
+    def assemble_abs(self):
+        """Per-entry sum of |element contributions| (scale of summation rounding)."""
+        S = np.zeros(lib().orc_nnz(self.h))
+        lib().orc_assemble_abs_f64(self.h, _p(S, _d))
+        return S
 
     def newton_step(self, prec=64, solver=1, max_iter=1000, eps=1e-10, mkyrl=30):
         st = StepStats()
