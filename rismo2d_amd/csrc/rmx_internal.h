@@ -111,6 +111,19 @@ struct Levels {
   DBuf<int> d_ptr, d_rows;
 };
 
+constexpr int RING = 8192;     // LDS ring of solved x values in the packed sweeps
+
+struct SweepPack {
+  int nlev = 0;
+  bool ring_ok = false;
+  int max_nr = 0;
+  long long ne = 0;
+  std::vector<int> h_lev;       // level row offsets (host)
+  DBuf<int> lev_ptr, prs, prow, pcol, psrc;
+  DBuf<double> pval64, pdiag64, pin, xp;
+  DBuf<float> pval32, pdiag32;
+};
+
 struct Ctx {
   int device = 0;
   hipStream_t stream = nullptr;
@@ -150,6 +163,8 @@ struct Ctx {
   long long nlo = 0, nup = 0;
   DBuf<double> lval64, uval64, diag64;
   DBuf<float> lval32, uval32, diag32;
+  SweepPack pk_lo, pk_up;                     // packed level-order sweeps
+  DBuf<int> fwd2bwd;                          // forward position of each backward position
   DBuf<int> srt_cols;                         // per-row sorted columns (factor lookups)
   DBuf<unsigned char> srt_pos;                // original position of each sorted column
   int max_width = 0;

@@ -176,13 +176,15 @@ def meander(n_along: int = 3200, n_across: int = 320, reach: float = 10000.0,
     Centreline ``y_c = amplitude*sin(2*pi*s/wavelength)`` over ``reach`` metres.  The
     cross-section is a trapezoidal main channel (``main_width`` wide at the bed,
     ``main_depth`` deep, banks of slope ``bank_slope``) cut into floodplains that span
-    ``width`` in total.  The lattice is mapped along the centreline (normal offsets) and
-    interior corner nodes are jittered by ``jitter`` of the local spacing, so element
+    ``width`` in total.  The lattice follows the centreline and interior corner nodes
+    are jittered by ``jitter`` of the local spacing, so element
     shapes and sizes vary (unstructured connectivity is not needed for the solver's cost
     model: row widths and the frontal ordering depend on the element adjacency, which is
     that of a triangulated lattice).  The diagonal of every cell alternates, giving
     vertices of degree 4, 6 and 8.  Bathymetry adds a seeded Gaussian random field;
     Manning n is drawn per element from ``U[n_range]``.  ``n_along*n_across*2`` triangles.
+    Lateral offsets are vertical (a shear of the straight lattice), so the section
+    measured normal to the centreline narrows by cos(slope angle) at the crossings.
     """
     nx, ny = n_along, n_across
     NX, NY = 2 * nx + 1, 2 * ny + 1
@@ -207,15 +209,12 @@ def meander(n_along: int = 3200, n_across: int = 320, reach: float = 10000.0,
     dt_ = t[1] - t[0]
     sc = S_ * ds + js * ds
     tc = (T_ - (NY - 1) / 2.0) * dt_ + jt * dt_
-    # corner positions
-    cx = np.interp(sc, s, s) + 0.0
-    xc_s = np.interp(sc, s, s)
-    yc_s = np.interp(sc, s, yc)
-    oxs = np.interp(sc, s, ox)
-    oys = np.interp(sc, s, oy)
-    Xc = xc_s + tc * oxs
-    Yc = yc_s + tc * oys
-    del cx
+    # sheared mapping: lateral offsets are taken in y, so cells can never fold over at
+    # the inner bends however tight the meander (a normal offset would invert cells
+    # wherever |t| exceeds the bend radius)
+    Xc = sc.copy()
+    Yc = np.interp(sc, s, yc) + tc
+    del tx, ty, ox, oy, dyc, nrm
     # bed: valley slope + trapezoidal section + GRF, evaluated at corner positions
     half_bed = 0.5 * main_width
     dist = np.abs(tc)
