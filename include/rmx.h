@@ -34,7 +34,10 @@ enum {
 
 enum { RMX_F64 = 64, RMX_F32 = 32 };           /* matrix / factor storage precision */
 enum { RMX_BICGSTAB = 1, RMX_FGMRES = 2 };      /* Krylov method (SOLVER types) */
-enum { RMX_PREC_ILU0 = 1, RMX_PREC_NONE = 0, RMX_PREC_MPNN = 2 };
+/* preconditioners: ILU(0) in the reference's equation order; ILU(0) of strips of
+ * consecutive equations with the couplings between strips dropped (strips factor and
+ * sweep in parallel); the learned message-passing network */
+enum { RMX_PREC_NONE = 0, RMX_PREC_ILU0 = 1, RMX_PREC_MPNN = 2, RMX_PREC_SILU0 = 3 };
 
 const char* rmx_last_error(void);
 
@@ -114,6 +117,15 @@ rmx_status rmx_ilu0_factor(rmx_ctx* ctx, int prec);
  * (Preco_ilu0.cpp:497).  Host pointers, length neq. */
 rmx_status rmx_ilu0_apply(rmx_ctx* ctx, int prec, const double* b, double* x);
 
+/* Strip ILU(0) (RMX_PREC_SILU0): factor and apply.  No reference counterpart: the
+ * reference's PRECO_ILU0 with the couplings between strips of consecutive equations
+ * dropped, the form its MPI build factors per subdomain (Preco_ilu0.cpp:201-485) without
+ * the interface pipeline.  rmx_set_strips chooses the number of strips (0 = one per
+ * compute unit) and rebuilds the schedule. */
+rmx_status rmx_silu0_factor(rmx_ctx* ctx, int prec);
+rmx_status rmx_silu0_apply(rmx_ctx* ctx, int prec, const double* b, double* x);
+rmx_status rmx_set_strips(rmx_ctx* ctx, int strips);
+
 /* r = A x: replaces CRSMAT::MulVec (CRSMat.cpp:466).  Host pointers. */
 rmx_status rmx_spmv(rmx_ctx* ctx, int prec, const double* x, double* r);
 
@@ -175,11 +187,12 @@ rmx_status rmx_newton_update(rmx_ctx* ctx, rmx_step_stats* st);
  * update. */
 rmx_status rmx_newton_step(rmx_ctx* ctx, const rmx_krylov_opts* opt, rmx_step_stats* st);
 
-/* Timing of the kernels of the last rmx_solve (HIP events on the context stream):
- * average duration in seconds of one SpMV, one ILU apply (forward + backward sweeps) and
- * the number of each that ran. */
+/* Kernel timing with HIP events on the context stream: average duration in seconds of
+ * one SpMV and of the forward / backward sweeps of each factored ILU(0), over `reps`
+ * runs on the current system. */
 typedef struct {
-  double spmv_s, ilu_apply_s, ilu_fwd_s, ilu_bwd_s;
+  double spmv_s, ilu_apply_s, ilu_fwd_s, ilu_bwd_s;   /* reference-order ILU(0) */
+  double silu_fwd_s, silu_bwd_s;                      /* strip ILU(0) (0 if not factored) */
   long long n_spmv, n_ilu;
 } rmx_kernel_times;
 rmx_status rmx_profile_kernels(rmx_ctx* ctx, int prec, int reps, rmx_kernel_times* out);

@@ -587,7 +587,7 @@ void dev_assemble(Ctx& c, int prec, bool zero)
     launch_groups<double>(c, c.A64.p);
     c.have64 = true; c.have32 = false;
   }
-  c.fac32 = c.fac64 = false;
+  c.ilu.fac32 = c.ilu.fac64 = c.silu.fac32 = c.silu.fac64 = false;
 }
 
 void dev_element_coefs(Ctx& c, int first, int count, double* K, double* F)

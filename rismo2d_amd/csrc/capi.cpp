@@ -211,7 +211,7 @@ static void first_prepare(Ctx& c)
   HIPCHECK(hipStreamSynchronize(s));
   ensure_work(c);
   setup_ilu_schedule(c);
-  c.have32 = c.have64 = c.fac32 = c.fac64 = false;
+  c.have32 = c.have64 = false;
   c.prepared = true;
 }
 
@@ -298,7 +298,7 @@ rmx_status rmx_scale_l2(rmx_ctx* p, double* scale)
   if (!c.have64 && !c.have32) fail(RMX_ERR_ARG, "scale: no matrix");
   const double s = dev_scale_l2(c);
   if (scale) *scale = s;
-  c.fac32 = c.fac64 = false;
+  c.ilu.fac32 = c.ilu.fac64 = c.silu.fac32 = c.silu.fac64 = false;
   API_END
 }
 
@@ -316,6 +316,37 @@ rmx_status rmx_ilu0_factor(rmx_ctx* p, int prec)
   API_BEGIN
   Ctx& c = *need(p);
   dev_ilu0_factor(c, prec == RMX_F32 ? RMX_F32 : RMX_F64);
+  API_END
+}
+
+rmx_status rmx_silu0_factor(rmx_ctx* p, int prec)
+{
+  API_BEGIN
+  Ctx& c = *need(p);
+  if (!c.prepared) fail(RMX_ERR_ARG, "no system");
+  ilu_factor(c, c.silu, prec == RMX_F32 ? RMX_F32 : RMX_F64);
+  API_END
+}
+
+rmx_status rmx_silu0_apply(rmx_ctx* p, int prec, const double* b, double* x)
+{
+  API_BEGIN
+  Ctx& c = *need(p);
+  if (!c.prepared) fail(RMX_ERR_ARG, "no system");
+  c.w[0].upload(b, c.n, c.stream);
+  ilu_apply(c, c.silu, prec == RMX_F32 ? RMX_F32 : RMX_F64, c.w[0].p, c.w[1].p, nullptr);
+  c.w[1].download(x, c.n, c.stream);
+  HIPCHECK(hipStreamSynchronize(c.stream));
+  API_END
+}
+
+rmx_status rmx_set_strips(rmx_ctx* p, int strips)
+{
+  API_BEGIN
+  Ctx& c = *need(p);
+  if (strips < 0) fail(RMX_ERR_ARG, "strips must be >= 0");
+  c.strips = strips;
+  if (c.prepared) setup_ilu_schedule(c);
   API_END
 }
 
@@ -378,11 +409,11 @@ rmx_status rmx_get_factors(rmx_ctx* p, int prec, void* vals)
   API_BEGIN
   Ctx& c = *need(p);
   if (prec == RMX_F32) {
-    if (!c.fac32) fail(RMX_ERR_ARG, "no FP32 factors");
-    c.L32.download(static_cast<float*>(vals), c.nnz, c.stream);
+    if (!c.ilu.fac32) fail(RMX_ERR_ARG, "no FP32 factors");
+    c.ilu.LU32.download(static_cast<float*>(vals), c.nnz, c.stream);
   } else {
-    if (!c.fac64) fail(RMX_ERR_ARG, "no FP64 factors");
-    c.L64.download(static_cast<double*>(vals), c.nnz, c.stream);
+    if (!c.ilu.fac64) fail(RMX_ERR_ARG, "no FP64 factors");
+    c.ilu.LU64.download(static_cast<double*>(vals), c.nnz, c.stream);
   }
   HIPCHECK(hipStreamSynchronize(c.stream));
   API_END
@@ -432,7 +463,7 @@ rmx_status rmx_csr_upload(rmx_ctx* p, int n, const int* rowptr, const int* cols,
   HIPCHECK(hipStreamSynchronize(c.stream));
   ensure_work(c);
   setup_ilu_schedule(c);
-  c.have64 = true; c.have32 = false; c.fac32 = c.fac64 = false;
+  c.have64 = true; c.have32 = false;
   c.prepared = true;
   API_END
 }
@@ -466,7 +497,8 @@ rmx_status rmx_newton_step(rmx_ctx* p, const rmx_krylov_opts* o, rmx_step_stats*
   auto t2 = now();
   const int prec = o->prec == RMX_F32 ? RMX_F32 : RMX_F64;
   if (prec == RMX_F32) dev_convert(c, RMX_F32);
-  if (o->precond == RMX_PREC_ILU0) dev_ilu0_factor(c, prec);
+  if (o->precond == RMX_PREC_ILU0 || o->precond == RMX_PREC_SILU0)
+    ilu_factor(c, ilu_of(c, o->precond), prec);
   HIPCHECK(hipStreamSynchronize(c.stream));
   auto t3 = now();
   dev_solve(c, *o, st->solve);

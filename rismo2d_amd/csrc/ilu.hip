@@ -1,0 +1,639 @@
+// ILU(0) preconditioner on gfx950: factorisation and triangular sweeps
+// (PRECO_ILU0::Factor / Solve, Preco_ilu0.cpp:64-186, 497-530, 958-980).
+//
+// An Ilu object holds one ILU(0) system: a CSR pattern (the system's own, or a
+// restriction of it), the level sets of its triangular dependency graphs, the packed
+// sweep layouts, and FP64 / FP32 factors.  Two instances exist per context:
+//  * the reference ILU(0) on the full pattern in the reference's equation order (the
+//    FP64 plain path; its FP32 factors are bit-identical to the reference's);
+//  * a strip ILU(0): rows are cut into contiguous ranges (the frontal order runs along
+//    the channel, so a range is a strip across the channel) and couplings between strips
+//    are dropped, so the strips factor and sweep independently, one workgroup each.
+//    It preconditions the FP32 solve inside the FP64 refinement, where the FP64 residual,
+//    not the preconditioner, decides the answer.
+//
+// Factorisation: row-wise, one wave per row, the rows of one level in one launch; row i
+// eliminates its lower entries in increasing column order, so every factor entry gets its
+// updates in the reference's order and, with FMA contraction off, the reference's
+// rounding.  Multipliers are stored negated as in the reference.
+//
+// Sweeps: one workgroup per strip walks its level sets.  Rows are stored in processing
+// order with each row's off-diagonal entries contiguous and columns replaced by
+// processing positions.  Solved values go to an LDS ring (slot = position & (RING-1)) and
+// to a position-ordered global copy; a level reads a dependency from the ring while its
+// slot has not been reused (position >= P + nr - RING for a level of nr rows starting at
+// P) and from the global copy otherwise, which is safe because a full barrier (global
+// release / acquire) is taken whenever the position crosses a multiple of FENCE, and any
+// value that has left the ring is older than the last such barrier (nr <= RING/4).
+// Wider levels read everything from the global copy between full barriers.  Other levels
+// are separated by LDS-only barriers, so the next level's rows, loaded into registers
+// while the current level finishes, stay in flight across the barrier.
+#include <algorithm>
+#include <climits>
+#include <cmath>
+#include <cstdio>
+#include <cstdlib>
+
+#include "rmx_internal.h"
+
+namespace rmx {
+
+constexpr int FW = 128;         // maximum row width handled by the factorisation
+constexpr int PW = 6;           // entries per lane held in registers (more are read on)
+constexpr int SG = 8;           // lanes per row in the sweeps
+constexpr int SKR = 2;          // row-rounds per level held in registers
+constexpr int LPC = 1024;       // level offsets staged in LDS at a time
+constexpr int FENCE = 2048;     // positions between full barriers
+constexpr int WIDE = RING / 4;  // levels wider than this use the global copy only
+
+// ---------------------------------------------------------------------------------------
+// factorisation
+
+template <class VT>
+__global__ void __launch_bounds__(64) k_ilu_level(const int* __restrict__ rows, int cnt,
+                                                  const int* __restrict__ rowptr,
+                                                  const int* __restrict__ cols,
+                                                  const int* __restrict__ srt_cols,
+                                                  const unsigned char* __restrict__ srt_pos,
+                                                  VT* __restrict__ LU, int* __restrict__ err)
+{
+#pragma clang fp contract(off)
+  __shared__ VT s_val[FW];
+  __shared__ int s_col[FW];
+  __shared__ unsigned char s_pos[FW];
+  const int lane = threadIdx.x;
+  const int w = blockIdx.x;
+  if (w >= cnt) return;                 // uniform per block
+  const int i = rows[w];
+  const int p0 = rowptr[i];
+  const int wi = rowptr[i + 1] - p0;
+  for (int t = lane; t < wi; t += 64) {
+    s_val[t] = LU[p0 + t];
+    s_col[t] = srt_cols[p0 + t];
+    s_pos[t] = srt_pos[p0 + t];
+  }
+  __syncthreads();
+  for (int t = 0; t < wi; t++) {
+    const int k = s_col[t];
+    if (k >= i) break;
+    const int pik = s_pos[t];
+    const int q0 = rowptr[k], q1 = rowptr[k + 1];
+    const VT ukk = LU[q0];
+    const VT f = -s_val[pik] / ukk;
+    __syncthreads();
+    if (lane == 0) s_val[pik] = f;
+    __syncthreads();
+    for (int q = q0 + 1 + lane; q < q1; q += 64) {
+      const int cc = cols[q];
+      if (cc <= k) continue;
+      int lo = 0, hi = wi - 1, pos = -1;
+      while (lo <= hi) {
+        const int mid = (lo + hi) >> 1;
+        const int v = s_col[mid];
+        if (v == cc) { pos = s_pos[mid]; break; }
+        if (v < cc) lo = mid + 1; else hi = mid - 1;
+      }
+      if (pos >= 0) {
+        const VT prod = f * LU[q];
+        s_val[pos] = s_val[pos] + prod;
+      }
+    }
+    __syncthreads();
+  }
+  for (int t = lane; t < wi; t += 64) LU[p0 + t] = s_val[t];
+  if (lane == 0 && fabs((double)s_val[0]) < 1.0e-20) atomicOr(err, 1);
+}
+
+// values of a restricted pattern from the system matrix
+template <class VT>
+__global__ void k_take(long long ne, const long long* __restrict__ src, const VT* __restrict__ A,
+                       VT* __restrict__ out)
+{
+  for (long long t = (long long)blockIdx.x * blockDim.x + threadIdx.x; t < ne;
+       t += (long long)gridDim.x * blockDim.x)
+    out[t] = A[src[t]];
+}
+
+template <class VT>
+__global__ void k_pack(long long ne, const int* __restrict__ psrc, const VT* __restrict__ LU,
+                       VT* __restrict__ pval)
+{
+  for (long long t = (long long)blockIdx.x * blockDim.x + threadIdx.x; t < ne;
+       t += (long long)gridDim.x * blockDim.x)
+    pval[t] = LU[psrc[t]];
+}
+
+template <class VT>
+__global__ void k_pack_diag(int n, const int* __restrict__ prow, const int* __restrict__ rowptr,
+                            const VT* __restrict__ LU, VT* __restrict__ pdiag)
+{
+  const int r = blockIdx.x * blockDim.x + threadIdx.x;
+  if (r < n) pdiag[r] = LU[rowptr[prow[r]]];
+}
+
+// ---------------------------------------------------------------------------------------
+// sweeps
+
+__device__ __forceinline__ void lds_barrier()
+{
+  __builtin_amdgcn_fence(__ATOMIC_RELEASE, "workgroup", "local");
+  __builtin_amdgcn_s_barrier();
+  __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "workgroup", "local");
+}
+
+__device__ __forceinline__ double ring_or_global(const double* ring, const double* xp, int c, int lim)
+{
+  double v;
+  if (c >= lim) v = ring[c & (RING - 1)];
+  else v = __builtin_nontemporal_load(xp + c);
+  return v;
+}
+
+// Block b of the sweep: levels [blev[b], blev[b+1]) of the concatenated level table, rows
+// at positions [bpos[b], bpos[b+1]).  Level offsets, columns and ring slots are local to
+// the block; each block table has one extra sentinel level offset.
+template <class VT, bool LOWER, int G, int KR>
+__global__ void __launch_bounds__(1024) k_sweep(const int* __restrict__ blev,
+                                                const int* __restrict__ bpos,
+                                                const int* __restrict__ lev_ptr_all,
+                                                const int* __restrict__ prs_all,
+                                                const int* __restrict__ pcol,
+                                                const VT* __restrict__ pval,
+                                                const VT* __restrict__ pdiag_all,
+                                                const double* __restrict__ pin_all,
+                                                double* __restrict__ xp_all)
+{
+  __shared__ double ring[RING];
+  __shared__ int s_lp[LPC + 2];
+  const int b = blockIdx.x;
+  const int lb = blev[b];
+  const int nlev = blev[b + 1] - lb;
+  const int* lev_ptr = lev_ptr_all + lb + 2 * b;   // nlev + 2 local offsets
+  const int base = bpos[b];
+  const int* prs = prs_all + base;
+  const VT* pdiag = LOWER ? nullptr : pdiag_all + base;
+  const double* pin = pin_all + base;
+  double* xp = xp_all + base;
+  const int t = threadIdx.x;
+  const int grp = t / G, lane = t % G, ngr = blockDim.x / G;
+  auto load_lp = [&](int l0) {
+    for (int k = t; k < LPC + 2; k += blockDim.x) s_lp[k] = lev_ptr[min(l0 + k, nlev + 1)];
+  };
+  load_lp(0);
+  __syncthreads();
+  if (nlev <= 0) return;
+  int lbase = 0;
+  int cr[KR][PW];
+  VT vr[KR][PW];
+  int qar[KR], qbr[KR];
+  double inr[KR];
+  VT dgr[KR];
+  int P = s_lp[0], nr = s_lp[1] - s_lp[0];
+  auto fetch = [&](int P_, int nr_) {
+#pragma unroll
+    for (int k = 0; k < KR; k++) {
+      const int g = grp + k * ngr;
+      if (g < nr_) {
+        const int r = P_ + g;
+        inr[k] = pin[r];
+        if (!LOWER) dgr[k] = pdiag[r];
+        qar[k] = prs[r];
+        qbr[k] = prs[r + 1];
+#pragma unroll
+        for (int j = 0; j < PW; j++) {
+          const int q = qar[k] + lane + j * G;
+          if (q < qbr[k]) { cr[k][j] = pcol[q]; vr[k][j] = pval[q]; }
+          else { cr[k][j] = -1; vr[k][j] = (VT)0; }
+        }
+      }
+    }
+  };
+  fetch(P, nr);
+  for (int l = 0; l < nlev; l++) {
+    // positions >= lim are still in the ring; a wide level reads the global copy only
+    const int lim = nr > WIDE ? INT_MAX : P + nr - RING;
+    const int rounds = (nr + ngr - 1) / ngr;
+    for (int k = 0; k < rounds; k++) {
+      const int g = grp + k * ngr;
+      const bool act = g < nr;
+      double s = 0.0;
+      double in = 0.0;
+      VT dg = (VT)1;
+      if (act) {
+        if (k < KR) {
+          const int kk = k < KR ? k : 0;
+          in = inr[kk];
+          dg = dgr[kk];
+          bool old = false;
+#pragma unroll
+          for (int j = 0; j < PW; j++) {
+            const int c = cr[kk][j];
+            if (c >= 0) {
+              old |= c < lim;
+              s += (double)vr[kk][j] * ring[c & (RING - 1)];
+            }
+          }
+          if (old) {                         // a dependency has left the ring: redo the row
+            s = 0.0;
+            for (int j = 0; j < PW; j++) {
+              const int c = cr[kk][j];
+              if (c >= 0) s += (double)vr[kk][j] * ring_or_global(ring, xp, c, lim);
+            }
+          }
+          for (int q = qar[kk] + lane + PW * G; q < qbr[kk]; q += G)
+            s += (double)pval[q] * ring_or_global(ring, xp, pcol[q], lim);
+        } else {
+          const int r = P + g;
+          in = pin[r];
+          if (!LOWER) dg = pdiag[r];
+          for (int q = prs[r] + lane; q < prs[r + 1]; q += G)
+            s += (double)pval[q] * ring_or_global(ring, xp, pcol[q], lim);
+        }
+      }
+#pragma unroll
+      for (int o = G / 2; o > 0; o >>= 1) s += __shfl_xor(s, o, G);
+      if (act && lane == 0) {
+        const double xi = LOWER ? in + s : (in - s) / (double)dg;
+        ring[(P + g) & (RING - 1)] = xi;
+        xp[P + g] = xi;
+      }
+    }
+    const int Pn = P + nr;
+    bool full = Pn / FENCE != P / FENCE || nr > WIDE;
+    if (l + 1 < nlev) {
+      if (l + 2 - lbase > LPC) {            // next chunk of level offsets (rare)
+        __syncthreads();
+        lbase = l + 1;
+        load_lp(lbase);
+        __syncthreads();
+      }
+      P = s_lp[l + 1 - lbase];
+      nr = s_lp[l + 2 - lbase] - P;
+      full |= nr > WIDE;                    // a wide level starts after a full barrier
+      fetch(P, nr);                         // next level's registers, in flight across the barrier
+    }
+    if (full) __syncthreads();
+    else lds_barrier();
+  }
+}
+
+__global__ void k_gather(int n, const int* __restrict__ idx, const double* __restrict__ src,
+                         double* __restrict__ dst)
+{
+  const int r = blockIdx.x * blockDim.x + threadIdx.x;
+  if (r < n) dst[r] = src[idx[r]];
+}
+
+__global__ void k_scatter(int n, const int* __restrict__ idx, const double* __restrict__ src,
+                          double* __restrict__ dst)
+{
+  const int r = blockIdx.x * blockDim.x + threadIdx.x;
+  if (r < n) dst[idx[r]] = src[r];
+}
+
+// ---------------------------------------------------------------------------------------
+// host set-up
+
+// Level sets of the lower or upper dependency graph of rows [a, e).
+static void levels(int a, int e, const std::vector<int>& rp, const std::vector<int>& cl, bool lower,
+                   std::vector<int>& lev, std::vector<int>& ptr, std::vector<int>& rows)
+{
+  int nlev = 0;
+  if (lower) {
+    for (int i = a; i < e; i++) {
+      int l = 0;
+      for (int q = rp[i] + 1; q < rp[i + 1]; q++)
+        if (cl[q] < i) l = std::max(l, lev[cl[q]] + 1);
+      lev[i] = l;
+      nlev = std::max(nlev, l + 1);
+    }
+  } else {
+    for (int i = e - 1; i >= a; i--) {
+      int l = 0;
+      for (int q = rp[i] + 1; q < rp[i + 1]; q++)
+        if (cl[q] > i) l = std::max(l, lev[cl[q]] + 1);
+      lev[i] = l;
+      nlev = std::max(nlev, l + 1);
+    }
+  }
+  ptr.assign(nlev + 1, 0);
+  for (int i = a; i < e; i++) ptr[lev[i] + 1]++;
+  for (int l = 0; l < nlev; l++) ptr[l + 1] += ptr[l];
+  rows.assign(e - a, 0);
+  std::vector<int> fill(ptr.begin(), ptr.end() - 1);
+  if (lower) { for (int i = a; i < e; i++) rows[fill[lev[i]]++] = i; }
+  else { for (int i = e - 1; i >= a; i--) rows[fill[lev[i]]++] = i; }
+}
+
+// Packed layout of one sweep over all blocks.
+static void build_sweep(SweepSet& S, const Ilu& I, bool lower, std::vector<int>& order,
+                        hipStream_t s)
+{
+  const int n = I.n, nb = I.nblk;
+  const std::vector<int>& rp = I.h_rp;
+  const std::vector<int>& cl = I.h_cols;
+  std::vector<int> lev(n, 0);
+  std::vector<int> blev(nb + 1, 0), bpos(nb + 1, 0), levp, prow;
+  levp.reserve(n / 8);
+  prow.reserve(n);
+  int maxnr = 0, nlev_tot = 0;
+  for (int b = 0; b < nb; b++) {
+    const int a = I.bstart[b], e = I.bstart[b + 1];
+    std::vector<int> ptr, rows;
+    levels(a, e, rp, cl, lower, lev, ptr, rows);
+    const int nl = (int)ptr.size() - 1;
+    blev[b + 1] = blev[b] + nl;
+    bpos[b + 1] = bpos[b] + (e - a);
+    for (int l = 0; l <= nl; l++) levp.push_back(ptr[l]);
+    levp.push_back(ptr[nl]);                       // sentinel: lev_ptr[l+2] for the last level
+    for (int l = 0; l < nl; l++) maxnr = std::max(maxnr, ptr[l + 1] - ptr[l]);
+    prow.insert(prow.end(), rows.begin(), rows.end());
+    nlev_tot = std::max(nlev_tot, nl);
+  }
+  // positions: local to each block
+  std::vector<int> pos(n);
+  for (int b = 0; b < nb; b++)
+    for (int r = bpos[b]; r < bpos[b + 1]; r++) pos[prow[r]] = r - bpos[b];
+  std::vector<int> prs(n + 1, 0);
+  for (int r = 0; r < n; r++) {
+    const int i = prow[r];
+    int k = 0;
+    for (int q = rp[i] + 1; q < rp[i + 1]; q++) k += lower ? (cl[q] < i) : (cl[q] > i);
+    prs[r + 1] = prs[r] + k;
+  }
+  const long long ne = prs[n];
+  std::vector<int> pcol(ne), psrc(ne);
+  parallel_for(n, [&](long long a, long long b) {
+    for (long long r = a; r < b; r++) {
+      const int i = prow[r];
+      int j = prs[r];
+      for (int q = rp[i] + 1; q < rp[i + 1]; q++)
+        if (lower ? (cl[q] < i) : (cl[q] > i)) { pcol[j] = pos[cl[q]]; psrc[j] = q; j++; }
+    }
+  });
+  S.nblk = nb;
+  S.nlev_max = nlev_tot;
+  S.max_nr = maxnr;
+  S.ne = ne;
+  S.nlev_sum = blev[nb];
+  if (getenv("RMX_DEBUG"))
+    fprintf(stderr, "[rmx] %s sweep, %d strip(s): %d levels (longest strip), %.1f rows/level, "
+            "max %d, %lld entries\n", lower ? "lower" : "upper", nb, nlev_tot,
+            (double)n / std::max(1, blev[nb]), maxnr, ne);
+  S.blev.upload(blev, s); S.bpos.upload(bpos, s); S.lev_ptr.upload(levp, s);
+  S.prs.upload(prs, s); S.prow.upload(prow, s); S.pcol.upload(pcol, s); S.psrc.upload(psrc, s);
+  S.pin.alloc(n); S.xp.alloc(n);
+  order = prow;
+}
+
+// Strips: contiguous row ranges of about equal size.  With `group` (the node of every
+// equation) a cut moves forward to the next row of another node: splitting the U, V, H
+// equations of one node between two strips leaves near-zero pivots.
+void ilu_setup(Ilu& I, int n, const std::vector<int>& rp, const std::vector<int>& cl, int nblk,
+               const std::vector<int>* group, hipStream_t s)
+{
+  I.ready = I.fac64 = I.fac32 = false;
+  I.src.release();
+  I.n = n;
+  I.nblk = std::max(1, std::min(nblk, n));
+  I.bstart.assign(I.nblk + 1, 0);
+  for (int b = 0; b <= I.nblk; b++) {
+    int cut = (int)((long long)n * b / I.nblk);
+    if (group && b > 0) {
+      cut = std::max(cut, I.bstart[b - 1]);
+      while (cut > 0 && cut < n && (*group)[cut] == (*group)[cut - 1]) cut++;
+    }
+    I.bstart[b] = cut;
+  }
+  I.bstart[I.nblk] = n;
+  std::vector<int> blk(n);
+  for (int b = 0; b < I.nblk; b++)
+    for (int i = I.bstart[b]; i < I.bstart[b + 1]; i++) blk[i] = b;
+  // restricted pattern (the full pattern when there is one strip)
+  std::vector<long long> src;
+  I.h_rp.assign(n + 1, 0);
+  if (I.nblk == 1) {
+    I.h_rp = rp;
+    I.h_cols = cl;
+  } else {
+    for (int i = 0; i < n; i++) {
+      int k = 0;
+      for (int q = rp[i]; q < rp[i + 1]; q++) k += blk[cl[q]] == blk[i];
+      I.h_rp[i + 1] = I.h_rp[i] + k;
+    }
+    I.h_cols.resize(I.h_rp[n]);
+    src.resize(I.h_rp[n]);
+    for (int i = 0; i < n; i++) {
+      int j = I.h_rp[i];
+      for (int q = rp[i]; q < rp[i + 1]; q++)
+        if (blk[cl[q]] == blk[i]) { I.h_cols[j] = cl[q]; src[j] = q; j++; }
+    }
+  }
+  I.nnz = I.h_rp[n];
+  // factorisation schedule: lower levels over all rows (strips are independent)
+  std::vector<int> lev(n, 0), fptr, frows;
+  levels(0, n, I.h_rp, I.h_cols, true, lev, fptr, frows);
+  I.flev = fptr;
+  I.flev_rows.upload(frows, s);
+  // per-row sorted columns for the factorisation lookups
+  std::vector<int> scol(I.nnz);
+  std::vector<unsigned char> spos(I.nnz);
+  for (int i = 0; i < n; i++)
+    if (I.h_rp[i + 1] - I.h_rp[i] > FW) fail(RMX_ERR_RANGE, "ILU(0): row wider than 128 columns");
+  parallel_for(n, [&](long long a, long long b) {
+    std::vector<std::pair<int, int>> tmp;
+    for (long long i = a; i < b; i++) {
+      tmp.clear();
+      for (int q = I.h_rp[i]; q < I.h_rp[i + 1]; q++) tmp.emplace_back(I.h_cols[q], q - I.h_rp[i]);
+      std::sort(tmp.begin(), tmp.end());
+      for (size_t t = 0; t < tmp.size(); t++) {
+        scol[I.h_rp[i] + t] = tmp[t].first;
+        spos[I.h_rp[i] + t] = (unsigned char)tmp[t].second;
+      }
+    }
+  });
+  I.rp.upload(I.h_rp, s); I.cols.upload(I.h_cols, s);
+  I.srt_cols.upload(scol, s); I.srt_pos.upload(spos, s);
+  if (!src.empty()) I.src.upload(src, s);
+  std::vector<int> olo, oup;
+  build_sweep(I.lo, I, true, olo, s);
+  build_sweep(I.up, I, false, oup, s);
+  // forward position (global, concatenated) of the row at each backward position
+  std::vector<int> fpos(n), f2b(n);
+  for (int r = 0; r < n; r++) fpos[olo[r]] = r;
+  for (int r = 0; r < n; r++) f2b[r] = fpos[oup[r]];
+  I.fwd2bwd.upload(f2b, s);
+  HIPCHECK(hipStreamSynchronize(s));
+  I.ready = true;
+}
+
+// ---------------------------------------------------------------------------------------
+// factor / apply
+
+template <class VT> static DBuf<VT>& LUbuf(Ilu& I);
+template <> DBuf<double>& LUbuf<double>(Ilu& I) { return I.LU64; }
+template <> DBuf<float>& LUbuf<float>(Ilu& I) { return I.LU32; }
+template <class VT> static DBuf<VT>& PLbuf(Ilu& I);
+template <> DBuf<double>& PLbuf<double>(Ilu& I) { return I.plo64; }
+template <> DBuf<float>& PLbuf<float>(Ilu& I) { return I.plo32; }
+template <class VT> static DBuf<VT>& PUbuf(Ilu& I);
+template <> DBuf<double>& PUbuf<double>(Ilu& I) { return I.pup64; }
+template <> DBuf<float>& PUbuf<float>(Ilu& I) { return I.pup32; }
+template <class VT> static DBuf<VT>& PDbuf(Ilu& I);
+template <> DBuf<double>& PDbuf<double>(Ilu& I) { return I.pdg64; }
+template <> DBuf<float>& PDbuf<float>(Ilu& I) { return I.pdg32; }
+
+template <class VT>
+static void factor_t(Ctx& c, Ilu& I, const VT* A)
+{
+  DBuf<VT>& LU = LUbuf<VT>(I);
+  LU.alloc(I.nnz);
+  if (I.nblk == 1)
+    HIPCHECK(hipMemcpyAsync(LU.p, A, sizeof(VT) * I.nnz, hipMemcpyDeviceToDevice, c.stream));
+  else
+    k_take<VT><<<4096, 256, 0, c.stream>>>(I.nnz, I.src.p, A, LU.p);
+  int* err = reinterpret_cast<int*>(c.red.p + 8);
+  HIPCHECK(hipMemsetAsync(err, 0, sizeof(int), c.stream));
+  for (size_t l = 0; l + 1 < I.flev.size(); l++) {
+    const int a = I.flev[l], cnt = I.flev[l + 1] - a;
+    k_ilu_level<VT><<<cnt, 64, 0, c.stream>>>(I.flev_rows.p + a, cnt, I.rp.p, I.cols.p,
+                                              I.srt_cols.p, I.srt_pos.p, LU.p, err);
+  }
+  HIPCHECK(hipGetLastError());
+  DBuf<VT>& pl = PLbuf<VT>(I);
+  DBuf<VT>& pu = PUbuf<VT>(I);
+  DBuf<VT>& pd = PDbuf<VT>(I);
+  pl.alloc(std::max<long long>(I.lo.ne, 1));
+  pu.alloc(std::max<long long>(I.up.ne, 1));
+  pd.alloc(I.n);
+  if (I.lo.ne) k_pack<VT><<<4096, 256, 0, c.stream>>>(I.lo.ne, I.lo.psrc.p, LU.p, pl.p);
+  if (I.up.ne) k_pack<VT><<<4096, 256, 0, c.stream>>>(I.up.ne, I.up.psrc.p, LU.p, pu.p);
+  k_pack_diag<VT><<<(I.n + 255) / 256, 256, 0, c.stream>>>(I.n, I.up.prow.p, I.rp.p, LU.p, pd.p);
+  HIPCHECK(hipGetLastError());
+  int h_err = 0;
+  HIPCHECK(hipMemcpyAsync(&h_err, err, sizeof(int), hipMemcpyDeviceToHost, c.stream));
+  HIPCHECK(hipStreamSynchronize(c.stream));
+  if (h_err) fail(RMX_ERR_PIVOT, "ILU(0): pivot below 1e-20 (row repair is not supported)");
+}
+
+void ilu_factor(Ctx& c, Ilu& I, int prec)
+{
+  if (!I.ready) fail(RMX_ERR_ARG, "ILU(0): no schedule");
+  if (prec == RMX_F32) {
+    if (!c.have32) fail(RMX_ERR_ARG, "ILU(0) FP32: no FP32 matrix (assemble or convert first)");
+    factor_t<float>(c, I, c.A32.p);
+    I.fac32 = true;
+  } else {
+    if (!c.have64) fail(RMX_ERR_ARG, "ILU(0) FP64: no FP64 matrix");
+    factor_t<double>(c, I, c.A64.p);
+    I.fac64 = true;
+  }
+}
+
+template <class VT>
+static void apply_t(Ctx& c, Ilu& I, const double* b, double* x, hipEvent_t* ev)
+{
+  const int gb = (I.n + 255) / 256;
+  const int nb = I.nblk;
+  if (ev) HIPCHECK(hipEventRecord(ev[0], c.stream));
+  k_gather<<<gb, 256, 0, c.stream>>>(I.n, I.lo.prow.p, b, I.lo.pin.p);
+  k_sweep<VT, true, SG, SKR><<<nb, 1024, 0, c.stream>>>(I.lo.blev.p, I.lo.bpos.p, I.lo.lev_ptr.p,
+      I.lo.prs.p, I.lo.pcol.p, PLbuf<VT>(I).p, nullptr, I.lo.pin.p, I.lo.xp.p);
+  k_gather<<<gb, 256, 0, c.stream>>>(I.n, I.fwd2bwd.p, I.lo.xp.p, I.up.pin.p);
+  if (ev) HIPCHECK(hipEventRecord(ev[1], c.stream));
+  k_sweep<VT, false, SG, SKR><<<nb, 1024, 0, c.stream>>>(I.up.blev.p, I.up.bpos.p, I.up.lev_ptr.p,
+      I.up.prs.p, I.up.pcol.p, PUbuf<VT>(I).p, PDbuf<VT>(I).p, I.up.pin.p, I.up.xp.p);
+  k_scatter<<<gb, 256, 0, c.stream>>>(I.n, I.up.prow.p, I.up.xp.p, x);
+  if (ev) HIPCHECK(hipEventRecord(ev[2], c.stream));
+  HIPCHECK(hipGetLastError());
+}
+
+void ilu_apply(Ctx& c, Ilu& I, int prec, const double* b, double* x, hipEvent_t* ev)
+{
+  if (prec == RMX_F32) {
+    if (!I.fac32) fail(RMX_ERR_ARG, "ILU(0) FP32 not factored");
+    apply_t<float>(c, I, b, x, ev);
+  } else {
+    if (!I.fac64) fail(RMX_ERR_ARG, "ILU(0) FP64 not factored");
+    apply_t<double>(c, I, b, x, ev);
+  }
+}
+
+// ---- the context's two instances ----
+
+void setup_ilu_schedule(Ctx& c)
+{
+  std::vector<int> hrp, hcols;
+  if (c.csr_only) {
+    hrp.resize(c.n + 1); hcols.resize(c.nnz);
+    c.rowptr.download(hrp.data(), c.n + 1, c.stream);
+    c.cols.download(hcols.data(), c.nnz, c.stream);
+    HIPCHECK(hipStreamSynchronize(c.stream));
+  }
+  const std::vector<int>& rp = c.csr_only ? hrp : c.hm.rowptr;
+  const std::vector<int>& cl = c.csr_only ? hcols : c.hm.cols;
+  ilu_setup(c.ilu, c.n, rp, cl, 1, nullptr, c.stream);
+  // strip ILU: one strip per compute unit, at least a few thousand rows each
+  hipDeviceProp_t prop;
+  HIPCHECK(hipGetDeviceProperties(&prop, c.device));
+  int nblk = c.strips > 0 ? c.strips : prop.multiProcessorCount;
+  nblk = std::max(1, std::min(nblk, c.n / 2048));
+  std::vector<int> group;
+  if (!c.csr_only) {
+    group.assign(c.n, -1);
+    for (size_t k = 0; k < c.hm.eqno.size(); k++)
+      if (c.hm.eqno[k] >= 0) group[c.hm.eqno[k]] = (int)(k / 3);
+  }
+  ilu_setup(c.silu, c.n, rp, cl, nblk, c.csr_only ? nullptr : &group, c.stream);
+}
+
+Ilu& ilu_of(Ctx& c, int kind) { return kind == RMX_PREC_SILU0 ? c.silu : c.ilu; }
+
+void dev_ilu0_factor(Ctx& c, int prec) { ilu_factor(c, c.ilu, prec); }
+
+void dev_ilu0_apply(Ctx& c, int prec, const double* b, double* x)
+{
+  ilu_apply(c, c.ilu, prec, b, x, nullptr);
+}
+
+// Average kernel durations (HIP events on the context stream) of SpMV and of the two
+// sweeps of an ILU instance, `reps` times each on the current system.
+void dev_profile(Ctx& c, int prec, int reps, rmx_kernel_times& out)
+{
+  hipEvent_t ev[3];
+  for (auto& e : ev) HIPCHECK(hipEventCreate(&e));
+  double* xv = c.w[0].p;
+  double* yv = c.w[1].p;
+  double t_spmv = 0, t_fwd = 0, t_bwd = 0, t_sf = 0, t_sb = 0;
+  for (int r = 0; r < reps; r++) {
+    HIPCHECK(hipEventRecord(ev[0], c.stream));
+    dev_spmv(c, prec, c.B.p, yv);
+    HIPCHECK(hipEventRecord(ev[1], c.stream));
+    HIPCHECK(hipEventSynchronize(ev[1]));
+    float ms = 0;
+    HIPCHECK(hipEventElapsedTime(&ms, ev[0], ev[1]));
+    t_spmv += ms * 1e-3;
+    for (int k = 0; k < 2; k++) {
+      Ilu& I = k == 0 ? c.ilu : c.silu;
+      if (!((prec == RMX_F32) ? I.fac32 : I.fac64)) continue;
+      ilu_apply(c, I, prec, c.B.p, xv, ev);
+      HIPCHECK(hipEventSynchronize(ev[2]));
+      float m1 = 0, m2 = 0;
+      HIPCHECK(hipEventElapsedTime(&m1, ev[0], ev[1]));
+      HIPCHECK(hipEventElapsedTime(&m2, ev[1], ev[2]));
+      if (k == 0) { t_fwd += m1 * 1e-3; t_bwd += m2 * 1e-3; }
+      else { t_sf += m1 * 1e-3; t_sb += m2 * 1e-3; }
+    }
+  }
+  for (auto& e : ev) HIPCHECK(hipEventDestroy(e));
+  out.spmv_s = t_spmv / reps;
+  out.ilu_fwd_s = t_fwd / reps;
+  out.ilu_bwd_s = t_bwd / reps;
+  out.ilu_apply_s = out.ilu_fwd_s + out.ilu_bwd_s;
+  out.silu_fwd_s = t_sf / reps;
+  out.silu_bwd_s = t_sb / reps;
+  out.n_spmv = reps;
+  out.n_ilu = reps;
+}
+
+}  // namespace rmx

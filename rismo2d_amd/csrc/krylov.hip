@@ -171,8 +171,8 @@ static int vblocks(long long n) { return (int)std::min<long long>(4096, (n + 255
 
 static void precond(Ctx& c, int prec, int kind, const double* in, double* out)
 {
-  if (kind == RMX_PREC_ILU0) {
-    dev_ilu0_apply(c, prec, in, out);
+  if (kind == RMX_PREC_ILU0 || kind == RMX_PREC_SILU0) {
+    ilu_apply(c, ilu_of(c, kind), prec, in, out, nullptr);
   } else if (kind == RMX_PREC_MPNN) {
     mpnn_apply(c, in, out);
   } else {
@@ -375,8 +375,11 @@ void dev_solve(Ctx& c, const rmx_krylov_opts& o, rmx_krylov_out& out)
   const auto t0 = std::chrono::steady_clock::now();
   const int prec = o.prec == RMX_F32 ? RMX_F32 : RMX_F64;
   const int pk = o.precond;
-  if (pk == RMX_PREC_ILU0 && !((prec == RMX_F32) ? c.fac32 : c.fac64))
-    fail(RMX_ERR_ARG, "solve: ILU(0) factors of the requested precision are missing");
+  if (pk == RMX_PREC_ILU0 || pk == RMX_PREC_SILU0) {
+    Ilu& I = ilu_of(c, pk);
+    if (!((prec == RMX_F32) ? I.fac32 : I.fac64))
+      fail(RMX_ERR_ARG, "solve: ILU(0) factors of the requested precision are missing");
+  }
   if (pk == RMX_PREC_MPNN && !mpnn_ready(c)) fail(RMX_ERR_ARG, "solve: no learned preconditioner loaded");
   if ((prec == RMX_F32 && !c.have32) || (prec == RMX_F64 && !c.have64))
     fail(RMX_ERR_ARG, "solve: matrix of the requested precision is missing");

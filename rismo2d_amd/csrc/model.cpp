@@ -375,38 +375,6 @@ void host_colouring(const HostModel& hm, std::vector<int>& col_elems, std::vecto
   }
 }
 
-// Level sets of the lower (forward) or upper (backward) triangular dependency graph.
-void host_levels(int n, const std::vector<int>& rowptr, const std::vector<int>& cols, bool lower,
-                 std::vector<int>& ptr, std::vector<int>& rows)
-{
-  std::vector<int> lev(n, 0);
-  int nlev = 0;
-  if (lower) {
-    for (int i = 0; i < n; i++) {
-      int l = 0;
-      for (int q = rowptr[i] + 1; q < rowptr[i + 1]; q++)
-        if (cols[q] < i) l = std::max(l, lev[cols[q]] + 1);
-      lev[i] = l;
-      nlev = std::max(nlev, l + 1);
-    }
-  } else {
-    for (int i = n - 1; i >= 0; i--) {
-      int l = 0;
-      for (int q = rowptr[i] + 1; q < rowptr[i + 1]; q++)
-        if (cols[q] > i) l = std::max(l, lev[cols[q]] + 1);
-      lev[i] = l;
-      nlev = std::max(nlev, l + 1);
-    }
-  }
-  ptr.assign(nlev + 1, 0);
-  for (int i = 0; i < n; i++) ptr[lev[i] + 1]++;
-  for (int l = 0; l < nlev; l++) ptr[l + 1] += ptr[l];
-  rows.assign(n, 0);
-  std::vector<int> fill(ptr.begin(), ptr.end() - 1);
-  if (lower) { for (int i = 0; i < n; i++) rows[fill[lev[i]]++] = i; }
-  else { for (int i = n - 1; i >= 0; i--) rows[fill[lev[i]]++] = i; }
-}
-
 // Element-to-CSR slot map: for every model element, for every pair of local dofs
 // (row ldof, column ldof) with ldof = i + j*nnd (U block, V block, corner H block), the
 // position of the column in the CSR row, or -1 if either dof is not an equation.

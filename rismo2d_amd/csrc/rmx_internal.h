@@ -113,15 +113,29 @@ struct Levels {
 
 constexpr int RING = 8192;     // LDS ring of solved x values in the packed sweeps
 
-struct SweepPack {
-  int nlev = 0;
-  bool ring_ok = false;
-  int max_nr = 0;
+// Packed layout of one triangular sweep over all strips of an ILU system.
+struct SweepSet {
+  int nblk = 0, nlev_max = 0, nlev_sum = 0, max_nr = 0;
   long long ne = 0;
-  std::vector<int> h_lev;       // level row offsets (host)
-  DBuf<int> lev_ptr, prs, prow, pcol, psrc;
-  DBuf<double> pval64, pdiag64, pin, xp;
-  DBuf<float> pval32, pdiag32;
+  DBuf<int> blev, bpos;         // per strip: first level, first position
+  DBuf<int> lev_ptr;            // per strip: local level offsets + sentinel
+  DBuf<int> prs, prow, pcol, psrc;
+  DBuf<double> pin, xp;
+};
+
+// One ILU(0) system: pattern, factorisation schedule, sweeps and factors.
+struct Ilu {
+  bool ready = false;
+  int n = 0, nblk = 1;
+  long long nnz = 0;
+  std::vector<int> bstart, h_rp, h_cols, flev;
+  DBuf<int> rp, cols, srt_cols, flev_rows, fwd2bwd;
+  DBuf<unsigned char> srt_pos;
+  DBuf<long long> src;          // entry of the system matrix for each restricted entry
+  SweepSet lo, up;
+  DBuf<double> LU64, plo64, pup64, pdg64;
+  DBuf<float> LU32, plo32, pup32, pdg32;
+  bool fac64 = false, fac32 = false;
 };
 
 struct Ctx {
@@ -154,20 +168,12 @@ struct Ctx {
   int n = 0;
   long long nnz = 0;
   DBuf<int> rowptr, cols;
-  DBuf<double> A64, L64, B, X;
-  DBuf<float> A32, L32;
-  bool have64 = false, have32 = false, fac64 = false, fac32 = false;
-  // ILU schedule
-  Levels lev_lo, lev_up;                      // forward / backward sweep levels
-  DBuf<int> lrp, lcol, urp, ucol;             // strictly lower / upper patterns
-  long long nlo = 0, nup = 0;
-  DBuf<double> lval64, uval64, diag64;
-  DBuf<float> lval32, uval32, diag32;
-  SweepPack pk_lo, pk_up;                     // packed level-order sweeps
-  DBuf<int> fwd2bwd;                          // forward position of each backward position
-  DBuf<int> srt_cols;                         // per-row sorted columns (factor lookups)
-  DBuf<unsigned char> srt_pos;                // original position of each sorted column
-  int max_width = 0;
+  DBuf<double> A64, B, X;
+  DBuf<float> A32;
+  bool have64 = false, have32 = false;
+  Ilu ilu;                                    // reference-order ILU(0)
+  Ilu silu;                                   // strip ILU(0)
+  int strips = 0;                             // 0: one strip per compute unit
   // Krylov work vectors
   DBuf<double> w[10];
   DBuf<double> red;                           // reduction scratch
@@ -186,8 +192,6 @@ void host_numbering(HostModel& hm);
 void host_pattern(HostModel& hm);
 void host_colouring(const HostModel& hm, std::vector<int>& col_elems, std::vector<int>& col_ptr,
                     std::vector<int>& col_type);
-void host_levels(int n, const std::vector<int>& rowptr, const std::vector<int>& cols, bool lower,
-                 std::vector<int>& ptr, std::vector<int>& rows);
 void host_slots(const HostModel& hm, std::vector<long long>& off, std::vector<int>& slots,
                 std::vector<int>& type);
 void parallel_for(long long n, const std::function<void(long long, long long)>& f);
@@ -208,6 +212,9 @@ void dev_resid(Ctx& c, const double* x, const double* b, double* r);
 void dev_resid_p(Ctx& c, int prec, const double* x, const double* b, double* r);
 void dev_dots(Ctx& c, int K, const double* const* x, const double* const* y, double* dst);
 void setup_ilu_schedule(Ctx& c);
+Ilu& ilu_of(Ctx& c, int kind);
+void ilu_factor(Ctx& c, Ilu& I, int prec);
+void ilu_apply(Ctx& c, Ilu& I, int prec, const double* b, double* x, hipEvent_t* ev);
 void upload_shapes(hipStream_t s);
 
 }  // namespace rmx

@@ -18,7 +18,7 @@ HEADER = os.path.join(os.path.dirname(HERE), "include", "rmx.h")
 
 F64, F32 = 64, 32
 BICGSTAB, FGMRES = 1, 2
-PREC_NONE, PREC_ILU0, PREC_MPNN = 0, 1, 2
+PREC_NONE, PREC_ILU0, PREC_MPNN, PREC_SILU0 = 0, 1, 2, 3
 ERR_NODEVICE = -4
 
 _d = C.POINTER(C.c_double)
@@ -67,7 +67,8 @@ class StepStats(C.Structure):
 
 class KernelTimes(C.Structure):
     _fields_ = [("spmv_s", C.c_double), ("ilu_apply_s", C.c_double), ("ilu_fwd_s", C.c_double),
-                ("ilu_bwd_s", C.c_double), ("n_spmv", C.c_longlong), ("n_ilu", C.c_longlong)]
+                ("ilu_bwd_s", C.c_double), ("silu_fwd_s", C.c_double), ("silu_bwd_s", C.c_double),
+                ("n_spmv", C.c_longlong), ("n_ilu", C.c_longlong)]
 
 
 _LIB = None
@@ -96,6 +97,8 @@ def lib():
             "rmx_assemble": [_v, C.c_int], "rmx_scale_l2": [_v, _d],
             "rmx_convert_matrix": [_v, C.c_int], "rmx_ilu0_factor": [_v, C.c_int],
             "rmx_ilu0_apply": [_v, C.c_int, _d, _d], "rmx_spmv": [_v, C.c_int, _d, _d],
+            "rmx_silu0_factor": [_v, C.c_int], "rmx_silu0_apply": [_v, C.c_int, _d, _d],
+            "rmx_set_strips": [_v, C.c_int],
             "rmx_solve": [_v, C.POINTER(KrylovOpts), C.POINTER(KrylovOut)],
             "rmx_get_matrix": [_v, C.c_int, _v, _d], "rmx_get_factors": [_v, C.c_int, _v],
             "rmx_get_solution": [_v, _d], "rmx_set_rhs": [_v, _d],
@@ -244,6 +247,18 @@ class Context:
         x = np.zeros(self.n)
         _check(lib().rmx_ilu0_apply(self.h, prec, _p(b, _d), _p(x, _d)))
         return x
+
+    def silu0_factor(self, prec=F64):
+        _check(lib().rmx_silu0_factor(self.h, prec))
+
+    def silu0_apply(self, b, prec=F64):
+        b = _c(b, np.float64)
+        x = np.zeros(self.n)
+        _check(lib().rmx_silu0_apply(self.h, prec, _p(b, _d), _p(x, _d)))
+        return x
+
+    def set_strips(self, strips):
+        _check(lib().rmx_set_strips(self.h, int(strips)))
 
     def spmv(self, x, prec=F64):
         x = _c(x, np.float64)

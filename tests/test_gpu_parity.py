@@ -189,6 +189,84 @@ def test_mixed_precision_refinement_reaches_fp64_residual(ctx, method):
     assert np.abs(X - X64).max() <= 1e-6 * np.abs(X64).max()
 
 
+def _restrict(rp, cols, vals, nblk, group):
+    """The strip ILU(0)'s pattern: rows cut into nblk ranges at n*b/nblk, each cut moved
+    forward to the first equation of the next node (group = node of each equation),
+    entries coupling two ranges dropped, row order and entry order kept."""
+    n = len(rp) - 1
+    bstart = (np.arange(nblk + 1, dtype=np.int64) * n) // nblk
+    for b in range(1, nblk):
+        bstart[b] = max(bstart[b], bstart[b - 1])
+        while 0 < bstart[b] < n and group[bstart[b]] == group[bstart[b] - 1]:
+            bstart[b] += 1
+    blk = np.searchsorted(bstart, np.arange(n), side="right") - 1
+    rows = np.repeat(np.arange(n), np.diff(rp))
+    keep = blk[cols] == blk[rows]
+    rp2 = np.concatenate([[0], np.cumsum(np.bincount(rows[keep], minlength=n))]).astype(np.int32)
+    return rp2, np.ascontiguousarray(cols[keep]), np.ascontiguousarray(vals[keep])
+
+
+@pytest.mark.parametrize("strips", [1, 4, 17])
+def test_strip_ilu_matches_oracle_on_restricted_pattern(ctx, strips):
+    """Strip ILU(0) = the oracle's ILU(0) of the matrix with the couplings between strips
+    removed: FP64 and FP32 factor + apply agree to 1e-10 of max|result|.  The sweeps sum in
+    another order than the oracle, and the strip factors have pivots down to 1e-6 of the
+    row scale (the full factors' are >= 1e-2), which amplifies that rounding by up to 1e4
+    over the 1e-12 of the full ILU(0) apply."""
+    print(f"[gpu] strip ILU, {strips} strips", flush=True)
+    g = load_golden("channel200x20")
+    ctx.upload_mesh(golden_deck(g))
+    n = ctx.prepare()
+    ctx.set_strips(strips)
+    ctx.assemble(rmx.F64)
+    ctx.scale_l2()
+    rp, cols = ctx.pattern()
+    A, B = ctx.matrix(rmx.F64)
+    eq = ctx.numbering().reshape(-1)
+    group = np.zeros(n, np.int64)
+    group[eq[eq >= 0]] = np.nonzero(eq >= 0)[0] // 3
+    nb = max(1, min(strips, n // 2048))
+    rp2, c2, a2 = _restrict(rp, cols, A, nb, group)
+    if nb > 1:
+        assert len(a2) < len(A)
+    ctx.silu0_factor(rmx.F64)
+    L, r = po.ilu0_factor(rp2, c2, a2)
+    assert r == 0
+    ref = po.ilu0_solve(rp2, c2, L, B)
+    y = ctx.silu0_apply(B, rmx.F64)
+    assert np.abs(y - ref).max() <= 1e-10 * np.abs(ref).max()
+    ctx.convert(rmx.F32)
+    ctx.silu0_factor(rmx.F32)
+    L32, r = po.ilu0_factor(rp2, c2, a2.astype(np.float32))
+    ref = po.ilu0_solve(rp2, c2, L32, B)
+    y = ctx.silu0_apply(B, rmx.F32)
+    assert np.abs(y - ref).max() <= 1e-10 * np.abs(ref).max()
+    # a strip ILU with one strip is the reference-order ILU(0)
+    if nb == 1:
+        ctx.ilu0_factor(rmx.F32)
+        np.testing.assert_array_equal(ctx.ilu0_apply(B, rmx.F32), y)
+
+
+def test_strip_ilu_refinement_newton_reaches_reference(ctx):
+    """Newton with FP32 strip-ILU(0) BiCGStab inside FP64 refinement (true FP64 residual
+    1e-10 per step) reaches the reference program's converged state within 1e-8."""
+    print("[gpu] Newton, strip ILU FP32 + FP64 refinement", flush=True)
+    g = load_golden("channel200x20")
+    ctx.upload_mesh(golden_deck(g))
+    ctx.prepare()
+    ctx.set_strips(0)
+    for k in range(int(g["nsteps"]) + 1):
+        st = ctx.newton_step(method=rmx.BICGSTAB, prec=rmx.F32, precond=rmx.PREC_SILU0,
+                             eps=1e-4, refine=True, refine_tol=1e-10, refine_max=20)
+        assert st.solve.converged and st.solve.true_resid <= 1e-10, (k, st.solve.true_resid)
+    U, V, S, z = ctx.state()
+    Hr = g["final_S"] - g["final_z"]
+    us = np.abs(g["final_U"]).max()
+    assert _relerr(U, g["final_U"], us) <= 1e-8
+    assert _relerr(V, g["final_V"], us) <= 1e-8
+    assert _relerr(S - z, Hr, np.abs(Hr).max()) <= 1e-8
+
+
 def _random_csr(n, seed, density):
     rng = np.random.default_rng(seed)
     M = rng.standard_normal((n, n)) * (rng.random((n, n)) < density)
