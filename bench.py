@@ -1,0 +1,213 @@
+"""Benchmark: time per Newton step to FP64 relative residual 1e-8 on the 2M-triangle
+meandering channel (one MI355X).
+
+One step = one Newton step of the UVS2D system from the deck's initial state (the state is
+reset before every step, so every step solves the same system): nodal friction and inlet
+set-up, FP64 assembly, L2 scaling, FP32 copy of the matrix, ILU(0) factorisation, the
+Krylov solve in FP32 inside FP64 iterative refinement to ||b - J x|| / ||b|| <= 1e-8 (FP64
+J), Newton update.  The inputs (mesh, pattern, schedules) are resident on the device
+before the timed region.
+
+Printed (one JSON line): the step time of the FP32/FP64 path, the plain FP64
+ILU0-BiCGStab path's measured cost per iteration on the same system (its full solve at
+this size takes far longer than a benchmark run and is not attempted), the roofline of the
+dominant kernel (HIP events on the library's stream) and a CPU baseline: the oracle's
+FP64 ILU0-BiCGStab Newton step on a bounded sample of the same workload.
+
+    python bench.py [--steps K] [--warmup W] [--gpus 1]
+"""
+from __future__ import annotations
+
+import argparse
+import json
+import os
+import sys
+import time
+
+import numpy as np
+
+HERE = os.path.dirname(os.path.abspath(__file__))
+sys.path.insert(0, HERE)
+
+METRIC = "time to FP64 residual 1e-8 per Newton step: learned+FP32 vs plain FP64"
+HBM_PEAK = 8.0e12          # MI355X HBM3E, bytes/s
+
+
+def parse():
+    ap = argparse.ArgumentParser()
+    ap.add_argument("--gpus", type=int, default=1)
+    ap.add_argument("--steps", type=int, default=1)
+    ap.add_argument("--warmup", type=int, default=1)
+    ap.add_argument("--n-along", type=int, default=3200)
+    ap.add_argument("--n-across", type=int, default=320)
+    ap.add_argument("--strips", type=int, default=0)
+    ap.add_argument("--segments", type=int, default=16)
+    ap.add_argument("--max-iter", type=int, default=1000, help="Krylov iterations per refinement sweep")
+    ap.add_argument("--refine-max", type=int, default=5)
+    ap.add_argument("--plain-iters", type=int, default=10)
+    ap.add_argument("--no-cpu", action="store_true")
+    return ap.parse_args()
+
+
+def log(msg):
+    print(f"[bench] {msg}", file=sys.stderr, flush=True)
+
+
+def sweep_bytes(entries, rows, vbytes):
+    """Algorithmic bytes of one triangular sweep: every kept factor entry (value + column)
+    once, per row its entry offset, diagonal (upper sweep), right-hand side and result."""
+    return entries * (vbytes + 4) + rows * (4 + 8 + 8)
+
+
+def spmv_bytes(nnz, n, vbytes):
+    """Algorithmic bytes of one SpMV: values + columns once, row pointers, x once, y."""
+    return nnz * (vbytes + 4) + 4 * (n + 1) + 8 * n + 8 * n
+
+
+def cpu_baseline():
+    """The oracle (C restatement of the reference's path, one core) on a bounded sample of
+    the same workload: one Newton step, FP64 assembly + ILU(0) + BiCGStab to 1e-8, on a
+    100-section reach of the meander generator."""
+    from rismo2d_amd import deck as dk
+    from oracle import pyoracle as po
+    na = 100
+    d = dk.meander(n_along=na, n_across=320, reach=10000.0 * na / 3200)
+    m = po.OracleModel(d)
+    m.set_params()
+    t0 = time.perf_counter()
+    n = m.prepare()
+    rp, cols = m.pattern()
+    A, B = m.assemble(64)
+    po.scale(rp, A, B)
+    L, _ = po.ilu0_factor(rp, cols, A)
+    X, conv, its, acc = po.bicgstab(rp, cols, A, L, B, max_iter=2000, eps=1e-8)
+    t = time.perf_counter() - t0
+    return dict(value=t, unit="s per Newton step", cores=1, kind="port",
+                sample=(f"meander {na}x320 reach of the same generator ({d.n_elems} tri, {n} eq), "
+                        f"one Newton step with the oracle: FP64 assembly, ILU(0), BiCGStab "
+                        f"{its} it to {acc:.1e} (converged {bool(conv)}), 1 core"),
+                n_eq=n, iters=its)
+
+
+def main():
+    a = parse()
+    if a.gpus != 1:
+        print(f"bench.py runs on one GPU; --gpus {a.gpus} is not supported (the multi-GPU "
+              "path is not built)", file=sys.stderr)
+        sys.exit(2)
+    from rismo2d_amd import deck as dk
+    from rismo2d_amd import rmx as R
+
+    t = time.perf_counter()
+    d = dk.meander(n_along=a.n_along, n_across=a.n_across, reach=10000.0 * a.n_along / 3200)
+    log(f"deck {a.n_along}x{a.n_across}: {d.n_elems} triangles, {d.n_nodes} nodes "
+        f"({time.perf_counter() - t:.1f}s)")
+    c = R.Context(0)
+    c.upload_mesh(d)
+    c.set_params()
+    t = time.perf_counter()
+    n = c.prepare()
+    c.set_strips(a.strips, a.segments)
+    log(f"prepare: {n} equations, {c.nnz} non-zeros ({time.perf_counter() - t:.1f}s)")
+
+    opts = dict(method=R.FGMRES, prec=R.F32, precond=R.PREC_SILU0, max_iter=a.max_iter,
+                mkyrl=30, eps=1e-3, refine=True, refine_tol=1e-8, refine_max=a.refine_max)
+
+    def step():
+        c.state_upload(d.U0, d.V0, d.S0)
+        return c.newton_step(**opts)
+
+    for w in range(a.warmup):
+        st = step()
+        log(f"warmup {w}: {st.solve.iters} it, residual {st.solve.true_resid:.2e}, "
+            f"{st.t_prepare + st.t_assemble + st.t_factor + st.t_solve + st.t_update:.2f}s")
+    times, stats = [], []
+    for k in range(a.steps):
+        t0 = time.perf_counter()
+        st = step()
+        times.append(time.perf_counter() - t0)
+        stats.append(st)
+        log(f"step {k}: {times[-1]:.2f}s (assemble {st.t_assemble:.3f}, factor {st.t_factor:.3f}, "
+            f"solve {st.t_solve:.2f}: {st.solve.iters} it, {st.solve.sweeps} sweeps, residual "
+            f"{st.solve.true_resid:.2e})")
+    T = sum(times) / len(times)
+    last = stats[-1]
+
+    # kernel timing on the last step's system (HIP events on the library stream)
+    kt = c.profile_kernels(R.F32, 5)
+    lo_e, up_e = c.sweep_entries()
+    fwd_bytes = sweep_bytes(lo_e, n, 4)
+    bwd_bytes = sweep_bytes(up_e, n, 4) + 4 * n
+    sp_bytes = spmv_bytes(c.nnz, n, 4)
+    its = max(last.solve.iters, 1)
+    share = {"silu_sweep_fwd": kt.silu_fwd_s * its, "silu_sweep_bwd": kt.silu_bwd_s * its,
+             "spmv_f32": kt.spmv_s * (its + last.solve.sweeps)}
+    dom = max(share, key=share.get)
+    dur, byt = {"silu_sweep_fwd": (kt.silu_fwd_s, fwd_bytes),
+                "silu_sweep_bwd": (kt.silu_bwd_s, bwd_bytes),
+                "spmv_f32": (kt.spmv_s, sp_bytes)}[dom]
+    achieved = byt / dur
+    log(f"kernels: spmv {kt.spmv_s * 1e3:.3f} ms ({sp_bytes / kt.spmv_s / 1e9:.0f} GB/s), strip "
+        f"sweeps {kt.silu_fwd_s * 1e3:.3f} / {kt.silu_bwd_s * 1e3:.3f} ms")
+
+    # plain FP64 path on the same system: reference-order ILU(0), BiCGStab, bounded
+    c.assemble(R.F64)
+    c.scale_l2()
+    t0 = time.perf_counter()
+    c.ilu0_factor(R.F64)
+    t_fac64 = time.perf_counter() - t0
+    out = c.solve(R.BICGSTAB, R.F64, R.PREC_ILU0, max_iter=a.plain_iters, eps=1e-8)
+    plain_it = out.seconds / max(out.iters, 1)
+    kt64 = c.profile_kernels(R.F64, 2)
+    log(f"plain FP64: factor {t_fac64:.2f}s, {out.iters} BiCGStab it in {out.seconds:.2f}s "
+        f"({plain_it:.3f} s/it), residual {out.accuracy:.2e}; sweeps {kt64.ilu_fwd_s * 1e3:.1f} / "
+        f"{kt64.ilu_bwd_s * 1e3:.1f} ms")
+    c.close()
+
+    cpu = None
+    if not a.no_cpu:
+        try:
+            cpu = cpu_baseline()
+            log(f"cpu baseline: {cpu['value']:.1f}s ({cpu['sample']})")
+        except Exception as e:       # the oracle library is test infrastructure: report its absence
+            log(f"cpu baseline unavailable: {e}")
+
+    line = {
+        "metric": METRIC,
+        "value": T,
+        "unit": "s per Newton step",
+        "n_gpus": 1,
+        "steps": a.steps,
+        "warmup": a.warmup,
+        "ms_per_step": T * 1e3,
+        "higher_is_better": False,
+        "scaling": "weak",
+        "vs_baseline": None,
+        "dtype": "f32+f64",
+        "data": "synthetic",
+        "config": {"workload": f"meander channel {a.n_along}x{a.n_across} cells, "
+                               f"{d.n_elems} Taylor-Hood triangles, {n} equations, {c.nnz} nnz; "
+                               "Newton step 1 from the initial state",
+                   "solver": "FGMRES(30), FP32 matrix + strip ILU(0) "
+                             f"({a.strips or 'per-CU'} strips x {a.segments} segments), "
+                             "FP64 iterative refinement",
+                   "parallelism": "1 GPU"},
+        "converged": bool(last.solve.converged),
+        "true_resid": last.solve.true_resid,
+        "krylov_iters": last.solve.iters,
+        "refine_sweeps": last.solve.sweeps,
+        "t_factor_s": last.t_factor,
+        "t_solve_s": last.t_solve,
+        "plain_fp64": {"factor_s": t_fac64, "s_per_bicgstab_iter": plain_it,
+                       "iters_run": out.iters, "residual_reached": out.accuracy,
+                       "sweep_fwd_ms": kt64.ilu_fwd_s * 1e3, "sweep_bwd_ms": kt64.ilu_bwd_s * 1e3},
+        "roofline": {"kernel": dom, "bound": "hbm", "achieved": achieved / 1e9,
+                     "peak": HBM_PEAK / 1e9, "unit": "GB/s", "frac": achieved / HBM_PEAK,
+                     "traffic": None, "algorithmic_bytes": byt, "avg_launch_ms": dur * 1e3},
+        "cpu_baseline": cpu and {k: cpu[k] for k in ("value", "unit", "cores", "kind", "sample")},
+    }
+    print(json.dumps(line), flush=True)
+
+
+if __name__ == "__main__":
+    main()

@@ -118,13 +118,25 @@ rmx_status rmx_ilu0_factor(rmx_ctx* ctx, int prec);
 rmx_status rmx_ilu0_apply(rmx_ctx* ctx, int prec, const double* b, double* x);
 
 /* Strip ILU(0) (RMX_PREC_SILU0): factor and apply.  No reference counterpart: the
- * reference's PRECO_ILU0 with the couplings between strips of consecutive equations
- * dropped, the form its MPI build factors per subdomain (Preco_ilu0.cpp:201-485) without
- * the interface pipeline.  rmx_set_strips chooses the number of strips (0 = one per
- * compute unit) and rebuilds the schedule. */
+ * reference's ILU(0) factors (PRECO_ILU0::Factor) applied with block-Jacobi triangular
+ * sweeps (PRECO_ILU0::Solve, Preco_ilu0.cpp:497, without the factor entries that couple two
+ * blocks).  A block is one of `segments` parts of a strip of consecutive equations, cut
+ * along the principal axis of the strip's node coordinates; every strip is swept by one
+ * workgroup.  rmx_set_strips chooses the number of strips (0 = one per compute unit) and
+ * segments (1 for a system loaded with rmx_csr_upload, which has no coordinates) and
+ * rebuilds the schedule; rmx_get_blocks returns the block of every equation. */
 rmx_status rmx_silu0_factor(rmx_ctx* ctx, int prec);
 rmx_status rmx_silu0_apply(rmx_ctx* ctx, int prec, const double* b, double* x);
-rmx_status rmx_set_strips(rmx_ctx* ctx, int strips);
+rmx_status rmx_set_strips(rmx_ctx* ctx, int strips, int segments);
+rmx_status rmx_get_blocks(rmx_ctx* ctx, int* block /* [neq] */);
+
+/* Shape of the sweeps of an ILU(0) instance (kind RMX_PREC_ILU0 or RMX_PREC_SILU0): factor
+ * entries each sweep reads and its number of levels (the longest strip's). */
+typedef struct {
+  long long lower_entries, upper_entries;
+  int lower_levels, upper_levels, strips, segments;
+} rmx_sweep_info;
+rmx_status rmx_get_sweep_info(rmx_ctx* ctx, int kind, rmx_sweep_info* out);
 
 /* r = A x: replaces CRSMAT::MulVec (CRSMat.cpp:466).  Host pointers. */
 rmx_status rmx_spmv(rmx_ctx* ctx, int prec, const double* x, double* r);

@@ -340,13 +340,41 @@ rmx_status rmx_silu0_apply(rmx_ctx* p, int prec, const double* b, double* x)
   API_END
 }
 
-rmx_status rmx_set_strips(rmx_ctx* p, int strips)
+rmx_status rmx_set_strips(rmx_ctx* p, int strips, int segments)
 {
   API_BEGIN
   Ctx& c = *need(p);
-  if (strips < 0) fail(RMX_ERR_ARG, "strips must be >= 0");
+  if (strips < 0 || segments < 1 || segments > 1024) fail(RMX_ERR_ARG, "strips >= 0, 1 <= segments <= 1024");
   c.strips = strips;
+  c.segments = segments;
   if (c.prepared) setup_ilu_schedule(c);
+  API_END
+}
+
+rmx_status rmx_get_sweep_info(rmx_ctx* p, int kind, rmx_sweep_info* out)
+{
+  API_BEGIN
+  Ctx& c = *need(p);
+  if (!out) fail(RMX_ERR_ARG, "null output");
+  if (kind != RMX_PREC_ILU0 && kind != RMX_PREC_SILU0) fail(RMX_ERR_ARG, "kind must be an ILU(0)");
+  const Ilu& I = ilu_of(c, kind);
+  if (!c.prepared || !I.ready) fail(RMX_ERR_ARG, "no system");
+  out->lower_entries = I.lo.ne; out->upper_entries = I.up.ne;
+  out->lower_levels = I.lo.nlev_max; out->upper_levels = I.up.nlev_max;
+  out->strips = I.nblk; out->segments = I.nseg;
+  API_END
+}
+
+rmx_status rmx_get_blocks(rmx_ctx* p, int* block)
+{
+  API_BEGIN
+  Ctx& c = *need(p);
+  if (!c.prepared || !c.silu.ready) fail(RMX_ERR_ARG, "no system");
+  if (!block) fail(RMX_ERR_ARG, "null output");
+  const Ilu& I = c.silu;
+  for (int b = 0; b < I.nblk; b++)
+    for (int i = I.bstart[b]; i < I.bstart[b + 1]; i++)
+      block[i] = b * I.nseg + (I.seg.empty() ? 0 : I.seg[i]);
   API_END
 }
 

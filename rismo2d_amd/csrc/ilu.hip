@@ -1,14 +1,17 @@
 // ILU(0) preconditioner on gfx950: factorisation and triangular sweeps
 // (PRECO_ILU0::Factor / Solve, Preco_ilu0.cpp:64-186, 497-530, 958-980).
 //
-// An Ilu object holds one ILU(0) system: a CSR pattern (the system's own, or a
-// restriction of it), the level sets of its triangular dependency graphs, the packed
-// sweep layouts, and FP64 / FP32 factors.  Two instances exist per context:
-//  * the reference ILU(0) on the full pattern in the reference's equation order (the
-//    FP64 plain path; its FP32 factors are bit-identical to the reference's);
-//  * a strip ILU(0): rows are cut into contiguous ranges (the frontal order runs along
-//    the channel, so a range is a strip across the channel) and couplings between strips
-//    are dropped, so the strips factor and sweep independently, one workgroup each.
+// An Ilu object holds one ILU(0) system: the CSR pattern, the level sets of its
+// factorisation, the packed sweep layouts, and FP64 / FP32 factors.  Both instances of a
+// context factor the whole matrix in the reference's equation order (its FP32 factors are
+// bit-identical to the reference's); they differ in the sweeps:
+//  * the reference sweeps: one workgroup walks the level sets of the whole lower / upper
+//    factor (the plain FP64 path);
+//  * strip sweeps: rows are cut into contiguous ranges (the frontal order runs along the
+//    channel, so a range is a strip of cross-sections) and the factor entries that couple
+//    two strips are left out of the sweeps, so each strip is swept by its own workgroup.
+//    The pivots stay those of the full factorisation (factoring the strips separately
+//    loses the upstream couplings at each strip start and leaves near-zero pivots).
 //    It preconditions the FP32 solve inside the FP64 refinement, where the FP64 residual,
 //    not the preconditioner, decides the answer.
 //
@@ -102,16 +105,6 @@ __global__ void __launch_bounds__(64) k_ilu_level(const int* __restrict__ rows, 
   }
   for (int t = lane; t < wi; t += 64) LU[p0 + t] = s_val[t];
   if (lane == 0 && fabs((double)s_val[0]) < 1.0e-20) atomicOr(err, 1);
-}
-
-// values of a restricted pattern from the system matrix
-template <class VT>
-__global__ void k_take(long long ne, const long long* __restrict__ src, const VT* __restrict__ A,
-                       VT* __restrict__ out)
-{
-  for (long long t = (long long)blockIdx.x * blockDim.x + threadIdx.x; t < ne;
-       t += (long long)gridDim.x * blockDim.x)
-    out[t] = A[src[t]];
 }
 
 template <class VT>
@@ -294,16 +287,19 @@ __global__ void k_scatter(int n, const int* __restrict__ idx, const double* __re
 // ---------------------------------------------------------------------------------------
 // host set-up
 
-// Level sets of the lower or upper dependency graph of rows [a, e).
+// Level sets of the lower or upper dependency graph of rows [a, e), entries with a column
+// outside [a, e) or (with `seg`) in another segment ignored.
 static void levels(int a, int e, const std::vector<int>& rp, const std::vector<int>& cl, bool lower,
-                   std::vector<int>& lev, std::vector<int>& ptr, std::vector<int>& rows)
+                   const std::vector<int>& seg, std::vector<int>& lev, std::vector<int>& ptr,
+                   std::vector<int>& rows)
 {
   int nlev = 0;
+  const bool sg = !seg.empty();
   if (lower) {
     for (int i = a; i < e; i++) {
       int l = 0;
       for (int q = rp[i] + 1; q < rp[i + 1]; q++)
-        if (cl[q] < i) l = std::max(l, lev[cl[q]] + 1);
+        if (cl[q] < i && cl[q] >= a && (!sg || seg[cl[q]] == seg[i])) l = std::max(l, lev[cl[q]] + 1);
       lev[i] = l;
       nlev = std::max(nlev, l + 1);
     }
@@ -311,7 +307,7 @@ static void levels(int a, int e, const std::vector<int>& rp, const std::vector<i
     for (int i = e - 1; i >= a; i--) {
       int l = 0;
       for (int q = rp[i] + 1; q < rp[i + 1]; q++)
-        if (cl[q] > i) l = std::max(l, lev[cl[q]] + 1);
+        if (cl[q] > i && cl[q] < e && (!sg || seg[cl[q]] == seg[i])) l = std::max(l, lev[cl[q]] + 1);
       lev[i] = l;
       nlev = std::max(nlev, l + 1);
     }
@@ -340,7 +336,7 @@ static void build_sweep(SweepSet& S, const Ilu& I, bool lower, std::vector<int>&
   for (int b = 0; b < nb; b++) {
     const int a = I.bstart[b], e = I.bstart[b + 1];
     std::vector<int> ptr, rows;
-    levels(a, e, rp, cl, lower, lev, ptr, rows);
+    levels(a, e, rp, cl, lower, I.seg, lev, ptr, rows);
     const int nl = (int)ptr.size() - 1;
     blev[b + 1] = blev[b] + nl;
     bpos[b + 1] = bpos[b] + (e - a);
@@ -354,11 +350,20 @@ static void build_sweep(SweepSet& S, const Ilu& I, bool lower, std::vector<int>&
   std::vector<int> pos(n);
   for (int b = 0; b < nb; b++)
     for (int r = bpos[b]; r < bpos[b + 1]; r++) pos[prow[r]] = r - bpos[b];
+  // entries of row i kept by the sweep: the strict lower / upper part inside i's strip
+  std::vector<int> blk(n);
+  for (int b = 0; b < nb; b++)
+    for (int i = I.bstart[b]; i < I.bstart[b + 1]; i++) blk[i] = b;
+  const bool sg = !I.seg.empty();
+  auto keep = [&](int i, int c) {
+    return (lower ? c < i : c > i) && c >= I.bstart[blk[i]] && c < I.bstart[blk[i] + 1] &&
+           (!sg || I.seg[c] == I.seg[i]);
+  };
   std::vector<int> prs(n + 1, 0);
   for (int r = 0; r < n; r++) {
     const int i = prow[r];
     int k = 0;
-    for (int q = rp[i] + 1; q < rp[i + 1]; q++) k += lower ? (cl[q] < i) : (cl[q] > i);
+    for (int q = rp[i] + 1; q < rp[i + 1]; q++) k += keep(i, cl[q]);
     prs[r + 1] = prs[r] + k;
   }
   const long long ne = prs[n];
@@ -368,7 +373,7 @@ static void build_sweep(SweepSet& S, const Ilu& I, bool lower, std::vector<int>&
       const int i = prow[r];
       int j = prs[r];
       for (int q = rp[i] + 1; q < rp[i + 1]; q++)
-        if (lower ? (cl[q] < i) : (cl[q] > i)) { pcol[j] = pos[cl[q]]; psrc[j] = q; j++; }
+        if (keep(i, cl[q])) { pcol[j] = pos[cl[q]]; psrc[j] = q; j++; }
     }
   });
   S.nblk = nb;
@@ -377,9 +382,9 @@ static void build_sweep(SweepSet& S, const Ilu& I, bool lower, std::vector<int>&
   S.ne = ne;
   S.nlev_sum = blev[nb];
   if (getenv("RMX_DEBUG"))
-    fprintf(stderr, "[rmx] %s sweep, %d strip(s): %d levels (longest strip), %.1f rows/level, "
-            "max %d, %lld entries\n", lower ? "lower" : "upper", nb, nlev_tot,
-            (double)n / std::max(1, blev[nb]), maxnr, ne);
+    fprintf(stderr, "[rmx] %s sweep, %d strip(s) x %d segment(s): %d levels (longest strip), "
+            "%.1f rows/level, max %d, %lld entries\n", lower ? "lower" : "upper", nb, I.nseg,
+            nlev_tot, (double)n / std::max(1, blev[nb]), maxnr, ne);
   S.blev.upload(blev, s); S.bpos.upload(bpos, s); S.lev_ptr.upload(levp, s);
   S.prs.upload(prs, s); S.prow.upload(prow, s); S.pcol.upload(pcol, s); S.psrc.upload(psrc, s);
   S.pin.alloc(n); S.xp.alloc(n);
@@ -387,13 +392,12 @@ static void build_sweep(SweepSet& S, const Ilu& I, bool lower, std::vector<int>&
 }
 
 // Strips: contiguous row ranges of about equal size.  With `group` (the node of every
-// equation) a cut moves forward to the next row of another node: splitting the U, V, H
-// equations of one node between two strips leaves near-zero pivots.
+// equation) a cut moves forward to the next row of another node, so the U, V, H equations
+// of one node stay in one strip.
 void ilu_setup(Ilu& I, int n, const std::vector<int>& rp, const std::vector<int>& cl, int nblk,
-               const std::vector<int>* group, hipStream_t s)
+               const std::vector<int>* group, hipStream_t s, const HostModel* hm, int nseg)
 {
   I.ready = I.fac64 = I.fac32 = false;
-  I.src.release();
   I.n = n;
   I.nblk = std::max(1, std::min(nblk, n));
   I.bstart.assign(I.nblk + 1, 0);
@@ -406,33 +410,53 @@ void ilu_setup(Ilu& I, int n, const std::vector<int>& rp, const std::vector<int>
     I.bstart[b] = cut;
   }
   I.bstart[I.nblk] = n;
-  std::vector<int> blk(n);
-  for (int b = 0; b < I.nblk; b++)
-    for (int i = I.bstart[b]; i < I.bstart[b + 1]; i++) blk[i] = b;
-  // restricted pattern (the full pattern when there is one strip)
-  std::vector<long long> src;
-  I.h_rp.assign(n + 1, 0);
-  if (I.nblk == 1) {
-    I.h_rp = rp;
-    I.h_cols = cl;
-  } else {
-    for (int i = 0; i < n; i++) {
-      int k = 0;
-      for (int q = rp[i]; q < rp[i + 1]; q++) k += blk[cl[q]] == blk[i];
-      I.h_rp[i + 1] = I.h_rp[i] + k;
-    }
-    I.h_cols.resize(I.h_rp[n]);
-    src.resize(I.h_rp[n]);
-    for (int i = 0; i < n; i++) {
-      int j = I.h_rp[i];
-      for (int q = rp[i]; q < rp[i + 1]; q++)
-        if (blk[cl[q]] == blk[i]) { I.h_cols[j] = cl[q]; src[j] = q; j++; }
-    }
+  // segments: each strip is cut across into nseg parts of equal row count along the
+  // principal axis of its nodes' coordinates (across the channel for a strip of
+  // cross-sections); entries coupling two segments are left out of the sweeps as well
+  I.seg.clear();
+  I.nseg = 1;
+  if (group && hm && nseg > 1) {
+    I.nseg = nseg;
+    I.seg.assign(n, 0);
+    parallel_for(I.nblk, [&](long long b0, long long b1) {
+      std::vector<std::pair<double, int>> proj;
+      for (long long b = b0; b < b1; b++) {
+        const int a = I.bstart[b], e = I.bstart[b + 1];
+        if (e <= a) continue;
+        double mx = 0, my = 0;
+        for (int i = a; i < e; i++) { mx += hm->x[(*group)[i]]; my += hm->y[(*group)[i]]; }
+        mx /= (e - a); my /= (e - a);
+        double sxx = 0, sxy = 0, syy = 0;
+        for (int i = a; i < e; i++) {
+          const double dx = hm->x[(*group)[i]] - mx, dy = hm->y[(*group)[i]] - my;
+          sxx += dx * dx; sxy += dx * dy; syy += dy * dy;
+        }
+        // eigenvector of the larger eigenvalue of [[sxx, sxy], [sxy, syy]]
+        const double th = 0.5 * std::atan2(2.0 * sxy, sxx - syy);
+        const double ex = std::cos(th), ey = std::sin(th);
+        proj.clear();
+        for (int i = a; i < e; i++)
+          proj.emplace_back((hm->x[(*group)[i]] - mx) * ex + (hm->y[(*group)[i]] - my) * ey, i);
+        std::sort(proj.begin(), proj.end());
+        // equal row counts, cuts moved to the next node so a node's rows share a segment
+        const int m = e - a;
+        int k = 0;
+        for (int t = 0; t < m; t++) {
+          if (k + 1 < nseg && t >= (long long)m * (k + 1) / nseg && t > 0 &&
+              (*group)[proj[t].second] != (*group)[proj[t - 1].second] &&
+              proj[t].first != proj[t - 1].first)
+            k++;
+          I.seg[proj[t].second] = k;
+        }
+      }
+    });
   }
+  I.h_rp = rp;
+  I.h_cols = cl;
   I.nnz = I.h_rp[n];
-  // factorisation schedule: lower levels over all rows (strips are independent)
+  // factorisation schedule: lower levels of the whole pattern
   std::vector<int> lev(n, 0), fptr, frows;
-  levels(0, n, I.h_rp, I.h_cols, true, lev, fptr, frows);
+  levels(0, n, I.h_rp, I.h_cols, true, std::vector<int>(), lev, fptr, frows);
   I.flev = fptr;
   I.flev_rows.upload(frows, s);
   // per-row sorted columns for the factorisation lookups
@@ -454,7 +478,6 @@ void ilu_setup(Ilu& I, int n, const std::vector<int>& rp, const std::vector<int>
   });
   I.rp.upload(I.h_rp, s); I.cols.upload(I.h_cols, s);
   I.srt_cols.upload(scol, s); I.srt_pos.upload(spos, s);
-  if (!src.empty()) I.src.upload(src, s);
   std::vector<int> olo, oup;
   build_sweep(I.lo, I, true, olo, s);
   build_sweep(I.up, I, false, oup, s);
@@ -488,10 +511,7 @@ static void factor_t(Ctx& c, Ilu& I, const VT* A)
 {
   DBuf<VT>& LU = LUbuf<VT>(I);
   LU.alloc(I.nnz);
-  if (I.nblk == 1)
-    HIPCHECK(hipMemcpyAsync(LU.p, A, sizeof(VT) * I.nnz, hipMemcpyDeviceToDevice, c.stream));
-  else
-    k_take<VT><<<4096, 256, 0, c.stream>>>(I.nnz, I.src.p, A, LU.p);
+  HIPCHECK(hipMemcpyAsync(LU.p, A, sizeof(VT) * I.nnz, hipMemcpyDeviceToDevice, c.stream));
   int* err = reinterpret_cast<int*>(c.red.p + 8);
   HIPCHECK(hipMemsetAsync(err, 0, sizeof(int), c.stream));
   for (size_t l = 0; l + 1 < I.flev.size(); l++) {
@@ -572,7 +592,7 @@ void setup_ilu_schedule(Ctx& c)
   }
   const std::vector<int>& rp = c.csr_only ? hrp : c.hm.rowptr;
   const std::vector<int>& cl = c.csr_only ? hcols : c.hm.cols;
-  ilu_setup(c.ilu, c.n, rp, cl, 1, nullptr, c.stream);
+  ilu_setup(c.ilu, c.n, rp, cl, 1, nullptr, c.stream, nullptr, 1);
   // strip ILU: one strip per compute unit, at least a few thousand rows each
   hipDeviceProp_t prop;
   HIPCHECK(hipGetDeviceProperties(&prop, c.device));
@@ -584,7 +604,8 @@ void setup_ilu_schedule(Ctx& c)
     for (size_t k = 0; k < c.hm.eqno.size(); k++)
       if (c.hm.eqno[k] >= 0) group[c.hm.eqno[k]] = (int)(k / 3);
   }
-  ilu_setup(c.silu, c.n, rp, cl, nblk, c.csr_only ? nullptr : &group, c.stream);
+  ilu_setup(c.silu, c.n, rp, cl, nblk, c.csr_only ? nullptr : &group, c.stream,
+            c.csr_only ? nullptr : &c.hm, c.segments);
 }
 
 Ilu& ilu_of(Ctx& c, int kind) { return kind == RMX_PREC_SILU0 ? c.silu : c.ilu; }

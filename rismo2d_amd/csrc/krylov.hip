@@ -8,6 +8,8 @@
 // FP32 matrix and factors and corrects with the FP64 residual.
 #include <chrono>
 #include <cmath>
+#include <cstdio>
+#include <cstdlib>
 
 #include "rmx_internal.h"
 
@@ -210,11 +212,13 @@ static bool bicgstab(Ctx& c, int prec, int pk, const double* b, double* x, int m
     dev_dots(c, 1, xs, ys, sc + S_RHO);
   }
   const double resid = std::sqrt(fetch(c, sc + S_RHO));
+  double r0r0 = resid * resid;
   accuracy = 1.0;
   iters = 0;
   bool conv = false;
   if (!(resid > 0.0)) { iters = 0; accuracy = 0.0; return true; }
   int cur = 0;
+  const bool trace = getenv("RMX_TRACE") != nullptr;
   while (!conv) {
     iters++;
     precond(c, prec, pk, p, tmp);
@@ -243,11 +247,26 @@ static bool bicgstab(Ctx& c, int prec, int pk, const double* b, double* x, int m
     k_bicg_p<<<vb, 256, 0, c.stream>>>(n, sc, S_RHO + cur, S_RHO + (cur ^ 1), r, v, p);
     HIPCHECK(hipGetLastError());
     cur ^= 1;
-    const double rr = fetch(c, sc + S_RR);
+    fetch(c, sc, 7);
+    const double rr = c.h_red[S_RR];
+    if (trace)
+      fprintf(stderr, "[rmx] bicgstab %d: rho %.6e -> %.6e (r0,v) %.6e (t,s) %.6e (t,t) %.6e (r,r) %.6e\n",
+              iters, c.h_red[S_RHO + (cur ^ 1)], c.h_red[S_RHO + cur], c.h_red[S_RV], c.h_red[S_TS],
+              c.h_red[S_TT], rr);
     const double relacc = std::sqrt(rr) / resid;
     accuracy = relacc;
     if (relacc < eps) conv = true;
     else if (iters >= maxIter || !std::isfinite(relacc)) break;
+    else if (std::fabs(c.h_red[S_RHO + cur]) < 1.0e-10 * std::sqrt(r0r0 * rr)) {
+      // breakdown: the shadow residual r0 has become orthogonal to r (the reference's
+      // recurrences then divide by rho = 0); restart with r0 = p = r, rho = (r, r)
+      k_copy<<<vb, 256, 0, c.stream>>>(n, r, r0);
+      k_copy<<<vb, 256, 0, c.stream>>>(n, r, p);
+      HIPCHECK(hipMemcpyAsync(sc + S_RHO + cur, sc + S_RR, sizeof(double), hipMemcpyDeviceToDevice,
+                              c.stream));
+      r0r0 = rr;
+      if (trace) fprintf(stderr, "[rmx] bicgstab %d: restart\n", iters);
+    }
   }
   // x = M^-1 x
   k_copy<<<vb, 256, 0, c.stream>>>(n, x, tmp);
@@ -408,6 +427,9 @@ void dev_solve(Ctx& c, const rmx_krylov_opts& o, rmx_krylov_out& out)
       out.accuracy = acc;
       k_axpy<<<vblocks(c.n), 256, 0, c.stream>>>(c.n, 1.0, d, xr);
       tr = true_resid(c, xr, r);
+      if (getenv("RMX_DEBUG"))
+        fprintf(stderr, "[rmx] refinement sweep %d: %d iterations (inner %.2e), FP64 residual %.3e\n",
+                sw + 1, it, acc, tr);
       if (tr <= o.refine_tol || !std::isfinite(tr)) { sw++; break; }
     }
     out.iters = total; out.sweeps = sw; out.true_resid = tr;

@@ -126,12 +126,12 @@ struct SweepSet {
 // One ILU(0) system: pattern, factorisation schedule, sweeps and factors.
 struct Ilu {
   bool ready = false;
-  int n = 0, nblk = 1;
+  int n = 0, nblk = 1, nseg = 1;
+  std::vector<int> seg;         // segment of every row (empty: one segment per strip)
   long long nnz = 0;
   std::vector<int> bstart, h_rp, h_cols, flev;
   DBuf<int> rp, cols, srt_cols, flev_rows, fwd2bwd;
   DBuf<unsigned char> srt_pos;
-  DBuf<long long> src;          // entry of the system matrix for each restricted entry
   SweepSet lo, up;
   DBuf<double> LU64, plo64, pup64, pdg64;
   DBuf<float> LU32, plo32, pup32, pdg32;
@@ -174,6 +174,7 @@ struct Ctx {
   Ilu ilu;                                    // reference-order ILU(0)
   Ilu silu;                                   // strip ILU(0)
   int strips = 0;                             // 0: one strip per compute unit
+  int segments = 16;                          // cross-strip cuts of the strip sweeps
   // Krylov work vectors
   DBuf<double> w[10];
   DBuf<double> red;                           // reduction scratch

@@ -65,6 +65,12 @@ class StepStats(C.Structure):
                 ("t_factor", C.c_double), ("t_solve", C.c_double), ("t_update", C.c_double)]
 
 
+class SweepInfo(C.Structure):
+    _fields_ = [("lower_entries", C.c_longlong), ("upper_entries", C.c_longlong),
+                ("lower_levels", C.c_int), ("upper_levels", C.c_int), ("strips", C.c_int),
+                ("segments", C.c_int)]
+
+
 class KernelTimes(C.Structure):
     _fields_ = [("spmv_s", C.c_double), ("ilu_apply_s", C.c_double), ("ilu_fwd_s", C.c_double),
                 ("ilu_bwd_s", C.c_double), ("silu_fwd_s", C.c_double), ("silu_bwd_s", C.c_double),
@@ -98,7 +104,8 @@ def lib():
             "rmx_convert_matrix": [_v, C.c_int], "rmx_ilu0_factor": [_v, C.c_int],
             "rmx_ilu0_apply": [_v, C.c_int, _d, _d], "rmx_spmv": [_v, C.c_int, _d, _d],
             "rmx_silu0_factor": [_v, C.c_int], "rmx_silu0_apply": [_v, C.c_int, _d, _d],
-            "rmx_set_strips": [_v, C.c_int],
+            "rmx_set_strips": [_v, C.c_int, C.c_int], "rmx_get_blocks": [_v, _i],
+            "rmx_get_sweep_info": [_v, C.c_int, C.POINTER(SweepInfo)],
             "rmx_solve": [_v, C.POINTER(KrylovOpts), C.POINTER(KrylovOut)],
             "rmx_get_matrix": [_v, C.c_int, _v, _d], "rmx_get_factors": [_v, C.c_int, _v],
             "rmx_get_solution": [_v, _d], "rmx_set_rhs": [_v, _d],
@@ -257,8 +264,22 @@ class Context:
         _check(lib().rmx_silu0_apply(self.h, prec, _p(b, _d), _p(x, _d)))
         return x
 
-    def set_strips(self, strips):
-        _check(lib().rmx_set_strips(self.h, int(strips)))
+    def set_strips(self, strips, segments=16):
+        _check(lib().rmx_set_strips(self.h, int(strips), int(segments)))
+
+    def sweep_info(self, kind=PREC_SILU0):
+        si = SweepInfo()
+        _check(lib().rmx_get_sweep_info(self.h, kind, C.byref(si)))
+        return si
+
+    def sweep_entries(self, kind=PREC_SILU0):
+        si = self.sweep_info(kind)
+        return si.lower_entries, si.upper_entries
+
+    def blocks(self):
+        b = np.zeros(self.n, np.int32)
+        _check(lib().rmx_get_blocks(self.h, _p(b, _i)))
+        return b
 
     def spmv(self, x, prec=F64):
         x = _c(x, np.float64)

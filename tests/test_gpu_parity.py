@@ -189,35 +189,40 @@ def test_mixed_precision_refinement_reaches_fp64_residual(ctx, method):
     assert np.abs(X - X64).max() <= 1e-6 * np.abs(X64).max()
 
 
-def _restrict(rp, cols, vals, nblk, group):
-    """The strip ILU(0)'s pattern: rows cut into nblk ranges at n*b/nblk, each cut moved
-    forward to the first equation of the next node (group = node of each equation),
-    entries coupling two ranges dropped, row order and entry order kept."""
+def _restrict(rp, cols, vals, block):
+    """Entries of a CSR matrix whose row and column lie in the same block, row order and
+    entry order kept."""
     n = len(rp) - 1
-    bstart = (np.arange(nblk + 1, dtype=np.int64) * n) // nblk
-    for b in range(1, nblk):
-        bstart[b] = max(bstart[b], bstart[b - 1])
-        while 0 < bstart[b] < n and group[bstart[b]] == group[bstart[b] - 1]:
-            bstart[b] += 1
-    blk = np.searchsorted(bstart, np.arange(n), side="right") - 1
     rows = np.repeat(np.arange(n), np.diff(rp))
-    keep = blk[cols] == blk[rows]
+    keep = block[cols] == block[rows]
     rp2 = np.concatenate([[0], np.cumsum(np.bincount(rows[keep], minlength=n))]).astype(np.int32)
     return rp2, np.ascontiguousarray(cols[keep]), np.ascontiguousarray(vals[keep])
 
 
-@pytest.mark.parametrize("strips", [1, 4, 17])
-def test_strip_ilu_matches_oracle_on_restricted_pattern(ctx, strips):
-    """Strip ILU(0) = the oracle's ILU(0) of the matrix with the couplings between strips
-    removed: FP64 and FP32 factor + apply agree to 1e-10 of max|result|.  The sweeps sum in
-    another order than the oracle, and the strip factors have pivots down to 1e-6 of the
-    row scale (the full factors' are >= 1e-2), which amplifies that rounding by up to 1e4
-    over the 1e-12 of the full ILU(0) apply."""
-    print(f"[gpu] strip ILU, {strips} strips", flush=True)
+def _check_blocks(block, n, strips, segments, group):
+    """Blocks: `strips` contiguous row ranges cut at n*b/strips (moved forward to the
+    next node), each split into at most `segments` parts; a node's rows share a block."""
+    strip = block // segments
+    assert (np.diff(strip) >= 0).all() and strip[0] == 0 and strip[-1] == strips - 1
+    starts = np.concatenate([[0], np.nonzero(np.diff(strip))[0] + 1])
+    for b, a in enumerate(starts):
+        nominal = n * b // strips
+        assert a >= nominal and (a == 0 or group[a] != group[a - 1])
+        assert (group[nominal:a] == group[max(nominal - 1, 0)]).all()
+    same = group[1:] == group[:-1]
+    assert (block[1:][same] == block[:-1][same]).all()
+
+
+@pytest.mark.parametrize("strips,segments", [(1, 1), (4, 1), (17, 1), (4, 8), (1, 16)])
+def test_strip_ilu_matches_oracle(ctx, strips, segments):
+    """Strip ILU(0) = the oracle's ILU(0) factors of the whole matrix, swept with the
+    entries that couple two blocks left out: FP64 and FP32 apply agree with the oracle's
+    triangular solves on that restricted pattern to 1e-12 of max|result|."""
+    print(f"[gpu] strip ILU, {strips} strips x {segments} segments", flush=True)
     g = load_golden("channel200x20")
     ctx.upload_mesh(golden_deck(g))
     n = ctx.prepare()
-    ctx.set_strips(strips)
+    ctx.set_strips(strips, segments)
     ctx.assemble(rmx.F64)
     ctx.scale_l2()
     rp, cols = ctx.pattern()
@@ -226,23 +231,23 @@ def test_strip_ilu_matches_oracle_on_restricted_pattern(ctx, strips):
     group = np.zeros(n, np.int64)
     group[eq[eq >= 0]] = np.nonzero(eq >= 0)[0] // 3
     nb = max(1, min(strips, n // 2048))
-    rp2, c2, a2 = _restrict(rp, cols, A, nb, group)
-    if nb > 1:
-        assert len(a2) < len(A)
-    ctx.silu0_factor(rmx.F64)
-    L, r = po.ilu0_factor(rp2, c2, a2)
-    assert r == 0
-    ref = po.ilu0_solve(rp2, c2, L, B)
-    y = ctx.silu0_apply(B, rmx.F64)
-    assert np.abs(y - ref).max() <= 1e-10 * np.abs(ref).max()
-    ctx.convert(rmx.F32)
-    ctx.silu0_factor(rmx.F32)
-    L32, r = po.ilu0_factor(rp2, c2, a2.astype(np.float32))
-    ref = po.ilu0_solve(rp2, c2, L32, B)
-    y = ctx.silu0_apply(B, rmx.F32)
-    assert np.abs(y - ref).max() <= 1e-10 * np.abs(ref).max()
-    # a strip ILU with one strip is the reference-order ILU(0)
-    if nb == 1:
+    block = ctx.blocks()
+    _check_blocks(block, n, nb, segments, group)
+    assert len(np.unique(block)) == nb * segments
+    for prec, dt in ((rmx.F64, np.float64), (rmx.F32, np.float32)):
+        if prec == rmx.F32:
+            ctx.convert(rmx.F32)
+        ctx.silu0_factor(prec)
+        L, r = po.ilu0_factor(rp, cols, A.astype(dt))
+        assert r == 0
+        rp2, c2, L2 = _restrict(rp, cols, L, block)
+        if nb * segments > 1:
+            assert len(L2) < len(L)
+        ref = po.ilu0_solve(rp2, c2, L2, B)
+        y = ctx.silu0_apply(B, prec)
+        assert np.abs(y - ref).max() <= 1e-12 * np.abs(ref).max(), prec
+    # one block is the reference-order ILU(0)
+    if nb * segments == 1:
         ctx.ilu0_factor(rmx.F32)
         np.testing.assert_array_equal(ctx.ilu0_apply(B, rmx.F32), y)
 
@@ -254,7 +259,7 @@ def test_strip_ilu_refinement_newton_reaches_reference(ctx):
     g = load_golden("channel200x20")
     ctx.upload_mesh(golden_deck(g))
     ctx.prepare()
-    ctx.set_strips(0)
+    ctx.set_strips(0, 16)
     for k in range(int(g["nsteps"]) + 1):
         st = ctx.newton_step(method=rmx.BICGSTAB, prec=rmx.F32, precond=rmx.PREC_SILU0,
                              eps=1e-4, refine=True, refine_tol=1e-10, refine_max=20)
