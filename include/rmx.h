@@ -34,10 +34,11 @@ enum {
 
 enum { RMX_F64 = 64, RMX_F32 = 32 };           /* matrix / factor storage precision */
 enum { RMX_BICGSTAB = 1, RMX_FGMRES = 2 };      /* Krylov method (SOLVER types) */
-/* preconditioners: ILU(0) in the reference's equation order; ILU(0) of strips of
- * consecutive equations with the couplings between strips dropped (strips factor and
- * sweep in parallel); the learned message-passing network */
-enum { RMX_PREC_NONE = 0, RMX_PREC_ILU0 = 1, RMX_PREC_MPNN = 2, RMX_PREC_SILU0 = 3 };
+/* preconditioners: ILU(0) in the reference's equation order; the same factors swept by
+ * independent blocks (strip ILU(0)); strip ILU(0) after a coarse-space correction
+ * (two-level); the learned message-passing network */
+enum { RMX_PREC_NONE = 0, RMX_PREC_ILU0 = 1, RMX_PREC_MPNN = 2, RMX_PREC_SILU0 = 3,
+       RMX_PREC_SILU0C = 4 };
 
 const char* rmx_last_error(void);
 
@@ -130,6 +131,21 @@ rmx_status rmx_silu0_apply(rmx_ctx* ctx, int prec, const double* b, double* x);
 rmx_status rmx_set_strips(rmx_ctx* ctx, int strips, int segments);
 rmx_status rmx_get_blocks(rmx_ctx* ctx, int* block /* [neq] */);
 
+/* Two-level preconditioner (RMX_PREC_SILU0C).  No reference counterpart.  Aggregates: one
+ * per (strip, coarse segment, variable U/V/H), coarse segments cut like the sweep
+ * segments; P is piecewise constant on aggregates and R = P^T.  Apply:
+ * z1 = P (R A P)^-1 R r, z = z1 + M^-1 (r - A z1) with M the strip ILU(0).
+ * rmx_set_coarse chooses the coarse segments per strip (0: no coarse space; needs a mesh,
+ * i.e. not after rmx_csr_upload); rmx_coarse_factor sums R A P from the FP32 or FP64
+ * matrix and factors it; rmx_get_aggregates writes the aggregate of every equation and
+ * returns the number of aggregates in *nc. */
+rmx_status rmx_set_coarse(rmx_ctx* ctx, int segments);
+rmx_status rmx_coarse_factor(rmx_ctx* ctx, int prec);
+rmx_status rmx_get_aggregates(rmx_ctx* ctx, int* agg /* [neq] */, int* nc);
+
+/* x = M^-1 b for any preconditioner kind (factored beforehand).  Host pointers. */
+rmx_status rmx_precond_apply(rmx_ctx* ctx, int kind, int prec, const double* b, double* x);
+
 /* Shape of the sweeps of an ILU(0) instance (kind RMX_PREC_ILU0 or RMX_PREC_SILU0): factor
  * entries each sweep reads and its number of levels (the longest strip's). */
 typedef struct {
@@ -144,7 +160,7 @@ rmx_status rmx_spmv(rmx_ctx* ctx, int prec, const double* x, double* r);
 typedef struct {
   int method;                 /* RMX_BICGSTAB | RMX_FGMRES */
   int prec;                   /* storage of matrix + factors used by the Krylov loop */
-  int precond;                /* RMX_PREC_ILU0 | RMX_PREC_NONE | RMX_PREC_MPNN */
+  int precond;                /* RMX_PREC_* */
   int max_iter;               /* SOLVER::maxIter */
   int mkyrl;                  /* FGMRES restart length (SOLVER::mkyrl) */
   double eps;                 /* SOLVER::maxDiff: relative residual tolerance */

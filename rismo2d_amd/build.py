@@ -11,7 +11,7 @@ ROOT = os.path.dirname(HERE)
 CSRC = os.path.join(HERE, "csrc")
 BUILD = os.path.join(HERE, "_build")
 LIB = os.path.join(HERE, "librmx.so")
-SOURCES = ["capi.cpp", "model.cpp", "assemble.hip", "sparse.hip", "ilu.hip", "krylov.hip", "newton.hip",
+SOURCES = ["capi.cpp", "model.cpp", "assemble.hip", "sparse.hip", "ilu.hip", "coarse.hip", "krylov.hip", "newton.hip",
            "mpnn.hip"]
 HIPCC = os.environ.get("HIPCC", "/opt/rocm/bin/hipcc")
 FLAGS = ["--offload-arch=gfx950", "-O3", "-fPIC", "-std=c++17", "-Wall", "-Wno-unused-function",

@@ -351,6 +351,51 @@ rmx_status rmx_set_strips(rmx_ctx* p, int strips, int segments)
   API_END
 }
 
+rmx_status rmx_set_coarse(rmx_ctx* p, int segments)
+{
+  API_BEGIN
+  Ctx& c = *need(p);
+  if (segments < 0 || segments > 1024) fail(RMX_ERR_ARG, "coarse segments must be 0..1024");
+  c.coarse_segments = segments;
+  if (c.prepared) setup_ilu_schedule(c);
+  API_END
+}
+
+rmx_status rmx_coarse_factor(rmx_ctx* p, int prec)
+{
+  API_BEGIN
+  Ctx& c = *need(p);
+  if (!c.prepared) fail(RMX_ERR_ARG, "no system");
+  coarse_factor(c, prec == RMX_F32 ? RMX_F32 : RMX_F64);
+  API_END
+}
+
+rmx_status rmx_get_aggregates(rmx_ctx* p, int* agg, int* nc)
+{
+  API_BEGIN
+  Ctx& c = *need(p);
+  if (!c.prepared || !c.coarse.ready) fail(RMX_ERR_ARG, "no coarse space");
+  if (!agg || !nc) fail(RMX_ERR_ARG, "null output");
+  c.coarse.agg.download(agg, c.n, c.stream);
+  HIPCHECK(hipStreamSynchronize(c.stream));
+  *nc = c.coarse.nc;
+  API_END
+}
+
+rmx_status rmx_precond_apply(rmx_ctx* p, int kind, int prec, const double* b, double* x)
+{
+  API_BEGIN
+  Ctx& c = *need(p);
+  if (!c.prepared) fail(RMX_ERR_ARG, "no system");
+  if (!b || !x) fail(RMX_ERR_ARG, "null vector");
+  const int pr = prec == RMX_F32 ? RMX_F32 : RMX_F64;
+  c.w[0].upload(b, c.n, c.stream);
+  dev_precond(c, pr, kind, c.w[0].p, c.w[1].p);
+  c.w[1].download(x, c.n, c.stream);
+  HIPCHECK(hipStreamSynchronize(c.stream));
+  API_END
+}
+
 rmx_status rmx_get_sweep_info(rmx_ctx* p, int kind, rmx_sweep_info* out)
 {
   API_BEGIN
@@ -525,8 +570,9 @@ rmx_status rmx_newton_step(rmx_ctx* p, const rmx_krylov_opts* o, rmx_step_stats*
   auto t2 = now();
   const int prec = o->prec == RMX_F32 ? RMX_F32 : RMX_F64;
   if (prec == RMX_F32) dev_convert(c, RMX_F32);
-  if (o->precond == RMX_PREC_ILU0 || o->precond == RMX_PREC_SILU0)
+  if (o->precond == RMX_PREC_ILU0 || o->precond == RMX_PREC_SILU0 || o->precond == RMX_PREC_SILU0C)
     ilu_factor(c, ilu_of(c, o->precond), prec);
+  if (o->precond == RMX_PREC_SILU0C) coarse_factor(c, prec);
   HIPCHECK(hipStreamSynchronize(c.stream));
   auto t3 = now();
   dev_solve(c, *o, st->solve);

@@ -391,6 +391,48 @@ static void build_sweep(SweepSet& S, const Ilu& I, bool lower, std::vector<int>&
   order = prow;
 }
 
+// Segment of every row: each strip [bstart[b], bstart[b+1]) is cut into nseg parts of
+// about equal row count along the principal axis of its nodes' coordinates (across the
+// channel for a strip of cross-sections); a node's rows share a segment.
+std::vector<int> cut_segments(const std::vector<int>& bstart, const std::vector<int>& group,
+                              const HostModel& hm, int nseg)
+{
+  const int nblk = (int)bstart.size() - 1;
+  std::vector<int> seg(bstart[nblk], 0);
+  parallel_for(nblk, [&](long long b0, long long b1) {
+    std::vector<std::pair<double, int>> proj;
+    for (long long b = b0; b < b1; b++) {
+      const int a = bstart[b], e = bstart[b + 1];
+      if (e <= a) continue;
+      double mx = 0, my = 0;
+      for (int i = a; i < e; i++) { mx += hm.x[group[i]]; my += hm.y[group[i]]; }
+      mx /= (e - a); my /= (e - a);
+      double sxx = 0, sxy = 0, syy = 0;
+      for (int i = a; i < e; i++) {
+        const double dx = hm.x[group[i]] - mx, dy = hm.y[group[i]] - my;
+        sxx += dx * dx; sxy += dx * dy; syy += dy * dy;
+      }
+      // eigenvector of the larger eigenvalue of [[sxx, sxy], [sxy, syy]]
+      const double th = 0.5 * std::atan2(2.0 * sxy, sxx - syy);
+      const double ex = std::cos(th), ey = std::sin(th);
+      proj.clear();
+      for (int i = a; i < e; i++)
+        proj.emplace_back((hm.x[group[i]] - mx) * ex + (hm.y[group[i]] - my) * ey, i);
+      std::sort(proj.begin(), proj.end());
+      // equal row counts, cuts moved to the next node so a node's rows share a segment
+      const int m = e - a;
+      int k = 0;
+      for (int t = 0; t < m; t++) {
+        if (k + 1 < nseg && t >= (long long)m * (k + 1) / nseg && t > 0 &&
+            group[proj[t].second] != group[proj[t - 1].second] && proj[t].first != proj[t - 1].first)
+          k++;
+        seg[proj[t].second] = k;
+      }
+    }
+  });
+  return seg;
+}
+
 // Strips: contiguous row ranges of about equal size.  With `group` (the node of every
 // equation) a cut moves forward to the next row of another node, so the U, V, H equations
 // of one node stay in one strip.
@@ -417,39 +459,7 @@ void ilu_setup(Ilu& I, int n, const std::vector<int>& rp, const std::vector<int>
   I.nseg = 1;
   if (group && hm && nseg > 1) {
     I.nseg = nseg;
-    I.seg.assign(n, 0);
-    parallel_for(I.nblk, [&](long long b0, long long b1) {
-      std::vector<std::pair<double, int>> proj;
-      for (long long b = b0; b < b1; b++) {
-        const int a = I.bstart[b], e = I.bstart[b + 1];
-        if (e <= a) continue;
-        double mx = 0, my = 0;
-        for (int i = a; i < e; i++) { mx += hm->x[(*group)[i]]; my += hm->y[(*group)[i]]; }
-        mx /= (e - a); my /= (e - a);
-        double sxx = 0, sxy = 0, syy = 0;
-        for (int i = a; i < e; i++) {
-          const double dx = hm->x[(*group)[i]] - mx, dy = hm->y[(*group)[i]] - my;
-          sxx += dx * dx; sxy += dx * dy; syy += dy * dy;
-        }
-        // eigenvector of the larger eigenvalue of [[sxx, sxy], [sxy, syy]]
-        const double th = 0.5 * std::atan2(2.0 * sxy, sxx - syy);
-        const double ex = std::cos(th), ey = std::sin(th);
-        proj.clear();
-        for (int i = a; i < e; i++)
-          proj.emplace_back((hm->x[(*group)[i]] - mx) * ex + (hm->y[(*group)[i]] - my) * ey, i);
-        std::sort(proj.begin(), proj.end());
-        // equal row counts, cuts moved to the next node so a node's rows share a segment
-        const int m = e - a;
-        int k = 0;
-        for (int t = 0; t < m; t++) {
-          if (k + 1 < nseg && t >= (long long)m * (k + 1) / nseg && t > 0 &&
-              (*group)[proj[t].second] != (*group)[proj[t - 1].second] &&
-              proj[t].first != proj[t - 1].first)
-            k++;
-          I.seg[proj[t].second] = k;
-        }
-      }
-    });
+    I.seg = cut_segments(I.bstart, *group, *hm, nseg);
   }
   I.h_rp = rp;
   I.h_cols = cl;
@@ -606,9 +616,11 @@ void setup_ilu_schedule(Ctx& c)
   }
   ilu_setup(c.silu, c.n, rp, cl, nblk, c.csr_only ? nullptr : &group, c.stream,
             c.csr_only ? nullptr : &c.hm, c.segments);
+  c.coarse.ready = c.coarse.fac = false;
+  if (!c.csr_only && c.coarse_segments > 0) coarse_setup(c, rp, cl, group);
 }
 
-Ilu& ilu_of(Ctx& c, int kind) { return kind == RMX_PREC_SILU0 ? c.silu : c.ilu; }
+Ilu& ilu_of(Ctx& c, int kind) { return (kind == RMX_PREC_SILU0 || kind == RMX_PREC_SILU0C) ? c.silu : c.ilu; }
 
 void dev_ilu0_factor(Ctx& c, int prec) { ilu_factor(c, c.ilu, prec); }
 

@@ -171,15 +171,24 @@ __global__ void k_mfinish(int K, const double* __restrict__ partial, double* __r
 
 static int vblocks(long long n) { return (int)std::min<long long>(4096, (n + 255) / 256); }
 
-static void precond(Ctx& c, int prec, int kind, const double* in, double* out)
+void dev_precond(Ctx& c, int prec, int kind, const double* in, double* out)
 {
   if (kind == RMX_PREC_ILU0 || kind == RMX_PREC_SILU0) {
     ilu_apply(c, ilu_of(c, kind), prec, in, out, nullptr);
+  } else if (kind == RMX_PREC_SILU0C) {
+    coarse_apply(c, prec, in, out);
   } else if (kind == RMX_PREC_MPNN) {
     mpnn_apply(c, in, out);
-  } else {
+  } else if (kind == RMX_PREC_NONE) {
     k_copy<<<vblocks(c.n), 256, 0, c.stream>>>(c.n, in, out);
+  } else {
+    fail(RMX_ERR_ARG, "unknown preconditioner kind " + std::to_string(kind));
   }
+}
+
+static void precond(Ctx& c, int prec, int kind, const double* in, double* out)
+{
+  dev_precond(c, prec, kind, in, out);
 }
 
 void ensure_work(Ctx& c)
@@ -394,11 +403,13 @@ void dev_solve(Ctx& c, const rmx_krylov_opts& o, rmx_krylov_out& out)
   const auto t0 = std::chrono::steady_clock::now();
   const int prec = o.prec == RMX_F32 ? RMX_F32 : RMX_F64;
   const int pk = o.precond;
-  if (pk == RMX_PREC_ILU0 || pk == RMX_PREC_SILU0) {
+  if (pk == RMX_PREC_ILU0 || pk == RMX_PREC_SILU0 || pk == RMX_PREC_SILU0C) {
     Ilu& I = ilu_of(c, pk);
     if (!((prec == RMX_F32) ? I.fac32 : I.fac64))
       fail(RMX_ERR_ARG, "solve: ILU(0) factors of the requested precision are missing");
   }
+  if (pk == RMX_PREC_SILU0C && !(c.coarse.fac && c.coarse.fac_prec == prec))
+    fail(RMX_ERR_ARG, "solve: coarse space not factored in the requested precision");
   if (pk == RMX_PREC_MPNN && !mpnn_ready(c)) fail(RMX_ERR_ARG, "solve: no learned preconditioner loaded");
   if ((prec == RMX_F32 && !c.have32) || (prec == RMX_F64 && !c.have64))
     fail(RMX_ERR_ARG, "solve: matrix of the requested precision is missing");

@@ -104,13 +104,6 @@ struct Params {
   double hmin() const { return dry_limit / 2.0; }
 };
 
-// Level-set schedule of a triangular sweep or of the ILU(0) factorisation.
-struct Levels {
-  int nlev = 0;
-  std::vector<int> ptr, rows;                 // host copy
-  DBuf<int> d_ptr, d_rows;
-};
-
 constexpr int RING = 8192;     // LDS ring of solved x values in the packed sweeps
 
 // Packed layout of one triangular sweep over all strips of an ILU system.
@@ -136,6 +129,24 @@ struct Ilu {
   DBuf<double> LU64, plo64, pup64, pdg64;
   DBuf<float> LU32, plo32, pup32, pdg32;
   bool fac64 = false, fac32 = false;
+};
+
+// Coarse space of the two-level preconditioner: one aggregate per (strip, coarse segment,
+// variable U/V/H); the coarse matrix R A P (R = P^T, P piecewise constant on aggregates) is
+// banded in aggregate order and factored on the host.
+struct Coarse {
+  bool ready = false, fac = false;
+  int nc = 0, kl = 0, ku = 0, fac_prec = 0;
+  long long nslot = 0;
+  std::vector<int> slot_row, slot_col;        // coarse entry (I, J) of every slot
+  DBuf<int> agg, agg_ptr, agg_rows;           // aggregate of every row; rows of each aggregate
+  DBuf<long long> slot_ptr;                   // entries of A summed into each slot
+  DBuf<int> perm;
+  DBuf<double> cval, rc, zc, z1, t;
+  std::vector<double> band;                   // LU factors, band storage (LAPACK layout)
+  std::vector<int> ipiv;
+  double* h_buf = nullptr;                    // pinned, max(nc, nslot) doubles
+  ~Coarse() { if (h_buf) (void)hipHostFree(h_buf); }
 };
 
 struct Ctx {
@@ -175,6 +186,8 @@ struct Ctx {
   Ilu silu;                                   // strip ILU(0)
   int strips = 0;                             // 0: one strip per compute unit
   int segments = 16;                          // cross-strip cuts of the strip sweeps
+  Coarse coarse;
+  int coarse_segments = 4;                    // cross-strip cuts of the coarse aggregates
   // Krylov work vectors
   DBuf<double> w[10];
   DBuf<double> red;                           // reduction scratch
@@ -217,5 +230,12 @@ Ilu& ilu_of(Ctx& c, int kind);
 void ilu_factor(Ctx& c, Ilu& I, int prec);
 void ilu_apply(Ctx& c, Ilu& I, int prec, const double* b, double* x, hipEvent_t* ev);
 void upload_shapes(hipStream_t s);
+std::vector<int> cut_segments(const std::vector<int>& bstart, const std::vector<int>& group,
+                              const HostModel& hm, int nseg);
+void coarse_setup(Ctx& c, const std::vector<int>& rp, const std::vector<int>& cl,
+                  const std::vector<int>& group);
+void coarse_factor(Ctx& c, int prec);
+void coarse_apply(Ctx& c, int prec, const double* r, double* z);   // device pointers
+void dev_precond(Ctx& c, int prec, int kind, const double* in, double* out);
 
 }  // namespace rmx

@@ -18,7 +18,7 @@ HEADER = os.path.join(os.path.dirname(HERE), "include", "rmx.h")
 
 F64, F32 = 64, 32
 BICGSTAB, FGMRES = 1, 2
-PREC_NONE, PREC_ILU0, PREC_MPNN, PREC_SILU0 = 0, 1, 2, 3
+PREC_NONE, PREC_ILU0, PREC_MPNN, PREC_SILU0, PREC_SILU0C = 0, 1, 2, 3, 4
 ERR_NODEVICE = -4
 
 _d = C.POINTER(C.c_double)
@@ -105,6 +105,8 @@ def lib():
             "rmx_ilu0_apply": [_v, C.c_int, _d, _d], "rmx_spmv": [_v, C.c_int, _d, _d],
             "rmx_silu0_factor": [_v, C.c_int], "rmx_silu0_apply": [_v, C.c_int, _d, _d],
             "rmx_set_strips": [_v, C.c_int, C.c_int], "rmx_get_blocks": [_v, _i],
+            "rmx_set_coarse": [_v, C.c_int], "rmx_coarse_factor": [_v, C.c_int],
+            "rmx_get_aggregates": [_v, _i, _i], "rmx_precond_apply": [_v, C.c_int, C.c_int, _d, _d],
             "rmx_get_sweep_info": [_v, C.c_int, C.POINTER(SweepInfo)],
             "rmx_solve": [_v, C.POINTER(KrylovOpts), C.POINTER(KrylovOut)],
             "rmx_get_matrix": [_v, C.c_int, _v, _d], "rmx_get_factors": [_v, C.c_int, _v],
@@ -275,6 +277,24 @@ class Context:
     def sweep_entries(self, kind=PREC_SILU0):
         si = self.sweep_info(kind)
         return si.lower_entries, si.upper_entries
+
+    def set_coarse(self, segments):
+        _check(lib().rmx_set_coarse(self.h, int(segments)))
+
+    def coarse_factor(self, prec=F64):
+        _check(lib().rmx_coarse_factor(self.h, prec))
+
+    def aggregates(self):
+        a = np.zeros(self.n, np.int32)
+        nc = C.c_int(0)
+        _check(lib().rmx_get_aggregates(self.h, _p(a, _i), C.byref(nc)))
+        return a, nc.value
+
+    def precond_apply(self, kind, b, prec=F64):
+        b = _c(b, np.float64)
+        x = np.zeros(self.n)
+        _check(lib().rmx_precond_apply(self.h, kind, prec, _p(b, _d), _p(x, _d)))
+        return x
 
     def blocks(self):
         b = np.zeros(self.n, np.int32)
