@@ -143,6 +143,31 @@ rmx_status rmx_set_coarse(rmx_ctx* ctx, int segments);
 rmx_status rmx_coarse_factor(rmx_ctx* ctx, int prec);
 rmx_status rmx_get_aggregates(rmx_ctx* ctx, int* agg /* [neq] */, int* nc);
 
+/* Learned preconditioner (RMX_PREC_MPNN).  No reference counterpart.  A message-passing
+ * network on the mesh node graph (an edge joins two nodes of one element): node features
+ * (u, v, depth/4, bed/4, 100 cf, residual u/v/h components / RMS(r)), an input layer
+ * 8 -> 64, 6 layers h = h + relu([h, mean of neighbours' h] W + b) (BF16 MFMA, FP32
+ * accumulate), an output layer 64 -> 3.  As a preconditioner it adds RMS(r) * network(r)
+ * to the two-level preconditioner's M2^-1 r.
+ * rmx_mpnn_n_params: number of FP32 weights (layout: W_in [8][64], b_in, then per layer
+ * W [128][64], b [64], then W_out [64][3], b_out [3]).
+ * rmx_mpnn_set_weights / get_weights: the weights (host, FP32); setting also resets Adam.
+ * rmx_mpnn_forward: z = RMS(r) * network(r) alone, and the node features [np][8] (nullable).
+ * rmx_mpnn_add_sample: store a training sample (features of the current state and r,
+ * target t / RMS(r) per node variable); capacity > 0 (re)allocates and clears the set.
+ * rmx_mpnn_loss_grad: mean squared error over the variables with an equation, and its
+ * gradient (nullable) for one stored sample.
+ * rmx_mpnn_train: Adam (beta 0.9 / 0.999) over the stored samples, one step per sample,
+ * `epochs` passes; *loss = mean loss of the last pass. */
+rmx_status rmx_mpnn_n_params(int* n);
+rmx_status rmx_mpnn_set_weights(rmx_ctx* ctx, const float* theta, int n);
+rmx_status rmx_mpnn_get_weights(rmx_ctx* ctx, float* theta, int n);
+rmx_status rmx_mpnn_forward(rmx_ctx* ctx, const double* r, double* z, float* feat);
+rmx_status rmx_mpnn_add_sample(rmx_ctx* ctx, const double* r, const double* t, int capacity,
+                               int* count);
+rmx_status rmx_mpnn_loss_grad(rmx_ctx* ctx, int sample, double* loss, float* grad);
+rmx_status rmx_mpnn_train(rmx_ctx* ctx, int epochs, double lr, double* loss);
+
 /* x = M^-1 b for any preconditioner kind (factored beforehand).  Host pointers. */
 rmx_status rmx_precond_apply(rmx_ctx* ctx, int kind, int prec, const double* b, double* x);
 

@@ -210,6 +210,7 @@ static void first_prepare(Ctx& c)
   c.slots.upload(slots, s);
   HIPCHECK(hipStreamSynchronize(s));
   ensure_work(c);
+  choose_spmv_lanes(c);
   setup_ilu_schedule(c);
   c.have32 = c.have64 = false;
   c.prepared = true;
@@ -396,6 +397,77 @@ rmx_status rmx_precond_apply(rmx_ctx* p, int kind, int prec, const double* b, do
   API_END
 }
 
+rmx_status rmx_mpnn_n_params(int* n)
+{
+  API_BEGIN
+  if (!n) fail(RMX_ERR_ARG, "null output");
+  *n = 50307;
+  API_END
+}
+
+rmx_status rmx_mpnn_set_weights(rmx_ctx* p, const float* theta, int n)
+{
+  API_BEGIN
+  Ctx& c = *need(p);
+  if (!theta) fail(RMX_ERR_ARG, "null weights");
+  mpnn_set_weights(c, theta, n);
+  API_END
+}
+
+rmx_status rmx_mpnn_get_weights(rmx_ctx* p, float* theta, int n)
+{
+  API_BEGIN
+  Ctx& c = *need(p);
+  if (!theta) fail(RMX_ERR_ARG, "null weights");
+  mpnn_get_weights(c, theta, n);
+  API_END
+}
+
+rmx_status rmx_mpnn_forward(rmx_ctx* p, const double* r, double* z, float* feat)
+{
+  API_BEGIN
+  Ctx& c = *need(p);
+  if (!r || !z) fail(RMX_ERR_ARG, "null vector");
+  if (!c.prepared) fail(RMX_ERR_ARG, "no system");
+  c.w[0].upload(r, c.n, c.stream);
+  mpnn_forward(c, c.w[0].p, c.w[1].p, feat);
+  c.w[1].download(z, c.n, c.stream);
+  HIPCHECK(hipStreamSynchronize(c.stream));
+  API_END
+}
+
+rmx_status rmx_mpnn_add_sample(rmx_ctx* p, const double* r, const double* t, int capacity, int* count)
+{
+  API_BEGIN
+  Ctx& c = *need(p);
+  if (!r || !t) fail(RMX_ERR_ARG, "null vector");
+  if (!c.prepared) fail(RMX_ERR_ARG, "no system");
+  c.w[0].upload(r, c.n, c.stream);
+  c.w[1].upload(t, c.n, c.stream);
+  const int k = mpnn_add_sample(c, c.w[0].p, c.w[1].p, capacity);
+  if (count) *count = k;
+  API_END
+}
+
+rmx_status rmx_mpnn_loss_grad(rmx_ctx* p, int sample, double* loss, float* grad)
+{
+  API_BEGIN
+  Ctx& c = *need(p);
+  const double l = mpnn_loss_grad(c, sample, grad);
+  if (loss) *loss = l;
+  API_END
+}
+
+rmx_status rmx_mpnn_train(rmx_ctx* p, int epochs, double lr, double* loss)
+{
+  API_BEGIN
+  Ctx& c = *need(p);
+  if (epochs < 1 || !(lr > 0.0)) fail(RMX_ERR_ARG, "epochs >= 1, lr > 0");
+  const double l = mpnn_train(c, epochs, lr);
+  if (loss) *loss = l;
+  API_END
+}
+
 rmx_status rmx_get_sweep_info(rmx_ctx* p, int kind, rmx_sweep_info* out)
 {
   API_BEGIN
@@ -535,6 +607,7 @@ rmx_status rmx_csr_upload(rmx_ctx* p, int n, const int* rowptr, const int* cols,
   c.X.alloc(n);
   HIPCHECK(hipStreamSynchronize(c.stream));
   ensure_work(c);
+  choose_spmv_lanes(c);
   setup_ilu_schedule(c);
   c.have64 = true; c.have32 = false;
   c.prepared = true;
@@ -570,9 +643,10 @@ rmx_status rmx_newton_step(rmx_ctx* p, const rmx_krylov_opts* o, rmx_step_stats*
   auto t2 = now();
   const int prec = o->prec == RMX_F32 ? RMX_F32 : RMX_F64;
   if (prec == RMX_F32) dev_convert(c, RMX_F32);
-  if (o->precond == RMX_PREC_ILU0 || o->precond == RMX_PREC_SILU0 || o->precond == RMX_PREC_SILU0C)
-    ilu_factor(c, ilu_of(c, o->precond), prec);
-  if (o->precond == RMX_PREC_SILU0C) coarse_factor(c, prec);
+  const int pk = o->precond;
+  if (pk == RMX_PREC_ILU0 || pk == RMX_PREC_SILU0 || pk == RMX_PREC_SILU0C || pk == RMX_PREC_MPNN)
+    ilu_factor(c, ilu_of(c, pk), prec);
+  if (pk == RMX_PREC_SILU0C || pk == RMX_PREC_MPNN) coarse_factor(c, prec);
   HIPCHECK(hipStreamSynchronize(c.stream));
   auto t3 = now();
   dev_solve(c, *o, st->solve);

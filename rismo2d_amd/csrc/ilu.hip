@@ -52,6 +52,16 @@ constexpr int WIDE = RING / 4;  // levels wider than this use the global copy on
 // ---------------------------------------------------------------------------------------
 // factorisation
 
+constexpr int UCAP = 1024;      // entries of the pivot rows staged in LDS per row
+constexpr int UPL = UCAP / 64;  // staged entries per lane
+
+// One row of one level (one wave): row i eliminates its lower entries k in increasing
+// column order (IKJ).  The pivot rows are first staged in LDS with independent loads, and
+// every staged entry gets its target position in row i (a binary search over row i's
+// sorted columns, done in parallel), so the sequential loop over k is one division and
+// one LDS update per entry.  The updates are the reference's, in its order, with FMA
+// contraction off.  Rows whose pivot rows hold more than UCAP entries search and read
+// from global memory inside the loop instead.
 template <class VT>
 __global__ void __launch_bounds__(64) k_ilu_level(const int* __restrict__ rows, int cnt,
                                                   const int* __restrict__ rowptr,
@@ -64,45 +74,113 @@ __global__ void __launch_bounds__(64) k_ilu_level(const int* __restrict__ rows, 
   __shared__ VT s_val[FW];
   __shared__ int s_col[FW];
   __shared__ unsigned char s_pos[FW];
+  __shared__ int s_kb[FW], s_off[FW + 1];
+  __shared__ VT s_dk[FW];
+  __shared__ unsigned char s_upos[UCAP];
+  __shared__ VT s_uval[UCAP];
+  __shared__ int s_nk;
   const int lane = threadIdx.x;
   const int w = blockIdx.x;
   if (w >= cnt) return;                 // uniform per block
   const int i = rows[w];
   const int p0 = rowptr[i];
   const int wi = rowptr[i + 1] - p0;
+  if (lane == 0) s_nk = 0;
   for (int t = lane; t < wi; t += 64) {
     s_val[t] = LU[p0 + t];
     s_col[t] = srt_cols[p0 + t];
     s_pos[t] = srt_pos[p0 + t];
   }
   __syncthreads();
-  for (int t = 0; t < wi; t++) {
+  // number of lower entries: the sorted columns below i
+  for (int t = lane; t < wi; t += 64)
+    if (s_col[t] < i && (t + 1 == wi || s_col[t + 1] >= i)) s_nk = t + 1;
+  __syncthreads();
+  const int nk = s_nk;
+  // pivot rows: start, diagonal, number of entries after the diagonal
+  for (int t = lane; t < nk; t += 64) {
     const int k = s_col[t];
-    if (k >= i) break;
-    const int pik = s_pos[t];
-    const int q0 = rowptr[k], q1 = rowptr[k + 1];
-    const VT ukk = LU[q0];
-    const VT f = -s_val[pik] / ukk;
-    __syncthreads();
-    if (lane == 0) s_val[pik] = f;
-    __syncthreads();
-    for (int q = q0 + 1 + lane; q < q1; q += 64) {
-      const int cc = cols[q];
-      if (cc <= k) continue;
-      int lo = 0, hi = wi - 1, pos = -1;
-      while (lo <= hi) {
-        const int mid = (lo + hi) >> 1;
-        const int v = s_col[mid];
-        if (v == cc) { pos = s_pos[mid]; break; }
-        if (v < cc) lo = mid + 1; else hi = mid - 1;
-      }
-      if (pos >= 0) {
-        const VT prod = f * LU[q];
-        s_val[pos] = s_val[pos] + prod;
+    const int kb = rowptr[k], ke = rowptr[k + 1];
+    s_kb[t] = kb;
+    s_off[t + 1] = ke - kb - 1;
+    s_dk[t] = LU[kb];
+  }
+  __syncthreads();
+  if (lane == 0) {
+    s_off[0] = 0;
+    for (int t = 0; t < nk; t++) s_off[t + 1] += s_off[t];
+  }
+  __syncthreads();
+  // target position in row i of a column cc of pivot row k (255: not in row i, or cc <= k)
+  auto target = [&](int cc, int k) -> int {
+    if (cc <= k) return 255;
+    int lo = 0, hi = wi - 1;
+    while (lo <= hi) {
+      const int mid = (lo + hi) >> 1;
+      const int v = s_col[mid];
+      if (v == cc) return s_pos[mid];
+      if (v < cc) lo = mid + 1; else hi = mid - 1;
+    }
+    return 255;
+  };
+  const int total = s_off[nk];
+  const bool staged = total <= UCAP;
+  if (staged) {
+    int uc[UPL], ut[UPL];
+    VT uv[UPL];
+#pragma unroll
+    for (int m = 0; m < UPL; m++) {
+      const int u = lane + 64 * m;
+      ut[m] = 0;
+      if (u < total) {
+        int lo = 0, hi = nk - 1;            // pivot row t with s_off[t] <= u < s_off[t+1]
+        while (lo < hi) {
+          const int mid = (lo + hi + 1) >> 1;
+          if (s_off[mid] <= u) lo = mid; else hi = mid - 1;
+        }
+        const int q = s_kb[lo] + 1 + (u - s_off[lo]);
+        ut[m] = lo;
+        uc[m] = cols[q];
+        uv[m] = LU[q];
       }
     }
-    __syncthreads();
+#pragma unroll
+    for (int m = 0; m < UPL; m++) {
+      const int u = lane + 64 * m;
+      if (u < total) {
+        s_upos[u] = (unsigned char)target(uc[m], s_col[ut[m]]);
+        s_uval[u] = uv[m];
+      }
+    }
   }
+  __syncthreads();
+  for (int t = 0; t < nk; t++) {
+    const int k = s_col[t];
+    const int pik = s_pos[t];
+    const VT f = -s_val[pik] / s_dk[t];   // every lane, from the same LDS values
+    const int ua = s_off[t], ub = s_off[t + 1];
+    if (staged) {
+      for (int e = ua + lane; e < ub; e += 64) {
+        const int pos = s_upos[e];
+        if (pos != 255) {
+          const VT prod = f * s_uval[e];
+          s_val[pos] = s_val[pos] + prod;
+        }
+      }
+    } else {
+      for (int e = lane; e < ub - ua; e += 64) {
+        const int q = s_kb[t] + 1 + e;
+        const int pos = target(cols[q], k);
+        if (pos != 255) {
+          const VT prod = f * LU[q];
+          s_val[pos] = s_val[pos] + prod;
+        }
+      }
+    }
+    __syncthreads();                      // the stored multiplier and updates before the next k
+    if (lane == 0) s_val[pik] = f;
+  }
+  __syncthreads();
   for (int t = lane; t < wi; t += 64) LU[p0 + t] = s_val[t];
   if (lane == 0 && fabs((double)s_val[0]) < 1.0e-20) atomicOr(err, 1);
 }
@@ -620,7 +698,10 @@ void setup_ilu_schedule(Ctx& c)
   if (!c.csr_only && c.coarse_segments > 0) coarse_setup(c, rp, cl, group);
 }
 
-Ilu& ilu_of(Ctx& c, int kind) { return (kind == RMX_PREC_SILU0 || kind == RMX_PREC_SILU0C) ? c.silu : c.ilu; }
+Ilu& ilu_of(Ctx& c, int kind)
+{
+  return (kind == RMX_PREC_SILU0 || kind == RMX_PREC_SILU0C || kind == RMX_PREC_MPNN) ? c.silu : c.ilu;
+}
 
 void dev_ilu0_factor(Ctx& c, int prec) { ilu_factor(c, c.ilu, prec); }
 

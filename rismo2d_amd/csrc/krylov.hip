@@ -15,8 +15,6 @@
 
 namespace rmx {
 
-void mpnn_apply(Ctx& c, const double* r, double* z);   // mpnn.hip
-bool mpnn_ready(Ctx& c);
 
 enum { S_RHO = 0, S_RV = 2, S_TS = 3, S_TT = 4, S_RR = 5, S_B2 = 6 };
 
@@ -141,20 +139,30 @@ __device__ __forceinline__ double bsum(double v, double* sh)
   return s;
 }
 
-// partial[j*RED_BLOCKS + block] = <V[j], y> for j < K
+// partial[j*RED_BLOCKS + block] = <V[j], y> for j < K <= 32: y read once, K accumulators
+// per thread
+constexpr int MDOT_MAX = 32;
 __global__ void __launch_bounds__(256) k_mdot(long long n, int K, const double* __restrict__ Vb,
                                               long long ld, const double* __restrict__ y,
                                               double* __restrict__ partial)
 {
   __shared__ double sh[4];
-  for (int j = 0; j < K; j++) {
-    double acc = 0.0;
-    const double* vj = Vb + j * ld;
-    for (long long i = (long long)blockIdx.x * blockDim.x + threadIdx.x; i < n;
-         i += (long long)gridDim.x * blockDim.x)
-      acc += vj[i] * y[i];
-    const double s = bsum(acc, sh);
-    if (threadIdx.x == 0) partial[j * RED_BLOCKS + blockIdx.x] = s;
+  double acc[MDOT_MAX];
+#pragma unroll
+  for (int j = 0; j < MDOT_MAX; j++) acc[j] = 0.0;
+  for (long long i = (long long)blockIdx.x * blockDim.x + threadIdx.x; i < n;
+       i += (long long)gridDim.x * blockDim.x) {
+    const double yi = y[i];
+#pragma unroll
+    for (int j = 0; j < MDOT_MAX; j++)
+      if (j < K) acc[j] += Vb[j * ld + i] * yi;
+  }
+#pragma unroll
+  for (int j = 0; j < MDOT_MAX; j++) {
+    if (j < K) {
+      const double s = bsum(acc[j], sh);
+      if (threadIdx.x == 0) partial[j * RED_BLOCKS + blockIdx.x] = s;
+    }
   }
 }
 
@@ -178,7 +186,7 @@ void dev_precond(Ctx& c, int prec, int kind, const double* in, double* out)
   } else if (kind == RMX_PREC_SILU0C) {
     coarse_apply(c, prec, in, out);
   } else if (kind == RMX_PREC_MPNN) {
-    mpnn_apply(c, in, out);
+    mpnn_apply(c, prec, in, out);
   } else if (kind == RMX_PREC_NONE) {
     k_copy<<<vblocks(c.n), 256, 0, c.stream>>>(c.n, in, out);
   } else {
@@ -398,17 +406,20 @@ static double true_resid(Ctx& c, const double* x, double* r)
 
 void dev_solve(Ctx& c, const rmx_krylov_opts& o, rmx_krylov_out& out)
 {
+  if (o.method == RMX_FGMRES && (o.mkyrl < 1 || o.mkyrl > MDOT_MAX - 2))
+    fail(RMX_ERR_ARG, "solve: FGMRES restart length must be 1..30");
+  if (o.method != RMX_FGMRES && o.method != RMX_BICGSTAB) fail(RMX_ERR_ARG, "solve: unknown method");
   ensure_work(c);
   c.X.alloc(c.n);
   const auto t0 = std::chrono::steady_clock::now();
   const int prec = o.prec == RMX_F32 ? RMX_F32 : RMX_F64;
   const int pk = o.precond;
-  if (pk == RMX_PREC_ILU0 || pk == RMX_PREC_SILU0 || pk == RMX_PREC_SILU0C) {
+  if (pk == RMX_PREC_ILU0 || pk == RMX_PREC_SILU0 || pk == RMX_PREC_SILU0C || pk == RMX_PREC_MPNN) {
     Ilu& I = ilu_of(c, pk);
     if (!((prec == RMX_F32) ? I.fac32 : I.fac64))
       fail(RMX_ERR_ARG, "solve: ILU(0) factors of the requested precision are missing");
   }
-  if (pk == RMX_PREC_SILU0C && !(c.coarse.fac && c.coarse.fac_prec == prec))
+  if ((pk == RMX_PREC_SILU0C || pk == RMX_PREC_MPNN) && !(c.coarse.fac && c.coarse.fac_prec == prec))
     fail(RMX_ERR_ARG, "solve: coarse space not factored in the requested precision");
   if (pk == RMX_PREC_MPNN && !mpnn_ready(c)) fail(RMX_ERR_ARG, "solve: no learned preconditioner loaded");
   if ((prec == RMX_F32 && !c.have32) || (prec == RMX_F64 && !c.have64))

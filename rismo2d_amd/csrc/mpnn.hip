@@ -1,15 +1,678 @@
-// Learned preconditioner (message-passing network on the mesh graph): placeholder until
-// the network kernels are in.
+// Learned preconditioner: a message-passing network on the mesh graph, evaluated and
+// trained on the device with this file's kernels.  No reference counterpart.
+//
+// Graph: mesh nodes; an edge joins two nodes of one element (both directions).  Node
+// features f (8): u, v, depth/4, bed/4, 100 cf, and the residual's u, v, h components
+// divided by the residual's RMS.
+//
+//     h0 = relu(f W_in + b_in)                  (FP32, stored BF16)
+//     6 x:  a = mean of h over the neighbours   (FP32 sums of BF16, stored BF16)
+//           p = [h, a] W_l + b_l                (BF16 MFMA 16x16x16, FP32 accumulate)
+//           h = h + relu(p)                     (stored BF16)
+//     out = h W_out + b_out                     (FP32): u, v, h corrections / RMS(r)
+//
+// As a preconditioner (RMX_PREC_MPNN) the network adds its correction to the two-level
+// preconditioner: z = M2^-1 r + RMS(r) * scatter(out), trained on pairs (r, x* - M2^-1 r)
+// with x* = A^-1 r from converged solves.  Training: mean squared error over the node
+// variables that carry an equation, gradients by the backward kernels below (weight
+// gradients as per-block partial sums reduced in a fixed order, so they are
+// deterministic), Adam on the FP32 master weights, BF16 copies refreshed after each step.
+#include <algorithm>
+#include <cmath>
+#include <cstring>
+
 #include "rmx_internal.h"
 
 namespace rmx {
 
-bool mpnn_ready(Ctx& c) { (void)c; return false; }
+typedef short s4 __attribute__((ext_vector_type(4)));
+typedef float f4 __attribute__((ext_vector_type(4)));
 
-void mpnn_apply(Ctx& c, const double* r, double* z)
+constexpr int MC = Mpnn::C, ML = Mpnn::L, MF = Mpnn::F, MO = Mpnn::O;
+constexpr int OFF_WIN = 0, OFF_BIN = MF * MC;
+constexpr int LAYER_SZ = 2 * MC * MC + MC;
+constexpr int OFF_L0 = OFF_BIN + MC;
+constexpr int OFF_WOUT = OFF_L0 + ML * LAYER_SZ, OFF_BOUT = OFF_WOUT + MC * MO;
+constexpr int NPARAM = OFF_BOUT + MO;
+static_assert(NPARAM == 50307, "parameter layout");
+constexpr int CHUNK = 1024;        // nodes per block of the small reductions
+constexpr int WCHUNK = 256;        // nodes per block of the layer weight gradients
+
+__device__ __forceinline__ uint16_t f2bf(float f)
 {
-  (void)c; (void)r; (void)z;
-  fail(RMX_ERR_ARG, "learned preconditioner not loaded");
+  uint32_t u = __float_as_uint(f);
+  u += 0x7FFFu + ((u >> 16) & 1u);
+  return (uint16_t)(u >> 16);
+}
+__device__ __forceinline__ float bf2f(uint16_t b) { return __uint_as_float((uint32_t)b << 16); }
+
+// ---------------------------------------------------------------------------------------
+// features, targets, outputs
+
+__global__ void k_feat(int np, const double* __restrict__ U, const double* __restrict__ V,
+                       const double* __restrict__ S, const double* __restrict__ z,
+                       const double* __restrict__ cf, const int* __restrict__ eqno,
+                       const double* __restrict__ r, const double* __restrict__ r2, int n,
+                       float* __restrict__ feat)
+{
+  const int i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= np) return;
+  const double rs = sqrt(r2[0] / n);
+  const double ri = rs > 0.0 ? 1.0 / rs : 0.0;
+  float* f = feat + (size_t)i * MF;
+  f[0] = (float)U[i];
+  f[1] = (float)V[i];
+  f[2] = (float)(fmax(S[i] - z[i], 0.0) * 0.25);
+  f[3] = (float)(z[i] * 0.25);
+  f[4] = (float)(cf[i] * 100.0);
+  for (int v = 0; v < 3; v++) {
+    const int e = eqno[3 * i + v];
+    f[5 + v] = e >= 0 ? (float)(r[e] * ri) : 0.0f;
+  }
+}
+
+__global__ void k_target(int np, const int* __restrict__ eqno, const double* __restrict__ t,
+                         const double* __restrict__ r2, int n, float* __restrict__ tgt,
+                         float* __restrict__ mask)
+{
+  const int i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= np) return;
+  const double rs = sqrt(r2[0] / n);
+  const double ri = rs > 0.0 ? 1.0 / rs : 0.0;
+  for (int v = 0; v < 3; v++) {
+    const int e = eqno[3 * i + v];
+    tgt[3 * i + v] = e >= 0 ? (float)(t[e] * ri) : 0.0f;
+    mask[3 * i + v] = e >= 0 ? 1.0f : 0.0f;
+  }
+}
+
+// z[e] (+)= RMS(r) * out
+__global__ void k_scatter_out(int np, const int* __restrict__ eqno, const float* __restrict__ out,
+                              const double* __restrict__ r2, int n, double* __restrict__ z, int add)
+{
+  const int i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= np) return;
+  const double rs = sqrt(r2[0] / n);
+  for (int v = 0; v < 3; v++) {
+    const int e = eqno[3 * i + v];
+    if (e >= 0) z[e] = (add ? z[e] : 0.0) + rs * (double)out[3 * i + v];
+  }
+}
+
+// ---------------------------------------------------------------------------------------
+// forward
+
+__global__ void k_in(int np, const float* __restrict__ feat, const float* __restrict__ th,
+                     uint16_t* __restrict__ H0, float* __restrict__ Q)
+{
+  const long long t = (long long)blockIdx.x * blockDim.x + threadIdx.x;
+  const int i = (int)(t / MC), c = (int)(t % MC);
+  if (i >= np) return;
+  float q = th[OFF_BIN + c];
+  for (int f = 0; f < MF; f++) q += feat[(size_t)i * MF + f] * th[OFF_WIN + f * MC + c];
+  if (Q) Q[(size_t)i * MC + c] = q;
+  H0[(size_t)i * MC + c] = f2bf(fmaxf(q, 0.0f));
+}
+
+// a_i = mean over neighbours of h (one wave per node, lane = channel)
+__global__ void __launch_bounds__(256) k_agg(int np, const int* __restrict__ ptr,
+                                             const int* __restrict__ adj,
+                                             const uint16_t* __restrict__ H,
+                                             uint16_t* __restrict__ A)
+{
+  const int i = (blockIdx.x * blockDim.x + threadIdx.x) >> 6;
+  const int c = threadIdx.x & 63;
+  if (i >= np) return;
+  const int a = ptr[i], b = ptr[i + 1];
+  float s = 0.0f;
+  for (int k = a; k < b; k++) s += bf2f(H[(size_t)adj[k] * MC + c]);
+  A[(size_t)i * MC + c] = f2bf(b > a ? s / (float)(b - a) : 0.0f);
+}
+
+// h_out = h + relu([h, a] W + b): 64 nodes per block, 16 per wave, 4 output tiles of 16
+// channels per wave, K = 128 in 8 MFMA steps.  Wt = W^T (BF16, [64][128]).
+__global__ void __launch_bounds__(256) k_layer_fwd(int np, const uint16_t* __restrict__ H,
+                                                   const uint16_t* __restrict__ A,
+                                                   const uint16_t* __restrict__ Wt,
+                                                   const float* __restrict__ bias,
+                                                   uint16_t* __restrict__ Hout,
+                                                   float* __restrict__ P)
+{
+  const int wave = threadIdx.x >> 6, lane = threadIdx.x & 63;
+  const int nb = blockIdx.x * 64 + wave * 16;
+  const int r16 = lane & 15, kq = (lane >> 4) * 4;
+  const int node = nb + r16;
+  f4 acc[4];
+#pragma unroll
+  for (int t = 0; t < 4; t++) acc[t] = f4{0.0f, 0.0f, 0.0f, 0.0f};
+#pragma unroll
+  for (int ks = 0; ks < 8; ks++) {
+    const int k0 = ks * 16 + kq;
+    s4 a = s4{0, 0, 0, 0};
+    if (node < np)
+      a = *reinterpret_cast<const s4*>(k0 < MC ? H + (size_t)node * MC + k0
+                                               : A + (size_t)node * MC + (k0 - MC));
+#pragma unroll
+    for (int t = 0; t < 4; t++) {
+      const s4 b = *reinterpret_cast<const s4*>(Wt + (size_t)(t * 16 + r16) * (2 * MC) + k0);
+      acc[t] = __builtin_amdgcn_mfma_f32_16x16x16bf16_1k(a, b, acc[t], 0, 0, 0);
+    }
+  }
+#pragma unroll
+  for (int t = 0; t < 4; t++)
+#pragma unroll
+    for (int j = 0; j < 4; j++) {
+      const int nd = nb + (lane >> 4) * 4 + j;
+      const int col = t * 16 + r16;
+      if (nd < np) {
+        const float p = acc[t][j] + bias[col];
+        const size_t o = (size_t)nd * MC + col;
+        Hout[o] = f2bf(bf2f(H[o]) + fmaxf(p, 0.0f));
+        if (P) P[o] = p;
+      }
+    }
+}
+
+// out = h W_out + b_out (FP32), one thread per node
+__global__ void k_out(int np, const uint16_t* __restrict__ H, const float* __restrict__ th,
+                      float* __restrict__ out)
+{
+  const int i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= np) return;
+  float o[MO];
+  for (int v = 0; v < MO; v++) o[v] = th[OFF_BOUT + v];
+  for (int c = 0; c < MC; c++) {
+    const float h = bf2f(H[(size_t)i * MC + c]);
+    for (int v = 0; v < MO; v++) o[v] += h * th[OFF_WOUT + c * MO + v];
+  }
+  for (int v = 0; v < MO; v++) out[(size_t)i * MO + v] = o[v];
+}
+
+// ---------------------------------------------------------------------------------------
+// backward
+
+// gout = 2 mask (out - t) / cnt; per-block partial sums of mask (out - t)^2
+__global__ void __launch_bounds__(256) k_gout(int np, const float* __restrict__ out,
+                                              const float* __restrict__ tgt,
+                                              const float* __restrict__ mask, float inv_cnt,
+                                              float* __restrict__ gout, double* __restrict__ lpart)
+{
+  __shared__ double sh[256];
+  const int i = blockIdx.x * blockDim.x + threadIdx.x;
+  double l = 0.0;
+  if (i < np)
+    for (int v = 0; v < MO; v++) {
+      const size_t o = (size_t)i * MO + v;
+      const float d = (out[o] - tgt[o]) * mask[o];
+      gout[o] = 2.0f * d * inv_cnt;
+      l += (double)d * d;
+    }
+  sh[threadIdx.x] = l;
+  __syncthreads();
+  for (int s = 128; s > 0; s >>= 1) {
+    if ((int)threadIdx.x < s) sh[threadIdx.x] += sh[threadIdx.x + s];
+    __syncthreads();
+  }
+  if (threadIdx.x == 0) lpart[blockIdx.x] = sh[0];
+}
+
+// gh = gout W_out^T
+__global__ void k_gh_out(int np, const float* __restrict__ gout, const float* __restrict__ th,
+                         float* __restrict__ gh)
+{
+  const long long t = (long long)blockIdx.x * blockDim.x + threadIdx.x;
+  const int i = (int)(t / MC), c = (int)(t % MC);
+  if (i >= np) return;
+  float s = 0.0f;
+  for (int v = 0; v < MO; v++) s += gout[(size_t)i * MO + v] * th[OFF_WOUT + c * MO + v];
+  gh[(size_t)i * MC + c] = s;
+}
+
+// gp = bf16(gh * (p > 0))
+__global__ void k_gp(long long m, const float* __restrict__ gh, const float* __restrict__ P,
+                     uint16_t* __restrict__ gp)
+{
+  for (long long t = (long long)blockIdx.x * blockDim.x + threadIdx.x; t < m;
+       t += (long long)gridDim.x * blockDim.x)
+    gp[t] = f2bf(P[t] > 0.0f ? gh[t] : 0.0f);
+}
+
+// gq = gh * (q > 0), in place
+__global__ void k_gq(long long m, float* __restrict__ gh, const float* __restrict__ Q)
+{
+  for (long long t = (long long)blockIdx.x * blockDim.x + threadIdx.x; t < m;
+       t += (long long)gridDim.x * blockDim.x)
+    if (!(Q[t] > 0.0f)) gh[t] = 0.0f;
+}
+
+// gx = gp W^T ([np x 64] x [64 x 128]): gh += gx[:, :64] (in place), ga = gx[:, 64:].
+// Wb = W (BF16, [128][64]).
+__global__ void __launch_bounds__(256) k_layer_bwd(int np, const uint16_t* __restrict__ gp,
+                                                   const uint16_t* __restrict__ Wb,
+                                                   float* __restrict__ gh,
+                                                   float* __restrict__ ga)
+{
+  const int wave = threadIdx.x >> 6, lane = threadIdx.x & 63;
+  const int nb = blockIdx.x * 64 + wave * 16;
+  const int r16 = lane & 15, kq = (lane >> 4) * 4;
+  const int node = nb + r16;
+  f4 acc[8];
+#pragma unroll
+  for (int t = 0; t < 8; t++) acc[t] = f4{0.0f, 0.0f, 0.0f, 0.0f};
+#pragma unroll
+  for (int ks = 0; ks < 4; ks++) {
+    const int k0 = ks * 16 + kq;
+    s4 a = s4{0, 0, 0, 0};
+    if (node < np) a = *reinterpret_cast<const s4*>(gp + (size_t)node * MC + k0);
+#pragma unroll
+    for (int t = 0; t < 8; t++) {
+      const s4 b = *reinterpret_cast<const s4*>(Wb + (size_t)(t * 16 + r16) * MC + k0);
+      acc[t] = __builtin_amdgcn_mfma_f32_16x16x16bf16_1k(a, b, acc[t], 0, 0, 0);
+    }
+  }
+#pragma unroll
+  for (int t = 0; t < 8; t++)
+#pragma unroll
+    for (int j = 0; j < 4; j++) {
+      const int nd = nb + (lane >> 4) * 4 + j;
+      const int col = t * 16 + r16;
+      if (nd < np) {
+        if (col < MC) gh[(size_t)nd * MC + col] += acc[t][j];
+        else ga[(size_t)nd * MC + (col - MC)] = acc[t][j];
+      }
+    }
+}
+
+// gh_j += sum over neighbours i of ga_i / deg_i (the transpose of the mean aggregation;
+// the graph is symmetric)
+__global__ void __launch_bounds__(256) k_aggT(int np, const int* __restrict__ ptr,
+                                              const int* __restrict__ adj,
+                                              const float* __restrict__ ga,
+                                              float* __restrict__ gh)
+{
+  const int j = (blockIdx.x * blockDim.x + threadIdx.x) >> 6;
+  const int c = threadIdx.x & 63;
+  if (j >= np) return;
+  float s = 0.0f;
+  for (int k = ptr[j]; k < ptr[j + 1]; k++) {
+    const int i = adj[k];
+    const int d = ptr[i + 1] - ptr[i];
+    s += ga[(size_t)i * MC + c] / (float)d;
+  }
+  gh[(size_t)j * MC + c] += s;
+}
+
+// partial[blk][m][n] = sum over the block's nodes of [h, a][node][m] * gp[node][n]
+// (MFMA, K = nodes): 32 output tiles, 8 per wave.
+__global__ void __launch_bounds__(256) k_wgrad(int np, const uint16_t* __restrict__ H,
+                                               const uint16_t* __restrict__ A,
+                                               const uint16_t* __restrict__ gp,
+                                               float* __restrict__ partial)
+{
+  const int wave = threadIdx.x >> 6, lane = threadIdx.x & 63;
+  const int r16 = lane & 15, kq = (lane >> 4) * 4;
+  const int n0 = blockIdx.x * WCHUNK;
+  f4 acc[2][4];
+#pragma unroll
+  for (int u = 0; u < 2; u++)
+#pragma unroll
+    for (int t = 0; t < 4; t++) acc[u][t] = f4{0.0f, 0.0f, 0.0f, 0.0f};
+  for (int kk = 0; kk < WCHUNK; kk += 16) {
+    s4 a[2], b[4];
+#pragma unroll
+    for (int u = 0; u < 2; u++) {
+      const int m = (wave * 2 + u) * 16 + r16;
+      const uint16_t* X = m < MC ? H : A;
+      const int mc = m < MC ? m : m - MC;
+#pragma unroll
+      for (int q = 0; q < 4; q++) {
+        const int nd = n0 + kk + kq + q;
+        a[u][q] = nd < np ? (short)X[(size_t)nd * MC + mc] : (short)0;
+      }
+    }
+#pragma unroll
+    for (int t = 0; t < 4; t++) {
+      const int n = t * 16 + r16;
+#pragma unroll
+      for (int q = 0; q < 4; q++) {
+        const int nd = n0 + kk + kq + q;
+        b[t][q] = nd < np ? (short)gp[(size_t)nd * MC + n] : (short)0;
+      }
+    }
+#pragma unroll
+    for (int u = 0; u < 2; u++)
+#pragma unroll
+      for (int t = 0; t < 4; t++)
+        acc[u][t] = __builtin_amdgcn_mfma_f32_16x16x16bf16_1k(a[u], b[t], acc[u][t], 0, 0, 0);
+  }
+  float* out = partial + (size_t)blockIdx.x * (2 * MC * MC);
+#pragma unroll
+  for (int u = 0; u < 2; u++)
+#pragma unroll
+    for (int t = 0; t < 4; t++)
+#pragma unroll
+      for (int j = 0; j < 4; j++) {
+        const int m = (wave * 2 + u) * 16 + (lane >> 4) * 4 + j;
+        const int n = t * 16 + r16;
+        out[m * MC + n] = acc[u][t][j];
+      }
+}
+
+// Small outer-product reductions (input / output layers, biases):
+// partial[blk][m][o] = sum over the block's nodes of x(node, m) * g(node, o), with
+// x(node, mx) = 1 (the bias row).  XT / GT: float or BF16 (uint16_t).
+template <class T> __device__ __forceinline__ float ld(const T* p, size_t i);
+template <> __device__ __forceinline__ float ld<float>(const float* p, size_t i) { return p[i]; }
+template <> __device__ __forceinline__ float ld<uint16_t>(const uint16_t* p, size_t i) { return bf2f(p[i]); }
+
+template <class XT, class GT>
+__global__ void k_outer(int np, const XT* __restrict__ X, int mx, const GT* __restrict__ G, int ng,
+                        float* __restrict__ partial)
+{
+  const int pairs = (mx + 1) * ng;
+  const int t = threadIdx.x;
+  const int n0 = blockIdx.x * CHUNK, n1 = min(np, n0 + CHUNK);
+  for (int pr = t; pr < pairs; pr += blockDim.x) {
+    const int m = pr / ng, o = pr % ng;
+    float s = 0.0f;
+    if (m < mx)
+      for (int i = n0; i < n1; i++) s += ld<XT>(X, (size_t)i * mx + m) * ld<GT>(G, (size_t)i * ng + o);
+    else
+      for (int i = n0; i < n1; i++) s += ld<GT>(G, (size_t)i * ng + o);
+    partial[(size_t)blockIdx.x * pairs + pr] = s;
+  }
+}
+
+// dst[map(p)] = sum over blocks of partial[b][p] (fixed order); map: p = m*ng + o with
+// m < mx -> offW + m*ng + o, m == mx -> offB + o
+__global__ void k_reduce(int nblk, int pairs, const float* __restrict__ partial, int mx, int ng,
+                         int offW, int offB, float* __restrict__ grad)
+{
+  const int p = blockIdx.x * blockDim.x + threadIdx.x;
+  if (p >= pairs) return;
+  float s = 0.0f;
+  for (int b = 0; b < nblk; b++) s += partial[(size_t)b * pairs + p];
+  const int m = p / ng, o = p % ng;
+  grad[m < mx ? offW + m * ng + o : offB + o] = s;
+}
+
+__global__ void k_adam(int n, float* __restrict__ th, const float* __restrict__ g,
+                       float* __restrict__ m, float* __restrict__ v, float lr, float b1, float b2,
+                       float c1, float c2, float eps)
+{
+  const int i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= n) return;
+  const float gi = g[i];
+  m[i] = b1 * m[i] + (1.0f - b1) * gi;
+  v[i] = b2 * v[i] + (1.0f - b2) * gi * gi;
+  th[i] -= lr * (m[i] / c1) / (sqrtf(v[i] / c2) + eps);
+}
+
+// BF16 copies of the layer weights: wb = W ([128][64]), wt = W^T ([64][128])
+__global__ void k_refresh(const float* __restrict__ th, uint16_t* __restrict__ wt,
+                          uint16_t* __restrict__ wb)
+{
+  const int t = blockIdx.x * blockDim.x + threadIdx.x;
+  if (t >= ML * 2 * MC * MC) return;
+  const int l = t / (2 * MC * MC), r = t % (2 * MC * MC);
+  const int m = r / MC, c = r % MC;
+  const uint16_t w = f2bf(th[OFF_L0 + l * LAYER_SZ + m * MC + c]);
+  wb[(size_t)l * 2 * MC * MC + m * MC + c] = w;
+  wt[(size_t)l * 2 * MC * MC + c * 2 * MC + m] = w;
+}
+
+// ---------------------------------------------------------------------------------------
+// host side
+
+static int gb(long long n, int b = 256) { return (int)((n + b - 1) / b); }
+
+// node graph: nodes sharing an element, sorted, no self loops
+static void build_graph(Ctx& c)
+{
+  Mpnn& M = c.mpnn;
+  const HostModel& hm = c.hm;
+  const int np = hm.np;
+  std::vector<int> eptr(np + 1, 0), elist;
+  for (int e = 0; e < hm.ne; e++)
+    for (int i = 0; i < hm.nnd[e]; i++) eptr[hm.elem[8LL * e + i] + 1]++;
+  for (int i = 0; i < np; i++) eptr[i + 1] += eptr[i];
+  elist.resize(eptr[np]);
+  {
+    std::vector<int> fill(eptr.begin(), eptr.end() - 1);
+    for (int e = 0; e < hm.ne; e++)
+      for (int i = 0; i < hm.nnd[e]; i++) elist[fill[hm.elem[8LL * e + i]]++] = e;
+  }
+  std::vector<std::vector<int>> nb(np);
+  parallel_for(np, [&](long long a, long long b) {
+    for (long long v = a; v < b; v++) {
+      std::vector<int>& L = nb[v];
+      for (int k = eptr[v]; k < eptr[v + 1]; k++) {
+        const int e = elist[k];
+        for (int i = 0; i < hm.nnd[e]; i++) {
+          const int w = hm.elem[8LL * e + i];
+          if (w != v) L.push_back(w);
+        }
+      }
+      std::sort(L.begin(), L.end());
+      L.erase(std::unique(L.begin(), L.end()), L.end());
+    }
+  });
+  std::vector<int> ptr(np + 1, 0), adj;
+  for (int v = 0; v < np; v++) ptr[v + 1] = ptr[v] + (int)nb[v].size();
+  adj.resize(ptr[np]);
+  for (int v = 0; v < np; v++) std::copy(nb[v].begin(), nb[v].end(), adj.begin() + ptr[v]);
+  M.adj_ptr.upload(ptr, c.stream);
+  M.adj.upload(adj, c.stream);
+  M.np = np;
+  const size_t nh = (size_t)np * MC;
+  M.H.alloc((ML + 1) * nh);
+  M.A.alloc(ML * nh);
+  M.feat.alloc((size_t)np * MF);
+  M.out.alloc((size_t)np * MO);
+  M.graph = true;
+}
+
+static void need_graph(Ctx& c)
+{
+  if (!c.prepared || c.csr_only) fail(RMX_ERR_ARG, "learned preconditioner: needs a prepared mesh");
+  if (!c.mpnn.graph || c.mpnn.np != c.hm.np) build_graph(c);
+}
+
+// features of the current state and residual r (device, length n) into M.feat; the
+// squared norm of r goes to M.r2
+static void features(Ctx& c, const double* r)
+{
+  Mpnn& M = c.mpnn;
+  M.r2.alloc(1);
+  const double* xs[1] = { r };
+  dev_dots(c, 1, xs, xs, M.r2.p);
+  k_feat<<<gb(M.np), 256, 0, c.stream>>>(M.np, c.U.p, c.V.p, c.S.p, c.z.p, c.cf.p, c.eqno.p, r,
+                                         M.r2.p, c.n, M.feat.p);
+}
+
+static void forward(Ctx& c, const float* feat, bool train)
+{
+  Mpnn& M = c.mpnn;
+  const int np = M.np;
+  const size_t nh = (size_t)np * MC;
+  if (train) { M.P.alloc(ML * nh); M.Q.alloc(nh); }
+  k_in<<<gb((long long)np * MC), 256, 0, c.stream>>>(np, feat, M.theta.p, M.H.p, train ? M.Q.p : nullptr);
+  for (int l = 0; l < ML; l++) {
+    const uint16_t* h = M.H.p + l * nh;
+    uint16_t* a = M.A.p + l * nh;
+    k_agg<<<gb((long long)np * 64), 256, 0, c.stream>>>(np, M.adj_ptr.p, M.adj.p, h, a);
+    k_layer_fwd<<<gb(np, 64), 256, 0, c.stream>>>(np, h, a, M.wt.p + (size_t)l * 2 * MC * MC,
+                                                  M.theta.p + OFF_L0 + l * LAYER_SZ + 2 * MC * MC,
+                                                  M.H.p + (l + 1) * nh, train ? M.P.p + l * nh : nullptr);
+  }
+  k_out<<<gb(np), 256, 0, c.stream>>>(np, M.H.p + ML * nh, M.theta.p, M.out.p);
+  HIPCHECK(hipGetLastError());
+}
+
+// loss and gradient (into M.grad) of one stored sample
+static double backward(Ctx& c, int s)
+{
+  Mpnn& M = c.mpnn;
+  const int np = M.np;
+  const size_t nh = (size_t)np * MC;
+  const float* feat = M.s_feat.p + (size_t)s * np * MF;
+  const float* tgt = M.s_tgt.p + (size_t)s * np * MO;
+  const float* mask = M.s_mask.p + (size_t)s * np * MO;
+  forward(c, feat, true);
+  M.gh.alloc(nh); M.ga.alloc(nh); M.gp.alloc(nh); M.gout.alloc((size_t)np * MO);
+  const int nbl = gb(np);
+  M.lpart.alloc(nbl);
+  const float inv_cnt = 1.0f / (float)std::max(1.0, M.s_cnt[s]);
+  k_gout<<<nbl, 256, 0, c.stream>>>(np, M.out.p, tgt, mask, inv_cnt, M.gout.p, M.lpart.p);
+  const int nchunk = gb(np, CHUNK), nw = gb(np, WCHUNK);
+  M.partial.alloc(std::max<size_t>((size_t)nchunk * (MF + 1) * MC, (size_t)nw * 2 * MC * MC));
+  // output layer
+  k_outer<uint16_t, float><<<nchunk, 256, 0, c.stream>>>(np, M.H.p + ML * nh, MC, M.gout.p, MO, M.partial.p);
+  k_reduce<<<gb((MC + 1) * MO), 256, 0, c.stream>>>(nchunk, (MC + 1) * MO, M.partial.p, MC, MO,
+                                                    OFF_WOUT, OFF_BOUT, M.grad.p);
+  k_gh_out<<<gb((long long)np * MC), 256, 0, c.stream>>>(np, M.gout.p, M.theta.p, M.gh.p);
+  for (int l = ML - 1; l >= 0; l--) {
+    k_gp<<<2048, 256, 0, c.stream>>>((long long)nh, M.gh.p, M.P.p + l * nh, M.gp.p);
+    const int off = OFF_L0 + l * LAYER_SZ;
+    k_wgrad<<<nw, 256, 0, c.stream>>>(np, M.H.p + l * nh, M.A.p + l * nh, M.gp.p, M.partial.p);
+    k_reduce<<<gb(2 * MC * MC), 256, 0, c.stream>>>(nw, 2 * MC * MC, M.partial.p, 2 * MC * MC, 1,
+                                                    off, off, M.grad.p);
+    k_outer<float, uint16_t><<<nchunk, 256, 0, c.stream>>>(np, nullptr, 0, M.gp.p, MC, M.partial.p);
+    k_reduce<<<1, 256, 0, c.stream>>>(nchunk, MC, M.partial.p, 0, MC, off, off + 2 * MC * MC, M.grad.p);
+    k_layer_bwd<<<gb(np, 64), 256, 0, c.stream>>>(np, M.gp.p, M.wb.p + (size_t)l * 2 * MC * MC,
+                                                  M.gh.p, M.ga.p);
+    k_aggT<<<gb((long long)np * 64), 256, 0, c.stream>>>(np, M.adj_ptr.p, M.adj.p, M.ga.p, M.gh.p);
+  }
+  k_gq<<<2048, 256, 0, c.stream>>>((long long)nh, M.gh.p, M.Q.p);
+  k_outer<float, float><<<nchunk, 256, 0, c.stream>>>(np, feat, MF, M.gh.p, MC, M.partial.p);
+  k_reduce<<<gb((MF + 1) * MC), 256, 0, c.stream>>>(nchunk, (MF + 1) * MC, M.partial.p, MF, MC,
+                                                    OFF_WIN, OFF_BIN, M.grad.p);
+  HIPCHECK(hipGetLastError());
+  std::vector<double> lp(nbl);
+  HIPCHECK(hipMemcpyAsync(lp.data(), M.lpart.p, sizeof(double) * nbl, hipMemcpyDeviceToHost, c.stream));
+  HIPCHECK(hipStreamSynchronize(c.stream));
+  double loss = 0.0;
+  for (double v : lp) loss += v;
+  return loss * inv_cnt;
+}
+
+bool mpnn_ready(Ctx& c) { return c.mpnn.weights && c.mpnn.graph && c.mpnn.np == c.hm.np; }
+
+void mpnn_set_weights(Ctx& c, const float* th, int n)
+{
+  if (n != NPARAM) fail(RMX_ERR_ARG, "learned preconditioner: expected " + std::to_string(NPARAM) + " weights");
+  need_graph(c);
+  Mpnn& M = c.mpnn;
+  M.theta.upload(th, NPARAM, c.stream);
+  M.grad.alloc(NPARAM); M.m.alloc(NPARAM); M.v.alloc(NPARAM);
+  HIPCHECK(hipMemsetAsync(M.m.p, 0, sizeof(float) * NPARAM, c.stream));
+  HIPCHECK(hipMemsetAsync(M.v.p, 0, sizeof(float) * NPARAM, c.stream));
+  M.wt.alloc((size_t)ML * 2 * MC * MC); M.wb.alloc((size_t)ML * 2 * MC * MC);
+  k_refresh<<<gb(ML * 2 * MC * MC), 256, 0, c.stream>>>(M.theta.p, M.wt.p, M.wb.p);
+  HIPCHECK(hipStreamSynchronize(c.stream));
+  M.step = 0;
+  M.weights = true;
+}
+
+void mpnn_get_weights(Ctx& c, float* th, int n)
+{
+  if (!c.mpnn.weights) fail(RMX_ERR_ARG, "learned preconditioner: no weights");
+  if (n != NPARAM) fail(RMX_ERR_ARG, "learned preconditioner: expected " + std::to_string(NPARAM) + " weights");
+  c.mpnn.theta.download(th, NPARAM, c.stream);
+  HIPCHECK(hipStreamSynchronize(c.stream));
+}
+
+// z = M2^-1 r + RMS(r) * network(r)   (device pointers)
+void mpnn_apply(Ctx& c, int prec, const double* r, double* z)
+{
+  if (!mpnn_ready(c)) fail(RMX_ERR_ARG, "learned preconditioner not loaded");
+  coarse_apply(c, prec, r, z);
+  features(c, r);
+  forward(c, c.mpnn.feat.p, false);
+  k_scatter_out<<<gb(c.mpnn.np), 256, 0, c.stream>>>(c.mpnn.np, c.eqno.p, c.mpnn.out.p, c.mpnn.r2.p,
+                                                     c.n, z, 1);
+  HIPCHECK(hipGetLastError());
+}
+
+// network correction alone: z = RMS(r) * network(r); feat (host, optional) gets the features
+void mpnn_forward(Ctx& c, const double* r, double* z, float* feat_out)
+{
+  if (!mpnn_ready(c)) fail(RMX_ERR_ARG, "learned preconditioner not loaded");
+  features(c, r);
+  forward(c, c.mpnn.feat.p, false);
+  k_scatter_out<<<gb(c.mpnn.np), 256, 0, c.stream>>>(c.mpnn.np, c.eqno.p, c.mpnn.out.p, c.mpnn.r2.p,
+                                                     c.n, z, 0);
+  HIPCHECK(hipGetLastError());
+  if (feat_out) c.mpnn.feat.download(feat_out, (size_t)c.mpnn.np * MF, c.stream);
+  HIPCHECK(hipStreamSynchronize(c.stream));
+}
+
+// store a training sample: features of (state, r), target t / RMS(r)   (device pointers)
+int mpnn_add_sample(Ctx& c, const double* r, const double* t, int capacity)
+{
+  need_graph(c);
+  Mpnn& M = c.mpnn;
+  const int np = M.np;
+  if (capacity > 0 && capacity != M.cap) {
+    M.cap = capacity;
+    M.nsamp = 0;
+    M.s_cnt.clear();
+    M.s_feat.alloc((size_t)capacity * np * MF);
+    M.s_tgt.alloc((size_t)capacity * np * MO);
+    M.s_mask.alloc((size_t)capacity * np * MO);
+  }
+  if (M.nsamp >= M.cap) fail(RMX_ERR_RANGE, "learned preconditioner: training set full");
+  const int s = M.nsamp;
+  features(c, r);
+  HIPCHECK(hipMemcpyAsync(M.s_feat.p + (size_t)s * np * MF, M.feat.p, sizeof(float) * np * MF,
+                          hipMemcpyDeviceToDevice, c.stream));
+  k_target<<<gb(np), 256, 0, c.stream>>>(np, c.eqno.p, t, M.r2.p, c.n, M.s_tgt.p + (size_t)s * np * MO,
+                                         M.s_mask.p + (size_t)s * np * MO);
+  HIPCHECK(hipGetLastError());
+  HIPCHECK(hipStreamSynchronize(c.stream));
+  int cnt = 0;
+  for (size_t k = 0; k < c.hm.eqno.size(); k++) cnt += c.hm.eqno[k] >= 0;
+  M.s_cnt.push_back((double)cnt);
+  return ++M.nsamp;
+}
+
+double mpnn_loss_grad(Ctx& c, int s, float* grad)
+{
+  Mpnn& M = c.mpnn;
+  if (!M.weights) fail(RMX_ERR_ARG, "learned preconditioner: no weights");
+  if (s < 0 || s >= M.nsamp) fail(RMX_ERR_ARG, "learned preconditioner: no such sample");
+  const double loss = backward(c, s);
+  if (grad) {
+    M.grad.download(grad, NPARAM, c.stream);
+    HIPCHECK(hipStreamSynchronize(c.stream));
+  }
+  return loss;
+}
+
+// Adam over the stored samples, one sample per step in order, `epochs` passes; returns
+// the mean loss of the last pass
+double mpnn_train(Ctx& c, int epochs, double lr)
+{
+  Mpnn& M = c.mpnn;
+  if (!M.weights) fail(RMX_ERR_ARG, "learned preconditioner: no weights");
+  if (M.nsamp == 0) fail(RMX_ERR_ARG, "learned preconditioner: no training samples");
+  const float b1 = 0.9f, b2 = 0.999f, eps = 1e-8f;
+  double last = 0.0;
+  for (int ep = 0; ep < epochs; ep++) {
+    double sum = 0.0;
+    for (int s = 0; s < M.nsamp; s++) {
+      sum += backward(c, s);
+      M.step++;
+      const float c1 = 1.0f - std::pow(b1, (float)M.step), c2 = 1.0f - std::pow(b2, (float)M.step);
+      k_adam<<<gb(NPARAM), 256, 0, c.stream>>>(NPARAM, M.theta.p, M.grad.p, M.m.p, M.v.p, (float)lr,
+                                               b1, b2, c1, c2, eps);
+      k_refresh<<<gb(ML * 2 * MC * MC), 256, 0, c.stream>>>(M.theta.p, M.wt.p, M.wb.p);
+    }
+    last = sum / M.nsamp;
+  }
+  HIPCHECK(hipGetLastError());
+  HIPCHECK(hipStreamSynchronize(c.stream));
+  return last;
 }
 
 }  // namespace rmx

@@ -149,6 +149,20 @@ struct Coarse {
   ~Coarse() { if (h_buf) (void)hipHostFree(h_buf); }
 };
 
+// Learned preconditioner: mesh node graph, weights (FP32 master + BF16 layer copies),
+// activations, gradients, Adam moments, stored training samples.
+struct Mpnn {
+  static constexpr int C = 64, L = 6, F = 8, O = 3;
+  bool graph = false, weights = false;
+  int np = 0, step = 0, cap = 0, nsamp = 0;
+  std::vector<double> s_cnt;                  // equations per stored sample
+  DBuf<int> adj_ptr, adj;
+  DBuf<float> theta, grad, m, v;
+  DBuf<uint16_t> wt, wb, H, A, gp;
+  DBuf<float> P, Q, feat, out, gh, ga, gout, partial, s_feat, s_tgt, s_mask;
+  DBuf<double> r2, lpart;
+};
+
 struct Ctx {
   int device = 0;
   hipStream_t stream = nullptr;
@@ -182,11 +196,13 @@ struct Ctx {
   DBuf<double> A64, B, X;
   DBuf<float> A32;
   bool have64 = false, have32 = false;
+  int spmv_g = 8;                             // SpMV lanes per row
   Ilu ilu;                                    // reference-order ILU(0)
   Ilu silu;                                   // strip ILU(0)
   int strips = 0;                             // 0: one strip per compute unit
   int segments = 16;                          // cross-strip cuts of the strip sweeps
   Coarse coarse;
+  Mpnn mpnn;
   int coarse_segments = 4;                    // cross-strip cuts of the coarse aggregates
   // Krylov work vectors
   DBuf<double> w[10];
@@ -237,5 +253,14 @@ void coarse_setup(Ctx& c, const std::vector<int>& rp, const std::vector<int>& cl
 void coarse_factor(Ctx& c, int prec);
 void coarse_apply(Ctx& c, int prec, const double* r, double* z);   // device pointers
 void dev_precond(Ctx& c, int prec, int kind, const double* in, double* out);
+void choose_spmv_lanes(Ctx& c);
+bool mpnn_ready(Ctx& c);
+void mpnn_apply(Ctx& c, int prec, const double* r, double* z);
+void mpnn_forward(Ctx& c, const double* r, double* z, float* feat_out);
+void mpnn_set_weights(Ctx& c, const float* th, int n);
+void mpnn_get_weights(Ctx& c, float* th, int n);
+int mpnn_add_sample(Ctx& c, const double* r, const double* t, int capacity);
+double mpnn_loss_grad(Ctx& c, int s, float* grad);
+double mpnn_train(Ctx& c, int epochs, double lr);
 
 }  // namespace rmx

@@ -15,13 +15,16 @@ namespace rmx {
 constexpr int RED_BLOCKS = 512;      // partial sums per reduction (fixed: deterministic)
 
 // ---------------------------------------------------------------------------------------
-// SpMV: G lanes per row, shuffle reduction, FP64 accumulation
+// SpMV: G lanes per row (G chosen from the mean row width so a row's values and columns
+// are read in one or two coalesced passes), shuffle reduction, FP64 accumulation.
+// RESID: r = b - A x instead of y = A x.
 
-template <class VT, int G>
+template <class VT, int G, bool RESID>
 __global__ void __launch_bounds__(256) k_spmv(int n, const int* __restrict__ rowptr,
                                               const int* __restrict__ cols,
                                               const VT* __restrict__ A,
                                               const double* __restrict__ x,
+                                              const double* __restrict__ b,
                                               double* __restrict__ y)
 {
   const long long t = (long long)blockIdx.x * blockDim.x + threadIdx.x;
@@ -29,62 +32,71 @@ __global__ void __launch_bounds__(256) k_spmv(int n, const int* __restrict__ row
   const int lane = (int)(t % G);
   if (row >= n) return;
   const int p0 = rowptr[row], p1 = rowptr[row + 1];
+  // four entries per lane per pass, all loads issued before the gathers of x
   double s = 0.0;
-  for (int p = p0 + lane; p < p1; p += G) s += (double)A[p] * x[cols[p]];
+  for (int p = p0 + lane; p < p1; p += 4 * G) {
+    VT a[4];
+    int cc[4];
+#pragma unroll
+    for (int u = 0; u < 4; u++) {
+      const int q = p + u * G;
+      a[u] = q < p1 ? A[q] : (VT)0;
+      cc[u] = q < p1 ? cols[q] : -1;
+    }
+    double xv[4];
+#pragma unroll
+    for (int u = 0; u < 4; u++) xv[u] = cc[u] >= 0 ? x[cc[u]] : 0.0;
+#pragma unroll
+    for (int u = 0; u < 4; u++) s += (double)a[u] * xv[u];
+  }
 #pragma unroll
   for (int o = G / 2; o > 0; o >>= 1) s += __shfl_xor(s, o, G);
-  if (lane == 0) y[row] = s;
+  if (lane == 0) y[row] = RESID ? b[row] - s : s;
 }
 
-// r = b - A x (FP64 true residual)
-template <class VT, int G>
-__global__ void __launch_bounds__(256) k_resid(int n, const int* __restrict__ rowptr,
-                                               const int* __restrict__ cols,
-                                               const VT* __restrict__ A,
-                                               const double* __restrict__ x,
-                                               const double* __restrict__ b,
-                                               double* __restrict__ r)
+template <class VT, bool RESID>
+static void launch_spmv(Ctx& c, const VT* A, const double* x, const double* b, double* y)
 {
-  const long long t = (long long)blockIdx.x * blockDim.x + threadIdx.x;
-  const int row = (int)(t / G);
-  const int lane = (int)(t % G);
-  if (row >= n) return;
-  const int p0 = rowptr[row], p1 = rowptr[row + 1];
-  double s = 0.0;
-  for (int p = p0 + lane; p < p1; p += G) s += (double)A[p] * x[cols[p]];
-#pragma unroll
-  for (int o = G / 2; o > 0; o >>= 1) s += __shfl_xor(s, o, G);
-  if (lane == 0) r[row] = b[row] - s;
+  const int G = c.spmv_g;
+  const int blocks = (int)(((long long)c.n * G + 255) / 256);
+  switch (G) {
+    case 4: k_spmv<VT, 4, RESID><<<blocks, 256, 0, c.stream>>>(c.n, c.rowptr.p, c.cols.p, A, x, b, y); break;
+    case 8: k_spmv<VT, 8, RESID><<<blocks, 256, 0, c.stream>>>(c.n, c.rowptr.p, c.cols.p, A, x, b, y); break;
+    case 16: k_spmv<VT, 16, RESID><<<blocks, 256, 0, c.stream>>>(c.n, c.rowptr.p, c.cols.p, A, x, b, y); break;
+    default: k_spmv<VT, 32, RESID><<<blocks, 256, 0, c.stream>>>(c.n, c.rowptr.p, c.cols.p, A, x, b, y); break;
+  }
+  HIPCHECK(hipGetLastError());
+}
+
+// lanes per row: the power of two nearest a quarter of the mean row width, 4..32 (each
+// lane then holds about four entries of its row)
+void choose_spmv_lanes(Ctx& c)
+{
+  const double w = c.n > 0 ? (double)c.nnz / c.n : 1.0;
+  int g = 4;
+  while (g < 32 && 4.0 * g * 1.41421356 < w) g *= 2;
+  if (const char* e = getenv("RMX_SPMV_G")) {
+    const int v = atoi(e);
+    if (v == 4 || v == 8 || v == 16 || v == 32) g = v;
+  }
+  c.spmv_g = g;
 }
 
 void dev_spmv(Ctx& c, int prec, const double* x, double* y)
 {
-  constexpr int G = 8;
-  const int blocks = (int)(((long long)c.n * G + 255) / 256);
-  if (prec == RMX_F32)
-    k_spmv<float, G><<<blocks, 256, 0, c.stream>>>(c.n, c.rowptr.p, c.cols.p, c.A32.p, x, y);
-  else
-    k_spmv<double, G><<<blocks, 256, 0, c.stream>>>(c.n, c.rowptr.p, c.cols.p, c.A64.p, x, y);
-  HIPCHECK(hipGetLastError());
+  if (prec == RMX_F32) launch_spmv<float, false>(c, c.A32.p, x, nullptr, y);
+  else launch_spmv<double, false>(c, c.A64.p, x, nullptr, y);
 }
 
 void dev_resid_p(Ctx& c, int prec, const double* x, const double* b, double* r)
 {
-  constexpr int G = 8;
-  const int blocks = (int)(((long long)c.n * G + 255) / 256);
-  if (prec == RMX_F32)
-    k_resid<float, G><<<blocks, 256, 0, c.stream>>>(c.n, c.rowptr.p, c.cols.p, c.A32.p, x, b, r);
-  else
-    k_resid<double, G><<<blocks, 256, 0, c.stream>>>(c.n, c.rowptr.p, c.cols.p, c.A64.p, x, b, r);
-  HIPCHECK(hipGetLastError());
+  if (prec == RMX_F32) launch_spmv<float, true>(c, c.A32.p, x, b, r);
+  else launch_spmv<double, true>(c, c.A64.p, x, b, r);
 }
 
 void dev_resid(Ctx& c, const double* x, const double* b, double* r)
 {
-  constexpr int G = 8;
-  const int blocks = (int)(((long long)c.n * G + 255) / 256);
-  k_resid<double, G><<<blocks, 256, 0, c.stream>>>(c.n, c.rowptr.p, c.cols.p, c.A64.p, x, b, r);
-  HIPCHECK(hipGetLastError());
+  launch_spmv<double, true>(c, c.A64.p, x, b, r);
 }
 
 // ---------------------------------------------------------------------------------------

@@ -105,6 +105,11 @@ def lib():
             "rmx_ilu0_apply": [_v, C.c_int, _d, _d], "rmx_spmv": [_v, C.c_int, _d, _d],
             "rmx_silu0_factor": [_v, C.c_int], "rmx_silu0_apply": [_v, C.c_int, _d, _d],
             "rmx_set_strips": [_v, C.c_int, C.c_int], "rmx_get_blocks": [_v, _i],
+            "rmx_mpnn_n_params": [_i], "rmx_mpnn_set_weights": [_v, _f, C.c_int],
+            "rmx_mpnn_get_weights": [_v, _f, C.c_int], "rmx_mpnn_forward": [_v, _d, _d, _f],
+            "rmx_mpnn_add_sample": [_v, _d, _d, C.c_int, _i],
+            "rmx_mpnn_loss_grad": [_v, C.c_int, _d, _f],
+            "rmx_mpnn_train": [_v, C.c_int, C.c_double, _d],
             "rmx_set_coarse": [_v, C.c_int], "rmx_coarse_factor": [_v, C.c_int],
             "rmx_get_aggregates": [_v, _i, _i], "rmx_precond_apply": [_v, C.c_int, C.c_int, _d, _d],
             "rmx_get_sweep_info": [_v, C.c_int, C.POINTER(SweepInfo)],
@@ -295,6 +300,48 @@ class Context:
         x = np.zeros(self.n)
         _check(lib().rmx_precond_apply(self.h, kind, prec, _p(b, _d), _p(x, _d)))
         return x
+
+    # ---- learned preconditioner ----------------------------------------------------------
+    @staticmethod
+    def mpnn_n_params():
+        n = C.c_int(0)
+        _check(lib().rmx_mpnn_n_params(C.byref(n)))
+        return n.value
+
+    def mpnn_set_weights(self, theta):
+        theta = _c(theta, np.float32)
+        _check(lib().rmx_mpnn_set_weights(self.h, _p(theta, _f), len(theta)))
+
+    def mpnn_get_weights(self):
+        theta = np.zeros(self.mpnn_n_params(), np.float32)
+        _check(lib().rmx_mpnn_get_weights(self.h, _p(theta, _f), len(theta)))
+        return theta
+
+    def mpnn_forward(self, r):
+        """Network correction RMS(r) * network(r) as an equation vector, and the node
+        features [np, 8]."""
+        r = _c(r, np.float64)
+        z = np.zeros(self.n)
+        feat = np.zeros((self.np, 8), np.float32)
+        _check(lib().rmx_mpnn_forward(self.h, _p(r, _d), _p(z, _d), _p(feat, _f)))
+        return z, feat
+
+    def mpnn_add_sample(self, r, t, capacity=0):
+        r, t = _c(r, np.float64), _c(t, np.float64)
+        k = C.c_int(0)
+        _check(lib().rmx_mpnn_add_sample(self.h, _p(r, _d), _p(t, _d), int(capacity), C.byref(k)))
+        return k.value
+
+    def mpnn_loss_grad(self, sample):
+        loss = C.c_double(0)
+        g = np.zeros(self.mpnn_n_params(), np.float32)
+        _check(lib().rmx_mpnn_loss_grad(self.h, int(sample), C.byref(loss), _p(g, _f)))
+        return loss.value, g
+
+    def mpnn_train(self, epochs, lr):
+        loss = C.c_double(0)
+        _check(lib().rmx_mpnn_train(self.h, int(epochs), float(lr), C.byref(loss)))
+        return loss.value
 
     def blocks(self):
         b = np.zeros(self.n, np.int32)
