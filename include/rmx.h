@@ -147,24 +147,25 @@ rmx_status rmx_get_aggregates(rmx_ctx* ctx, int* agg /* [neq] */, int* nc);
  * network on the mesh node graph (an edge joins two nodes of one element): node features
  * (u, v, depth/4, bed/4, 100 cf, residual u/v/h components / RMS(r)), an input layer
  * 8 -> 64, 6 layers h = h + relu([h, mean of neighbours' h] W + b) (BF16 MFMA, FP32
- * accumulate), an output layer 64 -> 3.  As a preconditioner it adds RMS(r) * network(r)
- * to the two-level preconditioner's M2^-1 r.
+ * accumulate), an output layer 64 -> 3; n(r) = RMS(r) * network(r) as an equation vector.
+ * As a preconditioner it corrects the two-level preconditioner M2 (RMX_PREC_SILU0C, which
+ * must be factored): z2 = M2^-1 r, r2 = r - A z2, z = z2 + n(r2), where n(r2) is kept only
+ * if ||r2 - A n(r2)|| < ||r2|| (decided on the device, per application).
  * rmx_mpnn_n_params: number of FP32 weights (layout: W_in [8][64], b_in, then per layer
  * W [128][64], b [64], then W_out [64][3], b_out [3]).
  * rmx_mpnn_set_weights / get_weights: the weights (host, FP32); setting also resets Adam.
- * rmx_mpnn_forward: z = RMS(r) * network(r) alone, and the node features [np][8] (nullable).
- * rmx_mpnn_add_sample: store a training sample (features of the current state and r,
- * target t / RMS(r) per node variable); capacity > 0 (re)allocates and clears the set.
- * rmx_mpnn_loss_grad: mean squared error over the variables with an equation, and its
- * gradient (nullable) for one stored sample.
+ * rmx_mpnn_forward: z = n(r) alone, and the node features [np][8] (nullable).
+ * rmx_mpnn_add_sample: store a training sample: the features of the current state and r,
+ * r itself and the current FP64 matrix A; capacity > 0 (re)allocates and clears the set.
+ * rmx_mpnn_loss_grad: the preconditioned residual ||r - A n(r)||^2 / ||r||^2 of one
+ * stored sample and its gradient (nullable).
  * rmx_mpnn_train: Adam (beta 0.9 / 0.999) over the stored samples, one step per sample,
  * `epochs` passes; *loss = mean loss of the last pass. */
 rmx_status rmx_mpnn_n_params(int* n);
 rmx_status rmx_mpnn_set_weights(rmx_ctx* ctx, const float* theta, int n);
 rmx_status rmx_mpnn_get_weights(rmx_ctx* ctx, float* theta, int n);
 rmx_status rmx_mpnn_forward(rmx_ctx* ctx, const double* r, double* z, float* feat);
-rmx_status rmx_mpnn_add_sample(rmx_ctx* ctx, const double* r, const double* t, int capacity,
-                               int* count);
+rmx_status rmx_mpnn_add_sample(rmx_ctx* ctx, const double* r, int capacity, int* count);
 rmx_status rmx_mpnn_loss_grad(rmx_ctx* ctx, int sample, double* loss, float* grad);
 rmx_status rmx_mpnn_train(rmx_ctx* ctx, int epochs, double lr, double* loss);
 
@@ -201,12 +202,16 @@ typedef struct {
   double accuracy;            /* last Krylov relative residual estimate */
   double true_resid;          /* ||B - A64 X||_2 / ||B||_2 in FP64 */
   double seconds;             /* wall time of the solve on the device stream */
+  int fallback;               /* 1: the learned preconditioner stalled and the solve finished
+                                 with BiCGStab and the two-level preconditioner */
 } rmx_krylov_out;
 
 /* Solve A X = B for the assembled system: replaces SOLVER::Iterate with P_BCGSTABD
  * (P_bcgstabd.cpp:98) or P_FGMRESD (P_fgmresd.cpp:90) and the ILU(0) preconditioner.
  * X starts at zero.  With refine = 1 the solve runs in the requested storage precision
- * inside an FP64 iterative refinement on the FP64 matrix. */
+ * inside an FP64 iterative refinement on the FP64 matrix; with the learned preconditioner,
+ * a refinement sweep that lowers the true FP64 residual less than tenfold switches the
+ * remaining sweeps to BiCGStab with the two-level preconditioner (out->fallback = 1). */
 rmx_status rmx_solve(rmx_ctx* ctx, const rmx_krylov_opts* opt, rmx_krylov_out* out);
 
 /* Host access to the system (tests, drop-in callers that keep host arrays). */

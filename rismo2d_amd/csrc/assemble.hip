@@ -588,6 +588,7 @@ void dev_assemble(Ctx& c, int prec, bool zero)
     c.have64 = true; c.have32 = false;
   }
   c.ilu.fac32 = c.ilu.fac64 = c.silu.fac32 = c.silu.fac64 = false;
+  c.mat_version++;
 }
 
 void dev_element_coefs(Ctx& c, int first, int count, double* K, double* F)

@@ -84,6 +84,15 @@ rmx_status rmx_mesh_upload(rmx_ctx* p, const rmx_mesh_desc* d)
     fail(RMX_ERR_ARG, "mesh: missing control-line segments");
   host_mesh(c.hm, *d);
   const HostModel& hm = c.hm;
+  {
+    unsigned long long h = 1469598103934665603ULL;            // FNV-1a over the topology
+    auto mix = [&](unsigned long long v) { h = (h ^ v) * 1099511628211ULL; };
+    mix((unsigned long long)hm.np); mix((unsigned long long)hm.ne);
+    for (int v : hm.elem) mix((unsigned long long)(unsigned)v);
+    for (int v : hm.in_segs) mix((unsigned long long)(unsigned)v);
+    for (int v : hm.out_segs) mix((unsigned long long)(unsigned)v);
+    c.topo_hash = h;
+  }
   const int np = hm.np;
   hipStream_t s = c.stream;
   c.x.upload(hm.x, s); c.y.upload(hm.y, s); c.zor.upload(hm.zor, s); c.z.upload(hm.zor, s);
@@ -401,7 +410,7 @@ rmx_status rmx_mpnn_n_params(int* n)
 {
   API_BEGIN
   if (!n) fail(RMX_ERR_ARG, "null output");
-  *n = 50307;
+  *n = 51282;
   API_END
 }
 
@@ -436,15 +445,14 @@ rmx_status rmx_mpnn_forward(rmx_ctx* p, const double* r, double* z, float* feat)
   API_END
 }
 
-rmx_status rmx_mpnn_add_sample(rmx_ctx* p, const double* r, const double* t, int capacity, int* count)
+rmx_status rmx_mpnn_add_sample(rmx_ctx* p, const double* r, int capacity, int* count)
 {
   API_BEGIN
   Ctx& c = *need(p);
-  if (!r || !t) fail(RMX_ERR_ARG, "null vector");
+  if (!r) fail(RMX_ERR_ARG, "null vector");
   if (!c.prepared) fail(RMX_ERR_ARG, "no system");
   c.w[0].upload(r, c.n, c.stream);
-  c.w[1].upload(t, c.n, c.stream);
-  const int k = mpnn_add_sample(c, c.w[0].p, c.w[1].p, capacity);
+  const int k = mpnn_add_sample(c, c.w[0].p, capacity);
   if (count) *count = k;
   API_END
 }
@@ -610,6 +618,7 @@ rmx_status rmx_csr_upload(rmx_ctx* p, int n, const int* rowptr, const int* cols,
   choose_spmv_lanes(c);
   setup_ilu_schedule(c);
   c.have64 = true; c.have32 = false;
+  c.mat_version++;
   c.prepared = true;
   API_END
 }

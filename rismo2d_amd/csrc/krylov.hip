@@ -442,9 +442,11 @@ void dev_solve(Ctx& c, const rmx_krylov_opts& o, rmx_krylov_out& out)
     int total = 0;
     const int sweeps_max = o.refine_max > 0 ? o.refine_max : 10;
     int sw = 0;
+    int method = o.method, kind = pk;
     for (; sw < sweeps_max; sw++) {
       int it = 0; double acc = 1.0;
-      krylov(c, o.method, prec, pk, r, d, o.max_iter, o.eps, o.mkyrl, it, acc);
+      const double tr_prev = tr;
+      krylov(c, method, prec, kind, r, d, o.max_iter, o.eps, o.mkyrl, it, acc);
       total += it;
       out.accuracy = acc;
       k_axpy<<<vblocks(c.n), 256, 0, c.stream>>>(c.n, 1.0, d, xr);
@@ -453,6 +455,13 @@ void dev_solve(Ctx& c, const rmx_krylov_opts& o, rmx_krylov_out& out)
         fprintf(stderr, "[rmx] refinement sweep %d: %d iterations (inner %.2e), FP64 residual %.3e\n",
                 sw + 1, it, acc, tr);
       if (tr <= o.refine_tol || !std::isfinite(tr)) { sw++; break; }
+      if (kind == RMX_PREC_MPNN && !(tr < 0.1 * tr_prev)) {
+        // the learned preconditioner stalled: finish with BiCGStab + two-level
+        method = RMX_BICGSTAB;
+        kind = RMX_PREC_SILU0C;
+        out.fallback = 1;
+        if (getenv("RMX_DEBUG")) fprintf(stderr, "[rmx] learned preconditioner stalled: fallback\n");
+      }
     }
     out.iters = total; out.sweeps = sw; out.true_resid = tr;
     out.converged = tr <= o.refine_tol;

@@ -1,22 +1,29 @@
 // Learned preconditioner: a message-passing network on the mesh graph, evaluated and
 // trained on the device with this file's kernels.  No reference counterpart.
 //
-// Graph: mesh nodes; an edge joins two nodes of one element (both directions).  Node
-// features f (8): u, v, depth/4, bed/4, 100 cf, and the residual's u, v, h components
-// divided by the residual's RMS.
+// Graph: mesh nodes; an edge joins two nodes of one element (both directions).  The
+// network sees the state of the Newton system, not the residual, so the preconditioner it
+// defines is linear in the residual and FGMRES keeps its Krylov structure.  Node features
+// (8): u, v, depth/4, bed/4, 100 cf, speed, open-boundary flag, rotated-node flag.
 //
 //     h0 = relu(f W_in + b_in)                  (FP32, stored BF16)
 //     6 x:  a = mean of h over the neighbours   (FP32 sums of BF16, stored BF16)
 //           p = [h, a] W_l + b_l                (BF16 MFMA 16x16x16, FP32 accumulate)
 //           h = h + relu(p)                     (stored BF16)
-//     out = h W_out + b_out                     (FP32): u, v, h corrections / RMS(r)
+//     out = h W_out + b_out                     (FP32, 18 per node): two 3x3 blocks D_i, E_i
 //
-// As a preconditioner (RMX_PREC_MPNN) the network adds its correction to the two-level
-// preconditioner: z = M2^-1 r + RMS(r) * scatter(out), trained on pairs (r, x* - M2^-1 r)
-// with x* = A^-1 r from converged solves.  Training: mean squared error over the node
-// variables that carry an equation, gradients by the backward kernels below (weight
-// gradients as per-block partial sums reduced in a fixed order, so they are
-// deterministic), Adam on the FP32 master weights, BF16 copies refreshed after each step.
+// The correction of a residual r: q = r / max_j |a_ij| (row-scaled), q_i its (u, v, h)
+// components at node i, qbar_i their mean over the neighbours, and
+//     n(r)_i = D_i q_i + E_i qbar_i.
+// As a preconditioner (RMX_PREC_MPNN) it corrects what the two-level preconditioner M2
+// leaves: z2 = M2^-1 r, r2 = r - A z2, z = z2 + n(r2).  The network runs once per Newton
+// system; at the first application of a system the correction is tested on that vector
+// and dropped for the whole system if it does not lower the residual.  Training
+// minimises the preconditioned residual ||r - A n(r)||^2 / ||r||^2 over stored samples
+// (r and the matrix A of the Newton system it came from); its gradient reaches D, E
+// through A^T.  Gradients by the backward kernels below (weight gradients as per-block
+// partial sums reduced in a fixed order, so they are deterministic), Adam on the FP32
+// master weights, BF16 copies refreshed after each step.
 #include <algorithm>
 #include <cmath>
 #include <cstring>
@@ -34,7 +41,7 @@ constexpr int LAYER_SZ = 2 * MC * MC + MC;
 constexpr int OFF_L0 = OFF_BIN + MC;
 constexpr int OFF_WOUT = OFF_L0 + ML * LAYER_SZ, OFF_BOUT = OFF_WOUT + MC * MO;
 constexpr int NPARAM = OFF_BOUT + MO;
-static_assert(NPARAM == 50307, "parameter layout");
+static_assert(NPARAM == 51282, "parameter layout");
 constexpr int CHUNK = 1024;        // nodes per block of the small reductions
 constexpr int WCHUNK = 256;        // nodes per block of the layer weight gradients
 
@@ -51,52 +58,116 @@ __device__ __forceinline__ float bf2f(uint16_t b) { return __uint_as_float((uint
 
 __global__ void k_feat(int np, const double* __restrict__ U, const double* __restrict__ V,
                        const double* __restrict__ S, const double* __restrict__ z,
-                       const double* __restrict__ cf, const int* __restrict__ eqno,
-                       const double* __restrict__ r, const double* __restrict__ r2, int n,
+                       const double* __restrict__ cf, const uint32_t* __restrict__ flag,
                        float* __restrict__ feat)
 {
   const int i = blockIdx.x * blockDim.x + threadIdx.x;
   if (i >= np) return;
-  const double rs = sqrt(r2[0] / n);
-  const double ri = rs > 0.0 ? 1.0 / rs : 0.0;
   float* f = feat + (size_t)i * MF;
   f[0] = (float)U[i];
   f[1] = (float)V[i];
   f[2] = (float)(fmax(S[i] - z[i], 0.0) * 0.25);
   f[3] = (float)(z[i] * 0.25);
   f[4] = (float)(cf[i] * 100.0);
-  for (int v = 0; v < 3; v++) {
-    const int e = eqno[3 * i + v];
-    f[5 + v] = e >= 0 ? (float)(r[e] * ri) : 0.0f;
-  }
+  f[5] = (float)sqrt(U[i] * U[i] + V[i] * V[i]);
+  f[6] = (flag[i] & (dInlet | dOutlet)) ? 1.0f : 0.0f;
+  f[7] = (flag[i] & dRotat) ? 1.0f : 0.0f;
 }
 
-__global__ void k_target(int np, const int* __restrict__ eqno, const double* __restrict__ t,
-                         const double* __restrict__ r2, int n, float* __restrict__ tgt,
-                         float* __restrict__ mask)
+// q components at node i and their neighbour mean
+__device__ __forceinline__ void node_q(int i, const int* __restrict__ eqno, const int* __restrict__ ptr,
+                                       const int* __restrict__ adj, const double* __restrict__ q,
+                                       double qi[3], double qb[3])
+{
+  for (int w = 0; w < 3; w++) {
+    const int e = eqno[3 * i + w];
+    qi[w] = e >= 0 ? q[e] : 0.0;
+    qb[w] = 0.0;
+  }
+  const int a = ptr[i], b = ptr[i + 1];
+  for (int k = a; k < b; k++) {
+    const int j = adj[k];
+    for (int w = 0; w < 3; w++) {
+      const int e = eqno[3 * j + w];
+      if (e >= 0) qb[w] += q[e];
+    }
+  }
+  if (b > a)
+    for (int w = 0; w < 3; w++) qb[w] /= (double)(b - a);
+}
+
+// n_i = D_i q_i + E_i qbar_i (out: [np][18], D then E, row-major 3x3)
+__global__ void k_lin_apply(int np, const int* __restrict__ eqno, const int* __restrict__ ptr,
+                            const int* __restrict__ adj, const float* __restrict__ out,
+                            const double* __restrict__ q, double* __restrict__ nv)
 {
   const int i = blockIdx.x * blockDim.x + threadIdx.x;
   if (i >= np) return;
-  const double rs = sqrt(r2[0] / n);
-  const double ri = rs > 0.0 ? 1.0 / rs : 0.0;
+  double qi[3], qb[3];
+  node_q(i, eqno, ptr, adj, q, qi, qb);
+  const float* o = out + (size_t)i * MO;
   for (int v = 0; v < 3; v++) {
     const int e = eqno[3 * i + v];
-    tgt[3 * i + v] = e >= 0 ? (float)(t[e] * ri) : 0.0f;
-    mask[3 * i + v] = e >= 0 ? 1.0f : 0.0f;
+    if (e < 0) continue;
+    double s = 0.0;
+    for (int w = 0; w < 3; w++) s += (double)o[3 * v + w] * qi[w] + (double)o[9 + 3 * v + w] * qb[w];
+    nv[e] = s;
   }
 }
 
-// z[e] (+)= RMS(r) * out
-__global__ void k_scatter_out(int np, const int* __restrict__ eqno, const float* __restrict__ out,
-                              const double* __restrict__ r2, int n, double* __restrict__ z, int add)
+// gout[i] = d loss / d (D_i, E_i) from g = d loss / d n (equation vector)
+__global__ void k_lin_grad(int np, const int* __restrict__ eqno, const int* __restrict__ ptr,
+                           const int* __restrict__ adj, const double* __restrict__ q,
+                           const double* __restrict__ g, float* __restrict__ gout)
 {
   const int i = blockIdx.x * blockDim.x + threadIdx.x;
   if (i >= np) return;
-  const double rs = sqrt(r2[0] / n);
+  double qi[3], qb[3];
+  node_q(i, eqno, ptr, adj, q, qi, qb);
+  float* o = gout + (size_t)i * MO;
   for (int v = 0; v < 3; v++) {
     const int e = eqno[3 * i + v];
-    if (e >= 0) z[e] = (add ? z[e] : 0.0) + rs * (double)out[3 * i + v];
+    const double gv = e >= 0 ? g[e] : 0.0;
+    for (int w = 0; w < 3; w++) {
+      o[3 * v + w] = (float)(gv * qi[w]);
+      o[9 + 3 * v + w] = (float)(gv * qb[w]);
+    }
   }
+}
+
+// res = b - A x for a stored sample's matrix values (FP64, 8 lanes per row)
+__global__ void __launch_bounds__(256) k_resid_s(int n, const int* __restrict__ rowptr,
+                                                 const int* __restrict__ cols,
+                                                 const double* __restrict__ A,
+                                                 const double* __restrict__ x,
+                                                 const double* __restrict__ b,
+                                                 double* __restrict__ res)
+{
+  const long long t = (long long)blockIdx.x * blockDim.x + threadIdx.x;
+  const int row = (int)(t >> 3), lane = (int)(t & 7);
+  if (row >= n) return;
+  double s = 0.0;
+  for (int p = rowptr[row] + lane; p < rowptr[row + 1]; p += 8) s += A[p] * x[cols[p]];
+  for (int o = 4; o > 0; o >>= 1) s += __shfl_xor(s, o, 8);
+  if (lane == 0) res[row] = b[row] - s;
+}
+
+// y = scale * A^T x with A given in CSR order and its transpose as column pointers, row
+// indices and the CSR position of every entry
+__global__ void __launch_bounds__(256) k_spmv_t(int n, const int* __restrict__ tptr,
+                                                const int* __restrict__ trow,
+                                                const int* __restrict__ tperm,
+                                                const double* __restrict__ A,
+                                                const double* __restrict__ x, double scale,
+                                                double* __restrict__ y)
+{
+  const long long t = (long long)blockIdx.x * blockDim.x + threadIdx.x;
+  const int j = (int)(t >> 3), lane = (int)(t & 7);
+  if (j >= n) return;
+  double s = 0.0;
+  for (int k = tptr[j] + lane; k < tptr[j + 1]; k += 8) s += A[tperm[k]] * x[trow[k]];
+  for (int o = 4; o > 0; o >>= 1) s += __shfl_xor(s, o, 8);
+  if (lane == 0) y[j] = scale * s;
 }
 
 // ---------------------------------------------------------------------------------------
@@ -190,31 +261,6 @@ __global__ void k_out(int np, const uint16_t* __restrict__ H, const float* __res
 
 // ---------------------------------------------------------------------------------------
 // backward
-
-// gout = 2 mask (out - t) / cnt; per-block partial sums of mask (out - t)^2
-__global__ void __launch_bounds__(256) k_gout(int np, const float* __restrict__ out,
-                                              const float* __restrict__ tgt,
-                                              const float* __restrict__ mask, float inv_cnt,
-                                              float* __restrict__ gout, double* __restrict__ lpart)
-{
-  __shared__ double sh[256];
-  const int i = blockIdx.x * blockDim.x + threadIdx.x;
-  double l = 0.0;
-  if (i < np)
-    for (int v = 0; v < MO; v++) {
-      const size_t o = (size_t)i * MO + v;
-      const float d = (out[o] - tgt[o]) * mask[o];
-      gout[o] = 2.0f * d * inv_cnt;
-      l += (double)d * d;
-    }
-  sh[threadIdx.x] = l;
-  __syncthreads();
-  for (int s = 128; s > 0; s >>= 1) {
-    if ((int)threadIdx.x < s) sh[threadIdx.x] += sh[threadIdx.x + s];
-    __syncthreads();
-  }
-  if (threadIdx.x == 0) lpart[blockIdx.x] = sh[0];
-}
 
 // gh = gout W_out^T
 __global__ void k_gh_out(int np, const float* __restrict__ gout, const float* __restrict__ th,
@@ -421,6 +467,12 @@ __global__ void k_refresh(const float* __restrict__ th, uint16_t* __restrict__ w
   wt[(size_t)l * 2 * MC * MC + c * 2 * MC + m] = w;
 }
 
+__global__ void k_add_v(int n, const double* __restrict__ nv, double* __restrict__ z)
+{
+  const int i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (i < n) z[i] += nv[i];
+}
+
 // ---------------------------------------------------------------------------------------
 // host side
 
@@ -472,22 +524,74 @@ static void build_graph(Ctx& c)
   M.graph = true;
 }
 
+// The graph follows the mesh topology; a mesh of another topology rebuilds it and drops
+// the stored training samples (they belong to the old graph).
 static void need_graph(Ctx& c)
 {
   if (!c.prepared || c.csr_only) fail(RMX_ERR_ARG, "learned preconditioner: needs a prepared mesh");
-  if (!c.mpnn.graph || c.mpnn.np != c.hm.np) build_graph(c);
+  Mpnn& M = c.mpnn;
+  if (!M.graph || M.topo != c.topo_hash) {
+    build_graph(c);
+    M.topo = c.topo_hash;
+    M.nsamp = 0;
+    M.s_rr.clear();
+    M.tn = 0;
+  }
 }
 
 // features of the current state and residual r (device, length n) into M.feat; the
 // squared norm of r goes to M.r2
-static void features(Ctx& c, const double* r)
+// 1 / max_j |a_ij| per row (0 for an empty row)
+__global__ void k_rowinv(int n, const int* __restrict__ rowptr, const double* __restrict__ A,
+                         double* __restrict__ rinv)
+{
+  const int i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= n) return;
+  double m = 0.0;
+  for (int p = rowptr[i]; p < rowptr[i + 1]; p++) m = fmax(m, fabs(A[p]));
+  rinv[i] = m > 0.0 ? 1.0 / m : 0.0;
+}
+
+__global__ void k_mul(int n, const double* __restrict__ a, const double* __restrict__ b,
+                      double* __restrict__ c)
+{
+  const int i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (i < n) c[i] = a[i] * b[i];
+}
+
+// node features of the current state into M.feat
+static void features(Ctx& c)
 {
   Mpnn& M = c.mpnn;
-  M.r2.alloc(1);
-  const double* xs[1] = { r };
-  dev_dots(c, 1, xs, xs, M.r2.p);
-  k_feat<<<gb(M.np), 256, 0, c.stream>>>(M.np, c.U.p, c.V.p, c.S.p, c.z.p, c.cf.p, c.eqno.p, r,
-                                         M.r2.p, c.n, M.feat.p);
+  k_feat<<<gb(M.np), 256, 0, c.stream>>>(M.np, c.U.p, c.V.p, c.S.p, c.z.p, c.cf.p, c.dflag.p,
+                                         M.feat.p);
+}
+
+// q = r / row max of the current FP64 matrix, into M.q
+static void scaled(Ctx& c, const double* r)
+{
+  Mpnn& M = c.mpnn;
+  if (!c.have64) fail(RMX_ERR_ARG, "learned preconditioner: needs the FP64 matrix");
+  if (M.rinv_version != c.mat_version) {
+    M.rinv.alloc(c.n);
+    k_rowinv<<<gb(c.n), 256, 0, c.stream>>>(c.n, c.rowptr.p, c.A64.p, M.rinv.p);
+    M.rinv_version = c.mat_version;
+  }
+  M.q.alloc(c.n);
+  k_mul<<<gb(c.n), 256, 0, c.stream>>>(c.n, r, M.rinv.p, M.q.p);
+}
+
+static void forward(Ctx& c, const float* feat, bool train);
+
+// network outputs (D, E per node) of the current Newton system, once per system
+static void trunk(Ctx& c)
+{
+  Mpnn& M = c.mpnn;
+  if (M.trunk_version == c.mat_version) return;
+  features(c);
+  forward(c, M.feat.p, false);
+  M.trunk_version = c.mat_version;
+  M.use_net = -1;
 }
 
 static void forward(Ctx& c, const float* feat, bool train)
@@ -509,23 +613,38 @@ static void forward(Ctx& c, const float* feat, bool train)
   HIPCHECK(hipGetLastError());
 }
 
-// loss and gradient (into M.grad) of one stored sample
+// loss ||r - A n||^2 / ||r||^2 and its gradient (into M.grad) for one stored sample
 static double backward(Ctx& c, int s)
 {
   Mpnn& M = c.mpnn;
-  const int np = M.np;
+  const int np = M.np, n = c.n;
   const size_t nh = (size_t)np * MC;
   const float* feat = M.s_feat.p + (size_t)s * np * MF;
-  const float* tgt = M.s_tgt.p + (size_t)s * np * MO;
-  const float* mask = M.s_mask.p + (size_t)s * np * MO;
+  const double* r = M.s_r.p + (size_t)s * n;
+  const double* A = M.s_A.p + (size_t)s * c.nnz;
   forward(c, feat, true);
   M.gh.alloc(nh); M.ga.alloc(nh); M.gp.alloc(nh); M.gout.alloc((size_t)np * MO);
-  const int nbl = gb(np);
-  M.lpart.alloc(nbl);
-  const float inv_cnt = 1.0f / (float)std::max(1.0, M.s_cnt[s]);
-  k_gout<<<nbl, 256, 0, c.stream>>>(np, M.out.p, tgt, mask, inv_cnt, M.gout.p, M.lpart.p);
+  M.ev.alloc(4 * (size_t)n);
+  double* nv = M.ev.p;
+  double* res = M.ev.p + n;
+  double* qs = M.ev.p + 2 * (size_t)n;
+  double* ri = M.ev.p + 3 * (size_t)n;
+  k_rowinv<<<gb(n), 256, 0, c.stream>>>(n, c.rowptr.p, A, ri);
+  k_mul<<<gb(n), 256, 0, c.stream>>>(n, r, ri, qs);
+  k_lin_apply<<<gb(np), 256, 0, c.stream>>>(np, c.eqno.p, M.adj_ptr.p, M.adj.p, M.out.p, qs, nv);
+  k_resid_s<<<gb((long long)n * 8), 256, 0, c.stream>>>(n, c.rowptr.p, c.cols.p, A, nv, r, res);
+  M.lpart.alloc(1);
+  const double* xs[1] = { res };
+  dev_dots(c, 1, xs, xs, M.lpart.p);
+  const double rr = M.s_rr[s];
+  // d loss / d n = -2 A^T res / ||r||^2, into nv; then d loss / d (D, E)
+  k_spmv_t<<<gb((long long)n * 8), 256, 0, c.stream>>>(n, M.tptr.p, M.trow.p, M.tperm.p, A, res,
+                                                       -2.0 / rr, nv);
+  k_lin_grad<<<gb(np), 256, 0, c.stream>>>(np, c.eqno.p, M.adj_ptr.p, M.adj.p, qs, nv, M.gout.p);
   const int nchunk = gb(np, CHUNK), nw = gb(np, WCHUNK);
-  M.partial.alloc(std::max<size_t>((size_t)nchunk * (MF + 1) * MC, (size_t)nw * 2 * MC * MC));
+  M.partial.alloc(std::max<size_t>(std::max<size_t>((size_t)nchunk * (MF + 1) * MC,
+                                                     (size_t)nchunk * (MC + 1) * MO),
+                                    (size_t)nw * 2 * MC * MC));
   // output layer
   k_outer<uint16_t, float><<<nchunk, 256, 0, c.stream>>>(np, M.H.p + ML * nh, MC, M.gout.p, MO, M.partial.p);
   k_reduce<<<gb((MC + 1) * MO), 256, 0, c.stream>>>(nchunk, (MC + 1) * MO, M.partial.p, MC, MO,
@@ -548,20 +667,17 @@ static double backward(Ctx& c, int s)
   k_reduce<<<gb((MF + 1) * MC), 256, 0, c.stream>>>(nchunk, (MF + 1) * MC, M.partial.p, MF, MC,
                                                     OFF_WIN, OFF_BIN, M.grad.p);
   HIPCHECK(hipGetLastError());
-  std::vector<double> lp(nbl);
-  HIPCHECK(hipMemcpyAsync(lp.data(), M.lpart.p, sizeof(double) * nbl, hipMemcpyDeviceToHost, c.stream));
-  HIPCHECK(hipStreamSynchronize(c.stream));
   double loss = 0.0;
-  for (double v : lp) loss += v;
-  return loss * inv_cnt;
+  HIPCHECK(hipMemcpyAsync(&loss, M.lpart.p, sizeof(double), hipMemcpyDeviceToHost, c.stream));
+  HIPCHECK(hipStreamSynchronize(c.stream));
+  return loss / rr;
 }
 
-bool mpnn_ready(Ctx& c) { return c.mpnn.weights && c.mpnn.graph && c.mpnn.np == c.hm.np; }
+bool mpnn_ready(Ctx& c) { return c.mpnn.weights; }
 
 void mpnn_set_weights(Ctx& c, const float* th, int n)
 {
   if (n != NPARAM) fail(RMX_ERR_ARG, "learned preconditioner: expected " + std::to_string(NPARAM) + " weights");
-  need_graph(c);
   Mpnn& M = c.mpnn;
   M.theta.upload(th, NPARAM, c.stream);
   M.grad.alloc(NPARAM); M.m.alloc(NPARAM); M.v.alloc(NPARAM);
@@ -572,6 +688,7 @@ void mpnn_set_weights(Ctx& c, const float* th, int n)
   HIPCHECK(hipStreamSynchronize(c.stream));
   M.step = 0;
   M.weights = true;
+  M.trunk_version = -1;
 }
 
 void mpnn_get_weights(Ctx& c, float* th, int n)
@@ -586,53 +703,105 @@ void mpnn_get_weights(Ctx& c, float* th, int n)
 void mpnn_apply(Ctx& c, int prec, const double* r, double* z)
 {
   if (!mpnn_ready(c)) fail(RMX_ERR_ARG, "learned preconditioner not loaded");
+  need_graph(c);
+  Mpnn& M = c.mpnn;
+  const int n = c.n;
+  trunk(c);
   coarse_apply(c, prec, r, z);
-  features(c, r);
-  forward(c, c.mpnn.feat.p, false);
-  k_scatter_out<<<gb(c.mpnn.np), 256, 0, c.stream>>>(c.mpnn.np, c.eqno.p, c.mpnn.out.p, c.mpnn.r2.p,
-                                                     c.n, z, 1);
+  if (M.use_net == 0) return;
+  M.rv.alloc(3 * (size_t)n);
+  double* r2v = M.rv.p;                        // r2 = r - A z2
+  double* nv = M.rv.p + n;                     // the network's correction n(r2)
+  double* tv = M.rv.p + 2 * (size_t)n;         // r2 - A n
+  dev_resid_p(c, prec, z, r, r2v);
+  scaled(c, r2v);
+  k_lin_apply<<<gb(M.np), 256, 0, c.stream>>>(M.np, c.eqno.p, M.adj_ptr.p, M.adj.p, M.out.p, M.q.p, nv);
+  if (M.use_net < 0) {
+    // first application of this system: keep the network only if it lowers the residual
+    dev_resid_p(c, prec, nv, r2v, tv);
+    M.acc.alloc(2);
+    const double* xs[2] = { tv, r2v };
+    dev_dots(c, 2, xs, xs, M.acc.p);
+    double h[2];
+    HIPCHECK(hipMemcpyAsync(h, M.acc.p, sizeof(h), hipMemcpyDeviceToHost, c.stream));
+    HIPCHECK(hipStreamSynchronize(c.stream));
+    M.use_net = h[0] < h[1] ? 1 : 0;
+    if (getenv("RMX_DEBUG"))
+      fprintf(stderr, "[rmx] learned correction %s (residual %.3e -> %.3e)\n",
+              M.use_net ? "kept" : "dropped", std::sqrt(h[1]), std::sqrt(h[0]));
+    if (!M.use_net) return;
+  }
+  k_add_v<<<gb(n), 256, 0, c.stream>>>(n, nv, z);
   HIPCHECK(hipGetLastError());
 }
 
-// network correction alone: z = RMS(r) * network(r); feat (host, optional) gets the features
+// network correction alone: z = n(r) for the current system; feat (host, optional) gets
+// the node features
 void mpnn_forward(Ctx& c, const double* r, double* z, float* feat_out)
 {
   if (!mpnn_ready(c)) fail(RMX_ERR_ARG, "learned preconditioner not loaded");
-  features(c, r);
-  forward(c, c.mpnn.feat.p, false);
-  k_scatter_out<<<gb(c.mpnn.np), 256, 0, c.stream>>>(c.mpnn.np, c.eqno.p, c.mpnn.out.p, c.mpnn.r2.p,
-                                                     c.n, z, 0);
+  need_graph(c);
+  Mpnn& M = c.mpnn;
+  M.trunk_version = -1;                        // weights may have changed
+  trunk(c);
+  scaled(c, r);
+  k_lin_apply<<<gb(M.np), 256, 0, c.stream>>>(M.np, c.eqno.p, M.adj_ptr.p, M.adj.p, M.out.p, M.q.p, z);
   HIPCHECK(hipGetLastError());
-  if (feat_out) c.mpnn.feat.download(feat_out, (size_t)c.mpnn.np * MF, c.stream);
+  if (feat_out) M.feat.download(feat_out, (size_t)M.np * MF, c.stream);
   HIPCHECK(hipStreamSynchronize(c.stream));
 }
 
-// store a training sample: features of (state, r), target t / RMS(r)   (device pointers)
-int mpnn_add_sample(Ctx& c, const double* r, const double* t, int capacity)
+// transpose of the system pattern: column pointers, row indices, CSR position of every
+// entry (for A^T in the training loss)
+static void build_transpose(Ctx& c)
+{
+  Mpnn& M = c.mpnn;
+  const int n = c.n;
+  const std::vector<int>& rp = c.hm.rowptr;
+  const std::vector<int>& cl = c.hm.cols;
+  std::vector<int> tptr(n + 1, 0), trow(rp[n]), tperm(rp[n]);
+  for (int q = 0; q < rp[n]; q++) tptr[cl[q] + 1]++;
+  for (int j = 0; j < n; j++) tptr[j + 1] += tptr[j];
+  std::vector<int> fill(tptr.begin(), tptr.end() - 1);
+  for (int i = 0; i < n; i++)
+    for (int q = rp[i]; q < rp[i + 1]; q++) { const int k = fill[cl[q]]++; trow[k] = i; tperm[k] = q; }
+  M.tptr.upload(tptr, c.stream); M.trow.upload(trow, c.stream); M.tperm.upload(tperm, c.stream);
+  M.tn = n;
+}
+
+// store a training sample: features of (state, r), r itself and the current FP64 matrix
+// (device pointer r)
+int mpnn_add_sample(Ctx& c, const double* r, int capacity)
 {
   need_graph(c);
+  if (!c.have64) fail(RMX_ERR_ARG, "learned preconditioner: a sample needs the FP64 matrix");
   Mpnn& M = c.mpnn;
-  const int np = M.np;
+  const int np = M.np, n = c.n;
+  if (M.tn != n) build_transpose(c);
   if (capacity > 0 && capacity != M.cap) {
     M.cap = capacity;
-    M.nsamp = 0;
-    M.s_cnt.clear();
     M.s_feat.alloc((size_t)capacity * np * MF);
-    M.s_tgt.alloc((size_t)capacity * np * MO);
-    M.s_mask.alloc((size_t)capacity * np * MO);
+    M.s_r.alloc((size_t)capacity * n);
+    M.s_A.alloc((size_t)capacity * c.nnz);
   }
+  if (capacity > 0) { M.nsamp = 0; M.s_rr.clear(); }
+  M.lpart.alloc(1);
   if (M.nsamp >= M.cap) fail(RMX_ERR_RANGE, "learned preconditioner: training set full");
   const int s = M.nsamp;
-  features(c, r);
+  features(c);
+  hipStream_t st = c.stream;
   HIPCHECK(hipMemcpyAsync(M.s_feat.p + (size_t)s * np * MF, M.feat.p, sizeof(float) * np * MF,
-                          hipMemcpyDeviceToDevice, c.stream));
-  k_target<<<gb(np), 256, 0, c.stream>>>(np, c.eqno.p, t, M.r2.p, c.n, M.s_tgt.p + (size_t)s * np * MO,
-                                         M.s_mask.p + (size_t)s * np * MO);
-  HIPCHECK(hipGetLastError());
-  HIPCHECK(hipStreamSynchronize(c.stream));
-  int cnt = 0;
-  for (size_t k = 0; k < c.hm.eqno.size(); k++) cnt += c.hm.eqno[k] >= 0;
-  M.s_cnt.push_back((double)cnt);
+                          hipMemcpyDeviceToDevice, st));
+  HIPCHECK(hipMemcpyAsync(M.s_r.p + (size_t)s * n, r, sizeof(double) * n, hipMemcpyDeviceToDevice, st));
+  HIPCHECK(hipMemcpyAsync(M.s_A.p + (size_t)s * c.nnz, c.A64.p, sizeof(double) * c.nnz,
+                          hipMemcpyDeviceToDevice, st));
+  double rr = 0.0;
+  const double* xs[1] = { r };
+  dev_dots(c, 1, xs, xs, M.lpart.p);
+  HIPCHECK(hipMemcpyAsync(&rr, M.lpart.p, sizeof(double), hipMemcpyDeviceToHost, st));
+  HIPCHECK(hipStreamSynchronize(st));
+  if (!(rr > 0.0)) fail(RMX_ERR_ARG, "learned preconditioner: zero residual sample");
+  M.s_rr.push_back(rr);
   return ++M.nsamp;
 }
 
@@ -641,6 +810,7 @@ double mpnn_loss_grad(Ctx& c, int s, float* grad)
   Mpnn& M = c.mpnn;
   if (!M.weights) fail(RMX_ERR_ARG, "learned preconditioner: no weights");
   if (s < 0 || s >= M.nsamp) fail(RMX_ERR_ARG, "learned preconditioner: no such sample");
+  need_graph(c);
   const double loss = backward(c, s);
   if (grad) {
     M.grad.download(grad, NPARAM, c.stream);
@@ -656,6 +826,7 @@ double mpnn_train(Ctx& c, int epochs, double lr)
   Mpnn& M = c.mpnn;
   if (!M.weights) fail(RMX_ERR_ARG, "learned preconditioner: no weights");
   if (M.nsamp == 0) fail(RMX_ERR_ARG, "learned preconditioner: no training samples");
+  need_graph(c);
   const float b1 = 0.9f, b2 = 0.999f, eps = 1e-8f;
   double last = 0.0;
   for (int ep = 0; ep < epochs; ep++) {
@@ -670,6 +841,7 @@ double mpnn_train(Ctx& c, int epochs, double lr)
     }
     last = sum / M.nsamp;
   }
+  M.trunk_version = -1;
   HIPCHECK(hipGetLastError());
   HIPCHECK(hipStreamSynchronize(c.stream));
   return last;

@@ -152,15 +152,18 @@ struct Coarse {
 // Learned preconditioner: mesh node graph, weights (FP32 master + BF16 layer copies),
 // activations, gradients, Adam moments, stored training samples.
 struct Mpnn {
-  static constexpr int C = 64, L = 6, F = 8, O = 3;
+  static constexpr int C = 64, L = 6, F = 8, O = 18;
   bool graph = false, weights = false;
-  int np = 0, step = 0, cap = 0, nsamp = 0;
-  std::vector<double> s_cnt;                  // equations per stored sample
-  DBuf<int> adj_ptr, adj;
+  int np = 0, step = 0, cap = 0, nsamp = 0, tn = 0;
+  unsigned long long topo = 0;                // topology fingerprint of the graph's mesh
+  std::vector<double> s_rr;                   // ||r||^2 of every stored sample
+  DBuf<int> adj_ptr, adj, tptr, trow, tperm;  // node graph; transpose of the system pattern
   DBuf<float> theta, grad, m, v;
   DBuf<uint16_t> wt, wb, H, A, gp;
-  DBuf<float> P, Q, feat, out, gh, ga, gout, partial, s_feat, s_tgt, s_mask;
-  DBuf<double> r2, lpart;
+  DBuf<float> P, Q, feat, out, gh, ga, gout, partial, s_feat;
+  DBuf<double> lpart, ev, rv, s_r, s_A, rinv, q, acc;
+  long long rinv_version = -1, trunk_version = -1;
+  int use_net = -1;                           // per system: -1 undecided, 0 dropped, 1 used
 };
 
 struct Ctx {
@@ -197,6 +200,8 @@ struct Ctx {
   DBuf<float> A32;
   bool have64 = false, have32 = false;
   int spmv_g = 8;                             // SpMV lanes per row
+  long long mat_version = 0;                  // bumped whenever A64 changes
+  unsigned long long topo_hash = 0;           // fingerprint of the mesh topology
   Ilu ilu;                                    // reference-order ILU(0)
   Ilu silu;                                   // strip ILU(0)
   int strips = 0;                             // 0: one strip per compute unit
@@ -259,7 +264,7 @@ void mpnn_apply(Ctx& c, int prec, const double* r, double* z);
 void mpnn_forward(Ctx& c, const double* r, double* z, float* feat_out);
 void mpnn_set_weights(Ctx& c, const float* th, int n);
 void mpnn_get_weights(Ctx& c, float* th, int n);
-int mpnn_add_sample(Ctx& c, const double* r, const double* t, int capacity);
+int mpnn_add_sample(Ctx& c, const double* r, int capacity);
 double mpnn_loss_grad(Ctx& c, int s, float* grad);
 double mpnn_train(Ctx& c, int epochs, double lr);
 

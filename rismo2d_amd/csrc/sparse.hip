@@ -212,6 +212,7 @@ double dev_scale_l2(Ctx& c)
   else
     k_scale_rows<float><<<2048, 256, 0, c.stream>>>(c.n, c.rowptr.p, c.A32.p, c.B.p, c.red.p);
   HIPCHECK(hipGetLastError());
+  c.mat_version++;
   HIPCHECK(hipMemcpyAsync(c.h_red, c.red.p, sizeof(double), hipMemcpyDeviceToHost, c.stream));
   HIPCHECK(hipStreamSynchronize(c.stream));
   return std::sqrt(c.h_red[0]);
@@ -242,6 +243,7 @@ void dev_convert(Ctx& c, int to_prec)
     c.A64.alloc(c.nnz);
     k_f2d<<<4096, 256, 0, c.stream>>>(c.nnz, c.A32.p, c.A64.p);
     c.have64 = true;
+    c.mat_version++;
   }
   HIPCHECK(hipGetLastError());
 }

@@ -55,7 +55,8 @@ class KrylovOpts(C.Structure):
 
 class KrylovOut(C.Structure):
     _fields_ = [("iters", C.c_int), ("sweeps", C.c_int), ("converged", C.c_int),
-                ("accuracy", C.c_double), ("true_resid", C.c_double), ("seconds", C.c_double)]
+                ("accuracy", C.c_double), ("true_resid", C.c_double), ("seconds", C.c_double),
+                ("fallback", C.c_int)]
 
 
 class StepStats(C.Structure):
@@ -107,7 +108,7 @@ def lib():
             "rmx_set_strips": [_v, C.c_int, C.c_int], "rmx_get_blocks": [_v, _i],
             "rmx_mpnn_n_params": [_i], "rmx_mpnn_set_weights": [_v, _f, C.c_int],
             "rmx_mpnn_get_weights": [_v, _f, C.c_int], "rmx_mpnn_forward": [_v, _d, _d, _f],
-            "rmx_mpnn_add_sample": [_v, _d, _d, C.c_int, _i],
+            "rmx_mpnn_add_sample": [_v, _d, C.c_int, _i],
             "rmx_mpnn_loss_grad": [_v, C.c_int, _d, _f],
             "rmx_mpnn_train": [_v, C.c_int, C.c_double, _d],
             "rmx_set_coarse": [_v, C.c_int], "rmx_coarse_factor": [_v, C.c_int],
@@ -326,10 +327,10 @@ class Context:
         _check(lib().rmx_mpnn_forward(self.h, _p(r, _d), _p(z, _d), _p(feat, _f)))
         return z, feat
 
-    def mpnn_add_sample(self, r, t, capacity=0):
-        r, t = _c(r, np.float64), _c(t, np.float64)
+    def mpnn_add_sample(self, r, capacity=0):
+        r = _c(r, np.float64)
         k = C.c_int(0)
-        _check(lib().rmx_mpnn_add_sample(self.h, _p(r, _d), _p(t, _d), int(capacity), C.byref(k)))
+        _check(lib().rmx_mpnn_add_sample(self.h, _p(r, _d), int(capacity), C.byref(k)))
         return k.value
 
     def mpnn_loss_grad(self, sample):

@@ -1,0 +1,166 @@
+"""Learned preconditioner on the GPU (through the C ABI) against the CPU oracle
+(oracle/mpnn_oracle.py, itself pinned by finite differences), and what training gains.
+
+Tolerances: the GPU stores activations, layer weights and layer gradients in BF16 at the
+points the oracle rounds with bf16=True and sums in FP32 where the oracle sums in FP64;
+the forward output and the gradient agree to 2e-2 relative (L2 over all entries), which
+is several BF16 ulps (2^-8) accumulated through six layers.
+"""
+import numpy as np
+import pytest
+
+from helpers import golden_deck, load_golden
+from oracle import mpnn_oracle as mo
+from rismo2d_amd import deck as dk
+from rismo2d_amd import mpnn
+from rismo2d_amd import rmx
+
+pytestmark = pytest.mark.gpu
+
+
+@pytest.fixture(scope="module")
+def ctx():
+    c = rmx.Context(0)
+    yield c
+    c.close()
+
+
+def _graph(deck):
+    """Nodes sharing an element, sorted neighbour lists without self loops."""
+    n = deck.n_nodes
+    nb = [set() for _ in range(n)]
+    for e in range(deck.n_elems):
+        nd = deck.elems[e, :deck.nnd[e]]
+        for a in nd:
+            nb[a].update(int(b) for b in nd if b != a)
+    ptr = np.concatenate([[0], np.cumsum([len(s) for s in nb])]).astype(np.int32)
+    adj = np.concatenate([sorted(s) for s in nb]).astype(np.int32)
+    return ptr, adj
+
+
+def _random_weights(seed):
+    """Initial weights with a non-zero output layer (the product's initialisation starts
+    it at zero), so that every parameter block carries gradient."""
+    theta = mpnn.init_weights(seed)
+    rng = np.random.default_rng(seed)
+    k0 = sum(int(np.prod(sh)) for name, sh in mo.shapes() if name not in ("W_out", "b_out"))
+    theta[k0:k0 + 64 * mo.O] = (rng.uniform(-1, 1, 64 * mo.O) * 0.05).astype(np.float32)
+    return theta
+
+
+def _setup(ctx):
+    g = load_golden("channel200x20")
+    deck = golden_deck(g)
+    ctx.upload_mesh(deck)
+    ctx.set_params()
+    ctx.prepare()
+    ctx.assemble(rmx.F64)
+    ctx.scale_l2()
+    _, B = ctx.matrix(rmx.F64)
+    return deck, B
+
+
+def test_mpnn_forward_matches_oracle(ctx):
+    print("[gpu] MPNN forward vs oracle", flush=True)
+    deck, B = _setup(ctx)
+    theta = _random_weights(7)
+    ctx.mpnn_set_weights(theta)
+    np.testing.assert_array_equal(ctx.mpnn_get_weights(), theta)
+    z, feat = ctx.mpnn_forward(B)
+    ptr, adj = _graph(deck)
+    out = mo.forward(theta.astype(np.float64), feat.astype(np.float64), ptr, adj, bf=True)
+    eq = ctx.numbering()
+    rp, _ = ctx.pattern()
+    A, _ = ctx.matrix(rmx.F64)
+    zo = mo.lin_apply(out, B * mo.row_inv(rp, A), eq, ptr, adj)
+    err = np.linalg.norm(z - zo) / np.linalg.norm(zo)
+    print(f"    forward: rel L2 err {err:.2e}")
+    assert err <= 2e-2
+    # features: u, v, depth/4, bed/4, 100 cf, speed, open-boundary and rotated flags
+    U, V, S, zb = ctx.state()
+    np.testing.assert_allclose(feat[:, 0], U, rtol=1e-6, atol=1e-7)
+    np.testing.assert_allclose(feat[:, 3], zb / 4, rtol=1e-6, atol=1e-7)
+    np.testing.assert_allclose(feat[:, 5], np.hypot(U, V), rtol=1e-6, atol=1e-7)
+    nod = ctx.nodal()
+    assert set(np.unique(feat[:, 6])) <= {0.0, 1.0} and feat[:, 6].sum() > 0
+
+
+def test_mpnn_gradient_matches_oracle(ctx):
+    print("[gpu] MPNN gradient vs oracle", flush=True)
+    deck, B = _setup(ctx)
+    theta = _random_weights(11)
+    ctx.mpnn_set_weights(theta)
+    k = ctx.mpnn_add_sample(B, capacity=2)
+    assert k == 1
+    loss, g = ctx.mpnn_loss_grad(0)
+    _, feat = ctx.mpnn_forward(B)
+    ptr, adj = _graph(deck)
+    rp, cols = ctx.pattern()
+    A, _ = ctx.matrix(rmx.F64)
+    eq = ctx.numbering()
+    lo, go = mo.resid_loss_grad(theta.astype(np.float64), feat.astype(np.float64), B, rp, cols, A,
+                                eq, ptr, adj, bf=True)
+    gerr = np.linalg.norm(g - go) / np.linalg.norm(go)
+    cos = float(g @ go / (np.linalg.norm(g) * np.linalg.norm(go)))
+    print(f"    loss {loss:.6e} vs {lo:.6e}; gradient rel L2 err {gerr:.2e}, cosine {cos:.6f}")
+    assert abs(loss - lo) <= 1e-3 * lo
+    assert gerr <= 2e-2 and cos >= 0.9995
+    k0 = 0
+    for name, sh in mo.shapes():
+        m = int(np.prod(sh))
+        blk = slice(k0, k0 + m)
+        e = np.linalg.norm(g[blk] - go[blk]) / max(np.linalg.norm(go[blk]), 1e-30)
+        assert e <= 5e-2, (name, e)
+        k0 += m
+
+
+def test_training_gain_on_held_out_seed(ctx):
+    """Train on residual samples harvested from Newton steps of seeded bathymetries of
+    the channel generator; on a held-out seed, compare the two-level preconditioner with
+    and without the network: the residual a single application leaves (must shrink), and
+    the FGMRES solve to the same FP64 residual (iterations printed; the learned solve may
+    finish through its BiCGStab + two-level fallback)."""
+    print("[gpu] MPNN training", flush=True)
+    gen = dict(nx=120, ny=12, length=600.0, width=60.0, noise_corr=15.0, Q=60.0)
+    seeds = list(range(1, 7))
+    n = 0
+    for i, s in enumerate(seeds):
+        n = mpnn.harvest(ctx, dk.channel(seed=s, **gen), newton_steps=2, per_system=3,
+                         capacity=6 * len(seeds) if i == 0 else 0)
+    assert n == 6 * len(seeds)
+    ctx.mpnn_set_weights(mpnn.init_weights(0))
+    l0 = np.mean([ctx.mpnn_loss_grad(k)[0] for k in range(n)])
+    loss = mpnn.train(ctx, epochs=30, lr=1e-3)
+    print(f"    training loss {l0:.4e} -> {loss:.4e}")
+    assert loss < 0.9 * l0
+    # held-out seed
+    d = dk.channel(seed=0, **gen)
+    ctx.upload_mesh(d)
+    ctx.set_params()
+    ctx.prepare()
+    ctx.assemble(rmx.F64)
+    ctx.scale_l2()
+    ctx.convert(rmx.F32)
+    ctx.silu0_factor(rmx.F32)
+    ctx.coarse_factor(rmx.F32)
+    _, B = ctx.matrix(rmx.F64)
+    z2 = ctx.precond_apply(rmx.PREC_SILU0C, B, rmx.F32)
+    zl = ctx.precond_apply(rmx.PREC_MPNN, B, rmx.F32)
+    r2 = np.linalg.norm(B - ctx.spmv(z2, rmx.F64)) / np.linalg.norm(B)
+    rl = np.linalg.norm(B - ctx.spmv(zl, rmx.F64)) / np.linalg.norm(B)
+    print(f"    held-out seed: residual left by the preconditioner {r2:.3e} (two-level) -> "
+          f"{rl:.3e} (with network)", flush=True)
+    runs = {}
+    for pk in (rmx.PREC_SILU0C, rmx.PREC_MPNN):
+        out = ctx.solve(rmx.FGMRES, rmx.F32, pk, max_iter=2000, eps=1e-3, refine=True,
+                        refine_tol=1e-8, refine_max=20)
+        runs[pk] = out
+        print(f"    precond {pk}: FGMRES {out.iters} it, {out.sweeps} sweeps, converged "
+              f"{out.converged}, fallback {out.fallback}, FP64 residual {out.true_resid:.2e}, "
+              f"{out.seconds:.3f}s", flush=True)
+    # the network lowers the residual a single application leaves on the held-out system
+    assert rl < r2
+    # both solves reach the FP64 residual (the learned one through its fallback if it stalls)
+    for pk, out in runs.items():
+        assert out.converged and out.true_resid <= 1e-8, pk
+    assert runs[rmx.PREC_SILU0C].fallback == 0

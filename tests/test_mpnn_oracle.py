@@ -68,3 +68,30 @@ def test_bf16_rounding_is_nearest_even():
     assert r[2] == 1.0 + 2.0 ** -7              # above the tie rounds up
     assert r[3] == -2.5
     assert np.isfinite(r[4])
+
+
+def test_residual_loss_gradient_matches_central_differences():
+    n_nodes = 12
+    ptr, adj = _graph(n_nodes, 6)
+    rng = np.random.default_rng(7)
+    eq = -np.ones((n_nodes, 3), np.int64)
+    m = rng.random((n_nodes, 3)) < 0.85
+    eq[m] = np.arange(int(m.sum()))
+    n = int(m.sum())
+    dense = rng.standard_normal((n, n)) * (rng.random((n, n)) < 0.3) + 4 * np.eye(n)
+    rowptr = np.concatenate([[0], np.cumsum((dense != 0).sum(1))])
+    cols = np.concatenate([np.nonzero(dense[i])[0] for i in range(n)])
+    vals = dense[dense != 0]
+    theta = rng.standard_normal(mo.n_params()) * 0.2
+    feat = rng.standard_normal((n_nodes, mo.F))
+    r = rng.standard_normal(n)
+    loss, grad = mo.resid_loss_grad(theta, feat, r, rowptr, cols, vals, eq, ptr, adj)
+    idx = np.unique(np.concatenate([rng.choice(len(theta), 40, replace=False),
+                                    np.cumsum([0] + [int(np.prod(sh)) for _, sh in mo.shapes()])[:-1]]))
+    h = 1e-4          # the loss is ~5e4 here: smaller steps lose digits to cancellation
+    for k in idx:
+        tp = theta.copy(); tp[k] += h
+        tm = theta.copy(); tm[k] -= h
+        fd = (mo.resid_loss_grad(tp, feat, r, rowptr, cols, vals, eq, ptr, adj)[0]
+              - mo.resid_loss_grad(tm, feat, r, rowptr, cols, vals, eq, ptr, adj)[0]) / (2 * h)
+        assert abs(fd - grad[k]) <= 1e-6 * max(1.0, abs(fd)), (k, fd, grad[k])
