@@ -43,8 +43,8 @@ def parse():
     ap.add_argument("--n-along", type=int, default=3200)
     ap.add_argument("--n-across", type=int, default=320)
     ap.add_argument("--strips", type=int, default=0)
-    ap.add_argument("--segments", type=int, default=16)
-    ap.add_argument("--coarse", type=int, default=4, help="coarse segments per strip (0: none)")
+    ap.add_argument("--segments", type=int, default=32)
+    ap.add_argument("--coarse", type=int, default=8, help="coarse segments per strip (0: none)")
     ap.add_argument("--max-iter", type=int, default=1000, help="Krylov iterations per refinement sweep")
     ap.add_argument("--refine-max", type=int, default=8)
     ap.add_argument("--plain-iters", type=int, default=40)
@@ -200,7 +200,7 @@ def main():
                                "Newton step 1 from the initial state",
                    "solver": "FGMRES(30), FP32 matrix, coarse correction "
                              f"({a.coarse} segments/strip x U,V,H aggregates) + strip ILU(0) "
-                             f"({a.strips or 'per-CU'} strips x {a.segments} segments), "
+                             f"({a.strips or '2 per CU'} strips x {a.segments} segments), "
                              "FP64 iterative refinement to 1e-8",
                    "parallelism": "1 GPU"},
         "converged": bool(last.solve.converged),

@@ -123,9 +123,10 @@ rmx_status rmx_ilu0_apply(rmx_ctx* ctx, int prec, const double* b, double* x);
  * sweeps (PRECO_ILU0::Solve, Preco_ilu0.cpp:497, without the factor entries that couple two
  * blocks).  A block is one of `segments` parts of a strip of consecutive equations, cut
  * along the principal axis of the strip's node coordinates; every strip is swept by one
- * workgroup.  rmx_set_strips chooses the number of strips (0 = one per compute unit) and
- * segments (1 for a system loaded with rmx_csr_upload, which has no coordinates) and
- * rebuilds the schedule; rmx_get_blocks returns the block of every equation. */
+ * workgroup.  rmx_set_strips chooses the number of strips (0 = two per compute unit,
+ * the default) and segments (default 32; 1 for a system loaded with rmx_csr_upload, which
+ * has no coordinates) and rebuilds the schedule; rmx_get_blocks returns the block of every
+ * equation. */
 rmx_status rmx_silu0_factor(rmx_ctx* ctx, int prec);
 rmx_status rmx_silu0_apply(rmx_ctx* ctx, int prec, const double* b, double* x);
 rmx_status rmx_set_strips(rmx_ctx* ctx, int strips, int segments);
@@ -135,7 +136,7 @@ rmx_status rmx_get_blocks(rmx_ctx* ctx, int* block /* [neq] */);
  * per (strip, coarse segment, variable U/V/H), coarse segments cut like the sweep
  * segments; P is piecewise constant on aggregates and R = P^T.  Apply:
  * z1 = P (R A P)^-1 R r, z = z1 + M^-1 (r - A z1) with M the strip ILU(0).
- * rmx_set_coarse chooses the coarse segments per strip (0: no coarse space; needs a mesh,
+ * rmx_set_coarse chooses the coarse segments per strip (default 8; 0: no coarse space; needs a mesh,
  * i.e. not after rmx_csr_upload); rmx_coarse_factor sums R A P from the FP32 or FP64
  * matrix and factors it; rmx_get_aggregates writes the aggregate of every equation and
  * returns the number of aggregates in *nc. */

@@ -7,7 +7,9 @@
 //
 //     z1 = P (R A P)^-1 R r,    z = z1 + M^-1 (r - A z1)        (M: strip ILU(0))
 //
-// i.e. a multiplicative coarse correction followed by the strip sweeps.  R A P is summed on
+// i.e. a multiplicative coarse correction followed by the strip sweeps.  A z1 = (A P) zc
+// with A P kept per row (a few aggregates per row), so the correction costs a pass over A P
+// instead of a second SpMV.  R A P is summed on
 // the device in a fixed order (every coarse entry is the sum of the matrix entries whose row
 // and column aggregates match, listed by a permutation built once per pattern).  Aggregates
 // are numbered strip by strip, so R A P is banded; its LU factorisation with partial
@@ -54,26 +56,36 @@ __global__ void __launch_bounds__(256) k_restrict(int nc, const int* __restrict_
   if (lane == 0) rc[I] = v;
 }
 
-// z1 = P zc
-__global__ void k_prolong(int n, const int* __restrict__ agg, const double* __restrict__ zc,
-                          double* __restrict__ z1)
+// apv[k] = sum of A over the entries of (row, aggregate) pair k (fixed order)
+template <class VT>
+__global__ void k_ap_vals(long long nap, const int* __restrict__ src_ptr, const int* __restrict__ src,
+                          const VT* __restrict__ A, double* __restrict__ apv)
 {
-  const int i = blockIdx.x * blockDim.x + threadIdx.x;
-  if (i < n) z1[i] = zc[agg[i]];
+  const long long k = (long long)blockIdx.x * blockDim.x + threadIdx.x;
+  if (k >= nap) return;
+  double v = 0.0;
+  for (int q = src_ptr[k]; q < src_ptr[k + 1]; q++) v += (double)A[src[q]];
+  apv[k] = v;
 }
 
-// t = r - t
-__global__ void k_rsub(int n, const double* __restrict__ r, double* __restrict__ t)
+// t = r - (A P) zc
+__global__ void k_ap_resid(int n, const int* __restrict__ ap_ptr, const int* __restrict__ ap_agg,
+                           const double* __restrict__ apv, const double* __restrict__ zc,
+                           const double* __restrict__ r, double* __restrict__ t)
 {
   const int i = blockIdx.x * blockDim.x + threadIdx.x;
-  if (i < n) t[i] = r[i] - t[i];
+  if (i >= n) return;
+  double s = 0.0;
+  for (int k = ap_ptr[i]; k < ap_ptr[i + 1]; k++) s += apv[k] * zc[ap_agg[k]];
+  t[i] = r[i] - s;
 }
 
-// z += z1
-__global__ void k_add(int n, const double* __restrict__ z1, double* __restrict__ z)
+// z += P zc
+__global__ void k_add_prolong(int n, const int* __restrict__ agg, const double* __restrict__ zc,
+                              double* __restrict__ z)
 {
   const int i = blockIdx.x * blockDim.x + threadIdx.x;
-  if (i < n) z[i] += z1[i];
+  if (i < n) z[i] += zc[agg[i]];
 }
 
 // ---------------------------------------------------------------------------------------
@@ -164,20 +176,28 @@ void coarse_setup(Ctx& c, const std::vector<int>& rp, const std::vector<int>& cl
     if (idx[k] == 0) idx[k] = nc++;
   std::vector<int> agg(n);
   for (int i = 0; i < n; i++) agg[i] = idx[raw[i]];
-  if ((long long)nc * nc > (1LL << 26))
-    fail(RMX_ERR_RANGE, "coarse space too large (" + std::to_string(nc) + " aggregates)");
-  // coarse entries (I, J) in row-major order and the matrix entries summed into each
-  std::vector<int> key_slot((size_t)nc * nc, -1);
-  for (int i = 0; i < n; i++)
-    for (int q = rp[i]; q < rp[i + 1]; q++) key_slot[(size_t)agg[i] * nc + agg[cl[q]]] = 0;
-  C.slot_row.clear(); C.slot_col.clear();
+  // band of the coarse matrix, then coarse entries (I, J) in row-major order via a map
+  // indexed by (I, J - I + kl)
   int kl = 0, ku = 0;
+  for (int i = 0; i < n; i++)
+    for (int q = rp[i]; q < rp[i + 1]; q++) {
+      const int d = agg[cl[q]] - agg[i];
+      kl = std::max(kl, -d); ku = std::max(ku, d);
+    }
+  const long long bw = (long long)kl + ku + 1;
+  if ((long long)nc * bw > (1LL << 28))
+    fail(RMX_ERR_RANGE, "coarse space too large (" + std::to_string(nc) + " aggregates, band " +
+                        std::to_string(bw) + ")");
+  std::vector<int> key_slot((size_t)nc * bw, -1);
+  auto key = [&](int I, int J) { return (size_t)I * bw + (size_t)(J - I + kl); };
+  for (int i = 0; i < n; i++)
+    for (int q = rp[i]; q < rp[i + 1]; q++) key_slot[key(agg[i], agg[cl[q]])] = 0;
+  C.slot_row.clear(); C.slot_col.clear();
   for (size_t k = 0; k < key_slot.size(); k++)
     if (key_slot[k] == 0) {
       key_slot[k] = (int)C.slot_row.size();
-      const int r = (int)(k / nc), col = (int)(k % nc);
+      const int r = (int)(k / bw), col = r + (int)(k % bw) - kl;
       C.slot_row.push_back(r); C.slot_col.push_back(col);
-      kl = std::max(kl, r - col); ku = std::max(ku, col - r);
     }
   const long long nslot = (long long)C.slot_row.size();
   const long long nnz = rp[n];
@@ -185,7 +205,7 @@ void coarse_setup(Ctx& c, const std::vector<int>& rp, const std::vector<int>& cl
   std::vector<int> slot_of(nnz);
   parallel_for(n, [&](long long a, long long e) {
     for (long long i = a; i < e; i++)
-      for (int q = rp[i]; q < rp[i + 1]; q++) slot_of[q] = key_slot[(size_t)agg[i] * nc + agg[cl[q]]];
+      for (int q = rp[i]; q < rp[i + 1]; q++) slot_of[q] = key_slot[key(agg[i], agg[cl[q]])];
   });
   for (long long q = 0; q < nnz; q++) sptr[slot_of[q] + 1]++;
   for (long long s = 0; s < nslot; s++) sptr[s + 1] += sptr[s];
@@ -202,11 +222,46 @@ void coarse_setup(Ctx& c, const std::vector<int>& rp, const std::vector<int>& cl
     std::vector<int> fill(aptr.begin(), aptr.end() - 1);
     for (int i = 0; i < n; i++) arows[fill[agg[i]]++] = i;
   }
+  // A P structure: per row its distinct column aggregates (ascending) and the row's entries
+  // grouped per aggregate
+  std::vector<int> apc(n + 1, 0);
+  parallel_for(n, [&](long long a, long long e) {
+    std::vector<int> tmp;
+    for (long long i = a; i < e; i++) {
+      tmp.clear();
+      for (int q = rp[i]; q < rp[i + 1]; q++) tmp.push_back(agg[cl[q]]);
+      std::sort(tmp.begin(), tmp.end());
+      apc[i + 1] = (int)(std::unique(tmp.begin(), tmp.end()) - tmp.begin());
+    }
+  });
+  for (int i = 0; i < n; i++) apc[i + 1] += apc[i];
+  const long long nap = apc[n];
+  std::vector<int> ap_agg(nap), src_ptr(nap + 1), src(rp[n]);
+  parallel_for(n, [&](long long a, long long e) {
+    std::vector<std::pair<int, int>> tmp;
+    for (long long i = a; i < e; i++) {
+      tmp.clear();
+      for (int q = rp[i]; q < rp[i + 1]; q++) tmp.emplace_back(agg[cl[q]], q);
+      std::stable_sort(tmp.begin(), tmp.end(),
+                       [](const std::pair<int, int>& x, const std::pair<int, int>& y) { return x.first < y.first; });
+      long long k = apc[i] - 1;
+      int pos = rp[i];
+      for (size_t t = 0; t < tmp.size(); t++) {
+        if (t == 0 || tmp[t].first != tmp[t - 1].first) { k++; ap_agg[k] = tmp[t].first; src_ptr[k] = pos; }
+        src[pos++] = tmp[t].second;
+      }
+    }
+  });
+  src_ptr[nap] = rp[n];
+  C.nap = nap;
+  C.ap_ptr.upload(apc, c.stream); C.ap_agg.upload(ap_agg, c.stream);
+  C.ap_src_ptr.upload(src_ptr, c.stream); C.ap_src.upload(src, c.stream);
+  C.apv.alloc(nap);
   C.nc = nc; C.kl = kl; C.ku = ku; C.nslot = nslot;
   hipStream_t s = c.stream;
   C.agg.upload(agg, s); C.agg_ptr.upload(aptr, s); C.agg_rows.upload(arows, s);
   C.slot_ptr.upload(sptr, s); C.perm.upload(perm, s);
-  C.cval.alloc(nslot); C.rc.alloc(nc); C.zc.alloc(nc); C.z1.alloc(n); C.t.alloc(n);
+  C.cval.alloc(nslot); C.rc.alloc(nc); C.zc.alloc(nc); C.t.alloc(n);
   if (C.h_buf) { (void)hipHostFree(C.h_buf); C.h_buf = nullptr; }
   HIPCHECK(hipHostMalloc(&C.h_buf, sizeof(double) * std::max<long long>(nc, nslot),
                          hipHostMallocDefault));
@@ -222,12 +277,15 @@ void coarse_factor(Ctx& c, int prec)
   Coarse& C = c.coarse;
   if (!C.ready) fail(RMX_ERR_ARG, "coarse space: no schedule (set up from a mesh)");
   const int blocks = (int)((C.nslot * 64 + 255) / 256);
+  const int gap = (int)((C.nap + 255) / 256);
   if (prec == RMX_F32) {
     if (!c.have32) fail(RMX_ERR_ARG, "coarse space: no FP32 matrix");
     k_coarse_vals<float><<<blocks, 256, 0, c.stream>>>(C.nslot, C.slot_ptr.p, C.perm.p, c.A32.p, C.cval.p);
+    k_ap_vals<float><<<gap, 256, 0, c.stream>>>(C.nap, C.ap_src_ptr.p, C.ap_src.p, c.A32.p, C.apv.p);
   } else {
     if (!c.have64) fail(RMX_ERR_ARG, "coarse space: no FP64 matrix");
     k_coarse_vals<double><<<blocks, 256, 0, c.stream>>>(C.nslot, C.slot_ptr.p, C.perm.p, c.A64.p, C.cval.p);
+    k_ap_vals<double><<<gap, 256, 0, c.stream>>>(C.nap, C.ap_src_ptr.p, C.ap_src.p, c.A64.p, C.apv.p);
   }
   HIPCHECK(hipGetLastError());
   HIPCHECK(hipMemcpyAsync(C.h_buf, C.cval.p, sizeof(double) * C.nslot, hipMemcpyDeviceToHost, c.stream));
@@ -253,11 +311,9 @@ void coarse_apply(Ctx& c, int prec, const double* r, double* z)
   HIPCHECK(hipStreamSynchronize(c.stream));
   band_solve(C.nc, C.kl, C.ku, C.band, C.ipiv, C.h_buf);
   HIPCHECK(hipMemcpyAsync(C.zc.p, C.h_buf, sizeof(double) * C.nc, hipMemcpyHostToDevice, c.stream));
-  k_prolong<<<g, 256, 0, c.stream>>>(n, C.agg.p, C.zc.p, C.z1.p);
-  dev_spmv(c, prec, C.z1.p, C.t.p);
-  k_rsub<<<g, 256, 0, c.stream>>>(n, r, C.t.p);
+  k_ap_resid<<<g, 256, 0, c.stream>>>(n, C.ap_ptr.p, C.ap_agg.p, C.apv.p, C.zc.p, r, C.t.p);
   ilu_apply(c, c.silu, prec, C.t.p, z, nullptr);
-  k_add<<<g, 256, 0, c.stream>>>(n, C.z1.p, z);
+  k_add_prolong<<<g, 256, 0, c.stream>>>(n, C.agg.p, C.zc.p, z);
   HIPCHECK(hipGetLastError());
 }
 

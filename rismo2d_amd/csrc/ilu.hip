@@ -56,18 +56,20 @@ constexpr int UCAP = 1024;      // entries of the pivot rows staged in LDS per r
 constexpr int UPL = UCAP / 64;  // staged entries per lane
 
 // One row of one level (one wave): row i eliminates its lower entries k in increasing
-// column order (IKJ).  The pivot rows are first staged in LDS with independent loads, and
-// every staged entry gets its target position in row i (a binary search over row i's
-// sorted columns, done in parallel), so the sequential loop over k is one division and
-// one LDS update per entry.  The updates are the reference's, in its order, with FMA
-// contraction off.  Rows whose pivot rows hold more than UCAP entries search and read
-// from global memory inside the loop instead.
+// column order (IKJ).  Per-pattern metadata keeps the dependent memory rounds to three: the
+// row's descriptor {i, start, width, lower count}; the row (values, sorted columns) with
+// the start and length of every pivot row; then all pivot rows at once, staged in LDS with
+// independent loads, every staged entry getting its target position in row i (a binary
+// search over row i's sorted columns, done in parallel).  The sequential loop over k is
+// then one division and one LDS update per entry.  The updates are the reference's, in its
+// order, with FMA contraction off.  Rows whose pivot rows hold more than UCAP entries
+// search and read from global memory inside the loop instead.
 template <class VT>
-__global__ void __launch_bounds__(64) k_ilu_level(const int* __restrict__ rows, int cnt,
-                                                  const int* __restrict__ rowptr,
+__global__ void __launch_bounds__(64) k_ilu_level(const int4* __restrict__ desc, int cnt,
                                                   const int* __restrict__ cols,
                                                   const int* __restrict__ srt_cols,
                                                   const unsigned char* __restrict__ srt_pos,
+                                                  const int2* __restrict__ piv,
                                                   VT* __restrict__ LU, int* __restrict__ err)
 {
 #pragma clang fp contract(off)
@@ -78,38 +80,36 @@ __global__ void __launch_bounds__(64) k_ilu_level(const int* __restrict__ rows, 
   __shared__ VT s_dk[FW];
   __shared__ unsigned char s_upos[UCAP];
   __shared__ VT s_uval[UCAP];
-  __shared__ int s_nk;
   const int lane = threadIdx.x;
   const int w = blockIdx.x;
   if (w >= cnt) return;                 // uniform per block
-  const int i = rows[w];
-  const int p0 = rowptr[i];
-  const int wi = rowptr[i + 1] - p0;
-  if (lane == 0) s_nk = 0;
+  const int4 d = desc[w];
+  const int i = d.x, p0 = d.y, wi = d.z, nk = d.w;
   for (int t = lane; t < wi; t += 64) {
     s_val[t] = LU[p0 + t];
     s_col[t] = srt_cols[p0 + t];
     s_pos[t] = srt_pos[p0 + t];
   }
-  __syncthreads();
-  // number of lower entries: the sorted columns below i
-  for (int t = lane; t < wi; t += 64)
-    if (s_col[t] < i && (t + 1 == wi || s_col[t + 1] >= i)) s_nk = t + 1;
-  __syncthreads();
-  const int nk = s_nk;
-  // pivot rows: start, diagonal, number of entries after the diagonal
-  for (int t = lane; t < nk; t += 64) {
-    const int k = s_col[t];
-    const int kb = rowptr[k], ke = rowptr[k + 1];
-    s_kb[t] = kb;
-    s_off[t + 1] = ke - kb - 1;
-    s_dk[t] = LU[kb];
+  // pivot rows (the first nk sorted columns): start and entries after the diagonal;
+  // offsets of their staged entries by an inclusive wave scan
+  int carry = 0;
+  for (int t0 = 0; t0 < nk; t0 += 64) {
+    const int t = t0 + lane;
+    int len = 0;
+    if (t < nk) {
+      const int2 pv = piv[p0 + t];
+      s_kb[t] = pv.x;
+      len = pv.y;
+    }
+    int incl = len;
+    for (int o = 1; o < 64; o <<= 1) {
+      const int v = __shfl_up(incl, o, 64);
+      if (lane >= o) incl += v;
+    }
+    if (t < nk) s_off[t + 1] = carry + incl;
+    carry += __shfl(incl, 63, 64);
   }
-  __syncthreads();
-  if (lane == 0) {
-    s_off[0] = 0;
-    for (int t = 0; t < nk; t++) s_off[t + 1] += s_off[t];
-  }
+  if (lane == 0) s_off[0] = 0;
   __syncthreads();
   // target position in row i of a column cc of pivot row k (255: not in row i, or cc <= k)
   auto target = [&](int cc, int k) -> int {
@@ -125,6 +125,7 @@ __global__ void __launch_bounds__(64) k_ilu_level(const int* __restrict__ rows, 
   };
   const int total = s_off[nk];
   const bool staged = total <= UCAP;
+  for (int t = lane; t < nk; t += 64) s_dk[t] = LU[s_kb[t]];
   if (staged) {
     int uc[UPL], ut[UPL];
     VT uv[UPL];
@@ -223,6 +224,9 @@ __device__ __forceinline__ double ring_or_global(const double* ring, const doubl
 // Block b of the sweep: levels [blev[b], blev[b+1]) of the concatenated level table, rows
 // at positions [bpos[b], bpos[b+1]).  Level offsets, columns and ring slots are local to
 // the block; each block table has one extra sentinel level offset.
+// The right-hand side of position r is gsrc[gidx[r]] (the input vector gathered into
+// processing order on the fly); with oidx the solution of position r is also written to
+// odst[oidx[r]] (scattered back to equation order).
 template <class VT, bool LOWER, int G, int KR>
 __global__ void __launch_bounds__(1024) k_sweep(const int* __restrict__ blev,
                                                 const int* __restrict__ bpos,
@@ -231,7 +235,10 @@ __global__ void __launch_bounds__(1024) k_sweep(const int* __restrict__ blev,
                                                 const int* __restrict__ pcol,
                                                 const VT* __restrict__ pval,
                                                 const VT* __restrict__ pdiag_all,
-                                                const double* __restrict__ pin_all,
+                                                const int* __restrict__ gidx_all,
+                                                const double* __restrict__ gsrc,
+                                                const int* __restrict__ oidx_all,
+                                                double* __restrict__ odst,
                                                 double* __restrict__ xp_all)
 {
   __shared__ double ring[RING];
@@ -243,7 +250,8 @@ __global__ void __launch_bounds__(1024) k_sweep(const int* __restrict__ blev,
   const int base = bpos[b];
   const int* prs = prs_all + base;
   const VT* pdiag = LOWER ? nullptr : pdiag_all + base;
-  const double* pin = pin_all + base;
+  const int* gidx = gidx_all + base;
+  const int* oidx = oidx_all ? oidx_all + base : nullptr;
   double* xp = xp_all + base;
   const int t = threadIdx.x;
   const int grp = t / G, lane = t % G, ngr = blockDim.x / G;
@@ -266,7 +274,7 @@ __global__ void __launch_bounds__(1024) k_sweep(const int* __restrict__ blev,
       const int g = grp + k * ngr;
       if (g < nr_) {
         const int r = P_ + g;
-        inr[k] = pin[r];
+        inr[k] = gsrc[gidx[r]];
         if (!LOWER) dgr[k] = pdiag[r];
         qar[k] = prs[r];
         qbr[k] = prs[r + 1];
@@ -315,7 +323,7 @@ __global__ void __launch_bounds__(1024) k_sweep(const int* __restrict__ blev,
             s += (double)pval[q] * ring_or_global(ring, xp, pcol[q], lim);
         } else {
           const int r = P + g;
-          in = pin[r];
+          in = gsrc[gidx[r]];
           if (!LOWER) dg = pdiag[r];
           for (int q = prs[r] + lane; q < prs[r + 1]; q += G)
             s += (double)pval[q] * ring_or_global(ring, xp, pcol[q], lim);
@@ -327,6 +335,7 @@ __global__ void __launch_bounds__(1024) k_sweep(const int* __restrict__ blev,
         const double xi = LOWER ? in + s : (in - s) / (double)dg;
         ring[(P + g) & (RING - 1)] = xi;
         xp[P + g] = xi;
+        if (oidx) odst[oidx[P + g]] = xi;
       }
     }
     const int Pn = P + nr;
@@ -346,20 +355,6 @@ __global__ void __launch_bounds__(1024) k_sweep(const int* __restrict__ blev,
     if (full) __syncthreads();
     else lds_barrier();
   }
-}
-
-__global__ void k_gather(int n, const int* __restrict__ idx, const double* __restrict__ src,
-                         double* __restrict__ dst)
-{
-  const int r = blockIdx.x * blockDim.x + threadIdx.x;
-  if (r < n) dst[r] = src[idx[r]];
-}
-
-__global__ void k_scatter(int n, const int* __restrict__ idx, const double* __restrict__ src,
-                          double* __restrict__ dst)
-{
-  const int r = blockIdx.x * blockDim.x + threadIdx.x;
-  if (r < n) dst[idx[r]] = src[r];
 }
 
 // ---------------------------------------------------------------------------------------
@@ -465,7 +460,7 @@ static void build_sweep(SweepSet& S, const Ilu& I, bool lower, std::vector<int>&
             nlev_tot, (double)n / std::max(1, blev[nb]), maxnr, ne);
   S.blev.upload(blev, s); S.bpos.upload(bpos, s); S.lev_ptr.upload(levp, s);
   S.prs.upload(prs, s); S.prow.upload(prow, s); S.pcol.upload(pcol, s); S.psrc.upload(psrc, s);
-  S.pin.alloc(n); S.xp.alloc(n);
+  S.xp.alloc(n);
   order = prow;
 }
 
@@ -566,6 +561,27 @@ void ilu_setup(Ilu& I, int n, const std::vector<int>& rp, const std::vector<int>
   });
   I.rp.upload(I.h_rp, s); I.cols.upload(I.h_cols, s);
   I.srt_cols.upload(scol, s); I.srt_pos.upload(spos, s);
+  // factorisation metadata: per level position {row, start, width, lower entries}; per
+  // sorted lower entry the start and length (after the diagonal) of its pivot row
+  {
+    std::vector<int4> desc(n);
+    std::vector<int2> piv(I.nnz, int2{0, 0});
+    parallel_for(n, [&](long long a, long long e) {
+      for (long long w = a; w < e; w++) {
+        const int i = frows[w];
+        const int p0 = I.h_rp[i], wi = I.h_rp[i + 1] - p0;
+        int nk = 0;
+        while (nk < wi && scol[p0 + nk] < i) {
+          const int k = scol[p0 + nk];
+          piv[p0 + nk] = int2{ I.h_rp[k], I.h_rp[k + 1] - I.h_rp[k] - 1 };
+          nk++;
+        }
+        desc[w] = int4{ i, p0, wi, nk };
+      }
+    });
+    I.fdesc.upload(desc, s);
+    I.fpiv.upload(piv, s);
+  }
   std::vector<int> olo, oup;
   build_sweep(I.lo, I, true, olo, s);
   build_sweep(I.up, I, false, oup, s);
@@ -604,8 +620,8 @@ static void factor_t(Ctx& c, Ilu& I, const VT* A)
   HIPCHECK(hipMemsetAsync(err, 0, sizeof(int), c.stream));
   for (size_t l = 0; l + 1 < I.flev.size(); l++) {
     const int a = I.flev[l], cnt = I.flev[l + 1] - a;
-    k_ilu_level<VT><<<cnt, 64, 0, c.stream>>>(I.flev_rows.p + a, cnt, I.rp.p, I.cols.p,
-                                              I.srt_cols.p, I.srt_pos.p, LU.p, err);
+    k_ilu_level<VT><<<cnt, 64, 0, c.stream>>>(I.fdesc.p + a, cnt, I.cols.p, I.srt_cols.p,
+                                              I.srt_pos.p, I.fpiv.p, LU.p, err);
   }
   HIPCHECK(hipGetLastError());
   DBuf<VT>& pl = PLbuf<VT>(I);
@@ -641,17 +657,14 @@ void ilu_factor(Ctx& c, Ilu& I, int prec)
 template <class VT>
 static void apply_t(Ctx& c, Ilu& I, const double* b, double* x, hipEvent_t* ev)
 {
-  const int gb = (I.n + 255) / 256;
   const int nb = I.nblk;
   if (ev) HIPCHECK(hipEventRecord(ev[0], c.stream));
-  k_gather<<<gb, 256, 0, c.stream>>>(I.n, I.lo.prow.p, b, I.lo.pin.p);
   k_sweep<VT, true, SG, SKR><<<nb, 1024, 0, c.stream>>>(I.lo.blev.p, I.lo.bpos.p, I.lo.lev_ptr.p,
-      I.lo.prs.p, I.lo.pcol.p, PLbuf<VT>(I).p, nullptr, I.lo.pin.p, I.lo.xp.p);
-  k_gather<<<gb, 256, 0, c.stream>>>(I.n, I.fwd2bwd.p, I.lo.xp.p, I.up.pin.p);
+      I.lo.prs.p, I.lo.pcol.p, PLbuf<VT>(I).p, nullptr, I.lo.prow.p, b, nullptr, nullptr, I.lo.xp.p);
   if (ev) HIPCHECK(hipEventRecord(ev[1], c.stream));
   k_sweep<VT, false, SG, SKR><<<nb, 1024, 0, c.stream>>>(I.up.blev.p, I.up.bpos.p, I.up.lev_ptr.p,
-      I.up.prs.p, I.up.pcol.p, PUbuf<VT>(I).p, PDbuf<VT>(I).p, I.up.pin.p, I.up.xp.p);
-  k_scatter<<<gb, 256, 0, c.stream>>>(I.n, I.up.prow.p, I.up.xp.p, x);
+      I.up.prs.p, I.up.pcol.p, PUbuf<VT>(I).p, PDbuf<VT>(I).p, I.fwd2bwd.p, I.lo.xp.p, I.up.prow.p, x,
+      I.up.xp.p);
   if (ev) HIPCHECK(hipEventRecord(ev[2], c.stream));
   HIPCHECK(hipGetLastError());
 }
@@ -681,10 +694,11 @@ void setup_ilu_schedule(Ctx& c)
   const std::vector<int>& rp = c.csr_only ? hrp : c.hm.rowptr;
   const std::vector<int>& cl = c.csr_only ? hcols : c.hm.cols;
   ilu_setup(c.ilu, c.n, rp, cl, 1, nullptr, c.stream, nullptr, 1);
-  // strip ILU: one strip per compute unit, at least a few thousand rows each
+  // strip ILU: two strips per compute unit (two sweep workgroups fit one CU's LDS), at
+  // least a few thousand rows each
   hipDeviceProp_t prop;
   HIPCHECK(hipGetDeviceProperties(&prop, c.device));
-  int nblk = c.strips > 0 ? c.strips : prop.multiProcessorCount;
+  int nblk = c.strips > 0 ? c.strips : 2 * prop.multiProcessorCount;
   nblk = std::max(1, std::min(nblk, c.n / 2048));
   std::vector<int> group;
   if (!c.csr_only) {

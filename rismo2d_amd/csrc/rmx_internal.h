@@ -113,7 +113,7 @@ struct SweepSet {
   DBuf<int> blev, bpos;         // per strip: first level, first position
   DBuf<int> lev_ptr;            // per strip: local level offsets + sentinel
   DBuf<int> prs, prow, pcol, psrc;
-  DBuf<double> pin, xp;
+  DBuf<double> xp;
 };
 
 // One ILU(0) system: pattern, factorisation schedule, sweeps and factors.
@@ -125,6 +125,8 @@ struct Ilu {
   std::vector<int> bstart, h_rp, h_cols, flev;
   DBuf<int> rp, cols, srt_cols, flev_rows, fwd2bwd;
   DBuf<unsigned char> srt_pos;
+  DBuf<int4> fdesc;             // factorisation: per level position {row, start, width, lower}
+  DBuf<int2> fpiv;              // per sorted lower entry: pivot row start, entries after diag
   SweepSet lo, up;
   DBuf<double> LU64, plo64, pup64, pdg64;
   DBuf<float> LU32, plo32, pup32, pdg32;
@@ -142,7 +144,12 @@ struct Coarse {
   DBuf<int> agg, agg_ptr, agg_rows;           // aggregate of every row; rows of each aggregate
   DBuf<long long> slot_ptr;                   // entries of A summed into each slot
   DBuf<int> perm;
-  DBuf<double> cval, rc, zc, z1, t;
+  DBuf<double> cval, rc, zc, t;
+  // A P: per row the aggregates of its columns and the summed entries (ap_ptr / ap_agg /
+  // apv), summed from the entries listed by ap_src (grouped per (row, aggregate) pair)
+  long long nap = 0;
+  DBuf<int> ap_ptr, ap_agg, ap_src_ptr, ap_src;
+  DBuf<double> apv;
   std::vector<double> band;                   // LU factors, band storage (LAPACK layout)
   std::vector<int> ipiv;
   double* h_buf = nullptr;                    // pinned, max(nc, nslot) doubles
@@ -204,11 +211,11 @@ struct Ctx {
   unsigned long long topo_hash = 0;           // fingerprint of the mesh topology
   Ilu ilu;                                    // reference-order ILU(0)
   Ilu silu;                                   // strip ILU(0)
-  int strips = 0;                             // 0: one strip per compute unit
-  int segments = 16;                          // cross-strip cuts of the strip sweeps
+  int strips = 0;                             // 0: two strips per compute unit
+  int segments = 32;                          // cross-strip cuts of the strip sweeps
   Coarse coarse;
   Mpnn mpnn;
-  int coarse_segments = 4;                    // cross-strip cuts of the coarse aggregates
+  int coarse_segments = 8;                    // cross-strip cuts of the coarse aggregates
   // Krylov work vectors
   DBuf<double> w[10];
   DBuf<double> red;                           // reduction scratch
