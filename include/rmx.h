@@ -62,6 +62,8 @@ typedef struct {
   int n_outlet;               /* OUTLET control-line segments */
   const int* outlet_segs;
   double Q, S_in, S_out;      /* inlet discharge, inlet reference level, outlet level */
+  const double* veg_dp;       /* [ne] emergent vegetation stem diameter (m), nullable */
+  const double* veg_sp;       /* [ne] stem spacing (m), nullable; dp or sp < 1e-4: none */
 } rmx_mesh_desc;
 
 typedef struct {
@@ -71,7 +73,9 @@ typedef struct {
 } rmx_params;
 
 /* Mesh, boundary lines and material data (MODEL::Input, MODEL::Initialize,
- * Model.cpp:121-215; GRID::AreaFactors, ArFact.cpp:41).  Builds the boundary line
+ * Model.cpp:121-215; GRID::AreaFactors, ArFact.cpp:41).  Vegetation enters the nodal
+ * friction after Lindner (TYPE::bottom, Friction.cpp:101-108; TYPE::lindner,
+ * Lindner.cpp:72-262).  Builds the boundary line
  * elements and the model element list on the host and uploads node/element arrays. */
 rmx_status rmx_mesh_upload(rmx_ctx* ctx, const rmx_mesh_desc* mesh);
 rmx_status rmx_set_params(rmx_ctx* ctx, const rmx_params* p);

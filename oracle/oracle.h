@@ -39,6 +39,10 @@ orc_model* orc_create(int np, const double* x, const double* y, const double* z,
                       const double* manning, const double* vt,
                       int n_in, const int* in_segs, int n_out, const int* out_segs,
                       double Q, double S_in, double S_out);
+/* Emergent vegetation per element (stem diameter dp, spacing sp; 0: none), used by the
+ * nodal friction (Lindner).  NULL arrays remove it. */
+void   orc_set_vegetation(orc_model* m, const double* dp, const double* sp);
+double orc_lindner(double dp, double sp, double va, double ha, double cb, double vk, double g);
 void   orc_destroy(orc_model* m);
 /* relaxMethod (0|2), dry method (0|2|3 marsh, else clip), dryLimit, maxDeltaUV, maxDeltaS,
  * convUV, convS, maxUs */

@@ -38,6 +38,9 @@ def lib():
         L.orc_create.argtypes = [C.c_int, _d, _d, _d, C.c_int, _i, _i, _d, _d, C.c_int, _i,
                                  C.c_int, _i, C.c_double, C.c_double, C.c_double]
         L.orc_destroy.argtypes = [C.c_void_p]
+        L.orc_set_vegetation.argtypes = [C.c_void_p, _d, _d]
+        L.orc_lindner.argtypes = [C.c_double] * 7
+        L.orc_lindner.restype = C.c_double
         L.orc_set_params.argtypes = [C.c_void_p, C.c_int, C.c_int] + [C.c_double] * 6
         L.orc_set_state.argtypes = [C.c_void_p, _d, _d, _d]
         L.orc_get_state.argtypes = [C.c_void_p, _d, _d, _d, _d]
@@ -105,6 +108,10 @@ class OracleModel:
                               deck.n_elems, _p(k["el"], _i), _p(k["nnd"], _i), _p(k["man"], _d),
                               _p(k["vt"], _d), len(k["inl"]), _p(k["inl"], _i), len(k["out"]),
                               _p(k["out"], _i), deck.Q_inlet, deck.S_inlet, deck.S_outlet)
+        if getattr(deck, "veg_dp", None) is not None:
+            k["vdp"] = np.ascontiguousarray(deck.veg_dp, np.float64)
+            k["vsp"] = np.ascontiguousarray(deck.veg_sp, np.float64)
+            L.orc_set_vegetation(self.h, _p(k["vdp"], _d), _p(k["vsp"], _d))
         self.set_state(deck.U0, deck.V0, deck.S0)
 
     def __del__(self):

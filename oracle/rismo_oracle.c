@@ -199,6 +199,7 @@ struct orc_model {
   double *x, *y, *z, *zor, *U, *V, *S, *dUdt, *dVdt, *dSdt, *cf;
   int *elem, *nnd;
   double *manning, *vt, *areaFact;
+  double *veg_dp, *veg_sp;          /* emergent vegetation per element (0: none) */
   int n_in, n_out;
   int *in_segs, *out_segs;
   double Q, S_in, S_out;
@@ -343,7 +344,7 @@ void orc_destroy(orc_model* m)
   void* ps[] = { m->x, m->y, m->z, m->zor, m->U, m->V, m->S, m->dUdt, m->dVdt, m->dSdt, m->cf,
                  m->nx, m->ny, m->niox, m->nioy, m->valU, m->valS, m->elem, m->nnd, m->manning,
                  m->vt, m->areaFact, m->in_segs, m->out_segs, m->noel, m->flag, m->kind, m->rot,
-                 m->eqno, m->bnd, m->bflag, m->me, m->rowptr, m->cols };
+                 m->eqno, m->bnd, m->bflag, m->me, m->rowptr, m->cols, m->veg_dp, m->veg_sp };
   for (size_t i = 0; i < sizeof(ps) / sizeof(ps[0]); i++) free(ps[i]);
   free(m);
 }
@@ -409,9 +410,106 @@ void orc_get_nodal(const orc_model* m, double* cf, int* bckind, int* rot, double
 /* ------------------------------------------------------------------------------------- */
 /* nodal coefficients                                                                     */
 
+/* Drag coefficient of one cylinder from its Reynolds number (TYPE::dragCoeff,
+ * Lindner.cpp:264-278). */
+static double drag_cylinder(double v, double d, double vk)
+{
+  double re = v * d / vk;
+  if (re <= 800.0) return 3.07 / pow(re, 0.168);
+  if (re <= 6000.0) return 1.0;
+  if (re <= 11000.0) return 1.0 + 0.2 * (re - 6000.0) / 5000.0;
+  return 1.2;
+}
+
+/* Real roots of a x^3 + b x^2 + c x + d (Cardano / trigonometric form, as
+ * TYPE::cubeEquation, Lindner.cpp:281-335); returns their number. */
+static int cubic_roots(double a, double b, double c, double d, double x[3])
+{
+  const double pi = 3.141592653589793;
+  double ba = b / a / 3.0, ca = c / a;
+  double p = ca / 3.0 - ba * ba;
+  double q = ba * ba * ba - ba * ca / 2.0 + d / a / 2.0;
+  double disc = q * q + p * p * p;
+  if (disc > 0.0) {
+    double t = -q + sqrt(disc);
+    double u = (t / fabs(t)) * pow(fabs(t), 1.0 / 3.0);
+    t = -q - sqrt(disc);
+    double v = (t / fabs(t)) * pow(fabs(t), 1.0 / 3.0);
+    x[0] = (u + v) - ba;
+    return 1;
+  }
+  double t = -q / pow(-p, 1.5);
+  double phi = t >= 1.0 ? 0.0 : (t <= -1.0 ? pi : acos(t));
+  x[0] = 2.0 * sqrt(-p) * cos(phi / 3.0) - ba;
+  x[1] = 2.0 * sqrt(-p) * cos((phi + 2.0 * pi) / 3.0) - ba;
+  x[2] = 2.0 * sqrt(-p) * cos((phi + 4.0 * pi) / 3.0) - ba;
+  return 3;
+}
+
+/* Friction coefficient with emergent vegetation after Lindner (TYPE::lindner,
+ * Lindner.cpp:72-262): the drag coefficient cWR of the stems is iterated (Pegasus
+ * updates after the third step, at most 200 steps, relative tolerance 1e-3), with the wake
+ * length aNL from its own fixed-point iteration (damped by 1/2) and the depth ratio from
+ * the cubic of the momentum balance; returns lambda / 8 with
+ * lambda = 8 cb + 4 cWR h dp / sp^2, cb without vegetation when dp, sp, v or h are below
+ * the thresholds, or -1 when cWR does not converge. */
+static double lindner(double dp, double sp, double va, double ha, double cb, double vk, double g)
+{
+  if (dp < 1.0e-4 || sp < 1.0e-4 || va < 1.0e-3 || ha < 1.0e-3) return cb;
+  double cwr = 1.0, anl = sp / 2.0;
+  double c1 = 1.0, c2 = 1.0, r1 = 0.0, r2 = 0.0, lambda = 0.0;
+  int it, done = 0;
+  for (it = 0; it < 200; it++) {
+    lambda = 8.0 * cb + 4.0 * cwr * ha * dp / sp / sp;
+    double cw = drag_cylinder(va, dp, vk);
+    int jt, wake = 0;
+    for (jt = 0; jt < 200; jt++) {
+      double t1 = 1.0 + anl * lambda / 4.0 / ha;
+      double t2 = 30.0 / pow(fabs(t1), 1.5);
+      double ra = cw * dp * pow(fabs(t2), 1.429);
+      if (fabs((ra - anl) / ra) < 1.0e-3) { anl = ra; wake = 1; break; }
+      anl = 0.5 * (ra + anl);
+    }
+    if (!wake) anl = sp / 2.0;
+    double anb = 0.24 * pow(fabs(anl), 0.59) * pow(fabs(cw * dp), 0.41);
+    double vr = 1.151 * pow(fabs(anl / sp), -0.483) + 0.5 * pow(fabs(anb / sp), 1.1);
+    double fr = va / sqrt(g * ha);
+    double al = dp / sp;
+    double ca = fr * fr * (1.0 - al * cwr / 2.0);
+    double cbb = -fr * fr - (1.0 - al) / 2.0;
+    double cd = (1.0 - al) / 2.0;
+    double hr = 1.0;
+    if (fabs(ca) < 1.0e-10) {
+      hr = sqrt(-cd / cbb);
+    } else {
+      double x[3];
+      int nr = cubic_roots(ca, cbb, 0.0, cd, x);
+      for (int i = 0; i < nr; i++)
+        if (x[i] > 0.0 && x[i] < 1.0) { hr = x[i]; break; }
+    }
+    double rc = 1.3124 * cw * vr + 2.0 * (1.0 - hr) / fr / fr;
+    if (fabs((rc - cwr) / rc) < 1.0e-3) { done = 1; break; }
+    double dc = rc - cwr;
+    if (it >= 3 && r1 * r2 < 0.0) {
+      if (r2 * dc < 0.0) r1 *= r2 / (r2 + dc);
+      else { c1 = c2; r1 = r2; }
+      c2 = cwr; r2 = dc;
+      cwr = c2 - r2 * (c2 - c1) / (r2 - r1);
+    } else {
+      c1 = c2; r1 = r2;
+      c2 = cwr; r2 = dc;
+      cwr = (it >= 2 && r1 * r2 < 0.0) ? c2 - r2 * (c2 - c1) / (r2 - r1) : rc;
+    }
+  }
+  if (!done) return -1.0;
+  return lambda / 8.0;
+}
+
 /* MODEL::DoFriction (DoFriction.cpp:57-205) with TYPE::bottom (Friction.cpp:49-120) for
  * Manning's law (Friction.cpp:238-240, exponent literally 0.333333): cb is Manning plus a
- * zero Nikuradse form-roughness term; no vegetation. */
+ * zero Nikuradse form-roughness term; emergent vegetation after Lindner
+ * (Friction.cpp:101-108: cb + 0.75 h dp / sp^2 where cWR does not converge); no submerged
+ * vegetation (TYPE::velzen returns 0 without it, Velzen.cpp:77-78). */
 static void do_friction(orc_model* m)
 {
   int* counter = xcalloc((size_t)m->np, sizeof(int));
@@ -419,19 +517,39 @@ static void do_friction(orc_model* m)
   for (int e = 0; e < m->ne; e++) {
     int nnd = m->nnd[e];
     double n = m->manning[e];
+    double dp = m->veg_dp ? m->veg_dp[e] : 0.0, sp = m->veg_sp ? m->veg_sp[e] : 0.0;
     for (int i = 0; i < nnd; i++) {
       int nd = m->elem[8 * e + i];
       double h = m->S[nd] - m->z[nd];
       if (h < m->hmin) h = m->hmin;
+      double vres = sqrt(m->U[nd] * m->U[nd] + m->V[nd] * m->V[nd]);
       double cb = m->g * (n * n) / pow(h, 0.333333);
       cb += 0.0;      /* Nikuradse term of zero form roughness: 0.41/log(inf) squared */
-      double cf = cb + 0.0;
+      double cp = lindner(dp, sp, vres, h, cb, m->vk, m->g);
+      if (cp < -0.9) cp = cb + 0.75 * h * dp / sp / sp;
+      double cf = cp + 0.0;
       m->cf[nd] += m->areaFact[e] * cf;
       counter[nd]++;
     }
   }
   for (int i = 0; i < m->np; i++) if (counter[i]) m->cf[i] /= counter[i];
   free(counter);
+}
+
+void orc_set_vegetation(orc_model* m, const double* dp, const double* sp)
+{
+  free(m->veg_dp); free(m->veg_sp);
+  m->veg_dp = m->veg_sp = NULL;
+  if (!dp || !sp) return;
+  m->veg_dp = xcalloc((size_t)m->ne, sizeof(double));
+  m->veg_sp = xcalloc((size_t)m->ne, sizeof(double));
+  memcpy(m->veg_dp, dp, sizeof(double) * m->ne);
+  memcpy(m->veg_sp, sp, sizeof(double) * m->ne);
+}
+
+double orc_lindner(double dp, double sp, double va, double ha, double cb, double vk, double g)
+{
+  return lindner(dp, sp, va, ha, cb, vk, g);
 }
 
 /* BCONSET::InitBcon (BconSet.cpp:92-480) with BCONLINE::GenBcon / Inlet / Outlet

@@ -449,11 +449,110 @@ __global__ void __launch_bounds__(256) k_element(NodeView nv, const int* __restr
   }
 }
 
-// ---- nodal friction (MODEL::DoFriction + TYPE::bottom, Manning) ----
+// ---- emergent vegetation (Lindner) ----
+// Drag coefficient of one cylinder from its Reynolds number (TYPE::dragCoeff,
+// Lindner.cpp:264-278).
+__device__ double drag_cylinder(double v, double d, double vk)
+{
+  const double re = v * d / vk;
+  if (re <= 800.0) return 3.07 / pow(re, 0.168);
+  if (re <= 6000.0) return 1.0;
+  if (re <= 11000.0) return 1.0 + 0.2 * (re - 6000.0) / 5000.0;
+  return 1.2;
+}
+
+// Real roots of a x^3 + b x^2 + d (no linear term) in Cardano or trigonometric form
+// (TYPE::cubeEquation, Lindner.cpp:281-335); returns their number.
+__device__ int cubic_roots(double a, double b, double d, double x[3])
+{
+  const double pi = 3.141592653589793;
+  const double ba = b / a / 3.0, ca = 0.0 / a;
+  const double p = ca / 3.0 - ba * ba;
+  const double q = ba * ba * ba - ba * ca / 2.0 + d / a / 2.0;
+  const double disc = q * q + p * p * p;
+  if (disc > 0.0) {
+    double t = -q + sqrt(disc);
+    const double u = (t / fabs(t)) * pow(fabs(t), 1.0 / 3.0);
+    t = -q - sqrt(disc);
+    const double v = (t / fabs(t)) * pow(fabs(t), 1.0 / 3.0);
+    x[0] = (u + v) - ba;
+    return 1;
+  }
+  const double t = -q / pow(-p, 1.5);
+  const double phi = t >= 1.0 ? 0.0 : (t <= -1.0 ? pi : acos(t));
+  x[0] = 2.0 * sqrt(-p) * cos(phi / 3.0) - ba;
+  x[1] = 2.0 * sqrt(-p) * cos((phi + 2.0 * pi) / 3.0) - ba;
+  x[2] = 2.0 * sqrt(-p) * cos((phi + 4.0 * pi) / 3.0) - ba;
+  return 3;
+}
+
+// Friction coefficient with emergent stems of diameter dp and spacing sp (TYPE::lindner,
+// Lindner.cpp:72-262): the stems' drag coefficient cWR is iterated to 1e-3 (Pegasus steps
+// after the third iteration, at most 200), each step with the wake length from a damped
+// fixed-point iteration and the depth ratio from the momentum-balance cubic.  Returns
+// lambda / 8 = cb + cWR h dp / (2 sp^2), cb itself below the thresholds, -1 without
+// convergence.
+__device__ double lindner(double dp, double sp, double va, double ha, double cb, double vk, double g)
+{
+  if (dp < 1.0e-4 || sp < 1.0e-4 || va < 1.0e-3 || ha < 1.0e-3) return cb;
+  double cwr = 1.0, anl = sp / 2.0;
+  double c1 = 1.0, c2 = 1.0, r1 = 0.0, r2 = 0.0, lambda = 0.0;
+  bool done = false;
+  for (int it = 0; it < 200 && !done; it++) {
+    lambda = 8.0 * cb + 4.0 * cwr * ha * dp / sp / sp;
+    const double cw = drag_cylinder(va, dp, vk);
+    bool wake = false;
+    for (int jt = 0; jt < 200; jt++) {
+      const double t1 = 1.0 + anl * lambda / 4.0 / ha;
+      const double t2 = 30.0 / pow(fabs(t1), 1.5);
+      const double ra = cw * dp * pow(fabs(t2), 1.429);
+      if (fabs((ra - anl) / ra) < 1.0e-3) { anl = ra; wake = true; break; }
+      anl = 0.5 * (ra + anl);
+    }
+    if (!wake) anl = sp / 2.0;
+    const double anb = 0.24 * pow(fabs(anl), 0.59) * pow(fabs(cw * dp), 0.41);
+    const double vr = 1.151 * pow(fabs(anl / sp), -0.483) + 0.5 * pow(fabs(anb / sp), 1.1);
+    const double fr = va / sqrt(g * ha);
+    const double al = dp / sp;
+    const double ca = fr * fr * (1.0 - al * cwr / 2.0);
+    const double cbb = -fr * fr - (1.0 - al) / 2.0;
+    const double cd = (1.0 - al) / 2.0;
+    double hr = 1.0;
+    if (fabs(ca) < 1.0e-10) {
+      hr = sqrt(-cd / cbb);
+    } else {
+      double x[3];
+      const int nr = cubic_roots(ca, cbb, cd, x);
+      for (int i = 0; i < nr; i++)
+        if (x[i] > 0.0 && x[i] < 1.0) { hr = x[i]; break; }
+    }
+    const double rc = 1.3124 * cw * vr + 2.0 * (1.0 - hr) / fr / fr;
+    if (fabs((rc - cwr) / rc) < 1.0e-3) { done = true; break; }
+    const double dc = rc - cwr;
+    if (it >= 3 && r1 * r2 < 0.0) {
+      if (r2 * dc < 0.0) r1 *= r2 / (r2 + dc);
+      else { c1 = c2; r1 = r2; }
+      c2 = cwr; r2 = dc;
+      cwr = c2 - r2 * (c2 - c1) / (r2 - r1);
+    } else {
+      c1 = c2; r1 = r2;
+      c2 = cwr; r2 = dc;
+      cwr = (it >= 2 && r1 * r2 < 0.0) ? c2 - r2 * (c2 - c1) / (r2 - r1) : rc;
+    }
+  }
+  return done ? lambda / 8.0 : -1.0;
+}
+
+// ---- nodal friction (MODEL::DoFriction + TYPE::bottom: Manning, emergent vegetation) ----
+// Per node the mean over its region elements of areaFact * cf_e, cf_e = Lindner's
+// coefficient over the Manning coefficient cb (cb + 0.75 h dp / sp^2 where it does not
+// converge, Friction.cpp:101-108), evaluated at the node's depth and speed.
 __global__ void k_friction(int np, const int* __restrict__ n2e_ptr, const int* __restrict__ n2e,
                            const double* __restrict__ manning,
+                           const double* __restrict__ veg_dp, const double* __restrict__ veg_sp,
                            const double* __restrict__ areaFact, const double* __restrict__ S,
-                           const double* __restrict__ z, double g, double hmin,
+                           const double* __restrict__ z, const double* __restrict__ U,
+                           const double* __restrict__ V, double g, double vk, double hmin,
                            double* __restrict__ cf)
 {
   const int n = blockIdx.x * blockDim.x + threadIdx.x;
@@ -461,12 +560,20 @@ __global__ void k_friction(int np, const int* __restrict__ n2e_ptr, const int* _
   double h = S[n] - z[n];
   if (h < hmin) h = hmin;
   const double ph = pow(h, 0.333333);
+  const double vres = sqrt(U[n] * U[n] + V[n] * V[n]);
   double sum = 0.0;
   const int a = n2e_ptr[n], b = n2e_ptr[n + 1];
   for (int q = a; q < b; q++) {
     const int e = n2e[q];
     const double nm = manning[e];
-    sum += areaFact[e] * (g * (nm * nm) / ph);
+    const double cb = g * (nm * nm) / ph;
+    const double dp = veg_dp[e], sp = veg_sp[e];
+    double cp = cb;
+    if (dp >= 1.0e-4 && sp >= 1.0e-4) {
+      cp = lindner(dp, sp, vres, h, cb, vk, g);
+      if (cp < -0.9) cp = cb + 0.75 * h * dp / sp / sp;
+    }
+    sum += areaFact[e] * cp;
   }
   cf[n] = (b > a) ? sum / (double)(b - a) : 0.0;
 }
@@ -535,8 +642,9 @@ void dev_prepare_step(Ctx& c)
   upload_shapes(c.stream);
   const int np = c.hm.np;
   k_friction<<<(np + 255) / 256, 256, 0, c.stream>>>(np, c.n2e_ptr.p, c.n2e.p, c.d_manning.p,
-                                                     c.d_areaFact.p, c.S.p, c.z.p, c.par.g,
-                                                     c.par.hmin(), c.cf.p);
+                                                     c.d_veg_dp.p, c.d_veg_sp.p, c.d_areaFact.p,
+                                                     c.S.p, c.z.p, c.U.p, c.V.p, c.par.g,
+                                                     c.par.vk, c.par.hmin(), c.cf.p);
   HIPCHECK(hipGetLastError());
   const int nseg = (int)(c.hm.in_segs.size() / 3), nin = (int)c.hm.inlet_nodes.size();
   if (nseg > 0) {

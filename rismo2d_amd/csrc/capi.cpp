@@ -101,6 +101,7 @@ rmx_status rmx_mesh_upload(rmx_ctx* p, const rmx_mesh_desc* d)
   HIPCHECK(hipMemsetAsync(c.V.p, 0, sizeof(double) * np, s));
   c.d_elem.upload(hm.elem, s); c.d_nnd.upload(hm.nnd, s);
   c.d_manning.upload(hm.manning, s); c.d_areaFact.upload(hm.areaFact, s);
+  c.d_veg_dp.upload(hm.veg_dp, s); c.d_veg_sp.upload(hm.veg_sp, s);
   // node -> region elements (ascending) for the friction gather
   std::vector<int> ptr(np + 1, 0), list;
   for (int e = 0; e < hm.ne; e++)

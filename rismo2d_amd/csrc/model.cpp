@@ -53,6 +53,11 @@ void host_mesh(HostModel& hm, const rmx_mesh_desc& d)
   hm.x.assign(d.x, d.x + d.np); hm.y.assign(d.y, d.y + d.np); hm.zor.assign(d.z, d.z + d.np);
   hm.elem.assign(d.elems, d.elems + 8LL * d.ne); hm.nnd.assign(d.nnd, d.nnd + d.ne);
   hm.manning.assign(d.manning, d.manning + d.ne); hm.vt.assign(d.vt, d.vt + d.ne);
+  hm.veg_dp.assign(d.ne, 0.0); hm.veg_sp.assign(d.ne, 0.0);
+  if (d.veg_dp && d.veg_sp) {
+    hm.veg_dp.assign(d.veg_dp, d.veg_dp + d.ne);
+    hm.veg_sp.assign(d.veg_sp, d.veg_sp + d.ne);
+  }
   hm.in_segs.assign(d.inlet_segs, d.inlet_segs + 3LL * d.n_inlet);
   hm.out_segs.assign(d.outlet_segs, d.outlet_segs + 3LL * d.n_outlet);
   hm.Q = d.Q; hm.S_in = d.S_in; hm.S_out = d.S_out;

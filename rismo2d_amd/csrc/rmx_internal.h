@@ -72,7 +72,7 @@ struct DBuf {
 // Host-side model: mesh, boundary line elements, boundary conditions, numbering, pattern.
 struct HostModel {
   int np = 0, ne = 0;
-  std::vector<double> x, y, zor, areaFact, manning, vt;
+  std::vector<double> x, y, zor, areaFact, manning, vt, veg_dp, veg_sp;
   std::vector<int> elem, nnd;                 // [ne*8], [ne]
   std::vector<int> in_segs, out_segs;
   double Q = 0, S_in = 0, S_out = 0;
@@ -187,7 +187,7 @@ struct Ctx {
   // friction gather: node -> incident region elements (ascending), areaFact*cf summed
   DBuf<int> n2e_ptr, n2e;
   DBuf<int> d_elem, d_nnd;                    // region elements [ne*8], [ne]
-  DBuf<double> d_manning, d_areaFact;
+  DBuf<double> d_manning, d_areaFact, d_veg_dp, d_veg_sp;
   DBuf<int> d_in_segs, d_inlet_nodes;
   // ---- model elements for assembly ----
   int nme = 0;

@@ -44,6 +44,8 @@ class Deck:
     S_inlet: float = 2.1     # QINLET reference water level used for the q distribution
     S_outlet: float = 2.0    # OUTLET water level
     name: str = "deck"
+    veg_dp: np.ndarray | None = None   # [ne] stem diameter of emergent vegetation (m), 0: none
+    veg_sp: np.ndarray | None = None   # [ne] stem spacing (m)
 
     @property
     def n_nodes(self) -> int:
@@ -95,7 +97,9 @@ def channel(nx: int = 200, ny: int = 20, length: float = 1000.0, width: float = 
             slope: float = 1.0e-4, z_out: float = 0.0, noise_sigma: float = 0.05,
             noise_corr: float = 20.0, seed: int = 0, manning: float = 0.03,
             vt: float = 0.1, Q: float = 100.0, S_out: float = 2.0, S_in: float = 2.1,
-            U_init: float = 0.5, quads: bool = False, name: str | None = None) -> Deck:
+            U_init: float = 0.5, quads: bool = False, name: str | None = None,
+            veg_frac: float = 0.0, veg_band: float = 0.3, veg_dp=(0.01, 0.05),
+            veg_density=(0.5, 5.0), veg_seed: int = 0) -> Deck:
     """Straight rectangular channel on a (2nx+1) x (2ny+1) lattice.
 
     Each cell becomes two six-node triangles split along its (i,j)->(i+2,j+2) diagonal,
@@ -104,6 +108,9 @@ def channel(nx: int = 200, ny: int = 20, length: float = 1000.0, width: float = 
     width (the order that a front-width minimisation gives for a channel).  Bed:
     ``z = z_out + slope*(length - x) + GRF``; midside nodes carry the mean of their
     corners so the bed is piecewise linear.  Inlet line at x=0, outlet line at x=length.
+    With ``veg_frac`` > 0, a seeded fraction of the elements in the outer ``veg_band`` of the
+    width on either bank (the floodplain strips) carries emergent vegetation: stem diameter
+    U[veg_dp] m and density U[veg_density] per m^2 (spacing = density^-1/2).
     """
     NX, NY = 2 * nx + 1, 2 * ny + 1
     xs = np.linspace(0.0, length, NX)
@@ -152,13 +159,24 @@ def channel(nx: int = 200, ny: int = 20, length: float = 1000.0, width: float = 
     nnd = np.full(elems.shape[0], 8 if quads else 6, np.int32)
     ne = elems.shape[0]
     npn = x.shape[0]
+    veg_d = veg_s = None
+    if veg_frac > 0.0:
+        ncn = 4 if quads else 3
+        yc = y[elems[:, :ncn]].mean(1)
+        bank = np.abs(yc - 0.5 * width) > (1.0 - veg_band) * 0.5 * width
+        rng = np.random.default_rng(veg_seed + 7)
+        pick = bank & (rng.random(ne) < veg_frac)
+        veg_d = np.where(pick, rng.uniform(veg_dp[0], veg_dp[1], ne), 0.0)
+        dens = rng.uniform(veg_density[0], veg_density[1], ne)
+        veg_s = np.where(pick, 1.0 / np.sqrt(dens), 0.0)
     return Deck(x=x.astype(np.float64), y=y.astype(np.float64), z=z.astype(np.float64),
                 elems=elems.astype(np.int32), nnd=nnd,
                 manning=np.full(ne, manning), vt=np.full(ne, vt),
                 lines={k: v.astype(np.int32) for k, v in lines.items()},
                 U0=np.full(npn, U_init), V0=np.zeros(npn), S0=np.full(npn, S_out),
                 Q_inlet=Q, S_inlet=S_in, S_outlet=S_out,
-                name=name or f"channel_{nx}x{ny}{'_quad' if quads else ''}")
+                name=name or f"channel_{nx}x{ny}{'_quad' if quads else ''}",
+                veg_dp=veg_d, veg_sp=veg_s)
 
 
 # ---------------------------------------------------------------------------------------
@@ -296,19 +314,23 @@ def write_reference_deck(deck: Deck, outdir: str, solver: str = "1 6 1 0 60 1000
                          dryrew: str = "2 1000 0.001 0.005 100 5") -> None:
     """Write the reference's keyword input files for ``deck`` into ``outdir``.
 
-    One material per distinct (Manning n, vt) pair.  Values are written with %.17g so
-    that the reference reads back the exact doubles used by the oracle.
+    One material per distinct (Manning n, vt, stem diameter, stem spacing).  Values are
+    written with %.17g so that the reference reads back the exact doubles used by the
+    oracle.
     """
     os.makedirs(outdir, exist_ok=True)
-    pairs = sorted(set(zip(deck.manning.tolist(), deck.vt.tolist())))
-    mat_of = {p: i + 1 for i, p in enumerate(pairs)}
+    ne = deck.n_elems
+    vd = deck.veg_dp if deck.veg_dp is not None else np.zeros(ne)
+    vs = deck.veg_sp if deck.veg_sp is not None else np.zeros(ne)
+    keys = list(zip(deck.manning.tolist(), deck.vt.tolist(), vd.tolist(), vs.tolist()))
+    mat_of = {p: i + 1 for i, p in enumerate(sorted(set(keys)))}
     with open(os.path.join(outdir, "region.dat"), "w") as f:
         f.write(f"{deck.n_nodes} {deck.n_elems} 0\n")
         for i in range(deck.n_nodes):
             f.write(f"{i + 1} {deck.x[i]:.17g} {deck.y[i]:.17g} {deck.z[i]:.17g}\n")
         for e in range(deck.n_elems):
             k = int(deck.nnd[e])
-            mat = mat_of[(float(deck.manning[e]), float(deck.vt[e]))]
+            mat = mat_of[keys[e]]
             shape = "tri" if k == 6 else "quad"
             ids = " ".join(str(int(v) + 1) for v in deck.elems[e, :k])
             f.write(f"{e + 1} {mat} {shape} {ids}\n")
@@ -324,9 +346,9 @@ def write_reference_deck(deck: Deck, outdir: str, solver: str = "1 6 1 0 60 1000
             f.write(f"{i + 1} {deck.U0[i]:.17g} {deck.V0[i]:.17g} {deck.S0[i]:.17g}\n")
     with open(os.path.join(outdir, "mat.dat"), "w") as f:
         f.write("$RISMO2D 40000\n")
-        for (n, vt), no in mat_of.items():
-            f.write(f"$ROUGHNESS {no} 5 {n:.17g} 0 0.0 0.0 0.0\n")
-        for (n, vt), no in mat_of.items():
+        for (n, vt, dp, sp), no in mat_of.items():
+            f.write(f"$ROUGHNESS {no} 5 {n:.17g} 0 0.0 {dp:.17g} {sp:.17g}\n")
+        for (n, vt, dp, sp), no in mat_of.items():
             f.write(f"$TURBULENCE {no} {vt:.17g} 0.0 0.0 0.0 0.0 0.0 0.0\n")
     with open(os.path.join(outdir, "tm.dat"), "w") as f:
         f.write("$RISMO2D 40100\n$TM_STEPS 1 1 0\n$TM_INTERVAL 1000000.0 0.0 0.0\n"

@@ -38,7 +38,7 @@ class MeshDesc(C.Structure):
                 ("elems", _i), ("nnd", _i), ("manning", _d), ("vt", _d),
                 ("n_inlet", C.c_int), ("inlet_segs", _i), ("n_outlet", C.c_int),
                 ("outlet_segs", _i), ("Q", C.c_double), ("S_in", C.c_double),
-                ("S_out", C.c_double)]
+                ("S_out", C.c_double), ("veg_dp", _d), ("veg_sp", _d)]
 
 
 class Params(C.Structure):
@@ -176,6 +176,11 @@ class Context:
                      _p(k["el"], _i), _p(k["nnd"], _i), _p(k["man"], _d), _p(k["vt"], _d),
                      len(k["inl"]), _p(k["inl"], _i), len(k["out"]), _p(k["out"], _i),
                      deck.Q_inlet, deck.S_inlet, deck.S_outlet)
+        if getattr(deck, "veg_dp", None) is not None:
+            k["vdp"] = _c(deck.veg_dp, np.float64)
+            k["vsp"] = _c(deck.veg_sp, np.float64)
+            d.veg_dp = _p(k["vdp"], _d)
+            d.veg_sp = _p(k["vsp"], _d)
         _check(lib().rmx_mesh_upload(self.h, C.byref(d)))
         self.np = deck.n_nodes
         self.state_upload(deck.U0, deck.V0, deck.S0)

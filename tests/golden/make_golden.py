@@ -10,13 +10,15 @@ the reference's input format, runs the reference and stores what the harness rec
                     correction; step 0's element matrices, scaled system, ILU(0) factors,
                     ILU apply and MulVec of the scaled RHS; the final state.
 * ``quad10x3.npz``  8-node quads, 10 x 3 cells, same records.
+* ``veg20x4.npz``   the 20 x 4 triangle channel with emergent vegetation (Lindner) on
+                    60 % of the elements in the outer halves of the width, same records.
 * ``channel200x20.npz``  the straight-channel workload (200 x 20 cells, 8,000 triangles,
                     bed noise sigma 0.05 m / 20 m, seed 0): numbering, per-step scale and
                     correction, BiCGStab iteration counts from the report, final state,
                     and a SHA-256 of the step-0 pattern.
 
 The reference ships no test vectors of its own; these are the only golden data.
-Usage:  python tests/golden/make_golden.py
+Usage:  python tests/golden/make_golden.py [deck names]
 """
 from __future__ import annotations
 
@@ -44,6 +46,8 @@ DECKS = {
                     Q=10.0),
     "quad10x3": dict(nx=10, ny=3, length=100.0, width=30.0, noise_sigma=0.05, noise_corr=5.0,
                      Q=10.0, quads=True),
+    "veg20x4": dict(nx=20, ny=4, length=100.0, width=20.0, noise_sigma=0.05, noise_corr=5.0,
+                    Q=10.0, veg_frac=0.6, veg_band=0.5),
     "channel200x20": dict(nx=200, ny=20),
 }
 
@@ -61,9 +65,13 @@ def run_reference(deck, workdir):
 
 
 def deck_arrays(d):
-    return dict(x=d.x, y=d.y, z=d.z, elems=d.elems, nnd=d.nnd, manning=d.manning, vt=d.vt,
-                inlet=d.lines[dk.INLET_LINE], outlet=d.lines[dk.OUTLET_LINE], U0=d.U0, V0=d.V0,
-                S0=d.S0, Q=d.Q_inlet, S_in=d.S_inlet, S_out=d.S_outlet)
+    out = dict(x=d.x, y=d.y, z=d.z, elems=d.elems, nnd=d.nnd, manning=d.manning, vt=d.vt,
+               inlet=d.lines[dk.INLET_LINE], outlet=d.lines[dk.OUTLET_LINE], U0=d.U0, V0=d.V0,
+               S0=d.S0, Q=d.Q_inlet, S_in=d.S_inlet, S_out=d.S_outlet)
+    if d.veg_dp is not None:
+        out["veg_dp"] = d.veg_dp
+        out["veg_sp"] = d.veg_sp
+    return out
 
 
 def state_arrays(prefix, st):
@@ -77,7 +85,9 @@ def state_arrays(prefix, st):
 def main():
     subprocess.run(["make", "-s", "-C", os.path.join(ROOT, "oracle"), "ref", f"REF={REF}"],
                    check=True)
-    for name, kw in DECKS.items():
+    names = sys.argv[1:] or list(DECKS)
+    for name in names:
+        kw = DECKS[name]
         d = dk.channel(**kw)
         with tempfile.TemporaryDirectory() as tmp:
             (steps, final), iters = run_reference(d, tmp)

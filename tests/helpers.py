@@ -18,4 +18,5 @@ def golden_deck(g):
                    manning=g["manning"], vt=g["vt"],
                    lines={dk.INLET_LINE: g["inlet"], dk.OUTLET_LINE: g["outlet"]},
                    U0=g["U0"], V0=g["V0"], S0=g["S0"], Q_inlet=float(g["Q"]),
-                   S_inlet=float(g["S_in"]), S_outlet=float(g["S_out"]))
+                   S_inlet=float(g["S_in"]), S_outlet=float(g["S_out"]),
+                   veg_dp=g.get("veg_dp"), veg_sp=g.get("veg_sp"))

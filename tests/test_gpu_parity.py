@@ -7,7 +7,8 @@ Tolerances (floating point):
   (the factor kernel keeps the reference's per-entry update order, FMA contraction off);
 * element matrices, assembled FP64 entries, RHS: |gpu - oracle| <= 1e-12 * max|row| (the
   GPU contracts to FMA and sums element contributions in colour order);
-* nodal friction / inlet discharge: 1e-13 relative;
+* nodal friction and inlet discharge: 1e-13 relative (1e-10 at nodes with Lindner's
+  vegetation iteration, which amplifies the ulp differences of device pow / acos / cos);
 * SpMV, ILU apply (FP64 accumulation in another order): 1e-12 relative to max|result|;
 * Newton-converged u, v, h: within 1e-8 relative of the reference / oracle.
 """
@@ -39,7 +40,7 @@ def _rowmax(rp, A):
     return m[rows], rows
 
 
-@pytest.mark.parametrize("name", ["tri20x4", "quad10x3"])
+@pytest.mark.parametrize("name", ["tri20x4", "quad10x3", "veg20x4"])
 def test_setup_and_element_kernel_match_oracle(ctx, name):
     print(f"[gpu] setup/element parity on {name}", flush=True)
     g = load_golden(name)
@@ -55,7 +56,15 @@ def test_setup_and_element_kernel_match_oracle(ctx, name):
     np.testing.assert_array_equal(cols, ocols)
     nod, onod = ctx.nodal(), om.nodal()
     np.testing.assert_array_equal(nod["kind"], onod["kind"])
-    np.testing.assert_allclose(nod["cf"], onod["cf"], rtol=1e-13, atol=0)
+    # Manning: 1e-13.  Nodes touching vegetated elements: 1e-10 (Lindner's drag iteration
+    # amplifies the ulp differences of device pow / acos / cos against the host libm; a
+    # flipped convergence decision would show as ~1e-3)
+    veg = np.zeros(deck.n_nodes, bool)
+    if deck.veg_dp is not None:
+        for e in np.nonzero(deck.veg_dp > 0)[0]:
+            veg[deck.elems[e, :deck.nnd[e]]] = True
+    np.testing.assert_allclose(nod["cf"][~veg], onod["cf"][~veg], rtol=1e-13, atol=0)
+    np.testing.assert_allclose(nod["cf"][veg], onod["cf"][veg], rtol=1e-10, atol=0)
     np.testing.assert_allclose(nod["bcU"], onod["bcU"], rtol=1e-13, atol=0)
     np.testing.assert_array_equal(nod["rotm"], onod["rotm"])
     ne = ctx.n_model_elems()
@@ -120,7 +129,7 @@ def _relerr(a, b, scale):
     return np.abs(a - b).max() / scale
 
 
-@pytest.mark.parametrize("name", ["tri20x4", "quad10x3", "channel200x20"])
+@pytest.mark.parametrize("name", ["tri20x4", "quad10x3", "veg20x4", "channel200x20"])
 def test_newton_converges_to_reference_solution(ctx, name):
     """Full Newton iteration on the GPU (FP64 assembly, FP64 ILU0-BiCGStab) reaches the
     reference program's converged u, v, h within 1e-8 relative."""

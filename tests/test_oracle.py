@@ -25,7 +25,7 @@ def _check_state(om, g, prefix):
     return nod
 
 
-@pytest.mark.parametrize("name", ["tri20x4", "quad10x3"])
+@pytest.mark.parametrize("name", ["tri20x4", "quad10x3", "veg20x4"])
 def test_oracle_matches_reference_every_record(name):
     g = load_golden(name)
     om = po.OracleModel(golden_deck(g))
