@@ -64,12 +64,22 @@ typedef struct {
   double Q, S_in, S_out;      /* inlet discharge, inlet reference level, outlet level */
   const double* veg_dp;       /* [ne] emergent vegetation stem diameter (m), nullable */
   const double* veg_sp;       /* [ne] stem spacing (m), nullable; dp or sp < 1e-4: none */
+  const double* turb_st;      /* [ne] st of the algebraic eddy viscosity model, nullable: 1.0
+                                 ($TURBULENCE no vt st estx esty KDest, Type.cpp:681, defaults :73) */
+  const double* turb_kdest;   /* [ne] KDest, nullable: 0.6 */
 } rmx_mesh_desc;
 
 typedef struct {
   int relax_method;           /* 0: none, 2: cap by max_delta_uv / max_delta_s */
   int dry_method;             /* 0,2,3: marsh bed lowering; else depth clip */
   double dry_limit, max_delta_uv, max_delta_s, conv_uv, conv_s, max_us;
+  int turb;                   /* turbulence model, the $TM_TURBULENCE code (Timeint.cpp:1333):
+                                 0 or 1 constant vt of the element, 2 algebraic model
+                                 (Turbulence.cpp:292); + 100: the element's vt is a lower
+                                 bound (kVtMin); + 200: nodal vt recomputed at every Newton
+                                 step (kVtIterat), else at the first step after a mesh,
+                                 state or parameter upload or rmx_turbulence */
+  double cm, cd;              /* k-epsilon constants ($KDCONST), <= 0: 0.09, 1.0 */
 } rmx_params;
 
 /* Mesh, boundary lines and material data (MODEL::Input, MODEL::Initialize,
@@ -82,7 +92,8 @@ rmx_status rmx_set_params(rmx_ctx* ctx, const rmx_params* p);
 rmx_status rmx_state_upload(rmx_ctx* ctx, const double* U, const double* V, const double* S);
 rmx_status rmx_state_download(rmx_ctx* ctx, double* U, double* V, double* S, double* z);
 
-/* Per Newton step set-up (EqsUVS2D.cpp:300-360): nodal friction (MODEL::DoFriction) and
+/* Per Newton step set-up (EqsUVS2D.cpp:300-360): nodal friction (MODEL::DoFriction), the
+ * algebraic model's nodal eddy viscosity when it is due (GRID::Turbulence) and the
  * inlet discharge distribution (BCONSET::InitBcon) on the device; boundary kinds,
  * normals, rotations (MODEL::SetLocation/SetNormal/SetRotation), equation numbering
  * (EQS::SetEqno), CSR pattern (EQS::SetIndexMat), element colouring and ILU level sets on
@@ -92,6 +103,13 @@ rmx_status rmx_get_numbering(rmx_ctx* ctx, int* eqno /* [3*np] */);
 rmx_status rmx_get_pattern(rmx_ctx* ctx, int* rowptr /* [neq+1] */, int* cols /* [nnz] */);
 rmx_status rmx_get_nodal(rmx_ctx* ctx, double* cf, double* bcU, int* kind, double* rotm);
 rmx_status rmx_n_model_elems(rmx_ctx* ctx, int* n);
+
+/* Nodal friction, then the algebraic model's nodal eddy viscosity vt and normal Reynolds
+ * stress uu (= vv; uv = 0) from the current state: replaces MODEL::DoFriction followed by
+ * GRID::Turbulence (Turbulence.cpp:292-352), as the flow cycle calls them at its first
+ * Newton step (EqsUVS2D.cpp:307-315).  Requires params.turb = 2 (+100/+200); vt, uu are
+ * [np] host arrays, nullable. */
+rmx_status rmx_turbulence(rmx_ctx* ctx, double* vt, double* uu);
 
 /* ------------------------------------------------------------------------------------ */
 /* drop-in group                                                                         */

@@ -39,6 +39,10 @@ def lib():
                                  C.c_int, _i, C.c_double, C.c_double, C.c_double]
         L.orc_destroy.argtypes = [C.c_void_p]
         L.orc_set_vegetation.argtypes = [C.c_void_p, _d, _d]
+        L.orc_set_turbulence.argtypes = [C.c_void_p, C.c_int, _d, _d, C.c_double, C.c_double]
+        L.orc_set_turbulence.restype = C.c_int
+        L.orc_turbulence.argtypes = [C.c_void_p, _d, _d]
+        L.orc_get_turbulence.argtypes = [C.c_void_p, _d, _d]
         L.orc_lindner.argtypes = [C.c_double] * 7
         L.orc_lindner.restype = C.c_double
         L.orc_set_params.argtypes = [C.c_void_p, C.c_int, C.c_int] + [C.c_double] * 6
@@ -112,7 +116,32 @@ class OracleModel:
             k["vdp"] = np.ascontiguousarray(deck.veg_dp, np.float64)
             k["vsp"] = np.ascontiguousarray(deck.veg_sp, np.float64)
             L.orc_set_vegetation(self.h, _p(k["vdp"], _d), _p(k["vsp"], _d))
+        self.turb = int(getattr(deck, "turb", 1))
+        for key, name in (("tst", "turb_st"), ("tkd", "turb_kdest")):
+            a = getattr(deck, name, None)
+            k[key] = None if a is None else np.ascontiguousarray(a, np.float64)
+        self.set_turbulence(self.turb)
         self.set_state(deck.U0, deck.V0, deck.S0)
+
+    def set_turbulence(self, code, cm=0.09, cd=1.0):
+        """Turbulence model code ($TM_TURBULENCE) with the deck's st, KDest."""
+        k = self._keep
+        st = None if k["tst"] is None else _p(k["tst"], _d)
+        kd = None if k["tkd"] is None else _p(k["tkd"], _d)
+        if lib().orc_set_turbulence(self.h, code, st, kd, cm, cd) != 0:
+            raise ValueError(f"turbulence model {code} not supported")
+
+    def turbulence(self):
+        """Friction, then the algebraic model's nodal vt and uu of the current state."""
+        vt, uu = np.zeros(self.np), np.zeros(self.np)
+        lib().orc_turbulence(self.h, _p(vt, _d), _p(uu, _d))
+        return vt, uu
+
+    def turbulence_state(self):
+        """The nodal vt, uu in use (as the last prepare left them)."""
+        vt, uu = np.zeros(self.np), np.zeros(self.np)
+        lib().orc_get_turbulence(self.h, _p(vt, _d), _p(uu, _d))
+        return vt, uu
 
     def __del__(self):
         if getattr(self, "h", None):

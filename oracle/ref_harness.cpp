@@ -72,8 +72,8 @@ static void dump_state( const char* tag, PROJECT* project, EQS* eqs )
   GRID* rg = project->M2D->region;
   int np = rg->Getnp();
   // per node: name, flag, bc.kind, bc.rot, eqno[3] (as doubles for a single array)
-  // then x,y,z,zor,U,V,S,cf,vt,nx,ny,niox,nioy,bcU,bcS,rot00,rot01,rot11,rot10
-  const int nf = 26;
+  // then x,y,z,zor,U,V,S,cf,vt,nx,ny,niox,nioy,bcU,bcS,rot00,rot01,rot11,rot10,uu,uv,vv
+  const int nf = 29;
   std::vector<double> buf( (size_t)np * nf );
   for( int n=0; n<np; n++ )
   {
@@ -94,6 +94,7 @@ static void dump_state( const char* tag, PROJECT* project, EQS* eqs )
     b[23] = nd->bc.Getrot(0,1);
     b[24] = nd->bc.Getrot(1,1);
     b[25] = nd->bc.Getrot(1,0);
+    b[26] = nd->uu;    b[27] = nd->uv;    b[28] = nd->vv;
   }
   int64_t hdr[2] = { np, nf };
   rec( "STATEHDR", hdr, sizeof(hdr) );

@@ -200,6 +200,11 @@ struct orc_model {
   int *elem, *nnd;
   double *manning, *vt, *areaFact;
   double *veg_dp, *veg_sp;          /* emergent vegetation per element (0: none) */
+  /* turbulence model: code of $TM_TURBULENCE, st * KDest per element, k-epsilon constants,
+   * nodal eddy viscosity and normal Reynolds stress (uu = vv, uv = 0) of the algebraic model */
+  int turb, turb_stale;
+  double cm, cd;
+  double *ste_e, *vtn, *uun;
   int n_in, n_out;
   int *in_segs, *out_segs;
   double Q, S_in, S_out;
@@ -333,6 +338,11 @@ orc_model* orc_create(int np, const double* x, const double* y, const double* z,
   m->vk = 1.78e-6 / (1.0 + 0.0337 * 10.0 + 0.00022 * 10.0 * 10.0);
   orc_set_params(m, 2, 2, 0.001, 10.0, 10.0, 1.0e-7, 1.0e-7, 100.0);
   for (int i = 0; i < np; i++) m->S[i] = m->z[i];
+  m->turb = 1; m->turb_stale = 1;
+  m->cm = 0.09; m->cd = 1.00;                   /* KDCONST defaults (Project.cpp:117-118) */
+  m->ste_e = xcalloc((size_t)ne, sizeof(double));
+  m->vtn = xcalloc((size_t)np, sizeof(double));
+  m->uun = xcalloc((size_t)np, sizeof(double));
   area_factors(m);
   topology(m);
   return m;
@@ -344,7 +354,8 @@ void orc_destroy(orc_model* m)
   void* ps[] = { m->x, m->y, m->z, m->zor, m->U, m->V, m->S, m->dUdt, m->dVdt, m->dSdt, m->cf,
                  m->nx, m->ny, m->niox, m->nioy, m->valU, m->valS, m->elem, m->nnd, m->manning,
                  m->vt, m->areaFact, m->in_segs, m->out_segs, m->noel, m->flag, m->kind, m->rot,
-                 m->eqno, m->bnd, m->bflag, m->me, m->rowptr, m->cols, m->veg_dp, m->veg_sp };
+                 m->eqno, m->bnd, m->bflag, m->me, m->rowptr, m->cols, m->veg_dp, m->veg_sp,
+                 m->ste_e, m->vtn, m->uun };
   for (size_t i = 0; i < sizeof(ps) / sizeof(ps[0]); i++) free(ps[i]);
   free(m);
 }
@@ -364,6 +375,7 @@ void orc_set_state(orc_model* m, const double* U, const double* V, const double*
 {
   memcpy(m->U, U, sizeof(double) * m->np); memcpy(m->V, V, sizeof(double) * m->np);
   memcpy(m->S, S, sizeof(double) * m->np);
+  m->turb_stale = 1;
 }
 
 void orc_get_state(const orc_model* m, double* U, double* V, double* S, double* z)
@@ -545,6 +557,70 @@ void orc_set_vegetation(orc_model* m, const double* dp, const double* sp)
   m->veg_sp = xcalloc((size_t)m->ne, sizeof(double));
   memcpy(m->veg_dp, dp, sizeof(double) * m->ne);
   memcpy(m->veg_sp, sp, sizeof(double) * m->ne);
+}
+
+/* Turbulence model of the flow cycle.  code is the $TM_TURBULENCE value as Timeint.cpp:
+ * 1333-1360 decodes it: last digit 1 = constant eddy viscosity of the element type
+ * (kVtConstant), 2 = algebraic model (kVtAlgebraic); hundreds digit bit 1 = the type's
+ * vt is a lower bound (kVtMin), bit 2 = the nodal quantities are recomputed at every
+ * Newton step (kVtIterat) instead of only at the first one (EqsUVS2D.cpp:191, 311).
+ * st, kdest: per element ($TURBULENCE no vt st estx esty KDest, Type.cpp:681-683); cm, cd: KDCONST.
+ * Returns -1 for a code outside these. */
+int orc_set_turbulence(orc_model* m, int code, const double* st, const double* kdest,
+                       double cm, double cd)
+{
+  const int model = code % 10, aniso = (code / 10) % 10, hund = (code / 100) % 10;
+  if (code < 0 || (model != 1 && model != 2) || aniso != 0 || hund > 3 || code >= 1000)
+    return -1;
+  m->turb = code;
+  for (int e = 0; e < m->ne; e++)
+    m->ste_e[e] = (st ? st[e] : 1.0) * (kdest ? kdest[e] : 0.6);   /* Type.cpp:73-74 */
+  m->cm = cm; m->cd = cd;
+  m->turb_stale = 1;
+  return 0;
+}
+
+/* GRID::Turbulence, algebraic branch (Turbulence.cpp:292-352): per node the mean of
+ * st * KDest over the incident elements (all wet here), then vt = ste H Utau with
+ * Utau = sqrt(cf (U^2 + V^2)), H = S - z unclipped, and uu = vv = -2/3 Utau^2
+ * sqrt(cK ste / cm / cd), cK = 1 / sqrt(cf); uv = 0.  Nodes of no element keep zeros.
+ * Uses the nodal friction coefficient of the current state (do_friction first). */
+static void turbulence_algebraic(orc_model* m)
+{
+  int* cnt = xcalloc((size_t)m->np, sizeof(int));
+  double* ste = xcalloc((size_t)m->np, sizeof(double));
+  for (int n = 0; n < m->np; n++) { m->vtn[n] = 0.0; m->uun[n] = 0.0; }
+  for (int e = 0; e < m->ne; e++)
+    for (int i = 0; i < m->nnd[e]; i++) {
+      const int no = m->elem[8 * e + i];
+      ste[no] += m->ste_e[e];
+      cnt[no]++;
+    }
+  for (int n = 0; n < m->np; n++) {
+    if (!cnt[n]) continue;
+    ste[n] /= cnt[n];
+    const double U = m->U[n], V = m->V[n], H = m->S[n] - m->z[n];
+    const double Utau = sqrt(m->cf[n] * (U * U + V * V));
+    const double cK = 1.0 / sqrt(m->cf[n]);
+    m->vtn[n] = ste[n] * H * Utau;
+    m->uun[n] = -2.0 / 3.0 * Utau * Utau * sqrt(cK * ste[n] / m->cm / m->cd);
+  }
+  free(cnt); free(ste);
+}
+
+void orc_get_turbulence(const orc_model* m, double* vt, double* uu)
+{
+  if (vt) memcpy(vt, m->vtn, sizeof(double) * m->np);
+  if (uu) memcpy(uu, m->uun, sizeof(double) * m->np);
+}
+
+void orc_turbulence(orc_model* m, double* vt, double* uu)
+{
+  do_friction(m);
+  turbulence_algebraic(m);
+  m->turb_stale = 0;
+  if (vt) memcpy(vt, m->vtn, sizeof(double) * m->np);
+  if (uu) memcpy(uu, m->uun, sizeof(double) * m->np);
 }
 
 double orc_lindner(double dp, double sp, double va, double ha, double cb, double vk, double g)
@@ -818,9 +894,14 @@ static void set_index_mat(orc_model* m)
 
 int orc_prepare(orc_model* m)
 {
-  /* EqsUVS2D.cpp:300-360 (DoFriction, Turbulence: constant model needs no nodal data,
-   * InitBcon, SetLocation, SetNormal, SetRotation, SetEqno on first use) */
+  /* EqsUVS2D.cpp:300-360 (DoFriction; Turbulence at the first step after a state change
+   * or every step with kVtIterat, the constant model needs no nodal data; InitBcon,
+   * SetLocation, SetNormal, SetRotation, SetEqno on first use) */
   do_friction(m);
+  if (m->turb % 10 == 2 && (m->turb_stale || ((m->turb / 100) % 10 & 2))) {
+    turbulence_algebraic(m);
+    m->turb_stale = 0;
+  }
   init_bcon(m);
   set_location(m);
   set_normal(m);
@@ -961,6 +1042,7 @@ static void coefs_region(orc_model* m, int e, double* K, double* F)
   const int eqidV = nnd, eqidS = 2 * nnd;
   const double gravity = m->g;
   const double vtype = m->vt[e];
+  const int alg = m->turb % 10 == 2, vtmin = (m->turb / 100) % 10 & 1;
   memset(K, 0, sizeof(double) * MAXEQ * MAXEQ);
   memset(F, 0, sizeof(double) * MAXEQ);
   double area = 0.0;
@@ -1001,6 +1083,7 @@ static void coefs_region(orc_model* m, int e, double* K, double* F)
     }
     double U = 0.0, dUdt = 0.0, dUdx = 0.0, dUdy = 0.0;
     double V = 0.0, dVdt = 0.0, dVdx = 0.0, dVdy = 0.0;
+    double uu = 0.0;              /* Reynolds stresses: vv = uu, uv = 0 */
     for (int j = 0; j < nnd; j++) {
       int k = nd[j];
       double ndU = m->U[k], ndV = m->V[k];
@@ -1008,8 +1091,14 @@ static void coefs_region(orc_model* m, int e, double* K, double* F)
       dUdx += dndx[j] * ndU; dUdy += dndy[j] * ndU;
       V += n[j] * ndV; dVdt += n[j] * m->dVdt[k];
       dVdx += dndx[j] * ndV; dVdy += dndy[j] * ndV;
+      if (alg) uu += n[j] * m->uun[k];
     }
-    double vt = vtype;            /* constant eddy viscosity of the element type */
+    const double vv = uu;
+    double vt = 0.0;
+    if (!alg) vt = vtype;         /* constant eddy viscosity of the element type */
+    else
+      for (int j = 0; j < nnd; j++) vt += n[j] * m->vtn[nd[j]];
+    if (vtmin && vt < vtype) vt = vtype;
     vt += m->vk;
     double Ures = sqrt(U * U + V * V);
 
@@ -1020,6 +1109,7 @@ static void coefs_region(orc_model* m, int e, double* K, double* F)
       f += H * (U * dUdx + V * dUdy);
       fx = H * vt * (dUdx + dUdx);
       fy = H * vt * (dUdy + dVdx);
+      fx -= H * uu;
       f += H * gravity * dadx;
       fx -= H * H * gravity / 2.0;
       f += cf * Ures * U;
@@ -1030,6 +1120,7 @@ static void coefs_region(orc_model* m, int e, double* K, double* F)
       f += H * (U * dVdx + V * dVdy);
       fx = H * vt * (dVdx + dUdy);
       fy = H * vt * (dVdy + dVdy);
+      fy -= H * vv;
       f += H * gravity * dady;
       fy -= H * H * gravity / 2.0;
       f += cf * Ures * V;
@@ -1078,6 +1169,7 @@ static void coefs_region(orc_model* m, int e, double* K, double* F)
       df__ += weight * (U * dUdx + V * dUdy);
       dfx_ = weight * vt * (dUdx + dUdx);
       dfy_ = weight * vt * (dUdy + dVdx);
+      dfx_ -= weight * uu;
       df__ += weight * gravity * dadx;
       dfx_ -= weight * gravity * H;
       for (int j = 0; j < ncn; j++) { t[j] = df__ * mm[j]; tx[j] = dfx_ * mm[j]; ty[j] = dfy_ * mm[j]; }
@@ -1116,6 +1208,7 @@ static void coefs_region(orc_model* m, int e, double* K, double* F)
       df__ += weight * (U * dVdx + V * dVdy);
       dfx_ = weight * vt * (dVdx + dUdy);
       dfy_ = weight * vt * (dVdy + dVdy);
+      dfy_ -= weight * vv;
       df__ += weight * gravity * dady;
       dfy_ -= weight * gravity * H;
       for (int j = 0; j < ncn; j++) { t[j] = df__ * mm[j]; tx[j] = dfx_ * mm[j]; ty[j] = dfy_ * mm[j]; }

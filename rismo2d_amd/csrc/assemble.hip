@@ -118,12 +118,14 @@ void upload_shapes(hipStream_t s)
 
 struct NodeView {
   const double *x, *y, *z, *S, *U, *V, *cf, *niox, *nioy, *valU, *valS, *rotm;
+  const double *vtn, *uun;      // algebraic model: nodal eddy viscosity, uu (= vv; uv = 0)
   const uint32_t* flag;
   const int* eqno;
 };
 
 struct AsmParams {
   double g, vk, hmin, relaxUV, relaxH;
+  int turbAlg, turbMin;         // vt from the nodes (kVtAlgebraic); element vt as floor (kVtMin)
 };
 
 // Rows of one region element node: U_i, V_i, H_i (corner only).
@@ -157,7 +159,6 @@ __device__ __forceinline__ void region_rows(const NodeView& nv, const int* nd, i
   for (int c = 0; c < NL; c++) { KU[c] = 0.0; KV[c] = 0.0; KH[c] = 0.0; }
   FU = FV = FH = 0.0;
   area = 0.0;
-  const double vt = vte + P.vk;
   const double grav = P.g;
 
   for (int g = 0; g < NGP; g++) {
@@ -200,6 +201,19 @@ __device__ __forceinline__ void region_rows(const NodeView& nv, const int* nd, i
       dUdx += dndx[k] * Un[k]; dUdy += dndy[k] * Un[k];
       dVdx += dndx[k] * Vn[k]; dVdy += dndy[k] * Vn[k];
     }
+    // eddy viscosity and Reynolds stresses (vv = uu, uv = 0 in the algebraic model)
+    double vt = vte, uu = 0.0;
+    if (P.turbAlg) {
+      vt = 0.0;
+#pragma unroll
+      for (int k = 0; k < NND; k++) {
+        vt += s.f[g][k] * nv.vtn[nd[k]];
+        uu += s.f[g][k] * nv.uun[nd[k]];
+      }
+    }
+    if (P.turbMin && vt < vte) vt = vte;
+    vt += P.vk;
+    const double vv = uu;
     const double Ures = sqrt(U * U + V * V);
     const double iUres = (Ures > 1.0e-9) ? 1.0 / Ures : 0.0;
     const double ni = s.f[g][i], nxi = dndx[i], nyi = dndy[i];
@@ -208,12 +222,12 @@ __device__ __forceinline__ void region_rows(const NodeView& nv, const int* nd, i
     // residual rows (force = -R)
     {
       double f = H * (U * dUdx + V * dUdy) + H * grav * dadx + cf * Ures * U;
-      double fx = H * vt * (dUdx + dUdx) - H * H * grav / 2.0;
+      double fx = (H * vt * (dUdx + dUdx) - H * uu) - H * H * grav / 2.0;
       double fy = H * vt * (dUdy + dVdx);
       FU -= wgt * (ni * f + nxi * fx + nyi * fy);
       f = H * (U * dVdx + V * dVdy) + H * grav * dady + cf * Ures * V;
       fx = H * vt * (dVdx + dUdy);
-      fy = H * vt * (dVdy + dVdy) - H * H * grav / 2.0;
+      fy = (H * vt * (dVdy + dVdy) - H * vv) - H * H * grav / 2.0;
       FV -= wgt * (ni * f + nxi * fx + nyi * fy);
       if (i < NCN) {
         const double fc = H * (dUdx + dVdy) + U * dHdx + V * dHdy;
@@ -233,7 +247,7 @@ __device__ __forceinline__ void region_rows(const NodeView& nv, const int* nd, i
         KU[NND + k] += ni * (dUV * s.f[g][k]) + nyi * (dyx * dndx[k]);
       }
       const double dH = wgt * (U * dUdx + V * dUdy) + wgt * grav * dadx;
-      const double dHx = wgt * vt * (dUdx + dUdx) - wgt * grav * H;
+      const double dHx = (wgt * vt * (dUdx + dUdx) - wgt * uu) - wgt * grav * H;
       const double dHy = wgt * vt * (dUdy + dVdx);
 #pragma unroll
       for (int k = 0; k < NCN; k++) {
@@ -255,7 +269,7 @@ __device__ __forceinline__ void region_rows(const NodeView& nv, const int* nd, i
       }
       const double dH = wgt * (U * dVdx + V * dVdy) + wgt * grav * dady;
       const double dHx = wgt * vt * (dVdx + dUdy);
-      const double dHy = wgt * vt * (dVdy + dVdy) - wgt * grav * H;
+      const double dHy = (wgt * vt * (dVdy + dVdy) - wgt * vv) - wgt * grav * H;
 #pragma unroll
       for (int k = 0; k < NCN; k++) {
         const double m = s.lf[g][k];
@@ -578,6 +592,34 @@ __global__ void k_friction(int np, const int* __restrict__ n2e_ptr, const int* _
   cf[n] = (b > a) ? sum / (double)(b - a) : 0.0;
 }
 
+// ---- algebraic eddy viscosity (GRID::Turbulence, kVtAlgebraic, Turbulence.cpp:292-352) ----
+// Per node: ste = mean of st*KDest over the incident elements (ascending element order),
+// H = S - z unclipped, Utau = sqrt(cf (U^2 + V^2)), vt = ste H Utau,
+// uu = vv = -2/3 Utau^2 sqrt(cK ste / cm / cd) with cK = 1/sqrt(cf); uv = 0.
+__global__ void k_turb_alg(int np, const int* __restrict__ n2e_ptr, const int* __restrict__ n2e,
+                           const double* __restrict__ ste_e, const double* __restrict__ S,
+                           const double* __restrict__ z, const double* __restrict__ U,
+                           const double* __restrict__ V, const double* __restrict__ cf,
+                           double cm, double cd, double* __restrict__ vt, double* __restrict__ uu)
+{
+  const int n = blockIdx.x * blockDim.x + threadIdx.x;
+  if (n >= np) return;
+  const int a = n2e_ptr[n], b = n2e_ptr[n + 1];
+  double v = 0.0, r = 0.0;
+  if (b > a) {
+    double ste = 0.0;
+    for (int q = a; q < b; q++) ste += ste_e[n2e[q]];
+    ste /= (double)(b - a);
+    const double u = U[n], w = V[n], H = S[n] - z[n];
+    const double Utau = sqrt(cf[n] * (u * u + w * w));
+    const double cK = 1.0 / sqrt(cf[n]);
+    v = ste * H * Utau;
+    r = -2.0 / 3.0 * Utau * Utau * sqrt(cK * ste / cm / cd);
+  }
+  vt[n] = v;
+  uu[n] = r;
+}
+
 // ---- inlet discharge distribution (BCONLINE::Inlet, QINLET, quadratic line rule) ----
 __global__ void k_inlet(int nseg, const int* __restrict__ segs, int nin,
                         const int* __restrict__ inodes, const double* __restrict__ x,
@@ -627,6 +669,7 @@ static NodeView node_view(Ctx& c)
   v.x = c.x.p; v.y = c.y.p; v.z = c.z.p; v.S = c.S.p; v.U = c.U.p; v.V = c.V.p; v.cf = c.cf.p;
   v.niox = c.niox.p; v.nioy = c.nioy.p; v.valU = c.valU.p; v.valS = c.valS.p; v.rotm = c.rotm.p;
   v.flag = c.dflag.p; v.eqno = c.eqno.p;
+  v.vtn = c.vtn.p; v.uun = c.uun.p;
   return v;
 }
 
@@ -634,18 +677,43 @@ static AsmParams asm_params(Ctx& c)
 {
   AsmParams p;
   p.g = c.par.g; p.vk = c.par.vk; p.hmin = c.par.hmin(); p.relaxUV = 0.0; p.relaxH = 0.0;
+  p.turbAlg = c.par.turb_alg() ? 1 : 0; p.turbMin = c.par.turb_min() ? 1 : 0;
   return p;
 }
 
-void dev_prepare_step(Ctx& c)
+static void launch_friction(Ctx& c)
 {
-  upload_shapes(c.stream);
   const int np = c.hm.np;
   k_friction<<<(np + 255) / 256, 256, 0, c.stream>>>(np, c.n2e_ptr.p, c.n2e.p, c.d_manning.p,
                                                      c.d_veg_dp.p, c.d_veg_sp.p, c.d_areaFact.p,
                                                      c.S.p, c.z.p, c.U.p, c.V.p, c.par.g,
                                                      c.par.vk, c.par.hmin(), c.cf.p);
   HIPCHECK(hipGetLastError());
+}
+
+static void launch_turbulence(Ctx& c)
+{
+  const int np = c.hm.np;
+  k_turb_alg<<<(np + 255) / 256, 256, 0, c.stream>>>(np, c.n2e_ptr.p, c.n2e.p, c.d_ste.p, c.S.p,
+                                                     c.z.p, c.U.p, c.V.p, c.cf.p, c.par.cm,
+                                                     c.par.cd, c.vtn.p, c.uun.p);
+  HIPCHECK(hipGetLastError());
+  c.turb_stale = false;
+}
+
+void dev_turbulence(Ctx& c)
+{
+  launch_friction(c);
+  launch_turbulence(c);
+}
+
+void dev_prepare_step(Ctx& c)
+{
+  upload_shapes(c.stream);
+  launch_friction(c);
+  // the flow cycle evaluates the eddy viscosity at its first Newton step only, unless
+  // kVtIterat asks for every step (EqsUVS2D.cpp:191, 311-315)
+  if (c.par.turb_alg() && (c.turb_stale || c.par.turb_iterat())) launch_turbulence(c);
   const int nseg = (int)(c.hm.in_segs.size() / 3), nin = (int)c.hm.inlet_nodes.size();
   if (nseg > 0) {
     k_inlet<<<1, 256, 0, c.stream>>>(nseg, c.d_in_segs.p, nin, c.d_inlet_nodes.p, c.x.p, c.y.p,

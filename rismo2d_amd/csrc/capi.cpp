@@ -102,6 +102,10 @@ rmx_status rmx_mesh_upload(rmx_ctx* p, const rmx_mesh_desc* d)
   c.d_elem.upload(hm.elem, s); c.d_nnd.upload(hm.nnd, s);
   c.d_manning.upload(hm.manning, s); c.d_areaFact.upload(hm.areaFact, s);
   c.d_veg_dp.upload(hm.veg_dp, s); c.d_veg_sp.upload(hm.veg_sp, s);
+  c.d_ste.upload(hm.ste, s); c.vtn.alloc(np); c.uun.alloc(np);
+  HIPCHECK(hipMemsetAsync(c.vtn.p, 0, sizeof(double) * np, s));
+  HIPCHECK(hipMemsetAsync(c.uun.p, 0, sizeof(double) * np, s));
+  c.turb_stale = true;
   // node -> region elements (ascending) for the friction gather
   std::vector<int> ptr(np + 1, 0), list;
   for (int e = 0; e < hm.ne; e++)
@@ -147,6 +151,17 @@ rmx_status rmx_set_params(rmx_ctx* p, const rmx_params* q)
   c.par.dry_limit = q->dry_limit; c.par.max_delta_uv = q->max_delta_uv;
   c.par.max_delta_s = q->max_delta_s; c.par.conv_uv = q->conv_uv; c.par.conv_s = q->conv_s;
   c.par.max_us = q->max_us;
+  {
+    const int t = q->turb == 0 ? 1 : q->turb;
+    const int model = t % 10, aniso = (t / 10) % 10, hund = (t / 100) % 10;
+    if (t < 0 || t >= 1000 || (model != 1 && model != 2) || aniso != 0 || hund > 3)
+      fail(RMX_ERR_ARG, "params: turbulence model " + std::to_string(q->turb) +
+                        " not supported (1 constant, 2 algebraic, + 100 kVtMin, + 200 kVtIterat)");
+    c.par.turb = t;
+  }
+  c.par.cm = q->cm > 0.0 ? q->cm : 0.09;
+  c.par.cd = q->cd > 0.0 ? q->cd : 1.00;
+  c.turb_stale = true;
   API_END
 }
 
@@ -159,6 +174,7 @@ rmx_status rmx_state_upload(rmx_ctx* p, const double* U, const double* V, const 
   if (U) c.U.upload(U, np, c.stream);
   if (V) c.V.upload(V, np, c.stream);
   if (S) c.S.upload(S, np, c.stream);
+  c.turb_stale = true;
   HIPCHECK(hipStreamSynchronize(c.stream));
   API_END
 }
@@ -236,6 +252,20 @@ rmx_status rmx_prepare(rmx_ctx* p, int* neq, long long* nnz)
   HIPCHECK(hipStreamSynchronize(c.stream));
   if (neq) *neq = c.n;
   if (nnz) *nnz = c.nnz;
+  API_END
+}
+
+rmx_status rmx_turbulence(rmx_ctx* p, double* vt, double* uu)
+{
+  API_BEGIN
+  Ctx& c = *need(p);
+  if (!c.mesh_ready) fail(RMX_ERR_ARG, "turbulence: upload a mesh first");
+  if (!c.par.turb_alg()) fail(RMX_ERR_ARG, "turbulence: params.turb selects no algebraic model");
+  dev_turbulence(c);
+  const int np = c.hm.np;
+  if (vt) HIPCHECK(hipMemcpyAsync(vt, c.vtn.p, sizeof(double) * np, hipMemcpyDeviceToHost, c.stream));
+  if (uu) HIPCHECK(hipMemcpyAsync(uu, c.uun.p, sizeof(double) * np, hipMemcpyDeviceToHost, c.stream));
+  HIPCHECK(hipStreamSynchronize(c.stream));
   API_END
 }
 

@@ -73,6 +73,7 @@ struct DBuf {
 struct HostModel {
   int np = 0, ne = 0;
   std::vector<double> x, y, zor, areaFact, manning, vt, veg_dp, veg_sp;
+  std::vector<double> ste;                    // [ne] st * KDest of the algebraic model
   std::vector<int> elem, nnd;                 // [ne*8], [ne]
   std::vector<int> in_segs, out_segs;
   double Q = 0, S_in = 0, S_out = 0;
@@ -101,7 +102,12 @@ struct Params {
          conv_s = 1e-7, max_us = 100.0;
   double g = 9.81;
   double vk = 1.78e-6 / (1.0 + 0.0337 * 10.0 + 0.00022 * 10.0 * 10.0);
+  int turb = 1;                               // $TM_TURBULENCE code
+  double cm = 0.09, cd = 1.00;                // KDCONST
   double hmin() const { return dry_limit / 2.0; }
+  bool turb_alg() const { return turb % 10 == 2; }
+  bool turb_min() const { return (turb / 100) % 10 & 1; }
+  bool turb_iterat() const { return (turb / 100) % 10 & 2; }
 };
 
 constexpr int RING = 8192;     // LDS ring of solved x values in the packed sweeps
@@ -188,6 +194,8 @@ struct Ctx {
   DBuf<int> n2e_ptr, n2e;
   DBuf<int> d_elem, d_nnd;                    // region elements [ne*8], [ne]
   DBuf<double> d_manning, d_areaFact, d_veg_dp, d_veg_sp;
+  DBuf<double> d_ste, vtn, uun;               // algebraic model: st*KDest [ne]; nodal vt, uu
+  bool turb_stale = true;                     // nodal vt due at the next Newton step
   DBuf<int> d_in_segs, d_inlet_nodes;
   // ---- model elements for assembly ----
   int nme = 0;
@@ -239,7 +247,8 @@ void host_slots(const HostModel& hm, std::vector<long long>& off, std::vector<in
 void parallel_for(long long n, const std::function<void(long long, long long)>& f);
 
 // device entry points implemented in the .hip files
-void dev_prepare_step(Ctx& c);                 // friction + inlet distribution
+void dev_prepare_step(Ctx& c);                 // friction (+ nodal vt when due) + inlet
+void dev_turbulence(Ctx& c);                   // friction + algebraic nodal vt, uu now
 void dev_assemble(Ctx& c, int prec, bool zero);
 double dev_scale_l2(Ctx& c);
 void dev_convert(Ctx& c, int to_prec);

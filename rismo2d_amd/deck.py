@@ -46,6 +46,10 @@ class Deck:
     name: str = "deck"
     veg_dp: np.ndarray | None = None   # [ne] stem diameter of emergent vegetation (m), 0: none
     veg_sp: np.ndarray | None = None   # [ne] stem spacing (m)
+    turb: int = 1                      # turbulence model code ($TM_TURBULENCE): 1 constant vt,
+                                       # 2 algebraic; +100 vt of the element is a lower bound
+    turb_st: np.ndarray | None = None  # [ne] st of the algebraic model (None: 1.0)
+    turb_kdest: np.ndarray | None = None   # [ne] KDest of the algebraic model (None: 0.6)
 
     @property
     def n_nodes(self) -> int:
@@ -99,7 +103,8 @@ def channel(nx: int = 200, ny: int = 20, length: float = 1000.0, width: float = 
             vt: float = 0.1, Q: float = 100.0, S_out: float = 2.0, S_in: float = 2.1,
             U_init: float = 0.5, quads: bool = False, name: str | None = None,
             veg_frac: float = 0.0, veg_band: float = 0.3, veg_dp=(0.01, 0.05),
-            veg_density=(0.5, 5.0), veg_seed: int = 0) -> Deck:
+            veg_density=(0.5, 5.0), veg_seed: int = 0, turb: int = 1, turb_st=None,
+            turb_kdest=None, turb_seed: int = 0) -> Deck:
     """Straight rectangular channel on a (2nx+1) x (2ny+1) lattice.
 
     Each cell becomes two six-node triangles split along its (i,j)->(i+2,j+2) diagonal,
@@ -111,6 +116,8 @@ def channel(nx: int = 200, ny: int = 20, length: float = 1000.0, width: float = 
     With ``veg_frac`` > 0, a seeded fraction of the elements in the outer ``veg_band`` of the
     width on either bank (the floodplain strips) carries emergent vegetation: stem diameter
     U[veg_dp] m and density U[veg_density] per m^2 (spacing = density^-1/2).
+    ``turb`` selects the turbulence model code; ``turb_st`` / ``turb_kdest`` are a value
+    or a (low, high) range sampled per element with ``turb_seed`` (None: model defaults).
     """
     NX, NY = 2 * nx + 1, 2 * ny + 1
     xs = np.linspace(0.0, length, NX)
@@ -169,6 +176,16 @@ def channel(nx: int = 200, ny: int = 20, length: float = 1000.0, width: float = 
         veg_d = np.where(pick, rng.uniform(veg_dp[0], veg_dp[1], ne), 0.0)
         dens = rng.uniform(veg_density[0], veg_density[1], ne)
         veg_s = np.where(pick, 1.0 / np.sqrt(dens), 0.0)
+    rng_t = np.random.default_rng(turb_seed + 11)
+
+    def per_elem(v):
+        if v is None:
+            return None
+        if np.ndim(v) == 0:
+            return np.full(ne, float(v))
+        return rng_t.uniform(v[0], v[1], ne)
+
+    t_st, t_kd = per_elem(turb_st), per_elem(turb_kdest)
     return Deck(x=x.astype(np.float64), y=y.astype(np.float64), z=z.astype(np.float64),
                 elems=elems.astype(np.int32), nnd=nnd,
                 manning=np.full(ne, manning), vt=np.full(ne, vt),
@@ -176,7 +193,7 @@ def channel(nx: int = 200, ny: int = 20, length: float = 1000.0, width: float = 
                 U0=np.full(npn, U_init), V0=np.zeros(npn), S0=np.full(npn, S_out),
                 Q_inlet=Q, S_inlet=S_in, S_outlet=S_out,
                 name=name or f"channel_{nx}x{ny}{'_quad' if quads else ''}",
-                veg_dp=veg_d, veg_sp=veg_s)
+                veg_dp=veg_d, veg_sp=veg_s, turb=turb, turb_st=t_st, turb_kdest=t_kd)
 
 
 # ---------------------------------------------------------------------------------------
@@ -314,15 +331,18 @@ def write_reference_deck(deck: Deck, outdir: str, solver: str = "1 6 1 0 60 1000
                          dryrew: str = "2 1000 0.001 0.005 100 5") -> None:
     """Write the reference's keyword input files for ``deck`` into ``outdir``.
 
-    One material per distinct (Manning n, vt, stem diameter, stem spacing).  Values are
-    written with %.17g so that the reference reads back the exact doubles used by the
-    oracle.
+    One material per distinct (Manning n, vt, stem diameter, stem spacing, st, KDest).
+    Values are written with %.17g so that the reference reads back the exact doubles used
+    by the oracle.
     """
     os.makedirs(outdir, exist_ok=True)
     ne = deck.n_elems
     vd = deck.veg_dp if deck.veg_dp is not None else np.zeros(ne)
     vs = deck.veg_sp if deck.veg_sp is not None else np.zeros(ne)
-    keys = list(zip(deck.manning.tolist(), deck.vt.tolist(), vd.tolist(), vs.tolist()))
+    st = deck.turb_st if deck.turb_st is not None else np.full(ne, 1.0)
+    kd = deck.turb_kdest if deck.turb_kdest is not None else np.full(ne, 0.6)
+    keys = list(zip(deck.manning.tolist(), deck.vt.tolist(), vd.tolist(), vs.tolist(),
+                    st.tolist(), kd.tolist()))
     mat_of = {p: i + 1 for i, p in enumerate(sorted(set(keys)))}
     with open(os.path.join(outdir, "region.dat"), "w") as f:
         f.write(f"{deck.n_nodes} {deck.n_elems} 0\n")
@@ -346,14 +366,15 @@ def write_reference_deck(deck: Deck, outdir: str, solver: str = "1 6 1 0 60 1000
             f.write(f"{i + 1} {deck.U0[i]:.17g} {deck.V0[i]:.17g} {deck.S0[i]:.17g}\n")
     with open(os.path.join(outdir, "mat.dat"), "w") as f:
         f.write("$RISMO2D 40000\n")
-        for (n, vt, dp, sp), no in mat_of.items():
+        for (n, vt, dp, sp, a, b), no in mat_of.items():
             f.write(f"$ROUGHNESS {no} 5 {n:.17g} 0 0.0 {dp:.17g} {sp:.17g}\n")
-        for (n, vt, dp, sp), no in mat_of.items():
-            f.write(f"$TURBULENCE {no} {vt:.17g} 0.0 0.0 0.0 0.0 0.0 0.0\n")
+        for (n, vt, dp, sp, a, b), no in mat_of.items():
+            # no vt st estx esty KDest lm betaSf (TYPE::Import_40000)
+            f.write(f"$TURBULENCE {no} {vt:.17g} {a:.17g} 0.0 0.0 {b:.17g} 0.0 0.0\n")
     with open(os.path.join(outdir, "tm.dat"), "w") as f:
         f.write("$RISMO2D 40100\n$TM_STEPS 1 1 0\n$TM_INTERVAL 1000000.0 0.0 0.0\n"
                 "$TM_WEIGHT 1.0 1.0 1.0\n$TM_STEP_NO 1\n$TM_CYCLE 1\n$TM_STATIONARY 1\n"
-                f"$TM_TURBULENCE 1\n$TM_MAXITER {maxiter}\n$TM_SOLVER {tm_solver}\n"
+                f"$TM_TURBULENCE {deck.turb}\n$TM_MAXITER {maxiter}\n$TM_SOLVER {tm_solver}\n"
                 f"$TM_LINE QINLET {INLET_LINE} {deck.Q_inlet:.17g} {deck.S_inlet:.17g}\n"
                 f"$TM_LINE OUTLET {OUTLET_LINE} {deck.S_outlet:.17g} 1000.0 0.0 0 0.0\n")
     with open(os.path.join(outdir, "input.dat"), "w") as f:

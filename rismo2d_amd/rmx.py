@@ -38,13 +38,15 @@ class MeshDesc(C.Structure):
                 ("elems", _i), ("nnd", _i), ("manning", _d), ("vt", _d),
                 ("n_inlet", C.c_int), ("inlet_segs", _i), ("n_outlet", C.c_int),
                 ("outlet_segs", _i), ("Q", C.c_double), ("S_in", C.c_double),
-                ("S_out", C.c_double), ("veg_dp", _d), ("veg_sp", _d)]
+                ("S_out", C.c_double), ("veg_dp", _d), ("veg_sp", _d), ("turb_st", _d),
+                ("turb_kdest", _d)]
 
 
 class Params(C.Structure):
     _fields_ = [("relax_method", C.c_int), ("dry_method", C.c_int), ("dry_limit", C.c_double),
                 ("max_delta_uv", C.c_double), ("max_delta_s", C.c_double),
-                ("conv_uv", C.c_double), ("conv_s", C.c_double), ("max_us", C.c_double)]
+                ("conv_uv", C.c_double), ("conv_s", C.c_double), ("max_us", C.c_double),
+                ("turb", C.c_int), ("cm", C.c_double), ("cd", C.c_double)]
 
 
 class KrylovOpts(C.Structure):
@@ -101,6 +103,7 @@ def lib():
             "rmx_prepare": [_v, _i, C.POINTER(C.c_longlong)], "rmx_get_numbering": [_v, _i],
             "rmx_get_pattern": [_v, _i, _i], "rmx_get_nodal": [_v, _d, _d, _i, _d],
             "rmx_n_model_elems": [_v, _i], "rmx_element_coefs": [_v, C.c_int, C.c_int, _d, _d],
+            "rmx_turbulence": [_v, _d, _d],
             "rmx_assemble": [_v, C.c_int], "rmx_scale_l2": [_v, _d],
             "rmx_convert_matrix": [_v, C.c_int], "rmx_ilu0_factor": [_v, C.c_int],
             "rmx_ilu0_apply": [_v, C.c_int, _d, _d], "rmx_spmv": [_v, C.c_int, _d, _d],
@@ -181,15 +184,40 @@ class Context:
             k["vsp"] = _c(deck.veg_sp, np.float64)
             d.veg_dp = _p(k["vdp"], _d)
             d.veg_sp = _p(k["vsp"], _d)
+        if getattr(deck, "turb_st", None) is not None:
+            k["tst"] = _c(deck.turb_st, np.float64)
+            d.turb_st = _p(k["tst"], _d)
+        if getattr(deck, "turb_kdest", None) is not None:
+            k["tkd"] = _c(deck.turb_kdest, np.float64)
+            d.turb_kdest = _p(k["tkd"], _d)
         _check(lib().rmx_mesh_upload(self.h, C.byref(d)))
         self.np = deck.n_nodes
+        # the deck's turbulence model becomes active with the other parameters kept
+        self.deck_turb = int(getattr(deck, "turb", 1))
+        kw = dict(getattr(self, "_params", {}))
+        kw["turb"] = self.deck_turb
+        self.set_params(**kw)
         self.state_upload(deck.U0, deck.V0, deck.S0)
 
     def set_params(self, relax_method=2, dry_method=2, dry_limit=1e-3, max_delta_uv=10.0,
-                   max_delta_s=10.0, conv_uv=1e-7, conv_s=1e-7, max_us=100.0):
+                   max_delta_s=10.0, conv_uv=1e-7, conv_s=1e-7, max_us=100.0, turb=None,
+                   cm=0.09, cd=1.0):
+        """turb: $TM_TURBULENCE code; None takes the uploaded deck's."""
+        if turb is None:
+            turb = getattr(self, "deck_turb", 1)
+        self._params = dict(relax_method=relax_method, dry_method=dry_method,
+                            dry_limit=dry_limit, max_delta_uv=max_delta_uv,
+                            max_delta_s=max_delta_s, conv_uv=conv_uv, conv_s=conv_s,
+                            max_us=max_us, cm=cm, cd=cd)
         p = Params(relax_method, dry_method, dry_limit, max_delta_uv, max_delta_s, conv_uv,
-                   conv_s, max_us)
+                   conv_s, max_us, turb, cm, cd)
         _check(lib().rmx_set_params(self.h, C.byref(p)))
+
+    def turbulence(self):
+        """Friction and the algebraic model's nodal vt, uu of the current state."""
+        vt, uu = np.zeros(self.np), np.zeros(self.np)
+        _check(lib().rmx_turbulence(self.h, _p(vt, _d), _p(uu, _d)))
+        return vt, uu
 
     def state_upload(self, U, V, S):
         U, V, S = _c(U, np.float64), _c(V, np.float64), _c(S, np.float64)

@@ -12,6 +12,11 @@ the reference's input format, runs the reference and stores what the harness rec
 * ``quad10x3.npz``  8-node quads, 10 x 3 cells, same records.
 * ``veg20x4.npz``   the 20 x 4 triangle channel with emergent vegetation (Lindner) on
                     60 % of the elements in the outer halves of the width, same records.
+* ``turb20x4.npz``  the 20 x 4 triangle channel with the algebraic eddy viscosity model
+                    ($TM_TURBULENCE 2, st uniform in [0.5, 1.5] per element, KDest 0.6),
+                    same records plus the nodal vt, uu, uv, vv of every step.
+* ``turbq10x3.npz`` the 10 x 3 quad channel with the algebraic model and the element's vt
+                    0.04 as lower bound ($TM_TURBULENCE 102), KDest in [0.4, 0.8].
 * ``channel200x20.npz``  the straight-channel workload (200 x 20 cells, 8,000 triangles,
                     bed noise sigma 0.05 m / 20 m, seed 0): numbering, per-step scale and
                     correction, BiCGStab iteration counts from the report, final state,
@@ -49,6 +54,11 @@ DECKS = {
     "veg20x4": dict(nx=20, ny=4, length=100.0, width=20.0, noise_sigma=0.05, noise_corr=5.0,
                     Q=10.0, veg_frac=0.6, veg_band=0.5),
     "channel200x20": dict(nx=200, ny=20),
+    "turb20x4": dict(nx=20, ny=4, length=100.0, width=20.0, noise_sigma=0.05, noise_corr=5.0,
+                     Q=10.0, turb=2, turb_st=(0.5, 1.5), turb_kdest=0.6),
+    "turbq10x3": dict(nx=10, ny=3, length=100.0, width=30.0, noise_sigma=0.05, noise_corr=5.0,
+                      Q=10.0, quads=True, vt=0.04, turb=102, turb_st=(0.5, 1.5),
+                      turb_kdest=(0.4, 0.8)),
 }
 
 
@@ -71,14 +81,19 @@ def deck_arrays(d):
     if d.veg_dp is not None:
         out["veg_dp"] = d.veg_dp
         out["veg_sp"] = d.veg_sp
+    if d.turb != 1:
+        out["turb"] = d.turb
+        out["turb_st"] = d.turb_st
+        out["turb_kdest"] = d.turb_kdest
     return out
 
 
 def state_arrays(prefix, st):
     out = {}
     for k in ("U", "V", "S", "z", "cf", "kind", "rot", "bcU", "bcS", "r00", "r01", "r10", "r11",
-              "eqU", "eqV", "eqS", "nx", "ny", "niox", "nioy"):
-        out[f"{prefix}{k}"] = st[k]
+              "eqU", "eqV", "eqS", "nx", "ny", "niox", "nioy", "vt", "uu", "uv", "vv"):
+        if k in st:
+            out[f"{prefix}{k}"] = st[k]
     return out
 
 

@@ -10,7 +10,7 @@ import numpy as np
 
 STATE_FIELDS = ["name", "flag", "kind", "rot", "eqU", "eqV", "eqS", "x", "y", "z", "zor",
                 "U", "V", "S", "cf", "vt", "nx", "ny", "niox", "nioy", "bcU", "bcS",
-                "r00", "r01", "r11", "r10"]
+                "r00", "r01", "r11", "r10", "uu", "uv", "vv"]
 
 
 def _records(path):
@@ -36,7 +36,7 @@ def read_dump(path):
             hdr = np.frombuffer(buf, np.int64)
         elif tag in ("STATE", "FINAL"):
             a = np.frombuffer(buf, np.float64).reshape(int(hdr[0]), int(hdr[1]))
-            st = {k: a[:, i].copy() for i, k in enumerate(STATE_FIELDS)}
+            st = {k: a[:, i].copy() for i, k in enumerate(STATE_FIELDS[:a.shape[1]])}
             if tag == "STATE":
                 cur["state"] = st
             else:

@@ -19,4 +19,6 @@ def golden_deck(g):
                    lines={dk.INLET_LINE: g["inlet"], dk.OUTLET_LINE: g["outlet"]},
                    U0=g["U0"], V0=g["V0"], S0=g["S0"], Q_inlet=float(g["Q"]),
                    S_inlet=float(g["S_in"]), S_outlet=float(g["S_out"]),
-                   veg_dp=g.get("veg_dp"), veg_sp=g.get("veg_sp"))
+                   veg_dp=g.get("veg_dp"), veg_sp=g.get("veg_sp"),
+                   turb=int(g["turb"]) if "turb" in g else 1, turb_st=g.get("turb_st"),
+                   turb_kdest=g.get("turb_kdest"))

@@ -9,6 +9,7 @@ Tolerances (floating point):
   GPU contracts to FMA and sums element contributions in colour order);
 * nodal friction and inlet discharge: 1e-13 relative (1e-10 at nodes with Lindner's
   vegetation iteration, which amplifies the ulp differences of device pow / acos / cos);
+* nodal eddy viscosity and Reynolds stress of the algebraic model: 1e-13 relative;
 * SpMV, ILU apply (FP64 accumulation in another order): 1e-12 relative to max|result|;
 * Newton-converged u, v, h: within 1e-8 relative of the reference / oracle.
 """
@@ -40,12 +41,13 @@ def _rowmax(rp, A):
     return m[rows], rows
 
 
-@pytest.mark.parametrize("name", ["tri20x4", "quad10x3", "veg20x4"])
+@pytest.mark.parametrize("name", ["tri20x4", "quad10x3", "veg20x4", "turb20x4", "turbq10x3"])
 def test_setup_and_element_kernel_match_oracle(ctx, name):
     print(f"[gpu] setup/element parity on {name}", flush=True)
     g = load_golden(name)
     deck = golden_deck(g)
     ctx.upload_mesh(deck)
+    ctx.set_params()
     n = ctx.prepare()
     om = po.OracleModel(deck)
     assert om.prepare() == n
@@ -67,6 +69,13 @@ def test_setup_and_element_kernel_match_oracle(ctx, name):
     np.testing.assert_allclose(nod["cf"][veg], onod["cf"][veg], rtol=1e-10, atol=0)
     np.testing.assert_allclose(nod["bcU"], onod["bcU"], rtol=1e-13, atol=0)
     np.testing.assert_array_equal(nod["rotm"], onod["rotm"])
+    if deck.turb % 10 == 2:
+        # algebraic model: the oracle's nodal vt, uu are bit-exact with the reference's
+        vt, uu = ctx.turbulence()
+        ovt, ouu = om.turbulence_state()
+        np.testing.assert_array_equal(ovt, g["s0_vt"])
+        np.testing.assert_allclose(vt, ovt, rtol=1e-13, atol=0)
+        np.testing.assert_allclose(uu, ouu, rtol=1e-13, atol=0)
     ne = ctx.n_model_elems()
     K, F = ctx.element_coefs(0, ne)
     for k in range(ne):
@@ -129,14 +138,17 @@ def _relerr(a, b, scale):
     return np.abs(a - b).max() / scale
 
 
-@pytest.mark.parametrize("name", ["tri20x4", "quad10x3", "veg20x4", "channel200x20"])
+@pytest.mark.parametrize("name", ["tri20x4", "quad10x3", "veg20x4", "turb20x4", "turbq10x3",
+                                  "channel200x20"])
 def test_newton_converges_to_reference_solution(ctx, name):
     """Full Newton iteration on the GPU (FP64 assembly, FP64 ILU0-BiCGStab) reaches the
-    reference program's converged u, v, h within 1e-8 relative."""
+    reference program's converged u, v, h within 1e-8 relative (the algebraic model's
+    eddy viscosity frozen at the first step, as the reference's flow cycle keeps it)."""
     print(f"[gpu] Newton FP64 on {name}", flush=True)
     g = load_golden(name)
     deck = golden_deck(g)
     ctx.upload_mesh(deck)
+    ctx.set_params()
     n = ctx.prepare()
     if name == "channel200x20":
         rp, cols = ctx.pattern()
@@ -387,3 +399,15 @@ def test_error_paths(ctx):
     with pytest.raises(rmx.RmxError) as e:
         ctx.ilu0_factor(rmx.F64)
     assert e.value.code == -3
+    # turbulence model codes outside the supported set are refused; the nodal algebraic
+    # model needs code 2 (+100 / +200)
+    for code in (3, 12, 402, -2):
+        with pytest.raises(rmx.RmxError):
+            ctx.set_params(turb=code)
+    ctx.upload_mesh(golden_deck(load_golden("tri20x4")))
+    with pytest.raises(rmx.RmxError):
+        ctx.turbulence()
+    ctx.set_params(turb=302)
+    vt, uu = ctx.turbulence()
+    assert (vt > 0).all() and (uu < 0).all()
+    ctx.set_params(turb=1)

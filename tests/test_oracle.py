@@ -25,7 +25,7 @@ def _check_state(om, g, prefix):
     return nod
 
 
-@pytest.mark.parametrize("name", ["tri20x4", "quad10x3", "veg20x4"])
+@pytest.mark.parametrize("name", ["tri20x4", "quad10x3", "veg20x4", "turb20x4", "turbq10x3"])
 def test_oracle_matches_reference_every_record(name):
     g = load_golden(name)
     om = po.OracleModel(golden_deck(g))
@@ -34,6 +34,16 @@ def test_oracle_matches_reference_every_record(name):
         neq = om.prepare()
         nod = _check_state(om, g, p)
         np.testing.assert_array_equal(nod["cf"], g[p + "cf"])
+        if "turb" in g:
+            # algebraic model: nodal vt and uu as the reference computed them at step 0 and
+            # kept through the later steps; the reference's vv equals uu and uv is zero
+            vt, uu = om.turbulence_state()
+            np.testing.assert_array_equal(vt, g[p + "vt"])
+            np.testing.assert_array_equal(uu, g[p + "uu"])
+            np.testing.assert_array_equal(g[p + "vv"], g[p + "uu"])
+            assert not np.any(g[p + "uv"])
+            if k:
+                np.testing.assert_array_equal(g[p + "vt"], g["s0_vt"])
         np.testing.assert_array_equal(nod["kind"], g[p + "kind"].astype(np.int32))
         np.testing.assert_array_equal(nod["rot"], g[p + "rot"].astype(np.int32))
         np.testing.assert_array_equal(nod["bcU"], g[p + "bcU"])
@@ -133,10 +143,12 @@ def test_lake_at_rest_residual_vanishes():
     assert np.abs(B).max() < 1e-11 * ref
 
 
-def test_element_jacobian_matches_finite_differences():
-    """estifm = d(-force)/dX with friction and eddy viscosity frozen (as the reference
-    freezes them), checked by central differences on interior triangles."""
-    d = _small_channel()
+@pytest.mark.parametrize("turb", [1, 2])
+def test_element_jacobian_matches_finite_differences(turb):
+    """estifm = d(-force)/dX with friction, eddy viscosity and Reynolds stresses frozen
+    (as the reference freezes them), checked by central differences on interior triangles;
+    turb 2 is the algebraic model (its stresses enter the depth column)."""
+    d = _small_channel(turb=turb, turb_st=(0.5, 1.5))
     rng = np.random.default_rng(3)
     U = 0.3 + 0.05 * rng.standard_normal(d.n_nodes)
     V = 0.05 * rng.standard_normal(d.n_nodes)
@@ -237,3 +249,18 @@ def test_fp64_newton_reaches_the_fp32_reference_solution():
         assert np.abs(U - g["final_U"]).max() <= 1e-8 * np.abs(g["final_U"]).max()
         assert np.abs(V - g["final_V"]).max() <= 1e-8 * np.abs(g["final_U"]).max()
         assert np.abs(H - Hr).max() <= 1e-8 * np.abs(Hr).max()
+
+
+def test_vt_floor_is_exercised_by_the_quad_turbulence_deck():
+    """turbq10x3 runs the algebraic model with the element's vt as a floor (code 102): at
+    its step-0 state the floor changes some element Jacobians and leaves others, so the
+    golden records pin both branches."""
+    g = load_golden("turbq10x3")
+    om = po.OracleModel(golden_deck(g))
+    om.prepare()
+    region = [i for i, (nodes, isb) in enumerate(om.model_elems()) if not isb]
+    K102 = [om.coefs(i)[0] for i in region]
+    om.set_turbulence(2)
+    K2 = [om.coefs(i)[0] for i in region]
+    changed = sum(not np.array_equal(a, b) for a, b in zip(K102, K2))
+    assert 0 < changed < len(region)
