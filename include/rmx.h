@@ -111,6 +111,22 @@ rmx_status rmx_n_model_elems(rmx_ctx* ctx, int* n);
  * [np] host arrays, nullable. */
 rmx_status rmx_turbulence(rmx_ctx* ctx, double* vt, double* uu);
 
+/* Time integration of the flow cycle.  A state upload is an initial condition: the
+ * previous time step's state becomes the same, time derivatives zero (GRID::InputInit,
+ * Grid.cpp:1119-1129).  rmx_time_step starts a time step: dt <= 0 stationary flow (no
+ * time terms, EqsUVS2D.cpp:105-127), else the theta scheme: 1 / (dt theta) on the time
+ * terms' diagonal (:130-147) and EQS_UVS2D::Predict (:1101-1180; first = first time step
+ * of the run); every Newton update then recomputes the time derivatives
+ * (EQS_UVS2D::Timegrad, :1183-1230).  The algebraic eddy viscosity becomes due.
+ * rmx_time_commit ends a time step: the state becomes the previous one (Compute.cpp:
+ * 773-796). */
+rmx_status rmx_time_step(rmx_ctx* ctx, double dt, double theta, int first);
+rmx_status rmx_time_commit(rmx_ctx* ctx);
+/* Host copies of the time-integration state, 9 [np] arrays (NULL entries skipped):
+ * Uo, Vo, So, dUdt_o, dVdt_o, dSdt_o (previous time step), dUdt, dVdt, dSdt (current).
+ * set = 1 uploads them, 0 downloads. */
+rmx_status rmx_time_state(rmx_ctx* ctx, int set, double* const* arrays);
+
 /* ------------------------------------------------------------------------------------ */
 /* drop-in group                                                                         */
 

@@ -64,6 +64,23 @@ void   orc_get_state(const orc_model* m, double* U, double* V, double* S, double
 void   orc_get_nodal(const orc_model* m, double* cf, int* bckind, int* rot,
                      double* rotm /* np*4: r00 r01 r10 r11 */, double* bcU, double* bcS);
 
+/* Time integration.  orc_set_state is an initial condition: it also sets the previous
+ * time step's state to the same values with zero time derivatives.  orc_time_step starts
+ * a flow time step: dt <= 0 stationary (no time terms), else the theta scheme with
+ * EQS_UVS2D::Predict (EqsUVS2D.cpp:1101-1180; first = first time step of the run); the
+ * Newton update then recomputes the time derivatives (EQS_UVS2D::Timegrad, :1183-1230).
+ * orc_time_commit ends the time step (Compute.cpp:773-796).  Returns -1 for theta or dt
+ * below 1e-10 in magnitude. */
+int    orc_time_step(orc_model* m, double dt, double theta, int first);
+void   orc_time_commit(orc_model* m);
+/* previous state (vo) and time derivatives (v.d*dt, vo.d*dt): set (NULL keeps) / get */
+void   orc_set_time_state(orc_model* m, const double* Uo, const double* Vo, const double* So,
+                          const double* dUdto, const double* dVdto, const double* dSdto,
+                          const double* dUdt, const double* dVdt, const double* dSdt);
+void   orc_get_time_state(const orc_model* m, double* Uo, double* Vo, double* So,
+                          double* dUdto, double* dVdto, double* dSdto,
+                          double* dUdt, double* dVdt, double* dSdt);
+
 /* Newton-step preparation (EqsUVS2D.cpp:300-360): friction, boundary conditions, normals,
  * rotation, and (once) numbering + CSR pattern.  Returns neq. */
 int    orc_prepare(orc_model* m);

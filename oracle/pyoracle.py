@@ -43,6 +43,11 @@ def lib():
         L.orc_set_turbulence.restype = C.c_int
         L.orc_turbulence.argtypes = [C.c_void_p, _d, _d]
         L.orc_get_turbulence.argtypes = [C.c_void_p, _d, _d]
+        L.orc_time_step.argtypes = [C.c_void_p, C.c_double, C.c_double, C.c_int]
+        L.orc_time_step.restype = C.c_int
+        L.orc_time_commit.argtypes = [C.c_void_p]
+        L.orc_set_time_state.argtypes = [C.c_void_p] + [_d] * 9
+        L.orc_get_time_state.argtypes = [C.c_void_p] + [_d] * 9
         L.orc_lindner.argtypes = [C.c_double] * 7
         L.orc_lindner.restype = C.c_double
         L.orc_set_params.argtypes = [C.c_void_p, C.c_int, C.c_int] + [C.c_double] * 6
@@ -136,6 +141,27 @@ class OracleModel:
         vt, uu = np.zeros(self.np), np.zeros(self.np)
         lib().orc_turbulence(self.h, _p(vt, _d), _p(uu, _d))
         return vt, uu
+
+    TIME_KEYS = ("Uo", "Vo", "So", "dUdto", "dVdto", "dSdto", "dUdt", "dVdt", "dSdt")
+
+    def time_step(self, dt, theta, first=False):
+        """Start a flow time step (dt <= 0: stationary)."""
+        if lib().orc_time_step(self.h, dt, theta, int(first)) != 0:
+            raise ValueError("theta or dt too small")
+
+    def time_commit(self):
+        lib().orc_time_commit(self.h)
+
+    def set_time_state(self, **kw):
+        """Previous state / time derivatives by name (TIME_KEYS); missing ones are kept."""
+        arrs = [None if kw.get(k) is None else np.ascontiguousarray(kw[k], np.float64)
+                for k in self.TIME_KEYS]
+        lib().orc_set_time_state(self.h, *[None if a is None else _p(a, _d) for a in arrs])
+
+    def time_state(self):
+        out = {k: np.zeros(self.np) for k in self.TIME_KEYS}
+        lib().orc_get_time_state(self.h, *[_p(out[k], _d) for k in self.TIME_KEYS])
+        return out
 
     def turbulence_state(self):
         """The nodal vt, uu in use (as the last prepare left them)."""

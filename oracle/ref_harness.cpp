@@ -72,8 +72,9 @@ static void dump_state( const char* tag, PROJECT* project, EQS* eqs )
   GRID* rg = project->M2D->region;
   int np = rg->Getnp();
   // per node: name, flag, bc.kind, bc.rot, eqno[3] (as doubles for a single array)
-  // then x,y,z,zor,U,V,S,cf,vt,nx,ny,niox,nioy,bcU,bcS,rot00,rot01,rot11,rot10,uu,uv,vv
-  const int nf = 29;
+  // then x,y,z,zor,U,V,S,cf,vt,nx,ny,niox,nioy,bcU,bcS,rot00,rot01,rot11,rot10,uu,uv,vv,
+  // dUdt,dVdt,dSdt, previous time step U,V,S,dUdt,dVdt,dSdt
+  const int nf = 38;
   std::vector<double> buf( (size_t)np * nf );
   for( int n=0; n<np; n++ )
   {
@@ -95,6 +96,9 @@ static void dump_state( const char* tag, PROJECT* project, EQS* eqs )
     b[24] = nd->bc.Getrot(1,1);
     b[25] = nd->bc.Getrot(1,0);
     b[26] = nd->uu;    b[27] = nd->uv;    b[28] = nd->vv;
+    b[29] = nd->v.dUdt;  b[30] = nd->v.dVdt;  b[31] = nd->v.dSdt;
+    b[32] = nd->vo.U;    b[33] = nd->vo.V;    b[34] = nd->vo.S;
+    b[35] = nd->vo.dUdt; b[36] = nd->vo.dVdt; b[37] = nd->vo.dSdt;
   }
   int64_t hdr[2] = { np, nf };
   rec( "STATEHDR", hdr, sizeof(hdr) );
@@ -137,6 +141,8 @@ extern "C" int __wrap__ZN3EQS5SolveEP5MODELiPdS2_P7PROJECTP6SOLVERPP6PRECONi(
   g_project = project;
   int64_t s = g_step;
   rec( "STEP", &s, 8 );
+  int64_t tm = project->GetTimeStep();
+  rec( "TSTEP", &tm, 8 );
   if( g_out ) dump_state( "STATE", project, self );
   int err = __real__ZN3EQS5SolveEP5MODELiPdS2_P7PROJECTP6SOLVERPP6PRECONi(
                 self, model, neq, rhs, x, project, solver, precon, assemble );

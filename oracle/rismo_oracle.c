@@ -205,6 +205,10 @@ struct orc_model {
   int turb, turb_stale;
   double cm, cd;
   double *ste_e, *vtn, *uun;
+  /* time integration: state of the previous time step (NODE::vo) and the theta scheme */
+  double *Uo, *Vo, *So, *dUdto, *dVdto, *dSdto;
+  int unsteady;
+  double dt, theta;
   int n_in, n_out;
   int *in_segs, *out_segs;
   double Q, S_in, S_out;
@@ -343,6 +347,9 @@ orc_model* orc_create(int np, const double* x, const double* y, const double* z,
   m->ste_e = xcalloc((size_t)ne, sizeof(double));
   m->vtn = xcalloc((size_t)np, sizeof(double));
   m->uun = xcalloc((size_t)np, sizeof(double));
+  m->Uo = xcalloc((size_t)np, sizeof(double)); m->Vo = xcalloc((size_t)np, sizeof(double));
+  m->So = xcalloc((size_t)np, sizeof(double)); m->dUdto = xcalloc((size_t)np, sizeof(double));
+  m->dVdto = xcalloc((size_t)np, sizeof(double)); m->dSdto = xcalloc((size_t)np, sizeof(double));
   area_factors(m);
   topology(m);
   return m;
@@ -355,7 +362,7 @@ void orc_destroy(orc_model* m)
                  m->nx, m->ny, m->niox, m->nioy, m->valU, m->valS, m->elem, m->nnd, m->manning,
                  m->vt, m->areaFact, m->in_segs, m->out_segs, m->noel, m->flag, m->kind, m->rot,
                  m->eqno, m->bnd, m->bflag, m->me, m->rowptr, m->cols, m->veg_dp, m->veg_sp,
-                 m->ste_e, m->vtn, m->uun };
+                 m->ste_e, m->vtn, m->uun, m->Uo, m->Vo, m->So, m->dUdto, m->dVdto, m->dSdto };
   for (size_t i = 0; i < sizeof(ps) / sizeof(ps[0]); i++) free(ps[i]);
   free(m);
 }
@@ -368,14 +375,103 @@ void orc_set_params(orc_model* m, int relaxMethod, int dryMethod, double dryLimi
   m->hmin = dryLimit / 2.0;                     /* Project.cpp:1125 */
   m->maxDeltaUV = maxDeltaUV; m->maxDeltaS = maxDeltaS;
   m->convUV = convUV; m->convS = convS; m->maxUs = maxUs;
-  m->relaxThdt_UV = 0.0; m->relaxThdt_H = 0.0;  /* steady, relaxMethod < 3 (EqsUVS2D.cpp:124) */
+  if (!m->unsteady) {           /* steady, relaxMethod < 3 (EqsUVS2D.cpp:124) */
+    m->relaxThdt_UV = 0.0; m->relaxThdt_H = 0.0;
+  }
 }
 
 void orc_set_state(orc_model* m, const double* U, const double* V, const double* S)
 {
-  memcpy(m->U, U, sizeof(double) * m->np); memcpy(m->V, V, sizeof(double) * m->np);
-  memcpy(m->S, S, sizeof(double) * m->np);
+  /* an initial condition: the previous time step's state is the same, with zero time
+   * derivatives (GRID::InputInit, Grid.cpp:1119-1129) */
+  const size_t b = sizeof(double) * (size_t)m->np;
+  memcpy(m->U, U, b); memcpy(m->V, V, b); memcpy(m->S, S, b);
+  memcpy(m->Uo, U, b); memcpy(m->Vo, V, b); memcpy(m->So, S, b);
+  memset(m->dUdt, 0, b); memset(m->dVdt, 0, b); memset(m->dSdt, 0, b);
+  memset(m->dUdto, 0, b); memset(m->dVdto, 0, b); memset(m->dSdto, 0, b);
   m->turb_stale = 1;
+}
+
+void orc_set_time_state(orc_model* m, const double* Uo, const double* Vo, const double* So,
+                        const double* dUdto, const double* dVdto, const double* dSdto,
+                        const double* dUdt, const double* dVdt, const double* dSdt)
+{
+  const size_t b = sizeof(double) * (size_t)m->np;
+  if (Uo) memcpy(m->Uo, Uo, b);
+  if (Vo) memcpy(m->Vo, Vo, b);
+  if (So) memcpy(m->So, So, b);
+  if (dUdto) memcpy(m->dUdto, dUdto, b);
+  if (dVdto) memcpy(m->dVdto, dVdto, b);
+  if (dSdto) memcpy(m->dSdto, dSdto, b);
+  if (dUdt) memcpy(m->dUdt, dUdt, b);
+  if (dVdt) memcpy(m->dVdt, dVdt, b);
+  if (dSdt) memcpy(m->dSdt, dSdt, b);
+}
+
+void orc_get_time_state(const orc_model* m, double* Uo, double* Vo, double* So,
+                        double* dUdto, double* dVdto, double* dSdto,
+                        double* dUdt, double* dVdt, double* dSdt)
+{
+  const size_t b = sizeof(double) * (size_t)m->np;
+  if (Uo) memcpy(Uo, m->Uo, b);
+  if (Vo) memcpy(Vo, m->Vo, b);
+  if (So) memcpy(So, m->So, b);
+  if (dUdto) memcpy(dUdto, m->dUdto, b);
+  if (dVdto) memcpy(dVdto, m->dVdto, b);
+  if (dSdto) memcpy(dSdto, m->dSdto, b);
+  if (dUdt) memcpy(dUdt, m->dUdt, b);
+  if (dVdt) memcpy(dVdt, m->dVdt, b);
+  if (dSdt) memcpy(dSdt, m->dSdt, b);
+}
+
+/* Start of a flow time step.  dt <= 0: stationary flow, no time terms (EqsUVS2D.cpp:
+ * 105-127 with relaxMethod < 3).  dt > 0: the theta scheme, relaxThdt = 1 / dt / theta in
+ * the Jacobian (:130-147), then EQS_UVS2D::Predict (:1101-1180): current time derivatives
+ * zero; at nodes whose previous level lies below zor + dryLimit the previous time
+ * derivatives are zero and, at the first time step, the previous state is the current
+ * one.  The algebraic eddy viscosity becomes due (first Newton step of the cycle).
+ * Returns -1 for theta or dt too small (:91-94). */
+int orc_time_step(orc_model* m, double dt, double theta, int first)
+{
+  m->turb_stale = 1;
+  if (dt <= 0.0) {
+    m->unsteady = 0;
+    m->relaxThdt_UV = m->relaxThdt_H = 0.0;
+    return 0;
+  }
+  if (fabs(theta) < 1.0e-10 || fabs(dt) < 1.0e-10) return -1;
+  m->unsteady = 1; m->dt = dt; m->theta = theta;
+  m->relaxThdt_H = 1.0 / dt / theta;
+  m->relaxThdt_UV = 1.0 / dt / theta;
+  for (int n = 0; n < m->np; n++) m->dUdt[n] = m->dVdt[n] = m->dSdt[n] = 0.0;
+  for (int n = 0; n < m->np; n++) {
+    if (m->So[n] < m->zor[n] + m->dryLimit) {
+      if (first) { m->Uo[n] = m->U[n]; m->Vo[n] = m->V[n]; m->So[n] = m->S[n]; }
+      m->dUdto[n] = m->dVdto[n] = m->dSdto[n] = 0.0;
+    }
+  }
+  return 0;
+}
+
+/* EQS_UVS2D::Timegrad (EqsUVS2D.cpp:1183-1230) for wet nodes (no node is dry or marsh
+ * here): d/dt = (1 - 1/theta) d/dt_old + (v - v_old) / (dt theta) */
+static void timegrad(orc_model* m)
+{
+  const double iTheta = 1.0 - 1.0 / m->theta;
+  const double thdt = 1.0 / m->dt / m->theta;
+  for (int n = 0; n < m->np; n++) {
+    m->dUdt[n] = iTheta * m->dUdto[n] + thdt * (m->U[n] - m->Uo[n]);
+    m->dVdt[n] = iTheta * m->dVdto[n] + thdt * (m->V[n] - m->Vo[n]);
+    m->dSdt[n] = iTheta * m->dSdto[n] + thdt * (m->S[n] - m->So[n]);
+  }
+}
+
+/* End of a time step: the state becomes the previous one (Compute.cpp:773-796) */
+void orc_time_commit(orc_model* m)
+{
+  const size_t b = sizeof(double) * (size_t)m->np;
+  memcpy(m->Uo, m->U, b); memcpy(m->Vo, m->V, b); memcpy(m->So, m->S, b);
+  memcpy(m->dUdto, m->dUdt, b); memcpy(m->dVdto, m->dVdt, b); memcpy(m->dSdto, m->dSdt, b);
 }
 
 void orc_get_state(const orc_model* m, double* U, double* V, double* S, double* z)
@@ -1409,6 +1505,8 @@ int orc_newton_step(orc_model* m, int prec, int solver, int maxIter, double eps,
       m->z[nd[i]] = 0.5 * (m->z[nd[l]] + m->z[nd[r]]);
     }
   }
+  /* time derivatives of the corrected state (EqsUVS2D.cpp:811) */
+  if (m->unsteady) timegrad(m);
   if (st) *st = s;
   free(B); free(X);
   return n;

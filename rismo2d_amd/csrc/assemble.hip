@@ -119,6 +119,7 @@ void upload_shapes(hipStream_t s)
 struct NodeView {
   const double *x, *y, *z, *S, *U, *V, *cf, *niox, *nioy, *valU, *valS, *rotm;
   const double *vtn, *uun;      // algebraic model: nodal eddy viscosity, uu (= vv; uv = 0)
+  const double *dUdt, *dVdt, *dSdt;   // time derivatives (zero for stationary flow)
   const uint32_t* flag;
   const int* eqno;
 };
@@ -138,7 +139,8 @@ __device__ __forceinline__ void region_rows(const NodeView& nv, const int* nd, i
   constexpr int NL = 2 * NND + NCN;
   constexpr int T = (NND == 6) ? ET_TRI : ET_QUAD;
   const ShapeTab& s = c_shape[T];
-  double x[NND], y[NND], zc[NCN], Hc[NCN], cfc[NCN], Un[NND], Vn[NND];
+  double x[NND], y[NND], zc[NCN], Hc[NCN], cfc[NCN], Un[NND], Vn[NND], St[NCN];
+  double Ut[NND], Vt[NND];
   const double x0 = nv.x[nd[0]], y0 = nv.y[nd[0]];
 #pragma unroll
   for (int k = 0; k < NND; k++) {
@@ -146,6 +148,8 @@ __device__ __forceinline__ void region_rows(const NodeView& nv, const int* nd, i
     y[k] = nv.y[nd[k]] - y0;
     Un[k] = nv.U[nd[k]];
     Vn[k] = nv.V[nd[k]];
+    Ut[k] = nv.dUdt[nd[k]];
+    Vt[k] = nv.dVdt[nd[k]];
   }
   x[0] = 0.0; y[0] = 0.0;
 #pragma unroll
@@ -154,6 +158,7 @@ __device__ __forceinline__ void region_rows(const NodeView& nv, const int* nd, i
     double h = nv.S[nd[k]] - zc[k];
     Hc[k] = (h <= 0.0) ? P.hmin : h;
     cfc[k] = nv.cf[nd[k]];
+    St[k] = nv.dSdt[nd[k]];
   }
 #pragma unroll
   for (int c = 0; c < NL; c++) { KU[c] = 0.0; KV[c] = 0.0; KH[c] = 0.0; }
@@ -185,19 +190,21 @@ __device__ __forceinline__ void region_rows(const NodeView& nv, const int* nd, i
       dmdx[k] = t00 * s.ldx[g][k] + t01 * s.ldy[g][k];
       dmdy[k] = t10 * s.ldx[g][k] + t11 * s.ldy[g][k];
     }
-    double H = 0, dHdx = 0, dHdy = 0, dadx = 0, dady = 0, cf = 0;
+    double H = 0, dHdx = 0, dHdy = 0, dadx = 0, dady = 0, cf = 0, dHdt = 0;
 #pragma unroll
     for (int k = 0; k < NCN; k++) {
       const double m = s.lf[g][k];
       H += m * Hc[k]; dHdx += dmdx[k] * Hc[k]; dHdy += dmdy[k] * Hc[k];
       dadx += dmdx[k] * zc[k]; dady += dmdy[k] * zc[k];
+      dHdt += m * St[k];
       cf += m * cfc[k];
     }
-    double U = 0, V = 0, dUdx = 0, dUdy = 0, dVdx = 0, dVdy = 0;
+    double U = 0, V = 0, dUdx = 0, dUdy = 0, dVdx = 0, dVdy = 0, dUdt = 0, dVdt = 0;
 #pragma unroll
     for (int k = 0; k < NND; k++) {
       const double n = s.f[g][k];
       U += n * Un[k]; V += n * Vn[k];
+      dUdt += n * Ut[k]; dVdt += n * Vt[k];
       dUdx += dndx[k] * Un[k]; dUdy += dndy[k] * Un[k];
       dVdx += dndx[k] * Vn[k]; dVdy += dndy[k] * Vn[k];
     }
@@ -221,16 +228,16 @@ __device__ __forceinline__ void region_rows(const NodeView& nv, const int* nd, i
 
     // residual rows (force = -R)
     {
-      double f = H * (U * dUdx + V * dUdy) + H * grav * dadx + cf * Ures * U;
+      double f = H * dUdt + H * (U * dUdx + V * dUdy) + H * grav * dadx + cf * Ures * U;
       double fx = (H * vt * (dUdx + dUdx) - H * uu) - H * H * grav / 2.0;
       double fy = H * vt * (dUdy + dVdx);
       FU -= wgt * (ni * f + nxi * fx + nyi * fy);
-      f = H * (U * dVdx + V * dVdy) + H * grav * dady + cf * Ures * V;
+      f = H * dVdt + H * (U * dVdx + V * dVdy) + H * grav * dady + cf * Ures * V;
       fx = H * vt * (dVdx + dUdy);
       fy = (H * vt * (dVdy + dVdy) - H * vv) - H * H * grav / 2.0;
       FV -= wgt * (ni * f + nxi * fx + nyi * fy);
       if (i < NCN) {
-        const double fc = H * (dUdx + dVdy) + U * dHdx + V * dHdy;
+        const double fc = dHdt + H * (dUdx + dVdy) + U * dHdx + V * dHdy;
         FH -= s.lf[g][i] * (wgt * fc);
       }
     }
@@ -246,7 +253,7 @@ __device__ __forceinline__ void region_rows(const NodeView& nv, const int* nd, i
         KU[k] += ni * t + nxi * (dxx * dndx[k]) + nyi * (dyy * dndy[k]);
         KU[NND + k] += ni * (dUV * s.f[g][k]) + nyi * (dyx * dndx[k]);
       }
-      const double dH = wgt * (U * dUdx + V * dUdy) + wgt * grav * dadx;
+      const double dH = wgt * dUdt + wgt * (U * dUdx + V * dUdy) + wgt * grav * dadx;
       const double dHx = (wgt * vt * (dUdx + dUdx) - wgt * uu) - wgt * grav * H;
       const double dHy = wgt * vt * (dUdy + dVdx);
 #pragma unroll
@@ -267,7 +274,7 @@ __device__ __forceinline__ void region_rows(const NodeView& nv, const int* nd, i
         const double t = dVV * s.f[g][k] + wH * U * dndx[k] + wH * V * dndy[k];
         KV[NND + k] += ni * t + nxi * (dxx * dndx[k]) + nyi * (dyy * dndy[k]);
       }
-      const double dH = wgt * (U * dVdx + V * dVdy) + wgt * grav * dady;
+      const double dH = wgt * dVdt + wgt * (U * dVdx + V * dVdy) + wgt * grav * dady;
       const double dHx = wgt * vt * (dVdx + dUdy);
       const double dHy = (wgt * vt * (dVdy + dVdy) - wgt * vv) - wgt * grav * H;
 #pragma unroll
@@ -670,13 +677,17 @@ static NodeView node_view(Ctx& c)
   v.niox = c.niox.p; v.nioy = c.nioy.p; v.valU = c.valU.p; v.valS = c.valS.p; v.rotm = c.rotm.p;
   v.flag = c.dflag.p; v.eqno = c.eqno.p;
   v.vtn = c.vtn.p; v.uun = c.uun.p;
+  v.dUdt = c.dUdt.p; v.dVdt = c.dVdt.p; v.dSdt = c.dSdt.p;
   return v;
 }
 
 static AsmParams asm_params(Ctx& c)
 {
   AsmParams p;
-  p.g = c.par.g; p.vk = c.par.vk; p.hmin = c.par.hmin(); p.relaxUV = 0.0; p.relaxH = 0.0;
+  p.g = c.par.g; p.vk = c.par.vk; p.hmin = c.par.hmin();
+  // 1 / (dt theta) on the time terms' diagonal for unsteady flow (EqsUVS2D.cpp:130-147)
+  p.relaxUV = c.unsteady ? 1.0 / c.dt / c.theta : 0.0;
+  p.relaxH = c.unsteady ? 1.0 / c.dt / c.theta : 0.0;
   p.turbAlg = c.par.turb_alg() ? 1 : 0; p.turbMin = c.par.turb_min() ? 1 : 0;
   return p;
 }

@@ -1,5 +1,6 @@
 // C ABI of the rmx library (declared in include/rmx.h).
 #include <chrono>
+#include <cmath>
 #include <cstring>
 
 #include "rmx_internal.h"
@@ -102,6 +103,11 @@ rmx_status rmx_mesh_upload(rmx_ctx* p, const rmx_mesh_desc* d)
   c.d_elem.upload(hm.elem, s); c.d_nnd.upload(hm.nnd, s);
   c.d_manning.upload(hm.manning, s); c.d_areaFact.upload(hm.areaFact, s);
   c.d_veg_dp.upload(hm.veg_dp, s); c.d_veg_sp.upload(hm.veg_sp, s);
+  for (DBuf<double>* b : {&c.dUdt, &c.dVdt, &c.dSdt, &c.Uo, &c.Vo, &c.So, &c.dUdto, &c.dVdto,
+                          &c.dSdto})
+    b->alloc(np);
+  dev_time_init(c);
+  c.unsteady = false;
   c.d_ste.upload(hm.ste, s); c.vtn.alloc(np); c.uun.alloc(np);
   HIPCHECK(hipMemsetAsync(c.vtn.p, 0, sizeof(double) * np, s));
   HIPCHECK(hipMemsetAsync(c.uun.p, 0, sizeof(double) * np, s));
@@ -174,6 +180,7 @@ rmx_status rmx_state_upload(rmx_ctx* p, const double* U, const double* V, const 
   if (U) c.U.upload(U, np, c.stream);
   if (V) c.V.upload(V, np, c.stream);
   if (S) c.S.upload(S, np, c.stream);
+  dev_time_init(c);
   c.turb_stale = true;
   HIPCHECK(hipStreamSynchronize(c.stream));
   API_END
@@ -265,6 +272,54 @@ rmx_status rmx_turbulence(rmx_ctx* p, double* vt, double* uu)
   const int np = c.hm.np;
   if (vt) HIPCHECK(hipMemcpyAsync(vt, c.vtn.p, sizeof(double) * np, hipMemcpyDeviceToHost, c.stream));
   if (uu) HIPCHECK(hipMemcpyAsync(uu, c.uun.p, sizeof(double) * np, hipMemcpyDeviceToHost, c.stream));
+  HIPCHECK(hipStreamSynchronize(c.stream));
+  API_END
+}
+
+rmx_status rmx_time_step(rmx_ctx* p, double dt, double theta, int first)
+{
+  API_BEGIN
+  Ctx& c = *need(p);
+  if (!c.mesh_ready) fail(RMX_ERR_ARG, "time step: upload a mesh first");
+  c.turb_stale = true;
+  if (dt <= 0.0) {
+    c.unsteady = false;
+  } else {
+    if (std::fabs(theta) < 1.0e-10 || std::fabs(dt) < 1.0e-10)
+      fail(RMX_ERR_ARG, "time step: theta or time interval too small");
+    c.unsteady = true; c.dt = dt; c.theta = theta;
+    dev_time_step(c, first ? 1 : 0);
+    HIPCHECK(hipStreamSynchronize(c.stream));
+  }
+  API_END
+}
+
+rmx_status rmx_time_commit(rmx_ctx* p)
+{
+  API_BEGIN
+  Ctx& c = *need(p);
+  if (!c.mesh_ready) fail(RMX_ERR_ARG, "time commit: upload a mesh first");
+  dev_time_commit(c);
+  HIPCHECK(hipStreamSynchronize(c.stream));
+  API_END
+}
+
+rmx_status rmx_time_state(rmx_ctx* p, int set, double* const* arrays)
+{
+  API_BEGIN
+  Ctx& c = *need(p);
+  if (!c.mesh_ready) fail(RMX_ERR_ARG, "time state: upload a mesh first");
+  if (!arrays) fail(RMX_ERR_ARG, "time state: null array list");
+  DBuf<double>* b[9] = {&c.Uo, &c.Vo, &c.So, &c.dUdto, &c.dVdto, &c.dSdto,
+                        &c.dUdt, &c.dVdt, &c.dSdt};
+  const size_t np = (size_t)c.hm.np;
+  for (int k = 0; k < 9; k++) {
+    if (!arrays[k]) continue;
+    if (set) HIPCHECK(hipMemcpyAsync(b[k]->p, arrays[k], sizeof(double) * np,
+                                     hipMemcpyHostToDevice, c.stream));
+    else HIPCHECK(hipMemcpyAsync(arrays[k], b[k]->p, sizeof(double) * np,
+                                 hipMemcpyDeviceToHost, c.stream));
+  }
   HIPCHECK(hipStreamSynchronize(c.stream));
   API_END
 }

@@ -99,6 +99,74 @@ __global__ void k_midside(int ne, const int* __restrict__ elem, const int* __res
   }
 }
 
+// ---- theta scheme (EQS_UVS2D::Predict / Timegrad, EqsUVS2D.cpp:1101-1230) ----
+
+// Predict: current time derivatives zero; where the previous level lies below
+// zor + dryLimit the previous derivatives are zero and, at the first time step, the
+// previous state is the current one.
+__global__ void k_predict(int np, int first, double dryLimit, const double* __restrict__ zor,
+                          const double* __restrict__ U, const double* __restrict__ V,
+                          const double* __restrict__ S, double* __restrict__ dUdt,
+                          double* __restrict__ dVdt, double* __restrict__ dSdt,
+                          double* __restrict__ Uo, double* __restrict__ Vo,
+                          double* __restrict__ So, double* __restrict__ dUdto,
+                          double* __restrict__ dVdto, double* __restrict__ dSdto)
+{
+  const int n = blockIdx.x * blockDim.x + threadIdx.x;
+  if (n >= np) return;
+  dUdt[n] = 0.0; dVdt[n] = 0.0; dSdt[n] = 0.0;
+  if (So[n] < zor[n] + dryLimit) {
+    if (first) { Uo[n] = U[n]; Vo[n] = V[n]; So[n] = S[n]; }
+    dUdto[n] = 0.0; dVdto[n] = 0.0; dSdto[n] = 0.0;
+  }
+}
+
+// Timegrad for wet nodes: d/dt = (1 - 1/theta) d/dt_old + (v - v_old) / (dt theta)
+__global__ void k_timegrad(int np, double iTheta, double thdt, const double* __restrict__ U,
+                           const double* __restrict__ V, const double* __restrict__ S,
+                           const double* __restrict__ Uo, const double* __restrict__ Vo,
+                           const double* __restrict__ So, const double* __restrict__ dUdto,
+                           const double* __restrict__ dVdto, const double* __restrict__ dSdto,
+                           double* __restrict__ dUdt, double* __restrict__ dVdt,
+                           double* __restrict__ dSdt)
+{
+  const int n = blockIdx.x * blockDim.x + threadIdx.x;
+  if (n >= np) return;
+  dUdt[n] = iTheta * dUdto[n] + thdt * (U[n] - Uo[n]);
+  dVdt[n] = iTheta * dVdto[n] + thdt * (V[n] - Vo[n]);
+  dSdt[n] = iTheta * dSdto[n] + thdt * (S[n] - So[n]);
+}
+
+void dev_time_init(Ctx& c)
+{
+  const size_t b = sizeof(double) * (size_t)c.hm.np;
+  HIPCHECK(hipMemcpyAsync(c.Uo.p, c.U.p, b, hipMemcpyDeviceToDevice, c.stream));
+  HIPCHECK(hipMemcpyAsync(c.Vo.p, c.V.p, b, hipMemcpyDeviceToDevice, c.stream));
+  HIPCHECK(hipMemcpyAsync(c.So.p, c.S.p, b, hipMemcpyDeviceToDevice, c.stream));
+  for (double* p : {c.dUdt.p, c.dVdt.p, c.dSdt.p, c.dUdto.p, c.dVdto.p, c.dSdto.p})
+    HIPCHECK(hipMemsetAsync(p, 0, b, c.stream));
+}
+
+void dev_time_step(Ctx& c, int first)
+{
+  const int np = c.hm.np;
+  k_predict<<<(np + 255) / 256, 256, 0, c.stream>>>(np, first, c.par.dry_limit, c.zor.p, c.U.p,
+      c.V.p, c.S.p, c.dUdt.p, c.dVdt.p, c.dSdt.p, c.Uo.p, c.Vo.p, c.So.p, c.dUdto.p, c.dVdto.p,
+      c.dSdto.p);
+  HIPCHECK(hipGetLastError());
+}
+
+void dev_time_commit(Ctx& c)
+{
+  const size_t b = sizeof(double) * (size_t)c.hm.np;
+  HIPCHECK(hipMemcpyAsync(c.Uo.p, c.U.p, b, hipMemcpyDeviceToDevice, c.stream));
+  HIPCHECK(hipMemcpyAsync(c.Vo.p, c.V.p, b, hipMemcpyDeviceToDevice, c.stream));
+  HIPCHECK(hipMemcpyAsync(c.So.p, c.S.p, b, hipMemcpyDeviceToDevice, c.stream));
+  HIPCHECK(hipMemcpyAsync(c.dUdto.p, c.dUdt.p, b, hipMemcpyDeviceToDevice, c.stream));
+  HIPCHECK(hipMemcpyAsync(c.dVdto.p, c.dVdt.p, b, hipMemcpyDeviceToDevice, c.stream));
+  HIPCHECK(hipMemcpyAsync(c.dSdto.p, c.dSdt.p, b, hipMemcpyDeviceToDevice, c.stream));
+}
+
 void dev_newton_update(Ctx& c, rmx_step_stats& st)
 {
   const int np = c.hm.np;
@@ -139,6 +207,12 @@ void dev_newton_update(Ctx& c, rmx_step_stats& st)
   k_midside<<<(c.hm.ne + 255) / 256, 256, 0, c.stream>>>(c.hm.ne, c.d_elem.p, c.d_nnd.p, c.S.p,
                                                          c.z.p);
   HIPCHECK(hipGetLastError());
+  if (c.unsteady) {                       // time derivatives of the corrected state
+    k_timegrad<<<(np + 255) / 256, 256, 0, c.stream>>>(np, 1.0 - 1.0 / c.theta,
+        1.0 / c.dt / c.theta, c.U.p, c.V.p, c.S.p, c.Uo.p, c.Vo.p, c.So.p, c.dUdto.p,
+        c.dVdto.p, c.dSdto.p, c.dUdt.p, c.dVdt.p, c.dSdt.p);
+    HIPCHECK(hipGetLastError());
+  }
   HIPCHECK(hipStreamSynchronize(c.stream));
 }
 

@@ -188,6 +188,10 @@ struct Ctx {
 
   // ---- device node data (SoA) ----
   DBuf<double> x, y, zor, z, U, V, S, cf, niox, nioy, valU, valS, rotm;  // rotm [np*4]
+  // time integration: time derivatives, previous time step's state and derivatives
+  DBuf<double> dUdt, dVdt, dSdt, Uo, Vo, So, dUdto, dVdto, dSdto;
+  bool unsteady = false;
+  double dt = 0.0, theta = 1.0;
   DBuf<uint32_t> dflag;
   DBuf<int> eqno;                             // [3*np]
   // friction gather: node -> incident region elements (ascending), areaFact*cf summed
@@ -257,6 +261,9 @@ void dev_ilu0_apply(Ctx& c, int prec, const double* b, double* x);   // device p
 void dev_spmv(Ctx& c, int prec, const double* x, double* r);        // device pointers
 void dev_solve(Ctx& c, const rmx_krylov_opts& o, rmx_krylov_out& out);
 void dev_newton_update(Ctx& c, rmx_step_stats& st);
+void dev_time_init(Ctx& c);                    // previous state := state, derivatives 0
+void dev_time_step(Ctx& c, int first);         // EQS_UVS2D::Predict
+void dev_time_commit(Ctx& c);                  // previous state := state
 void dev_element_coefs(Ctx& c, int first, int count, double* K, double* F);
 void dev_profile(Ctx& c, int prec, int reps, rmx_kernel_times& out);
 void dev_resid(Ctx& c, const double* x, const double* b, double* r);

@@ -328,12 +328,15 @@ def write_reference_deck(deck: Deck, outdir: str, solver: str = "1 6 1 0 60 1000
                          tm_solver: int = 1, maxiter: int = 12,
                          convergence: str = "1.0e-7 1.0e-7 1.0e-4 1.0e-4",
                          relax: str = "2 0.001 0.1 10.0 10.0 0.1",
-                         dryrew: str = "2 1000 0.001 0.005 100 5") -> None:
+                         dryrew: str = "2 1000 0.001 0.005 100 5", tm_steps: int = 1,
+                         dt: float = 1000000.0, theta: float = 1.0,
+                         stationary: int = 1) -> None:
     """Write the reference's keyword input files for ``deck`` into ``outdir``.
 
     One material per distinct (Manning n, vt, stem diameter, stem spacing, st, KDest).
     Values are written with %.17g so that the reference reads back the exact doubles used
-    by the oracle.
+    by the oracle.  ``tm_steps`` time steps of ``dt`` seconds with weight ``theta``;
+    ``stationary`` 0 selects the unsteady flow cycle.
     """
     os.makedirs(outdir, exist_ok=True)
     ne = deck.n_elems
@@ -372,8 +375,9 @@ def write_reference_deck(deck: Deck, outdir: str, solver: str = "1 6 1 0 60 1000
             # no vt st estx esty KDest lm betaSf (TYPE::Import_40000)
             f.write(f"$TURBULENCE {no} {vt:.17g} {a:.17g} 0.0 0.0 {b:.17g} 0.0 0.0\n")
     with open(os.path.join(outdir, "tm.dat"), "w") as f:
-        f.write("$RISMO2D 40100\n$TM_STEPS 1 1 0\n$TM_INTERVAL 1000000.0 0.0 0.0\n"
-                "$TM_WEIGHT 1.0 1.0 1.0\n$TM_STEP_NO 1\n$TM_CYCLE 1\n$TM_STATIONARY 1\n"
+        f.write(f"$RISMO2D 40100\n$TM_STEPS 1 {tm_steps} 0\n$TM_INTERVAL {dt:.17g} 0.0 0.0\n"
+                f"$TM_WEIGHT {theta:.17g} 1.0 1.0\n$TM_STEP_NO 1\n$TM_CYCLE 1\n"
+                f"$TM_STATIONARY {stationary}\n"
                 f"$TM_TURBULENCE {deck.turb}\n$TM_MAXITER {maxiter}\n$TM_SOLVER {tm_solver}\n"
                 f"$TM_LINE QINLET {INLET_LINE} {deck.Q_inlet:.17g} {deck.S_inlet:.17g}\n"
                 f"$TM_LINE OUTLET {OUTLET_LINE} {deck.S_outlet:.17g} 1000.0 0.0 0 0.0\n")

@@ -104,6 +104,8 @@ def lib():
             "rmx_get_pattern": [_v, _i, _i], "rmx_get_nodal": [_v, _d, _d, _i, _d],
             "rmx_n_model_elems": [_v, _i], "rmx_element_coefs": [_v, C.c_int, C.c_int, _d, _d],
             "rmx_turbulence": [_v, _d, _d],
+            "rmx_time_step": [_v, C.c_double, C.c_double, C.c_int], "rmx_time_commit": [_v],
+            "rmx_time_state": [_v, C.c_int, C.POINTER(_d)],
             "rmx_assemble": [_v, C.c_int], "rmx_scale_l2": [_v, _d],
             "rmx_convert_matrix": [_v, C.c_int], "rmx_ilu0_factor": [_v, C.c_int],
             "rmx_ilu0_apply": [_v, C.c_int, _d, _d], "rmx_spmv": [_v, C.c_int, _d, _d],
@@ -212,6 +214,26 @@ class Context:
         p = Params(relax_method, dry_method, dry_limit, max_delta_uv, max_delta_s, conv_uv,
                    conv_s, max_us, turb, cm, cd)
         _check(lib().rmx_set_params(self.h, C.byref(p)))
+
+    TIME_KEYS = ("Uo", "Vo", "So", "dUdto", "dVdto", "dSdto", "dUdt", "dVdt", "dSdt")
+
+    def time_step(self, dt, theta, first=False):
+        """Start a flow time step (dt <= 0: stationary flow)."""
+        _check(lib().rmx_time_step(self.h, dt, theta, int(first)))
+
+    def time_commit(self):
+        _check(lib().rmx_time_commit(self.h))
+
+    def time_state(self):
+        out = {k: np.zeros(self.np) for k in self.TIME_KEYS}
+        ptrs = (_d * 9)(*[_p(out[k], _d) for k in self.TIME_KEYS])
+        _check(lib().rmx_time_state(self.h, 0, ptrs))
+        return out
+
+    def set_time_state(self, **kw):
+        arrs = [None if kw.get(k) is None else _c(kw[k], np.float64) for k in self.TIME_KEYS]
+        ptrs = (_d * 9)(*[None if a is None else _p(a, _d) for a in arrs])
+        _check(lib().rmx_time_state(self.h, 1, ptrs))
 
     def turbulence(self):
         """Friction and the algebraic model's nodal vt, uu of the current state."""

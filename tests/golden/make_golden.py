@@ -17,6 +17,13 @@ the reference's input format, runs the reference and stores what the harness rec
                     same records plus the nodal vt, uu, uv, vv of every step.
 * ``turbq10x3.npz`` the 10 x 3 quad channel with the algebraic model and the element's vt
                     0.04 as lower bound ($TM_TURBULENCE 102), KDest in [0.4, 0.8].
+* ``unst20x4.npz``  the 20 x 4 triangle channel as an unsteady flow: 3 time steps of
+                    30 s with theta 0.7 from the uniform initial state (a transient);
+                    every Newton step's records with the time derivatives and previous
+                    time step's state, element matrices of Newton step 1 (non-zero time
+                    derivatives).
+* ``unstq10x3.npz`` the 10 x 3 quad channel, unsteady with the algebraic eddy viscosity
+                    (3 steps of 20 s, theta 0.6; element matrices of Newton step 2).
 * ``channel200x20.npz``  the straight-channel workload (200 x 20 cells, 8,000 triangles,
                     bed noise sigma 0.05 m / 20 m, seed 0): numbering, per-step scale and
                     correction, BiCGStab iteration counts from the report, final state,
@@ -59,13 +66,26 @@ DECKS = {
     "turbq10x3": dict(nx=10, ny=3, length=100.0, width=30.0, noise_sigma=0.05, noise_corr=5.0,
                       Q=10.0, quads=True, vt=0.04, turb=102, turb_st=(0.5, 1.5),
                       turb_kdest=(0.4, 0.8)),
+    "unst20x4": dict(nx=20, ny=4, length=100.0, width=20.0, noise_sigma=0.05, noise_corr=5.0,
+                     Q=10.0),
+    "unstq10x3": dict(nx=10, ny=3, length=100.0, width=30.0, noise_sigma=0.05, noise_corr=5.0,
+                      Q=10.0, quads=True, turb=2, turb_st=(0.5, 1.5)),
+}
+
+# time integration of the decks that are not one stationary flow step, and the Newton step
+# whose element matrices are recorded (one with non-zero time derivatives)
+RUNS = {
+    "unst20x4": dict(tm_steps=3, dt=30.0, theta=0.7, stationary=0, elem_step=1),
+    "unstq10x3": dict(tm_steps=3, dt=20.0, theta=0.6, stationary=0, elem_step=2),
 }
 
 
-def run_reference(deck, workdir):
-    dk.write_reference_deck(deck, workdir)
-    env = dict(os.environ, RMX_DUMP=os.path.join(workdir, "dump.bin"), RMX_DUMP_ELEM_STEP="0",
-               RMX_DUMP_ILU_STEP="0")
+def run_reference(deck, workdir, run):
+    run = dict(run)
+    elem_step = run.pop("elem_step", 0)
+    dk.write_reference_deck(deck, workdir, **run)
+    env = dict(os.environ, RMX_DUMP=os.path.join(workdir, "dump.bin"),
+               RMX_DUMP_ELEM_STEP=str(elem_step), RMX_DUMP_ILU_STEP="0")
     exe = os.path.join(ROOT, "oracle", "_ref", "rismo_harness")
     subprocess.run([exe, "input.dat"], cwd=workdir, env=env, check=True,
                    stdout=subprocess.DEVNULL, stderr=subprocess.DEVNULL, timeout=600)
@@ -83,15 +103,17 @@ def deck_arrays(d):
         out["veg_sp"] = d.veg_sp
     if d.turb != 1:
         out["turb"] = d.turb
-        out["turb_st"] = d.turb_st
-        out["turb_kdest"] = d.turb_kdest
+        for key in ("turb_st", "turb_kdest"):
+            if getattr(d, key) is not None:
+                out[key] = getattr(d, key)
     return out
 
 
 def state_arrays(prefix, st):
     out = {}
     for k in ("U", "V", "S", "z", "cf", "kind", "rot", "bcU", "bcS", "r00", "r01", "r10", "r11",
-              "eqU", "eqV", "eqS", "nx", "ny", "niox", "nioy", "vt", "uu", "uv", "vv"):
+              "eqU", "eqV", "eqS", "nx", "ny", "niox", "nioy", "vt", "uu", "uv", "vv",
+              "dUdt", "dVdt", "dSdt", "Uo", "Vo", "So", "dUdto", "dVdto", "dSdto"):
         if k in st:
             out[f"{prefix}{k}"] = st[k]
     return out
@@ -103,17 +125,22 @@ def main():
     names = sys.argv[1:] or list(DECKS)
     for name in names:
         kw = DECKS[name]
+        run = RUNS.get(name, {})
         d = dk.channel(**kw)
         with tempfile.TemporaryDirectory() as tmp:
-            (steps, final), iters = run_reference(d, tmp)
+            (steps, final), iters = run_reference(d, tmp, run)
         out = deck_arrays(d)
         out["nsteps"] = len(steps)
+        for key in ("tm_steps", "dt", "theta", "stationary", "elem_step"):
+            if key in run:
+                out[key] = run[key]
         out["bicgstab_iters"] = np.asarray(iters, np.int64)
         full = name != "channel200x20"
         for k, st in enumerate(steps):
             p = f"s{k}_"
             out[p + "scale"] = st["scale"]
             out[p + "xsol"] = st["xsol"]
+            out[p + "tstep"] = st["tstep"]
             if full:
                 out.update(state_arrays(p, st["state"]))
                 out[p + "width"] = st["width"]
@@ -127,15 +154,16 @@ def main():
                 out["s0_nnz"] = int(st["nnz"])
                 out["s0_bvec"] = st["bvec"]
         s0 = steps[0]
+        se = steps[run.get("elem_step", 0)]
         if full:
-            ne = len(s0["elems"])
+            ne = len(se["elems"])
             out["el_names"] = np.stack([np.pad(e["names"], (0, 8 - e["nnd"]), constant_values=0)
-                                        for e in s0["elems"]])
-            out["el_nnd"] = np.asarray([e["nnd"] for e in s0["elems"]])
-            out["el_bound"] = np.asarray([e["bound"] for e in s0["elems"]])
+                                        for e in se["elems"]])
+            out["el_nnd"] = np.asarray([e["nnd"] for e in se["elems"]])
+            out["el_bound"] = np.asarray([e["bound"] for e in se["elems"]])
             K = np.zeros((ne, 24, 24))
             F = np.zeros((ne, 24))
-            for i, e in enumerate(s0["elems"]):
+            for i, e in enumerate(se["elems"]):
                 nd = 3 * e["nnd"]
                 K[i, :nd, :nd] = e["estifm"]
                 F[i, :nd] = e["force"]

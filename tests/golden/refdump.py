@@ -10,7 +10,8 @@ import numpy as np
 
 STATE_FIELDS = ["name", "flag", "kind", "rot", "eqU", "eqV", "eqS", "x", "y", "z", "zor",
                 "U", "V", "S", "cf", "vt", "nx", "ny", "niox", "nioy", "bcU", "bcS",
-                "r00", "r01", "r11", "r10", "uu", "uv", "vv"]
+                "r00", "r01", "r11", "r10", "uu", "uv", "vv", "dUdt", "dVdt", "dSdt",
+                "Uo", "Vo", "So", "dUdto", "dVdto", "dSdto"]
 
 
 def _records(path):
@@ -32,6 +33,8 @@ def read_dump(path):
         if tag == "STEP":
             cur = {"elems": []}
             steps.append(cur)
+        elif tag == "TSTEP":
+            cur["tstep"] = int(np.frombuffer(buf, np.int64)[0])
         elif tag == "STATEHDR":
             hdr = np.frombuffer(buf, np.int64)
         elif tag in ("STATE", "FINAL"):

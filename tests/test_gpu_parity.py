@@ -411,3 +411,98 @@ def test_error_paths(ctx):
     vt, uu = ctx.turbulence()
     assert (vt > 0).all() and (uu < 0).all()
     ctx.set_params(turb=1)
+
+
+UNSTEADY = ["unst20x4", "unstq10x3"]
+
+
+@pytest.mark.parametrize("name", UNSTEADY)
+def test_unsteady_element_kernel_matches_oracle(ctx, name):
+    """Theta-scheme time terms: at the golden state of the Newton step whose element
+    matrices the reference recorded (non-zero time derivatives, previous time step's
+    state), the GPU element kernel and assembly agree with the oracle (itself bit-exact
+    with those records)."""
+    print(f"[gpu] unsteady element parity on {name}", flush=True)
+    g = load_golden(name)
+    deck = golden_deck(g)
+    p = f"s{int(g['elem_step'])}_"
+    dt, th = float(g["dt"]), float(g["theta"])
+    tkw = {k: g[p + k] for k in rmx.Context.TIME_KEYS}
+    ctx.upload_mesh(deck)
+    ctx.state_upload(g[p + "U"], g[p + "V"], g[p + "S"])
+    ctx.time_step(dt, th, first=False)
+    ctx.set_time_state(**tkw)
+    n = ctx.prepare()
+    ts = ctx.time_state()
+    for k in rmx.Context.TIME_KEYS:
+        np.testing.assert_array_equal(ts[k], g[p + k])
+    om = po.OracleModel(deck)
+    om.set_state(g[p + "U"], g[p + "V"], g[p + "S"])
+    om.time_step(dt, th)
+    om.set_time_state(**tkw)
+    assert om.prepare() == n
+    ne = ctx.n_model_elems()
+    K, F = ctx.element_coefs(0, ne)
+    for k in range(ne):
+        Ko, Fo = om.coefs(k)
+        sk = max(np.abs(Ko).max(), 1e-300)
+        assert np.abs(K[k] - Ko).max() <= 1e-12 * sk, k
+        sf = max(np.abs(Fo).max(), np.abs(Ko).max() * 1e-3)
+        assert np.abs(F[k] - Fo).max() <= 1e-12 * sf, k
+    ctx.assemble(rmx.F64)
+    A, B = ctx.matrix(rmx.F64)
+    Ao, Bo = om.assemble(64)
+    rp, _ = ctx.pattern()
+    rm, _ = _rowmax(rp, Ao)
+    print(f"    FP64 assembly: max |gpu - oracle| / max|row| = {(np.abs(A - Ao) / rm).max():.2e}")
+    assert (np.abs(A - Ao) <= 1e-12 * rm).all()
+    # near a converged time step the residual is a small difference of large terms: scale
+    # as for the element forces
+    sb = max(np.abs(Bo).max(), 1e-3 * np.abs(Ao).max())
+    assert np.abs(B - Bo).max() <= 1e-12 * sb
+    ctx.time_step(0.0, 1.0)              # back to stationary flow for the tests that follow
+
+
+@pytest.mark.parametrize("name", UNSTEADY)
+def test_unsteady_newton_follows_reference(ctx, name):
+    """The reference's time steps on the GPU: per time step Predict, as many Newton steps
+    (FP64 assembly, FP64 ILU0-BiCGStab) as the reference took, Timegrad after every
+    update, commit at the end.  At each later time step's start the state and the
+    previous state agree with the reference's within 1e-8 of their scale, the previous
+    time derivatives within 1e-9 of theirs; the final state within 1e-8."""
+    print(f"[gpu] unsteady Newton on {name}", flush=True)
+    g = load_golden(name)
+    deck = golden_deck(g)
+    dt, th = float(g["dt"]), float(g["theta"])
+    ctx.upload_mesh(deck)
+    ts = [int(g[f"s{k}_tstep"]) for k in range(int(g["nsteps"]))]
+    first = True
+    for t in sorted(set(ts)):
+        ks = [k for k in range(len(ts)) if ts[k] == t]
+        p = f"s{ks[0]}_"
+        if not first:
+            ctx.time_commit()
+            U, V, S, z = ctx.state()
+            us = np.abs(g[p + "U"]).max()
+            assert _relerr(U, g[p + "U"], us) <= 1e-8, t
+            assert _relerr(V, g[p + "V"], us) <= 1e-8, t
+            assert _relerr(S - z, g[p + "S"] - g[p + "z"], np.abs(g[p + "S"] - g[p + "z"]).max()) <= 1e-8
+            tsd = ctx.time_state()
+            for k in ("dUdto", "dVdto", "dSdto"):
+                sc = max(np.abs(g[p + k]).max(), 1e-12)
+                err = _relerr(tsd[k], g[p + k], sc)
+                print(f"    time step {t}: {k} rel err {err:.2e}")
+                assert err <= 1e-9, (t, k)
+        ctx.time_step(dt, th, first=first)
+        first = False
+        for k in ks:
+            st = ctx.newton_step(method=rmx.BICGSTAB, prec=rmx.F64, eps=1e-12)
+            assert st.solve.converged and st.solve.true_resid < 1e-10, (t, k)
+        assert st.nrconv, t
+    U, V, S, z = ctx.state()
+    us = np.abs(g["final_U"]).max()
+    assert _relerr(U, g["final_U"], us) <= 1e-8
+    assert _relerr(V, g["final_V"], us) <= 1e-8
+    Hr = g["final_S"] - g["final_z"]
+    assert _relerr(S - z, Hr, np.abs(Hr).max()) <= 1e-8
+    ctx.time_step(0.0, 1.0)
