@@ -221,6 +221,22 @@ __device__ __forceinline__ double ring_or_global(const double* ring, const doubl
   return v;
 }
 
+// One lane's share of a row's sum over entries q0, q0 + G, ... below q1, each solved value
+// from the LDS ring or, when it has left the ring, from the global copy (out of line: the
+// sweeps take it only for long rows, wide levels and old dependencies).
+template <class VT, int G>
+__device__ __noinline__ double lane_sum(const VT* __restrict__ pval, const int* __restrict__ pcol,
+                                        const double* ring, const double* __restrict__ xp,
+                                        int q0, int q1, int lim)
+{
+  double s = 0.0;
+  for (int q = q0; q < q1; q += G) {
+    const int c = pcol[q];
+    s += (double)pval[q] * (c >= lim ? ring[c & (RING - 1)] : xp[c]);
+  }
+  return s;
+}
+
 // Block b of the sweep: levels [blev[b], blev[b+1]) of the concatenated level table, rows
 // at positions [bpos[b], bpos[b+1]).  Level offsets, columns and ring slots are local to
 // the block; each block table has one extra sentinel level offset.
@@ -355,6 +371,201 @@ __global__ void __launch_bounds__(1024) k_sweep(const int* __restrict__ blev,
     if (full) __syncthreads();
     else lds_barrier();
   }
+}
+
+// Sweep with a two-stage load pipeline and no global stores inside a level.  Level l
+// computes from registers loaded during level l - 1 (stage 2: columns, values, right-hand
+// side, diagonal); during level l the stage-2 loads of level l + 1 and the stage-1 loads
+// of level l + 2 (row bounds, right-hand-side index) are issued, and every level starts by
+// waiting for what the previous level issued, so no load is waited for in the level that
+// issues it.  Results stay in the LDS ring and are flushed to the global copy (and the
+// scatter target) at the full barriers: every FENCE positions, around wide levels and at
+// the end.  A wide level (more than WIDE rows) reads and writes the global copy directly.
+template <class VT, bool LOWER, int G, int PWN>
+__global__ void __launch_bounds__(512) k_sweep_stage(const int* __restrict__ blev,
+                                                     const int* __restrict__ bpos,
+                                                     const int* __restrict__ lev_ptr_all,
+                                                     const int* __restrict__ prs_all,
+                                                     const int* __restrict__ pcol,
+                                                     const VT* __restrict__ pval,
+                                                     const VT* __restrict__ pdiag_all,
+                                                     const int* __restrict__ gidx_all,
+                                                     const double* __restrict__ gsrc,
+                                                     const int* __restrict__ oidx_all,
+                                                     double* __restrict__ odst,
+                                                     double* __restrict__ xp_all)
+{
+  __shared__ double ring[RING];
+  __shared__ int s_lp[LPC + 2];
+  const int b = blockIdx.x;
+  const int lb = blev[b];
+  const int nlev = blev[b + 1] - lb;
+  const int* lev_ptr = lev_ptr_all + lb + 2 * b;   // nlev + 2 local offsets
+  const int base = bpos[b];
+  const int* prs = prs_all + base;
+  const VT* pdiag = LOWER ? nullptr : pdiag_all + base;
+  const int* gidx = gidx_all + base;
+  const int* oidx = oidx_all ? oidx_all + base : nullptr;
+  double* xp = xp_all + base;
+  const int t = threadIdx.x, nt = blockDim.x;
+  const int grp = t / G, lane = t % G, ngr = nt / G;
+  int lbase = 0;
+  auto load_lp = [&](int l0) {
+    for (int k = t; k < LPC + 2; k += nt) s_lp[k] = lev_ptr[min(l0 + k, nlev + 1)];
+  };
+  load_lp(0);
+  __syncthreads();
+  if (nlev <= 0) return;
+  auto need_lp = [&](int L) {           // offsets of levels L and L + 1 staged
+    if (L + 1 - lbase > LPC + 1) {
+      __syncthreads();
+      lbase = L - 1;
+      load_lp(lbase);
+      __syncthreads();
+    }
+  };
+  int flushed = 0;                      // positions below are in the global copy
+  // stage 1: row bounds and right-hand-side index of level [P_, P_ + nr_)
+  auto stage1 = [&](int P_, int nr_, int& gi, int& qa, int& qb) {
+    if (grp < nr_) {
+      const int r = P_ + grp;
+      gi = gidx[r];
+      qa = prs[r];
+      qb = prs[r + 1];
+    }
+  };
+  // stage 2: right-hand side, diagonal, first PWN entries per lane
+  auto stage2 = [&](int P_, int nr_, int gi, int qa, int qb, int* c, VT* v, double& in,
+                    VT& dg) {
+    if (grp < nr_) {
+      in = gsrc[gi];
+      if (!LOWER) dg = pdiag[P_ + grp];
+#pragma unroll
+      for (int j = 0; j < PWN; j++) {
+        const int q = qa + lane + j * G;
+        if (q < qb) { c[j] = pcol[q]; v[j] = pval[q]; }
+        else { c[j] = -1; v[j] = (VT)0; }
+      }
+    }
+  };
+  int cN[PWN], cW[PWN];
+  VT vN[PWN], vW[PWN];
+  double inN = 0.0, inW = 0.0;
+  VT dgN = (VT)1, dgW = (VT)1;
+  int gi1 = 0, qa1 = 0, qb1 = 0, gi2 = 0, qa2 = 0, qb2 = 0, qaW = 0, qbW = 0;
+#pragma unroll
+  for (int j = 0; j < PWN; j++) { cN[j] = -1; vN[j] = (VT)0; cW[j] = -1; vW[j] = (VT)0; }
+  // prologue: indices of levels 0 and 1, values of level 0
+  int P0 = s_lp[0], nr0 = s_lp[1] - s_lp[0];
+  int P1 = 0, nr1 = 0;
+  stage1(P0, nr0, gi1, qa1, qb1);
+  if (nlev > 1) {
+    P1 = s_lp[1];
+    nr1 = s_lp[2] - P1;
+    stage1(P1, nr1, gi2, qa2, qb2);
+  }
+  stage2(P0, nr0, gi1, qa1, qb1, cN, vN, inN, dgN);
+  for (int l = 0; l < nlev; l++) {
+    // everything in flight was issued during the previous level
+    __builtin_amdgcn_s_waitcnt(0x0F70);          // vmcnt(0)
+    __builtin_amdgcn_sched_barrier(0);
+#pragma unroll
+    for (int j = 0; j < PWN; j++) { cW[j] = cN[j]; vW[j] = vN[j]; }
+    inW = inN; dgW = dgN; qaW = qa1; qbW = qb1;
+    gi1 = gi2; qa1 = qa2; qb1 = qb2;             // indices of level l + 1
+    int P2 = 0, nr2 = 0;
+    if (l + 2 < nlev) {
+      need_lp(l + 2);
+      P2 = s_lp[l + 2 - lbase];
+      nr2 = s_lp[l + 3 - lbase] - P2;
+    }
+    __builtin_amdgcn_sched_barrier(0);
+    if (l + 1 < nlev) stage2(P1, nr1, gi1, qa1, qb1, cN, vN, inN, dgN);
+    if (l + 2 < nlev) stage1(P2, nr2, gi2, qa2, qb2);
+    // level l: round 0 from the working registers, straight-line so that nothing issued
+    // above is waited for; dependencies that left the ring, row tails beyond PWN entries
+    // per lane and further rounds take the out-of-line path
+    const int P = P0, nr = nr0;
+    const bool wide = nr > WIDE;
+    const int lim = wide ? INT_MAX : P + nr - RING;
+    {
+      const bool act = grp < nr;
+      double s = 0.0;
+      if (act) {
+        bool old = false;
+#pragma unroll
+        for (int j = 0; j < PWN; j++) {
+          const int cc = cW[j];
+          if (cc >= 0) {
+            old |= cc < lim;
+            s += (double)vW[j] * ring[cc & (RING - 1)];
+          }
+        }
+        if (old) s = lane_sum<VT, G>(pval, pcol, ring, xp, qaW + lane, min(qbW, qaW + PWN * G), lim);
+        if (qbW > qaW + PWN * G)
+          s += lane_sum<VT, G>(pval, pcol, ring, xp, qaW + lane + PWN * G, qbW, lim);
+      }
+#pragma unroll
+      for (int o = G / 2; o > 0; o >>= 1) s += __shfl_xor(s, o, G);
+      if (act && lane == 0) {
+        const double xi = LOWER ? inW + s : (inW - s) / (double)dgW;
+        ring[(P + grp) & (RING - 1)] = xi;
+        if (wide) {
+          xp[P + grp] = xi;
+          if (oidx) odst[oidx[P + grp]] = xi;
+        }
+      }
+    }
+    const int rounds = (nr + ngr - 1) / ngr;
+    for (int k = 1; k < rounds; k++) {
+      const int g = grp + k * ngr;
+      const bool act = g < nr;
+      double s = 0.0, in = 0.0;
+      VT dg = (VT)1;
+      if (act) {
+        const int r = P + g;
+        in = gsrc[gidx[r]];
+        if (!LOWER) dg = pdiag[r];
+        s = lane_sum<VT, G>(pval, pcol, ring, xp, prs[r] + lane, prs[r + 1], lim);
+      }
+#pragma unroll
+      for (int o = G / 2; o > 0; o >>= 1) s += __shfl_xor(s, o, G);
+      if (act && lane == 0) {
+        const double xi = LOWER ? in + s : (in - s) / (double)dg;
+        ring[(P + g) & (RING - 1)] = xi;
+        if (wide) {
+          xp[P + g] = xi;
+          if (oidx) odst[oidx[P + g]] = xi;
+        }
+      }
+    }
+    if (l + 1 >= nlev) break;
+    const bool full = (P + nr) / FENCE != P / FENCE || wide || nr1 > WIDE;
+    if (full) {
+      if (!wide) {
+        lds_barrier();
+        for (int p = flushed + t; p < P + nr; p += nt) {
+          const double v = ring[p & (RING - 1)];
+          xp[p] = v;
+          if (oidx) odst[oidx[p]] = v;
+        }
+      }
+      flushed = P + nr;
+      __syncthreads();
+    } else {
+      lds_barrier();
+    }
+    P0 = P1; nr0 = nr1;
+    P1 = P2; nr1 = nr2;
+  }
+  // the last level's results and whatever is still unflushed
+  lds_barrier();
+  if (nr0 <= WIDE)
+    for (int p = flushed + t; p < P0 + nr0; p += nt) {
+      const double v = ring[p & (RING - 1)];
+      xp[p] = v;
+      if (oidx) odst[oidx[p]] = v;
+    }
 }
 
 // ---------------------------------------------------------------------------------------
@@ -654,19 +865,47 @@ void ilu_factor(Ctx& c, Ilu& I, int prec)
   }
 }
 
+template <class VT, int V>
+static void launch_sweeps(Ctx& c, Ilu& I, const double* b, double* x, hipEvent_t* ev)
+{
+  // V = 1: staged pipeline, 4 lanes per row with 6 register entries each, 512 threads (two
+  // workgroups per compute unit by registers and LDS); V = 0: one-deep prefetch, 8 lanes
+  // per row, 1024 threads
+  const int nb = I.nblk;
+  if (ev) HIPCHECK(hipEventRecord(ev[0], c.stream));
+  if constexpr (V == 1)
+    k_sweep_stage<VT, true, 4, 6><<<nb, 512, 0, c.stream>>>(I.lo.blev.p, I.lo.bpos.p,
+        I.lo.lev_ptr.p, I.lo.prs.p, I.lo.pcol.p, PLbuf<VT>(I).p, nullptr, I.lo.prow.p, b, nullptr,
+        nullptr, I.lo.xp.p);
+  else
+    k_sweep<VT, true, SG, SKR><<<nb, 1024, 0, c.stream>>>(I.lo.blev.p, I.lo.bpos.p,
+        I.lo.lev_ptr.p, I.lo.prs.p, I.lo.pcol.p, PLbuf<VT>(I).p, nullptr, I.lo.prow.p, b, nullptr,
+        nullptr, I.lo.xp.p);
+  if (ev) HIPCHECK(hipEventRecord(ev[1], c.stream));
+  if constexpr (V == 1)
+    k_sweep_stage<VT, false, 4, 6><<<nb, 512, 0, c.stream>>>(I.up.blev.p, I.up.bpos.p,
+        I.up.lev_ptr.p, I.up.prs.p, I.up.pcol.p, PUbuf<VT>(I).p, PDbuf<VT>(I).p, I.fwd2bwd.p,
+        I.lo.xp.p, I.up.prow.p, x, I.up.xp.p);
+  else
+    k_sweep<VT, false, SG, SKR><<<nb, 1024, 0, c.stream>>>(I.up.blev.p, I.up.bpos.p,
+        I.up.lev_ptr.p, I.up.prs.p, I.up.pcol.p, PUbuf<VT>(I).p, PDbuf<VT>(I).p, I.fwd2bwd.p,
+        I.lo.xp.p, I.up.prow.p, x, I.up.xp.p);
+  if (ev) HIPCHECK(hipEventRecord(ev[2], c.stream));
+  HIPCHECK(hipGetLastError());
+}
+
+// RMX_SWEEP=0 selects the one-deep sweep kernel (tuning and comparison); default staged.
+static int sweep_variant()
+{
+  const char* e = getenv("RMX_SWEEP");
+  return (e && atoi(e) == 0) ? 0 : 1;
+}
+
 template <class VT>
 static void apply_t(Ctx& c, Ilu& I, const double* b, double* x, hipEvent_t* ev)
 {
-  const int nb = I.nblk;
-  if (ev) HIPCHECK(hipEventRecord(ev[0], c.stream));
-  k_sweep<VT, true, SG, SKR><<<nb, 1024, 0, c.stream>>>(I.lo.blev.p, I.lo.bpos.p, I.lo.lev_ptr.p,
-      I.lo.prs.p, I.lo.pcol.p, PLbuf<VT>(I).p, nullptr, I.lo.prow.p, b, nullptr, nullptr, I.lo.xp.p);
-  if (ev) HIPCHECK(hipEventRecord(ev[1], c.stream));
-  k_sweep<VT, false, SG, SKR><<<nb, 1024, 0, c.stream>>>(I.up.blev.p, I.up.bpos.p, I.up.lev_ptr.p,
-      I.up.prs.p, I.up.pcol.p, PUbuf<VT>(I).p, PDbuf<VT>(I).p, I.fwd2bwd.p, I.lo.xp.p, I.up.prow.p, x,
-      I.up.xp.p);
-  if (ev) HIPCHECK(hipEventRecord(ev[2], c.stream));
-  HIPCHECK(hipGetLastError());
+  if (sweep_variant() == 1) launch_sweeps<VT, 1>(c, I, b, x, ev);
+  else launch_sweeps<VT, 0>(c, I, b, x, ev);
 }
 
 void ilu_apply(Ctx& c, Ilu& I, int prec, const double* b, double* x, hipEvent_t* ev)

@@ -139,9 +139,10 @@ __device__ __forceinline__ double bsum(double v, double* sh)
   return s;
 }
 
-// partial[j*RED_BLOCKS + block] = <V[j], y> for j < K <= 32: y read once, K accumulators
-// per thread
+// partial[j*MDOT_BLOCKS + block] = <V[j], y> for j < K <= 32: y read once, K accumulators
+// per thread; enough workgroups for eight waves per SIMD
 constexpr int MDOT_MAX = 32;
+constexpr int MDOT_BLOCKS = 2048;
 __global__ void __launch_bounds__(256) k_mdot(long long n, int K, const double* __restrict__ Vb,
                                               long long ld, const double* __restrict__ y,
                                               double* __restrict__ partial)
@@ -161,20 +162,20 @@ __global__ void __launch_bounds__(256) k_mdot(long long n, int K, const double* 
   for (int j = 0; j < MDOT_MAX; j++) {
     if (j < K) {
       const double s = bsum(acc[j], sh);
-      if (threadIdx.x == 0) partial[j * RED_BLOCKS + blockIdx.x] = s;
+      if (threadIdx.x == 0) partial[j * MDOT_BLOCKS + blockIdx.x] = s;
     }
   }
 }
 
-__global__ void k_mfinish(int K, const double* __restrict__ partial, double* __restrict__ out)
+// out[k] = sum of the MDOT_BLOCKS partials of dot k; one workgroup per dot
+__global__ void k_mfinish(const double* __restrict__ partial, double* __restrict__ out)
 {
   __shared__ double sh[4];
-  for (int k = 0; k < K; k++) {
-    double v = 0.0;
-    for (int i = threadIdx.x; i < RED_BLOCKS; i += blockDim.x) v += partial[k * RED_BLOCKS + i];
-    const double s = bsum(v, sh);
-    if (threadIdx.x == 0) out[k] = s;
-  }
+  const int k = blockIdx.x;
+  double v = 0.0;
+  for (int i = threadIdx.x; i < MDOT_BLOCKS; i += blockDim.x) v += partial[k * MDOT_BLOCKS + i];
+  const double s = bsum(v, sh);
+  if (threadIdx.x == 0) out[k] = s;
 }
 
 static int vblocks(long long n) { return (int)std::min<long long>(4096, (n + 255) / 256); }
@@ -202,7 +203,7 @@ static void precond(Ctx& c, int prec, int kind, const double* in, double* out)
 void ensure_work(Ctx& c)
 {
   for (int k = 0; k < 10; k++) c.w[k].alloc(c.n);
-  c.red.alloc(64 + 40 * RED_BLOCKS);
+  c.red.alloc(64 + 40 * MDOT_BLOCKS);
   if (!c.h_red) HIPCHECK(hipHostMalloc(&c.h_red, 64 * sizeof(double), hipHostMallocDefault));
 }
 
@@ -339,8 +340,8 @@ static bool fgmres(Ctx& c, int prec, int pk, const double* b, double* x, int m, 
       precond(c, prec, pk, vi, zi);
       dev_spmv(c, prec, zi, vi1);
       // classical Gram-Schmidt: all dots, then all updates
-      k_mdot<<<RED_BLOCKS, 256, 0, c.stream>>>(n, i1, Vb, n, vi1, partial);
-      k_mfinish<<<1, 256, 0, c.stream>>>(i1, partial, sc + 16);
+      k_mdot<<<MDOT_BLOCKS, 256, 0, c.stream>>>(n, i1, Vb, n, vi1, partial);
+      k_mfinish<<<i1, 256, 0, c.stream>>>(partial, sc + 16);
       k_gs_update<<<vb, 256, 0, c.stream>>>(n, i1, sc + 16, Vb, n, vi1);
       {
         const double* xs[1] = { vi1 };
