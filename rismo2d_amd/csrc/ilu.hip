@@ -52,7 +52,7 @@ constexpr int WIDE = RING / 4;  // levels wider than this use the global copy on
 // ---------------------------------------------------------------------------------------
 // factorisation
 
-constexpr int UCAP = 1024;      // entries of the pivot rows staged in LDS per row
+constexpr int UCAP = 2048;      // entries of the pivot rows staged in LDS per row
 constexpr int UPL = UCAP / 64;  // staged entries per lane
 
 // One row of one level (one wave): row i eliminates its lower entries k in increasing
@@ -64,12 +64,83 @@ constexpr int UPL = UCAP / 64;  // staged entries per lane
 // then one division and one LDS update per entry.  The updates are the reference's, in its
 // order, with FMA contraction off.  Rows whose pivot rows hold more than UCAP entries
 // search and read from global memory inside the loop instead.
+// Symbolic plan of the factorisation (once per pattern, all rows in one launch, one wave
+// per row): every entry u of a row's concatenated pivot rows (the entries after each pivot
+// row's diagonal, in sorted lower-column order of the row) gets its pivot index t and its
+// target position in the row (255: not in the row, or a column at or left of the pivot).
+__global__ void __launch_bounds__(64) k_ilu_plan(const int4* __restrict__ desc, int cnt,
+                                                 const int* __restrict__ cols,
+                                                 const int* __restrict__ srt_cols,
+                                                 const unsigned char* __restrict__ srt_pos,
+                                                 const int2* __restrict__ piv,
+                                                 const long long* __restrict__ pbase,
+                                                 unsigned char* __restrict__ plan_t,
+                                                 unsigned char* __restrict__ plan_pos)
+{
+  __shared__ int s_col[FW], s_kb[FW], s_off[FW + 1];
+  __shared__ unsigned char s_pos[FW];
+  const int lane = threadIdx.x;
+  const int w = blockIdx.x;
+  if (w >= cnt) return;
+  const int4 d = desc[w];
+  const int p0 = d.y, wi = d.z, nk = d.w;
+  for (int t = lane; t < wi; t += 64) {
+    s_col[t] = srt_cols[p0 + t];
+    s_pos[t] = srt_pos[p0 + t];
+  }
+  int carry = 0;
+  for (int t0 = 0; t0 < nk; t0 += 64) {
+    const int t = t0 + lane;
+    int len = 0;
+    if (t < nk) {
+      const int2 pv = piv[p0 + t];
+      s_kb[t] = pv.x;
+      len = pv.y;
+    }
+    int incl = len;
+    for (int o = 1; o < 64; o <<= 1) {
+      const int v = __shfl_up(incl, o, 64);
+      if (lane >= o) incl += v;
+    }
+    if (t < nk) s_off[t + 1] = carry + incl;
+    carry += __shfl(incl, 63, 64);
+  }
+  if (lane == 0) s_off[0] = 0;
+  __syncthreads();
+  const int total = s_off[nk];
+  const long long pb = pbase[w];
+  for (int u = lane; u < total; u += 64) {
+    int lo = 0, hi = nk - 1;                // pivot t with s_off[t] <= u < s_off[t+1]
+    while (lo < hi) {
+      const int mid = (lo + hi + 1) >> 1;
+      if (s_off[mid] <= u) lo = mid; else hi = mid - 1;
+    }
+    const int k = s_col[lo];
+    const int cc = cols[s_kb[lo] + 1 + (u - s_off[lo])];
+    int pos = 255;
+    if (cc > k) {
+      int a = 0, b = wi - 1;
+      while (a <= b) {
+        const int mid = (a + b) >> 1;
+        const int v = s_col[mid];
+        if (v == cc) { pos = s_pos[mid]; break; }
+        if (v < cc) a = mid + 1; else b = mid - 1;
+      }
+    }
+    plan_t[pb + u] = (unsigned char)lo;
+    plan_pos[pb + u] = (unsigned char)pos;
+  }
+}
+
 template <class VT>
 __global__ void __launch_bounds__(64) k_ilu_level(const int4* __restrict__ desc, int cnt,
                                                   const int* __restrict__ cols,
                                                   const int* __restrict__ srt_cols,
                                                   const unsigned char* __restrict__ srt_pos,
                                                   const int2* __restrict__ piv,
+                                                  const long long* __restrict__ pbase,
+                                                  const unsigned char* __restrict__ plan_t,
+                                                  const unsigned char* __restrict__ plan_pos,
                                                   VT* __restrict__ LU, int* __restrict__ err)
 {
 #pragma clang fp contract(off)
@@ -126,30 +197,28 @@ __global__ void __launch_bounds__(64) k_ilu_level(const int4* __restrict__ desc,
   const int total = s_off[nk];
   const bool staged = total <= UCAP;
   for (int t = lane; t < nk; t += 64) s_dk[t] = LU[s_kb[t]];
-  if (staged) {
-    int uc[UPL], ut[UPL];
+  if (staged) {                         // pivot-row values, targets from the plan
+    const long long pb = pbase[w];
+    unsigned char pt[UPL], pp[UPL];
+#pragma unroll
+    for (int m = 0; m < UPL; m++) {
+      const int u = lane + 64 * m;
+      if (u < total) { pt[m] = plan_t[pb + u]; pp[m] = plan_pos[pb + u]; }
+    }
     VT uv[UPL];
 #pragma unroll
     for (int m = 0; m < UPL; m++) {
       const int u = lane + 64 * m;
-      ut[m] = 0;
       if (u < total) {
-        int lo = 0, hi = nk - 1;            // pivot row t with s_off[t] <= u < s_off[t+1]
-        while (lo < hi) {
-          const int mid = (lo + hi + 1) >> 1;
-          if (s_off[mid] <= u) lo = mid; else hi = mid - 1;
-        }
-        const int q = s_kb[lo] + 1 + (u - s_off[lo]);
-        ut[m] = lo;
-        uc[m] = cols[q];
-        uv[m] = LU[q];
+        const int t = pt[m];
+        uv[m] = LU[s_kb[t] + 1 + (u - s_off[t])];
       }
     }
 #pragma unroll
     for (int m = 0; m < UPL; m++) {
       const int u = lane + 64 * m;
       if (u < total) {
-        s_upos[u] = (unsigned char)target(uc[m], s_col[ut[m]]);
+        s_upos[u] = pp[m];
         s_uval[u] = uv[m];
       }
     }
@@ -790,8 +859,21 @@ void ilu_setup(Ilu& I, int n, const std::vector<int>& rp, const std::vector<int>
         desc[w] = int4{ i, p0, wi, nk };
       }
     });
+    // plan offsets: every row's concatenated pivot-row entries
+    std::vector<long long> pbase(n + 1, 0);
+    for (int w = 0; w < n; w++) {
+      long long tot = 0;
+      for (int t = 0; t < desc[w].w; t++) tot += piv[desc[w].y + t].y;
+      pbase[w + 1] = pbase[w] + tot;
+    }
     I.fdesc.upload(desc, s);
     I.fpiv.upload(piv, s);
+    I.fpbase.upload(pbase, s);
+    I.plan_t.alloc(std::max<long long>(pbase[n], 1));
+    I.plan_pos.alloc(std::max<long long>(pbase[n], 1));
+    k_ilu_plan<<<n, 64, 0, s>>>(I.fdesc.p, n, I.cols.p, I.srt_cols.p, I.srt_pos.p, I.fpiv.p,
+                                I.fpbase.p, I.plan_t.p, I.plan_pos.p);
+    HIPCHECK(hipGetLastError());
   }
   std::vector<int> olo, oup;
   build_sweep(I.lo, I, true, olo, s);
@@ -832,7 +914,8 @@ static void factor_t(Ctx& c, Ilu& I, const VT* A)
   for (size_t l = 0; l + 1 < I.flev.size(); l++) {
     const int a = I.flev[l], cnt = I.flev[l + 1] - a;
     k_ilu_level<VT><<<cnt, 64, 0, c.stream>>>(I.fdesc.p + a, cnt, I.cols.p, I.srt_cols.p,
-                                              I.srt_pos.p, I.fpiv.p, LU.p, err);
+                                              I.srt_pos.p, I.fpiv.p, I.fpbase.p + a,
+                                              I.plan_t.p, I.plan_pos.p, LU.p, err);
   }
   HIPCHECK(hipGetLastError());
   DBuf<VT>& pl = PLbuf<VT>(I);

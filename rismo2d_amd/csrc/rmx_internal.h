@@ -133,6 +133,8 @@ struct Ilu {
   DBuf<unsigned char> srt_pos;
   DBuf<int4> fdesc;             // factorisation: per level position {row, start, width, lower}
   DBuf<int2> fpiv;              // per sorted lower entry: pivot row start, entries after diag
+  DBuf<long long> fpbase;       // per level position: offset of its plan entries
+  DBuf<unsigned char> plan_t, plan_pos;   // per plan entry: pivot index, target position
   SweepSet lo, up;
   DBuf<double> LU64, plo64, pup64, pdg64;
   DBuf<float> LU32, plo32, pup32, pdg32;
