@@ -224,20 +224,42 @@ __global__ void __launch_bounds__(64) k_ilu_level(const int4* __restrict__ desc,
     }
   }
   __syncthreads();
-  for (int t = 0; t < nk; t++) {
-    const int k = s_col[t];
-    const int pik = s_pos[t];
-    const VT f = -s_val[pik] / s_dk[t];   // every lane, from the same LDS values
-    const int ua = s_off[t], ub = s_off[t + 1];
-    if (staged) {
-      for (int e = ua + lane; e < ub; e += 64) {
-        const int pos = s_upos[e];
-        if (pos != 255) {
-          const VT prod = f * s_uval[e];
-          s_val[pos] = s_val[pos] + prod;
-        }
-      }
-    } else {
+  if (staged) {
+    // pivot t's diagonal position and value and this lane's (at most two) entries of its
+    // pivot row (target, value) are read one pivot ahead, so that a pivot's chain is the
+    // read of the two row values, the division and the update
+    auto load_piv = [&](int t, int& pik, VT& dk, int& pa, VT& ua, int& pb, VT& ub_) {
+      pik = s_pos[t];
+      dk = s_dk[t];
+      const int lo = s_off[t], hi = s_off[t + 1];
+      const int e0 = lo + lane, e1 = e0 + 64;
+      pa = 255; pb = 255; ua = (VT)0; ub_ = (VT)0;
+      if (e0 < hi) { pa = s_upos[e0]; ua = s_uval[e0]; }
+      if (e1 < hi) { pb = s_upos[e1]; ub_ = s_uval[e1]; }
+    };
+    int pik = 0, pa = 255, pb = 255;
+    VT dk = (VT)1, ua = (VT)0, ub_ = (VT)0;
+    if (nk > 0) load_piv(0, pik, dk, pa, ua, pb, ub_);
+    for (int t = 0; t < nk; t++) {
+      int pikn = 0, pan = 255, pbn = 255;
+      VT dkn = (VT)1, uan = (VT)0, ubn = (VT)0;
+      if (t + 1 < nk) load_piv(t + 1, pikn, dkn, pan, uan, pbn, ubn);
+      const VT a = s_val[pik];
+      const VT ca = pa != 255 ? s_val[pa] : (VT)0;
+      const VT cb = pb != 255 ? s_val[pb] : (VT)0;
+      const VT f = -a / dk;                // every lane, from the same LDS values
+      if (pa != 255) { const VT prod = f * ua; s_val[pa] = ca + prod; }
+      if (pb != 255) { const VT prod = f * ub_; s_val[pb] = cb + prod; }
+      __syncthreads();                    // the updates before the next pivot
+      if (lane == 0) s_val[pik] = f;      // the stored multiplier (no later pivot reads it)
+      pik = pikn; dk = dkn; pa = pan; ua = uan; pb = pbn; ub_ = ubn;
+    }
+  } else {
+    for (int t = 0; t < nk; t++) {
+      const int k = s_col[t];
+      const int pik = s_pos[t];
+      const VT f = -s_val[pik] / s_dk[t];
+      const int ua = s_off[t], ub = s_off[t + 1];
       for (int e = lane; e < ub - ua; e += 64) {
         const int q = s_kb[t] + 1 + e;
         const int pos = target(cols[q], k);
@@ -246,9 +268,9 @@ __global__ void __launch_bounds__(64) k_ilu_level(const int4* __restrict__ desc,
           s_val[pos] = s_val[pos] + prod;
         }
       }
+      __syncthreads();                    // the stored multiplier and updates before the next k
+      if (lane == 0) s_val[pik] = f;
     }
-    __syncthreads();                      // the stored multiplier and updates before the next k
-    if (lane == 0) s_val[pik] = f;
   }
   __syncthreads();
   for (int t = lane; t < wi; t += 64) LU[p0 + t] = s_val[t];
