@@ -979,7 +979,7 @@ static void launch_sweeps(Ctx& c, Ilu& I, const double* b, double* x, hipEvent_t
   const int nb = I.nblk;
   if (ev) HIPCHECK(hipEventRecord(ev[0], c.stream));
   if constexpr (V == 1)
-    k_sweep_stage<VT, true, 4, 6><<<nb, 512, 0, c.stream>>>(I.lo.blev.p, I.lo.bpos.p,
+    k_sweep_stage<VT, true, 4, 12><<<nb, 512, 0, c.stream>>>(I.lo.blev.p, I.lo.bpos.p,
         I.lo.lev_ptr.p, I.lo.prs.p, I.lo.pcol.p, PLbuf<VT>(I).p, nullptr, I.lo.prow.p, b, nullptr,
         nullptr, I.lo.xp.p);
   else
