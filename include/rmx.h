@@ -230,7 +230,9 @@ typedef struct {
   int mkyrl;                  /* FGMRES restart length (SOLVER::mkyrl) */
   double eps;                 /* SOLVER::maxDiff: relative residual tolerance */
   int refine;                 /* 1: FP64 iterative refinement around the solve */
-  double refine_tol;          /* true FP64 relative residual to reach when refining */
+  double refine_tol;          /* true FP64 relative residual to reach when refining; a
+                                 refinement sweep's inner tolerance is max(eps, 0.5 refine_tol
+                                 / current residual), so the last sweep stops near the target */
   int refine_max;             /* maximum refinement sweeps */
 } rmx_krylov_opts;
 

@@ -447,7 +447,10 @@ void dev_solve(Ctx& c, const rmx_krylov_opts& o, rmx_krylov_out& out)
     for (; sw < sweeps_max; sw++) {
       int it = 0; double acc = 1.0;
       const double tr_prev = tr;
-      krylov(c, method, prec, kind, r, d, o.max_iter, o.eps, o.mkyrl, it, acc);
+      // a sweep solves for the remaining factor only: the inner tolerance is relaxed to half
+      // of what takes the true residual from tr to refine_tol, never tightened below eps
+      const double eps_k = std::max(o.eps, 0.5 * o.refine_tol / tr);
+      krylov(c, method, prec, kind, r, d, o.max_iter, std::min(eps_k, 0.5), o.mkyrl, it, acc);
       total += it;
       out.accuracy = acc;
       k_axpy<<<vblocks(c.n), 256, 0, c.stream>>>(c.n, 1.0, d, xr);
