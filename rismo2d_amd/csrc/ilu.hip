@@ -592,7 +592,23 @@ __global__ void __launch_bounds__(512) k_sweep_stage(const int* __restrict__ ble
             s += (double)vW[j] * ring[cc & (RING - 1)];
           }
         }
-        if (old) s = lane_sum<VT, G>(pval, pcol, ring, xp, qaW + lane, min(qbW, qaW + PWN * G), lim);
+        if (old) {                           // dependencies that left the ring: global copy
+          double xg[PWN];
+#pragma unroll
+          for (int j = 0; j < PWN; j++) {
+            const int cc = cW[j];
+            xg[j] = xp[(cc >= 0 && cc < lim) ? cc : 0];
+          }
+          s = 0.0;
+#pragma unroll
+          for (int j = 0; j < PWN; j++) {
+            const int cc = cW[j];
+            if (cc >= 0) {
+              const double rv = ring[cc & (RING - 1)];
+              s += (double)vW[j] * (cc < lim ? xg[j] : rv);
+            }
+          }
+        }
         if (qbW > qaW + PWN * G)
           s += lane_sum<VT, G>(pval, pcol, ring, xp, qaW + lane + PWN * G, qbW, lim);
       }
@@ -977,9 +993,10 @@ static void launch_sweeps(Ctx& c, Ilu& I, const double* b, double* x, hipEvent_t
   // workgroups per compute unit by registers and LDS); V = 0: one-deep prefetch, 8 lanes
   // per row, 1024 threads
   const int nb = I.nblk;
+  const int nt = 512;
   if (ev) HIPCHECK(hipEventRecord(ev[0], c.stream));
   if constexpr (V == 1)
-    k_sweep_stage<VT, true, 4, 12><<<nb, 512, 0, c.stream>>>(I.lo.blev.p, I.lo.bpos.p,
+    k_sweep_stage<VT, true, 4, 12><<<nb, nt, 0, c.stream>>>(I.lo.blev.p, I.lo.bpos.p,
         I.lo.lev_ptr.p, I.lo.prs.p, I.lo.pcol.p, PLbuf<VT>(I).p, nullptr, I.lo.prow.p, b, nullptr,
         nullptr, I.lo.xp.p);
   else
@@ -988,7 +1005,7 @@ static void launch_sweeps(Ctx& c, Ilu& I, const double* b, double* x, hipEvent_t
         nullptr, I.lo.xp.p);
   if (ev) HIPCHECK(hipEventRecord(ev[1], c.stream));
   if constexpr (V == 1)
-    k_sweep_stage<VT, false, 4, 6><<<nb, 512, 0, c.stream>>>(I.up.blev.p, I.up.bpos.p,
+    k_sweep_stage<VT, false, 4, 6><<<nb, nt, 0, c.stream>>>(I.up.blev.p, I.up.bpos.p,
         I.up.lev_ptr.p, I.up.prs.p, I.up.pcol.p, PUbuf<VT>(I).p, PDbuf<VT>(I).p, I.fwd2bwd.p,
         I.lo.xp.p, I.up.prow.p, x, I.up.xp.p);
   else
