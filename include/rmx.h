@@ -121,6 +121,17 @@ rmx_status rmx_turbulence(rmx_ctx* ctx, double* vt, double* uu);
  * rmx_time_commit ends a time step: the state becomes the previous one (Compute.cpp:
  * 773-796). */
 rmx_status rmx_time_step(rmx_ctx* ctx, double dt, double theta, int first);
+
+/* Inlet discharge and reference water level of the QINLET line for the following Newton
+ * steps: a hydrograph (QTINLET) gives them per time step (BCONLINE::Inlet,
+ * BconLine.cpp:263-276, at the time step's end time, EqsUVS2D.cpp:318-320); hydrograph != 0
+ * gives the line's nodes the QTINLET kind as well (BconLine.cpp:86). */
+rmx_status rmx_set_inflow(rmx_ctx* ctx, double Q, double S_in, int hydrograph);
+/* Hydrograph lookup of BCONLINE::FindInterval (BconLine.cpp:167-226) with BconLine.cpp:
+ * 268-275: the first interval [t_i, t_i+1] (widened by 0.1 %) that holds `time` gives Q, S
+ * by linear interpolation; outside every interval Q[0], S[0].  No context needed. */
+rmx_status rmx_hydrograph(int n, const double* t, const double* Q, const double* S, double time,
+                          double* Qout, double* Sout);
 rmx_status rmx_time_commit(rmx_ctx* ctx);
 /* Host copies of the time-integration state, 9 [np] arrays (NULL entries skipped):
  * Uo, Vo, So, dUdt_o, dVdt_o, dSdt_o (previous time step), dUdt, dVdt, dSdt (current).

@@ -72,6 +72,12 @@ void   orc_get_nodal(const orc_model* m, double* cf, int* bckind, int* rot,
  * orc_time_commit ends the time step (Compute.cpp:773-796).  Returns -1 for theta or dt
  * below 1e-10 in magnitude. */
 int    orc_time_step(orc_model* m, double dt, double theta, int first);
+/* QINLET discharge and reference level for the following Newton steps; hydrograph != 0
+ * marks the line as QTINLET (its nodes' kind) */
+void   orc_set_inflow(orc_model* m, double Q, double S_in, int hydrograph);
+/* hydrograph lookup (BCONLINE::FindInterval + interpolation): Q, S at `time` */
+void   orc_hydrograph(int n, const double* t, const double* Q, const double* S, double time,
+                      double* Qout, double* Sout);
 void   orc_time_commit(orc_model* m);
 /* previous state (vo) and time derivatives (v.d*dt, vo.d*dt): set (NULL keeps) / get */
 void   orc_set_time_state(orc_model* m, const double* Uo, const double* Vo, const double* So,

@@ -46,6 +46,8 @@ def lib():
         L.orc_time_step.argtypes = [C.c_void_p, C.c_double, C.c_double, C.c_int]
         L.orc_time_step.restype = C.c_int
         L.orc_time_commit.argtypes = [C.c_void_p]
+        L.orc_set_inflow.argtypes = [C.c_void_p, C.c_double, C.c_double, C.c_int]
+        L.orc_hydrograph.argtypes = [C.c_int, _d, _d, _d, C.c_double, _d, _d]
         L.orc_set_time_state.argtypes = [C.c_void_p] + [_d] * 9
         L.orc_get_time_state.argtypes = [C.c_void_p] + [_d] * 9
         L.orc_lindner.argtypes = [C.c_double] * 7
@@ -151,6 +153,9 @@ class OracleModel:
 
     def time_commit(self):
         lib().orc_time_commit(self.h)
+
+    def set_inflow(self, Q, S_in, hydrograph=True):
+        lib().orc_set_inflow(self.h, Q, S_in, int(hydrograph))
 
     def set_time_state(self, **kw):
         """Previous state / time derivatives by name (TIME_KEYS); missing ones are kept."""
@@ -259,6 +264,15 @@ class OracleModel:
 
 # ---------------------------------------------------------------------------------------
 # CSR kernels on plain arrays
+
+def hydrograph(tQS, time):
+    """Q, S of a hydrograph (rows t, Q, S) at `time` (BCONLINE::FindInterval)."""
+    a = np.ascontiguousarray(np.asarray(tQS, np.float64).T)
+    q, s_ = C.c_double(), C.c_double()
+    lib().orc_hydrograph(a.shape[1], _p(a[0], _d), _p(a[1], _d), _p(a[2], _d), time,
+                         C.byref(q), C.byref(s_))
+    return q.value, s_.value
+
 
 def _vp(A):
     return (_f, "f32") if A.dtype == np.float32 else (_d, "f64")

@@ -22,7 +22,7 @@
 
 /* ------------------------------------------------------------------------------------- */
 /* reference flag values (Bcon.h:160-190, Node.h, Elem.h) so dumps compare directly      */
-enum { kInlet = 1, kOutlet = 2, kOpenBnd = 4, kQInlet = 8, kSlip = 128, kLoglaw = 256,
+enum { kInlet = 1, kOutlet = 2, kOpenBnd = 4, kQInlet = 8, kQTInlet = 16, kSlip = 128, kLoglaw = 256,
        kSetUV = 512, kSetS = 1024, kFixU = 32768, kFixV = 65536, kAutoSlip = 4194304 };
 enum { rotTangent = 1, rotNormal = 2 };
 /* node flags of this restatement (own bit values) */
@@ -209,6 +209,7 @@ struct orc_model {
   double *Uo, *Vo, *So, *dUdto, *dVdto, *dSdto;
   int unsteady;
   double dt, theta;
+  int inflow_qt;                    /* the inlet line is a hydrograph (QTINLET) */
   int n_in, n_out;
   int *in_segs, *out_segs;
   double Q, S_in, S_out;
@@ -463,6 +464,42 @@ static void timegrad(orc_model* m)
     m->dUdt[n] = iTheta * m->dUdto[n] + thdt * (m->U[n] - m->Uo[n]);
     m->dVdt[n] = iTheta * m->dVdto[n] + thdt * (m->V[n] - m->Vo[n]);
     m->dSdt[n] = iTheta * m->dSdto[n] + thdt * (m->S[n] - m->So[n]);
+  }
+}
+
+/* QINLET discharge and reference level for the following Newton steps (a hydrograph gives
+ * them per time step: BCONLINE::Inlet, BconLine.cpp:263-276; its line carries the QTINLET
+ * kind as well) */
+void orc_set_inflow(orc_model* m, double Q, double S_in, int hydrograph)
+{
+  m->Q = Q;
+  m->S_in = S_in;
+  m->inflow_qt = hydrograph != 0;
+}
+
+/* BCONLINE::FindInterval (BconLine.cpp:167-226) without a second coordinate, then the
+ * interpolation of BconLine.cpp:268-275; Q[0], S[0] when no interval holds the time */
+void orc_hydrograph(int n, const double* t, const double* Q, const double* S, double time,
+                    double* Qout, double* Sout)
+{
+  const double kEps = 1.0e-20;           /* Defs.h:85 */
+  *Qout = Q[0];
+  *Sout = S[0];
+  for (int i = 0; i < n - 1; i++) {
+    double dx = t[i + 1] - t[i];
+    double lx = time - t[i];
+    if (fabs(dx) > kEps) lx /= dx;
+    else lx = dx = 0.0;
+    double l;
+    if (fabs(dx) < kEps && fabs(lx) < kEps) {
+      l = 0.0;                           /* ly = 0 */
+    } else {
+      if (!(lx > -0.001 && lx < 1.001)) continue;
+      l = lx;
+    }
+    *Qout = Q[i] + l * (Q[i + 1] - Q[i]);
+    *Sout = S[i] + l * (S[i + 1] - S[i]);
+    return;
   }
 }
 
@@ -739,7 +776,7 @@ static void init_bcon(orc_model* m)
   double* bS = xcalloc((size_t)np, sizeof(double));
   /* --- QINLET ----------------------------------------------------------------------- */
   {
-    unsigned kind = kQInlet | kInlet;
+    unsigned kind = kQInlet | kInlet | (m->inflow_qt ? kQTInlet : 0u);  /* BconLine.cpp:86-87 */
     double Qspec = m->Q, Sspec = m->S_in;
     double Q = 0.0;
     int any = 0;

@@ -294,6 +294,54 @@ rmx_status rmx_time_step(rmx_ctx* p, double dt, double theta, int first)
   API_END
 }
 
+rmx_status rmx_set_inflow(rmx_ctx* p, double Q, double S_in, int hydrograph)
+{
+  API_BEGIN
+  Ctx& c = *need(p);
+  if (!c.mesh_ready) fail(RMX_ERR_ARG, "inflow: upload a mesh first");
+  if (!std::isfinite(Q) || !std::isfinite(S_in)) fail(RMX_ERR_ARG, "inflow: not finite");
+  c.hm.Q = Q;
+  c.hm.S_in = S_in;
+  c.hm.inflow_qt = hydrograph != 0;
+  if (c.prepared && !c.csr_only) {                 // prescribed level and kind of the inlet nodes
+    for (int n : c.hm.inlet_nodes) {
+      c.hm.valS[n] = S_in;
+      c.hm.kind[n] = (c.hm.kind[n] & ~(uint32_t)kQTInlet) | (hydrograph ? kQTInlet : 0u);
+    }
+    c.valS.upload(c.hm.valS, c.stream);
+    HIPCHECK(hipStreamSynchronize(c.stream));
+  }
+  API_END
+}
+
+rmx_status rmx_hydrograph(int n, const double* t, const double* Q, const double* S, double time,
+                          double* Qout, double* Sout)
+{
+  API_BEGIN
+  if (n < 1 || !t || !Q || !S || !Qout || !Sout) fail(RMX_ERR_ARG, "hydrograph: bad arguments");
+  const double eps = 1.0e-20;                      // kEpsilon (Defs.h:85) of the interval test
+  double q = Q[0], s = S[0];
+  for (int i = 0; i + 1 < n; i++) {
+    double dx = t[i + 1] - t[i];
+    double lx = time - t[i];
+    if (std::fabs(dx) > eps) lx /= dx;
+    else lx = dx = 0.0;
+    // with no second coordinate the interval test is on lx alone
+    if (std::fabs(dx) < eps && std::fabs(lx) < eps) {
+      q = Q[i]; s = S[i];                          // degenerate interval at its own time
+      break;
+    }
+    if (lx > -0.001 && lx < 1.001) {
+      q = Q[i] + lx * (Q[i + 1] - Q[i]);
+      s = S[i] + lx * (S[i + 1] - S[i]);
+      break;
+    }
+  }
+  *Qout = q;
+  *Sout = s;
+  API_END
+}
+
 rmx_status rmx_time_commit(rmx_ctx* p)
 {
   API_BEGIN

@@ -146,7 +146,11 @@ void host_boundary_conditions(HostModel& hm)
   for (int v : hm.in_segs) mark[v] = 1;
   hm.inlet_nodes.clear();
   for (int n = 0; n < np; n++)
-    if (mark[n]) { hm.inlet_nodes.push_back(n); hm.kind[n] = kQInlet | kInlet; hm.valS[n] = hm.S_in; }
+    if (mark[n]) {
+      hm.inlet_nodes.push_back(n);
+      hm.kind[n] = kQInlet | kInlet | (hm.inflow_qt ? kQTInlet : 0u);   // BconLine.cpp:86-87
+      hm.valS[n] = hm.S_in;
+    }
   for (int v : hm.out_segs)
     if (!(hm.kind[v] & kInlet)) { hm.kind[v] |= kOutlet; hm.valS[v] = hm.S_out; }
   // SetLocation: line element flags from its corner kinds, node flags from the element

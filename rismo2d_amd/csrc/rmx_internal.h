@@ -14,7 +14,7 @@ namespace rmx {
 
 // Boundary-condition kind bits: the reference's BCON flag values (Bcon.h:160-190).
 enum : uint32_t {
-  kInlet = 1, kOutlet = 2, kOpenBnd = 4, kQInlet = 8, kSlip = 128, kSetUV = 512,
+  kInlet = 1, kOutlet = 2, kOpenBnd = 4, kQInlet = 8, kQTInlet = 16, kSlip = 128, kSetUV = 512,
   kSetS = 1024, kFixU = 32768, kFixV = 65536, kAutoSlip = 4194304
 };
 enum { rotTangent = 1, rotNormal = 2 };
@@ -77,6 +77,7 @@ struct HostModel {
   std::vector<int> elem, nnd;                 // [ne*8], [ne]
   std::vector<int> in_segs, out_segs;
   double Q = 0, S_in = 0, S_out = 0;
+  bool inflow_qt = false;                     // the inlet line is a hydrograph (QTINLET)
   // topology
   std::vector<int> noel;
   std::vector<uint32_t> flag;

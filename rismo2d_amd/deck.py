@@ -50,6 +50,7 @@ class Deck:
                                        # 2 algebraic; +100 vt of the element is a lower bound
     turb_st: np.ndarray | None = None  # [ne] st of the algebraic model (None: 1.0)
     turb_kdest: np.ndarray | None = None   # [ne] KDest of the algebraic model (None: 0.6)
+    hydrograph: np.ndarray | None = None   # [k, 3] inlet hydrograph rows (t s, Q, S) (QTINLET)
 
     @property
     def n_nodes(self) -> int:
@@ -374,12 +375,17 @@ def write_reference_deck(deck: Deck, outdir: str, solver: str = "1 6 1 0 60 1000
         for (n, vt, dp, sp, a, b), no in mat_of.items():
             # no vt st estx esty KDest lm betaSf (TYPE::Import_40000)
             f.write(f"$TURBULENCE {no} {vt:.17g} {a:.17g} 0.0 0.0 {b:.17g} 0.0 0.0\n")
+    if deck.hydrograph is not None:
+        inlet_lines = "".join(f"$TM_LINE QTINLET {INLET_LINE} {q:.17g} {sv:.17g} {t:.17g}\n"
+                              for t, q, sv in np.asarray(deck.hydrograph, np.float64))
+    else:
+        inlet_lines = f"$TM_LINE QINLET {INLET_LINE} {deck.Q_inlet:.17g} {deck.S_inlet:.17g}\n"
     with open(os.path.join(outdir, "tm.dat"), "w") as f:
         f.write(f"$RISMO2D 40100\n$TM_STEPS 1 {tm_steps} 0\n$TM_INTERVAL {dt:.17g} 0.0 0.0\n"
                 f"$TM_WEIGHT {theta:.17g} 1.0 1.0\n$TM_STEP_NO 1\n$TM_CYCLE 1\n"
                 f"$TM_STATIONARY {stationary}\n"
                 f"$TM_TURBULENCE {deck.turb}\n$TM_MAXITER {maxiter}\n$TM_SOLVER {tm_solver}\n"
-                f"$TM_LINE QINLET {INLET_LINE} {deck.Q_inlet:.17g} {deck.S_inlet:.17g}\n"
+                + inlet_lines +
                 f"$TM_LINE OUTLET {OUTLET_LINE} {deck.S_outlet:.17g} 1000.0 0.0 0 0.0\n")
     with open(os.path.join(outdir, "input.dat"), "w") as f:
         f.write("$RISMO2D 40700\n$REPORTLEVEL 3\n$REPORTFILE report.txt\n"

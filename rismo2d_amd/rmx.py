@@ -106,6 +106,8 @@ def lib():
             "rmx_turbulence": [_v, _d, _d],
             "rmx_time_step": [_v, C.c_double, C.c_double, C.c_int], "rmx_time_commit": [_v],
             "rmx_time_state": [_v, C.c_int, C.POINTER(_d)],
+            "rmx_set_inflow": [_v, C.c_double, C.c_double, C.c_int],
+            "rmx_hydrograph": [C.c_int, _d, _d, _d, C.c_double, _d, _d],
             "rmx_assemble": [_v, C.c_int], "rmx_scale_l2": [_v, _d],
             "rmx_convert_matrix": [_v, C.c_int], "rmx_ilu0_factor": [_v, C.c_int],
             "rmx_ilu0_apply": [_v, C.c_int, _d, _d], "rmx_spmv": [_v, C.c_int, _d, _d],
@@ -223,6 +225,11 @@ class Context:
 
     def time_commit(self):
         _check(lib().rmx_time_commit(self.h))
+
+    def set_inflow(self, Q, S_in, hydrograph=True):
+        """QINLET discharge and reference level for the following Newton steps (from a
+        hydrograph: the line is QTINLET)."""
+        _check(lib().rmx_set_inflow(self.h, Q, S_in, int(hydrograph)))
 
     def time_state(self):
         out = {k: np.zeros(self.np) for k in self.TIME_KEYS}
@@ -456,3 +463,12 @@ class Context:
         kt = KernelTimes()
         _check(lib().rmx_profile_kernels(self.h, prec, reps, C.byref(kt)))
         return kt
+
+
+def hydrograph(tQS, time):
+    """Q, S of a hydrograph (rows t, Q, S) at `time`, as the reference's QTINLET lookup."""
+    a = np.ascontiguousarray(np.asarray(tQS, np.float64).T)
+    q, s_ = C.c_double(), C.c_double()
+    _check(lib().rmx_hydrograph(a.shape[1], _p(a[0], _d), _p(a[1], _d), _p(a[2], _d), time,
+                                C.byref(q), C.byref(s_)))
+    return q.value, s_.value

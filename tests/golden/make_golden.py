@@ -24,6 +24,9 @@ the reference's input format, runs the reference and stores what the harness rec
                     derivatives).
 * ``unstq10x3.npz`` the 10 x 3 quad channel, unsteady with the algebraic eddy viscosity
                     (3 steps of 20 s, theta 0.6; element matrices of Newton step 2).
+* ``hydro20x4.npz`` the 20 x 4 triangle channel, unsteady (4 steps of 30 s, theta 0.7)
+                    under an inlet hydrograph (QTINLET: Q 10 -> 16 -> 8 m3/s over 120 s);
+                    element matrices of Newton step 5 (second time step).
 * ``channel200x20.npz``  the straight-channel workload (200 x 20 cells, 8,000 triangles,
                     bed noise sigma 0.05 m / 20 m, seed 0): numbering, per-step scale and
                     correction, BiCGStab iteration counts from the report, final state,
@@ -70,6 +73,13 @@ DECKS = {
                      Q=10.0),
     "unstq10x3": dict(nx=10, ny=3, length=100.0, width=30.0, noise_sigma=0.05, noise_corr=5.0,
                       Q=10.0, quads=True, turb=2, turb_st=(0.5, 1.5)),
+    "hydro20x4": dict(nx=20, ny=4, length=100.0, width=20.0, noise_sigma=0.05, noise_corr=5.0,
+                      Q=10.0),
+}
+
+# inlet hydrographs (QTINLET), rows (t s, Q m3/s, S m)
+HYDRO = {
+    "hydro20x4": [[0.0, 10.0, 2.10], [60.0, 16.0, 2.16], [120.0, 8.0, 2.04]],
 }
 
 # time integration of the decks that are not one stationary flow step, and the Newton step
@@ -77,6 +87,7 @@ DECKS = {
 RUNS = {
     "unst20x4": dict(tm_steps=3, dt=30.0, theta=0.7, stationary=0, elem_step=1),
     "unstq10x3": dict(tm_steps=3, dt=20.0, theta=0.6, stationary=0, elem_step=2),
+    "hydro20x4": dict(tm_steps=4, dt=30.0, theta=0.7, stationary=0, elem_step=5),
 }
 
 
@@ -101,6 +112,8 @@ def deck_arrays(d):
     if d.veg_dp is not None:
         out["veg_dp"] = d.veg_dp
         out["veg_sp"] = d.veg_sp
+    if d.hydrograph is not None:
+        out["hydrograph"] = d.hydrograph
     if d.turb != 1:
         out["turb"] = d.turb
         for key in ("turb_st", "turb_kdest"):
@@ -127,6 +140,8 @@ def main():
         kw = DECKS[name]
         run = RUNS.get(name, {})
         d = dk.channel(**kw)
+        if name in HYDRO:
+            d.hydrograph = np.asarray(HYDRO[name], np.float64)
         with tempfile.TemporaryDirectory() as tmp:
             (steps, final), iters = run_reference(d, tmp, run)
         out = deck_arrays(d)

@@ -416,7 +416,7 @@ def test_error_paths(ctx):
     ctx.set_params(turb=1)
 
 
-UNSTEADY = ["unst20x4", "unstq10x3"]
+UNSTEADY = ["unst20x4", "unstq10x3", "hydro20x4"]
 
 
 @pytest.mark.parametrize("name", UNSTEADY)
@@ -435,6 +435,10 @@ def test_unsteady_element_kernel_matches_oracle(ctx, name):
     ctx.state_upload(g[p + "U"], g[p + "V"], g[p + "S"])
     ctx.time_step(dt, th, first=False)
     ctx.set_time_state(**tkw)
+    inflow = None
+    if "hydrograph" in g:
+        inflow = rmx.hydrograph(g["hydrograph"], int(g[p + "tstep"]) * dt)
+        ctx.set_inflow(*inflow)
     n = ctx.prepare()
     ts = ctx.time_state()
     for k in rmx.Context.TIME_KEYS:
@@ -443,6 +447,8 @@ def test_unsteady_element_kernel_matches_oracle(ctx, name):
     om.set_state(g[p + "U"], g[p + "V"], g[p + "S"])
     om.time_step(dt, th)
     om.set_time_state(**tkw)
+    if inflow is not None:
+        om.set_inflow(*inflow)
     assert om.prepare() == n
     ne = ctx.n_model_elems()
     K, F = ctx.element_coefs(0, ne)
@@ -497,6 +503,9 @@ def test_unsteady_newton_follows_reference(ctx, name):
                 print(f"    time step {t}: {k} rel err {err:.2e}")
                 assert err <= 1e-9, (t, k)
         ctx.time_step(dt, th, first=first)
+        if "hydrograph" in g:
+            # QTINLET: the inflow at the time step's end (EqsUVS2D.cpp:318-320)
+            ctx.set_inflow(*rmx.hydrograph(g["hydrograph"], t * dt))
         first = False
         for k in ks:
             st = ctx.newton_step(method=rmx.BICGSTAB, prec=rmx.F64, eps=1e-12)

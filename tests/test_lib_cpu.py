@@ -49,3 +49,13 @@ def test_deck_generators_shapes():
     for i, (a, b) in enumerate([(0, 1), (1, 2), (2, 0)]):
         mid = m.elems[:, 3 + i]
         np.testing.assert_allclose(m.x[mid], 0.5 * (m.x[m.elems[:, a]] + m.x[m.elems[:, b]]), atol=1e-9)
+
+
+def test_hydrograph_lookup_matches_oracle():
+    """The library's QTINLET lookup (host code, no device needed) against the oracle's,
+    bit for bit, inside, at and beyond the hydrograph's points."""
+    from oracle import pyoracle as po
+    h = np.array([[0.0, 10.0, 2.10], [60.0, 16.0, 2.16], [120.0, 8.0, 2.04], [120.0, 9.0, 2.0],
+                  [300.0, 12.0, 2.2]])
+    for t in np.concatenate([np.linspace(-10.0, 320.0, 331), [60.0, 120.0, 120.06, 300.2]]):
+        assert rmx.hydrograph(h, t) == po.hydrograph(h, t), t

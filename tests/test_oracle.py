@@ -29,7 +29,7 @@ TIME_FIELDS = ("dUdt", "dVdt", "dSdt", "Uo", "Vo", "So", "dUdto", "dVdto", "dSdt
 
 
 @pytest.mark.parametrize("name", ["tri20x4", "quad10x3", "veg20x4", "turb20x4", "turbq10x3",
-                                  "unst20x4", "unstq10x3"])
+                                  "unst20x4", "unstq10x3", "hydro20x4"])
 def test_oracle_matches_reference_every_record(name):
     g = load_golden(name)
     om = po.OracleModel(golden_deck(g))
@@ -45,6 +45,9 @@ def test_oracle_matches_reference_every_record(name):
                 om.time_commit()
             tstep = int(g[p + "tstep"])
             om.time_step(float(g["dt"]), float(g["theta"]), first=(k == 0))
+            if "hydrograph" in g:
+                # QTINLET: the inflow at the time step's end (EqsUVS2D.cpp:318-320)
+                om.set_inflow(*po.hydrograph(g["hydrograph"], tstep * float(g["dt"])))
         neq = om.prepare()
         nod = _check_state(om, g, p)
         if unsteady:
@@ -282,3 +285,23 @@ def test_vt_floor_is_exercised_by_the_quad_turbulence_deck():
     K2 = [om.coefs(i)[0] for i in region]
     changed = sum(not np.array_equal(a, b) for a, b in zip(K102, K2))
     assert 0 < changed < len(region)
+
+
+def test_hydrograph_lookup_closed_forms():
+    """BCONLINE::FindInterval semantics: linear inside [t0, tn] (ends included, 0.1 %
+    tolerance), the first point outside, a zero-length interval gives its own values."""
+    h = np.array([[0.0, 10.0, 2.0], [60.0, 16.0, 2.6], [120.0, 8.0, 2.0]])
+
+    def lin(i, t):
+        lx = (t - h[i, 0]) / (h[i + 1, 0] - h[i, 0])
+        return h[i, 1] + lx * (h[i + 1, 1] - h[i, 1]), h[i, 2] + lx * (h[i + 1, 2] - h[i, 2])
+
+    assert po.hydrograph(h, 30.0) == lin(0, 30.0) and lin(0, 30.0)[0] == 13.0
+    assert po.hydrograph(h, 90.0) == lin(1, 90.0) and lin(1, 90.0)[0] == 12.0
+    assert po.hydrograph(h, 60.0) == lin(0, 60.0)           # the first interval holding t
+    assert po.hydrograph(h, 120.0) == lin(1, 120.0)
+    assert po.hydrograph(h, 120.05) == lin(1, 120.05)       # within the 0.1 % margin
+    assert po.hydrograph(h, 500.0) == (10.0, 2.0)           # outside: the first point
+    assert po.hydrograph(h, -1.0) == (10.0, 2.0)
+    d = np.array([[0.0, 5.0, 1.0], [0.0, 7.0, 1.5], [10.0, 9.0, 2.0]])
+    assert po.hydrograph(d, 0.0) == (5.0, 1.0)              # zero-length interval at t
