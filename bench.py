@@ -149,7 +149,8 @@ def main():
     lo_e, up_e = c.sweep_entries()
     fwd_bytes = sweep_bytes(lo_e, n, 4)
     bwd_bytes = sweep_bytes(up_e, n, 4) + 4 * n
-    sp_bytes = spmv_bytes(c.nnz, n, 4)
+    # the library reports the algorithmic bytes of the layout the FP32 SpMV ran on
+    sp_bytes = kt.spmv_bytes if kt.spmv_bytes > 0 else spmv_bytes(c.nnz, n, 4)
     its = max(last.solve.iters, 1)
     spmv_per_it = 1                               # FGMRES: one product per iteration
     share = {"silu_sweep_fwd": kt.silu_fwd_s * its, "silu_sweep_bwd": kt.silu_bwd_s * its,
@@ -165,7 +166,7 @@ def main():
     tf = os.path.join(HERE, "profiles", "meander2M_spmv_f32_traffic.json")
     if dom == "spmv_f32" and os.path.exists(tf):
         rec = json.load(open(tf))
-        if rec.get("n") == n and rec.get("nnz") == c.nnz:
+        if rec.get("n") == n and rec.get("nnz") == c.nnz and rec.get("spmv_bytes") == sp_bytes:
             traffic = rec["traffic_bytes"]
     log(f"kernels: spmv {kt.spmv_s * 1e3:.3f} ms ({sp_bytes / kt.spmv_s / 1e9:.0f} GB/s), strip "
         f"sweeps {kt.silu_fwd_s * 1e3:.3f} / {kt.silu_bwd_s * 1e3:.3f} ms")

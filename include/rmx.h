@@ -299,11 +299,13 @@ rmx_status rmx_newton_step(rmx_ctx* ctx, const rmx_krylov_opts* opt, rmx_step_st
 
 /* Kernel timing with HIP events on the context stream: average duration in seconds of
  * one SpMV and of the forward / backward sweeps of each factored ILU(0), over `reps`
- * runs on the current system. */
+ * runs on the current system; spmv_bytes: algorithmic bytes of that SpMV in the layout
+ * it ran on (FP32: the row-group layout when built, else CSR). */
 typedef struct {
   double spmv_s, ilu_apply_s, ilu_fwd_s, ilu_bwd_s;   /* reference-order ILU(0) */
   double silu_fwd_s, silu_bwd_s;                      /* strip ILU(0) (0 if not factored) */
   long long n_spmv, n_ilu;
+  double spmv_bytes;
 } rmx_kernel_times;
 rmx_status rmx_profile_kernels(rmx_ctx* ctx, int prec, int reps, rmx_kernel_times* out);
 

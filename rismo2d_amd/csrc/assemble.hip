@@ -768,6 +768,7 @@ void dev_assemble(Ctx& c, int prec, bool zero)
     if (zero) HIPCHECK(hipMemsetAsync(c.A32.p, 0, sizeof(float) * c.nnz, c.stream));
     launch_groups<float>(c, c.A32.p);
     c.have32 = true; c.have64 = false;
+    c.a32_version++;
   } else {
     c.A64.alloc(c.nnz);
     if (zero) HIPCHECK(hipMemsetAsync(c.A64.p, 0, sizeof(double) * c.nnz, c.stream));

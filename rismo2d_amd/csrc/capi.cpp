@@ -243,7 +243,7 @@ static void first_prepare(Ctx& c)
   c.slots.upload(slots, s);
   HIPCHECK(hipStreamSynchronize(s));
   ensure_work(c);
-  choose_spmv_lanes(c);
+  spmv_setup(c, hm.rowptr.data(), hm.cols.data());
   setup_ilu_schedule(c);
   c.have32 = c.have64 = false;
   c.prepared = true;
@@ -749,7 +749,7 @@ rmx_status rmx_csr_upload(rmx_ctx* p, int n, const int* rowptr, const int* cols,
   c.X.alloc(n);
   HIPCHECK(hipStreamSynchronize(c.stream));
   ensure_work(c);
-  choose_spmv_lanes(c);
+  spmv_setup(c, rowptr, cols);
   setup_ilu_schedule(c);
   c.have64 = true; c.have32 = false;
   c.mat_version++;

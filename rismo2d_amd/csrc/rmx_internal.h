@@ -142,6 +142,24 @@ struct Ilu {
   bool fac64 = false, fac32 = false;
 };
 
+// FP32 SpMV layout.  Consecutive rows with one column set (the U, V and H rows of a node)
+// form a group of up to 4 rows; a group stores its columns once, ascending, as 16-bit
+// offsets from its smallest column, and the FP32 values of each of its rows in that column
+// order (row after row, so group g's values are the CSR range of its rows).  Values are
+// gathered from A32 again whenever A32 changes (a32_version).
+struct SpmvGroups {
+  bool on = false;                 // layout built (off: CSR kernel)
+  long long ng = 0, noff = 0;      // groups, stored offsets
+  int lanes = 8;                   // lanes per group (4, 8, 16, 32)
+  DBuf<int> row0;                  // [ng+1] first row of each group
+  DBuf<int4> meta;                 // [ng] first value (CSR offset of row0), first offset,
+                                   //      smallest column, columns
+  DBuf<uint16_t> off;              // [noff] column - base
+  DBuf<int> perm;                  // [nnz] CSR position of every grouped value
+  DBuf<float> vals;                // [nnz] FP32 values in group order
+  long long version = -1;          // a32_version the values were gathered at
+};
+
 // Coarse space of the two-level preconditioner: one aggregate per (strip, coarse segment,
 // variable U/V/H); the coarse matrix R A P (R = P^T, P piecewise constant on aggregates) is
 // banded in aggregate order and factored on the host.
@@ -222,6 +240,8 @@ struct Ctx {
   DBuf<float> A32;
   bool have64 = false, have32 = false;
   int spmv_g = 8;                             // SpMV lanes per row
+  SpmvGroups sg;                              // FP32 SpMV layout
+  long long a32_version = 0;                  // bumped whenever A32 changes
   long long mat_version = 0;                  // bumped whenever A64 changes
   unsigned long long topo_hash = 0;           // fingerprint of the mesh topology
   Ilu ilu;                                    // reference-order ILU(0)
@@ -284,7 +304,8 @@ void coarse_setup(Ctx& c, const std::vector<int>& rp, const std::vector<int>& cl
 void coarse_factor(Ctx& c, int prec);
 void coarse_apply(Ctx& c, int prec, const double* r, double* z);   // device pointers
 void dev_precond(Ctx& c, int prec, int kind, const double* in, double* out);
-void choose_spmv_lanes(Ctx& c);
+void spmv_setup(Ctx& c, const int* rowptr, const int* cols);   // host pattern
+double spmv_f32_bytes(const Ctx& c);          // algorithmic bytes of one FP32 SpMV
 bool mpnn_ready(Ctx& c);
 void mpnn_apply(Ctx& c, int prec, const double* r, double* z);
 void mpnn_forward(Ctx& c, const double* r, double* z, float* feat_out);

@@ -11,6 +11,7 @@ Tolerances (floating point):
   vegetation iteration, which amplifies the ulp differences of device pow / acos / cos);
 * nodal eddy viscosity and Reynolds stress of the algebraic model: 1e-13 relative;
 * SpMV, ILU apply (FP64 accumulation in another order): 1e-12 relative to max|result|;
+* FP32 SpMV on the row-group layout: 1e-13 of sum |a| |x| per row;
 * Newton-converged u, v, h: within 1e-8 relative of the reference / oracle.
 """
 import hashlib
@@ -375,6 +376,65 @@ def test_csr_kernels_against_dense(ctx):
         out = ctx.solve(method, rmx.F64, rmx.PREC_ILU0, eps=1e-13)
         assert out.converged
         np.testing.assert_allclose(ctx.solution(), np.linalg.solve(M, b), rtol=1e-9, atol=1e-10)
+
+
+def _grouped_csr(n_nodes, seed, far=None, nbr=5):
+    """CSR whose rows come in runs of 1..6 consecutive rows with one column set (the rows
+    of a node, coupled to up to `nbr` other nodes), columns in random order after the
+    diagonal; `far` adds a coupling of the first node to the last row (a column span of the
+    whole matrix)."""
+    rng = np.random.default_rng(seed)
+    sizes = 1 + np.arange(n_nodes) % 6
+    first = np.concatenate([[0], np.cumsum(sizes)])
+    n = int(first[-1])
+    rows_of = [np.arange(first[k], first[k + 1]) for k in range(n_nodes)]
+    rp, cols, vals = [0], [], []
+    for k in range(n_nodes):
+        nb = {k} | set(np.clip(k + rng.integers(-40, 41, size=nbr), 0, n_nodes - 1).tolist())
+        cset = np.concatenate([rows_of[j] for j in sorted(nb)])
+        if far and k == 0:
+            cset = np.concatenate([cset, [n - 1]])
+        for i in rows_of[k]:
+            others = rng.permutation(cset[cset != i])
+            js = np.concatenate([[i], others])
+            v = rng.standard_normal(len(js))
+            v[0] = np.abs(v[1:]).sum() + 1.0
+            cols += js.tolist()
+            vals += v.tolist()
+            rp.append(len(cols))
+    return n, np.asarray(rp, np.int32), np.asarray(cols, np.int32), np.asarray(vals)
+
+
+def _csr_mul(rp, cols, vals, x):
+    rows = np.repeat(np.arange(len(rp) - 1), np.diff(rp))
+    return np.bincount(rows, weights=vals * x[cols], minlength=len(rp) - 1)
+
+
+def test_f32_spmv_row_groups(ctx):
+    """FP32 SpMV on the row-group layout (rows sharing a column set, up to 4 per group,
+    runs of 5 and 6 split), against a float64 product of the FP32 values; again after the
+    FP32 matrix changes (the layout's values follow A32); with narrow rows (4 lanes per
+    group), and on a matrix whose column span exceeds 16-bit offsets (CSR kernel)."""
+    for n_nodes, far, nbr in ((1500, False, 5), (1500, False, 1), (22000, True, 5)):
+        n, rp, cols, vals = _grouped_csr(n_nodes, 11, far, nbr)
+        print(f"[gpu] FP32 SpMV row groups: n {n}, nnz {rp[-1]}, far coupling {far}", flush=True)
+        b = np.random.default_rng(12).standard_normal(n)
+        ctx.csr_upload(rp, cols, vals, b)
+        ctx.convert(rmx.F32)
+        for rnd in range(2):
+            A32, _ = ctx.matrix(rmx.F32)
+            a = A32.astype(np.float64)
+            ref = _csr_mul(rp, cols, a, b)
+            scale = _csr_mul(rp, cols, np.abs(a), np.abs(b))
+            y = ctx.spmv(b, rmx.F32)
+            assert (np.abs(y - ref) <= 1e-13 * scale).all(), rnd
+            ctx.scale_l2()                       # new FP64 values, then a new FP32 copy
+            ctx.convert(rmx.F32)
+        # the FP32 products inside FGMRES (restart residual and basis) on the same layout
+        ctx.ilu0_factor(rmx.F32)
+        out = ctx.solve(rmx.FGMRES, rmx.F32, rmx.PREC_ILU0, eps=1e-5, mkyrl=20, max_iter=400,
+                        refine=True, refine_tol=1e-10, refine_max=20)
+        assert out.converged and out.true_resid <= 1e-10
 
 
 def test_full_pattern_ilu_is_exact_lu(ctx):
