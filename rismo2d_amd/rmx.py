@@ -77,7 +77,7 @@ class SweepInfo(C.Structure):
 class KernelTimes(C.Structure):
     _fields_ = [("spmv_s", C.c_double), ("ilu_apply_s", C.c_double), ("ilu_fwd_s", C.c_double),
                 ("ilu_bwd_s", C.c_double), ("silu_fwd_s", C.c_double), ("silu_bwd_s", C.c_double),
-                ("n_spmv", C.c_longlong), ("n_ilu", C.c_longlong)]
+                ("n_spmv", C.c_longlong), ("n_ilu", C.c_longlong), ("spmv_bytes", C.c_double)]
 
 
 _LIB = None
