@@ -52,6 +52,10 @@ def parse():
     ap.add_argument("--refine-max", type=int, default=8)
     ap.add_argument("--plain-iters", type=int, default=40)
     ap.add_argument("--no-cpu", action="store_true")
+    ap.add_argument("--kernels-only", action="store_true",
+                    help="profiling run: build and factor the system, time the FP32 SpMV and "
+                         "strip sweeps, run a plain device copy of known size, print their "
+                         "times and exit (no Newton step, no JSON benchmark line)")
     return ap.parse_args()
 
 
@@ -95,6 +99,29 @@ def cpu_baseline():
                 n_eq=n, iters=its)
 
 
+def kernels_only(c, n, a):
+    """The system of the first Newton step (FP64 assembly, L2 scaling, FP32 copy), its strip
+    ILU(0) in FP32, five timed SpMV / sweep pairs, then three plain copies of an n-vector
+    (the identity preconditioner's device copy: 8n bytes read, 8n written), the reference
+    size for calibrating HBM byte counters."""
+    from rismo2d_amd import rmx as R
+    c.assemble(R.F64)
+    c.scale_l2()
+    c.convert(R.F32)
+    c.silu0_factor(R.F32)
+    kt = c.profile_kernels(R.F32, 5)
+    x = np.random.default_rng(0).standard_normal(n)
+    for _ in range(3):
+        c.precond_apply(R.PREC_NONE, x, R.F32)
+    lo_e, up_e = c.sweep_entries()
+    print(json.dumps({"n": n, "nnz": c.nnz, "spmv_ms": kt.spmv_s * 1e3, "spmv_bytes": kt.spmv_bytes,
+                      "sweep_fwd_ms": kt.silu_fwd_s * 1e3, "sweep_bwd_ms": kt.silu_bwd_s * 1e3,
+                      "sweep_fwd_bytes": sweep_bytes(lo_e, n, 4),
+                      "sweep_bwd_bytes": sweep_bytes(up_e, n, 4) + 4 * n,
+                      "copy_bytes_each_way": 8 * n}), flush=True)
+    c.close()
+
+
 def main():
     a = parse()
     if a.gpus != 1:
@@ -117,6 +144,9 @@ def main():
     c.set_strips(a.strips, a.segments)
     log(f"prepare: {n} equations, {c.nnz} non-zeros ({time.perf_counter() - t:.1f}s)")
 
+    if a.kernels_only:
+        kernels_only(c, n, a)
+        return
     pk = R.PREC_SILU0C if a.coarse > 0 else R.PREC_SILU0
     opts = dict(method=R.FGMRES, prec=R.F32, precond=pk, max_iter=a.max_iter,
                 mkyrl=a.mkyrl, eps=a.eps, refine=True, refine_tol=1e-8, refine_max=a.refine_max)

@@ -312,8 +312,8 @@ void coarse_apply(Ctx& c, int prec, const double* r, double* z)
   band_solve(C.nc, C.kl, C.ku, C.band, C.ipiv, C.h_buf);
   HIPCHECK(hipMemcpyAsync(C.zc.p, C.h_buf, sizeof(double) * C.nc, hipMemcpyHostToDevice, c.stream));
   k_ap_resid<<<g, 256, 0, c.stream>>>(n, C.ap_ptr.p, C.ap_agg.p, C.apv.p, C.zc.p, r, C.t.p);
-  ilu_apply(c, c.silu, prec, C.t.p, z, nullptr);
-  k_add_prolong<<<g, 256, 0, c.stream>>>(n, C.agg.p, C.zc.p, z);
+  if (!ilu_apply(c, c.silu, prec, C.t.p, z, nullptr, C.agg.p, C.zc.p))
+    k_add_prolong<<<g, 256, 0, c.stream>>>(n, C.agg.p, C.zc.p, z);
   HIPCHECK(hipGetLastError());
 }
 

@@ -472,6 +472,8 @@ __global__ void __launch_bounds__(1024) k_sweep(const int* __restrict__ blev,
 // issues it.  Results stay in the LDS ring and are flushed to the global copy (and the
 // scatter target) at the full barriers: every FENCE positions, around wide levels and at
 // the end.  A wide level (more than WIDE rows) reads and writes the global copy directly.
+// With aidx, the scatter adds aval[aidx[e]] to the value written to odst[e] (the coarse
+// prolongation P zc of the two-level preconditioner, folded into the write).
 template <class VT, bool LOWER, int G, int PWN>
 __global__ void __launch_bounds__(512) k_sweep_stage(const int* __restrict__ blev,
                                                      const int* __restrict__ bpos,
@@ -484,7 +486,9 @@ __global__ void __launch_bounds__(512) k_sweep_stage(const int* __restrict__ ble
                                                      const double* __restrict__ gsrc,
                                                      const int* __restrict__ oidx_all,
                                                      double* __restrict__ odst,
-                                                     double* __restrict__ xp_all)
+                                                     double* __restrict__ xp_all,
+                                                     const int* __restrict__ aidx,
+                                                     const double* __restrict__ aval)
 {
   __shared__ double ring[RING];
   __shared__ int s_lp[LPC + 2];
@@ -498,6 +502,10 @@ __global__ void __launch_bounds__(512) k_sweep_stage(const int* __restrict__ ble
   const int* gidx = gidx_all + base;
   const int* oidx = oidx_all ? oidx_all + base : nullptr;
   double* xp = xp_all + base;
+  auto scatter = [&](int p, double v) {     // odst in equation order (+ the folded term)
+    const int e = oidx[p];
+    odst[e] = aidx ? v + aval[aidx[e]] : v;
+  };
   const int t = threadIdx.x, nt = blockDim.x;
   const int grp = t / G, lane = t % G, ngr = nt / G;
   int lbase = 0;
@@ -619,7 +627,7 @@ __global__ void __launch_bounds__(512) k_sweep_stage(const int* __restrict__ ble
         ring[(P + grp) & (RING - 1)] = xi;
         if (wide) {
           xp[P + grp] = xi;
-          if (oidx) odst[oidx[P + grp]] = xi;
+          if (oidx) scatter(P + grp, xi);
         }
       }
     }
@@ -642,7 +650,7 @@ __global__ void __launch_bounds__(512) k_sweep_stage(const int* __restrict__ ble
         ring[(P + g) & (RING - 1)] = xi;
         if (wide) {
           xp[P + g] = xi;
-          if (oidx) odst[oidx[P + g]] = xi;
+          if (oidx) scatter(P + g, xi);
         }
       }
     }
@@ -654,7 +662,7 @@ __global__ void __launch_bounds__(512) k_sweep_stage(const int* __restrict__ ble
         for (int p = flushed + t; p < P + nr; p += nt) {
           const double v = ring[p & (RING - 1)];
           xp[p] = v;
-          if (oidx) odst[oidx[p]] = v;
+          if (oidx) scatter(p, v);
         }
       }
       flushed = P + nr;
@@ -671,7 +679,7 @@ __global__ void __launch_bounds__(512) k_sweep_stage(const int* __restrict__ ble
     for (int p = flushed + t; p < P0 + nr0; p += nt) {
       const double v = ring[p & (RING - 1)];
       xp[p] = v;
-      if (oidx) odst[oidx[p]] = v;
+      if (oidx) scatter(p, v);
     }
 }
 
@@ -987,7 +995,8 @@ void ilu_factor(Ctx& c, Ilu& I, int prec)
 }
 
 template <class VT, int V>
-static void launch_sweeps(Ctx& c, Ilu& I, const double* b, double* x, hipEvent_t* ev)
+static void launch_sweeps(Ctx& c, Ilu& I, const double* b, double* x, hipEvent_t* ev,
+                          const int* aidx, const double* aval)
 {
   // V = 1: staged pipeline, 4 lanes per row with 6 register entries each, 512 threads (two
   // workgroups per compute unit by registers and LDS); V = 0: one-deep prefetch, 8 lanes
@@ -998,7 +1007,7 @@ static void launch_sweeps(Ctx& c, Ilu& I, const double* b, double* x, hipEvent_t
   if constexpr (V == 1)
     k_sweep_stage<VT, true, 4, 12><<<nb, nt, 0, c.stream>>>(I.lo.blev.p, I.lo.bpos.p,
         I.lo.lev_ptr.p, I.lo.prs.p, I.lo.pcol.p, PLbuf<VT>(I).p, nullptr, I.lo.prow.p, b, nullptr,
-        nullptr, I.lo.xp.p);
+        nullptr, I.lo.xp.p, nullptr, nullptr);
   else
     k_sweep<VT, true, SG, SKR><<<nb, 1024, 0, c.stream>>>(I.lo.blev.p, I.lo.bpos.p,
         I.lo.lev_ptr.p, I.lo.prs.p, I.lo.pcol.p, PLbuf<VT>(I).p, nullptr, I.lo.prow.p, b, nullptr,
@@ -1007,7 +1016,7 @@ static void launch_sweeps(Ctx& c, Ilu& I, const double* b, double* x, hipEvent_t
   if constexpr (V == 1)
     k_sweep_stage<VT, false, 4, 6><<<nb, nt, 0, c.stream>>>(I.up.blev.p, I.up.bpos.p,
         I.up.lev_ptr.p, I.up.prs.p, I.up.pcol.p, PUbuf<VT>(I).p, PDbuf<VT>(I).p, I.fwd2bwd.p,
-        I.lo.xp.p, I.up.prow.p, x, I.up.xp.p);
+        I.lo.xp.p, I.up.prow.p, x, I.up.xp.p, aidx, aval);
   else
     k_sweep<VT, false, SG, SKR><<<nb, 1024, 0, c.stream>>>(I.up.blev.p, I.up.bpos.p,
         I.up.lev_ptr.p, I.up.prs.p, I.up.pcol.p, PUbuf<VT>(I).p, PDbuf<VT>(I).p, I.fwd2bwd.p,
@@ -1024,21 +1033,23 @@ static int sweep_variant()
 }
 
 template <class VT>
-static void apply_t(Ctx& c, Ilu& I, const double* b, double* x, hipEvent_t* ev)
+static bool apply_t(Ctx& c, Ilu& I, const double* b, double* x, hipEvent_t* ev, const int* aidx,
+                    const double* aval)
 {
-  if (sweep_variant() == 1) launch_sweeps<VT, 1>(c, I, b, x, ev);
-  else launch_sweeps<VT, 0>(c, I, b, x, ev);
+  if (sweep_variant() == 1) { launch_sweeps<VT, 1>(c, I, b, x, ev, aidx, aval); return aidx != nullptr; }
+  launch_sweeps<VT, 0>(c, I, b, x, ev, nullptr, nullptr);
+  return false;
 }
 
-void ilu_apply(Ctx& c, Ilu& I, int prec, const double* b, double* x, hipEvent_t* ev)
+bool ilu_apply(Ctx& c, Ilu& I, int prec, const double* b, double* x, hipEvent_t* ev,
+               const int* aidx, const double* aval)
 {
   if (prec == RMX_F32) {
     if (!I.fac32) fail(RMX_ERR_ARG, "ILU(0) FP32 not factored");
-    apply_t<float>(c, I, b, x, ev);
-  } else {
-    if (!I.fac64) fail(RMX_ERR_ARG, "ILU(0) FP64 not factored");
-    apply_t<double>(c, I, b, x, ev);
+    return apply_t<float>(c, I, b, x, ev, aidx, aval);
   }
+  if (!I.fac64) fail(RMX_ERR_ARG, "ILU(0) FP64 not factored");
+  return apply_t<double>(c, I, b, x, ev, aidx, aval);
 }
 
 // ---- the context's two instances ----

@@ -84,20 +84,18 @@ __global__ void k_axpy(long long n, double a, const double* __restrict__ x, doub
     y[i] += a * x[i];
 }
 
-__global__ void k_scal(long long n, double a, double* __restrict__ x)
+// x *= 1 / sqrt(sc[slot]) (scale from the device), or x *= a when sc is null; with x32 the
+// scaled vector is also stored in FP32 (the Gram-Schmidt copy of the basis)
+__global__ void k_scal_dev(long long n, const double* __restrict__ sc, int slot, double a,
+                           double* __restrict__ x, float* __restrict__ x32)
 {
+  const double t = sc ? 1.0 / sqrt(sc[slot]) : a;
   for (long long i = (long long)blockIdx.x * blockDim.x + threadIdx.x; i < n;
-       i += (long long)gridDim.x * blockDim.x)
-    x[i] *= a;
-}
-
-__global__ void k_scal_dev(long long n, const double* __restrict__ sc, int slot, double* __restrict__ x)
-{
-  const double nr = sqrt(sc[slot]);
-  const double t = 1.0 / nr;
-  for (long long i = (long long)blockIdx.x * blockDim.x + threadIdx.x; i < n;
-       i += (long long)gridDim.x * blockDim.x)
-    x[i] *= t;
+       i += (long long)gridDim.x * blockDim.x) {
+    const double v = x[i] * t;
+    x[i] = v;
+    if (x32) x32[i] = (float)v;
+  }
 }
 
 // y -= sum_j h[j] * V[j]   (classical Gram-Schmidt update, h on the device)
@@ -165,6 +163,82 @@ __global__ void __launch_bounds__(256) k_mdot(long long n, int K, const double* 
       if (threadIdx.x == 0) partial[j * MDOT_BLOCKS + blockIdx.x] = s;
     }
   }
+}
+
+// FP32 basis (leading dimension ld a multiple of 4, zero padding past n): four entries per
+// thread with 16-byte loads.  k_mdot32: partial[j*MDOT_BLOCKS + block] = <V[j], y>;
+// k_gs_update32: y -= sum_j h[j] V[j] and partial[block] = the block's part of <y, y> (the
+// norm of the updated vector in the same pass).  Both run on exactly MDOT_BLOCKS workgroups.
+__global__ void __launch_bounds__(256) k_mdot32(long long n, int K, const float* __restrict__ Vb,
+                                                long long ld, const double* __restrict__ y,
+                                                double* __restrict__ partial)
+{
+  __shared__ double sh[4];
+  double acc[MDOT_MAX];
+#pragma unroll
+  for (int j = 0; j < MDOT_MAX; j++) acc[j] = 0.0;
+  const long long nq = (n + 3) / 4;
+  for (long long q = (long long)blockIdx.x * blockDim.x + threadIdx.x; q < nq;
+       q += (long long)gridDim.x * blockDim.x) {
+    const long long i = 4 * q;
+    const double y0 = y[i];
+    const double y1 = i + 1 < n ? y[i + 1] : 0.0;
+    const double y2 = i + 2 < n ? y[i + 2] : 0.0;
+    const double y3 = i + 3 < n ? y[i + 3] : 0.0;
+#pragma unroll
+    for (int j = 0; j < MDOT_MAX; j++)
+      if (j < K) {
+        const float4 v = *reinterpret_cast<const float4*>(Vb + j * ld + i);
+        acc[j] += (double)v.x * y0 + (double)v.y * y1 + (double)v.z * y2 + (double)v.w * y3;
+      }
+  }
+#pragma unroll
+  for (int j = 0; j < MDOT_MAX; j++) {
+    if (j < K) {
+      const double s = bsum(acc[j], sh);
+      if (threadIdx.x == 0) partial[j * MDOT_BLOCKS + blockIdx.x] = s;
+    }
+  }
+}
+
+__global__ void __launch_bounds__(256) k_gs_update32(long long n, int K, const double* __restrict__ h,
+                                                     const float* __restrict__ Vb, long long ld,
+                                                     double* __restrict__ y,
+                                                     double* __restrict__ partial)
+{
+  __shared__ double sh[4];
+  double hr[MDOT_MAX];
+#pragma unroll
+  for (int j = 0; j < MDOT_MAX; j++) hr[j] = j < K ? -h[j] : 0.0;
+  double ss = 0.0;
+  const long long nq = (n + 3) / 4;
+  for (long long q = (long long)blockIdx.x * blockDim.x + threadIdx.x; q < nq;
+       q += (long long)gridDim.x * blockDim.x) {
+    const long long i = 4 * q;
+    double v0 = y[i];
+    double v1 = i + 1 < n ? y[i + 1] : 0.0;
+    double v2 = i + 2 < n ? y[i + 2] : 0.0;
+    double v3 = i + 3 < n ? y[i + 3] : 0.0;
+#pragma unroll
+    for (int j = 0; j < MDOT_MAX; j++)
+      if (j < K) {
+        const float4 w = *reinterpret_cast<const float4*>(Vb + j * ld + i);
+        v0 += hr[j] * (double)w.x;
+        v1 += hr[j] * (double)w.y;
+        v2 += hr[j] * (double)w.z;
+        v3 += hr[j] * (double)w.w;
+      }
+    if (i + 1 >= n) v1 = 0.0;         // past the end: padding of V, not part of y
+    if (i + 2 >= n) v2 = 0.0;
+    if (i + 3 >= n) v3 = 0.0;
+    y[i] = v0;
+    if (i + 1 < n) y[i + 1] = v1;
+    if (i + 2 < n) y[i + 2] = v2;
+    if (i + 3 < n) y[i + 3] = v3;
+    ss += v0 * v0 + v1 * v1 + v2 * v2 + v3 * v3;
+  }
+  const double s = bsum(ss, sh);
+  if (threadIdx.x == 0) partial[blockIdx.x] = s;
 }
 
 // out[k] = sum of the MDOT_BLOCKS partials of dot k; one workgroup per dot
@@ -292,7 +366,9 @@ static bool bicgstab(Ctx& c, int prec, int pk, const double* b, double* x, int m
   return conv;
 }
 
-// Flexible GMRES(m) on A(prec) x = b, x0 = 0.
+// Flexible GMRES(m) on A(prec) x = b, x0 = 0.  In an FP32 solve the Gram-Schmidt dots and
+// updates read an FP32 copy of the basis (written by the normalisation, half the bytes);
+// the preconditioner input, the new vector and the solution update stay FP64.
 static bool fgmres(Ctx& c, int prec, int pk, const double* b, double* x, int m, int maxIter,
                    double eps, int& iters, double& accuracy)
 {
@@ -300,9 +376,17 @@ static bool fgmres(Ctx& c, int prec, int pk, const double* b, double* x, int m, 
   if (c.basis_m != m || c.V_basis.n != (size_t)(m + 1) * n) {
     c.V_basis.alloc((size_t)(m + 1) * n);
     c.Z_basis.alloc((size_t)m * n);
+    c.V32_basis.release();
     c.basis_m = m;
   }
+  const bool b32 = prec == RMX_F32;
+  const long long ld32 = (n + 63) / 64 * 64;      // FP32 rows padded for 16-byte loads
+  if (b32 && c.V32_basis.n != (size_t)(m + 1) * ld32) {
+    c.V32_basis.alloc((size_t)(m + 1) * ld32);
+    HIPCHECK(hipMemsetAsync(c.V32_basis.p, 0, sizeof(float) * c.V32_basis.n, c.stream));
+  }
   double* Vb = c.V_basis.p;
+  float* V32 = b32 ? c.V32_basis.p : nullptr;
   double* Zb = c.Z_basis.p;
   double* sc = c.red.p;
   double* partial = c.red.p + 64;
@@ -326,7 +410,7 @@ static bool fgmres(Ctx& c, int prec, int pk, const double* b, double* x, int m, 
     }
     ro = std::sqrt(fetch(c, sc + S_B2));
     if (std::fabs(ro) <= EPSILON) { conv = true; break; }
-    k_scal<<<vb, 256, 0, c.stream>>>(n, 1.0 / ro, Vb);
+    k_scal_dev<<<vb, 256, 0, c.stream>>>(n, nullptr, 0, 1.0 / ro, Vb, V32);
     if (iters == 0) { resid = ro; eps1 = eps * ro; }
     rs[0] = ro;
     int i = -1;
@@ -340,10 +424,15 @@ static bool fgmres(Ctx& c, int prec, int pk, const double* b, double* x, int m, 
       precond(c, prec, pk, vi, zi);
       dev_spmv(c, prec, zi, vi1);
       // classical Gram-Schmidt: all dots, then all updates
-      k_mdot<<<MDOT_BLOCKS, 256, 0, c.stream>>>(n, i1, Vb, n, vi1, partial);
-      k_mfinish<<<i1, 256, 0, c.stream>>>(partial, sc + 16);
-      k_gs_update<<<vb, 256, 0, c.stream>>>(n, i1, sc + 16, Vb, n, vi1);
-      {
+      if (b32) {
+        k_mdot32<<<MDOT_BLOCKS, 256, 0, c.stream>>>(n, i1, V32, ld32, vi1, partial);
+        k_mfinish<<<i1, 256, 0, c.stream>>>(partial, sc + 16);
+        k_gs_update32<<<MDOT_BLOCKS, 256, 0, c.stream>>>(n, i1, sc + 16, V32, ld32, vi1, partial);
+        k_mfinish<<<1, 256, 0, c.stream>>>(partial, sc + S_B2);
+      } else {
+        k_mdot<<<MDOT_BLOCKS, 256, 0, c.stream>>>(n, i1, Vb, n, vi1, partial);
+        k_mfinish<<<i1, 256, 0, c.stream>>>(partial, sc + 16);
+        k_gs_update<<<vb, 256, 0, c.stream>>>(n, i1, sc + 16, Vb, n, vi1);
         const double* xs[1] = { vi1 };
         dev_dots(c, 1, xs, xs, sc + S_B2);
       }
@@ -352,7 +441,10 @@ static bool fgmres(Ctx& c, int prec, int pk, const double* b, double* x, int m, 
       for (int j = 0; j <= i; j++) hh[i][j] = c.h_red[16 + j];
       double tn = std::sqrt(c.h_red[S_B2]);
       hh[i][i1] = tn;
-      if (std::fabs(tn) > EPSILON) k_scal_dev<<<vb, 256, 0, c.stream>>>(n, sc, S_B2, vi1);
+      if (std::fabs(tn) > EPSILON)
+        k_scal_dev<<<vb, 256, 0, c.stream>>>(n, sc, S_B2, 1.0, vi1, V32 ? V32 + i1 * ld32 : nullptr);
+      else if (V32)      // no scaling (breakdown): the FP32 copy of the vector as it is
+        k_scal_dev<<<vb, 256, 0, c.stream>>>(n, nullptr, 0, 1.0, vi1, V32 + i1 * ld32);
       for (int k = 1; k <= i; k++) {
         const int k1 = k - 1;
         const double tt = hh[i][k1];

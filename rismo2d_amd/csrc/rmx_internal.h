@@ -257,6 +257,7 @@ struct Ctx {
   double* h_red = nullptr;                    // pinned host mirror of red
   // FGMRES basis
   DBuf<double> V_basis, Z_basis;
+  DBuf<float> V32_basis;                      // FP32 copy for the Gram-Schmidt of FP32 solves
   int basis_m = 0;
   // kernel timing of the last profiled solve
   rmx_kernel_times kt{};
@@ -295,7 +296,10 @@ void dev_dots(Ctx& c, int K, const double* const* x, const double* const* y, dou
 void setup_ilu_schedule(Ctx& c);
 Ilu& ilu_of(Ctx& c, int kind);
 void ilu_factor(Ctx& c, Ilu& I, int prec);
-void ilu_apply(Ctx& c, Ilu& I, int prec, const double* b, double* x, hipEvent_t* ev);
+// x = M^-1 b; with aidx, x[e] += aval[aidx[e]] folded into the sweep's output when the
+// staged sweep runs (returns whether it was folded in)
+bool ilu_apply(Ctx& c, Ilu& I, int prec, const double* b, double* x, hipEvent_t* ev,
+               const int* aidx = nullptr, const double* aval = nullptr);
 void upload_shapes(hipStream_t s);
 std::vector<int> cut_segments(const std::vector<int>& bstart, const std::vector<int>& group,
                               const HostModel& hm, int nseg);
