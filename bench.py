@@ -191,13 +191,15 @@ def main():
                 "spmv_f32": (kt.spmv_s, sp_bytes)}[dom]
     achieved = byt / dur
     # HBM bytes per launch of the dominant kernel from the committed counter summary of the
-    # same workload (rocprofv3 --pmc FETCH_SIZE / WRITE_SIZE, calibrated on a copy kernel)
+    # same workload (rocprofv3 --pmc FETCH_SIZE / WRITE_SIZE, calibrated on a device copy of
+    # known size), used when the system and the kernel's algorithmic bytes match
     traffic = None
-    tf = os.path.join(HERE, "profiles", "meander2M_spmv_f32_traffic.json")
-    if dom == "spmv_f32" and os.path.exists(tf):
+    tf = os.path.join(HERE, "profiles", "meander2M_hbm_traffic.json")
+    if os.path.exists(tf):
         rec = json.load(open(tf))
-        if rec.get("n") == n and rec.get("nnz") == c.nnz and rec.get("spmv_bytes") == sp_bytes:
-            traffic = rec["traffic_bytes"]
+        k = rec.get("kernels", {}).get(dom)
+        if k and rec.get("n") == n and rec.get("nnz") == c.nnz and k.get("algorithmic_bytes") == byt:
+            traffic = k["traffic_bytes"]
     log(f"kernels: spmv {kt.spmv_s * 1e3:.3f} ms ({sp_bytes / kt.spmv_s / 1e9:.0f} GB/s), strip "
         f"sweeps {kt.silu_fwd_s * 1e3:.3f} / {kt.silu_bwd_s * 1e3:.3f} ms")
 
