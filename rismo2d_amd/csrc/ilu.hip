@@ -141,7 +141,11 @@ __global__ void __launch_bounds__(64) k_ilu_level(const int4* __restrict__ desc,
                                                   const long long* __restrict__ pbase,
                                                   const unsigned char* __restrict__ plan_t,
                                                   const unsigned char* __restrict__ plan_pos,
-                                                  VT* __restrict__ LU, int* __restrict__ err)
+                                                  VT* __restrict__ LU, int* __restrict__ err,
+                                                  const int4* __restrict__ desc_next,
+                                                  int cnt_next,
+                                                  const long long* __restrict__ pbase_next,
+                                                  long long plan_n)
 {
 #pragma clang fp contract(off)
   __shared__ VT s_val[FW];
@@ -156,6 +160,13 @@ __global__ void __launch_bounds__(64) k_ilu_level(const int4* __restrict__ desc,
   if (w >= cnt) return;                 // uniform per block
   const int4 d = desc[w];
   const int i = d.x, p0 = d.y, wi = d.z, nk = d.w;
+  // the next level's row at this index: its descriptor now, its static data (and initial
+  // values, which no other row writes) touched after this row's own loads are issued, so
+  // the next launch finds them in the memory-side cache
+  const bool pre = w < cnt_next;
+  int4 dn = make_int4(0, 0, 0, 0);
+  long long pbn = 0;
+  if (pre) { dn = desc_next[w]; pbn = pbase_next[w]; }
   for (int t = lane; t < wi; t += 64) {
     s_val[t] = LU[p0 + t];
     s_col[t] = srt_cols[p0 + t];
@@ -223,6 +234,15 @@ __global__ void __launch_bounds__(64) k_ilu_level(const int4* __restrict__ desc,
       }
     }
   }
+  unsigned int sink = 0;
+  if (pre) {
+    for (int t = lane; t < dn.z; t += 64)
+      sink += (unsigned int)srt_cols[dn.y + t] + (unsigned int)srt_pos[dn.y + t] +
+              (unsigned int)(LU[dn.y + t] != (VT)0);
+    for (int t = lane; t < dn.w; t += 64) sink += (unsigned int)piv[dn.y + t].x;
+    for (long long u = pbn + 128LL * lane; u < pbn + UCAP && u < plan_n; u += 128LL * 64)
+      sink += (unsigned int)plan_t[u] + (unsigned int)plan_pos[u];
+  }
   __syncthreads();
   if (staged) {
     // pivot t's diagonal position and value and this lane's (at most two) entries of its
@@ -275,6 +295,7 @@ __global__ void __launch_bounds__(64) k_ilu_level(const int4* __restrict__ desc,
   __syncthreads();
   for (int t = lane; t < wi; t += 64) LU[p0 + t] = s_val[t];
   if (lane == 0 && fabs((double)s_val[0]) < 1.0e-20) atomicOr(err, 1);
+  if (sink == 0x9E3779B9u && cnt < 0) err[1] = (int)sink;   // keeps the touches (never taken)
 }
 
 template <class VT>
@@ -957,11 +978,16 @@ static void factor_t(Ctx& c, Ilu& I, const VT* A)
   HIPCHECK(hipMemcpyAsync(LU.p, A, sizeof(VT) * I.nnz, hipMemcpyDeviceToDevice, c.stream));
   int* err = reinterpret_cast<int*>(c.red.p + 8);
   HIPCHECK(hipMemsetAsync(err, 0, sizeof(int), c.stream));
+  const long long plan_n = (long long)I.plan_t.n;
   for (size_t l = 0; l + 1 < I.flev.size(); l++) {
     const int a = I.flev[l], cnt = I.flev[l + 1] - a;
+    const bool last = l + 2 >= I.flev.size();
+    const int an = last ? a : I.flev[l + 1];
+    const int cntn = last ? 0 : I.flev[l + 2] - an;
     k_ilu_level<VT><<<cnt, 64, 0, c.stream>>>(I.fdesc.p + a, cnt, I.cols.p, I.srt_cols.p,
                                               I.srt_pos.p, I.fpiv.p, I.fpbase.p + a,
-                                              I.plan_t.p, I.plan_pos.p, LU.p, err);
+                                              I.plan_t.p, I.plan_pos.p, LU.p, err,
+                                              I.fdesc.p + an, cntn, I.fpbase.p + an, plan_n);
   }
   HIPCHECK(hipGetLastError());
   DBuf<VT>& pl = PLbuf<VT>(I);
