@@ -48,6 +48,8 @@ constexpr int SKR = 2;          // row-rounds per level held in registers
 constexpr int LPC = 1024;       // level offsets staged in LDS at a time
 constexpr int FENCE = 2048;     // positions between full barriers
 constexpr int WIDE = RING / 4;  // levels wider than this use the global copy only
+constexpr int STAGE_THREADS = 512;                 // staged sweep: threads per workgroup
+constexpr int STAGE_ROWS = STAGE_THREADS / 4;      // rows per round (4 lanes per row)
 
 // ---------------------------------------------------------------------------------------
 // factorisation
@@ -743,7 +745,7 @@ static void levels(int a, int e, const std::vector<int>& rp, const std::vector<i
 
 // Packed layout of one sweep over all blocks.
 static void build_sweep(SweepSet& S, const Ilu& I, bool lower, std::vector<int>& order,
-                        hipStream_t s)
+                        hipStream_t s, int cap)
 {
   const int n = I.n, nb = I.nblk;
   const std::vector<int>& rp = I.h_rp;
@@ -757,6 +759,12 @@ static void build_sweep(SweepSet& S, const Ilu& I, bool lower, std::vector<int>&
     const int a = I.bstart[b], e = I.bstart[b + 1];
     std::vector<int> ptr, rows;
     levels(a, e, rp, cl, lower, I.seg, lev, ptr, rows);
+    if (cap > 0) {                                 // levels of more than cap rows split
+      std::vector<int> cp(1, 0);
+      for (size_t l = 0; l + 1 < ptr.size(); l++)
+        for (int r = ptr[l]; r < ptr[l + 1]; r += cap) cp.push_back(std::min(r + cap, ptr[l + 1]));
+      ptr.swap(cp);
+    }
     const int nl = (int)ptr.size() - 1;
     blev[b + 1] = blev[b] + nl;
     bpos[b + 1] = bpos[b] + (e - a);
@@ -855,9 +863,11 @@ std::vector<int> cut_segments(const std::vector<int>& bstart, const std::vector<
 
 // Strips: contiguous row ranges of about equal size.  With `group` (the node of every
 // equation) a cut moves forward to the next row of another node, so the U, V, H equations
-// of one node stay in one strip.
+// of one node stay in one strip.  level_cap > 0 splits the sweeps' levels into parts of at
+// most that many rows.
 void ilu_setup(Ilu& I, int n, const std::vector<int>& rp, const std::vector<int>& cl, int nblk,
-               const std::vector<int>* group, hipStream_t s, const HostModel* hm, int nseg)
+               const std::vector<int>* group, hipStream_t s, const HostModel* hm, int nseg,
+               int level_cap)
 {
   I.ready = I.fac64 = I.fac32 = false;
   I.n = n;
@@ -942,9 +952,14 @@ void ilu_setup(Ilu& I, int n, const std::vector<int>& rp, const std::vector<int>
                                 I.fpbase.p, I.plan_t.p, I.plan_pos.p);
     HIPCHECK(hipGetLastError());
   }
+  // strip sweeps: a level of more rows than the staged sweep's row groups (512 threads, 4
+  // lanes per row) is split, so that no level needs a second round, whose entries would be
+  // loaded inside the level instead of one level ahead
+  int cap = level_cap;
+  if (const char* e = getenv("RMX_LEVEL_CAP")) cap = std::max(0, atoi(e));
   std::vector<int> olo, oup;
-  build_sweep(I.lo, I, true, olo, s);
-  build_sweep(I.up, I, false, oup, s);
+  build_sweep(I.lo, I, true, olo, s, cap);
+  build_sweep(I.up, I, false, oup, s, cap);
   // forward position (global, concatenated) of the row at each backward position
   std::vector<int> fpos(n), f2b(n);
   for (int r = 0; r < n; r++) fpos[olo[r]] = r;
@@ -1028,7 +1043,7 @@ static void launch_sweeps(Ctx& c, Ilu& I, const double* b, double* x, hipEvent_t
   // workgroups per compute unit by registers and LDS); V = 0: one-deep prefetch, 8 lanes
   // per row, 1024 threads
   const int nb = I.nblk;
-  const int nt = 512;
+  const int nt = STAGE_THREADS;
   if (ev) HIPCHECK(hipEventRecord(ev[0], c.stream));
   if constexpr (V == 1)
     k_sweep_stage<VT, true, 4, 12><<<nb, nt, 0, c.stream>>>(I.lo.blev.p, I.lo.bpos.p,
@@ -1091,7 +1106,7 @@ void setup_ilu_schedule(Ctx& c)
   }
   const std::vector<int>& rp = c.csr_only ? hrp : c.hm.rowptr;
   const std::vector<int>& cl = c.csr_only ? hcols : c.hm.cols;
-  ilu_setup(c.ilu, c.n, rp, cl, 1, nullptr, c.stream, nullptr, 1);
+  ilu_setup(c.ilu, c.n, rp, cl, 1, nullptr, c.stream, nullptr, 1, 0);
   // strip ILU: two strips per compute unit (two sweep workgroups fit one CU's LDS), at
   // least a few thousand rows each
   hipDeviceProp_t prop;
@@ -1105,7 +1120,7 @@ void setup_ilu_schedule(Ctx& c)
       if (c.hm.eqno[k] >= 0) group[c.hm.eqno[k]] = (int)(k / 3);
   }
   ilu_setup(c.silu, c.n, rp, cl, nblk, c.csr_only ? nullptr : &group, c.stream,
-            c.csr_only ? nullptr : &c.hm, c.segments);
+            c.csr_only ? nullptr : &c.hm, c.segments, STAGE_ROWS);
   c.coarse.ready = c.coarse.fac = false;
   if (!c.csr_only && c.coarse_segments > 0) coarse_setup(c, rp, cl, group);
 }
