@@ -394,3 +394,298 @@ def write_reference_deck(deck: Deck, outdir: str, solver: str = "1 6 1 0 60 1000
                 f"$SOLVER {solver}\n$SOLVER {fgmres_solver}\n"
                 f"$CONVERGENCE {convergence}\n$RELAX {relax}\n$DRYREW {dryrew}\n"
                 "$GPDEGREE 5\n")
+
+
+# ---------------------------------------------------------------------------------------
+# reading the reference's keyword input files
+
+class DeckFormatError(ValueError):
+    """An input file that the reference's readers accept but this solve path does not
+    support, or one they would reject."""
+
+
+def _data_lines(path: str, skip_first: bool = False):
+    """Lines of a reference input file the way ASCIIFILE::nextLine serves them
+    (Asciifile.cpp:84-102): lines whose first character is '*' or '#' are comments.  With
+    skip_first the first line (read before by ASCIIFILE::next, whatever it holds) is not
+    served."""
+    with open(path) as f:
+        if skip_first:
+            f.readline()
+        for line in f:
+            if line[:1] in ("*", "#"):
+                continue
+            yield line
+
+
+# corner nodes of every midside node (Shape getCornerNodes: tri 01 12 20, quad 01 12 23 30)
+_MIDSIDE_CORNERS = {6: ((3, 0, 1), (4, 1, 2), (5, 2, 0)),
+                    8: ((4, 0, 1), (5, 1, 2), (6, 2, 3), (7, 3, 0))}
+
+
+def _read_region(path: str):
+    """GRID::InputRegion (Grid.cpp:137-376): 'np ne ndata', np node lines 'name x y z',
+    ne element lines 'name mat shape nodes...' of which only tri (6 nodes) and quad (8
+    nodes) are elements; elements are indexed by name; midside bed elevations are set to
+    the mean of their corners."""
+    it = _data_lines(path)
+    head = next(it).split()
+    npn, ne = int(head[0]), int(head[1])
+    x = np.zeros(npn); y = np.zeros(npn); z = np.zeros(npn)
+    seen = np.zeros(npn, bool)
+    for _ in range(npn):
+        t = next(it).split()
+        name = int(t[0])
+        if not 1 <= name <= npn:
+            raise DeckFormatError(f"{path}: node numbers must be between 1 and {npn} ({name})")
+        x[name - 1], y[name - 1], z[name - 1] = float(t[1]), float(t[2]), float(t[3])
+        seen[name - 1] = True
+    if not seen.all():
+        raise DeckFormatError(f"{path}: {int((~seen).sum())} nodes missing")
+    elems = {}
+    for _ in range(ne):
+        t = next(it).split()
+        name, mat, shape = int(t[0]), int(t[1]), t[2]
+        if shape not in ("tri", "quad"):
+            continue                      # the region reader keeps only tri and quad
+        if not 1 <= name <= ne:
+            raise DeckFormatError(f"{path}: element numbers must be between 1 and {ne}")
+        k = 6 if shape == "tri" else 8
+        nodes = [int(v) for v in t[3:3 + k]]
+        if len(nodes) != k or min(nodes) < 1 or max(nodes) > npn:
+            raise DeckFormatError(f"{path}: element {name}: {k} node numbers in 1..{npn} expected")
+        elems[name] = (mat, k, nodes)
+    names = sorted(elems)
+    el = np.full((len(names), 8), -1, np.int32)
+    nnd = np.zeros(len(names), np.int32)
+    mat = np.zeros(len(names), np.int64)
+    for e, name in enumerate(names):
+        m, k, nodes = elems[name]
+        el[e, :k] = np.asarray(nodes) - 1
+        nnd[e] = k
+        mat[e] = m
+    zor = z.copy()
+    for e in range(len(names)):
+        for i, a, b in _MIDSIDE_CORNERS[int(nnd[e])]:
+            z[el[e, i]] = 0.5 * (zor[el[e, a]] + zor[el[e, b]])
+    return x, y, z, el, nnd, mat
+
+
+def _read_control(path: str):
+    """GRID::InputControl (Grid.cpp:378-508): 'np ne', np node lines (skipped), ne element
+    lines 'name line_no shape n0 n1 n2' of which only shape 'line' counts; segments are
+    kept per line number in file order."""
+    it = _data_lines(path)
+    head = next(it).split()
+    npc, nec = int(head[0]), int(head[1])
+    for _ in range(npc):
+        next(it)
+    lines: dict = {}
+    for _ in range(nec):
+        t = next(it).split()
+        if t[2] != "line":
+            continue
+        lines.setdefault(int(t[1]), []).append([int(t[3]) - 1, int(t[4]) - 1, int(t[5]) - 1])
+    return {k: np.asarray(v, np.int32) for k, v in lines.items()}
+
+
+def _read_initial(path: str, z: np.ndarray):
+    """GRID::InputInitial, ASCII (Grid.cpp:510-760): a header line '#time x release vars...'
+    (release >= 40000: the variables of the columns by name), 'np ne', then per node 'name
+    values...'.  Unset values: U = V = 0, S = bed elevation."""
+    with open(path) as f:
+        header = f.readline()
+    toks = header[1:].replace(",", " ").split()
+    release = int(toks[2]) if len(toks) > 2 else 0
+    names = toks[3:] if release >= 40000 else []
+    it = _data_lines(path, skip_first=True)
+    t = next(it).split()
+    npi = int(t[0])
+    if npi != len(z):
+        raise DeckFormatError(f"{path}: wrong number of nodes in file ({npi}, region {len(z)})")
+    U = np.zeros(len(z)); V = np.zeros(len(z)); S = z.copy()
+    for _ in range(npi):
+        t = next(it).split()
+        name = int(t[0])
+        vals = {"U": 0.0, "V": 0.0, "S": z[name - 1]}
+        if release >= 40000:
+            for var, tok in zip(names, t[1:]):
+                vals[var] = float(tok)
+        elif release >= 280:
+            vals["U"], vals["V"], vals["S"] = (float(v) for v in t[1:4])
+        else:
+            vals["U"], vals["V"], vals["S"] = float(t[1]), float(t[2]), float(t[4])
+        U[name - 1], V[name - 1], S[name - 1] = vals["U"], vals["V"], vals["S"]
+    return U, V, S
+
+
+def _keyword_lines(path: str):
+    """Keyword lines of a $RISMO2D file: (release, [(keyword, tokens)]), leading blanks
+    removed, comments skipped."""
+    with open(path) as f:
+        first = f.readline().split()
+    release = int(first[1]) if len(first) > 1 and first[0] == "$RISMO2D" else 0
+    out = []
+    for n, line in enumerate(_data_lines(path)):
+        s = line.lstrip(" ")
+        if n == 0 and s.startswith("$RISMO2D"):
+            continue
+        if s.startswith("$"):
+            t = s[1:].split()
+            if t:
+                out.append((t[0], t[1:]))
+    return release, out
+
+
+_MANNING = 5                                   # TYPE rtype kMANN (Type.h:120-123)
+
+
+def _read_materials(path: str):
+    """TYPE::Import_40000 (Type.cpp:434-700): per material no, $ROUGHNESS (law, coefficient,
+    vegetation stem diameter / spacing in the release's layout), $VEGETATION and
+    $TURBULENCE (vt st estx esty KDest lm betaSf).  A material starts from the TYPE
+    defaults (vt 1e-4, st 1.0, KDest 0.6; Type.cpp:55-74); lines with no <= 0 set defaults
+    after the materials exist and so change none of them."""
+    release, kws = _keyword_lines(path)
+    if release < 40000:
+        raise DeckFormatError(f"{path}: material file release {release} (40000 or later supported)")
+    mats: dict = {}
+
+    def m(no):
+        return mats.setdefault(no, dict(rtype=3, rcoef=0.0, dp=0.0, sp=0.0, vt=1.0e-4, st=1.0,
+                                        kdest=0.6))
+    laws = {"COWI": 2, "NIKU": 3, "CHEZ": 4, "MANN": 5}
+    for key, t in kws:
+        if key == "ROUGHNESS" and int(t[0]) > 0:
+            r = m(int(t[0]))
+            r["rtype"], r["rcoef"] = int(t[1]), float(t[2])
+            if release >= 40075:
+                r["dp"], r["sp"], dp1 = float(t[10]), float(t[11]), float(t[13])
+            elif release >= 40070:
+                r["dp"], r["sp"], dp1 = float(t[7]), float(t[8]), float(t[10])
+            elif release >= 40060:
+                r["dp"], r["sp"], dp1 = float(t[5]), float(t[6]), float(t[8])
+            else:
+                r["dp"], r["sp"], dp1 = float(t[5]), float(t[6]), 0.0
+            if dp1 > 0.0:
+                raise DeckFormatError(f"{path}: material {t[0]}: a second vegetation layer "
+                                      "is not supported")
+        elif key == "BOTTOM" and int(t[0]) > 0:
+            r = m(int(t[0]))
+            r["rtype"], r["rcoef"] = laws.get(t[1], 0), float(t[2])
+        elif key == "VEGETATION" and int(t[0]) > 0:
+            r = m(int(t[0]))
+            r["dp"], r["sp"] = float(t[1]), float(t[2])
+            if len(t) > 4 and float(t[4]) > 0.0:
+                raise DeckFormatError(f"{path}: material {t[0]}: a second vegetation layer "
+                                      "is not supported")
+        elif key == "TURBULENCE" and int(t[0]) > 0:
+            r = m(int(t[0]))
+            r["vt"], r["st"], r["kdest"] = float(t[1]), float(t[2]), float(t[5])
+    return mats
+
+
+def _read_timesteps(path: str):
+    """The time-step file (TIMEINT input, Timeint.cpp:1466-2340): $TM_STEPS, $TM_INTERVAL,
+    $TM_WEIGHT, $TM_STATIONARY, $TM_TURBULENCE, $TM_MAXITER and the line conditions
+    $TM_LINE QINLET no Q S (one value, later lines overwrite), QTINLET no Q S t (hydrograph
+    rows) and OUTLET no S x y ... (Timeint.cpp:2230-2330)."""
+    release, kws = _keyword_lines(path)
+    run = dict(release=release, steps=1, dt=None, theta=1.0, stationary=1, turb=1, maxiter=None)
+    cond: dict = {}
+    for key, t in kws:
+        if key == "TM_STEPS":
+            run["steps"] = int(t[1]) - int(t[0]) + 1
+        elif key == "TM_INTERVAL":
+            run["dt"] = float(t[0])
+        elif key == "TM_WEIGHT":
+            run["theta"] = float(t[0])
+        elif key == "TM_STATIONARY":
+            run["stationary"] = int(t[0])
+        elif key == "TM_TURBULENCE":
+            run["turb"] = int(t[0])
+        elif key == "TM_MAXITER":
+            run["maxiter"] = int(t[0])
+        elif key == "TM_LINE" and int(t[1]) > 0:
+            kind, no = t[0], int(t[1])
+            c = cond.setdefault((kind, no), [])
+            vals = [float(v) for v in t[2:]]
+            if kind == "QINLET":
+                c[:] = [vals[:2]]
+            elif kind in ("QTINLET", "OUTLET"):
+                c.append(vals)
+            else:
+                raise DeckFormatError(f"{path}: line condition {kind} is not supported "
+                                      "(QINLET, QTINLET, OUTLET)")
+    return run, cond
+
+
+def read_reference_deck(input_file: str) -> tuple[Deck, dict]:
+    """Read a deck in the reference's keyword input format: the project file
+    (``$REGIONFILE``, ``$CONTROLFILE``, ``$ASC_INITFILE``, ``$MATERIALFILE``,
+    ``$TIMESTEPFILE``; Project.cpp:1296-1440, names relative to the project file's folder)
+    and the files it names, read as the reference's readers read them.  Returns the deck
+    (line 1 the inlet, line 2 the outlet, the solve path's convention) and the run settings
+    (steps, dt, theta, stationary flag, Newton iterations, and the project file's other
+    keyword lines as text).
+
+    Supported: Manning roughness, emergent vegetation, constant / algebraic eddy viscosity,
+    one QINLET or QTINLET line and one OUTLET line with a single water level.  Anything else
+    raises DeckFormatError."""
+    base = os.path.dirname(os.path.abspath(input_file))
+    files, other = {}, []
+    for key, t in _keyword_lines(input_file)[1]:
+        if key in ("REGIONFILE", "CONTROLFILE", "ASC_INITFILE", "ASC_INIFILE", "MATERIALFILE",
+                   "TIMESTEPFILE"):
+            files["ASC_INITFILE" if key == "ASC_INIFILE" else key] = os.path.join(base, t[0])
+        else:
+            other.append("$" + " ".join([key] + t))
+    for key in ("REGIONFILE", "CONTROLFILE", "MATERIALFILE", "TIMESTEPFILE"):
+        if key not in files:
+            raise DeckFormatError(f"{input_file}: no ${key}")
+    x, y, z, elems, nnd, mat = _read_region(files["REGIONFILE"])
+    lines_in = _read_control(files["CONTROLFILE"])
+    if "ASC_INITFILE" in files:
+        U0, V0, S0 = _read_initial(files["ASC_INITFILE"], z)
+    else:
+        U0, V0, S0 = np.zeros(len(z)), np.zeros(len(z)), z.copy()
+    mats = _read_materials(files["MATERIALFILE"])
+    run, cond = _read_timesteps(files["TIMESTEPFILE"])
+    missing = sorted(set(int(v) for v in np.unique(mat)) - set(mats))
+    if missing:
+        raise DeckFormatError(f"material(s) {missing} used by elements but not defined")
+    law = {no: r["rtype"] for no, r in mats.items() if no in set(mat.tolist())}
+    if any(v != _MANNING for v in law.values()):
+        raise DeckFormatError(f"roughness laws {sorted(set(law.values()))}: only Manning (5) "
+                              "is supported")
+
+    def per_elem(k):
+        return np.array([mats[int(v)][k] for v in mat], np.float64)
+    inlets = [(k, no) for (k, no) in cond if k in ("QINLET", "QTINLET")]
+    outlets = [(k, no) for (k, no) in cond if k == "OUTLET"]
+    if len(inlets) != 1 or len(outlets) != 1:
+        raise DeckFormatError(f"one inlet and one outlet line expected ({len(inlets)}, {len(outlets)})")
+    (ik, ino), (_, ono) = inlets[0], outlets[0]
+    if len(cond[("OUTLET", ono)]) != 1:
+        raise DeckFormatError("OUTLET: a water-level profile along the line is not supported")
+    for no in (ino, ono):
+        if no not in lines_in:
+            raise DeckFormatError(f"control line {no} has no line elements")
+    q = cond[(ik, ino)]
+    hydro = None
+    if ik == "QTINLET":
+        hydro = np.asarray([[r[2], r[0], r[1]] for r in q], np.float64)   # rows (t, Q, S)
+        Q_in, S_in = float(hydro[0, 1]), float(hydro[0, 2])
+    else:
+        Q_in, S_in = float(q[0][0]), float(q[0][1])
+    dp, sp = per_elem("dp"), per_elem("sp")
+    deck = Deck(x=x, y=y, z=z, elems=elems, nnd=nnd, manning=per_elem("rcoef"),
+                vt=per_elem("vt"), lines={INLET_LINE: lines_in[ino], OUTLET_LINE: lines_in[ono]},
+                U0=U0, V0=V0, S0=S0, Q_inlet=Q_in, S_inlet=S_in,
+                S_outlet=float(cond[("OUTLET", ono)][0][0]),
+                name=os.path.basename(base),
+                veg_dp=dp if (dp > 0).any() else None, veg_sp=sp if (dp > 0).any() else None,
+                turb=run["turb"], turb_st=per_elem("st"), turb_kdest=per_elem("kdest"),
+                hydrograph=hydro)
+    run["project"] = other
+    return deck, run
