@@ -620,6 +620,33 @@ def _read_timesteps(path: str):
     return run, cond
 
 
+def write_reference_restart(path: str, deck: Deck, U, V, S, time: str = "0.0",
+                            dUdt=None, dVdt=None, dSdt=None, release: int = 40700,
+                            exact: bool = False) -> None:
+    """Write a state as the reference's ASCII restart file (GRID::OutputData, Grid.cpp:1183-
+    1244): header '# time Release n U,V,S,dUdt,dVdt,dSdt,K,D,C,qb,Zb', 'np ne', per node
+    its name and the eleven values, per element its name and U V P dz (element means of
+    the corner values; dz 0).  The reference prints %14.6le; ``exact`` prints %.17g so that
+    the state reads back bit for bit.  The file is also an initial file ($ASC_INITFILE) for
+    the reference and for read_reference_deck."""
+    n = deck.n_nodes
+    z0 = np.zeros(n)
+    cols = [np.asarray(a, np.float64) for a in
+            (U, V, S, dUdt if dUdt is not None else z0, dVdt if dVdt is not None else z0,
+             dSdt if dSdt is not None else z0, z0, z0, z0, z0, deck.z)]
+    fmt = (lambda v: f" {v:.17g}") if exact else (lambda v: f" {v:14.6e}")
+    ncn = np.where(deck.nnd == 6, 3, 4)
+    with open(path, "w") as f:
+        f.write(f"# {time:>22s}   Release {release}   U,V,S,dUdt,dVdt,dSdt,K,D,C,qb,Zb\n")
+        f.write(f"{n}  {deck.n_elems}\n")
+        for i in range(n):
+            f.write(f"{i + 1:7d}" + "".join(fmt(c[i]) for c in cols) + "\n")
+        for e in range(deck.n_elems):
+            corners = deck.elems[e, :ncn[e]]
+            vals = (cols[0][corners].mean(), cols[1][corners].mean(), cols[2][corners].mean(), 0.0)
+            f.write(f"{e + 1:7d}" + "".join(fmt(v) for v in vals) + "\n")
+
+
 def read_reference_deck(input_file: str) -> tuple[Deck, dict]:
     """Read a deck in the reference's keyword input format: the project file
     (``$REGIONFILE``, ``$CONTROLFILE``, ``$ASC_INITFILE``, ``$MATERIALFILE``,
@@ -635,6 +662,9 @@ def read_reference_deck(input_file: str) -> tuple[Deck, dict]:
     base = os.path.dirname(os.path.abspath(input_file))
     files, other = {}, []
     for key, t in _keyword_lines(input_file)[1]:
+        if key in ("BIN_INITFILE", "BIN_INIFILE", "STA_INITFILE"):
+            raise DeckFormatError(f"{input_file}: ${key}: only ASCII initial files "
+                                  "($ASC_INITFILE) are supported")
         if key in ("REGIONFILE", "CONTROLFILE", "ASC_INITFILE", "ASC_INIFILE", "MATERIALFILE",
                    "TIMESTEPFILE"):
             files["ASC_INITFILE" if key == "ASC_INIFILE" else key] = os.path.join(base, t[0])

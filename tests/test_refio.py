@@ -215,6 +215,7 @@ def test_initial_file_fixed_columns_and_defaults(tmp_path):
       "$TM_LINE OUTLET 9 1.4 0 0 0 0.0\n$TM_LINE OUTLET 9 1.5 1 1 0 0.0\n"), "profile"),
     (("tm.dat", "$TM_LINE OUTLET 9", "$TM_LINE SLIP 9"), "not supported"),
     (("init.dat", "9 0\n", "8 0\n"), "wrong number of nodes"),
+    (("input.dat", "$ASC_INITFILE", "$BIN_INITFILE"), "only ASCII"),
 ])
 def test_reader_rejects_what_the_path_does_not_support(tmp_path, edit, msg):
     init = "# 0.0 x 40000 U V S\n9 0\n" + "".join(f"{i} 0 0 1.0\n" for i in range(1, 10))
@@ -224,3 +225,26 @@ def test_reader_rejects_what_the_path_does_not_support(tmp_path, edit, msg):
     files[name] = files[name].replace(old, new)
     with pytest.raises(dk.DeckFormatError, match=msg):
         dk.read_reference_deck(_write(tmp_path, files))
+
+
+@pytest.mark.parametrize("exact", [True, False])
+def test_restart_file_reads_back_as_initial_state(tmp_path, exact):
+    """A state written as the reference's ASCII restart (header with the comma-separated
+    column list, eleven node values, element lines) is an initial file: read back exactly
+    with %.17g, to the reference's printed 7 digits otherwise."""
+    d = dk.channel(nx=10, ny=3)
+    rng = np.random.default_rng(5)
+    U, V = rng.standard_normal(d.n_nodes), rng.standard_normal(d.n_nodes)
+    S = d.z + 1.0 + 0.1 * rng.random(d.n_nodes)
+    dk.write_reference_deck(d, str(tmp_path))
+    dk.write_reference_restart(str(tmp_path / "init.dat"), d, U, V, S, time="3600.0",
+                               exact=exact)
+    header = (tmp_path / "init.dat").read_text().splitlines()[0]
+    assert header.split()[1:4] == ["3600.0", "Release", "40700"]
+    r, _ = dk.read_reference_deck(str(tmp_path / "input.dat"))
+    if exact:
+        for a, b in ((r.U0, U), (r.V0, V), (r.S0, S)):
+            np.testing.assert_array_equal(a, b)
+    else:
+        for a, b in ((r.U0, U), (r.V0, V), (r.S0, S)):
+            np.testing.assert_allclose(a, b, rtol=5e-7, atol=0)
