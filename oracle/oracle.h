@@ -43,16 +43,20 @@ orc_model* orc_create(int np, const double* x, const double* y, const double* z,
  * nodal friction (Lindner).  NULL arrays remove it. */
 void   orc_set_vegetation(orc_model* m, const double* dp, const double* sp);
 double orc_lindner(double dp, double sp, double va, double ha, double cb, double vk, double g);
-/* Turbulence model: code as $TM_TURBULENCE (1 constant, 2 algebraic; +100 type vt is a
- * lower bound, +200 recompute at every Newton step), st and KDest per element (NULL:
- * Type.cpp defaults 1.0, 0.6), k-epsilon constants cm, cd.  Returns -1 for other codes. */
+/* Turbulence model: code as $TM_TURBULENCE (1 constant, 2 algebraic; +10 anisotropic
+ * (Elder); +100 type vt is a lower bound, +200 recompute at every Newton step), st, KDest,
+ * estx, esty per element (NULL: Type.cpp defaults 1.0, 0.6, 0, 0), k-epsilon constants
+ * cm, cd.  Returns -1 for other codes. */
 int    orc_set_turbulence(orc_model* m, int code, const double* st, const double* kdest,
-                          double cm, double cd);
-/* friction, then the algebraic model's nodal eddy viscosity vt and normal Reynolds stress
- * uu (= vv; uv = 0) of the current state (Turbulence.cpp:292-352); outputs nullable */
+                          const double* estx, const double* esty, double cm, double cd);
+/* friction, then the model's nodal eddy viscosity vt and normal Reynolds stress uu (= vv;
+ * uv = 0) of the current state (constant: Turbulence.cpp:110-165, algebraic: 292-352) and,
+ * anisotropic, the nodal Elder tensor (996-1090); outputs nullable */
 void   orc_turbulence(orc_model* m, double* vt, double* uu);
 /* the nodal vt, uu the last prepare or orc_turbulence left (no recomputation) */
 void   orc_get_turbulence(const orc_model* m, double* vt, double* uu);
+/* the nodal Elder tensor exx, exy, eyy (global axes) the last prepare left */
+void   orc_get_elder(const orc_model* m, double* exx, double* exy, double* eyy);
 void   orc_destroy(orc_model* m);
 /* relaxMethod (0|2), dry method (0|2|3 marsh, else clip), dryLimit, maxDeltaUV, maxDeltaS,
  * convUV, convS, maxUs */

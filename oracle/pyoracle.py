@@ -39,7 +39,9 @@ def lib():
                                  C.c_int, _i, C.c_double, C.c_double, C.c_double]
         L.orc_destroy.argtypes = [C.c_void_p]
         L.orc_set_vegetation.argtypes = [C.c_void_p, _d, _d]
-        L.orc_set_turbulence.argtypes = [C.c_void_p, C.c_int, _d, _d, C.c_double, C.c_double]
+        L.orc_set_turbulence.argtypes = [C.c_void_p, C.c_int, _d, _d, _d, _d, C.c_double,
+                                         C.c_double]
+        L.orc_get_elder.argtypes = [C.c_void_p, _d, _d, _d]
         L.orc_set_turbulence.restype = C.c_int
         L.orc_turbulence.argtypes = [C.c_void_p, _d, _d]
         L.orc_get_turbulence.argtypes = [C.c_void_p, _d, _d]
@@ -124,18 +126,21 @@ class OracleModel:
             k["vsp"] = np.ascontiguousarray(deck.veg_sp, np.float64)
             L.orc_set_vegetation(self.h, _p(k["vdp"], _d), _p(k["vsp"], _d))
         self.turb = int(getattr(deck, "turb", 1))
-        for key, name in (("tst", "turb_st"), ("tkd", "turb_kdest")):
+        for key, name in (("tst", "turb_st"), ("tkd", "turb_kdest"), ("tex", "turb_estx"),
+                          ("tey", "turb_esty")):
             a = getattr(deck, name, None)
             k[key] = None if a is None else np.ascontiguousarray(a, np.float64)
         self.set_turbulence(self.turb)
         self.set_state(deck.U0, deck.V0, deck.S0)
 
     def set_turbulence(self, code, cm=0.09, cd=1.0):
-        """Turbulence model code ($TM_TURBULENCE) with the deck's st, KDest."""
+        """Turbulence model code ($TM_TURBULENCE) with the deck's st, KDest, estx, esty."""
         k = self._keep
         st = None if k["tst"] is None else _p(k["tst"], _d)
         kd = None if k["tkd"] is None else _p(k["tkd"], _d)
-        if lib().orc_set_turbulence(self.h, code, st, kd, cm, cd) != 0:
+        ex = None if k["tex"] is None else _p(k["tex"], _d)
+        ey = None if k["tey"] is None else _p(k["tey"], _d)
+        if lib().orc_set_turbulence(self.h, code, st, kd, ex, ey, cm, cd) != 0:
             raise ValueError(f"turbulence model {code} not supported")
 
     def turbulence(self):
@@ -167,6 +172,12 @@ class OracleModel:
         out = {k: np.zeros(self.np) for k in self.TIME_KEYS}
         lib().orc_get_time_state(self.h, *[_p(out[k], _d) for k in self.TIME_KEYS])
         return out
+
+    def elder_state(self):
+        """The nodal Elder tensor exx, exy, eyy in use (as the last prepare left it)."""
+        out = [np.zeros(self.np) for _ in range(3)]
+        lib().orc_get_elder(self.h, *[_p(a, _d) for a in out])
+        return tuple(out)
 
     def turbulence_state(self):
         """The nodal vt, uu in use (as the last prepare left them)."""

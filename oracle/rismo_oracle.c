@@ -205,6 +205,9 @@ struct orc_model {
   int turb, turb_stale;
   double cm, cd;
   double *ste_e, *vtn, *uun;
+  /* anisotropic (Elder) model: estx / KDest, esty / KDest per element; nodal exx, exy, eyy
+   * in global axes (the local tensor turned into the flow direction) */
+  double *exi_e, *eyi_e, *exx, *exy, *eyy;
   /* time integration: state of the previous time step (NODE::vo) and the theta scheme */
   double *Uo, *Vo, *So, *dUdto, *dVdto, *dSdto;
   int unsteady;
@@ -348,6 +351,9 @@ orc_model* orc_create(int np, const double* x, const double* y, const double* z,
   m->ste_e = xcalloc((size_t)ne, sizeof(double));
   m->vtn = xcalloc((size_t)np, sizeof(double));
   m->uun = xcalloc((size_t)np, sizeof(double));
+  m->exi_e = xcalloc((size_t)ne, sizeof(double)); m->eyi_e = xcalloc((size_t)ne, sizeof(double));
+  m->exx = xcalloc((size_t)np, sizeof(double)); m->exy = xcalloc((size_t)np, sizeof(double));
+  m->eyy = xcalloc((size_t)np, sizeof(double));
   m->Uo = xcalloc((size_t)np, sizeof(double)); m->Vo = xcalloc((size_t)np, sizeof(double));
   m->So = xcalloc((size_t)np, sizeof(double)); m->dUdto = xcalloc((size_t)np, sizeof(double));
   m->dVdto = xcalloc((size_t)np, sizeof(double)); m->dSdto = xcalloc((size_t)np, sizeof(double));
@@ -363,7 +369,8 @@ void orc_destroy(orc_model* m)
                  m->nx, m->ny, m->niox, m->nioy, m->valU, m->valS, m->elem, m->nnd, m->manning,
                  m->vt, m->areaFact, m->in_segs, m->out_segs, m->noel, m->flag, m->kind, m->rot,
                  m->eqno, m->bnd, m->bflag, m->me, m->rowptr, m->cols, m->veg_dp, m->veg_sp,
-                 m->ste_e, m->vtn, m->uun, m->Uo, m->Vo, m->So, m->dUdto, m->dVdto, m->dSdto };
+                 m->ste_e, m->vtn, m->uun, m->Uo, m->Vo, m->So, m->dUdto, m->dVdto, m->dSdto,
+                 m->exi_e, m->eyi_e, m->exx, m->exy, m->eyy };
   for (size_t i = 0; i < sizeof(ps) / sizeof(ps[0]); i++) free(ps[i]);
   free(m);
 }
@@ -694,23 +701,84 @@ void orc_set_vegetation(orc_model* m, const double* dp, const double* sp)
 
 /* Turbulence model of the flow cycle.  code is the $TM_TURBULENCE value as Timeint.cpp:
  * 1333-1360 decodes it: last digit 1 = constant eddy viscosity of the element type
- * (kVtConstant), 2 = algebraic model (kVtAlgebraic); hundreds digit bit 1 = the type's
- * vt is a lower bound (kVtMin), bit 2 = the nodal quantities are recomputed at every
- * Newton step (kVtIterat) instead of only at the first one (EqsUVS2D.cpp:191, 311).
- * st, kdest: per element ($TURBULENCE no vt st estx esty KDest, Type.cpp:681-683); cm, cd: KDCONST.
+ * (kVtConstant), 2 = algebraic model (kVtAlgebraic); tens digit 1 = anisotropic eddy
+ * viscosity after Elder (kVtAnisotrop, the flow cycle then runs EQS_UVS2D_AI, Compute.cpp:
+ * 451-458); hundreds digit bit 1 = the type's vt is a lower bound (kVtMin), bit 2 = the
+ * nodal quantities are recomputed at every Newton step (kVtIterat) instead of only at the
+ * first one (EqsUVS2D.cpp:191, 311).
+ * st, kdest, estx, esty: per element ($TURBULENCE no vt st estx esty KDest, Type.cpp:
+ * 681-683; null: the TYPE defaults 1.0, 0.6, 0, 0); cm, cd: KDCONST.
  * Returns -1 for a code outside these. */
 int orc_set_turbulence(orc_model* m, int code, const double* st, const double* kdest,
-                       double cm, double cd)
+                       const double* estx, const double* esty, double cm, double cd)
 {
   const int model = code % 10, aniso = (code / 10) % 10, hund = (code / 100) % 10;
-  if (code < 0 || (model != 1 && model != 2) || aniso != 0 || hund > 3 || code >= 1000)
+  if (code < 0 || (model != 1 && model != 2) || aniso > 1 || hund > 3 || code >= 1000)
     return -1;
   m->turb = code;
-  for (int e = 0; e < m->ne; e++)
-    m->ste_e[e] = (st ? st[e] : 1.0) * (kdest ? kdest[e] : 0.6);   /* Type.cpp:73-74 */
+  for (int e = 0; e < m->ne; e++) {
+    const double kd = kdest ? kdest[e] : 0.6;
+    m->ste_e[e] = (st ? st[e] : 1.0) * kd;                         /* Type.cpp:73-74 */
+    m->exi_e[e] = (estx ? estx[e] : 0.0) / kd;                     /* Turbulence.cpp:1024-1025 */
+    m->eyi_e[e] = (esty ? esty[e] : 0.0) / kd;
+  }
   m->cm = cm; m->cd = cd;
   m->turb_stale = 1;
   return 0;
+}
+
+/* GRID::Turbulence, constant branch (Turbulence.cpp:110-165): nodal vt = the mean of the
+ * element types' vt over the incident elements (every element is wet here), uu = 0.  The
+ * isotropic flow cycle does not use it (its elements take the type's vt); the anisotropic
+ * one interpolates it. */
+static void turbulence_constant(orc_model* m)
+{
+  int* cnt = xcalloc((size_t)m->np, sizeof(int));
+  for (int n = 0; n < m->np; n++) { m->vtn[n] = 0.0; m->uun[n] = 0.0; }
+  for (int e = 0; e < m->ne; e++)
+    for (int i = 0; i < m->nnd[e]; i++) {
+      const int no = m->elem[8 * e + i];
+      m->vtn[no] += m->vt[e];
+      cnt[no]++;
+    }
+  for (int n = 0; n < m->np; n++)
+    if (cnt[n]) m->vtn[n] /= cnt[n];
+  free(cnt);
+}
+
+/* GRID::Turbulence, anisotropic part (Turbulence.cpp:996-1090): per node the mean of
+ * estx / KDest and esty / KDest over the incident elements, turned into the direction of
+ * the nodal velocity (angle to the x axis; none below |U| = 1e-6):
+ * exx = ex ca^2 + ey sa^2, exy = (ex - ey) sa ca, eyy = ex sa^2 + ey ca^2. */
+static void turbulence_elder(orc_model* m)
+{
+  int* cnt = xcalloc((size_t)m->np, sizeof(int));
+  for (int n = 0; n < m->np; n++) { m->exx[n] = 0.0; m->eyy[n] = 0.0; }
+  for (int e = 0; e < m->ne; e++)
+    for (int i = 0; i < m->nnd[e]; i++) {
+      const int no = m->elem[8 * e + i];
+      m->exx[no] += m->exi_e[e];
+      m->eyy[no] += m->eyi_e[e];
+      cnt[no]++;
+    }
+  for (int n = 0; n < m->np; n++) {
+    if (cnt[n]) { m->exx[n] /= cnt[n]; m->eyy[n] /= cnt[n]; }
+    const double U = m->U[n], V = m->V[n], Us = sqrt(U * U + V * V);
+    double sa = 0.0, ca = 1.0;
+    if (Us > 1.0e-6) { sa = V / Us; ca = U / Us; }
+    const double ex = m->exx[n], ey = m->eyy[n];
+    m->exx[n] = ex * ca * ca + ey * sa * sa;
+    m->exy[n] = (ex - ey) * sa * ca;
+    m->eyy[n] = ex * sa * sa + ey * ca * ca;
+  }
+  free(cnt);
+}
+
+void orc_get_elder(const orc_model* m, double* exx, double* exy, double* eyy)
+{
+  if (exx) memcpy(exx, m->exx, sizeof(double) * m->np);
+  if (exy) memcpy(exy, m->exy, sizeof(double) * m->np);
+  if (eyy) memcpy(eyy, m->eyy, sizeof(double) * m->np);
 }
 
 /* GRID::Turbulence, algebraic branch (Turbulence.cpp:292-352): per node the mean of
@@ -750,7 +818,9 @@ void orc_get_turbulence(const orc_model* m, double* vt, double* uu)
 void orc_turbulence(orc_model* m, double* vt, double* uu)
 {
   do_friction(m);
-  turbulence_algebraic(m);
+  if (m->turb % 10 == 2) turbulence_algebraic(m);
+  else turbulence_constant(m);
+  if ((m->turb / 10) % 10 == 1) turbulence_elder(m);
   m->turb_stale = 0;
   if (vt) memcpy(vt, m->vtn, sizeof(double) * m->np);
   if (uu) memcpy(uu, m->uun, sizeof(double) * m->np);
@@ -1031,8 +1101,11 @@ int orc_prepare(orc_model* m)
    * or every step with kVtIterat, the constant model needs no nodal data; InitBcon,
    * SetLocation, SetNormal, SetRotation, SetEqno on first use) */
   do_friction(m);
-  if (m->turb % 10 == 2 && (m->turb_stale || ((m->turb / 100) % 10 & 2))) {
-    turbulence_algebraic(m);
+  const int aniso = (m->turb / 10) % 10 == 1;
+  if ((m->turb % 10 == 2 || aniso) && (m->turb_stale || ((m->turb / 100) % 10 & 2))) {
+    if (m->turb % 10 == 2) turbulence_algebraic(m);
+    else turbulence_constant(m);
+    if (aniso) turbulence_elder(m);
     m->turb_stale = 0;
   }
   init_bcon(m);
@@ -1176,6 +1249,9 @@ static void coefs_region(orc_model* m, int e, double* K, double* F)
   const double gravity = m->g;
   const double vtype = m->vt[e];
   const int alg = m->turb % 10 == 2, vtmin = (m->turb / 100) % 10 & 1;
+  /* tens digit 1: EQS_UVS2D_AI::Region (CoefsUVS2D_AI.cpp:49-620), the same terms with the
+   * eddy viscosity tensor vt (exx, exy, eyy) interpolated from the nodes */
+  const int aniso = (m->turb / 10) % 10 == 1;
   memset(K, 0, sizeof(double) * MAXEQ * MAXEQ);
   memset(F, 0, sizeof(double) * MAXEQ);
   double area = 0.0;
@@ -1227,12 +1303,26 @@ static void coefs_region(orc_model* m, int e, double* K, double* F)
       if (alg) uu += n[j] * m->uun[k];
     }
     const double vv = uu;
-    double vt = 0.0;
-    if (!alg) vt = vtype;         /* constant eddy viscosity of the element type */
-    else
-      for (int j = 0; j < nnd; j++) vt += n[j] * m->vtn[nd[j]];
-    if (vtmin && vt < vtype) vt = vtype;
-    vt += m->vk;
+    double vt = 0.0, vtxx = 0.0, vtxy = 0.0, vtyy = 0.0;
+    if (aniso) {                  /* CoefsUVS2D_AI.cpp:288-310 */
+      for (int j = 0; j < nnd; j++) {
+        const int k = nd[j];
+        vtxx += n[j] * m->exx[k] * m->vtn[k];
+        vtxy += n[j] * m->exy[k] * m->vtn[k];
+        vtyy += n[j] * m->eyy[k] * m->vtn[k];
+      }
+      if (vtmin) {
+        if (vtxx < vtype) vtxx = vtype;
+        if (vtyy < vtype) vtyy = vtype;
+      }
+      vtxx += m->vk; vtxy += m->vk; vtyy += m->vk;
+    } else {
+      if (!alg) vt = vtype;       /* constant eddy viscosity of the element type */
+      else
+        for (int j = 0; j < nnd; j++) vt += n[j] * m->vtn[nd[j]];
+      if (vtmin && vt < vtype) vt = vtype;
+      vt += m->vk;
+    }
     double Ures = sqrt(U * U + V * V);
 
     /* residual ------------------------------------------------------------------------ */
@@ -1240,8 +1330,13 @@ static void coefs_region(orc_model* m, int e, double* K, double* F)
       double f, fx, fy;
       f = H * dUdt;
       f += H * (U * dUdx + V * dUdy);
-      fx = H * vt * (dUdx + dUdx);
-      fy = H * vt * (dUdy + dVdx);
+      if (aniso) {
+        fx = H * (vtxx * dUdx + vtxy * dUdy);
+        fy = H * (vtxy * dUdx + vtyy * dUdy);
+      } else {
+        fx = H * vt * (dUdx + dUdx);
+        fy = H * vt * (dUdy + dVdx);
+      }
       fx -= H * uu;
       f += H * gravity * dadx;
       fx -= H * H * gravity / 2.0;
@@ -1251,8 +1346,13 @@ static void coefs_region(orc_model* m, int e, double* K, double* F)
 
       f = H * dVdt;
       f += H * (U * dVdx + V * dVdy);
-      fx = H * vt * (dVdx + dUdy);
-      fy = H * vt * (dVdy + dVdy);
+      if (aniso) {
+        fx = H * (vtxx * dVdx + vtxy * dVdy);
+        fy = H * (vtxy * dVdx + vtyy * dVdy);
+      } else {
+        fx = H * vt * (dVdx + dUdy);
+        fy = H * vt * (dVdy + dVdy);
+      }
       fy -= H * vv;
       f += H * gravity * dady;
       fy -= H * H * gravity / 2.0;
@@ -1269,28 +1369,35 @@ static void coefs_region(orc_model* m, int e, double* K, double* F)
     {
       double t[8], tx[8], ty[8];
       double df__, df_x, df_y, dfx_, dfy_, dfxx, dfxy, dfyx, dfyy;
-      double iUres = (Ures > 1.0e-9) ? 1.0 / Ures : 0.0;
+      /* the anisotropic routine guards 1/|U| at 1e-8 (CoefsUVS2D_AI.cpp:379) */
+      double iUres = (Ures > (aniso ? 1.0e-8 : 1.0e-9)) ? 1.0 / Ures : 0.0;
 
       /* U-derivative of x-momentum */
       df__ = weight * H * m->relaxThdt_UV;
       df__ += weight * H * dUdx;
       df_x = weight * H * U;
       df_y = weight * H * V;
-      dfxx = weight * H * vt * 2.0;
-      dfyy = weight * H * vt;
+      if (aniso) {
+        dfxx = weight * H * vtxx; dfxy = weight * H * vtxy;
+        dfyx = weight * H * vtxy; dfyy = weight * H * vtyy;
+      } else {
+        dfxx = weight * H * vt * 2.0;
+        dfyy = weight * H * vt;
+        dfxy = dfyx = 0.0;
+      }
       df__ += weight * cf * (iUres * U * U + Ures);
       for (int j = 0; j < nnd; j++) {
         t[j] = df__ * n[j] + df_x * dndx[j] + df_y * dndy[j];
-        tx[j] = dfxx * dndx[j];
-        ty[j] = dfyy * dndy[j];
+        tx[j] = aniso ? dfxx * dndx[j] + dfxy * dndy[j] : dfxx * dndx[j];
+        ty[j] = aniso ? dfyx * dndx[j] + dfyy * dndy[j] : dfyy * dndy[j];
       }
       for (int j = 0; j < nnd; j++)
         for (int k = 0; k < nnd; k++)
           K[j * MAXEQ + k] += n[j] * t[k] + dndx[j] * tx[k] + dndy[j] * ty[k];
 
-      /* V-derivative of x-momentum */
+      /* V-derivative of x-momentum (no viscous coupling in the anisotropic routine) */
       df__ = weight * H * dUdy;
-      dfyx = weight * H * vt;
+      dfyx = aniso ? 0.0 : weight * H * vt;
       df__ += weight * cf * iUres * U * V;
       for (int j = 0; j < nnd; j++) { t[j] = df__ * n[j]; ty[j] = dfyx * dndx[j]; }
       for (int j = 0; j < nnd; j++)
@@ -1300,8 +1407,13 @@ static void coefs_region(orc_model* m, int e, double* K, double* F)
       /* H-derivative of x-momentum */
       df__ = weight * dUdt;
       df__ += weight * (U * dUdx + V * dUdy);
-      dfx_ = weight * vt * (dUdx + dUdx);
-      dfy_ = weight * vt * (dUdy + dVdx);
+      if (aniso) {
+        dfx_ = weight * (vtxx * dUdx + vtxy * dUdy);
+        dfy_ = weight * (vtxy * dUdx + vtyy * dUdy);
+      } else {
+        dfx_ = weight * vt * (dUdx + dUdx);
+        dfy_ = weight * vt * (dUdy + dVdx);
+      }
       dfx_ -= weight * uu;
       df__ += weight * gravity * dadx;
       dfx_ -= weight * gravity * H;
@@ -1310,9 +1422,9 @@ static void coefs_region(orc_model* m, int e, double* K, double* F)
         for (int k = 0; k < ncn; k++)
           K[j * MAXEQ + eqidS + k] += n[j] * t[k] + dndx[j] * tx[k] + dndy[j] * ty[k];
 
-      /* U-derivative of y-momentum */
+      /* U-derivative of y-momentum (no viscous coupling in the anisotropic routine) */
       df__ = weight * H * dVdx;
-      dfxy = weight * H * vt;
+      dfxy = aniso ? 0.0 : weight * H * vt;
       df__ += weight * cf * iUres * U * V;
       for (int j = 0; j < nnd; j++) { t[j] = df__ * n[j]; tx[j] = dfxy * dndy[j]; }
       for (int j = 0; j < nnd; j++)
@@ -1324,13 +1436,19 @@ static void coefs_region(orc_model* m, int e, double* K, double* F)
       df__ += weight * H * dVdy;
       df_x = weight * H * U;
       df_y = weight * H * V;
-      dfxx = weight * H * vt;
-      dfyy = weight * H * vt * 2.0;
+      if (aniso) {
+        dfxx = weight * H * vtxx; dfxy = weight * H * vtxy;
+        dfyx = weight * H * vtxy; dfyy = weight * H * vtyy;
+      } else {
+        dfxx = weight * H * vt;
+        dfyy = weight * H * vt * 2.0;
+        dfxy = dfyx = 0.0;
+      }
       df__ += weight * cf * (iUres * V * V + Ures);
       for (int j = 0; j < nnd; j++) {
         t[j] = df__ * n[j] + df_x * dndx[j] + df_y * dndy[j];
-        tx[j] = dfxx * dndx[j];
-        ty[j] = dfyy * dndy[j];
+        tx[j] = aniso ? dfxx * dndx[j] + dfxy * dndy[j] : dfxx * dndx[j];
+        ty[j] = aniso ? dfyx * dndx[j] + dfyy * dndy[j] : dfyy * dndy[j];
       }
       for (int j = 0; j < nnd; j++)
         for (int k = 0; k < nnd; k++)
@@ -1339,8 +1457,13 @@ static void coefs_region(orc_model* m, int e, double* K, double* F)
       /* H-derivative of y-momentum */
       df__ = weight * dVdt;
       df__ += weight * (U * dVdx + V * dVdy);
-      dfx_ = weight * vt * (dVdx + dUdy);
-      dfy_ = weight * vt * (dVdy + dVdy);
+      if (aniso) {
+        dfx_ = weight * (vtxx * dVdx + vtxy * dVdy);
+        dfy_ = weight * (vtxy * dVdx + vtyy * dVdy);
+      } else {
+        dfx_ = weight * vt * (dVdx + dUdy);
+        dfy_ = weight * vt * (dVdy + dVdy);
+      }
       dfy_ -= weight * vv;
       df__ += weight * gravity * dady;
       dfy_ -= weight * gravity * H;

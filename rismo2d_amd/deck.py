@@ -50,6 +50,8 @@ class Deck:
                                        # 2 algebraic; +100 vt of the element is a lower bound
     turb_st: np.ndarray | None = None  # [ne] st of the algebraic model (None: 1.0)
     turb_kdest: np.ndarray | None = None   # [ne] KDest of the algebraic model (None: 0.6)
+    turb_estx: np.ndarray | None = None    # [ne] estx / esty of the anisotropic (Elder) model,
+    turb_esty: np.ndarray | None = None    # tens digit 1 of turb (None: 0)
     hydrograph: np.ndarray | None = None   # [k, 3] inlet hydrograph rows (t s, Q, S) (QTINLET)
 
     @property
@@ -105,7 +107,7 @@ def channel(nx: int = 200, ny: int = 20, length: float = 1000.0, width: float = 
             U_init: float = 0.5, quads: bool = False, name: str | None = None,
             veg_frac: float = 0.0, veg_band: float = 0.3, veg_dp=(0.01, 0.05),
             veg_density=(0.5, 5.0), veg_seed: int = 0, turb: int = 1, turb_st=None,
-            turb_kdest=None, turb_seed: int = 0) -> Deck:
+            turb_kdest=None, turb_seed: int = 0, turb_estx=None, turb_esty=None) -> Deck:
     """Straight rectangular channel on a (2nx+1) x (2ny+1) lattice.
 
     Each cell becomes two six-node triangles split along its (i,j)->(i+2,j+2) diagonal,
@@ -117,8 +119,9 @@ def channel(nx: int = 200, ny: int = 20, length: float = 1000.0, width: float = 
     With ``veg_frac`` > 0, a seeded fraction of the elements in the outer ``veg_band`` of the
     width on either bank (the floodplain strips) carries emergent vegetation: stem diameter
     U[veg_dp] m and density U[veg_density] per m^2 (spacing = density^-1/2).
-    ``turb`` selects the turbulence model code; ``turb_st`` / ``turb_kdest`` are a value
-    or a (low, high) range sampled per element with ``turb_seed`` (None: model defaults).
+    ``turb`` selects the turbulence model code; ``turb_st`` / ``turb_kdest`` (and
+    ``turb_estx`` / ``turb_esty`` of the anisotropic model) are a value or a (low, high)
+    range sampled per element with ``turb_seed`` (None: model defaults).
     """
     NX, NY = 2 * nx + 1, 2 * ny + 1
     xs = np.linspace(0.0, length, NX)
@@ -187,6 +190,7 @@ def channel(nx: int = 200, ny: int = 20, length: float = 1000.0, width: float = 
         return rng_t.uniform(v[0], v[1], ne)
 
     t_st, t_kd = per_elem(turb_st), per_elem(turb_kdest)
+    t_ex, t_ey = per_elem(turb_estx), per_elem(turb_esty)
     return Deck(x=x.astype(np.float64), y=y.astype(np.float64), z=z.astype(np.float64),
                 elems=elems.astype(np.int32), nnd=nnd,
                 manning=np.full(ne, manning), vt=np.full(ne, vt),
@@ -194,7 +198,8 @@ def channel(nx: int = 200, ny: int = 20, length: float = 1000.0, width: float = 
                 U0=np.full(npn, U_init), V0=np.zeros(npn), S0=np.full(npn, S_out),
                 Q_inlet=Q, S_inlet=S_in, S_outlet=S_out,
                 name=name or f"channel_{nx}x{ny}{'_quad' if quads else ''}",
-                veg_dp=veg_d, veg_sp=veg_s, turb=turb, turb_st=t_st, turb_kdest=t_kd)
+                veg_dp=veg_d, veg_sp=veg_s, turb=turb, turb_st=t_st, turb_kdest=t_kd,
+                turb_estx=t_ex, turb_esty=t_ey)
 
 
 # ---------------------------------------------------------------------------------------
@@ -334,7 +339,8 @@ def write_reference_deck(deck: Deck, outdir: str, solver: str = "1 6 1 0 60 1000
                          stationary: int = 1) -> None:
     """Write the reference's keyword input files for ``deck`` into ``outdir``.
 
-    One material per distinct (Manning n, vt, stem diameter, stem spacing, st, KDest).
+    One material per distinct (Manning n, vt, stem diameter, stem spacing, st, KDest, estx,
+    esty).
     Values are written with %.17g so that the reference reads back the exact doubles used
     by the oracle.  ``tm_steps`` time steps of ``dt`` seconds with weight ``theta``;
     ``stationary`` 0 selects the unsteady flow cycle.
@@ -345,8 +351,10 @@ def write_reference_deck(deck: Deck, outdir: str, solver: str = "1 6 1 0 60 1000
     vs = deck.veg_sp if deck.veg_sp is not None else np.zeros(ne)
     st = deck.turb_st if deck.turb_st is not None else np.full(ne, 1.0)
     kd = deck.turb_kdest if deck.turb_kdest is not None else np.full(ne, 0.6)
+    ex = deck.turb_estx if deck.turb_estx is not None else np.zeros(ne)
+    ey = deck.turb_esty if deck.turb_esty is not None else np.zeros(ne)
     keys = list(zip(deck.manning.tolist(), deck.vt.tolist(), vd.tolist(), vs.tolist(),
-                    st.tolist(), kd.tolist()))
+                    st.tolist(), kd.tolist(), ex.tolist(), ey.tolist()))
     mat_of = {p: i + 1 for i, p in enumerate(sorted(set(keys)))}
     with open(os.path.join(outdir, "region.dat"), "w") as f:
         f.write(f"{deck.n_nodes} {deck.n_elems} 0\n")
@@ -370,11 +378,11 @@ def write_reference_deck(deck: Deck, outdir: str, solver: str = "1 6 1 0 60 1000
             f.write(f"{i + 1} {deck.U0[i]:.17g} {deck.V0[i]:.17g} {deck.S0[i]:.17g}\n")
     with open(os.path.join(outdir, "mat.dat"), "w") as f:
         f.write("$RISMO2D 40000\n")
-        for (n, vt, dp, sp, a, b), no in mat_of.items():
+        for (n, vt, dp, sp, a, b, cx, cy), no in mat_of.items():
             f.write(f"$ROUGHNESS {no} 5 {n:.17g} 0 0.0 {dp:.17g} {sp:.17g}\n")
-        for (n, vt, dp, sp, a, b), no in mat_of.items():
+        for (n, vt, dp, sp, a, b, cx, cy), no in mat_of.items():
             # no vt st estx esty KDest lm betaSf (TYPE::Import_40000)
-            f.write(f"$TURBULENCE {no} {vt:.17g} {a:.17g} 0.0 0.0 {b:.17g} 0.0 0.0\n")
+            f.write(f"$TURBULENCE {no} {vt:.17g} {a:.17g} {cx:.17g} {cy:.17g} {b:.17g} 0.0 0.0\n")
     if deck.hydrograph is not None:
         inlet_lines = "".join(f"$TM_LINE QTINLET {INLET_LINE} {q:.17g} {sv:.17g} {t:.17g}\n"
                               for t, q, sv in np.asarray(deck.hydrograph, np.float64))
@@ -553,7 +561,7 @@ def _read_materials(path: str):
 
     def m(no):
         return mats.setdefault(no, dict(rtype=3, rcoef=0.0, dp=0.0, sp=0.0, vt=1.0e-4, st=1.0,
-                                        kdest=0.6))
+                                        kdest=0.6, estx=0.0, esty=0.0))
     laws = {"COWI": 2, "NIKU": 3, "CHEZ": 4, "MANN": 5}
     for key, t in kws:
         if key == "ROUGHNESS" and int(t[0]) > 0:
@@ -582,6 +590,7 @@ def _read_materials(path: str):
         elif key == "TURBULENCE" and int(t[0]) > 0:
             r = m(int(t[0]))
             r["vt"], r["st"], r["kdest"] = float(t[1]), float(t[2]), float(t[5])
+            r["estx"], r["esty"] = float(t[3]), float(t[4])
     return mats
 
 
@@ -716,6 +725,7 @@ def read_reference_deck(input_file: str) -> tuple[Deck, dict]:
                 name=os.path.basename(base),
                 veg_dp=dp if (dp > 0).any() else None, veg_sp=sp if (dp > 0).any() else None,
                 turb=run["turb"], turb_st=per_elem("st"), turb_kdest=per_elem("kdest"),
+                turb_estx=per_elem("estx"), turb_esty=per_elem("esty"),
                 hydrograph=hydro)
     run["project"] = other
     return deck, run

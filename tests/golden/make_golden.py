@@ -27,6 +27,12 @@ the reference's input format, runs the reference and stores what the harness rec
 * ``hydro20x4.npz`` the 20 x 4 triangle channel, unsteady (4 steps of 30 s, theta 0.7)
                     under an inlet hydrograph (QTINLET: Q 10 -> 16 -> 8 m3/s over 120 s);
                     element matrices of Newton step 5 (second time step).
+* ``elder20x4.npz`` the 20 x 4 triangle channel with the anisotropic (Elder) eddy viscosity
+                    on the constant model, recomputed at every Newton step so that the
+                    tensor turns with the developing flow ($TM_TURBULENCE 211; estx in
+                    [0.4, 1.2], esty in [0.05, 0.3] per element, KDest 0.6), same records.
+* ``elderq10x3.npz`` the 10 x 3 quad channel, Elder on the algebraic model with the
+                    element's vt as lower bound ($TM_TURBULENCE 112).
 * ``channel200x20.npz``  the straight-channel workload (200 x 20 cells, 8,000 triangles,
                     bed noise sigma 0.05 m / 20 m, seed 0): numbering, per-step scale and
                     correction, BiCGStab iteration counts from the report, final state,
@@ -75,6 +81,11 @@ DECKS = {
                       Q=10.0, quads=True, turb=2, turb_st=(0.5, 1.5)),
     "hydro20x4": dict(nx=20, ny=4, length=100.0, width=20.0, noise_sigma=0.05, noise_corr=5.0,
                       Q=10.0),
+    "elder20x4": dict(nx=20, ny=4, length=100.0, width=20.0, noise_sigma=0.05, noise_corr=5.0,
+                      Q=10.0, turb=211, turb_estx=(0.4, 1.2), turb_esty=(0.05, 0.3)),
+    "elderq10x3": dict(nx=10, ny=3, length=100.0, width=30.0, noise_sigma=0.05, noise_corr=5.0,
+                       Q=10.0, quads=True, vt=0.04, turb=112, turb_st=(0.5, 1.5),
+                       turb_kdest=(0.4, 0.8), turb_estx=(0.4, 1.2), turb_esty=(0.05, 0.3)),
 }
 
 # inlet hydrographs (QTINLET), rows (t s, Q m3/s, S m)
@@ -116,7 +127,7 @@ def deck_arrays(d):
         out["hydrograph"] = d.hydrograph
     if d.turb != 1:
         out["turb"] = d.turb
-        for key in ("turb_st", "turb_kdest"):
+        for key in ("turb_st", "turb_kdest", "turb_estx", "turb_esty"):
             if getattr(d, key) is not None:
                 out[key] = getattr(d, key)
     return out

@@ -21,4 +21,5 @@ def golden_deck(g):
                    S_inlet=float(g["S_in"]), S_outlet=float(g["S_out"]),
                    veg_dp=g.get("veg_dp"), veg_sp=g.get("veg_sp"),
                    turb=int(g["turb"]) if "turb" in g else 1, turb_st=g.get("turb_st"),
-                   turb_kdest=g.get("turb_kdest"), hydrograph=g.get("hydrograph"))
+                   turb_kdest=g.get("turb_kdest"), hydrograph=g.get("hydrograph"),
+                   turb_estx=g.get("turb_estx"), turb_esty=g.get("turb_esty"))

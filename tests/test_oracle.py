@@ -29,7 +29,8 @@ TIME_FIELDS = ("dUdt", "dVdt", "dSdt", "Uo", "Vo", "So", "dUdto", "dVdto", "dSdt
 
 
 @pytest.mark.parametrize("name", ["tri20x4", "quad10x3", "veg20x4", "turb20x4", "turbq10x3",
-                                  "unst20x4", "unstq10x3", "hydro20x4"])
+                                  "unst20x4", "unstq10x3", "hydro20x4", "elder20x4",
+                                  "elderq10x3"])
 def test_oracle_matches_reference_every_record(name):
     g = load_golden(name)
     om = po.OracleModel(golden_deck(g))

@@ -35,7 +35,8 @@ def _same(a, b):
         np.testing.assert_array_equal(a.lines[k], b.lines[k])
     assert a.S_outlet == b.S_outlet and a.turb == b.turb
     ne = a.n_elems
-    for f, dflt in (("veg_dp", 0.0), ("veg_sp", 0.0), ("turb_st", 1.0), ("turb_kdest", 0.6)):
+    for f, dflt in (("veg_dp", 0.0), ("veg_sp", 0.0), ("turb_st", 1.0), ("turb_kdest", 0.6),
+                    ("turb_estx", 0.0), ("turb_esty", 0.0)):
         va = getattr(a, f) if getattr(a, f) is not None else np.full(ne, dflt)
         vb = getattr(b, f) if getattr(b, f) is not None else np.full(ne, dflt)
         np.testing.assert_array_equal(va, vb, err_msg=f)
@@ -51,7 +52,8 @@ def _same(a, b):
     lambda: dk.channel(nx=10, ny=3, quads=True),
     lambda: dk.channel(nx=20, ny=4, veg_frac=0.5),
     lambda: dk.channel(nx=20, ny=4, turb=102, turb_st=(0.5, 2.0), turb_kdest=(0.4, 0.8)),
-], ids=["tri", "quad", "vegetation", "algebraic-vt"])
+    lambda: dk.channel(nx=20, ny=4, turb=112, turb_estx=(0.4, 1.2), turb_esty=0.2),
+], ids=["tri", "quad", "vegetation", "algebraic-vt", "elder"])
 def test_round_trip_is_exact(make, tmp_path):
     d = make()
     r, run = _roundtrip(d, tmp_path, tm_steps=3, dt=600.0, theta=0.7, stationary=0)
@@ -68,7 +70,8 @@ def test_round_trip_hydrograph(tmp_path):
     assert (r.Q_inlet, r.S_inlet) == (80.0, 2.05)
 
 
-@pytest.mark.parametrize("name", ["tri20x4", "quad10x3", "veg20x4", "turb20x4", "hydro20x4"])
+@pytest.mark.parametrize("name", ["tri20x4", "quad10x3", "veg20x4", "turb20x4", "hydro20x4",
+                                  "elderq10x3"])
 def test_read_deck_gives_reference_numbering_and_pattern(name, tmp_path):
     """The reference program read these very files (written by write_reference_deck) to make
     the golden vectors: the deck read back must reproduce its numbering, pattern, node kinds
