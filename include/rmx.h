@@ -67,6 +67,9 @@ typedef struct {
   const double* turb_st;      /* [ne] st of the algebraic eddy viscosity model, nullable: 1.0
                                  ($TURBULENCE no vt st estx esty KDest, Type.cpp:681, defaults :73) */
   const double* turb_kdest;   /* [ne] KDest, nullable: 0.6 */
+  const double* turb_estx;    /* [ne] estx, esty of the anisotropic (Elder) model, nullable:
+                                 0 (Turbulence.cpp:1024-1025 divides both by KDest) */
+  const double* turb_esty;
 } rmx_mesh_desc;
 
 typedef struct {
@@ -75,7 +78,9 @@ typedef struct {
   double dry_limit, max_delta_uv, max_delta_s, conv_uv, conv_s, max_us;
   int turb;                   /* turbulence model, the $TM_TURBULENCE code (Timeint.cpp:1333):
                                  0 or 1 constant vt of the element, 2 algebraic model
-                                 (Turbulence.cpp:292); + 100: the element's vt is a lower
+                                 (Turbulence.cpp:292); + 10: anisotropic eddy viscosity
+                                 after Elder (Turbulence.cpp:996-1090, the element terms of
+                                 CoefsUVS2D_AI.cpp); + 100: the element's vt is a lower
                                  bound (kVtMin); + 200: nodal vt recomputed at every Newton
                                  step (kVtIterat), else at the first step after a mesh,
                                  state or parameter upload or rmx_turbulence */

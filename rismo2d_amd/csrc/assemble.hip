@@ -119,6 +119,7 @@ void upload_shapes(hipStream_t s)
 struct NodeView {
   const double *x, *y, *z, *S, *U, *V, *cf, *niox, *nioy, *valU, *valS, *rotm;
   const double *vtn, *uun;      // algebraic model: nodal eddy viscosity, uu (= vv; uv = 0)
+  const double *exx, *exy, *eyy;   // anisotropic model: nodal Elder tensor (global axes)
   const double *dUdt, *dVdt, *dSdt;   // time derivatives (zero for stationary flow)
   const uint32_t* flag;
   const int* eqno;
@@ -127,6 +128,7 @@ struct NodeView {
 struct AsmParams {
   double g, vk, hmin, relaxUV, relaxH;
   int turbAlg, turbMin;         // vt from the nodes (kVtAlgebraic); element vt as floor (kVtMin)
+  int turbAniso;                // viscosity tensor exx, exy, eyy times the nodal vt (Elder)
 };
 
 // Rows of one region element node: U_i, V_i, H_i (corner only).
@@ -218,23 +220,48 @@ __device__ __forceinline__ void region_rows(const NodeView& nv, const int* nd, i
         uu += s.f[g][k] * nv.uun[nd[k]];
       }
     }
-    if (P.turbMin && vt < vte) vt = vte;
-    vt += P.vk;
+    // anisotropic model (EQS_UVS2D_AI::Region, CoefsUVS2D_AI.cpp:288-310): the tensor
+    // sum_k n_k e_k vt_k; the floor applies to its diagonal, vk is added to all three
+    double vxx = 0.0, vxy = 0.0, vyy = 0.0;
+    if (P.turbAniso) {
+#pragma unroll
+      for (int k = 0; k < NND; k++) {
+        const double n = s.f[g][k];
+        const int q = nd[k];
+        vxx += n * nv.exx[q] * nv.vtn[q];
+        vxy += n * nv.exy[q] * nv.vtn[q];
+        vyy += n * nv.eyy[q] * nv.vtn[q];
+      }
+      if (P.turbMin) {
+        if (vxx < vte) vxx = vte;
+        if (vyy < vte) vyy = vte;
+      }
+      vxx += P.vk; vxy += P.vk; vyy += P.vk;
+    } else {
+      if (P.turbMin && vt < vte) vt = vte;
+      vt += P.vk;
+    }
     const double vv = uu;
     const double Ures = sqrt(U * U + V * V);
-    const double iUres = (Ures > 1.0e-9) ? 1.0 / Ures : 0.0;
+    const double iUres = (Ures > (P.turbAniso ? 1.0e-8 : 1.0e-9)) ? 1.0 / Ures : 0.0;
+    // anisotropic viscous fluxes per unit H: x-momentum (ax, ay), y-momentum (bx, by); the
+    // isotropic terms below keep their own expressions (and rounding)
+    const double ax = vxx * dUdx + vxy * dUdy, ay = vxy * dUdx + vyy * dUdy;
+    const double bx = vxx * dVdx + vxy * dVdy, by = vxy * dVdx + vyy * dVdy;
     const double ni = s.f[g][i], nxi = dndx[i], nyi = dndy[i];
     const double wH = wgt * H;
 
     // residual rows (force = -R)
     {
       double f = H * dUdt + H * (U * dUdx + V * dUdy) + H * grav * dadx + cf * Ures * U;
-      double fx = (H * vt * (dUdx + dUdx) - H * uu) - H * H * grav / 2.0;
-      double fy = H * vt * (dUdy + dVdx);
+      double fx = P.turbAniso ? (H * ax - H * uu) - H * H * grav / 2.0
+                              : (H * vt * (dUdx + dUdx) - H * uu) - H * H * grav / 2.0;
+      double fy = P.turbAniso ? H * ay : H * vt * (dUdy + dVdx);
       FU -= wgt * (ni * f + nxi * fx + nyi * fy);
       f = H * dVdt + H * (U * dVdx + V * dVdy) + H * grav * dady + cf * Ures * V;
-      fx = H * vt * (dVdx + dUdy);
-      fy = (H * vt * (dVdy + dVdy) - H * vv) - H * H * grav / 2.0;
+      fx = P.turbAniso ? H * bx : H * vt * (dVdx + dUdy);
+      fy = P.turbAniso ? (H * by - H * vv) - H * H * grav / 2.0
+                       : (H * vt * (dVdy + dVdy) - H * vv) - H * H * grav / 2.0;
       FV -= wgt * (ni * f + nxi * fx + nyi * fy);
       if (i < NCN) {
         const double fc = dHdt + H * (dUdx + dVdy) + U * dHdx + V * dHdy;
@@ -244,18 +271,31 @@ __device__ __forceinline__ void region_rows(const NodeView& nv, const int* nd, i
     // x-momentum row U_i
     {
       const double dUU = wH * P.relaxUV + wH * dUdx + wgt * cf * (iUres * U * U + Ures);
-      const double dxx = wH * vt * 2.0, dyy = wH * vt;
       const double dUV = wH * dUdy + wgt * cf * iUres * U * V;
-      const double dyx = wH * vt;
+      if (P.turbAniso) {
+        // d(flux)/d(U_k): x-flux cxx dndx + cxy dndy, y-flux cxy dndx + cyy dndy; no V
+        const double cxx = wH * vxx, cxy = wH * vxy, cyy = wH * vyy;
 #pragma unroll
-      for (int k = 0; k < NND; k++) {
-        const double t = dUU * s.f[g][k] + wH * U * dndx[k] + wH * V * dndy[k];
-        KU[k] += ni * t + nxi * (dxx * dndx[k]) + nyi * (dyy * dndy[k]);
-        KU[NND + k] += ni * (dUV * s.f[g][k]) + nyi * (dyx * dndx[k]);
+        for (int k = 0; k < NND; k++) {
+          const double t = dUU * s.f[g][k] + wH * U * dndx[k] + wH * V * dndy[k];
+          KU[k] += ni * t + nxi * (cxx * dndx[k] + cxy * dndy[k]) +
+                   nyi * (cxy * dndx[k] + cyy * dndy[k]);
+          KU[NND + k] += ni * (dUV * s.f[g][k]);
+        }
+      } else {
+        const double dxx = wH * vt * 2.0, dyy = wH * vt;
+        const double dyx = wH * vt;
+#pragma unroll
+        for (int k = 0; k < NND; k++) {
+          const double t = dUU * s.f[g][k] + wH * U * dndx[k] + wH * V * dndy[k];
+          KU[k] += ni * t + nxi * (dxx * dndx[k]) + nyi * (dyy * dndy[k]);
+          KU[NND + k] += ni * (dUV * s.f[g][k]) + nyi * (dyx * dndx[k]);
+        }
       }
       const double dH = wgt * dUdt + wgt * (U * dUdx + V * dUdy) + wgt * grav * dadx;
-      const double dHx = (wgt * vt * (dUdx + dUdx) - wgt * uu) - wgt * grav * H;
-      const double dHy = wgt * vt * (dUdy + dVdx);
+      const double dHx = P.turbAniso ? (wgt * ax - wgt * uu) - wgt * grav * H
+                                     : (wgt * vt * (dUdx + dUdx) - wgt * uu) - wgt * grav * H;
+      const double dHy = P.turbAniso ? wgt * ay : wgt * vt * (dUdy + dVdx);
 #pragma unroll
       for (int k = 0; k < NCN; k++) {
         const double m = s.lf[g][k];
@@ -265,18 +305,30 @@ __device__ __forceinline__ void region_rows(const NodeView& nv, const int* nd, i
     // y-momentum row V_i
     {
       const double dVU = wH * dVdx + wgt * cf * iUres * U * V;
-      const double dxy = wH * vt;
       const double dVV = wH * P.relaxUV + wH * dVdy + wgt * cf * (iUres * V * V + Ures);
-      const double dxx = wH * vt, dyy = wH * vt * 2.0;
+      if (P.turbAniso) {
+        const double cxx = wH * vxx, cxy = wH * vxy, cyy = wH * vyy;
 #pragma unroll
-      for (int k = 0; k < NND; k++) {
-        KV[k] += ni * (dVU * s.f[g][k]) + nxi * (dxy * dndy[k]);
-        const double t = dVV * s.f[g][k] + wH * U * dndx[k] + wH * V * dndy[k];
-        KV[NND + k] += ni * t + nxi * (dxx * dndx[k]) + nyi * (dyy * dndy[k]);
+        for (int k = 0; k < NND; k++) {
+          KV[k] += ni * (dVU * s.f[g][k]);
+          const double t = dVV * s.f[g][k] + wH * U * dndx[k] + wH * V * dndy[k];
+          KV[NND + k] += ni * t + nxi * (cxx * dndx[k] + cxy * dndy[k]) +
+                         nyi * (cxy * dndx[k] + cyy * dndy[k]);
+        }
+      } else {
+        const double dxy = wH * vt;
+        const double dxx = wH * vt, dyy = wH * vt * 2.0;
+#pragma unroll
+        for (int k = 0; k < NND; k++) {
+          KV[k] += ni * (dVU * s.f[g][k]) + nxi * (dxy * dndy[k]);
+          const double t = dVV * s.f[g][k] + wH * U * dndx[k] + wH * V * dndy[k];
+          KV[NND + k] += ni * t + nxi * (dxx * dndx[k]) + nyi * (dyy * dndy[k]);
+        }
       }
       const double dH = wgt * dVdt + wgt * (U * dVdx + V * dVdy) + wgt * grav * dady;
-      const double dHx = wgt * vt * (dVdx + dUdy);
-      const double dHy = (wgt * vt * (dVdy + dVdy) - wgt * vv) - wgt * grav * H;
+      const double dHx = P.turbAniso ? wgt * bx : wgt * vt * (dVdx + dUdy);
+      const double dHy = P.turbAniso ? (wgt * by - wgt * vv) - wgt * grav * H
+                                     : (wgt * vt * (dVdy + dVdy) - wgt * vv) - wgt * grav * H;
 #pragma unroll
       for (int k = 0; k < NCN; k++) {
         const double m = s.lf[g][k];
@@ -627,6 +679,45 @@ __global__ void k_turb_alg(int np, const int* __restrict__ n2e_ptr, const int* _
   uu[n] = r;
 }
 
+// ---- constant eddy viscosity at the nodes (GRID::Turbulence, kVtConstant,
+// Turbulence.cpp:110-165): the mean of the incident elements' vt (all elements wet here);
+// uu = 0.  Only the anisotropic model reads it (the isotropic elements take their own vt).
+__global__ void k_turb_const(int np, const int* __restrict__ n2e_ptr, const int* __restrict__ n2e,
+                             const double* __restrict__ vt_e, double* __restrict__ vt,
+                             double* __restrict__ uu)
+{
+  const int n = blockIdx.x * blockDim.x + threadIdx.x;
+  if (n >= np) return;
+  const int a = n2e_ptr[n], b = n2e_ptr[n + 1];
+  double v = 0.0;
+  for (int q = a; q < b; q++) v += vt_e[n2e[q]];
+  vt[n] = b > a ? v / (double)(b - a) : 0.0;
+  uu[n] = 0.0;
+}
+
+// ---- anisotropic eddy viscosity after Elder (Turbulence.cpp:996-1090): per node the mean
+// of estx / KDest and esty / KDest over the incident elements, turned into the direction
+// of the nodal velocity (angle to the x axis; none below |U| = 1e-6) ----
+__global__ void k_turb_elder(int np, const int* __restrict__ n2e_ptr, const int* __restrict__ n2e,
+                             const double* __restrict__ exi, const double* __restrict__ eyi,
+                             const double* __restrict__ U, const double* __restrict__ V,
+                             double* __restrict__ exx, double* __restrict__ exy,
+                             double* __restrict__ eyy)
+{
+  const int n = blockIdx.x * blockDim.x + threadIdx.x;
+  if (n >= np) return;
+  const int a = n2e_ptr[n], b = n2e_ptr[n + 1];
+  double ex = 0.0, ey = 0.0;
+  for (int q = a; q < b; q++) { ex += exi[n2e[q]]; ey += eyi[n2e[q]]; }
+  if (b > a) { ex /= (double)(b - a); ey /= (double)(b - a); }
+  const double u = U[n], w = V[n], Us = sqrt(u * u + w * w);
+  double sa = 0.0, ca = 1.0;
+  if (Us > 1.0e-6) { sa = w / Us; ca = u / Us; }
+  exx[n] = ex * ca * ca + ey * sa * sa;
+  exy[n] = (ex - ey) * sa * ca;
+  eyy[n] = ex * sa * sa + ey * ca * ca;
+}
+
 // ---- inlet discharge distribution (BCONLINE::Inlet, QINLET, quadratic line rule) ----
 __global__ void k_inlet(int nseg, const int* __restrict__ segs, int nin,
                         const int* __restrict__ inodes, const double* __restrict__ x,
@@ -677,6 +768,7 @@ static NodeView node_view(Ctx& c)
   v.niox = c.niox.p; v.nioy = c.nioy.p; v.valU = c.valU.p; v.valS = c.valS.p; v.rotm = c.rotm.p;
   v.flag = c.dflag.p; v.eqno = c.eqno.p;
   v.vtn = c.vtn.p; v.uun = c.uun.p;
+  v.exx = c.exx.p; v.exy = c.exy.p; v.eyy = c.eyy.p;
   v.dUdt = c.dUdt.p; v.dVdt = c.dVdt.p; v.dSdt = c.dSdt.p;
   return v;
 }
@@ -689,6 +781,7 @@ static AsmParams asm_params(Ctx& c)
   p.relaxUV = c.unsteady ? 1.0 / c.dt / c.theta : 0.0;
   p.relaxH = c.unsteady ? 1.0 / c.dt / c.theta : 0.0;
   p.turbAlg = c.par.turb_alg() ? 1 : 0; p.turbMin = c.par.turb_min() ? 1 : 0;
+  p.turbAniso = c.par.turb_aniso() ? 1 : 0;
   return p;
 }
 
@@ -704,10 +797,15 @@ static void launch_friction(Ctx& c)
 
 static void launch_turbulence(Ctx& c)
 {
-  const int np = c.hm.np;
-  k_turb_alg<<<(np + 255) / 256, 256, 0, c.stream>>>(np, c.n2e_ptr.p, c.n2e.p, c.d_ste.p, c.S.p,
-                                                     c.z.p, c.U.p, c.V.p, c.cf.p, c.par.cm,
-                                                     c.par.cd, c.vtn.p, c.uun.p);
+  const int np = c.hm.np, g = (np + 255) / 256;
+  if (c.par.turb_alg())
+    k_turb_alg<<<g, 256, 0, c.stream>>>(np, c.n2e_ptr.p, c.n2e.p, c.d_ste.p, c.S.p, c.z.p, c.U.p,
+                                        c.V.p, c.cf.p, c.par.cm, c.par.cd, c.vtn.p, c.uun.p);
+  else
+    k_turb_const<<<g, 256, 0, c.stream>>>(np, c.n2e_ptr.p, c.n2e.p, c.d_vte.p, c.vtn.p, c.uun.p);
+  if (c.par.turb_aniso())
+    k_turb_elder<<<g, 256, 0, c.stream>>>(np, c.n2e_ptr.p, c.n2e.p, c.d_exi.p, c.d_eyi.p, c.U.p,
+                                          c.V.p, c.exx.p, c.exy.p, c.eyy.p);
   HIPCHECK(hipGetLastError());
   c.turb_stale = false;
 }
@@ -724,7 +822,8 @@ void dev_prepare_step(Ctx& c)
   launch_friction(c);
   // the flow cycle evaluates the eddy viscosity at its first Newton step only, unless
   // kVtIterat asks for every step (EqsUVS2D.cpp:191, 311-315)
-  if (c.par.turb_alg() && (c.turb_stale || c.par.turb_iterat())) launch_turbulence(c);
+  if ((c.par.turb_alg() || c.par.turb_aniso()) && (c.turb_stale || c.par.turb_iterat()))
+    launch_turbulence(c);
   const int nseg = (int)(c.hm.in_segs.size() / 3), nin = (int)c.hm.inlet_nodes.size();
   if (nseg > 0) {
     k_inlet<<<1, 256, 0, c.stream>>>(nseg, c.d_in_segs.p, nin, c.d_inlet_nodes.p, c.x.p, c.y.p,

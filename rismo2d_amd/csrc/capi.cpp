@@ -109,6 +109,10 @@ rmx_status rmx_mesh_upload(rmx_ctx* p, const rmx_mesh_desc* d)
   dev_time_init(c);
   c.unsteady = false;
   c.d_ste.upload(hm.ste, s); c.vtn.alloc(np); c.uun.alloc(np);
+  c.d_vte.upload(hm.vt, s); c.d_exi.upload(hm.exi, s); c.d_eyi.upload(hm.eyi, s);
+  c.exx.alloc(np); c.exy.alloc(np); c.eyy.alloc(np);
+  for (DBuf<double>* b : {&c.exx, &c.exy, &c.eyy})
+    HIPCHECK(hipMemsetAsync(b->p, 0, sizeof(double) * np, s));
   HIPCHECK(hipMemsetAsync(c.vtn.p, 0, sizeof(double) * np, s));
   HIPCHECK(hipMemsetAsync(c.uun.p, 0, sizeof(double) * np, s));
   c.turb_stale = true;
@@ -160,9 +164,10 @@ rmx_status rmx_set_params(rmx_ctx* p, const rmx_params* q)
   {
     const int t = q->turb == 0 ? 1 : q->turb;
     const int model = t % 10, aniso = (t / 10) % 10, hund = (t / 100) % 10;
-    if (t < 0 || t >= 1000 || (model != 1 && model != 2) || aniso != 0 || hund > 3)
+    if (t < 0 || t >= 1000 || (model != 1 && model != 2) || aniso > 1 || hund > 3)
       fail(RMX_ERR_ARG, "params: turbulence model " + std::to_string(q->turb) +
-                        " not supported (1 constant, 2 algebraic, + 100 kVtMin, + 200 kVtIterat)");
+                        " not supported (1 constant, 2 algebraic, + 10 anisotropic (Elder), "
+                        "+ 100 kVtMin, + 200 kVtIterat)");
     c.par.turb = t;
   }
   c.par.cm = q->cm > 0.0 ? q->cm : 0.09;
@@ -267,7 +272,8 @@ rmx_status rmx_turbulence(rmx_ctx* p, double* vt, double* uu)
   API_BEGIN
   Ctx& c = *need(p);
   if (!c.mesh_ready) fail(RMX_ERR_ARG, "turbulence: upload a mesh first");
-  if (!c.par.turb_alg()) fail(RMX_ERR_ARG, "turbulence: params.turb selects no algebraic model");
+  if (!c.par.turb_alg() && !c.par.turb_aniso())
+    fail(RMX_ERR_ARG, "turbulence: params.turb selects no nodal eddy viscosity model");
   dev_turbulence(c);
   const int np = c.hm.np;
   if (vt) HIPCHECK(hipMemcpyAsync(vt, c.vtn.p, sizeof(double) * np, hipMemcpyDeviceToHost, c.stream));

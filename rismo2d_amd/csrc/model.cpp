@@ -58,9 +58,13 @@ void host_mesh(HostModel& hm, const rmx_mesh_desc& d)
     hm.veg_dp.assign(d.veg_dp, d.veg_dp + d.ne);
     hm.veg_sp.assign(d.veg_sp, d.veg_sp + d.ne);
   }
-  hm.ste.resize(d.ne);
-  for (int e = 0; e < d.ne; e++)               // Type.cpp:73-74 defaults st 1.0, KDest 0.6
-    hm.ste[e] = (d.turb_st ? d.turb_st[e] : 1.0) * (d.turb_kdest ? d.turb_kdest[e] : 0.6);
+  hm.ste.resize(d.ne); hm.exi.resize(d.ne); hm.eyi.resize(d.ne);
+  for (int e = 0; e < d.ne; e++) {             // Type.cpp:73-74 defaults st 1.0, KDest 0.6
+    const double kd = d.turb_kdest ? d.turb_kdest[e] : 0.6;
+    hm.ste[e] = (d.turb_st ? d.turb_st[e] : 1.0) * kd;
+    hm.exi[e] = (d.turb_estx ? d.turb_estx[e] : 0.0) / kd;   // Turbulence.cpp:1024-1025
+    hm.eyi[e] = (d.turb_esty ? d.turb_esty[e] : 0.0) / kd;
+  }
   hm.in_segs.assign(d.inlet_segs, d.inlet_segs + 3LL * d.n_inlet);
   hm.out_segs.assign(d.outlet_segs, d.outlet_segs + 3LL * d.n_outlet);
   hm.Q = d.Q; hm.S_in = d.S_in; hm.S_out = d.S_out;

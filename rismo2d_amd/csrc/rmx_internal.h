@@ -74,6 +74,7 @@ struct HostModel {
   int np = 0, ne = 0;
   std::vector<double> x, y, zor, areaFact, manning, vt, veg_dp, veg_sp;
   std::vector<double> ste;                    // [ne] st * KDest of the algebraic model
+  std::vector<double> exi, eyi;               // [ne] estx / KDest, esty / KDest (Elder)
   std::vector<int> elem, nnd;                 // [ne*8], [ne]
   std::vector<int> in_segs, out_segs;
   double Q = 0, S_in = 0, S_out = 0;
@@ -107,6 +108,7 @@ struct Params {
   double cm = 0.09, cd = 1.00;                // KDCONST
   double hmin() const { return dry_limit / 2.0; }
   bool turb_alg() const { return turb % 10 == 2; }
+  bool turb_aniso() const { return (turb / 10) % 10 == 1; }
   bool turb_min() const { return (turb / 100) % 10 & 1; }
   bool turb_iterat() const { return (turb / 100) % 10 & 2; }
 };
@@ -220,6 +222,8 @@ struct Ctx {
   DBuf<int> d_elem, d_nnd;                    // region elements [ne*8], [ne]
   DBuf<double> d_manning, d_areaFact, d_veg_dp, d_veg_sp;
   DBuf<double> d_ste, vtn, uun;               // algebraic model: st*KDest [ne]; nodal vt, uu
+  DBuf<double> d_vte, d_exi, d_eyi;           // Elder: element vt, estx/KDest, esty/KDest [ne]
+  DBuf<double> exx, exy, eyy;                 // Elder: nodal tensor in global axes [np]
   bool turb_stale = true;                     // nodal vt due at the next Newton step
   DBuf<int> d_in_segs, d_inlet_nodes;
   // ---- model elements for assembly ----

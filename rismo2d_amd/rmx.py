@@ -39,7 +39,7 @@ class MeshDesc(C.Structure):
                 ("n_inlet", C.c_int), ("inlet_segs", _i), ("n_outlet", C.c_int),
                 ("outlet_segs", _i), ("Q", C.c_double), ("S_in", C.c_double),
                 ("S_out", C.c_double), ("veg_dp", _d), ("veg_sp", _d), ("turb_st", _d),
-                ("turb_kdest", _d)]
+                ("turb_kdest", _d), ("turb_estx", _d), ("turb_esty", _d)]
 
 
 class Params(C.Structure):
@@ -194,6 +194,10 @@ class Context:
         if getattr(deck, "turb_kdest", None) is not None:
             k["tkd"] = _c(deck.turb_kdest, np.float64)
             d.turb_kdest = _p(k["tkd"], _d)
+        for key, name in (("tex", "turb_estx"), ("tey", "turb_esty")):
+            if getattr(deck, name, None) is not None:
+                k[key] = _c(getattr(deck, name), np.float64)
+                setattr(d, name, _p(k[key], _d))
         _check(lib().rmx_mesh_upload(self.h, C.byref(d)))
         self.np = deck.n_nodes
         # the deck's turbulence model becomes active with the other parameters kept

@@ -42,7 +42,8 @@ def _rowmax(rp, A):
     return m[rows], rows
 
 
-@pytest.mark.parametrize("name", ["tri20x4", "quad10x3", "veg20x4", "turb20x4", "turbq10x3"])
+@pytest.mark.parametrize("name", ["tri20x4", "quad10x3", "veg20x4", "turb20x4", "turbq10x3",
+                                  "elder20x4", "elderq10x3"])
 def test_setup_and_element_kernel_match_oracle(ctx, name):
     print(f"[gpu] setup/element parity on {name}", flush=True)
     g = load_golden(name)
@@ -140,7 +141,7 @@ def _relerr(a, b, scale):
 
 
 @pytest.mark.parametrize("name", ["tri20x4", "quad10x3", "veg20x4", "turb20x4", "turbq10x3",
-                                  "channel200x20"])
+                                  "elder20x4", "elderq10x3", "channel200x20"])
 def test_newton_converges_to_reference_solution(ctx, name):
     """Full Newton iteration on the GPU (FP64 assembly, FP64 ILU0-BiCGStab) reaches the
     reference program's converged u, v, h within 1e-8 relative (the algebraic model's
@@ -162,8 +163,10 @@ def test_newton_converges_to_reference_solution(ctx, name):
         assert st.solve.converged, (k, st.solve.accuracy)
         # BiCGStab's recursive residual reaches 1e-12; the true FP64 residual of these
         # small, ill-conditioned systems stays within 1e-9 (the gap depends on the
-        # summation order of the sweeps)
-        assert st.solve.true_resid < 1e-9
+        # summation order of the sweeps), 1e-8 on the quad deck with the anisotropic
+        # algebraic model, the worst conditioned (the reference's own first solve takes
+        # 327 BiCGStab iterations)
+        assert st.solve.true_resid < (1e-8 if name == "elderq10x3" else 1e-9)
     U, V, S, z = ctx.state()
     Hr = g["final_S"] - g["final_z"]
     us = np.abs(g["final_U"]).max()
@@ -462,9 +465,10 @@ def test_error_paths(ctx):
     with pytest.raises(rmx.RmxError) as e:
         ctx.ilu0_factor(rmx.F64)
     assert e.value.code == -3
-    # turbulence model codes outside the supported set are refused; the nodal algebraic
-    # model needs code 2 (+100 / +200)
-    for code in (3, 12, 402, -2):
+    # turbulence model codes outside the supported set are refused (tens digit 1 is the
+    # anisotropic model, 2 is no model); the nodal quantities need the algebraic or the
+    # anisotropic model (code 2 or + 10, with + 100 / + 200)
+    for code in (3, 22, 402, -2):
         with pytest.raises(rmx.RmxError):
             ctx.set_params(turb=code)
     ctx.upload_mesh(golden_deck(load_golden("tri20x4")))
