@@ -306,3 +306,42 @@ def test_hydrograph_lookup_closed_forms():
     assert po.hydrograph(h, -1.0) == (10.0, 2.0)
     d = np.array([[0.0, 5.0, 1.0], [0.0, 7.0, 1.5], [10.0, 9.0, 2.0]])
     assert po.hydrograph(d, 0.0) == (5.0, 1.0)              # zero-length interval at t
+
+
+def test_elder_tensor_invariants():
+    """The nodal Elder tensor is the incident elements' mean of (estx, esty) / KDest turned
+    into the flow direction: trace and determinant are those of the unturned diagonal, and
+    the velocity is the eigenvector of the estx eigenvalue (where |U| > 1e-6); the
+    constant model's nodal vt is the incident elements' mean vt."""
+    d = dk.channel(nx=8, ny=3, length=80.0, width=24.0, noise_sigma=0.05, noise_corr=5.0,
+                   Q=10.0, turb=11, turb_estx=(0.4, 1.2), turb_esty=(0.05, 0.3),
+                   turb_kdest=(0.5, 0.8), vt=0.05)
+    rng = np.random.default_rng(3)
+    d.V0 = 0.3 * rng.standard_normal(d.n_nodes)       # directions all around
+    d.U0 = 0.3 * rng.standard_normal(d.n_nodes)
+    d.vt = rng.uniform(0.02, 0.08, d.n_elems)
+    om = po.OracleModel(d)
+    om.set_params()
+    om.prepare()
+    exx, exy, eyy = om.elder_state()
+    vt, uu = om.turbulence_state()
+    cnt = np.zeros(d.n_nodes); ex = np.zeros(d.n_nodes); ey = np.zeros(d.n_nodes)
+    vs = np.zeros(d.n_nodes)
+    for e in range(d.n_elems):
+        nodes = d.elems[e, :d.nnd[e]]
+        cnt[nodes] += 1
+        ex[nodes] += d.turb_estx[e] / d.turb_kdest[e]
+        ey[nodes] += d.turb_esty[e] / d.turb_kdest[e]
+        vs[nodes] += d.vt[e]
+    ex /= cnt; ey /= cnt
+    np.testing.assert_allclose(vt, vs / cnt, rtol=1e-14)
+    np.testing.assert_array_equal(uu, 0.0)
+    np.testing.assert_allclose(exx + eyy, ex + ey, rtol=1e-13)
+    np.testing.assert_allclose(exx * eyy - exy * exy, ex * ey, rtol=1e-10, atol=1e-14)
+    U, V, _, _ = om.state()
+    us = np.hypot(U, V)
+    k = us > 1e-6
+    tu, tv = U[k] / us[k], V[k] / us[k]
+    # E t = ex t for the unit velocity t
+    np.testing.assert_allclose(exx[k] * tu + exy[k] * tv, ex[k] * tu, atol=1e-12)
+    np.testing.assert_allclose(exy[k] * tu + eyy[k] * tv, ex[k] * tv, atol=1e-12)
