@@ -1,0 +1,70 @@
+"""Run a deck given in the reference's input files through the solve path.
+
+The time loop is that of the reference's flow cycle (Compute.cpp:196-796 with
+EQS_UVS2D::Execute, EqsUVS2D.cpp:71-1096). For each time step of ``$TM_STEPS``:
+- unsteady (``$TM_STATIONARY 0``): Predict with dt and θ; QTINLET takes the inflow at the
+  step's end (EqsUVS2D.cpp:318-320);
+- Newton steps until the reference's convergence test holds (``nrconv``) or
+  ``$TM_MAXITER`` steps;
+- the step is committed as the old state of the next one.
+
+A stationary deck solves its flow once.
+"""
+from __future__ import annotations
+
+import numpy as np
+
+from . import deck as dk
+from . import rmx
+
+# the solve of each Newton step: FGMRES(15) on the FP32 matrix with the two-level strip
+# preconditioner inside FP64 refinement to 1e-8, or the reference's own plain path
+FAST = dict(method=rmx.FGMRES, prec=rmx.F32, precond=rmx.PREC_SILU0C, mkyrl=15, eps=1e-2,
+            refine=True, refine_tol=1e-8, refine_max=8, max_iter=1000)
+PLAIN = dict(method=rmx.BICGSTAB, prec=rmx.F64, precond=rmx.PREC_ILU0, eps=1e-12,
+             max_iter=1000)
+
+
+def run_reference_deck(input_file: str, ctx: rmx.Context | None = None, solve: dict | None = None,
+                       on_step=None):
+    """Read ``input_file`` (read_reference_deck) and run it. ``solve``: the options of each
+    Newton step (FAST by default, PLAIN the reference's BiCGStab / ILU(0) in FP64).
+    ``on_step(t, k, stats)`` is called after every Newton step. Returns (deck, run settings,
+    per time step the Newton step count, final (U, V, S, z))."""
+    d, run = dk.read_reference_deck(input_file)
+    own = ctx is None
+    c = ctx or rmx.Context(0)
+    try:
+        c.upload_mesh(d)
+        c.set_params()
+        if solve is None:
+            solve = FAST
+        if solve.get("precond") == rmx.PREC_SILU0C:
+            c.prepare()
+            c.set_coarse(8)
+        maxiter = run["maxiter"] or 12
+        unsteady = run["stationary"] == 0
+        steps = run["steps"] if unsteady else 1
+        dt = run["dt"]
+        counts = []
+        for t in range(1, steps + 1):
+            if unsteady:
+                if t > 1:
+                    c.time_commit()
+                c.time_step(dt, run["theta"], first=(t == 1))
+                if d.hydrograph is not None:
+                    c.set_inflow(*rmx.hydrograph(d.hydrograph, t * dt))
+            k = 0
+            while True:
+                st = c.newton_step(**solve)
+                k += 1
+                if on_step:
+                    on_step(t, k, st)
+                if st.nrconv or k >= maxiter:
+                    break
+            counts.append(k)
+        U, V, S, z = c.state()
+        return d, run, counts, (np.asarray(U), np.asarray(V), np.asarray(S), np.asarray(z))
+    finally:
+        if own:
+            c.close()
