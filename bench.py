@@ -75,6 +75,54 @@ def spmv_bytes(nnz, n, vbytes):
 
 
 def cpu_baseline():
+    """The reference program itself when its build (oracle/_ref/rismo_harness, compiled from
+    the reference's sources) is present, else the oracle; both on one core."""
+    exe = os.path.join(HERE, "oracle", "_ref", "rismo_harness")
+    if os.path.exists(exe):
+        try:
+            return cpu_baseline_reference(exe)
+        except Exception as e:      # noqa: BLE001 - a failed baseline falls back, never stops
+            log(f"reference baseline failed ({e}); timing the oracle instead")
+    return cpu_baseline_port()
+
+
+def _cpu_seconds(field):
+    """'h:mm:ss:cc' of the reference's report (REPORT::PrintTime) in seconds."""
+    h, m, sec, cc = (int(v) for v in field.split(":"))
+    return 3600 * h + 60 * m + sec + cc / 100.0
+
+
+def cpu_baseline_reference(exe):
+    """One Newton step of the reference program on a 60-section reach of the meander
+    generator: its own numbering, assembly, ILU(0) and BiCGStab to 1e-8 (at most 2000
+    iterations), written as its input files; the value is the CPU time its report gives for
+    the time step (the report's last CPU line)."""
+    import re
+    import subprocess
+    import tempfile
+    from rismo2d_amd import deck as dk
+    na = 60
+    d = dk.meander(n_along=na, n_across=320, reach=10000.0 * na / 3200)
+    with tempfile.TemporaryDirectory() as wd:
+        dk.write_reference_deck(d, wd, solver="1 6 1 0 60 2000 1.0e-8", maxiter=1)
+        subprocess.run([exe, "input.dat"], cwd=wd, check=True, timeout=120,
+                       stdout=subprocess.DEVNULL, stderr=subprocess.DEVNULL)
+        rep = open(os.path.join(wd, "report.txt")).read()
+    cpu = re.findall(r"\bCPU:\s+(\d+:\d+:\d+:\d+)", rep)
+    it = re.findall(r"P_BCGSTABD::Iterate\)\s+(\d+)\. Iteration \| accuracy = (\S+)", rep)
+    if not cpu or not it:
+        raise RuntimeError("no timing or iteration record in the reference's report")
+    its, acc = int(it[-1][0]), float(it[-1][1])
+    return dict(value=_cpu_seconds(cpu[-1]), unit="s per Newton step", cores=1,
+                kind="reference",
+                sample=(f"meander {na}x320 reach of the same generator ({d.n_elems} tri), one "
+                        f"Newton step of the reference program built from its sources: "
+                        f"numbering, assembly, ILU(0), BiCGStab {its} it to {acc:.1e}; CPU time "
+                        f"of the step from its report, 1 core"),
+                iters=its)
+
+
+def cpu_baseline_port():
     """The oracle (C restatement of the reference's path, one core) on a bounded sample of
     the same workload: one Newton step, FP64 assembly + ILU(0) + BiCGStab to 1e-8, on a
     100-section reach of the meander generator."""
