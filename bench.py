@@ -177,6 +177,17 @@ def main():
               "path is not built)", file=sys.stderr)
         sys.exit(2)
     from rismo2d_amd import deck as dk
+
+    # the CPU baseline first: it starts the reference program as a child process, which is
+    # done before this process loads the HIP library or touches the GPU
+    cpu = None
+    if not a.no_cpu and not a.kernels_only:
+        try:
+            cpu = cpu_baseline()
+            log(f"cpu baseline: {cpu['value']:.1f}s ({cpu['sample']})")
+        except Exception as e:       # the oracle library is test infrastructure: report its absence
+            log(f"cpu baseline unavailable: {e}")
+
     from rismo2d_amd import rmx as R
 
     t = time.perf_counter()
@@ -268,14 +279,6 @@ def main():
         f"({plain_it:.3f} s/it), residual {out.accuracy:.2e}; sweeps {kt64.ilu_fwd_s * 1e3:.1f} / "
         f"{kt64.ilu_bwd_s * 1e3:.1f} ms")
     c.close()
-
-    cpu = None
-    if not a.no_cpu:
-        try:
-            cpu = cpu_baseline()
-            log(f"cpu baseline: {cpu['value']:.1f}s ({cpu['sample']})")
-        except Exception as e:       # the oracle library is test infrastructure: report its absence
-            log(f"cpu baseline unavailable: {e}")
 
     line = {
         "metric": METRIC,
