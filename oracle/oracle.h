@@ -101,6 +101,15 @@ void   orc_get_pattern(const orc_model* m, int* rowptr, int* cols);
 int    orc_n_model_elems(const orc_model* m);
 /* model element k: returns nnd; nodes[] gets the node indices; *is_bound set */
 int    orc_model_elem(const orc_model* m, int k, int* nodes, int* is_bound);
+/* drying and rewetting, method 2 (MODEL::DoDryRewet, DoDryRew.cpp:50; GRID::DryRewet,
+ * DryRewet.cpp:696): returns the elements that got wet (-1: method not 2), *dried those
+ * that got dry */
+int    orc_dry_rewet(orc_model* m, int* dried);
+void   orc_set_dryrew(orc_model* m, double rewetLimit, int countDown);
+/* the slip flow condition on the rotated nodes (GRID::SetSlipFlow, SlipFlow.cpp:43) */
+void   orc_slip_flow(orc_model* m);
+/* per node the reference's dry (64) and marsh (256) bits; per element 1 dry, 2 marsh */
+void   orc_get_wetdry(const orc_model* m, int* nflag, int* eflag);
 /* element coefficients of model element k (CoefsUVS2D.cpp:49): estifm[24*24], force[24];
  * returns 0 for a skipped (dry) element */
 int    orc_coefs(orc_model* m, int k, double* estifm, double* force);

@@ -42,6 +42,11 @@ def lib():
         L.orc_set_turbulence.argtypes = [C.c_void_p, C.c_int, _d, _d, _d, _d, C.c_double,
                                          C.c_double]
         L.orc_get_elder.argtypes = [C.c_void_p, _d, _d, _d]
+        L.orc_dry_rewet.argtypes = [C.c_void_p, C.POINTER(C.c_int)]
+        L.orc_dry_rewet.restype = C.c_int
+        L.orc_set_dryrew.argtypes = [C.c_void_p, C.c_double, C.c_int]
+        L.orc_slip_flow.argtypes = [C.c_void_p]
+        L.orc_get_wetdry.argtypes = [C.c_void_p, _i, _i]
         L.orc_set_turbulence.restype = C.c_int
         L.orc_turbulence.argtypes = [C.c_void_p, _d, _d]
         L.orc_get_turbulence.argtypes = [C.c_void_p, _d, _d]
@@ -117,6 +122,7 @@ class OracleModel:
             out=np.ascontiguousarray(deck.lines[2], np.int32))
         k = self._keep
         self.np = deck.n_nodes
+        self.ne = deck.n_elems
         self.h = L.orc_create(self.np, _p(k["x"], _d), _p(k["y"], _d), _p(k["z"], _d),
                               deck.n_elems, _p(k["el"], _i), _p(k["nnd"], _i), _p(k["man"], _d),
                               _p(k["vt"], _d), len(k["inl"]), _p(k["inl"], _i), len(k["out"]),
@@ -172,6 +178,29 @@ class OracleModel:
         out = {k: np.zeros(self.np) for k in self.TIME_KEYS}
         lib().orc_get_time_state(self.h, *[_p(out[k], _d) for k in self.TIME_KEYS])
         return out
+
+    def set_dryrew(self, rewet_limit=0.005, count_down=5):
+        lib().orc_set_dryrew(self.h, rewet_limit, count_down)
+
+    def dry_rewet(self):
+        """One call of the drying and rewetting (method 2): (elements got dry, got wet)."""
+        d = C.c_int(0)
+        w = lib().orc_dry_rewet(self.h, C.byref(d))
+        if w < 0:
+            raise ValueError("drying and rewetting: only method 2 is restated")
+        return d.value, w
+
+    def slip_flow(self):
+        """The slip flow condition on the rotated nodes, as at the end of the flow cycle."""
+        lib().orc_slip_flow(self.h)
+
+    def wetdry(self):
+        """Per node the reference's dry (64) / marsh (256) flag bits, per element 1 dry /
+        2 marsh."""
+        nf = np.zeros(self.np, np.int32)
+        ef = np.zeros(self.ne, np.int32)
+        lib().orc_get_wetdry(self.h, _p(nf, _i), _p(ef, _i))
+        return nf, ef
 
     def elder_state(self):
         """The nodal Elder tensor exx, exy, eyy in use (as the last prepare left it)."""

@@ -27,9 +27,13 @@ enum { kInlet = 1, kOutlet = 2, kOpenBnd = 4, kQInlet = 8, kQTInlet = 16, kSlip 
 enum { rotTangent = 1, rotNormal = 2 };
 /* node flags of this restatement (own bit values) */
 enum { fCorn = 1, fMids = 2, fBound = 4, fInlet = 8, fOutlet = 16, fOpen = 32, fRotat = 64,
-       fNoMoment = 128 };
+       fNoMoment = 128, fDry = 256, fMarsh = 512 };
 /* boundary element flags */
 enum { eInlet = 1, eOutlet = 2, eOpen = 4 };
+/* region element flags */
+enum { elDry = 1, elMarsh = 2 };
+/* the reference's node flag bits for dry and marsh nodes (Node.h:100-102) */
+enum { refNodeDry = 64, refNodeMarsh = 256 };
 
 #define MAXEQ 24
 
@@ -238,6 +242,14 @@ struct orc_model {
   double g, vk, hmin, dryLimit, maxUs, maxDeltaUV, maxDeltaS, convUV, convS;
   double relaxThdt_UV, relaxThdt_H;
   int relaxMethod, dryMethod;
+  /* drying and rewetting (DRYREW, Grid.h): per node the count down and whether it was dry
+   * before the last call, per region element its flags and whether it was dry before */
+  double rewetLimit;
+  int countDown, firstDryRew;
+  int bcon_init;       /* boundary conditions generated at least once */
+  int *cdown;
+  unsigned char *nmark, *emark;
+  unsigned *eflag;
 };
 
 static void* xcalloc(size_t n, size_t s) { void* p = calloc(n ? n : 1, s); if (!p) abort(); return p; }
@@ -275,12 +287,20 @@ static void area_factors(orc_model* m)
  * element followed by its boundary elements. */
 static void topology(orc_model* m)
 {
+  /* GRID::Connection(ELEM::kDry) counts wet elements only; MODEL::Boundary clears the
+   * boundary flags and the automatic slip conditions of every node first (Bound.cpp:64-80) */
+  free(m->bnd); free(m->bflag); free(m->me);
+  for (int n = 0; n < m->np; n++) {
+    m->noel[n] = 0;
+    m->flag[n] &= ~(unsigned)fBound;
+    if (m->kind[n] & kAutoSlip) m->kind[n] &= ~(unsigned)(kAutoSlip | kFixU | kFixV);
+  }
   for (int e = 0; e < m->ne; e++) {
     int k = m->nnd[e], ncn = k == 6 ? 3 : 4;
     for (int i = 0; i < k; i++) {
       int n = m->elem[8 * e + i];
-      m->noel[n]++;
       m->flag[n] |= (i < ncn) ? fCorn : fMids;
+      if (!(m->eflag[e] & elDry)) m->noel[n]++;
     }
   }
   int nb = 0;
@@ -292,6 +312,7 @@ static void topology(orc_model* m)
   m->me = xcalloc((size_t)(m->ne + nb), sizeof(int));
   int b = 0, k = 0;
   for (int e = 0; e < m->ne; e++) {
+    if (m->eflag[e] & elDry) continue;
     int nnd = m->nnd[e], ncn = nnd == 6 ? 3 : 4;
     m->me[k++] = e;
     for (int i = ncn; i < nnd; i++) {
@@ -354,6 +375,10 @@ orc_model* orc_create(int np, const double* x, const double* y, const double* z,
   m->exi_e = xcalloc((size_t)ne, sizeof(double)); m->eyi_e = xcalloc((size_t)ne, sizeof(double));
   m->exx = xcalloc((size_t)np, sizeof(double)); m->exy = xcalloc((size_t)np, sizeof(double));
   m->eyy = xcalloc((size_t)np, sizeof(double));
+  m->cdown = xcalloc((size_t)np, sizeof(int));
+  m->nmark = xcalloc((size_t)np, 1); m->emark = xcalloc((size_t)ne, 1);
+  m->eflag = xcalloc((size_t)ne, sizeof(unsigned));
+  m->rewetLimit = 0.005; m->countDown = 5; m->firstDryRew = 1;   /* the decks' $DRYREW */
   m->Uo = xcalloc((size_t)np, sizeof(double)); m->Vo = xcalloc((size_t)np, sizeof(double));
   m->So = xcalloc((size_t)np, sizeof(double)); m->dUdto = xcalloc((size_t)np, sizeof(double));
   m->dVdto = xcalloc((size_t)np, sizeof(double)); m->dSdto = xcalloc((size_t)np, sizeof(double));
@@ -370,7 +395,8 @@ void orc_destroy(orc_model* m)
                  m->vt, m->areaFact, m->in_segs, m->out_segs, m->noel, m->flag, m->kind, m->rot,
                  m->eqno, m->bnd, m->bflag, m->me, m->rowptr, m->cols, m->veg_dp, m->veg_sp,
                  m->ste_e, m->vtn, m->uun, m->Uo, m->Vo, m->So, m->dUdto, m->dVdto, m->dSdto,
-                 m->exi_e, m->eyi_e, m->exx, m->exy, m->eyy };
+                 m->exi_e, m->eyi_e, m->exx, m->exy, m->eyy, m->cdown, m->nmark, m->emark,
+                 m->eflag };
   for (size_t i = 0; i < sizeof(ps) / sizeof(ps[0]); i++) free(ps[i]);
   free(m);
 }
@@ -856,7 +882,7 @@ static void init_bcon(orc_model* m)
       double q[3];
       for (int i = 0; i < 3; i++) {
         int n = nd[i];
-        if (m->S[n] - m->z[n] < 1.0e-9) {
+        if (m->S[n] - m->z[n] < 1.0e-9 || (m->flag[n] & (fDry | fMarsh))) {
           q[i] = 0.0;
         } else {
           double h = Sspec - m->z[n];
@@ -884,6 +910,7 @@ static void init_bcon(orc_model* m)
       double q = (h > 0.0) ? sqrt(h * h * h) : 0.0;
       double cf = m->cf[n];
       if (cf > 0.0) q /= sqrt(cf);
+      if (m->flag[n] & (fDry | fMarsh)) q = 0.0;
       bno[nbc] = n; bkind[nbc] = kind; bU[nbc] = q * Qratio; bS[nbc] = Sspec; nbc++;
     }
   }
@@ -913,6 +940,7 @@ static void init_bcon(orc_model* m)
     m->kind[bno[b]] = k; m->valU[bno[b]] = bU[b]; m->valS[bno[b]] = bS[b];
   }
   free(mark); free(bno); free(bkind); free(bU); free(bS);
+  m->bcon_init = 1;
 }
 
 /* MODEL::SetLocation (Locate.cpp:47-85) */
@@ -1031,6 +1059,7 @@ static void set_eqno(orc_model* m)
   for (int i = 0; i < 3 * np; i++) eq[i] = -1;
   int neq = 0;
   for (int n = 0; n < np; n++) {
+    if (m->flag[n] & fDry) continue;
     int ndf = (m->flag[n] & fCorn) ? 3 : 2;
     for (int j = 0; j < ndf; j++) {
       if (m->kind[n] & fix[j]) continue;
@@ -1093,6 +1122,155 @@ static void set_index_mat(orc_model* m)
     memcpy(m->cols + m->rowptr[r], index + (size_t)r * mceq, sizeof(int) * width[r]);
   m->nnz = nnz;
   free(width); free(index);
+}
+
+/* GRID::DryRewet, method 2 (DryRewet.cpp:696-900): nodes with H = S - zor below dryLimit
+ * are dry (a count down keeps them dry for countDown further calls); elements with all
+ * corners dry are dry, with some corners dry marsh; every node of a wet element is wet.
+ * Marsh nodes that were dry before (or on the first call) take the water level of their
+ * wet neighbours, clipped to zor + rewetLimit, and the bed is lowered to S - dryLimit
+ * where the level lies below zor + dryLimit; midside nodes interpolate S and z; dry
+ * nodes are reset.  Counts the elements that got dry and got wet. */
+static void dry_rewet_method2(orc_model* m, int* dried, int* wetted)
+{
+  const double dryLimit = m->dryLimit, rewetLimit = m->rewetLimit;
+  for (int n = 0; n < m->np; n++) {
+    m->nmark[n] = (m->flag[n] & fDry) != 0;
+    m->flag[n] &= ~(unsigned)fMarsh;
+    double H = m->S[n] - m->zor[n];
+    if (H < dryLimit) {
+      m->flag[n] |= fDry;
+      m->cdown[n] = m->countDown;
+    } else {
+      if (m->cdown[n] > 0) m->cdown[n]--;
+      else m->flag[n] &= ~(unsigned)fDry;
+    }
+    m->z[n] = m->zor[n];
+  }
+  for (int e = 0; e < m->ne; e++) {
+    const int* nd = m->elem + 8 * e;
+    int ncn = m->nnd[e] == 6 ? 3 : 4;
+    m->emark[e] = (m->eflag[e] & elDry) != 0;
+    m->eflag[e] &= ~(unsigned)(elDry | elMarsh);
+    int ndry = 0;
+    for (int i = 0; i < ncn; i++) if (m->flag[nd[i]] & fDry) ndry++;
+    if (ndry == ncn) {
+      m->eflag[e] |= elDry;
+    } else if (ndry) {
+      m->eflag[e] |= elMarsh;
+      for (int i = 0; i < ncn; i++) if (m->flag[nd[i]] & fDry) m->flag[nd[i]] |= fMarsh;
+    }
+  }
+  for (int n = 0; n < m->np; n++) m->flag[n] |= fDry;
+  for (int e = 0; e < m->ne; e++) {
+    if (m->eflag[e] & elDry) continue;
+    const int* nd = m->elem + 8 * e;
+    int nnd = m->nnd[e], ncn = nnd == 6 ? 3 : 4;
+    for (int i = 0; i < nnd; i++) m->flag[nd[i]] &= ~(unsigned)fDry;
+    for (int i = 0; i < ncn; i++) {
+      int n = nd[i];
+      if (!(m->flag[n] & fMarsh)) continue;
+      if (m->nmark[n] || m->firstDryRew) {
+        int j = (i + 1) % ncn, k = (i + ncn - 1) % ncn;
+        int mj = (m->flag[nd[j]] & fMarsh) != 0, mk = (m->flag[nd[k]] & fMarsh) != 0;
+        double Si;
+        if (!mj && !mk) Si = (m->S[nd[j]] + m->S[nd[k]]) / 2.0;
+        else if (!mj) Si = m->S[nd[j]];
+        else if (!mk) Si = m->S[nd[k]];
+        else Si = m->S[nd[(i + 2) % ncn]];     /* the reference reports an error if marsh */
+        m->S[n] = Si;
+        if (m->S[n] > m->zor[n] + rewetLimit) m->S[n] = m->zor[n] + rewetLimit;
+        if (m->S[n] < m->zor[n] + dryLimit) m->z[n] = m->S[n] - dryLimit;
+      } else if (m->S[n] < m->zor[n] + dryLimit) {
+        m->z[n] = m->S[n] - dryLimit;
+      }
+    }
+  }
+  for (int e = 0; e < m->ne; e++) {
+    const int* nd = m->elem + 8 * e;
+    int nnd = m->nnd[e], ncn = nnd == 6 ? 3 : 4;
+    for (int i = ncn; i < nnd; i++) {
+      int l = nd[i - ncn], r = nd[(i - ncn + 1) % ncn], n = nd[i];
+      m->S[n] = 0.5 * (m->S[l] + m->S[r]);
+      m->z[n] = 0.5 * (m->z[l] + m->z[r]);
+      if ((m->eflag[e] & elMarsh) && m->z[n] + dryLimit / 10.0 < m->zor[n]) m->flag[n] |= fMarsh;
+    }
+  }
+  for (int n = 0; n < m->np; n++) {
+    if (!(m->flag[n] & fDry)) continue;
+    m->U[n] = m->V[n] = 0.0;
+    m->S[n] = m->z[n] = m->zor[n];
+    m->dUdt[n] = m->dVdt[n] = m->dSdt[n] = 0.0;
+  }
+  int del = 0, wel = 0;
+  for (int e = 0; e < m->ne; e++) {
+    if ((m->eflag[e] & elDry) && !m->emark[e]) del++;
+    if (!(m->eflag[e] & elDry) && m->emark[e]) wel++;
+  }
+  *dried = del; *wetted = wel;
+}
+
+/* GRID::SetSlipFlow (SlipFlow.cpp:43-78): at rotated nodes the velocity and its time
+ * derivative keep only their component along the first column of the rotation; applied
+ * after a new set-up of the wet model and at the end of the flow cycle
+ * (EqsUVS2D.cpp:1078) */
+void orc_slip_flow(orc_model* m)
+{
+  for (int n = 0; n < m->np; n++) {
+    if (!(m->flag[n] & fRotat)) continue;
+    double r00 = getrot(m, n, 0, 0), r10 = getrot(m, n, 1, 0);
+    double Ut = r00 * m->U[n] + r10 * m->V[n];
+    m->U[n] = r00 * Ut; m->V[n] = r10 * Ut;
+    Ut = r00 * m->dUdt[n] + r10 * m->dVdt[n];
+    m->dUdt[n] = r00 * Ut; m->dVdt[n] = r10 * Ut;
+  }
+}
+
+/* MODEL::DoDryRewet (DoDryRew.cpp:50-238), serial build, method 2.  When elements got
+ * dry or wet the model is set up again on the wet elements (MODEL::Initialize, SetNormal,
+ * SetRotation: Model.cpp:121-215, Transform.cpp) with the slip flow condition applied to
+ * the rotated nodes (GRID::SetSlipFlow, SlipFlow.cpp:43-78), and the equations are
+ * numbered anew at the next Newton step.  Returns the number of elements that got wet
+ * (-1 for a method other than 2); *dried the number that got dry. */
+int orc_dry_rewet(orc_model* m, int* dried)
+{
+  int del = 0, wel = 0;
+  if (m->dryMethod != 2) return -1;        /* methods 1 and 3 are not restated */
+  /* the time step's boundary conditions (BCONSET::GenBcon at the start of the time step,
+   * Compute.cpp) precede the first call */
+  if (!m->bcon_init) init_bcon(m);
+  dry_rewet_method2(m, &del, &wel);
+  if (del + wel) {
+    topology(m);
+    set_location(m);
+    set_normal(m);
+    set_rotation(m);
+    orc_slip_flow(m);
+    m->init_done = 0;
+  } else {
+    /* GRID::Connection(ELEM::kDry) only */
+    for (int n = 0; n < m->np; n++) m->noel[n] = 0;
+    for (int e = 0; e < m->ne; e++)
+      if (!(m->eflag[e] & elDry))
+        for (int i = 0; i < m->nnd[e]; i++) m->noel[m->elem[8 * e + i]]++;
+  }
+  m->firstDryRew = 0;
+  if (dried) *dried = del;
+  return wel;
+}
+
+void orc_set_dryrew(orc_model* m, double rewetLimit, int countDown)
+{
+  m->rewetLimit = rewetLimit; m->countDown = countDown;
+}
+
+/* per node the reference's dry and marsh bits (Node.h:100-102), per region element
+ * 1 dry, 2 marsh */
+void orc_get_wetdry(const orc_model* m, int* nflag, int* eflag)
+{
+  for (int n = 0; n < m->np; n++)
+    nflag[n] = ((m->flag[n] & fDry) ? refNodeDry : 0) | ((m->flag[n] & fMarsh) ? refNodeMarsh : 0);
+  for (int e = 0; e < m->ne; e++) eflag[e] = (int)m->eflag[e];
 }
 
 int orc_prepare(orc_model* m)

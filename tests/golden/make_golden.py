@@ -86,6 +86,25 @@ DECKS = {
     "elderq10x3": dict(nx=10, ny=3, length=100.0, width=30.0, noise_sigma=0.05, noise_corr=5.0,
                        Q=10.0, quads=True, vt=0.04, turb=112, turb_st=(0.5, 1.5),
                        turb_kdest=(0.4, 0.8), turb_estx=(0.4, 1.2), turb_esty=(0.05, 0.3)),
+    "wetdry20x6": dict(nx=20, ny=6, length=100.0, width=30.0, noise_sigma=0.05, noise_corr=5.0,
+                       Q=10.0),
+    "rewet20x6": dict(nx=20, ny=6, length=100.0, width=30.0, noise_sigma=0.05, noise_corr=5.0,
+                      Q=10.0),
+}
+
+# a bank that rises out of the water: the bed goes up linearly across the width from y0 to
+# rise m at y1 (lattice lines, so that midside nodes stay the mean of their corners); with
+# drying and rewetting every few Newton steps the far bank becomes dry, the strip beside it
+# marsh, and the wet/dry line a slip boundary
+BANKS = {
+    "wetdry20x6": (15.0, 30.0, 4.5),
+    "rewet20x6": (15.0, 30.0, 2.4),
+}
+
+# initial water level below the outlet's: the level rises over the Newton steps and the
+# marsh strip and the dry bank above it get wet
+INIT_S = {
+    "rewet20x6": 1.5,
 }
 
 # inlet hydrographs (QTINLET), rows (t s, Q m3/s, S m)
@@ -99,6 +118,8 @@ RUNS = {
     "unst20x4": dict(tm_steps=3, dt=30.0, theta=0.7, stationary=0, elem_step=1),
     "unstq10x3": dict(tm_steps=3, dt=20.0, theta=0.6, stationary=0, elem_step=2),
     "hydro20x4": dict(tm_steps=4, dt=30.0, theta=0.7, stationary=0, elem_step=5),
+    "wetdry20x6": dict(dryrew="2 3 0.001 0.005 100 5", elem_step=1),
+    "rewet20x6": dict(dryrew="2 2 0.001 0.005 100 1", maxiter=5, elem_step=3),
 }
 
 
@@ -153,13 +174,29 @@ def main():
         d = dk.channel(**kw)
         if name in HYDRO:
             d.hydrograph = np.asarray(HYDRO[name], np.float64)
+        if name in BANKS:
+            y0, y1, rise = BANKS[name]
+            z = d.z + rise * np.clip((d.y - y0) / (y1 - y0), 0.0, 1.0)
+            # midside nodes: the mean of their corners, as the region reader sets them
+            for e in range(d.n_elems):
+                nnd = int(d.nnd[e])
+                ncn = nnd // 2
+                for i in range(ncn, nnd):
+                    left, right = d.elems[e, i - ncn], d.elems[e, (i - ncn + 1) % ncn]
+                    z[d.elems[e, i]] = 0.5 * (z[left] + z[right])
+            d.z = z
+        if name in INIT_S:
+            d.S0 = np.full(d.n_nodes, INIT_S[name])
         with tempfile.TemporaryDirectory() as tmp:
             (steps, final), iters = run_reference(d, tmp, run)
         out = deck_arrays(d)
         out["nsteps"] = len(steps)
-        for key in ("tm_steps", "dt", "theta", "stationary", "elem_step"):
+        for key in ("tm_steps", "dt", "theta", "stationary", "elem_step", "maxiter"):
             if key in run:
                 out[key] = run[key]
+        if "dryrew" in run:
+            # $DRYREW method freq dryLimit rewetLimit rewetPasses countDown
+            out["dryrew"] = np.asarray([float(v) for v in run["dryrew"].split()])
         out["bicgstab_iters"] = np.asarray(iters, np.int64)
         full = name != "channel200x20"
         for k, st in enumerate(steps):
@@ -169,6 +206,8 @@ def main():
             out[p + "tstep"] = st["tstep"]
             if full:
                 out.update(state_arrays(p, st["state"]))
+                if "dryrew" in run:
+                    out[p + "flag"] = st["state"]["flag"]
                 out[p + "width"] = st["width"]
                 out[p + "cols"] = st["cols"]
                 out[p + "aval"] = st["aval"]
@@ -198,6 +237,8 @@ def main():
             for k in ("ascaled", "bscaled", "ilu", "iluapply", "mulvec"):
                 out["s0_" + k] = s0[k]
         out.update(state_arrays("final_", final))
+        if "dryrew" in run:
+            out["final_flag"] = final["flag"]
         path = os.path.join(HERE, f"{name}.npz")
         np.savez_compressed(path, **out)
         print(f"{name}: {len(steps)} Newton steps, BiCGStab iterations {iters}, "

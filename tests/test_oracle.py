@@ -108,6 +108,78 @@ def test_oracle_matches_reference_every_record(name):
     _check_state(om, g, "final_")
 
 
+@pytest.mark.parametrize("name", ["wetdry20x6", "rewet20x6"])
+def test_oracle_wet_dry_matches_reference_every_record(name):
+    """A bank that rises out of the water, drying and rewetting (method 2) at Newton step 0
+    and then every few steps: the reference's dry and marsh node flags, lowered bed,
+    reset dry nodes, boundary conditions on the wet/dry line, numbering without the dry
+    nodes, pattern, matrices and corrections, record by record.  wetdry20x6: the far bank
+    dry, the strip beside it marsh; rewet20x6: the water rises from below the outlet level
+    and the dry bank gets wet."""
+    g = load_golden(name)
+    method, freq, dry_limit, rewet_limit, _passes, count_down = g["dryrew"]
+    om = po.OracleModel(golden_deck(g))
+    om.set_params(dry_method=int(method), dry_limit=float(dry_limit))
+    om.set_dryrew(float(rewet_limit), int(count_down))
+    # the schedule of EqsUVS2D::Execute (EqsUVS2D.cpp:201-222): a call at step 0, then every
+    # freq steps or after a converged step
+    maxit = int(g["maxiter"]) if "maxiter" in g else 12
+    next_dw, nrconv, calls = 0, True, []
+    el_step = int(g["elem_step"])
+    for k in range(int(g["nsteps"])):
+        p = f"s{k}_"
+        if next_dw >= 0 and (k == next_dw or nrconv):
+            next_dw = k + int(freq)
+            calls.append((k,) + om.dry_rewet())
+            if next_dw > maxit:
+                maxit, next_dw = next_dw, -1
+        neq = om.prepare()
+        nf, ef = om.wetdry()
+        np.testing.assert_array_equal(nf, g[p + "flag"].astype(np.int64) & (64 | 256),
+                                      err_msg=f"dry/marsh flags at step {k}")
+        nod = _check_state(om, g, p)
+        np.testing.assert_array_equal(nod["cf"], g[p + "cf"])
+        np.testing.assert_array_equal(nod["kind"], g[p + "kind"].astype(np.int32))
+        np.testing.assert_array_equal(nod["rot"], g[p + "rot"].astype(np.int32))
+        np.testing.assert_array_equal(nod["bcU"], g[p + "bcU"])
+        rotm = np.stack([g[p + "r00"], g[p + "r01"], g[p + "r10"], g[p + "r11"]], 1)
+        np.testing.assert_array_equal(nod["rotm"], rotm)
+        eq = om.numbering()
+        np.testing.assert_array_equal(eq, np.stack([g[p + "eqU"], g[p + "eqV"], g[p + "eqS"]], 1))
+        rp, cols = om.pattern()
+        np.testing.assert_array_equal(np.diff(rp), g[p + "width"])
+        np.testing.assert_array_equal(cols, g[p + "cols"])
+        if k == el_step:
+            mes = om.model_elems()
+            assert len(mes) == len(g["el_nnd"])
+            for i, (nodes, isb) in enumerate(mes):
+                nnd = len(nodes)
+                assert nnd == g["el_nnd"][i] and isb == bool(g["el_bound"][i])
+                np.testing.assert_array_equal(nodes + 1, g["el_names"][i, :nnd])
+                K, F = om.coefs(i)
+                np.testing.assert_array_equal(K[:3 * nnd, :3 * nnd], g["el_estifm"][i, :3 * nnd, :3 * nnd])
+                np.testing.assert_array_equal(F[:3 * nnd], g["el_force"][i, :3 * nnd])
+        A, B = om.assemble(32)
+        np.testing.assert_array_equal(A.astype(np.float64), g[p + "aval"])
+        np.testing.assert_array_equal(B, g[p + "bvec"])
+        X, st = om.newton_step(prec=32, solver=1, max_iter=1000, eps=1e-10)
+        nrconv = bool(st.nrconv)
+        assert st.iters == g["bicgstab_iters"][k]
+        np.testing.assert_array_equal(X, g[p + "xsol"])
+        assert neq == len(X)
+    om.slip_flow()              # the end of the flow cycle (EqsUVS2D.cpp:1078)
+    _check_state(om, g, "final_")
+    nf, ef = om.wetdry()
+    np.testing.assert_array_equal(nf, g["final_flag"].astype(np.int64) & (64 | 256))
+    # the far bank dry, the strip beside it marsh, from the first call on
+    assert calls[0][0] == 0 and calls[0][1] > 0 and (ef == 2).any()
+    if name == "rewet20x6":
+        # the rising water wets elements of the bank at a later call
+        assert any(k > 0 and w > 0 for k, _, w in calls)
+    else:
+        assert (ef == 1).sum() == calls[0][1]
+
+
 def test_oracle_matches_reference_channel_200x20():
     """The 8,000-triangle straight channel: numbering, pattern hash, every Newton
     correction, BiCGStab iteration counts and the final state, bit for bit."""
