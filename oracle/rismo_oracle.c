@@ -10,9 +10,10 @@
  * Reynolds stresses, constant eddy-viscosity model; point sources, none defined) are
  * left out; dropping an exact zero changes at most the sign of a zero result.
  *
- * Scope: steady or pseudo-time-relaxed Newton, triangles (6 nodes) and quads (8 nodes),
- * QINLET / OUTLET control lines, closed (tangential-flow) walls, Manning friction,
- * constant eddy viscosity, GPdeg 5 Gauss rules.
+ * Scope: steady, pseudo-time-relaxed or unsteady (theta scheme) Newton, triangles (6 nodes)
+ * and quads (8 nodes), QINLET / QTINLET / OUTLET control lines, closed (tangential-flow)
+ * walls, Manning friction with Lindner vegetation, constant, algebraic and anisotropic
+ * (Elder) eddy viscosity, drying and rewetting by method 2, GPdeg 5 Gauss rules.
  */
 #include "oracle.h"
 
