@@ -656,21 +656,55 @@ def write_reference_restart(path: str, deck: Deck, U, V, S, time: str = "0.0",
             f.write(f"{e + 1:7d}" + "".join(fmt(v) for v in vals) + "\n")
 
 
+def _scan(tokens, defaults, types):
+    """The fields of a keyword line the way sscanf fills them: in order, until the first
+    token that does not convert; the rest keep their previous values."""
+    out = list(defaults)
+    for i, (tok, ty) in enumerate(zip(tokens, types)):
+        try:
+            out[i] = ty(tok)
+        except ValueError:
+            break
+    return out
+
+
+# PROJECT and DRYREW defaults (Project.cpp:71-92, Grid.h:109-114) for the project keywords
+# read by PROJECT::Input (Project.cpp:1780-1852)
+_PROJECT_KEYS = {
+    # convUV convS convKD convQb
+    "CONVERGENCE": ((1.0e-3, 1.0e-4, 1.0e-4, 1.0e-3), (float,) * 4),
+    # relaxMethod relaxMin relaxMax maxDeltaUV maxDeltaS maxDeltaKD
+    "RELAX": ((3, 0.001, 0.1, 0.1, 0.1, 0.1), (int,) + (float,) * 5),
+    # method dryRewFreq dryLimit rewetLimit rewetPasses countDown
+    "DRYREW": ((2, 10, 0.001, 0.005, 100, 5), (int, int, float, float, int, int)),
+    # minVt minK minD maxUs minC
+    "MINMAX": ((0.0, 0.0, 0.0, 100.0, 0.0), (float,) * 5),
+}
+
+
 def read_reference_deck(input_file: str) -> tuple[Deck, dict]:
     """Read a deck in the reference's keyword input format: the project file
     (``$REGIONFILE``, ``$CONTROLFILE``, ``$ASC_INITFILE``, ``$MATERIALFILE``,
     ``$TIMESTEPFILE``; Project.cpp:1296-1440, names relative to the project file's folder)
     and the files it names, read as the reference's readers read them.  Returns the deck
     (line 1 the inlet, line 2 the outlet, the solve path's convention) and the run settings
-    (steps, dt, theta, stationary flag, Newton iterations, and the project file's other
-    keyword lines as text).
+    (steps, dt, theta, stationary flag, Newton iterations, ``params``: the solve parameters
+    of ``$CONVERGENCE``, ``$RELAX``, ``$DRYREW`` and ``$MINMAX`` with the reference's
+    defaults, and the project file's other keyword lines as text).
 
-    Supported: Manning roughness, emergent vegetation, constant / algebraic eddy viscosity,
-    one QINLET or QTINLET line and one OUTLET line with a single water level.  Anything else
-    raises DeckFormatError."""
+    Supported: Manning roughness, emergent vegetation, constant / algebraic / anisotropic
+    eddy viscosity, one QINLET or QTINLET line and one OUTLET line with a single water
+    level, relaxation method 2, drying and rewetting not called inside the flow cycle
+    (``$DRYREW`` frequency not below ``$TM_MAXITER``).  Anything else raises
+    DeckFormatError."""
     base = os.path.dirname(os.path.abspath(input_file))
     files, other = {}, []
+    proj = {k: list(v[0]) for k, v in _PROJECT_KEYS.items()}
     for key, t in _keyword_lines(input_file)[1]:
+        if key in _PROJECT_KEYS:
+            proj[key] = _scan(t, proj[key], _PROJECT_KEYS[key][1])
+            other.append("$" + " ".join([key] + t))
+            continue
         if key in ("BIN_INITFILE", "BIN_INIFILE", "STA_INITFILE"):
             raise DeckFormatError(f"{input_file}: ${key}: only ASCII initial files "
                                   "($ASC_INITFILE) are supported")
@@ -728,4 +762,20 @@ def read_reference_deck(input_file: str) -> tuple[Deck, dict]:
                 turb_estx=per_elem("estx"), turb_esty=per_elem("esty"),
                 hydrograph=hydro)
     run["project"] = other
+    conv, relax, dry, minmax = (proj[k] for k in ("CONVERGENCE", "RELAX", "DRYREW", "MINMAX"))
+    dry[3] = max(dry[3], dry[2])                 # rewetLimit not below dryLimit
+    if relax[0] != 2:
+        raise DeckFormatError(f"$RELAX: relaxation method {relax[0]} is not supported (2)")
+    # without $TM_MAXITER the cycle's iteration count stays 0 (BCONSET::BCONSET,
+    # BconSet.cpp:58), and the flow cycle never calls the drying and rewetting
+    maxiter = run["maxiter"] or 0
+    if dry[0] and dry[1] < maxiter:
+        # EqsUVS2D.cpp:201: the flow cycle calls the drying and rewetting
+        raise DeckFormatError(f"$DRYREW: drying and rewetting every {dry[1]} Newton steps "
+                              f"(below $TM_MAXITER {maxiter}) is not supported on the device")
+    run["params"] = dict(relax_method=relax[0], max_delta_uv=relax[3], max_delta_s=relax[4],
+                         conv_uv=conv[0], conv_s=conv[1], dry_method=dry[0], dry_limit=dry[2],
+                         max_us=minmax[3])
+    run["dryrew"] = dict(method=dry[0], freq=dry[1], dry_limit=dry[2], rewet_limit=dry[3],
+                         rewet_passes=dry[4], count_down=dry[5])
     return deck, run

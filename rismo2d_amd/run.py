@@ -36,7 +36,7 @@ def run_reference_deck(input_file: str, ctx: rmx.Context | None = None, solve: d
     c = ctx or rmx.Context(0)
     try:
         c.upload_mesh(d)
-        c.set_params()
+        c.set_params(**run["params"])
         if solve is None:
             solve = FAST
         if solve.get("precond") == rmx.PREC_SILU0C:

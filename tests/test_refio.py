@@ -151,6 +151,7 @@ TM = """$RISMO2D 40100
 $TM_STEPS 1 1 0
 $TM_INTERVAL 1.0e6 0 0
 $TM_TURBULENCE 1
+$TM_MAXITER 6
 $TM_LINE QINLET 4 50.0 1.5
 $TM_LINE QINLET 4 60.0 1.6
 $TM_LINE OUTLET 9 1.4 0 0 0 0.0
@@ -161,6 +162,7 @@ $CONTROLFILE control.dat
 $ASC_INITFILE init.dat
 $MATERIALFILE mat.dat
 $TIMESTEPFILE tm.dat
+$RELAX 2
 """
 
 
@@ -219,6 +221,9 @@ def test_initial_file_fixed_columns_and_defaults(tmp_path):
     (("tm.dat", "$TM_LINE OUTLET 9", "$TM_LINE SLIP 9"), "not supported"),
     (("init.dat", "9 0\n", "8 0\n"), "wrong number of nodes"),
     (("input.dat", "$ASC_INITFILE", "$BIN_INITFILE"), "only ASCII"),
+    (("input.dat", "$RELAX 2\n", ""), "relaxation method 3"),
+    (("input.dat", "$RELAX 2", "$RELAX 1"), "relaxation method 1"),
+    (("input.dat", "$RELAX 2\n", "$RELAX 2\n$DRYREW 2 5\n"), "drying and rewetting every 5"),
 ])
 def test_reader_rejects_what_the_path_does_not_support(tmp_path, edit, msg):
     init = "# 0.0 x 40000 U V S\n9 0\n" + "".join(f"{i} 0 0 1.0\n" for i in range(1, 10))
@@ -228,6 +233,35 @@ def test_reader_rejects_what_the_path_does_not_support(tmp_path, edit, msg):
     files[name] = files[name].replace(old, new)
     with pytest.raises(dk.DeckFormatError, match=msg):
         dk.read_reference_deck(_write(tmp_path, files))
+
+
+def test_project_keywords_as_the_reference_scans_them(tmp_path):
+    """$CONVERGENCE, $RELAX, $DRYREW and $MINMAX (Project.cpp:1780-1852): fields in order
+    up to the first that does not convert, the reference's defaults for the rest, the
+    rewetting limit raised to the drying limit; drying and rewetting switched off
+    (method 0) may name any frequency."""
+    init = "# 0.0 x 40000 U V S\n9 0\n" + "".join(f"{i} 0 0 1.0\n" for i in range(1, 10))
+    files = _files(init)
+    files["input.dat"] = files["input.dat"].replace(
+        "$RELAX 2\n", "$RELAX 2 0.001 0.1 5.0 x 7.0\n$CONVERGENCE 1e-6\n"
+                       "$DRYREW 0 3 0.01 0.002\n$MINMAX 0 0 0 50\n")
+    _, run = dk.read_reference_deck(_write(tmp_path, files))
+    assert run["params"] == dict(relax_method=2, max_delta_uv=5.0, max_delta_s=0.1,
+                                 conv_uv=1e-6, conv_s=1e-4, dry_method=0, dry_limit=0.01,
+                                 max_us=50.0)
+    assert run["dryrew"] == dict(method=0, freq=3, dry_limit=0.01, rewet_limit=0.01,
+                                 rewet_passes=100, count_down=5)
+
+
+def test_golden_deck_project_keywords_are_the_solve_defaults(tmp_path):
+    """The golden decks' $RELAX / $CONVERGENCE / $DRYREW give the parameters the oracle and
+    the device solve use by default."""
+    g = load_golden("tri20x4")
+    dk.write_reference_deck(golden_deck(g), str(tmp_path))
+    _, run = dk.read_reference_deck(str(tmp_path / "input.dat"))
+    assert run["params"] == dict(relax_method=2, max_delta_uv=10.0, max_delta_s=10.0,
+                                 conv_uv=1e-7, conv_s=1e-7, dry_method=2, dry_limit=0.001,
+                                 max_us=100.0)
 
 
 @pytest.mark.parametrize("exact", [True, False])
