@@ -157,6 +157,11 @@ rmx_status rmx_set_params(rmx_ctx* p, const rmx_params* q)
   API_BEGIN
   Ctx& c = *need(p);
   if (!q) fail(RMX_ERR_ARG, "null params");
+  if (q->relax_method != 0 && q->relax_method != 2)
+    fail(RMX_ERR_ARG, "params: relaxation method " + std::to_string(q->relax_method) +
+                      " not supported (0 none, 2 cap by max_delta_uv / max_delta_s)");
+  if (!(q->dry_limit > 0.0))
+    fail(RMX_ERR_ARG, "params: dry_limit must be positive (the minimum depth is dry_limit / 2)");
   c.par.relax_method = q->relax_method; c.par.dry_method = q->dry_method;
   c.par.dry_limit = q->dry_limit; c.par.max_delta_uv = q->max_delta_uv;
   c.par.max_delta_s = q->max_delta_s; c.par.conv_uv = q->conv_uv; c.par.conv_s = q->conv_s;

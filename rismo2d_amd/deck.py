@@ -594,14 +594,34 @@ def _read_materials(path: str):
     return mats
 
 
+_UVS_CYCLE = 1                 # kUVSCyc (Project.h:102): the UVS Taylor-Hood flow cycle
+# per-cycle lists of a boundary set and the reference's defaults (BCONSET::BCONSET)
+_TM_CYCLE_LISTS = {"TM_STATIONARY": "stationary", "TM_TURBULENCE": "turb",
+                   "TM_DISPERSION": "disp", "TM_MAXITER": "maxiter", "TM_SOLVER": "solver"}
+
+
 def _read_timesteps(path: str):
-    """The time-step file (TIMEINT input, Timeint.cpp:1466-2340): $TM_STEPS, $TM_INTERVAL,
-    $TM_WEIGHT, $TM_STATIONARY, $TM_TURBULENCE, $TM_MAXITER and the line conditions
-    $TM_LINE QINLET no Q S (one value, later lines overwrite), QTINLET no Q S t (hydrograph
-    rows) and OUTLET no S x y ... (Timeint.cpp:2230-2330)."""
+    """The time-step file (TIMEINT::Input_40100, Timeint.cpp:1466-2340): $TM_STEPS,
+    $TM_INTERVAL, $TM_WEIGHT, one boundary set ($TM_STEP_NO 1) with its cycle list
+    ($TM_CYCLE) and the per-cycle lists $TM_STATIONARY, $TM_TURBULENCE, $TM_DISPERSION,
+    $TM_MAXITER and $TM_SOLVER, and the line conditions $TM_LINE QINLET no Q S (one value,
+    later lines overwrite), QTINLET no Q S t (hydrograph rows) and OUTLET no S x y ...
+    (Timeint.cpp:2230-2330).
+
+    A per-cycle list is read as the reference reads it (Timeint.cpp:2021-2135): its
+    tokens are the values of cycles 0, 1, ...; the cycles of the set's $TM_CYCLE list read
+    so far beyond the last token take the last token's value.  The flow cycle's entry is
+    kept.  $TM_OUTPUT and $TM_RESET_STATIST (output and statistics only) are read past.
+    Anything else (more than one boundary set, another cycle than the flow cycle 1,
+    dispersion, $TM_SETTIME, boundary conditions by $TM_BOUND_* / $TM_NODE / periodic
+    lines) raises DeckFormatError, as does a set without $TM_CYCLE or $TM_MAXITER (the
+    reference then runs no Newton step: BCONSET::BCONSET leaves cycit = 0)."""
     release, kws = _keyword_lines(path)
     run = dict(release=release, steps=1, dt=None, theta=1.0, stationary=1, turb=1, maxiter=None)
     cond: dict = {}
+    sets = 0
+    cycles: list = []
+    lists: dict = {}
     for key, t in kws:
         if key == "TM_STEPS":
             run["steps"] = int(t[1]) - int(t[0]) + 1
@@ -609,12 +629,23 @@ def _read_timesteps(path: str):
             run["dt"] = float(t[0])
         elif key == "TM_WEIGHT":
             run["theta"] = float(t[0])
-        elif key == "TM_STATIONARY":
-            run["stationary"] = int(t[0])
-        elif key == "TM_TURBULENCE":
-            run["turb"] = int(t[0])
-        elif key == "TM_MAXITER":
-            run["maxiter"] = int(t[0])
+        elif key in ("TM_OUTPUT", "TM_RESET_STATIST"):
+            continue
+        elif key == "TM_STEP_NO":
+            sets += 1
+            if sets > 1:
+                raise DeckFormatError(f"{path}: more than one boundary set ($TM_STEP_NO) is "
+                                      "not supported")
+            cycles = []                          # ncyc = 0 (Timeint.cpp:1998)
+        elif key == "TM_CYCLE":
+            cycles.extend(int(v) for v in t)
+        elif key in _TM_CYCLE_LISTS:
+            vals = [int(v) for v in t]
+            if not vals:
+                raise DeckFormatError(f"{path}: ${key} without values")
+            # tokens for cycles 0..n-1, the last one repeated up to the cycles read so far
+            vals += [vals[-1]] * max(0, len(cycles) - len(vals))
+            lists[_TM_CYCLE_LISTS[key]] = vals
         elif key == "TM_LINE" and int(t[1]) > 0:
             kind, no = t[0], int(t[1])
             c = cond.setdefault((kind, no), [])
@@ -626,6 +657,20 @@ def _read_timesteps(path: str):
             else:
                 raise DeckFormatError(f"{path}: line condition {kind} is not supported "
                                       "(QINLET, QTINLET, OUTLET)")
+        elif key.startswith("TM_") and key != "TM_LINE":
+            raise DeckFormatError(f"{path}: ${key} is not supported")
+    if not cycles:
+        raise DeckFormatError(f"{path}: no $TM_CYCLE: the reference runs no cycle")
+    if cycles != [_UVS_CYCLE]:
+        raise DeckFormatError(f"{path}: $TM_CYCLE {' '.join(map(str, cycles))}: only the flow "
+                              f"cycle {_UVS_CYCLE} alone is supported")
+    if "maxiter" not in lists:
+        raise DeckFormatError(f"{path}: no $TM_MAXITER: the reference runs no Newton step")
+    if lists.get("disp", [0])[0] != 0:
+        raise DeckFormatError(f"{path}: $TM_DISPERSION {lists['disp'][0]} is not supported")
+    for k in ("stationary", "turb", "maxiter"):
+        if k in lists:
+            run[k] = lists[k][0]
     return run, cond
 
 
@@ -705,6 +750,13 @@ def read_reference_deck(input_file: str) -> tuple[Deck, dict]:
             proj[key] = _scan(t, proj[key], _PROJECT_KEYS[key][1])
             other.append("$" + " ".join([key] + t))
             continue
+        if key == "MAX_Us":
+            # maxUs, also the fourth field of $MINMAX: the later line wins (Project.cpp:1786,
+            # 1830).  ($MIN_VT*, $MIN_K, $MIN_D, $MIN_C only bound the k-epsilon models'
+            # eddy viscosity, CoefsKD2D.cpp:374; they are kept as text.)
+            proj["MINMAX"][3] = _scan(t[:1], [proj["MINMAX"][3]], (float,))[0]
+            other.append("$" + " ".join([key] + t))
+            continue
         if key in ("BIN_INITFILE", "BIN_INIFILE", "STA_INITFILE"):
             raise DeckFormatError(f"{input_file}: ${key}: only ASCII initial files "
                                   "($ASC_INITFILE) are supported")
@@ -766,9 +818,7 @@ def read_reference_deck(input_file: str) -> tuple[Deck, dict]:
     dry[3] = max(dry[3], dry[2])                 # rewetLimit not below dryLimit
     if relax[0] != 2:
         raise DeckFormatError(f"$RELAX: relaxation method {relax[0]} is not supported (2)")
-    # without $TM_MAXITER the cycle's iteration count stays 0 (BCONSET::BCONSET,
-    # BconSet.cpp:58), and the flow cycle never calls the drying and rewetting
-    maxiter = run["maxiter"] or 0
+    maxiter = run["maxiter"]                     # the flow cycle's $TM_MAXITER entry
     if dry[0] and dry[1] < maxiter:
         # EqsUVS2D.cpp:201: the flow cycle calls the drying and rewetting
         raise DeckFormatError(f"$DRYREW: drying and rewetting every {dry[1]} Newton steps "

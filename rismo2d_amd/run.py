@@ -42,7 +42,7 @@ def run_reference_deck(input_file: str, ctx: rmx.Context | None = None, solve: d
         if solve.get("precond") == rmx.PREC_SILU0C:
             c.prepare()
             c.set_coarse(8)
-        maxiter = run["maxiter"] or 12
+        maxiter = run["maxiter"]
         unsteady = run["stationary"] == 0
         steps = run["steps"] if unsteady else 1
         dt = run["dt"]

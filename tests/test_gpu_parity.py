@@ -471,6 +471,12 @@ def test_error_paths(ctx):
     for code in (3, 22, 402, -2):
         with pytest.raises(rmx.RmxError):
             ctx.set_params(turb=code)
+    # relaxation methods the update does not implement, and a dry limit that leaves no
+    # positive minimum depth, are refused at the boundary
+    for kw in (dict(relax_method=1), dict(relax_method=5), dict(dry_limit=0.0),
+               dict(dry_limit=-1e-3)):
+        with pytest.raises(rmx.RmxError, match="params"):
+            ctx.set_params(**kw)
     ctx.upload_mesh(golden_deck(load_golden("tri20x4")))
     with pytest.raises(rmx.RmxError):
         ctx.turbulence()

@@ -18,11 +18,13 @@ def test_library_exports_every_declared_symbol():
 
 
 def test_create_without_gpu_reports_no_device():
-    import torch
-    if torch.cuda.is_available():
-        pytest.skip("a GPU is present")
+    # no torch here: torch brings its own HIP and HSA runtimes, and a process that maps
+    # them next to the ones librmx.so links has two runtimes on one device
     h = ctypes.c_void_p()
     rc = rmx.lib().rmx_create(0, ctypes.byref(h))
+    if rc == 0:
+        rmx.lib().rmx_destroy(h)
+        pytest.skip("a gfx950 GPU is present")
     assert rc == rmx.ERR_NODEVICE
     assert b"device" in rmx.lib().rmx_last_error()
 

@@ -150,6 +150,8 @@ $TURBULENCE 0 9.0 9.0 0 0 9.0 0 0
 TM = """$RISMO2D 40100
 $TM_STEPS 1 1 0
 $TM_INTERVAL 1.0e6 0 0
+$TM_STEP_NO 1
+$TM_CYCLE 1
 $TM_TURBULENCE 1
 $TM_MAXITER 6
 $TM_LINE QINLET 4 50.0 1.5
@@ -224,6 +226,17 @@ def test_initial_file_fixed_columns_and_defaults(tmp_path):
     (("input.dat", "$RELAX 2\n", ""), "relaxation method 3"),
     (("input.dat", "$RELAX 2", "$RELAX 1"), "relaxation method 1"),
     (("input.dat", "$RELAX 2\n", "$RELAX 2\n$DRYREW 2 5\n"), "drying and rewetting every 5"),
+    # the time-step file: one boundary set with the flow cycle alone, and its Newton limit
+    (("tm.dat", "$TM_CYCLE 1\n", "$TM_CYCLE 90 1\n"), "only the flow cycle"),
+    (("tm.dat", "$TM_CYCLE 1\n", ""), "no \\$TM_CYCLE"),
+    (("tm.dat", "$TM_MAXITER 6\n", ""), "no \\$TM_MAXITER"),
+    (("tm.dat", "$TM_LINE OUTLET 9 1.4 0 0 0 0.0\n",
+      "$TM_LINE OUTLET 9 1.4 0 0 0 0.0\n$TM_STEP_NO 5\n$TM_CYCLE 1\n"), "more than one boundary set"),
+    (("tm.dat", "$TM_MAXITER 6\n", "$TM_MAXITER 6\n$TM_NODE QT 3 1.0 2.0\n"), "TM_NODE is not"),
+    (("tm.dat", "$TM_MAXITER 6\n", "$TM_MAXITER 6\n$TM_BOUND_LINE 4 8\n"), "TM_BOUND_LINE is not"),
+    (("tm.dat", "$TM_MAXITER 6\n", "$TM_MAXITER 6\n$TM_DISPERSION 1\n"), "DISPERSION 1"),
+    # the drying check takes the flow cycle's limit: 20 Newton steps, drying every 10
+    (("tm.dat", "$TM_MAXITER 6\n", "$TM_MAXITER 20 1\n"), "drying and rewetting every 10"),
 ])
 def test_reader_rejects_what_the_path_does_not_support(tmp_path, edit, msg):
     init = "# 0.0 x 40000 U V S\n9 0\n" + "".join(f"{i} 0 0 1.0\n" for i in range(1, 10))
@@ -251,6 +264,26 @@ def test_project_keywords_as_the_reference_scans_them(tmp_path):
                                  max_us=50.0)
     assert run["dryrew"] == dict(method=0, freq=3, dry_limit=0.01, rewet_limit=0.01,
                                  rewet_passes=100, count_down=5)
+    # $MAX_Us sets maxUs as $MINMAX's fourth field does; the later line wins
+    # (Project.cpp:1786, 1830)
+    for extra, want in (("$MINMAX 0 0 0 50\n$MAX_Us 5.0\n", 5.0),
+                        ("$MAX_Us 5.0\n$MINMAX 0 0 0 50\n", 50.0),
+                        ("$MAX_Us 5.0\n$MINMAX 0 0\n", 5.0)):
+        files = _files(init)
+        files["input.dat"] = files["input.dat"].replace("$RELAX 2\n", "$RELAX 2\n" + extra)
+        _, run = dk.read_reference_deck(_write(tmp_path, files))
+        assert run["params"]["max_us"] == want, extra
+
+
+def test_time_step_lists_are_read_per_cycle(tmp_path):
+    """$TM_MAXITER / $TM_TURBULENCE / $TM_STATIONARY hold one value per cycle of the set
+    (Timeint.cpp:2021-2135): the flow cycle's is the first token, whatever follows."""
+    init = "# 0.0 x 40000 U V S\n9 0\n" + "".join(f"{i} 0 0 1.0\n" for i in range(1, 10))
+    files = _files(init)
+    files["tm.dat"] = files["tm.dat"].replace("$TM_MAXITER 6\n", "$TM_MAXITER 4 9\n").replace(
+        "$TM_TURBULENCE 1\n", "$TM_TURBULENCE 2 1\n$TM_STATIONARY 1 0\n")
+    _, run = dk.read_reference_deck(_write(tmp_path, files))
+    assert (run["maxiter"], run["turb"], run["stationary"]) == (4, 2, 1)
 
 
 def test_golden_deck_project_keywords_are_the_solve_defaults(tmp_path):
