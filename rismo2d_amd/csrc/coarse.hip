@@ -19,6 +19,7 @@
 #include <cmath>
 #include <cstdio>
 #include <cstdlib>
+#include <atomic>
 
 #include "rmx_internal.h"
 
@@ -148,6 +149,283 @@ static void band_solve(int n, int kl, int ku, std::vector<double>& ab, const std
 }
 
 // ---------------------------------------------------------------------------------------
+// block cyclic reduction of the coarse system (block tridiagonal in strips)
+//
+// Blocks i = 0..nb-1 (strip i's aggregates, padded to bs with identity rows), A_i the
+// diagonal block, B_i the coupling to block i - s and C_i to block i + s at level l
+// (s = 2^l).  Level l eliminates the odd multiples j of s:
+//   kept i (multiples of 2s):  P_i = B_i A_{i-s}^-1,  Q_i = C_i A_{i+s}^-1,
+//     A_i <- A_i - P_i C_{i-s} - Q_i B_{i+s},  B_i <- -P_i B_{i-s},  C_i <- -Q_i C_{i+s};
+//   apply, reduction:        w_i <- w_i - P_i w_{i-s} - Q_i w_{i+s};
+//   apply, back-substitution: w_j <- A_j^-1 (w_j - B_j w_{j-s} - C_j w_{j+s}).
+// After the last level block 0 alone remains: w_0 <- A_0^-1 w_0.  Matrices are stored
+// column-major (entry (r, c) at c * bs + r), so row r of a matrix-vector product reads
+// consecutive entries across the lanes of a wave.
+
+// w[i * bs + r] <- rc[blk_start[i] + r] (0 on padding)
+__global__ void k_cr_in(int nb, int bs, const int* __restrict__ blk_start,
+                        const double* __restrict__ rc, double* __restrict__ w)
+{
+  const int i = blockIdx.x, r = threadIdx.x;
+  if (i >= nb || r >= bs) return;
+  const int a = blk_start[i], m = blk_start[i + 1] - a;
+  w[(long long)i * bs + r] = r < m ? rc[a + r] : 0.0;
+}
+
+__global__ void k_cr_out(int nb, int bs, const int* __restrict__ blk_start,
+                         const double* __restrict__ w, double* __restrict__ zc)
+{
+  const int i = blockIdx.x, r = threadIdx.x;
+  if (i >= nb || r >= bs) return;
+  const int a = blk_start[i], m = blk_start[i + 1] - a;
+  if (r < m) zc[a + r] = w[(long long)i * bs + r];
+}
+
+// one wave per kept block k (i = 2 s k) of level s: w_i -= P_i w_{i-s} + Q_i w_{i+s}
+__global__ void __launch_bounds__(64) k_cr_fwd(int nb, int bs, int s, const double* __restrict__ M,
+                                               double* __restrict__ w)
+{
+  __shared__ double lo[64], hi[64];
+  const int k = blockIdx.x, r = threadIdx.x;
+  const int i = 2 * s * k;
+  if (i >= nb) return;
+  const bool hl = i - s >= 0, hh = i + s < nb;
+  if (r < bs) {
+    lo[r] = hl ? w[(long long)(i - s) * bs + r] : 0.0;
+    hi[r] = hh ? w[(long long)(i + s) * bs + r] : 0.0;
+  }
+  __syncthreads();
+  if (r >= bs) return;
+  const long long b2 = (long long)bs * bs;
+  const double* P = M + 2 * b2 * k;
+  const double* Q = P + b2;
+  double acc = 0.0;
+  for (int c = 0; c < bs; c++) acc += P[c * bs + r] * lo[c] + Q[c * bs + r] * hi[c];
+  w[(long long)i * bs + r] -= acc;
+}
+
+// one wave per eliminated block k (j = s + 2 s k):
+// w_j <- A_j^-1 (w_j - B_j w_{j-s} - C_j w_{j+s}); with s = 0 the top block, w_0 <- A^-1 w_0
+__global__ void __launch_bounds__(64) k_cr_bwd(int nb, int bs, int s, const double* __restrict__ M,
+                                               double* __restrict__ w)
+{
+  __shared__ double lo[64], hi[64], t[64];
+  const int k = blockIdx.x, r = threadIdx.x;
+  const int j = s + 2 * s * k;
+  if (j >= nb) return;
+  const long long b2 = (long long)bs * bs;
+  const double* Ai = M + (s ? 3 * b2 * k : 0);
+  double v = 0.0;
+  if (s) {
+    const bool hl = j - s >= 0, hh = j + s < nb;
+    if (r < bs) {
+      lo[r] = hl ? w[(long long)(j - s) * bs + r] : 0.0;
+      hi[r] = hh ? w[(long long)(j + s) * bs + r] : 0.0;
+    }
+    __syncthreads();
+    if (r < bs) {
+      const double* B = Ai + b2;
+      const double* C = B + b2;
+      v = w[(long long)j * bs + r];
+      for (int c = 0; c < bs; c++) v -= B[c * bs + r] * lo[c] + C[c * bs + r] * hi[c];
+    }
+  } else if (r < bs) {
+    v = w[(long long)j * bs + r];
+  }
+  if (r < bs) t[r] = v;
+  __syncthreads();
+  if (r >= bs) return;
+  double x = 0.0;
+  for (int c = 0; c < bs; c++) x += Ai[c * bs + r] * t[c];
+  w[(long long)j * bs + r] = x;
+}
+
+// host dense helpers on column-major bs x bs blocks
+static bool dense_inverse(int n, const double* a, double* inv)
+{
+  std::vector<double> m(a, a + (size_t)n * n);
+  std::vector<int> piv(n);
+  for (int c = 0; c < n; c++) {                 // LU with partial pivoting, column c
+    int p = c;
+    double best = std::fabs(m[(size_t)c * n + c]);
+    for (int r = c + 1; r < n; r++)
+      if (std::fabs(m[(size_t)c * n + r]) > best) { best = std::fabs(m[(size_t)c * n + r]); p = r; }
+    if (!(best > 1.0e-300)) return false;
+    piv[c] = p;
+    if (p != c)
+      for (int k = 0; k < n; k++) std::swap(m[(size_t)k * n + c], m[(size_t)k * n + p]);
+    const double d = m[(size_t)c * n + c];
+    for (int r = c + 1; r < n; r++) m[(size_t)c * n + r] /= d;
+    for (int k = c + 1; k < n; k++) {
+      const double u = m[(size_t)k * n + c];
+      if (u == 0.0) continue;
+      for (int r = c + 1; r < n; r++) m[(size_t)k * n + r] -= m[(size_t)c * n + r] * u;
+    }
+  }
+  for (int e = 0; e < n; e++) {                 // inverse column e: solve with unit vector
+    double* x = inv + (size_t)e * n;
+    for (int r = 0; r < n; r++) x[r] = r == e ? 1.0 : 0.0;
+    for (int c = 0; c < n; c++) {
+      if (piv[c] != c) std::swap(x[c], x[piv[c]]);
+    }
+    for (int c = 0; c < n; c++)
+      for (int r = c + 1; r < n; r++) x[r] -= m[(size_t)c * n + r] * x[c];
+    for (int c = n - 1; c >= 0; c--) {
+      x[c] /= m[(size_t)c * n + c];
+      for (int r = 0; r < c; r++) x[r] -= m[(size_t)c * n + r] * x[c];
+    }
+  }
+  return true;
+}
+
+// out = alpha * a b (+ out when acc), column-major n x n
+static void dense_mm(int n, const double* a, const double* b, double alpha, bool acc, double* out)
+{
+  for (int c = 0; c < n; c++) {
+    double* o = out + (size_t)c * n;
+    if (!acc) std::fill(o, o + n, 0.0);
+    for (int k = 0; k < n; k++) {
+      const double bk = alpha * b[(size_t)c * n + k];
+      if (bk == 0.0) continue;
+      const double* ak = a + (size_t)k * n;
+      for (int r = 0; r < n; r++) o[r] += ak[r] * bk;
+    }
+  }
+}
+
+// Block structure of the coarse matrix: usable when every slot couples aggregates of one
+// strip or of neighbouring strips and no strip holds more than 64 aggregates.
+static void cr_setup(Coarse& C, const std::vector<int>& agg_strip, int nstrips)
+{
+  C.cr_ok = C.cr_fac = false;
+  if (getenv("RMX_COARSE_HOST")) return;
+  std::vector<int> cnt(nstrips, 0);
+  for (int v : agg_strip) cnt[v]++;
+  int bs = 0;
+  for (int v : cnt) bs = std::max(bs, v);
+  if (bs < 1 || bs > 64) return;
+  for (long long k = 0; k < C.nslot; k++)
+    if (std::abs(agg_strip[C.slot_row[k]] - agg_strip[C.slot_col[k]]) > 1) return;
+  const int nb = nstrips;
+  C.nb = nb; C.bs = bs;
+  C.blk_start.assign(nb + 1, 0);
+  for (int i = 0; i < nb; i++) C.blk_start[i + 1] = C.blk_start[i] + cnt[i];
+  for (int I = 0; I < C.nc; I++)                // aggregates are numbered strip by strip
+    if (I < C.blk_start[agg_strip[I]] || I >= C.blk_start[agg_strip[I] + 1]) return;
+  const long long b2 = (long long)bs * bs;
+  C.slot_dst.resize(C.nslot);
+  for (long long k = 0; k < C.nslot; k++) {
+    const int I = C.slot_row[k], J = C.slot_col[k];
+    const int bi = agg_strip[I], bj = agg_strip[J];
+    const int r = I - C.blk_start[bi], c = J - C.blk_start[bj];
+    const int which = bj - bi + 1;              // 0: B (i - 1), 1: A, 2: C (i + 1)
+    C.slot_dst[k] = ((long long)which * nb + bi) * b2 + (long long)c * bs + r;
+  }
+  // matrix offsets of every level: eliminated blocks (A^-1, B, C), kept blocks (P, Q)
+  C.lev_elim.clear(); C.lev_keep.clear();
+  long long off = 0;
+  int levels = 0;
+  for (int s = 1; s < nb; s *= 2, levels++) {
+    const int ne = (nb - s + 2 * s - 1) / (2 * s);
+    const int nk = (nb + 2 * s - 1) / (2 * s);
+    C.lev_elim.push_back(off); off += 3 * b2 * ne;
+    C.lev_keep.push_back(off); off += 2 * b2 * nk;
+  }
+  C.levels = levels;
+  C.top_off = off; off += b2;
+  C.mats_n = off;
+  C.h_mats.assign(off, 0.0);
+  C.mats.alloc(off);
+  C.w.alloc((size_t)nb * bs);
+  C.d_blk_start.upload(C.blk_start, nullptr);
+  HIPCHECK(hipDeviceSynchronize());
+  C.cr_ok = true;
+}
+
+// Host factorisation of the reduction from the coarse entries cval (per slot).  Returns false
+// when a reduced block is singular (the band LU then serves).
+static bool cr_factor(Coarse& C, const double* cval)
+{
+  const int nb = C.nb, bs = C.bs;
+  const long long b2 = (long long)bs * bs;
+  std::vector<double> Bm(nb * b2, 0.0), Am(nb * b2, 0.0), Cm(nb * b2, 0.0);
+  double* blocks[3] = {Bm.data(), Am.data(), Cm.data()};
+  for (long long k = 0; k < C.nslot; k++) {
+    const long long d = C.slot_dst[k];
+    const int which = (int)(d / ((long long)nb * b2));
+    blocks[which][d - (long long)which * nb * b2] = cval[k];
+  }
+  for (int i = 0; i < nb; i++) {                 // padding: identity
+    const int m = C.blk_start[i + 1] - C.blk_start[i];
+    for (int r = m; r < bs; r++) Am[i * b2 + (long long)r * bs + r] = 1.0;
+  }
+  std::vector<double> inv(nb * b2);
+  std::atomic<bool> ok{true};
+  int lev = 0;
+  for (int s = 1; s < nb; s *= 2, lev++) {
+    const int ne = (nb - s + 2 * s - 1) / (2 * s);
+    const int nk = (nb + 2 * s - 1) / (2 * s);
+    double* E = C.h_mats.data() + C.lev_elim[lev];
+    double* K = C.h_mats.data() + C.lev_keep[lev];
+    parallel_for(ne, [&](long long a, long long e) {
+      for (long long k = a; k < e; k++) {
+        const int j = (int)(s + 2 * s * k);
+        double* Ai = E + 3 * b2 * k;
+        if (!dense_inverse(bs, &Am[j * b2], Ai)) ok = false;
+        std::copy(&Bm[j * b2], &Bm[j * b2] + b2, Ai + b2);
+        std::copy(&Cm[j * b2], &Cm[j * b2] + b2, Ai + 2 * b2);
+        std::copy(Ai, Ai + b2, &inv[j * b2]);
+      }
+    });
+    if (!ok) return false;
+    parallel_for(nk, [&](long long a, long long e) {
+      std::vector<double> nB(b2), nC(b2);
+      for (long long k = a; k < e; k++) {
+        const int i = (int)(2 * s * k);
+        double* P = K + 2 * b2 * k;
+        double* Q = P + b2;
+        std::fill(P, P + 2 * b2, 0.0);
+        std::fill(nB.begin(), nB.end(), 0.0);
+        std::fill(nC.begin(), nC.end(), 0.0);
+        if (i - s >= 0) {
+          dense_mm(bs, &Bm[i * b2], &inv[(i - s) * b2], 1.0, false, P);
+          dense_mm(bs, P, &Cm[(i - s) * b2], -1.0, true, &Am[i * b2]);
+          dense_mm(bs, P, &Bm[(i - s) * b2], -1.0, false, nB.data());
+        }
+        if (i + s < nb) {
+          dense_mm(bs, &Cm[i * b2], &inv[(i + s) * b2], 1.0, false, Q);
+          dense_mm(bs, Q, &Bm[(i + s) * b2], -1.0, true, &Am[i * b2]);
+          dense_mm(bs, Q, &Cm[(i + s) * b2], -1.0, false, nC.data());
+        }
+        std::copy(nB.begin(), nB.end(), &Bm[i * b2]);
+        std::copy(nC.begin(), nC.end(), &Cm[i * b2]);
+      }
+    });
+  }
+  return dense_inverse(bs, &Am[0], C.h_mats.data() + C.top_off);
+}
+
+static void cr_apply(Ctx& c, Coarse& C)
+{
+  const int nb = C.nb, bs = C.bs;
+  hipStream_t st = c.stream;
+  k_cr_in<<<nb, 64, 0, st>>>(nb, bs, C.d_blk_start.p, C.rc.p, C.w.p);
+  int lev = 0;
+  for (int s = 1; s < nb; s *= 2, lev++) {
+    const int nk = (nb + 2 * s - 1) / (2 * s);
+    k_cr_fwd<<<nk, 64, 0, st>>>(nb, bs, s, C.mats.p + C.lev_keep[lev], C.w.p);
+  }
+  k_cr_bwd<<<1, 64, 0, st>>>(nb, bs, 0, C.mats.p + C.top_off, C.w.p);
+  for (int l = C.levels - 1; l >= 0; l--) {
+    const int s = 1 << l;
+    const int ne = (nb - s + 2 * s - 1) / (2 * s);
+    k_cr_bwd<<<ne, 64, 0, st>>>(nb, bs, s, C.mats.p + C.lev_elim[l], C.w.p);
+  }
+  k_cr_out<<<nb, 64, 0, st>>>(nb, bs, C.d_blk_start.p, C.w.p, C.zc.p);
+  HIPCHECK(hipGetLastError());
+}
+
+// ---------------------------------------------------------------------------------------
 
 void coarse_setup(Ctx& c, const std::vector<int>& rp, const std::vector<int>& cl,
                   const std::vector<int>& group)
@@ -172,8 +450,12 @@ void coarse_setup(Ctx& c, const std::vector<int>& rp, const std::vector<int>& cl
       idx[raw[i]] = 0;
     }
   int nc = 0;
+  std::vector<int> agg_strip;
   for (long long k = 0; k < nraw; k++)
-    if (idx[k] == 0) idx[k] = nc++;
+    if (idx[k] == 0) {
+      idx[k] = nc++;
+      agg_strip.push_back((int)(k / (3LL * nseg)));
+    }
   std::vector<int> agg(n);
   for (int i = 0; i < n; i++) agg[i] = idx[raw[i]];
   // band of the coarse matrix, then coarse entries (I, J) in row-major order via a map
@@ -266,10 +548,12 @@ void coarse_setup(Ctx& c, const std::vector<int>& rp, const std::vector<int>& cl
   HIPCHECK(hipHostMalloc(&C.h_buf, sizeof(double) * std::max<long long>(nc, nslot),
                          hipHostMallocDefault));
   HIPCHECK(hipStreamSynchronize(s));
+  cr_setup(C, agg_strip, I.nblk);
   C.ready = true;
   if (getenv("RMX_DEBUG"))
     fprintf(stderr, "[rmx] coarse space: %d aggregates (%d strips x %d segments x U/V/H), "
-            "%lld entries, band kl %d ku %d\n", nc, I.nblk, nseg, nslot, kl, ku);
+            "%lld entries, band kl %d ku %d; %s\n", nc, I.nblk, nseg, nslot, kl, ku,
+            C.cr_ok ? "device block cyclic reduction" : "host band LU");
 }
 
 void coarse_factor(Ctx& c, int prec)
@@ -290,6 +574,17 @@ void coarse_factor(Ctx& c, int prec)
   HIPCHECK(hipGetLastError());
   HIPCHECK(hipMemcpyAsync(C.h_buf, C.cval.p, sizeof(double) * C.nslot, hipMemcpyDeviceToHost, c.stream));
   HIPCHECK(hipStreamSynchronize(c.stream));
+  C.fac = false;
+  C.cr_fac = false;
+  if (C.cr_ok && cr_factor(C, C.h_buf)) {
+    HIPCHECK(hipMemcpyAsync(C.mats.p, C.h_mats.data(), sizeof(double) * C.mats_n,
+                            hipMemcpyHostToDevice, c.stream));
+    HIPCHECK(hipStreamSynchronize(c.stream));
+    C.cr_fac = true;
+    C.fac = true;
+    C.fac_prec = prec;
+    return;
+  }
   const int ldab = 2 * C.kl + C.ku + 1, kv = C.kl + C.ku;
   C.band.assign((size_t)ldab * C.nc, 0.0);
   for (long long k = 0; k < C.nslot; k++)
@@ -307,10 +602,14 @@ void coarse_apply(Ctx& c, int prec, const double* r, double* z)
   if (!C.fac || C.fac_prec != prec) fail(RMX_ERR_ARG, "coarse space not factored in this precision");
   const int n = c.n, g = (n + 255) / 256;
   k_restrict<<<(C.nc * 64 + 255) / 256, 256, 0, c.stream>>>(C.nc, C.agg_ptr.p, C.agg_rows.p, r, C.rc.p);
-  HIPCHECK(hipMemcpyAsync(C.h_buf, C.rc.p, sizeof(double) * C.nc, hipMemcpyDeviceToHost, c.stream));
-  HIPCHECK(hipStreamSynchronize(c.stream));
-  band_solve(C.nc, C.kl, C.ku, C.band, C.ipiv, C.h_buf);
-  HIPCHECK(hipMemcpyAsync(C.zc.p, C.h_buf, sizeof(double) * C.nc, hipMemcpyHostToDevice, c.stream));
+  if (C.cr_fac) {
+    cr_apply(c, C);
+  } else {
+    HIPCHECK(hipMemcpyAsync(C.h_buf, C.rc.p, sizeof(double) * C.nc, hipMemcpyDeviceToHost, c.stream));
+    HIPCHECK(hipStreamSynchronize(c.stream));
+    band_solve(C.nc, C.kl, C.ku, C.band, C.ipiv, C.h_buf);
+    HIPCHECK(hipMemcpyAsync(C.zc.p, C.h_buf, sizeof(double) * C.nc, hipMemcpyHostToDevice, c.stream));
+  }
   k_ap_resid<<<g, 256, 0, c.stream>>>(n, C.ap_ptr.p, C.ap_agg.p, C.apv.p, C.zc.p, r, C.t.p);
   if (!ilu_apply(c, c.silu, prec, C.t.p, z, nullptr, C.agg.p, C.zc.p))
     k_add_prolong<<<g, 256, 0, c.stream>>>(n, C.agg.p, C.zc.p, z);

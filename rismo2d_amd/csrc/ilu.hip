@@ -68,16 +68,16 @@ constexpr int UPL = UCAP / 64;  // staged entries per lane
 // search and read from global memory inside the loop instead.
 // Symbolic plan of the factorisation (once per pattern, all rows in one launch, one wave
 // per row): every entry u of a row's concatenated pivot rows (the entries after each pivot
-// row's diagonal, in sorted lower-column order of the row) gets its pivot index t and its
-// target position in the row (255: not in the row, or a column at or left of the pivot).
+// row's diagonal, in sorted lower-column order of the row) gets its pivot index t (low
+// byte) and its target position in the row (high byte; 255: not in the row, or a column at
+// or left of the pivot).
 __global__ void __launch_bounds__(64) k_ilu_plan(const int4* __restrict__ desc, int cnt,
                                                  const int* __restrict__ cols,
                                                  const int* __restrict__ srt_cols,
                                                  const unsigned char* __restrict__ srt_pos,
                                                  const int2* __restrict__ piv,
                                                  const long long* __restrict__ pbase,
-                                                 unsigned char* __restrict__ plan_t,
-                                                 unsigned char* __restrict__ plan_pos)
+                                                 unsigned short* __restrict__ plan)
 {
   __shared__ int s_col[FW], s_kb[FW], s_off[FW + 1];
   __shared__ unsigned char s_pos[FW];
@@ -129,9 +129,56 @@ __global__ void __launch_bounds__(64) k_ilu_plan(const int4* __restrict__ desc, 
         if (v < cc) a = mid + 1; else b = mid - 1;
       }
     }
-    plan_t[pb + u] = (unsigned char)lo;
-    plan_pos[pb + u] = (unsigned char)pos;
+    plan[pb + u] = (unsigned short)(lo | (pos << 8));
   }
+}
+
+// Staged pivot-row entries of one row, in rounds of 64 (entry u = lane + 64 m) grouped in
+// chunks of PCH rounds: a chunk is taken when the row has entries in it (a wave-uniform
+// test), and inside a chunk no lane branches around a load (lanes past the last entry read
+// it again), so a chunk's loads are all issued before the first wait.  plan_offsets: the
+// target position of every entry and the offset (from element base) of its value in its
+// pivot row; stage_entries: the loaded values and targets into LDS.
+constexpr int PCH = 8;
+__device__ __forceinline__ void plan_offsets(const unsigned short* __restrict__ plan, int total,
+                                             const int* s_kb, const int* s_off, int base,
+                                             unsigned* offs, unsigned char* pp)
+{
+  const int lane = threadIdx.x;
+  unsigned short tp[UPL];
+#pragma unroll
+  for (int c = 0; c < UPL / PCH; c++)
+    if (c * PCH * 64 < total) {
+#pragma unroll
+      for (int m = c * PCH; m < (c + 1) * PCH; m++) tp[m] = plan[min(lane + 64 * m, total - 1)];
+#pragma unroll
+      for (int m = c * PCH; m < (c + 1) * PCH; m++) {
+        const int u = min(lane + 64 * m, total - 1);
+        const int t = tp[m] & 0xFF;
+        pp[m] = (unsigned char)(tp[m] >> 8);
+        offs[m] = (unsigned)(s_kb[t] - base + 1 + (u - s_off[t]));
+      }
+    }
+}
+
+template <class VT>
+__device__ __forceinline__ void stage_entries(int total, const unsigned* offs,
+                                              const unsigned char* pp, const VT* uv,
+                                              unsigned char* s_upos, VT* s_uval)
+{
+  const int lane = threadIdx.x;
+#pragma unroll
+  for (int c = 0; c < UPL / PCH; c++)
+    if (c * PCH * 64 < total) {
+#pragma unroll
+      for (int m = c * PCH; m < (c + 1) * PCH; m++) {
+        const int u = lane + 64 * m;
+        if (u < total) {
+          s_upos[u] = pp[m];
+          s_uval[u] = uv[m];
+        }
+      }
+    }
 }
 
 template <class VT>
@@ -141,8 +188,7 @@ __global__ void __launch_bounds__(64) k_ilu_level(const int4* __restrict__ desc,
                                                   const unsigned char* __restrict__ srt_pos,
                                                   const int2* __restrict__ piv,
                                                   const long long* __restrict__ pbase,
-                                                  const unsigned char* __restrict__ plan_t,
-                                                  const unsigned char* __restrict__ plan_pos,
+                                                  const unsigned short* __restrict__ plan,
                                                   VT* __restrict__ LU, int* __restrict__ err,
                                                   const int4* __restrict__ desc_next,
                                                   int cnt_next,
@@ -212,29 +258,17 @@ __global__ void __launch_bounds__(64) k_ilu_level(const int4* __restrict__ desc,
   for (int t = lane; t < nk; t += 64) s_dk[t] = LU[s_kb[t]];
   if (staged) {                         // pivot-row values, targets from the plan
     const long long pb = pbase[w];
-    unsigned char pt[UPL], pp[UPL];
-#pragma unroll
-    for (int m = 0; m < UPL; m++) {
-      const int u = lane + 64 * m;
-      if (u < total) { pt[m] = plan_t[pb + u]; pp[m] = plan_pos[pb + u]; }
-    }
+    unsigned offs[UPL];
+    unsigned char pp[UPL];
+    plan_offsets(plan + pb, total, s_kb, s_off, 0, offs, pp);
     VT uv[UPL];
 #pragma unroll
-    for (int m = 0; m < UPL; m++) {
-      const int u = lane + 64 * m;
-      if (u < total) {
-        const int t = pt[m];
-        uv[m] = LU[s_kb[t] + 1 + (u - s_off[t])];
-      }
-    }
+    for (int c = 0; c < UPL / PCH; c++)
+      if (c * PCH * 64 < total) {
 #pragma unroll
-    for (int m = 0; m < UPL; m++) {
-      const int u = lane + 64 * m;
-      if (u < total) {
-        s_upos[u] = pp[m];
-        s_uval[u] = uv[m];
+        for (int m = c * PCH; m < (c + 1) * PCH; m++) uv[m] = LU[offs[m]];
       }
-    }
+    stage_entries(total, offs, pp, uv, s_upos, s_uval);
   }
   unsigned int sink = 0;
   if (pre) {
@@ -243,7 +277,7 @@ __global__ void __launch_bounds__(64) k_ilu_level(const int4* __restrict__ desc,
               (unsigned int)(LU[dn.y + t] != (VT)0);
     for (int t = lane; t < dn.w; t += 64) sink += (unsigned int)piv[dn.y + t].x;
     for (long long u = pbn + 128LL * lane; u < pbn + UCAP && u < plan_n; u += 128LL * 64)
-      sink += (unsigned int)plan_t[u] + (unsigned int)plan_pos[u];
+      sink += (unsigned int)plan[u];
   }
   __syncthreads();
   if (staged) {
@@ -298,6 +332,291 @@ __global__ void __launch_bounds__(64) k_ilu_level(const int4* __restrict__ desc,
   for (int t = lane; t < wi; t += 64) LU[p0 + t] = s_val[t];
   if (lane == 0 && fabs((double)s_val[0]) < 1.0e-20) atomicOr(err, 1);
   if (sink == 0x9E3779B9u && cnt < 0) err[1] = (int)sink;   // keeps the touches (never taken)
+}
+
+// ---------------------------------------------------------------------------------------
+// dataflow factorisation: one persistent launch instead of one launch per level
+
+// Loads and stores of factor values that other waves of the same launch write or read are
+// agent-scope atomics (sc1: coherent across the XCDs' L2 caches), and so are the row
+// completion flags.  A row publishes its values, waits until they are performed
+// (vmcnt(0): the stores are acknowledged at agent scope) and then stores its flag; a
+// consumer issues its loads of a pivot row only after its flag load has returned the
+// current epoch.  No cache-wide write-back or invalidate is needed.
+__device__ __forceinline__ int ld_flag(const int* p)
+{
+  return __hip_atomic_load(p, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+}
+__device__ __forceinline__ void st_flag(int* p, int v)
+{
+  __hip_atomic_store(p, v, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+}
+// Factor values through a buffer resource based at a wave-uniform element (32-bit byte
+// offsets from it), with the sc1 cache policy: plain loads and stores, so the compiler
+// batches them under one wait, and coherent at agent scope.
+__device__ __forceinline__ __amdgpu_buffer_rsrc_t sc1_rsrc(const void* base)
+{
+  return __builtin_amdgcn_make_buffer_rsrc(const_cast<void*>(base), 0, -1, 0x00020000);
+}
+__device__ __forceinline__ float ld_sc1(__amdgpu_buffer_rsrc_t r, unsigned e, float)
+{
+  return __builtin_bit_cast(float, __builtin_amdgcn_raw_buffer_load_b32(r, e * 4u, 0, 16));
+}
+__device__ __forceinline__ double ld_sc1(__amdgpu_buffer_rsrc_t r, unsigned e, double)
+{
+  return __builtin_bit_cast(double, __builtin_amdgcn_raw_buffer_load_b64(r, e * 8u, 0, 16));
+}
+__device__ __forceinline__ void st_sc1(__amdgpu_buffer_rsrc_t r, unsigned e, float v)
+{
+  __builtin_amdgcn_raw_buffer_store_b32(__builtin_bit_cast(unsigned, v), r, e * 4u, 0, 16);
+}
+typedef unsigned u32x2 __attribute__((ext_vector_type(2)));
+__device__ __forceinline__ void st_sc1(__amdgpu_buffer_rsrc_t r, unsigned e, double v)
+{
+  __builtin_amdgcn_raw_buffer_store_b64(__builtin_bit_cast(u32x2, v), r, e * 8u, 0, 16);
+}
+
+enum { FLOW_COUNTER = 0, FLOW_ERR = 1, FLOW_ABORT = 2 };
+
+// Ordering of LDS accesses inside one wave: a one-wave workgroup needs no barrier (a wave's
+// LDS operations complete in order); the fences keep the compiler from moving accesses
+// across it.
+__device__ __forceinline__ void wave_sync()
+{
+  __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
+  __builtin_amdgcn_wave_barrier();
+  __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
+}
+
+// Waves take rows in level order from a counter (one device-scope fetch-add per row, about
+// 12 ns on this part).  A pivot row is earlier in that order than the rows that need it, so
+// it has been taken by a running wave and every wait ends; no wave has to be resident
+// before it takes work.  Per row: the row's own values, its sorted columns and plan are
+// read with plain loads (no other wave writes them).  The pivots are then eliminated in
+// increasing column order, as the reference does (update order kept, FMA contraction off),
+// progressively: the run of pivots from the next one whose rows are done (one flag load
+// per pivot) is staged and eliminated; when the next pivot's row is not done, one lane
+// polls its flag (bounded by spin_limit polls, after which the row sets the abort flag
+// that ends every wave's waits).  The rows finished last are usually the last pivots, so
+// most of a row's work is done before its last pivot row arrives.  The finished row is
+// stored and its flag set to the factorisation's epoch.  Rows whose pivot rows hold more
+// entries than the LDS staging first wait for all their pivots and read the entries from
+// memory.
+template <class VT>
+__global__ void __launch_bounds__(64) k_ilu_flow(const int4* __restrict__ desc, int n,
+                                                 const int* __restrict__ cols,
+                                                 const int* __restrict__ srt_cols,
+                                                 const unsigned char* __restrict__ srt_pos,
+                                                 const int2* __restrict__ piv,
+                                                 const long long* __restrict__ pbase,
+                                                 const unsigned short* __restrict__ plan,
+                                                 VT* __restrict__ LU, int* __restrict__ done,
+                                                 int epoch, int* __restrict__ state,
+                                                 int spin_limit)
+{
+#pragma clang fp contract(off)
+  __shared__ VT s_val[FW];
+  __shared__ int s_col[FW];
+  __shared__ unsigned char s_pos[FW];
+  __shared__ int s_kb[FW], s_off[FW + 1];
+  __shared__ VT s_dk[FW];
+  __shared__ unsigned char s_upos[UCAP];
+  __shared__ VT s_uval[UCAP];
+  const int lane = threadIdx.x;
+  int polls = 0;
+  // one lane polls the flag of row k until it holds the epoch; false: aborted
+  auto wait_row = [&](int k) -> bool {
+    for (;;) {
+      int v = 0;
+      if (lane == 0) v = ld_flag(done + k);
+      if (__builtin_amdgcn_readfirstlane(v) == epoch) return true;
+      if ((++polls & 31) == 0) {
+        int ab = 0;
+        if (lane == 0) ab = ld_flag(state + FLOW_ABORT);
+        if (__builtin_amdgcn_readfirstlane(ab) != 0 || polls > spin_limit) return false;
+      }
+      __builtin_amdgcn_s_sleep(1);
+    }
+  };
+  for (;;) {
+    int w = 0;
+    if (lane == 0) w = __hip_atomic_fetch_add(state + FLOW_COUNTER, 1, __ATOMIC_RELAXED,
+                                              __HIP_MEMORY_SCOPE_AGENT);
+    w = __builtin_amdgcn_readfirstlane(w);
+    if (w >= n) break;
+    const int4 d = desc[w];
+    const int i = d.x, p0 = d.y, wi = d.z, nk = d.w;
+    for (int t = lane; t < wi; t += 64) {
+      s_val[t] = LU[p0 + t];
+      s_col[t] = srt_cols[p0 + t];
+      s_pos[t] = srt_pos[p0 + t];
+    }
+    int carry = 0;
+    for (int t0 = 0; t0 < nk; t0 += 64) {
+      const int t = t0 + lane;
+      int len = 0;
+      if (t < nk) {
+        const int2 pv = piv[p0 + t];
+        s_kb[t] = pv.x;
+        len = pv.y;
+      }
+      int incl = len;
+      for (int o = 1; o < 64; o <<= 1) {
+        const int v = __shfl_up(incl, o, 64);
+        if (lane >= o) incl += v;
+      }
+      if (t < nk) s_off[t + 1] = carry + incl;
+      carry += __shfl(incl, 63, 64);
+    }
+    if (lane == 0) s_off[0] = 0;
+    wave_sync();
+    const int total = s_off[nk];
+    const bool staged = total <= UCAP;
+    // pivot rows lie within one band of rows: offsets from the first fit 32 bits
+    const int kb0 = nk > 0 ? __builtin_amdgcn_readfirstlane(s_kb[0]) : p0;
+    const __amdgpu_buffer_rsrc_t rp = sc1_rsrc(LU + kb0);
+    bool aborted = false;
+    polls = 0;
+    if (staged) {
+      unsigned offs[UPL];
+      unsigned char pp[UPL];
+      plan_offsets(plan + pbase[w], total, s_kb, s_off, kb0, offs, pp);
+      int ta = 0;
+      bool waited = false;                        // pivot ta's row was just seen done
+      while (ta < nk) {
+        // tb: the first pivot from ta on whose row is not done (after a wait: ta alone,
+        // whose flag was read last, so its entries load without another flag round trip)
+        int tb = ta + 1;
+        if (!waited) {
+          tb = nk;
+          for (int t0 = ta; t0 < nk; t0 += 64) {
+            const int t = t0 + lane;
+            const bool ok = t >= nk || ld_flag(done + s_col[t]) == epoch;
+            const unsigned long long pend = __ballot(!ok);
+            if (pend) { tb = t0 + (int)__builtin_ctzll(pend); break; }
+          }
+          if (tb == ta) {
+            if (!wait_row(s_col[ta])) { aborted = true; break; }
+            waited = true;
+            continue;
+          }
+        }
+        waited = false;
+        // pivots [ta, tb) are done: loads of their rows only after the flags (an acquire
+        // at the IR level: nothing below is hoisted above the flag loads; no instruction)
+        __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "workgroup");
+        for (int t = ta + lane; t < tb; t += 64) s_dk[t] = ld_sc1(rp, (unsigned)(s_kb[t] - kb0), VT());
+        const int ua = s_off[ta], ub = s_off[tb];
+        VT uv[UPL];
+#pragma unroll
+        for (int c = 0; c < UPL / PCH; c++)
+          if (c * PCH * 64 < ub && (c + 1) * PCH * 64 > ua) {
+#pragma unroll
+            for (int m = c * PCH; m < (c + 1) * PCH; m++) uv[m] = ld_sc1(rp, offs[m], VT());
+          }
+#pragma unroll
+        for (int c = 0; c < UPL / PCH; c++)
+          if (c * PCH * 64 < ub && (c + 1) * PCH * 64 > ua) {
+#pragma unroll
+            for (int m = c * PCH; m < (c + 1) * PCH; m++) {
+              const int u = lane + 64 * m;
+              if (u >= ua && u < ub) {
+                s_upos[u] = pp[m];
+                s_uval[u] = uv[m];
+              }
+            }
+          }
+        wave_sync();
+        // eliminate pivots [ta, tb); pivot t's diagonal position and value and this lane's
+        // (at most two) entries of its row are read one pivot ahead
+        auto load_piv = [&](int t, int& pik, VT& dk, int& pa, VT& ua_, int& pb, VT& ub_) {
+          pik = s_pos[t];
+          dk = s_dk[t];
+          const int lo = s_off[t], hi = s_off[t + 1];
+          const int e0 = lo + lane, e1 = e0 + 64;
+          pa = 255; pb = 255; ua_ = (VT)0; ub_ = (VT)0;
+          if (e0 < hi) { pa = s_upos[e0]; ua_ = s_uval[e0]; }
+          if (e1 < hi) { pb = s_upos[e1]; ub_ = s_uval[e1]; }
+        };
+        int pik = 0, pa = 255, pb = 255;
+        VT dk = (VT)1, va = (VT)0, vb = (VT)0;
+        load_piv(ta, pik, dk, pa, va, pb, vb);
+        for (int t = ta; t < tb; t++) {
+          int pikn = 0, pan = 255, pbn = 255;
+          VT dkn = (VT)1, van = (VT)0, vbn = (VT)0;
+          if (t + 1 < tb) load_piv(t + 1, pikn, dkn, pan, van, pbn, vbn);
+          const VT a = s_val[pik];
+          const VT ca = pa != 255 ? s_val[pa] : (VT)0;
+          const VT cb = pb != 255 ? s_val[pb] : (VT)0;
+          const VT f = -a / dk;
+          if (pa != 255) { const VT prod = f * va; s_val[pa] = ca + prod; }
+          if (pb != 255) { const VT prod = f * vb; s_val[pb] = cb + prod; }
+          wave_sync();
+          if (lane == 0) s_val[pik] = f;           // the stored multiplier
+          wave_sync();
+          pik = pikn; dk = dkn; pa = pan; va = van; pb = pbn; vb = vbn;
+        }
+        ta = tb;
+      }
+    } else {
+      for (int t0 = 0; t0 < nk && !aborted; t0 += 64) {
+        const int t = t0 + lane;
+        const bool ok = t >= nk || ld_flag(done + s_col[t]) == epoch;
+        unsigned long long pending = __ballot(!ok);
+        while (pending && !aborted) {
+          if (!wait_row(s_col[t0 + (int)__builtin_ctzll(pending)])) aborted = true;
+          pending &= pending - 1;
+        }
+      }
+      if (!aborted) {
+        __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "workgroup");
+        for (int t = lane; t < nk; t += 64) s_dk[t] = ld_sc1(rp, (unsigned)(s_kb[t] - kb0), VT());
+        wave_sync();
+        for (int t = 0; t < nk; t++) {
+          const int k = s_col[t];
+          const int pik = s_pos[t];
+          const VT f = -s_val[pik] / s_dk[t];
+          const int ua = s_off[t], ub = s_off[t + 1];
+          for (int e = lane; e < ub - ua; e += 64) {
+            const int q = s_kb[t] + 1 + e;
+            const int cc = cols[q];
+            int pos = 255;
+            if (cc > k) {
+              int lo = 0, hi = wi - 1;
+              while (lo <= hi) {
+                const int mid = (lo + hi) >> 1;
+                const int v = s_col[mid];
+                if (v == cc) { pos = s_pos[mid]; break; }
+                if (v < cc) lo = mid + 1; else hi = mid - 1;
+              }
+            }
+            if (pos != 255) {
+              const VT prod = f * ld_sc1(rp, (unsigned)(q - kb0), VT());
+              s_val[pos] = s_val[pos] + prod;
+            }
+          }
+          wave_sync();
+          if (lane == 0) s_val[pik] = f;
+          wave_sync();
+        }
+      }
+    }
+    if (aborted) {
+      if (lane == 0) st_flag(state + FLOW_ABORT, 1);
+      break;
+    }
+    wave_sync();
+    const __amdgpu_buffer_rsrc_t ro = sc1_rsrc(LU + p0);
+    for (int t = lane; t < wi; t += 64) st_sc1(ro, (unsigned)t, s_val[t]);
+    const bool bad = fabs((double)s_val[0]) < 1.0e-20;
+    __builtin_amdgcn_s_waitcnt(0x0F70);            // vmcnt(0): the row is performed
+    __atomic_signal_fence(__ATOMIC_SEQ_CST);
+    if (lane == 0) {
+      st_flag(done + i, epoch);
+      if (bad) atomicOr(state + FLOW_ERR, 1);
+    }
+    wave_sync();                                   // LDS reused by the next row
+  }
 }
 
 template <class VT>
@@ -946,10 +1265,9 @@ void ilu_setup(Ilu& I, int n, const std::vector<int>& rp, const std::vector<int>
     I.fdesc.upload(desc, s);
     I.fpiv.upload(piv, s);
     I.fpbase.upload(pbase, s);
-    I.plan_t.alloc(std::max<long long>(pbase[n], 1));
-    I.plan_pos.alloc(std::max<long long>(pbase[n], 1));
+    I.plan.alloc(std::max<long long>(pbase[n], 1));
     k_ilu_plan<<<n, 64, 0, s>>>(I.fdesc.p, n, I.cols.p, I.srt_cols.p, I.srt_pos.p, I.fpiv.p,
-                                I.fpbase.p, I.plan_t.p, I.plan_pos.p);
+                                I.fpbase.p, I.plan.p);
     HIPCHECK(hipGetLastError());
   }
   // strip sweeps: a level of more rows than the staged sweep's row groups (512 threads, 4
@@ -993,7 +1311,48 @@ static void factor_t(Ctx& c, Ilu& I, const VT* A)
   HIPCHECK(hipMemcpyAsync(LU.p, A, sizeof(VT) * I.nnz, hipMemcpyDeviceToDevice, c.stream));
   int* err = reinterpret_cast<int*>(c.red.p + 8);
   HIPCHECK(hipMemsetAsync(err, 0, sizeof(int), c.stream));
-  const long long plan_n = (long long)I.plan_t.n;
+  const long long plan_n = (long long)I.plan.n;
+  const char* lv = getenv("RMX_ILU_LEVELS");
+  int flow_err = 0;
+  if (!(lv && atoi(lv) == 1)) {
+    // one persistent launch: rows in level order, pivot rows awaited by completion flags
+    if (I.done.n != (size_t)I.n) {
+      I.done.alloc(I.n);
+      HIPCHECK(hipMemsetAsync(I.done.p, 0, sizeof(int) * I.n, c.stream));
+      I.epoch = 0;
+    }
+    I.flow_state.alloc(4);
+    HIPCHECK(hipMemsetAsync(I.flow_state.p, 0, sizeof(int) * 4, c.stream));
+    if (++I.epoch == INT_MAX) {                   // epochs exhausted: clear the flags
+      HIPCHECK(hipMemsetAsync(I.done.p, 0, sizeof(int) * I.n, c.stream));
+      I.epoch = 1;
+    }
+    // one-wave workgroups, three per CU: enough rows in flight that a row is taken before
+    // its pivot rows finish (about three levels ahead on the 2M-triangle meander), while
+    // more waves lengthen every row (FP32 factorisation there: 640 waves 178 ms, 768
+    // 168 ms, 896 182 ms, 1024 245 ms)
+    hipDeviceProp_t prop;
+    HIPCHECK(hipGetDeviceProperties(&prop, c.device));
+    int per_cu = 0;
+    HIPCHECK(hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, (const void*)k_ilu_flow<VT>,
+                                                          64, 0));
+    per_cu = std::max(1, std::min(per_cu, 3));
+    int nwg = (int)std::min<long long>(I.n, (long long)per_cu * prop.multiProcessorCount);
+    if (const char* e = getenv("RMX_ILU_WAVES")) nwg = std::max(1, std::min(nwg, atoi(e)));
+    int spin = 1 << 22;
+    if (const char* e = getenv("RMX_ILU_SPIN")) spin = std::max(32, atoi(e));
+    k_ilu_flow<VT><<<nwg, 64, 0, c.stream>>>(I.fdesc.p, I.n, I.cols.p, I.srt_cols.p, I.srt_pos.p,
+                                             I.fpiv.p, I.fpbase.p, I.plan.p, LU.p, I.done.p,
+                                             I.epoch, I.flow_state.p, spin);
+    HIPCHECK(hipGetLastError());
+    int st[4] = {0, 0, 0, 0};
+    HIPCHECK(hipMemcpyAsync(st, I.flow_state.p, sizeof(int) * 4, hipMemcpyDeviceToHost, c.stream));
+    HIPCHECK(hipStreamSynchronize(c.stream));
+    if (st[FLOW_ABORT])
+      fail(RMX_ERR_HIP, "ILU(0): a row's wait for its pivot rows exceeded its bound (dataflow "
+                        "factorisation aborted)");
+    flow_err = st[FLOW_ERR];
+  } else
   for (size_t l = 0; l + 1 < I.flev.size(); l++) {
     const int a = I.flev[l], cnt = I.flev[l + 1] - a;
     const bool last = l + 2 >= I.flev.size();
@@ -1001,7 +1360,7 @@ static void factor_t(Ctx& c, Ilu& I, const VT* A)
     const int cntn = last ? 0 : I.flev[l + 2] - an;
     k_ilu_level<VT><<<cnt, 64, 0, c.stream>>>(I.fdesc.p + a, cnt, I.cols.p, I.srt_cols.p,
                                               I.srt_pos.p, I.fpiv.p, I.fpbase.p + a,
-                                              I.plan_t.p, I.plan_pos.p, LU.p, err,
+                                              I.plan.p, LU.p, err,
                                               I.fdesc.p + an, cntn, I.fpbase.p + an, plan_n);
   }
   HIPCHECK(hipGetLastError());
@@ -1018,7 +1377,7 @@ static void factor_t(Ctx& c, Ilu& I, const VT* A)
   int h_err = 0;
   HIPCHECK(hipMemcpyAsync(&h_err, err, sizeof(int), hipMemcpyDeviceToHost, c.stream));
   HIPCHECK(hipStreamSynchronize(c.stream));
-  if (h_err) fail(RMX_ERR_PIVOT, "ILU(0): pivot below 1e-20 (row repair is not supported)");
+  if (h_err || flow_err) fail(RMX_ERR_PIVOT, "ILU(0): pivot below 1e-20 (row repair is not supported)");
 }
 
 void ilu_factor(Ctx& c, Ilu& I, int prec)

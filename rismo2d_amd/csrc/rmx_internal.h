@@ -137,11 +137,15 @@ struct Ilu {
   DBuf<int4> fdesc;             // factorisation: per level position {row, start, width, lower}
   DBuf<int2> fpiv;              // per sorted lower entry: pivot row start, entries after diag
   DBuf<long long> fpbase;       // per level position: offset of its plan entries
-  DBuf<unsigned char> plan_t, plan_pos;   // per plan entry: pivot index, target position
+  DBuf<unsigned short> plan;    // per plan entry: pivot index | target position << 8
   SweepSet lo, up;
   DBuf<double> LU64, plo64, pup64, pdg64;
   DBuf<float> LU32, plo32, pup32, pdg32;
   bool fac64 = false, fac32 = false;
+  // dataflow factorisation: per row the epoch of the factorisation that finished it, and
+  // {row counter, error bits, abort flag, unused}
+  DBuf<int> done, flow_state;
+  int epoch = 0;
 };
 
 // FP32 SpMV layout.  Consecutive rows with one column set (the U, V and H rows of a node)
@@ -182,6 +186,19 @@ struct Coarse {
   std::vector<double> band;                   // LU factors, band storage (LAPACK layout)
   std::vector<int> ipiv;
   double* h_buf = nullptr;                    // pinned, max(nc, nslot) doubles
+  // Block cyclic reduction of R A P on the device, when it is block tridiagonal in strips
+  // (every coarse entry couples aggregates of one strip or of neighbouring strips): block
+  // i = the aggregates of strip i, padded to bs; factored on the host per factorisation,
+  // applied by level kernels.  Off (or a singular reduced block): the host band LU.
+  bool cr_ok = false, cr_fac = false;
+  int nb = 0, bs = 0, levels = 0;
+  std::vector<int> blk_start;                 // [nb + 1] first aggregate of every block
+  std::vector<long long> slot_dst;            // slot -> index in the level-0 block arrays
+  std::vector<long long> lev_elim, lev_keep;  // per level: offset of its matrices in mats
+  long long top_off = 0, mats_n = 0;
+  std::vector<double> h_mats;
+  DBuf<double> mats, w;                       // factor matrices; padded work vector
+  DBuf<int> d_blk_start;
   ~Coarse() { if (h_buf) (void)hipHostFree(h_buf); }
 };
 

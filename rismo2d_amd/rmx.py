@@ -213,13 +213,14 @@ class Context:
         """turb: $TM_TURBULENCE code; None takes the uploaded deck's."""
         if turb is None:
             turb = getattr(self, "deck_turb", 1)
+        p = Params(relax_method, dry_method, dry_limit, max_delta_uv, max_delta_s, conv_uv,
+                   conv_s, max_us, turb, cm, cd)
+        _check(lib().rmx_set_params(self.h, C.byref(p)))
+        # kept (and re-applied by upload_mesh) only once the library has accepted them
         self._params = dict(relax_method=relax_method, dry_method=dry_method,
                             dry_limit=dry_limit, max_delta_uv=max_delta_uv,
                             max_delta_s=max_delta_s, conv_uv=conv_uv, conv_s=conv_s,
                             max_us=max_us, cm=cm, cd=cd)
-        p = Params(relax_method, dry_method, dry_limit, max_delta_uv, max_delta_s, conv_uv,
-                   conv_s, max_us, turb, cm, cd)
-        _check(lib().rmx_set_params(self.h, C.byref(p)))
 
     TIME_KEYS = ("Uo", "Vo", "So", "dUdto", "dVdto", "dSdto", "dUdt", "dVdt", "dSdt")
 

@@ -280,14 +280,23 @@ def test_strip_ilu_matches_oracle(ctx, strips, segments):
         np.testing.assert_array_equal(ctx.ilu0_apply(B, rmx.F32), y)
 
 
+@pytest.mark.parametrize("coarse_solve", ["device", "host"])
 @pytest.mark.parametrize("strips,segments,cseg", [(4, 8, 2), (17, 4, 4), (1, 1, 1)])
-def test_two_level_apply_matches_dense_reference(ctx, strips, segments, cseg):
+def test_two_level_apply_matches_dense_reference(ctx, strips, segments, cseg, coarse_solve,
+                                                 monkeypatch):
     """Two-level apply z = z1 + M^-1 (r - A z1), z1 = P (R A P)^-1 R r, against the same
     formula in numpy: R A P summed and solved densely, M^-1 by the oracle's triangular
     solves of the block-restricted full ILU(0) factors.  FP64 and FP32 (matrix, factors and
-    coarse matrix from the FP32 values): 1e-9 of max|z|."""
+    coarse matrix from the FP32 values): 1e-9 of max|z|.  The coarse system is solved by
+    block cyclic reduction on the device (R A P is block tridiagonal in strips here) or,
+    with RMX_COARSE_HOST set when the coarse space is built, by the host band LU."""
     import scipy.sparse as sp
-    print(f"[gpu] two-level apply, {strips} strips x {segments} segments, {cseg} coarse", flush=True)
+    print(f"[gpu] two-level apply, {strips} strips x {segments} segments, {cseg} coarse, "
+          f"{coarse_solve} coarse solve", flush=True)
+    if coarse_solve == "host":
+        monkeypatch.setenv("RMX_COARSE_HOST", "1")
+    else:
+        monkeypatch.delenv("RMX_COARSE_HOST", raising=False)
     g = load_golden("channel200x20")
     ctx.upload_mesh(golden_deck(g))
     n = ctx.prepare()
@@ -446,6 +455,44 @@ def test_full_pattern_ilu_is_exact_lu(ctx):
     ctx.csr_upload(rp, cols, vals, b)
     ctx.ilu0_factor(rmx.F64)
     np.testing.assert_allclose(ctx.ilu0_apply(b), np.linalg.solve(M, b), rtol=1e-10)
+
+
+def _band_csr(n, half, seed):
+    """Full band |i - j| <= half (row width up to 2 half + 1 <= 128), diagonally dominant,
+    columns after the diagonal in random order."""
+    rng = np.random.default_rng(seed)
+    M = np.zeros((n, n))
+    for i in range(n):
+        lo, hi = max(0, i - half), min(n, i + half + 1)
+        M[i, lo:hi] = rng.standard_normal(hi - lo)
+        M[i, i] = np.abs(M[i, lo:hi]).sum() + 1.0
+    rp, cols, vals = [0], [], []
+    for i in range(n):
+        js = [i] + [j for j in rng.permutation(n) if j != i and abs(i - j) <= half]
+        cols += js
+        vals += [M[i, j] for j in js]
+        rp.append(len(cols))
+    return M, np.asarray(rp, np.int32), np.asarray(cols, np.int32), np.asarray(vals)
+
+
+def test_factorisation_of_rows_with_long_pivot_rows(ctx):
+    """Rows whose pivot rows hold more entries after their diagonals than the
+    factorisation stages in LDS (2048: 63 pivots x 63 entries in the band's middle) take
+    the branch that reads them from memory; the band's first rows stay staged.  ILU(0) on
+    a full band is its exact LU: the FP64 apply equals a dense solve, the FP64 factors the
+    oracle's, and the FP32 factors are bit-identical to the oracle's FP32 factorisation."""
+    print("[gpu] ILU(0) with pivot rows beyond the LDS staging (band 127)", flush=True)
+    M, rp, cols, vals = _band_csr(400, 63, 11)
+    b = np.random.default_rng(12).standard_normal(400)
+    ctx.csr_upload(rp, cols, vals, b)
+    ctx.ilu0_factor(rmx.F64)
+    np.testing.assert_allclose(ctx.ilu0_apply(b), np.linalg.solve(M, b), rtol=1e-10, atol=1e-12)
+    L64, _ = po.ilu0_factor(rp, cols, vals)
+    np.testing.assert_allclose(ctx.factors(rmx.F64), L64, rtol=1e-12, atol=1e-14)
+    ctx.convert(rmx.F32)
+    ctx.ilu0_factor(rmx.F32)
+    L32, _ = po.ilu0_factor(rp, cols, vals.astype(np.float32))
+    np.testing.assert_array_equal(ctx.factors(rmx.F32), L32)
 
 
 def test_error_paths(ctx):
