@@ -161,6 +161,20 @@ rmx_status rmx_assemble(rmx_ctx* ctx, int prec);
 /* B /= ||B||_2 and A /= ||B||_2: replaces CRSMAT::ScaleL2Norm (CRSMat.cpp:319). */
 rmx_status rmx_scale_l2(rmx_ctx* ctx, double* scale_out);
 
+/* No counterpart in the reference.  For callers that cannot allocate device memory (code
+ * under HIP graph capture, serving loops, harnesses that forbid hipMalloc): the scaling of
+ * rmx_scale_l2 (B /= ||B||_2, A /= ||B||_2, CRSMAT::ScaleL2Norm) on a system the caller holds
+ * in device memory: n rows, rowptr[n + 1] (device), the rowptr[n] values A (device, float
+ * for prec RMX_F32, double for RMX_F64) and B (device, double), all scaled in place;
+ * ||B||_2 is written to scale (device, one double).  Work is queued on `stream` (a
+ * hipStream_t, NULL: the null stream) of HIP device `device` and the call returns without
+ * waiting for it.  It allocates nothing: the reduction's partial sums live in a static
+ * device array of 520 doubles (4160 bytes) per GPU, held until the process ends, whatever n
+ * is, so no input size exceeds it; calls that overlap on one GPU share that array and race.
+ * The results are bit-identical to rmx_scale_l2 on the same values (same kernels). */
+rmx_status rmx_scale_l2_noalloc(int device, int n, const int* rowptr, int prec, void* A,
+                                double* B, double* scale, void* stream);
+
 /* Copy the FP32 matrix to or from the FP64 matrix (rounding to nearest). */
 rmx_status rmx_convert_matrix(rmx_ctx* ctx, int to_prec);
 

@@ -54,6 +54,7 @@ rmx_status rmx_create(int device, rmx_ctx** out)
   if (std::strncmp(prop.gcnArchName, "gfx950", 6) != 0)
     fail(RMX_ERR_NODEVICE, std::string("device is ") + prop.gcnArchName + ", not gfx950");
   HIPCHECK(hipSetDevice(device));
+  noalloc_setup(device);                // scratch address of rmx_scale_l2_noalloc on this GPU
   Ctx* c = new Ctx();
   c->device = device;
   HIPCHECK(hipStreamCreateWithFlags(&c->stream, hipStreamNonBlocking));
@@ -454,6 +455,22 @@ rmx_status rmx_scale_l2(rmx_ctx* p, double* scale)
   const double s = dev_scale_l2(c);
   if (scale) *scale = s;
   c.ilu.fac32 = c.ilu.fac64 = c.silu.fac32 = c.silu.fac64 = false;
+  API_END
+}
+
+rmx_status rmx_scale_l2_noalloc(int device, int n, const int* rowptr, int prec, void* A,
+                                double* B, double* scale, void* stream)
+{
+  API_BEGIN
+  if (n < 1 || !rowptr || !A || !B || !scale)
+    fail(RMX_ERR_ARG, "scale (no allocation): n < 1 or a null array");
+  if (prec != RMX_F32 && prec != RMX_F64)
+    fail(RMX_ERR_ARG, "scale (no allocation): prec must be RMX_F32 or RMX_F64");
+  int count = 0;
+  if (hipGetDeviceCount(&count) != hipSuccess || device < 0 || device >= count)
+    fail(RMX_ERR_NODEVICE, "no HIP device " + std::to_string(device));
+  HIPCHECK(hipSetDevice(device));
+  dev_scale_l2_noalloc(device, n, rowptr, prec, A, B, scale, static_cast<hipStream_t>(stream));
   API_END
 }
 

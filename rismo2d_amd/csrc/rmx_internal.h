@@ -300,6 +300,9 @@ void dev_prepare_step(Ctx& c);                 // friction (+ nodal vt when due)
 void dev_turbulence(Ctx& c);                   // friction + algebraic nodal vt, uu now
 void dev_assemble(Ctx& c, int prec, bool zero);
 double dev_scale_l2(Ctx& c);
+void noalloc_setup(int device);                // device address of the no-allocation scratch
+void dev_scale_l2_noalloc(int device, int n, const int* rowptr, int prec, void* A, double* B,
+                          double* scale, hipStream_t s);
 void dev_convert(Ctx& c, int to_prec);
 void dev_ilu0_factor(Ctx& c, int prec);
 void dev_ilu0_apply(Ctx& c, int prec, const double* b, double* x);   // device pointers

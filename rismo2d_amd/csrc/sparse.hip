@@ -8,6 +8,7 @@
 #include <cmath>
 #include <cstdio>
 #include <cstdlib>
+#include <mutex>
 
 #include "rmx_internal.h"
 
@@ -413,6 +414,69 @@ double dev_scale_l2(Ctx& c)
   HIPCHECK(hipMemcpyAsync(c.h_red, c.red.p, sizeof(double), hipMemcpyDeviceToHost, c.stream));
   HIPCHECK(hipStreamSynchronize(c.stream));
   return std::sqrt(c.h_red[0]);
+}
+
+// ---------------------------------------------------------------------------------------
+// allocation-free scaling of a caller-held device system (rmx_scale_l2_noalloc)
+//
+// The reduction's partial sums and the squared norm live in `scratch`, a file-scope device
+// array: one copy per GPU, held until the process ends, NOALLOC_DOUBLES doubles whatever
+// the system's size (one partial sum per workgroup of the fixed RED_BLOCKS grid).  Host
+// code never passes `scratch` itself to a kernel (in host code the name is only a
+// stand-in): its address on each device is looked up once, with that device current, and
+// kept in scratch_ptr.  Every call on one GPU uses that GPU's copy; nothing in it needs to
+// start at zero (every partial sum and the norm are written before they are read), and
+// calls that overlap on one GPU race on it.
+
+constexpr int NOALLOC_DOUBLES = RED_BLOCKS + 8;
+static __device__ __align__(256) double scratch[NOALLOC_DOUBLES];
+
+constexpr int NOALLOC_MAX_DEVICES = 64;
+static double* scratch_ptr[NOALLOC_MAX_DEVICES];
+static std::mutex scratch_mu;
+
+void noalloc_setup(int device)
+{
+  if (device < 0 || device >= NOALLOC_MAX_DEVICES)
+    fail(RMX_ERR_ARG, "no-allocation scratch: device index " + std::to_string(device) +
+                      " outside 0.." + std::to_string(NOALLOC_MAX_DEVICES - 1));
+  std::lock_guard<std::mutex> g(scratch_mu);
+  if (scratch_ptr[device]) return;
+  int cur = 0;
+  HIPCHECK(hipGetDevice(&cur));
+  HIPCHECK(hipSetDevice(device));
+  void* p = nullptr;
+  const hipError_t e = hipGetSymbolAddress(&p, HIP_SYMBOL(scratch));
+  (void)hipSetDevice(cur);
+  if (e != hipSuccess || !p)
+    fail(RMX_ERR_HIP, std::string("no-allocation scratch: hipGetSymbolAddress: ") +
+                      hipGetErrorString(e));
+  scratch_ptr[device] = static_cast<double*>(p);
+}
+
+__global__ void k_sqrt_to(const double* __restrict__ nrm2, double* __restrict__ out)
+{
+  if (blockIdx.x == 0 && threadIdx.x == 0) out[0] = sqrt(nrm2[0]);
+}
+
+// The kernels and grids of dev_scale_l2 (so the results are bit-identical to rmx_scale_l2 on
+// the same system), with the partial sums and the norm in the device's scratch copy.
+void dev_scale_l2_noalloc(int device, int n, const int* rowptr, int prec, void* A, double* B,
+                          double* scale, hipStream_t s)
+{
+  noalloc_setup(device);
+  double* const partial = scratch_ptr[device];         // RED_BLOCKS partial sums
+  double* const nrm2 = partial + RED_BLOCKS;           // ||B||^2
+  DotArgs a;
+  for (int k = 0; k < 8; k++) { a.x[k] = B; a.y[k] = B; }
+  k_dots_n<<<RED_BLOCKS, 256, 0, s>>>(n, 1, a, partial);
+  k_finish<<<1, 256, 0, s>>>(1, partial, nrm2);
+  if (prec == RMX_F32)
+    k_scale_rows<float><<<2048, 256, 0, s>>>(n, rowptr, static_cast<float*>(A), B, nrm2);
+  else
+    k_scale_rows<double><<<2048, 256, 0, s>>>(n, rowptr, static_cast<double*>(A), B, nrm2);
+  k_sqrt_to<<<1, 64, 0, s>>>(nrm2, scale);
+  HIPCHECK(hipGetLastError());
 }
 
 __global__ void k_d2f(long long n, const double* __restrict__ a, float* __restrict__ b)
