@@ -138,6 +138,11 @@ rmx_status rmx_set_inflow(rmx_ctx* ctx, double Q, double S_in, int hydrograph);
 rmx_status rmx_hydrograph(int n, const double* t, const double* Q, const double* S, double time,
                           double* Qout, double* Sout);
 rmx_status rmx_time_commit(rmx_ctx* ctx);
+/* The slip condition at the end of the flow cycle (GRID::SetSlipFlow, SlipFlow.cpp:43-78,
+ * called at the end of EQS_UVS2D::Execute, EqsUVS2D.cpp:1078): at rotated nodes (walls,
+ * inlet and outlet) the velocity and its time derivative keep only their component along
+ * the first column of the node's rotation.  Needs rmx_prepare. */
+rmx_status rmx_slip_flow(rmx_ctx* ctx);
 /* Host copies of the time-integration state, 9 [np] arrays (NULL entries skipped):
  * Uo, Vo, So, dUdt_o, dVdt_o, dSdt_o (previous time step), dUdt, dVdt, dSdt (current).
  * set = 1 uploads them, 0 downloads. */
@@ -327,6 +332,25 @@ typedef struct {
   double spmv_bytes;
 } rmx_kernel_times;
 rmx_status rmx_profile_kernels(rmx_ctx* ctx, int prec, int reps, rmx_kernel_times* out);
+
+/* Launch timing per kernel class over any sequence of calls (no reference counterpart):
+ * with timing enabled, every launch of the classes below is bracketed by HIP events on the
+ * context stream (a few microseconds of recording per launch); the sums are read back when
+ * the stream synchronises.  Classes: 0 ILU(0) factorisation, 1 lower and 2 upper triangular
+ * sweep, 3 Krylov SpMV (the storage precision of the solve, product or residual form),
+ * 4 coarse correction (restriction, coarse solve, A P residual), 5 Gram-Schmidt (dots and
+ * update), 6 FP64 residual of the refinement.  bytes: the algorithmic bytes of each launch
+ * summed (factorisation 3 nnz (v + 4); sweep: kept entries (v + 4) + 20 or 24 bytes per row;
+ * SpMV: the bytes of the layout it ran on; Gram-Schmidt: the basis vectors read, FP32 or FP64,
+ * and the vector read and written; coarse: the vectors, maps and A P entries it reads). */
+typedef struct {
+  double seconds[8];
+  double bytes[8];
+  long long launches[8];
+} rmx_kernel_stats;
+/* on != 0 resets the sums and starts timing; 0 stops it (the sums stay readable). */
+rmx_status rmx_kernel_stats_enable(rmx_ctx* ctx, int on);
+rmx_status rmx_kernel_stats_get(rmx_ctx* ctx, rmx_kernel_stats* out);
 
 #ifdef __cplusplus
 }

@@ -30,6 +30,46 @@ using namespace rmx;
   }                                                                                      \
   return RMX_OK;
 
+Ctx::~Ctx()
+{
+  for (hipEvent_t e : ktm.ev) (void)hipEventDestroy(e);
+}
+
+int rmx::kt_begin(Ctx& c, int cls, double bytes)
+{
+  KTimer& T = c.ktm;
+  if (!T.on) return -1;
+  if (2 * (T.used + 1) > T.ev.size()) {          // pool full: wait, read back, reuse
+    HIPCHECK(hipStreamSynchronize(c.stream));
+    kt_collect(c);
+  }
+  const size_t k = T.used++;
+  T.cls[k] = cls;
+  T.pbytes[k] = bytes;
+  HIPCHECK(hipEventRecord(T.ev[2 * k], c.stream));
+  return (int)k;
+}
+
+void rmx::kt_end(Ctx& c, int k)
+{
+  if (k < 0) return;
+  HIPCHECK(hipEventRecord(c.ktm.ev[2 * (size_t)k + 1], c.stream));
+}
+
+void rmx::kt_collect(Ctx& c)
+{
+  KTimer& T = c.ktm;
+  for (size_t k = 0; k < T.used; k++) {
+    float ms = 0.0f;
+    HIPCHECK(hipEventSynchronize(T.ev[2 * k + 1]));
+    HIPCHECK(hipEventElapsedTime(&ms, T.ev[2 * k], T.ev[2 * k + 1]));
+    T.sec[T.cls[k]] += ms * 1e-3;
+    T.bytes[T.cls[k]] += T.pbytes[k];
+    T.cnt[T.cls[k]]++;
+  }
+  T.used = 0;
+}
+
 static Ctx* need(rmx_ctx* p)
 {
   if (!p) fail(RMX_ERR_ARG, "null context");
@@ -351,6 +391,16 @@ rmx_status rmx_hydrograph(int n, const double* t, const double* Q, const double*
   }
   *Qout = q;
   *Sout = s;
+  API_END
+}
+
+rmx_status rmx_slip_flow(rmx_ctx* p)
+{
+  API_BEGIN
+  Ctx& c = *need(p);
+  if (!c.prepared || c.csr_only) fail(RMX_ERR_ARG, "slip flow: call rmx_prepare first");
+  dev_slip_flow(c);
+  HIPCHECK(hipStreamSynchronize(c.stream));
   API_END
 }
 
@@ -826,6 +876,41 @@ rmx_status rmx_newton_step(rmx_ctx* p, const rmx_krylov_opts* o, rmx_step_stats*
   auto t5 = now();
   st->t_prepare = sec(t0, t1); st->t_assemble = sec(t1, t2); st->t_factor = sec(t2, t3);
   st->t_solve = sec(t3, t4); st->t_update = sec(t4, t5);
+  API_END
+}
+
+rmx_status rmx_kernel_stats_enable(rmx_ctx* p, int on)
+{
+  API_BEGIN
+  Ctx& c = *need(p);
+  KTimer& T = c.ktm;
+  HIPCHECK(hipStreamSynchronize(c.stream));
+  kt_collect(c);
+  for (int k = 0; k < KT_N; k++) { T.sec[k] = 0.0; T.bytes[k] = 0.0; T.cnt[k] = 0; }
+  if (on && T.ev.empty()) {
+    const size_t pairs = 8192;
+    T.ev.resize(2 * pairs);
+    for (auto& e : T.ev) HIPCHECK(hipEventCreate(&e));
+    T.cls.resize(pairs);
+    T.pbytes.resize(pairs);
+  }
+  T.on = on != 0;
+  API_END
+}
+
+rmx_status rmx_kernel_stats_get(rmx_ctx* p, rmx_kernel_stats* out)
+{
+  API_BEGIN
+  Ctx& c = *need(p);
+  if (!out) fail(RMX_ERR_ARG, "null output");
+  HIPCHECK(hipStreamSynchronize(c.stream));
+  kt_collect(c);
+  for (int k = 0; k < KT_N && k < 8; k++) {
+    out->seconds[k] = c.ktm.sec[k];
+    out->bytes[k] = c.ktm.bytes[k];
+    out->launches[k] = c.ktm.cnt[k];
+  }
+  for (int k = KT_N; k < 8; k++) { out->seconds[k] = 0.0; out->bytes[k] = 0.0; out->launches[k] = 0; }
   API_END
 }
 

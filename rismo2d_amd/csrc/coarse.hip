@@ -601,6 +601,11 @@ void coarse_apply(Ctx& c, int prec, const double* r, double* z)
   Coarse& C = c.coarse;
   if (!C.fac || C.fac_prec != prec) fail(RMX_ERR_ARG, "coarse space not factored in this precision");
   const int n = c.n, g = (n + 255) / 256;
+  // bytes: restriction (row lists, r), coarse solve (its matrices), A P residual (A P
+  // entries and aggregates, r, t)
+  const int kt = kt_begin(c, KT_COARSE, 4.0 * (C.nc + 1) + 12.0 * n + 8.0 * C.nc +
+                                        (C.cr_fac ? 8.0 * (double)C.mats_n : 0.0) +
+                                        12.0 * (double)C.nap + 4.0 * (n + 1) + 16.0 * n);
   k_restrict<<<(C.nc * 64 + 255) / 256, 256, 0, c.stream>>>(C.nc, C.agg_ptr.p, C.agg_rows.p, r, C.rc.p);
   if (C.cr_fac) {
     cr_apply(c, C);
@@ -611,6 +616,7 @@ void coarse_apply(Ctx& c, int prec, const double* r, double* z)
     HIPCHECK(hipMemcpyAsync(C.zc.p, C.h_buf, sizeof(double) * C.nc, hipMemcpyHostToDevice, c.stream));
   }
   k_ap_resid<<<g, 256, 0, c.stream>>>(n, C.ap_ptr.p, C.ap_agg.p, C.apv.p, C.zc.p, r, C.t.p);
+  kt_end(c, kt);
   if (!ilu_apply(c, c.silu, prec, C.t.p, z, nullptr, C.agg.p, C.zc.p))
     k_add_prolong<<<g, 256, 0, c.stream>>>(n, C.agg.p, C.zc.p, z);
   HIPCHECK(hipGetLastError());

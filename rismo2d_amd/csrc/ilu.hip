@@ -1341,9 +1341,11 @@ static void factor_t(Ctx& c, Ilu& I, const VT* A)
     if (const char* e = getenv("RMX_ILU_WAVES")) nwg = std::max(1, std::min(nwg, atoi(e)));
     int spin = 1 << 22;
     if (const char* e = getenv("RMX_ILU_SPIN")) spin = std::max(32, atoi(e));
+    const int kt = kt_begin(c, KT_FACTOR, 3.0 * (double)I.nnz * (sizeof(VT) + 4));
     k_ilu_flow<VT><<<nwg, 64, 0, c.stream>>>(I.fdesc.p, I.n, I.cols.p, I.srt_cols.p, I.srt_pos.p,
                                              I.fpiv.p, I.fpbase.p, I.plan.p, LU.p, I.done.p,
                                              I.epoch, I.flow_state.p, spin);
+    kt_end(c, kt);
     HIPCHECK(hipGetLastError());
     int st[4] = {0, 0, 0, 0};
     HIPCHECK(hipMemcpyAsync(st, I.flow_state.p, sizeof(int) * 4, hipMemcpyDeviceToHost, c.stream));
@@ -1352,7 +1354,8 @@ static void factor_t(Ctx& c, Ilu& I, const VT* A)
       fail(RMX_ERR_HIP, "ILU(0): a row's wait for its pivot rows exceeded its bound (dataflow "
                         "factorisation aborted)");
     flow_err = st[FLOW_ERR];
-  } else
+  } else {
+  const int kt = kt_begin(c, KT_FACTOR, 3.0 * (double)I.nnz * (sizeof(VT) + 4));
   for (size_t l = 0; l + 1 < I.flev.size(); l++) {
     const int a = I.flev[l], cnt = I.flev[l + 1] - a;
     const bool last = l + 2 >= I.flev.size();
@@ -1362,6 +1365,8 @@ static void factor_t(Ctx& c, Ilu& I, const VT* A)
                                               I.srt_pos.p, I.fpiv.p, I.fpbase.p + a,
                                               I.plan.p, LU.p, err,
                                               I.fdesc.p + an, cntn, I.fpbase.p + an, plan_n);
+  }
+  kt_end(c, kt);
   }
   HIPCHECK(hipGetLastError());
   DBuf<VT>& pl = PLbuf<VT>(I);
@@ -1404,6 +1409,8 @@ static void launch_sweeps(Ctx& c, Ilu& I, const double* b, double* x, hipEvent_t
   const int nb = I.nblk;
   const int nt = STAGE_THREADS;
   if (ev) HIPCHECK(hipEventRecord(ev[0], c.stream));
+  const double vb = (double)sizeof(VT) + 4.0;
+  int kt = kt_begin(c, KT_SWEEP_LO, (double)I.lo.ne * vb + 20.0 * I.n);
   if constexpr (V == 1)
     k_sweep_stage<VT, true, 4, 12><<<nb, nt, 0, c.stream>>>(I.lo.blev.p, I.lo.bpos.p,
         I.lo.lev_ptr.p, I.lo.prs.p, I.lo.pcol.p, PLbuf<VT>(I).p, nullptr, I.lo.prow.p, b, nullptr,
@@ -1412,7 +1419,9 @@ static void launch_sweeps(Ctx& c, Ilu& I, const double* b, double* x, hipEvent_t
     k_sweep<VT, true, SG, SKR><<<nb, 1024, 0, c.stream>>>(I.lo.blev.p, I.lo.bpos.p,
         I.lo.lev_ptr.p, I.lo.prs.p, I.lo.pcol.p, PLbuf<VT>(I).p, nullptr, I.lo.prow.p, b, nullptr,
         nullptr, I.lo.xp.p);
+  kt_end(c, kt);
   if (ev) HIPCHECK(hipEventRecord(ev[1], c.stream));
+  kt = kt_begin(c, KT_SWEEP_UP, (double)I.up.ne * vb + 24.0 * I.n);
   if constexpr (V == 1)
     k_sweep_stage<VT, false, 4, 6><<<nb, nt, 0, c.stream>>>(I.up.blev.p, I.up.bpos.p,
         I.up.lev_ptr.p, I.up.prs.p, I.up.pcol.p, PUbuf<VT>(I).p, PDbuf<VT>(I).p, I.fwd2bwd.p,
@@ -1421,6 +1430,7 @@ static void launch_sweeps(Ctx& c, Ilu& I, const double* b, double* x, hipEvent_t
     k_sweep<VT, false, SG, SKR><<<nb, 1024, 0, c.stream>>>(I.up.blev.p, I.up.bpos.p,
         I.up.lev_ptr.p, I.up.prs.p, I.up.pcol.p, PUbuf<VT>(I).p, PDbuf<VT>(I).p, I.fwd2bwd.p,
         I.lo.xp.p, I.up.prow.p, x, I.up.xp.p);
+  kt_end(c, kt);
   if (ev) HIPCHECK(hipEventRecord(ev[2], c.stream));
   HIPCHECK(hipGetLastError());
 }

@@ -424,6 +424,7 @@ static bool fgmres(Ctx& c, int prec, int pk, const double* b, double* x, int m, 
       precond(c, prec, pk, vi, zi);
       dev_spmv(c, prec, zi, vi1);
       // classical Gram-Schmidt: all dots, then all updates
+      const int kt = kt_begin(c, KT_GS, b32 ? 8.0 * i1 * ld32 + 24.0 * n : 16.0 * i1 * n + 40.0 * n);
       if (b32) {
         k_mdot32<<<MDOT_BLOCKS, 256, 0, c.stream>>>(n, i1, V32, ld32, vi1, partial);
         k_mfinish<<<i1, 256, 0, c.stream>>>(partial, sc + 16);
@@ -436,6 +437,7 @@ static bool fgmres(Ctx& c, int prec, int pk, const double* b, double* x, int m, 
         const double* xs[1] = { vi1 };
         dev_dots(c, 1, xs, xs, sc + S_B2);
       }
+      kt_end(c, kt);
       HIPCHECK(hipMemcpyAsync(c.h_red, sc, sizeof(double) * (16 + i1), hipMemcpyDeviceToHost, c.stream));
       HIPCHECK(hipStreamSynchronize(c.stream));
       for (int j = 0; j <= i; j++) hh[i][j] = c.h_red[16 + j];

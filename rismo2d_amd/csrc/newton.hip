@@ -137,6 +137,32 @@ __global__ void k_timegrad(int np, double iTheta, double thdt, const double* __r
   dSdt[n] = iTheta * dSdto[n] + thdt * (S[n] - So[n]);
 }
 
+// GRID::SetSlipFlow (SlipFlow.cpp:43-78): at rotated nodes the velocity and its time
+// derivative keep only their component along the first column of the rotation (the
+// tangent of a wall, the inflow or outflow normal of an open boundary)
+__global__ void k_slip_flow(int np, const uint32_t* __restrict__ flag, const double* __restrict__ rotm,
+                            double* __restrict__ U, double* __restrict__ V,
+                            double* __restrict__ dUdt, double* __restrict__ dVdt)
+{
+  const int n = blockIdx.x * blockDim.x + threadIdx.x;
+  if (n >= np || !(flag[n] & dRotat)) return;
+  const double r00 = rotm[4 * (size_t)n], r10 = rotm[4 * (size_t)n + 2];
+  double t = r00 * U[n] + r10 * V[n];
+  U[n] = r00 * t;
+  V[n] = r10 * t;
+  t = r00 * dUdt[n] + r10 * dVdt[n];
+  dUdt[n] = r00 * t;
+  dVdt[n] = r10 * t;
+}
+
+void dev_slip_flow(Ctx& c)
+{
+  const int np = c.hm.np;
+  k_slip_flow<<<(np + 255) / 256, 256, 0, c.stream>>>(np, c.dflag.p, c.rotm.p, c.U.p, c.V.p,
+                                                      c.dUdt.p, c.dVdt.p);
+  HIPCHECK(hipGetLastError());
+}
+
 void dev_time_init(Ctx& c)
 {
   const size_t b = sizeof(double) * (size_t)c.hm.np;

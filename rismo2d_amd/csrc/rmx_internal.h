@@ -219,6 +219,19 @@ struct Mpnn {
   int use_net = -1;                           // per system: -1 undecided, 0 dropped, 1 used
 };
 
+// Launch timing per kernel class (opt-in): a pair of HIP events on the context stream
+// around every launch of the class, read back when the stream has synchronised.
+enum { KT_FACTOR = 0, KT_SWEEP_LO, KT_SWEEP_UP, KT_SPMV, KT_COARSE, KT_GS, KT_RESID64, KT_N };
+struct KTimer {
+  bool on = false;
+  std::vector<hipEvent_t> ev;                 // event pool: pair k = ev[2k], ev[2k+1]
+  std::vector<int> cls;                       // class of every pair in use
+  std::vector<double> pbytes;                 // algorithmic bytes of every pair in use
+  size_t used = 0;
+  double sec[KT_N] = {}, bytes[KT_N] = {};
+  long long cnt[KT_N] = {};
+};
+
 struct Ctx {
   int device = 0;
   hipStream_t stream = nullptr;
@@ -282,7 +295,15 @@ struct Ctx {
   int basis_m = 0;
   // kernel timing of the last profiled solve
   rmx_kernel_times kt{};
+  KTimer ktm;                                 // launch timing per kernel class
+  ~Ctx();
 };
+
+// launch timing: kt_begin returns the pair to end (-1 when timing is off); kt_collect reads
+// the finished pairs (the stream must have synchronised)
+int kt_begin(Ctx& c, int cls, double bytes);
+void kt_end(Ctx& c, int k);
+void kt_collect(Ctx& c);
 
 // host model set-up (model.cpp)
 void host_mesh(HostModel& hm, const rmx_mesh_desc& d);
@@ -310,6 +331,7 @@ void dev_spmv(Ctx& c, int prec, const double* x, double* r);        // device po
 void dev_solve(Ctx& c, const rmx_krylov_opts& o, rmx_krylov_out& out);
 void dev_newton_update(Ctx& c, rmx_step_stats& st);
 void dev_time_init(Ctx& c);                    // previous state := state, derivatives 0
+void dev_slip_flow(Ctx& c);                    // GRID::SetSlipFlow
 void dev_time_step(Ctx& c, int first);         // EQS_UVS2D::Predict
 void dev_time_commit(Ctx& c);                  // previous state := state
 void dev_element_coefs(Ctx& c, int first, int count, double* K, double* F);

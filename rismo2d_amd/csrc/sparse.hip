@@ -270,29 +270,44 @@ double spmv_f32_bytes(const Ctx& c)
   return (double)c.nnz * 4.0 + 2.0 * S.noff + 20.0 * S.ng + 16.0 * c.n;
 }
 
+// algorithmic bytes of one product in the layout it runs on (+ 8n for b in residual form)
+static double spmv_bytes(const Ctx& c, int prec, bool resid)
+{
+  double b;
+  if (prec == RMX_F32 && c.sg.on) b = spmv_f32_bytes(c);
+  else b = (double)c.nnz * (prec == RMX_F32 ? 8.0 : 12.0) + 4.0 * (c.n + 1) + 16.0 * c.n;
+  return b + (resid ? 8.0 * c.n : 0.0);
+}
+
 void dev_spmv(Ctx& c, int prec, const double* x, double* y)
 {
+  const int kt = kt_begin(c, KT_SPMV, spmv_bytes(c, prec, false));
   if (prec == RMX_F32) {
     if (c.sg.on) launch_spmv_grp<false>(c, x, nullptr, y);
     else launch_spmv<float, false>(c, c.A32.p, x, nullptr, y);
   } else {
     launch_spmv<double, false>(c, c.A64.p, x, nullptr, y);
   }
+  kt_end(c, kt);
 }
 
 void dev_resid_p(Ctx& c, int prec, const double* x, const double* b, double* r)
 {
+  const int kt = kt_begin(c, KT_SPMV, spmv_bytes(c, prec, true));
   if (prec == RMX_F32) {
     if (c.sg.on) launch_spmv_grp<true>(c, x, b, r);
     else launch_spmv<float, true>(c, c.A32.p, x, b, r);
   } else {
     launch_spmv<double, true>(c, c.A64.p, x, b, r);
   }
+  kt_end(c, kt);
 }
 
 void dev_resid(Ctx& c, const double* x, const double* b, double* r)
 {
+  const int kt = kt_begin(c, KT_RESID64, spmv_bytes(c, RMX_F64, true));
   launch_spmv<double, true>(c, c.A64.p, x, b, r);
+  kt_end(c, kt);
 }
 
 // ---------------------------------------------------------------------------------------

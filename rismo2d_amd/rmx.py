@@ -80,6 +80,15 @@ class KernelTimes(C.Structure):
                 ("n_spmv", C.c_longlong), ("n_ilu", C.c_longlong), ("spmv_bytes", C.c_double)]
 
 
+class KernelStats(C.Structure):
+    _fields_ = [("seconds", C.c_double * 8), ("bytes", C.c_double * 8),
+                ("launches", C.c_longlong * 8)]
+
+
+# kernel classes of rmx_kernel_stats
+KERNEL_CLASSES = ("ilu_factor", "sweep_lower", "sweep_upper", "spmv", "coarse", "gram_schmidt",
+                  "resid_fp64")
+
 _LIB = None
 
 
@@ -105,6 +114,7 @@ def lib():
             "rmx_n_model_elems": [_v, _i], "rmx_element_coefs": [_v, C.c_int, C.c_int, _d, _d],
             "rmx_turbulence": [_v, _d, _d],
             "rmx_time_step": [_v, C.c_double, C.c_double, C.c_int], "rmx_time_commit": [_v],
+            "rmx_slip_flow": [_v],
             "rmx_time_state": [_v, C.c_int, C.POINTER(_d)],
             "rmx_set_inflow": [_v, C.c_double, C.c_double, C.c_int],
             "rmx_hydrograph": [C.c_int, _d, _d, _d, C.c_double, _d, _d],
@@ -128,6 +138,8 @@ def lib():
             "rmx_newton_update": [_v, C.POINTER(StepStats)],
             "rmx_newton_step": [_v, C.POINTER(KrylovOpts), C.POINTER(StepStats)],
             "rmx_profile_kernels": [_v, C.c_int, C.c_int, C.POINTER(KernelTimes)],
+            "rmx_kernel_stats_enable": [_v, C.c_int],
+            "rmx_kernel_stats_get": [_v, C.POINTER(KernelStats)],
         }
         for name, args in sig.items():
             fn = getattr(L, name)
@@ -227,6 +239,10 @@ class Context:
     def time_step(self, dt, theta, first=False):
         """Start a flow time step (dt <= 0: stationary flow)."""
         _check(lib().rmx_time_step(self.h, dt, theta, int(first)))
+
+    def slip_flow(self):
+        """The slip condition at rotated nodes that ends the flow cycle (GRID::SetSlipFlow)."""
+        _check(lib().rmx_slip_flow(self.h))
 
     def time_commit(self):
         _check(lib().rmx_time_commit(self.h))
@@ -463,6 +479,16 @@ class Context:
 
     def prepare_sizes(self):
         return self.prepare()
+
+    def kernel_stats(self, on=None):
+        """Launch timing per kernel class: on=True resets and starts it, on=False stops it;
+        returns {class: (seconds, algorithmic bytes, launches)} summed since the reset."""
+        if on is not None:
+            _check(lib().rmx_kernel_stats_enable(self.h, int(bool(on))))
+        ks = KernelStats()
+        _check(lib().rmx_kernel_stats_get(self.h, C.byref(ks)))
+        return {name: (ks.seconds[k], ks.bytes[k], ks.launches[k])
+                for k, name in enumerate(KERNEL_CLASSES)}
 
     def profile_kernels(self, prec=F64, reps=10):
         kt = KernelTimes()

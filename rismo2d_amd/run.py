@@ -5,7 +5,7 @@ EQS_UVS2D::Execute, EqsUVS2D.cpp:71-1096). For each time step of ``$TM_STEPS``:
 - unsteady (``$TM_STATIONARY 0``): Predict with dt and θ; QTINLET takes the inflow at the
   step's end (EqsUVS2D.cpp:318-320);
 - Newton steps until the reference's convergence test holds (``nrconv``) or
-  ``$TM_MAXITER`` steps;
+  ``$TM_MAXITER`` steps, then the slip condition at rotated nodes;
 - the step is committed as the old state of the next one.
 
 A stationary deck solves its flow once.
@@ -62,6 +62,8 @@ def run_reference_deck(input_file: str, ctx: rmx.Context | None = None, solve: d
                     on_step(t, k, st)
                 if st.nrconv or k >= maxiter:
                     break
+            # EQS_UVS2D::Execute ends with the slip condition (EqsUVS2D.cpp:1078)
+            c.slip_flow()
             counts.append(k)
         U, V, S, z = c.state()
         return d, run, counts, (np.asarray(U), np.asarray(V), np.asarray(S), np.asarray(z))

@@ -37,6 +37,21 @@ the reference's input format, runs the reference and stores what the harness rec
                     bed noise sigma 0.05 m / 20 m, seed 0): numbering, per-step scale and
                     correction, BiCGStab iteration counts from the report, final state,
                     and a SHA-256 of the step-0 pattern.
+* ``vegchannel200x20.npz``  the same channel with emergent vegetation (Lindner) on 30 % of
+                    the elements of the outer 30 % of the width on either bank (stem
+                    diameter U[0.01, 0.05] m, density U[0.5, 5] per m2), same records.
+* ``meander40x64.npz``  a 40 x 64-cell reach of the meandering-channel generator (5,120
+                    triangles, 125 m of the 10 km valley, the full 900 m section): numbering,
+                    pattern SHA-256, the first two Newton steps' scale and correction and the
+                    state after them (without drying and rewetting the floodplains'
+                    steady flow does not converge, so the golden stops at the two steps the
+                    reference's BiCGStab solves to its tolerance).
+* ``flood200x6.npz``  a 2 km channel (200 x 6 cells) under the flood hydrograph 100 ->
+                    1500 m3/s over 6,000 s, its first 10 implicit time steps of 30 s (theta
+                    0.7), anisotropic (Elder) eddy viscosity recomputed every Newton step: the
+                    Newton steps per time step and the state at the end of every time step.
+* ``cap20x4.npz``   the 20 x 4 triangle channel with a binding relaxation cap ($RELAX 2
+                    with maxDeltaUV 0.02, maxDeltaS 0.005): every Newton step's records.
 
 The reference ships no test vectors of its own; these are the only golden data.
 Usage:  python tests/golden/make_golden.py [deck names]
@@ -88,6 +103,12 @@ DECKS = {
                        turb_kdest=(0.4, 0.8), turb_estx=(0.4, 1.2), turb_esty=(0.05, 0.3)),
     "wetdry20x6": dict(nx=20, ny=6, length=100.0, width=30.0, noise_sigma=0.05, noise_corr=5.0,
                        Q=10.0),
+    "vegchannel200x20": dict(nx=200, ny=20, veg_frac=0.3, veg_band=0.3),
+    "meander40x64": dict(gen="meander", n_along=40, n_across=64, reach=125.0),
+    "flood200x6": dict(nx=200, ny=6, length=2000.0, turb=211, turb_estx=(0.4, 1.2),
+                       turb_esty=(0.05, 0.3)),
+    "cap20x4": dict(nx=20, ny=4, length=100.0, width=20.0, noise_sigma=0.05, noise_corr=5.0,
+                    Q=10.0),
     "rewet20x6": dict(nx=20, ny=6, length=100.0, width=30.0, noise_sigma=0.05, noise_corr=5.0,
                       Q=10.0),
 }
@@ -110,7 +131,13 @@ INIT_S = {
 # inlet hydrographs (QTINLET), rows (t s, Q m3/s, S m)
 HYDRO = {
     "hydro20x4": [[0.0, 10.0, 2.10], [60.0, 16.0, 2.16], [120.0, 8.0, 2.04]],
+    "flood200x6": [[0.0, 100.0, 2.2], [6000.0, 1500.0, 2.2]],
 }
+
+# decks kept to compact records (no per-step matrices or element records); "steps": the
+# per-step corrections, "timesteps": the committed state of every time step instead
+COMPACT = {"channel200x20": "steps", "vegchannel200x20": "steps", "meander40x64": "steps",
+           "flood200x6": "timesteps"}
 
 # time integration of the decks that are not one stationary flow step, and the Newton step
 # whose element matrices are recorded (one with non-zero time derivatives)
@@ -120,6 +147,9 @@ RUNS = {
     "hydro20x4": dict(tm_steps=4, dt=30.0, theta=0.7, stationary=0, elem_step=5),
     "wetdry20x6": dict(dryrew="2 3 0.001 0.005 100 5", elem_step=1),
     "rewet20x6": dict(dryrew="2 2 0.001 0.005 100 1", maxiter=5, elem_step=3),
+    "meander40x64": dict(maxiter=2, solver="1 6 1 0 60 2000 1.0e-10"),
+    "flood200x6": dict(tm_steps=10, dt=30.0, theta=0.7, stationary=0),
+    "cap20x4": dict(relax="2 0.001 0.1 0.02 0.005 0.1", maxiter=40),
 }
 
 
@@ -169,9 +199,9 @@ def main():
                    check=True)
     names = sys.argv[1:] or list(DECKS)
     for name in names:
-        kw = DECKS[name]
+        kw = dict(DECKS[name])
         run = RUNS.get(name, {})
-        d = dk.channel(**kw)
+        d = dk.meander(**kw) if kw.pop("gen", "channel") == "meander" else dk.channel(**kw)
         if name in HYDRO:
             d.hydrograph = np.asarray(HYDRO[name], np.float64)
         if name in BANKS:
@@ -194,16 +224,25 @@ def main():
         for key in ("tm_steps", "dt", "theta", "stationary", "elem_step", "maxiter"):
             if key in run:
                 out[key] = run[key]
+        if "relax" in run:
+            # $RELAX method relaxMin relaxMax maxDeltaUV maxDeltaS maxDeltaKD
+            out["relax"] = np.asarray([float(v) for v in run["relax"].split()])
         if "dryrew" in run:
             # $DRYREW method freq dryLimit rewetLimit rewetPasses countDown
             out["dryrew"] = np.asarray([float(v) for v in run["dryrew"].split()])
         out["bicgstab_iters"] = np.asarray(iters, np.int64)
-        full = name != "channel200x20"
+        full = name not in COMPACT
         for k, st in enumerate(steps):
             p = f"s{k}_"
             out[p + "scale"] = st["scale"]
-            out[p + "xsol"] = st["xsol"]
+            if COMPACT.get(name) != "timesteps":
+                out[p + "xsol"] = st["xsol"]
             out[p + "tstep"] = st["tstep"]
+            if COMPACT.get(name) == "timesteps" and k > 0 and st["tstep"] != steps[k - 1]["tstep"]:
+                # the previous time step's committed state, as the next one starts
+                t = int(steps[k - 1]["tstep"])
+                for v in ("U", "V", "S"):
+                    out[f"t{t}_{v}"] = st["state"][v + "o"]
             if full:
                 out.update(state_arrays(p, st["state"]))
                 if "dryrew" in run:

@@ -30,10 +30,14 @@ TIME_FIELDS = ("dUdt", "dVdt", "dSdt", "Uo", "Vo", "So", "dUdto", "dVdto", "dSdt
 
 @pytest.mark.parametrize("name", ["tri20x4", "quad10x3", "veg20x4", "turb20x4", "turbq10x3",
                                   "unst20x4", "unstq10x3", "hydro20x4", "elder20x4",
-                                  "elderq10x3"])
+                                  "elderq10x3", "cap20x4"])
 def test_oracle_matches_reference_every_record(name):
     g = load_golden(name)
     om = po.OracleModel(golden_deck(g))
+    if "relax" in g:
+        # $RELAX method relaxMin relaxMax maxDeltaUV maxDeltaS (cap20x4: the cap binds)
+        om.set_params(relax_method=int(g["relax"][0]), max_delta_uv=float(g["relax"][3]),
+                      max_delta_s=float(g["relax"][4]))
     unsteady = "stationary" in g and int(g["stationary"]) == 0
     el_step = int(g["elem_step"]) if "elem_step" in g else 0
     tstep = None
@@ -180,10 +184,13 @@ def test_oracle_wet_dry_matches_reference_every_record(name):
         assert (ef == 1).sum() == calls[0][1]
 
 
-def test_oracle_matches_reference_channel_200x20():
-    """The 8,000-triangle straight channel: numbering, pattern hash, every Newton
-    correction, BiCGStab iteration counts and the final state, bit for bit."""
-    g = load_golden("channel200x20")
+@pytest.mark.parametrize("name,max_iter", [("channel200x20", 1000), ("vegchannel200x20", 1000),
+                                           ("meander40x64", 2000)])
+def test_oracle_matches_reference_compact_decks(name, max_iter):
+    """The 8,000-triangle straight channel, the same with vegetation on 30 % of the bank
+    elements, and a 5,120-triangle reach of the meandering channel: numbering, pattern hash,
+    every Newton correction, BiCGStab iteration counts and the final state, bit for bit."""
+    g = load_golden(name)
     om = po.OracleModel(golden_deck(g))
     om.prepare()
     eq = om.numbering()
@@ -192,11 +199,41 @@ def test_oracle_matches_reference_channel_200x20():
     h = hashlib.sha256(np.diff(rp).astype(np.int32).tobytes() + cols.astype(np.int32).tobytes())
     assert h.hexdigest() == str(g["s0_pattern_sha256"])
     for k in range(int(g["nsteps"])):
-        X, st = om.newton_step(prec=32, solver=1, max_iter=1000, eps=1e-10)
+        X, st = om.newton_step(prec=32, solver=1, max_iter=max_iter, eps=1e-10)
         assert st.scale == g[f"s{k}_scale"]
         assert st.iters == g["bicgstab_iters"][k]
         np.testing.assert_array_equal(X, g[f"s{k}_xsol"])
+    # the flow cycle ends with the slip condition at the rotated nodes (EqsUVS2D.cpp:1078):
+    # on the meander's curved walls the initial velocity's normal part is removed there
+    om.slip_flow()
     _check_state(om, g, "final_")
+
+
+def test_oracle_matches_reference_flood_time_steps():
+    """The 2 km channel under the rising flood hydrograph, 10 implicit time steps with
+    Elder's eddy viscosity: the Newton steps of every time step, each step's scale and
+    BiCGStab iteration count, and the state at the end of every time step, bit for bit."""
+    g = load_golden("flood200x6")
+    om = po.OracleModel(golden_deck(g))
+    dt, theta = float(g["dt"]), float(g["theta"])
+    tsteps = [int(g[f"s{k}_tstep"]) for k in range(int(g["nsteps"]))]
+    k = 0
+    for t in range(1, int(g["tm_steps"]) + 1):
+        if t > 1:
+            om.time_commit()
+        om.time_step(dt, theta, first=(t == 1))
+        om.set_inflow(*po.hydrograph(g["hydrograph"], t * dt))
+        for _ in range(tsteps.count(t)):
+            om.prepare()
+            _, st = om.newton_step(prec=32, solver=1, max_iter=1000, eps=1e-10)
+            assert st.scale == g[f"s{k}_scale"], k
+            assert st.iters == g["bicgstab_iters"][k], k
+            k += 1
+        U, V, S, _ = om.state()
+        pre = f"t{t}_" if t < int(g["tm_steps"]) else "final_"
+        np.testing.assert_array_equal(U, g[pre + "U"], err_msg=f"time step {t}")
+        np.testing.assert_array_equal(V, g[pre + "V"], err_msg=f"time step {t}")
+        np.testing.assert_array_equal(S, g[pre + "S"], err_msg=f"time step {t}")
 
 
 # ---------------------------------------------------------------------------------------
