@@ -197,9 +197,10 @@ rmx_status rmx_ilu0_apply(rmx_ctx* ctx, int prec, const double* b, double* x);
  * blocks).  A block is one of `segments` parts of a strip of consecutive equations, cut
  * along the principal axis of the strip's node coordinates; every strip is swept by one
  * workgroup.  rmx_set_strips chooses the number of strips (0 = two per compute unit,
- * the default) and segments (default 32; 1 for a system loaded with rmx_csr_upload, which
- * has no coordinates) and rebuilds the schedule; rmx_get_blocks returns the block of every
- * equation. */
+ * the default, and at most one per 2048 rows) and segments (0 = the default: blocks of
+ * about 560 rows, at most 32 segments per strip; 1 for a system loaded with
+ * rmx_csr_upload, which has no coordinates) and rebuilds the schedule; rmx_get_blocks
+ * returns the block of every equation. */
 rmx_status rmx_silu0_factor(rmx_ctx* ctx, int prec);
 rmx_status rmx_silu0_apply(rmx_ctx* ctx, int prec, const double* b, double* x);
 rmx_status rmx_set_strips(rmx_ctx* ctx, int strips, int segments);
@@ -257,7 +258,7 @@ rmx_status rmx_get_sweep_info(rmx_ctx* ctx, int kind, rmx_sweep_info* out);
 /* r = A x: replaces CRSMAT::MulVec (CRSMat.cpp:466).  Host pointers. */
 rmx_status rmx_spmv(rmx_ctx* ctx, int prec, const double* x, double* r);
 
-typedef struct {
+typedef struct rmx_krylov_opts {
   int method;                 /* RMX_BICGSTAB | RMX_FGMRES */
   int prec;                   /* storage of matrix + factors used by the Krylov loop */
   int precond;                /* RMX_PREC_* */
@@ -269,6 +270,13 @@ typedef struct {
                                  refinement sweep's inner tolerance is max(eps, 0.5 refine_tol
                                  / current residual), so the last sweep stops near the target */
   int refine_max;             /* maximum refinement sweeps */
+  /* what follows a solve that has not converged (SOLVER::proceed, EQS::Solve,
+     Solve.cpp:325-362): 0 keep the unconverged X; -1 set X to zero and report the solve
+     interrupted; > 0 continue with the solver `next` (the reference's solver number
+     `proceed`) when its tolerance is below the accuracy reached, from X (from zero when
+     that accuracy is 1 or more); `next` may chain further */
+  int proceed;
+  const struct rmx_krylov_opts* next;
 } rmx_krylov_opts;
 
 typedef struct {
@@ -278,16 +286,21 @@ typedef struct {
   double accuracy;            /* last Krylov relative residual estimate */
   double true_resid;          /* ||B - A64 X||_2 / ||B||_2 in FP64 */
   double seconds;             /* wall time of the solve on the device stream */
-  int fallback;               /* 1: the learned preconditioner stalled and the solve finished
-                                 with BiCGStab and the two-level preconditioner */
+  int fallback;               /* 1: a refinement sweep stalled and the remaining sweeps ran
+                                 BiCGStab (with the two-level preconditioner after the
+                                 learned one) */
+  int solvers;                /* solvers of the proceed chain that ran (1 without it) */
+  int interrupted;            /* 1: not converged with proceed = -1 (X is zero) */
 } rmx_krylov_out;
 
 /* Solve A X = B for the assembled system: replaces SOLVER::Iterate with P_BCGSTABD
  * (P_bcgstabd.cpp:98) or P_FGMRESD (P_fgmresd.cpp:90) and the ILU(0) preconditioner.
  * X starts at zero.  With refine = 1 the solve runs in the requested storage precision
- * inside an FP64 iterative refinement on the FP64 matrix; with the learned preconditioner,
- * a refinement sweep that lowers the true FP64 residual less than tenfold switches the
- * remaining sweeps to BiCGStab with the two-level preconditioner (out->fallback = 1). */
+ * inside an FP64 iterative refinement on the FP64 matrix; a refinement sweep that lowers
+ * the true FP64 residual less than tenfold switches the remaining sweeps to BiCGStab (with
+ * the two-level preconditioner in place of the learned one; out->fallback = 1).  FGMRES
+ * ends a solve whose restart cycle made no progress (restarted GMRES can stagnate on these
+ * indefinite systems).  A solve that does not converge follows opt->proceed. */
 rmx_status rmx_solve(rmx_ctx* ctx, const rmx_krylov_opts* opt, rmx_krylov_out* out);
 
 /* Host access to the system (tests, drop-in callers that keep host arrays). */

@@ -566,7 +566,8 @@ rmx_status rmx_set_strips(rmx_ctx* p, int strips, int segments)
 {
   API_BEGIN
   Ctx& c = *need(p);
-  if (strips < 0 || segments < 1 || segments > 1024) fail(RMX_ERR_ARG, "strips >= 0, 1 <= segments <= 1024");
+  if (strips < 0 || segments < 0 || segments > 1024)
+    fail(RMX_ERR_ARG, "strips >= 0, 0 <= segments <= 1024 (0: automatic)");
   c.strips = strips;
   c.segments = segments;
   if (c.prepared) setup_ilu_schedule(c);
@@ -886,7 +887,8 @@ rmx_status rmx_kernel_stats_enable(rmx_ctx* p, int on)
   KTimer& T = c.ktm;
   HIPCHECK(hipStreamSynchronize(c.stream));
   kt_collect(c);
-  for (int k = 0; k < KT_N; k++) { T.sec[k] = 0.0; T.bytes[k] = 0.0; T.cnt[k] = 0; }
+  if (on)
+    for (int k = 0; k < KT_N; k++) { T.sec[k] = 0.0; T.bytes[k] = 0.0; T.cnt[k] = 0; }
   if (on && T.ev.empty()) {
     const size_t pairs = 8192;
     T.ev.resize(2 * pairs);

@@ -1464,6 +1464,17 @@ bool ilu_apply(Ctx& c, Ilu& I, int prec, const double* b, double* x, hipEvent_t*
 
 // ---- the context's two instances ----
 
+// Segments per strip: the requested count, or blocks of about 560 rows (at most 32 per
+// strip).  Blocks much smaller than that drop most of the factor's couplings from the
+// sweeps: on a 23k-equation meander reach, 32 segments of its 11 strips (66-row blocks)
+// left FGMRES stalled where blocks of ~700 rows converge.
+static int strip_segments(const Ctx& c, int nblk)
+{
+  if (c.segments > 0) return c.segments;
+  const long long rows = c.n / std::max(1, nblk);
+  return (int)std::max<long long>(1, std::min<long long>(32, rows / 560));
+}
+
 void setup_ilu_schedule(Ctx& c)
 {
   std::vector<int> hrp, hcols;
@@ -1489,7 +1500,7 @@ void setup_ilu_schedule(Ctx& c)
       if (c.hm.eqno[k] >= 0) group[c.hm.eqno[k]] = (int)(k / 3);
   }
   ilu_setup(c.silu, c.n, rp, cl, nblk, c.csr_only ? nullptr : &group, c.stream,
-            c.csr_only ? nullptr : &c.hm, c.segments, STAGE_ROWS);
+            c.csr_only ? nullptr : &c.hm, strip_segments(c, nblk), STAGE_ROWS);
   c.coarse.ready = c.coarse.fac = false;
   if (!c.csr_only && c.coarse_segments > 0) coarse_setup(c, rp, cl, group);
 }

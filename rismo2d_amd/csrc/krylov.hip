@@ -412,6 +412,7 @@ static bool fgmres(Ctx& c, int prec, int pk, const double* b, double* x, int m, 
     if (std::fabs(ro) <= EPSILON) { conv = true; break; }
     k_scal_dev<<<vb, 256, 0, c.stream>>>(n, nullptr, 0, 1.0 / ro, Vb, V32);
     if (iters == 0) { resid = ro; eps1 = eps * ro; }
+    const double ro_cycle = ro;
     rs[0] = ro;
     int i = -1;
     while (true) {
@@ -477,6 +478,9 @@ static bool fgmres(Ctx& c, int prec, int pk, const double* b, double* x, int m, 
     HIPCHECK(hipStreamSynchronize(c.stream));
     if (ro <= eps1) { conv = true; break; }
     if (iters >= maxIter || !std::isfinite(ro)) { conv = false; break; }
+    // a restart cycle that lowered the residual by less than a thousandth has stagnated
+    // (restarted GMRES can stall on indefinite systems): end the solve unconverged
+    if (ro > 0.999 * ro_cycle) { conv = false; break; }
   }
   return conv;
 }
@@ -499,14 +503,11 @@ static double true_resid(Ctx& c, const double* x, double* r)
   return nb > 0 ? std::sqrt(c.h_red[0]) / nb : 0.0;
 }
 
-void dev_solve(Ctx& c, const rmx_krylov_opts& o, rmx_krylov_out& out)
+static void check_opts(Ctx& c, const rmx_krylov_opts& o)
 {
   if (o.method == RMX_FGMRES && (o.mkyrl < 1 || o.mkyrl > MDOT_MAX - 2))
     fail(RMX_ERR_ARG, "solve: FGMRES restart length must be 1..30");
   if (o.method != RMX_FGMRES && o.method != RMX_BICGSTAB) fail(RMX_ERR_ARG, "solve: unknown method");
-  ensure_work(c);
-  c.X.alloc(c.n);
-  const auto t0 = std::chrono::steady_clock::now();
   const int prec = o.prec == RMX_F32 ? RMX_F32 : RMX_F64;
   const int pk = o.precond;
   if (pk == RMX_PREC_ILU0 || pk == RMX_PREC_SILU0 || pk == RMX_PREC_SILU0C || pk == RMX_PREC_MPNN) {
@@ -519,25 +520,59 @@ void dev_solve(Ctx& c, const rmx_krylov_opts& o, rmx_krylov_out& out)
   if (pk == RMX_PREC_MPNN && !mpnn_ready(c)) fail(RMX_ERR_ARG, "solve: no learned preconditioner loaded");
   if ((prec == RMX_F32 && !c.have32) || (prec == RMX_F64 && !c.have64))
     fail(RMX_ERR_ARG, "solve: matrix of the requested precision is missing");
+  if (o.refine && !c.have64) fail(RMX_ERR_ARG, "refinement needs the FP64 matrix");
+}
+
+// One solver of the chain on A X = B, from X = 0 or from the current X (continue).
+static void solve_one(Ctx& c, const rmx_krylov_opts& o, bool from_zero, rmx_krylov_out& out)
+{
+  const int prec = o.prec == RMX_F32 ? RMX_F32 : RMX_F64;
+  const int pk = o.precond;
   out = rmx_krylov_out{};
   if (!o.refine) {
     int it = 0; double acc = 1.0;
-    const bool conv = krylov(c, o.method, prec, pk, c.B.p, c.X.p, o.max_iter, o.eps, o.mkyrl, it, acc);
-    out.iters = it; out.accuracy = acc; out.converged = conv;
+    if (from_zero) {
+      const bool conv = krylov(c, o.method, prec, pk, c.B.p, c.X.p, o.max_iter, o.eps, o.mkyrl, it, acc);
+      out.converged = conv;
+      out.accuracy = acc;
+    } else {
+      // continue from X: the residual equation A d = B - A X, X += d, the accuracy relative
+      // to ||B|| as a solve from zero reports it
+      double* r = c.w[7].p;
+      double* d = c.w[8].p;
+      dev_resid_p(c, prec, c.X.p, c.B.p, r);
+      const double* xs[2] = { r, c.B.p };
+      dev_dots(c, 2, xs, xs, c.red.p + 8);
+      HIPCHECK(hipMemcpyAsync(c.h_red, c.red.p + 8, sizeof(double) * 2, hipMemcpyDeviceToHost, c.stream));
+      HIPCHECK(hipStreamSynchronize(c.stream));
+      const double rel = c.h_red[1] > 0 ? std::sqrt(c.h_red[0] / c.h_red[1]) : 0.0;
+      const bool conv = rel <= o.eps ||
+                        krylov(c, o.method, prec, pk, r, d, o.max_iter, o.eps / std::max(rel, 1e-300),
+                               o.mkyrl, it, acc);
+      if (rel > o.eps) k_axpy<<<vblocks(c.n), 256, 0, c.stream>>>(c.n, 1.0, d, c.X.p);
+      out.converged = conv;
+      out.accuracy = rel > o.eps ? acc * rel : rel;
+    }
+    out.iters = it;
     out.true_resid = c.have64 ? true_resid(c, c.X.p, c.w[7].p) : -1.0;
-  } else {
-    if (!c.have64) fail(RMX_ERR_ARG, "refinement needs the FP64 matrix");
-    // FP64 iterative refinement: x = 0, r = B; repeat: solve A_p d = r; x += d; r = B - A64 x
-    double* xr = c.X.p;
-    double* r = c.w[7].p;
-    double* d = c.w[8].p;
+    return;
+  }
+  // FP64 iterative refinement: r = B - A64 x; repeat: solve A_p d = r; x += d
+  double* xr = c.X.p;
+  double* r = c.w[7].p;
+  double* d = c.w[8].p;
+  double tr = 1.0;
+  if (from_zero) {
     HIPCHECK(hipMemsetAsync(xr, 0, sizeof(double) * c.n, c.stream));
     k_copy<<<vblocks(c.n), 256, 0, c.stream>>>(c.n, c.B.p, r);
-    double tr = 1.0;
-    int total = 0;
-    const int sweeps_max = o.refine_max > 0 ? o.refine_max : 10;
-    int sw = 0;
-    int method = o.method, kind = pk;
+  } else {
+    tr = true_resid(c, xr, r);
+  }
+  int total = 0;
+  const int sweeps_max = o.refine_max > 0 ? o.refine_max : 10;
+  int sw = 0;
+  int method = o.method, kind = pk;
+  if (tr > o.refine_tol) {
     for (; sw < sweeps_max; sw++) {
       int it = 0; double acc = 1.0;
       const double tr_prev = tr;
@@ -553,18 +588,60 @@ void dev_solve(Ctx& c, const rmx_krylov_opts& o, rmx_krylov_out& out)
         fprintf(stderr, "[rmx] refinement sweep %d: %d iterations (inner %.2e), FP64 residual %.3e\n",
                 sw + 1, it, acc, tr);
       if (tr <= o.refine_tol || !std::isfinite(tr)) { sw++; break; }
-      if (kind == RMX_PREC_MPNN && !(tr < 0.1 * tr_prev)) {
-        // the learned preconditioner stalled: finish with BiCGStab + two-level
+      if (!(tr < 0.1 * tr_prev) && (method == RMX_FGMRES || kind == RMX_PREC_MPNN)) {
+        // the sweep stalled (FGMRES stagnating, or the learned preconditioner): the remaining
+        // sweeps run BiCGStab, with the two-level preconditioner in place of the learned one
         method = RMX_BICGSTAB;
-        kind = RMX_PREC_SILU0C;
+        if (kind == RMX_PREC_MPNN) kind = RMX_PREC_SILU0C;
         out.fallback = 1;
-        if (getenv("RMX_DEBUG")) fprintf(stderr, "[rmx] learned preconditioner stalled: fallback\n");
+        if (getenv("RMX_DEBUG")) fprintf(stderr, "[rmx] refinement sweep stalled: BiCGStab\n");
       }
     }
-    out.iters = total; out.sweeps = sw; out.true_resid = tr;
-    out.converged = tr <= o.refine_tol;
   }
+  out.iters = total; out.sweeps = sw; out.true_resid = tr;
+  out.converged = tr <= o.refine_tol;
+  if (out.converged && sw == 0) out.accuracy = tr;
+}
+
+void dev_solve(Ctx& c, const rmx_krylov_opts& o0, rmx_krylov_out& out)
+{
+  check_opts(c, o0);
+  ensure_work(c);
+  c.X.alloc(c.n);
+  const auto t0 = std::chrono::steady_clock::now();
+  const rmx_krylov_opts* o = &o0;
+  bool from_zero = true;
+  int iters = 0, sweeps = 0, fallback = 0, solvers = 0;
+  rmx_krylov_out one{};
+  for (;;) {
+    solve_one(c, *o, from_zero, one);
+    solvers++;
+    iters += one.iters;
+    sweeps += one.sweeps;
+    fallback |= one.fallback;
+    if (one.converged || o->proceed == 0) break;
+    if (o->proceed < 0) {
+      // the reference sets X to zero and interrupts (Solve.cpp:345-353)
+      HIPCHECK(hipMemsetAsync(c.X.p, 0, sizeof(double) * c.n, c.stream));
+      one.interrupted = 1;
+      break;
+    }
+    const rmx_krylov_opts* nx = o->next;
+    if (!nx) fail(RMX_ERR_ARG, "solve: proceed > 0 without a next solver");
+    if (solvers >= 16) fail(RMX_ERR_ARG, "solve: proceed chain longer than 16 solvers");
+    check_opts(c, *nx);
+    const double reached = o->refine ? one.true_resid : one.accuracy;
+    if (!(nx->eps < reached)) break;             // Solve.cpp:337: next only when tighter
+    from_zero = reached >= 1.0;                  // Solve.cpp:331: X = 0 when nothing was gained
+    o = nx;
+  }
+  out = one;
+  out.iters = iters;
+  out.sweeps = sweeps;
+  out.fallback = fallback;
+  out.solvers = solvers;
   HIPCHECK(hipStreamSynchronize(c.stream));
+  kt_collect(c);
   out.seconds = std::chrono::duration<double>(std::chrono::steady_clock::now() - t0).count();
 }
 

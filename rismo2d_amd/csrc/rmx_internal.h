@@ -281,7 +281,7 @@ struct Ctx {
   Ilu ilu;                                    // reference-order ILU(0)
   Ilu silu;                                   // strip ILU(0)
   int strips = 0;                             // 0: two strips per compute unit
-  int segments = 32;                          // cross-strip cuts of the strip sweeps
+  int segments = 0;                           // cross-strip cuts of the strip sweeps (0: auto)
   Coarse coarse;
   Mpnn mpnn;
   int coarse_segments = 8;                    // cross-strip cuts of the coarse aggregates

@@ -50,15 +50,32 @@ class Params(C.Structure):
 
 
 class KrylovOpts(C.Structure):
-    _fields_ = [("method", C.c_int), ("prec", C.c_int), ("precond", C.c_int),
-                ("max_iter", C.c_int), ("mkyrl", C.c_int), ("eps", C.c_double),
-                ("refine", C.c_int), ("refine_tol", C.c_double), ("refine_max", C.c_int)]
+    pass
+
+
+KrylovOpts._fields_ = [("method", C.c_int), ("prec", C.c_int), ("precond", C.c_int),
+                       ("max_iter", C.c_int), ("mkyrl", C.c_int), ("eps", C.c_double),
+                       ("refine", C.c_int), ("refine_tol", C.c_double), ("refine_max", C.c_int),
+                       ("proceed", C.c_int), ("next", C.POINTER(KrylovOpts))]
+
+
+def krylov_opts(method=1, prec=64, precond=1, max_iter=1000, mkyrl=30, eps=1e-10, refine=False,
+                refine_tol=1e-8, refine_max=10, proceed=0, next=None):
+    """A KrylovOpts chain: ``next`` (a dict of the same keywords) is the solver that follows
+    an unconverged solve when ``proceed`` > 0 (the reference's SOLVER::proceed).  The
+    returned structure keeps its successors alive."""
+    o = KrylovOpts(method, prec, precond, max_iter, mkyrl, eps, int(refine), refine_tol,
+                   refine_max, proceed, None)
+    if next is not None:
+        o._next = krylov_opts(**next)
+        o.next = C.pointer(o._next)
+    return o
 
 
 class KrylovOut(C.Structure):
     _fields_ = [("iters", C.c_int), ("sweeps", C.c_int), ("converged", C.c_int),
                 ("accuracy", C.c_double), ("true_resid", C.c_double), ("seconds", C.c_double),
-                ("fallback", C.c_int)]
+                ("fallback", C.c_int), ("solvers", C.c_int), ("interrupted", C.c_int)]
 
 
 class StepStats(C.Structure):
@@ -355,7 +372,8 @@ class Context:
         _check(lib().rmx_silu0_apply(self.h, prec, _p(b, _d), _p(x, _d)))
         return x
 
-    def set_strips(self, strips, segments=16):
+    def set_strips(self, strips, segments=0):
+        """Strips (0: two per CU) and segments per strip (0: blocks of about 560 rows)."""
         _check(lib().rmx_set_strips(self.h, int(strips), int(segments)))
 
     def sweep_info(self, kind=PREC_SILU0):
@@ -439,9 +457,9 @@ class Context:
         return r
 
     def solve(self, method=BICGSTAB, prec=F64, precond=PREC_ILU0, max_iter=1000, mkyrl=30,
-              eps=1e-10, refine=False, refine_tol=1e-8, refine_max=10):
-        o = KrylovOpts(method, prec, precond, max_iter, mkyrl, eps, int(refine), refine_tol,
-                       refine_max)
+              eps=1e-10, refine=False, refine_tol=1e-8, refine_max=10, proceed=0, next=None):
+        o = krylov_opts(method, prec, precond, max_iter, mkyrl, eps, refine, refine_tol,
+                        refine_max, proceed, next)
         out = KrylovOut()
         _check(lib().rmx_solve(self.h, C.byref(o), C.byref(out)))
         return out
@@ -468,9 +486,9 @@ class Context:
         return st
 
     def newton_step(self, method=BICGSTAB, prec=F64, precond=PREC_ILU0, max_iter=1000, mkyrl=30,
-                    eps=1e-10, refine=False, refine_tol=1e-8, refine_max=10):
-        o = KrylovOpts(method, prec, precond, max_iter, mkyrl, eps, int(refine), refine_tol,
-                       refine_max)
+                    eps=1e-10, refine=False, refine_tol=1e-8, refine_max=10, proceed=0, next=None):
+        o = krylov_opts(method, prec, precond, max_iter, mkyrl, eps, refine, refine_tol,
+                        refine_max, proceed, next)
         st = StepStats()
         _check(lib().rmx_newton_step(self.h, C.byref(o), C.byref(st)))
         if not self.n:

@@ -449,6 +449,41 @@ def test_f32_spmv_row_groups(ctx):
         assert out.converged and out.true_resid <= 1e-10
 
 
+def test_solve_proceed_chain_follows_the_reference(ctx):
+    """EQS::Solve's fallback after a solver that has not converged (Solve.cpp:325-362), on
+    a random diagonally dominant system: proceed > 0 continues with the next solver from X
+    (when its tolerance is below the accuracy reached) and reaches the dense solution;
+    proceed = -1 zeroes X and reports the solve interrupted; proceed = 0 keeps the
+    unconverged X; a next solver whose tolerance is not below the accuracy reached is not
+    run."""
+    print("[gpu] proceed chain of the solve", flush=True)
+    M, rp, cols, vals = _random_csr(300, 9, 0.03)
+    b = np.random.default_rng(10).standard_normal(300)
+    ref = np.linalg.solve(M, b)
+    ctx.csr_upload(rp, cols, vals, b)
+    ctx.ilu0_factor(rmx.F64)
+    weak = dict(method=rmx.BICGSTAB, prec=rmx.F64, precond=rmx.PREC_NONE, max_iter=2, eps=1e-12)
+    strong = dict(method=rmx.FGMRES, prec=rmx.F64, precond=rmx.PREC_ILU0, mkyrl=20, max_iter=500,
+                  eps=1e-13)
+    out = ctx.solve(**weak, proceed=1, next=strong)
+    assert out.converged and out.solvers == 2 and out.iters > 2
+    np.testing.assert_allclose(ctx.solution(), ref, rtol=1e-9, atol=1e-10)
+    out = ctx.solve(**weak, proceed=-1)
+    assert not out.converged and out.interrupted == 1 and out.solvers == 1
+    assert not ctx.solution().any()
+    out = ctx.solve(**weak, proceed=0)
+    X = ctx.solution()
+    assert not out.converged and out.interrupted == 0 and X.any()
+    acc = out.accuracy
+    out = ctx.solve(**weak, proceed=1, next=dict(strong, eps=1.5 * acc))
+    assert not out.converged and out.solvers == 1
+    # in refinement the chain continues from X as well (FP64 true residual reached)
+    out = ctx.solve(**dict(weak, refine=True, refine_tol=1e-12, refine_max=1), proceed=1,
+                    next=dict(strong, refine=True, refine_tol=1e-12, refine_max=5))
+    assert out.converged and out.solvers == 2 and out.true_resid <= 1e-12
+    np.testing.assert_allclose(ctx.solution(), ref, rtol=1e-9, atol=1e-10)
+
+
 def test_full_pattern_ilu_is_exact_lu(ctx):
     M, rp, cols, vals = _random_csr(40, 3, 1.0)
     b = np.random.default_rng(4).standard_normal(40)
