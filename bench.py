@@ -166,16 +166,23 @@ def kernels_only(c, n, a):
     c.scale_l2()
     c.convert(R.F32)
     c.silu0_factor(R.F32)
+    c.kernel_stats(True)
     kt = c.profile_kernels(R.F32, 5)
+    ks = c.kernel_stats(False)
     x = np.random.default_rng(0).standard_normal(n)
     for _ in range(3):
         c.precond_apply(R.PREC_NONE, x, R.F32)
-    lo_e, up_e = c.sweep_entries()
-    print(json.dumps({"n": n, "nnz": c.nnz, "spmv_ms": kt.spmv_s * 1e3, "spmv_bytes": kt.spmv_bytes,
-                      "sweep_fwd_ms": kt.silu_fwd_s * 1e3, "sweep_bwd_ms": kt.silu_bwd_s * 1e3,
-                      "sweep_fwd_bytes": sweep_bytes(lo_e, n, 4),
-                      "sweep_bwd_bytes": sweep_bytes(up_e, n, 4) + 4 * n,
-                      "copy_bytes_each_way": 8 * n}), flush=True)
+    rec = {"n": n, "nnz": c.nnz, "spmv_ms": kt.spmv_s * 1e3, "spmv_bytes": kt.spmv_bytes,
+           "copy_bytes_each_way": 8 * n}
+    sec, byt, cnt = ks["sweeps_block"]
+    if cnt:                                   # both triangles in one launch per apply
+        rec.update(block_sweep_ms=sec / cnt * 1e3, block_sweep_bytes=byt / cnt)
+    else:
+        lo_e, up_e = c.sweep_entries()
+        rec.update(sweep_fwd_ms=kt.silu_fwd_s * 1e3, sweep_bwd_ms=kt.silu_bwd_s * 1e3,
+                   sweep_fwd_bytes=sweep_bytes(lo_e, n, 4),
+                   sweep_bwd_bytes=sweep_bytes(up_e, n, 4) + 4 * n)
+    print(json.dumps(rec), flush=True)
     c.close()
 
 
@@ -230,8 +237,11 @@ KERNEL_NAMES = {"ilu_factor": "k_ilu_flow (dataflow ILU(0) factorisation, FP32)"
                 "spmv": "k_spmv_grp (FP32 row-group SpMV)",
                 "coarse": "coarse correction (restrict, block cyclic reduction, A P residual)",
                 "gram_schmidt": "k_mdot32 + k_gs_update32 (FP32 basis)",
-                "resid_fp64": "k_spmv<double> (FP64 refinement residual)"}
-TRAFFIC_KEYS = {"sweep_lower": "silu_sweep_fwd", "sweep_upper": "silu_sweep_bwd", "spmv": "spmv_f32"}
+                "resid_fp64": "k_spmv<double> (FP64 refinement residual)",
+                "sweeps_block": "k_block_sweep (strip ILU(0), both triangles, one wave per block, "
+                                "+ coarse prolongation)"}
+TRAFFIC_KEYS = {"sweep_lower": "silu_sweep_fwd", "sweep_upper": "silu_sweep_bwd", "spmv": "spmv_f32",
+                "sweeps_block": "silu_block_sweep"}
 
 
 def main():

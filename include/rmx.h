@@ -352,10 +352,13 @@ rmx_status rmx_profile_kernels(rmx_ctx* ctx, int prec, int reps, rmx_kernel_time
  * the stream synchronises.  Classes: 0 ILU(0) factorisation, 1 lower and 2 upper triangular
  * sweep, 3 Krylov SpMV (the storage precision of the solve, product or residual form),
  * 4 coarse correction (restriction, coarse solve, A P residual), 5 Gram-Schmidt (dots and
- * update), 6 FP64 residual of the refinement.  bytes: the algorithmic bytes of each launch
- * summed (factorisation 3 nnz (v + 4); sweep: kept entries (v + 4) + 20 or 24 bytes per row;
- * SpMV: the bytes of the layout it ran on; Gram-Schmidt: the basis vectors read, FP32 or FP64,
- * and the vector read and written; coarse: the vectors, maps and A P entries it reads). */
+ * update), 6 FP64 residual of the refinement, 7 both triangular sweeps of the strip ILU(0)
+ * in one launch (block sweeps: one wave per block).  bytes: the algorithmic bytes of each
+ * launch summed (factorisation 3 nnz (v + 4); sweep: kept entries (v + 4) + 20 or 24 bytes
+ * per row; block sweeps: kept entries of both triangles (v + 2) + (22 + v) bytes per row and
+ * 4 per level, + 12 per row with the folded prolongation; SpMV: the bytes of the layout it
+ * ran on; Gram-Schmidt: the basis vectors read, FP32 or FP64, and the vector read and
+ * written; coarse: the vectors, maps and A P entries it reads). */
 typedef struct {
   double seconds[8];
   double bytes[8];

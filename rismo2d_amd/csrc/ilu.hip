@@ -625,7 +625,17 @@ __global__ void k_pack(long long ne, const int* __restrict__ psrc, const VT* __r
 {
   for (long long t = (long long)blockIdx.x * blockDim.x + threadIdx.x; t < ne;
        t += (long long)gridDim.x * blockDim.x)
-    pval[t] = LU[psrc[t]];
+    pval[t] = psrc[t] >= 0 ? LU[psrc[t]] : (VT)0;       // negative: a zero (padding) entry
+}
+
+// Reciprocal diagonal in FP64 (1 for a negative source: an absent row).
+template <class VT>
+__global__ void k_pack_rcp(long long ne, const int* __restrict__ psrc, const VT* __restrict__ LU,
+                           double* __restrict__ out)
+{
+  for (long long t = (long long)blockIdx.x * blockDim.x + threadIdx.x; t < ne;
+       t += (long long)gridDim.x * blockDim.x)
+    out[t] = psrc[t] >= 0 ? 1.0 / (double)LU[psrc[t]] : 1.0;
 }
 
 template <class VT>
@@ -1026,6 +1036,239 @@ __global__ void __launch_bounds__(512) k_sweep_stage(const int* __restrict__ ble
 }
 
 // ---------------------------------------------------------------------------------------
+// block sweeps: one wave per block, both triangles in one launch
+
+constexpr int BS_ROWS = 2047;      // rows per block (11-bit positions)
+constexpr int BS_LROWS = 32;       // rows per level (5-bit row numbers)
+constexpr int BS_LPAIRS = 64;      // entry pairs per level (one per lane)
+constexpr int BS_PAD = 256;        // padding after every entry stream (clamped loads)
+constexpr unsigned BS_EM = (1u << 21) - 1;
+constexpr int BS_DEPTH = 4;        // levels whose loads are in flight ahead of the level
+constexpr int BS_GATHER = 8;       // rows per lane whose loads are issued together
+
+template <class VT> struct VPair;
+template <> struct VPair<float> { using T = float2; using Q = float4; };
+template <> struct VPair<double> { using T = double2; using Q = double4; };
+
+// One step of an inclusive scan over lanes segmented by key (equal keys are contiguous,
+// keys >= 1): every lane adds the value DPP control C brings from another lane when that
+// lane has the same key and the step applies to the lane's row (ok).  A lane the control
+// leaves without a source reads zeros (key 0: no match).
+template <int C>
+__device__ __forceinline__ void seg_step(double& v, int key, bool ok)
+{
+  const int ks = __builtin_amdgcn_mov_dpp(key, C, 0xF, 0xF, true);
+  const int lo = __builtin_amdgcn_mov_dpp(__double2loint(v), C, 0xF, 0xF, true);
+  const int hi = __builtin_amdgcn_mov_dpp(__double2hiint(v), C, 0xF, 0xF, true);
+  const double vs = __hiloint2double(hi, lo);
+  v = (ks == key && ok) ? v + vs : v;
+}
+
+// Sum of v over each run of equal keys, complete at the run's last lane: shifts by 1, 2, 4
+// and 8 inside each row of 16 lanes, then lane 15 of rows 0 and 2 into rows 1 and 3, then
+// lane 31 into rows 2 and 3 (each row already holds its own part when it is passed on).
+__device__ __forceinline__ double seg_sum(double v, int key, bool rows13, bool rows23)
+{
+  seg_step<0x111>(v, key, true);     // row_shr:1
+  seg_step<0x112>(v, key, true);     // row_shr:2
+  seg_step<0x114>(v, key, true);     // row_shr:4
+  seg_step<0x118>(v, key, true);     // row_shr:8
+  seg_step<0x142>(v, key, rows13);   // row_bcast:15
+  seg_step<0x143>(v, key, rows23);   // row_bcast:31
+  return v;
+}
+
+// The levels of one triangle of one block.  A level lists nodes (units of 1-3 rows); its
+// entries are the couplings of those rows to rows outside their node, a row's entries
+// padded to an even count (at least one pair), lane L taking pair L.  Loaded BS_DEPTH
+// levels ahead into a register ring: the value pair, the packed pair of column | row-in-
+// level words, the level's table entries (kept uniform) and, for lane j, node j's record
+// and couplings.  A level: every lane multiplies its pair by the solved values of the
+// columns, a segmented scan sums each row at its last lane, which leaves the sum in LDS;
+// then lane j solves node j's rows in order with the couplings inside the node: the lower
+// sweep x_r += sum_r + sum of l_rs x_s (factors stored negated), the upper sweep (rows in
+// reverse) x_r = (x_r - sum_r - sum of u_rs x_s) / u_rr.  One wave works the block, so a
+// level ends without a workgroup barrier.
+template <class VT, bool LOWER>
+__device__ __forceinline__ void block_levels(int nlev, const unsigned* tab,
+                                             const VT* __restrict__ vals,
+                                             const unsigned short* __restrict__ meta,
+                                             const unsigned* __restrict__ unit,
+                                             const VT* __restrict__ uval,
+                                             const double4* __restrict__ rdg, double* x,
+                                             double* acc, double* sink, int lane, int m)
+{
+  using V2 = typename VPair<VT>::T;
+  using V4 = typename VPair<VT>::Q;
+  constexpr int D = BS_DEPTH;
+  const V2* vp = reinterpret_cast<const V2*>(vals);           // entry bases are even
+  const unsigned* mp = reinterpret_cast<const unsigned*>(meta);
+  const V4* up4 = reinterpret_cast<const V4*>(uval);          // 4 couplings per node
+  V2 vv[D];
+  V4 ua[D];
+  double4 ub[D];
+  unsigned cm[D], ui[D], ta[D], tb[D];
+  const bool rows13 = (lane >> 4) & 1, rows23 = lane >= 32;
+#pragma unroll
+  for (int k = 0; k < D; k++) {                            // the first D levels are empty
+    ta[k] = 0; tb[k] = 0; cm[k] = 0; ui[k] = 1u << 11;
+    vv[k].x = (VT)0; vv[k].y = (VT)0;
+  }
+  // stages run for levels 0, 1, 2, ...; past the last level the doubled sentinel makes an
+  // empty level with the same loads.  The table entry after the stage's level is read one
+  // stage ahead, so no stage waits for it.
+  unsigned t_lo = __builtin_amdgcn_readfirstlane(tab[0]);
+  unsigned t_hi_v = tab[min(1, nlev + 1)];
+  int ls = 0;
+  auto stage = [&](int k) {
+    const unsigned t0 = t_lo, t1 = __builtin_amdgcn_readfirstlane(t_hi_v);
+    t_hi_v = tab[min(ls + 2, nlev + 1)];
+    t_lo = t1;
+    ls++;
+    ta[k] = t0; tb[k] = t1;
+    const int e0 = (int)(t0 & BS_EM), np = ((int)(t1 & BS_EM) - e0) >> 1;
+    const int q = (e0 >> 1) + min(lane, max(np - 1, 0));
+    vv[k] = vp[q];
+    cm[k] = mp[q];
+    const int U0 = (int)(t0 >> 21), nu = (int)(t1 >> 21) - U0;
+    const int u = U0 + min(lane, max(nu - 1, 0));
+    ui[k] = unit[u];
+    ua[k] = up4[u];
+    if (!LOWER) ub[k] = rdg[u];
+  };
+  auto level = [&](int k) {
+    const unsigned t0 = ta[k], t1 = tb[k];
+    const int np = ((int)(t1 & BS_EM) - (int)(t0 & BS_EM)) >> 1;
+    const int nu = (int)(t1 >> 21) - (int)(t0 >> 21);
+    const unsigned ca = cm[k] & 0xffffu, cb = cm[k] >> 16;   // one row: the same row bits
+    double v = (double)vv[k].x * x[ca & 2047u] + (double)vv[k].y * x[cb & 2047u];
+    const bool on = lane < np;
+    const int key = on ? 1 + (int)(ca >> 11) : 64;
+    v = on ? v : 0.0;
+    v = seg_sum(v, key, rows13, rows23);
+    const int kn = __builtin_amdgcn_mov_dpp(key, 0x130, 0xF, 0xF, true);   // wave_shl:1
+    if (on && kn != key) acc[key - 1] = v;
+    wave_sync();
+    if (lane < nu) {
+      // branch-free: rows past the node's last read valid slots, take neutral operands and
+      // store to the sink
+      const unsigned inf = ui[k];
+      const int p = (int)(inf & 2047u), nr = (int)((inf >> 11) & 3u), s0 = (int)(inf >> 13);
+      const bool h1 = nr > 1, h2 = nr > 2;
+      const int r1 = h1 ? 1 : 0, r2 = h2 ? 2 : r1;
+      const double a0 = acc[s0], a1 = acc[s0 + r1], a2 = acc[s0 + r2];
+      const double x0 = x[p], x1 = x[p + r1], x2 = x[p + r2];
+      double y0, y1, y2;
+      if (LOWER) {
+        // couplings l10, l20, l21 (zero where absent)
+        y0 = x0 + a0;
+        y1 = x1 + a1 + (double)ua[k].x * y0;
+        y2 = x2 + a2 + (double)ua[k].y * y0 + (double)ua[k].z * y1;
+      } else {
+        // couplings u01, u02, u12 (zero where absent), reciprocal diagonals (1 where
+        // absent); last row first
+        y2 = (h2 ? x2 - a2 : 0.0) * ub[k].z;
+        y1 = (h1 ? x1 - a1 - (double)ua[k].z * y2 : 0.0) * ub[k].y;
+        y0 = (x0 - a0 - (double)ua[k].x * y1 - (double)ua[k].y * y2) * ub[k].x;
+      }
+      x[p] = y0;
+      *(h1 ? x + p + 1 : sink) = y1;
+      *(h2 ? x + p + 2 : sink) = y2;
+    }
+    wave_sync();
+  };
+  // no branch around a stage, so that every level issues the same loads and the wait
+  // before a level's first use of its registers leaves the later levels' loads in flight;
+  // the first D (empty) levels issue the loads of levels 0 .. D - 1, so that every load
+  // of the ring is issued inside the loop, into the same registers
+  for (int l0 = -D; l0 < nlev; l0 += D) {
+#pragma unroll
+    for (int k = 0; k < D; k++) {
+      level(k);
+      stage(k);
+    }
+  }
+}
+
+// One block per one-wave workgroup: the block's right-hand side is gathered into LDS (in
+// local positions), swept down and up in place, and written back in equation order (with
+// aidx, plus aval[aidx[e]]: the coarse prolongation folded into the write).  Blocks hold
+// disjoint rows, so b and x may be the same vector.
+template <class VT>
+__global__ void __launch_bounds__(64) k_block_sweep(const int4* __restrict__ info,
+                                                    const int* __restrict__ gidx,
+                                                    const unsigned* __restrict__ levt,
+                                                    const VT* __restrict__ vlo,
+                                                    const unsigned short* __restrict__ mlo,
+                                                    const unsigned* __restrict__ ulo,
+                                                    const VT* __restrict__ wlo,
+                                                    const VT* __restrict__ vup,
+                                                    const unsigned short* __restrict__ mup,
+                                                    const unsigned* __restrict__ uup,
+                                                    const VT* __restrict__ wup,
+                                                    const double4* __restrict__ rdg,
+                                                    const double* rhs, double* out,
+                                                    const int* __restrict__ aidx,
+                                                    const double* __restrict__ aval)
+{
+  constexpr int G = BS_GATHER;
+  extern __shared__ double smem[];
+  const int4 i0 = info[2 * blockIdx.x], i1 = info[2 * blockIdx.x + 1];
+  const int rb = i0.x, m = i0.y, lb = i0.z, nl = i0.w, nu = i1.x, ub = i1.y;
+  const int lane = threadIdx.x;
+  double* x = smem;                                           // [m] local positions
+  double* acc = smem + m;                                     // [BS_LROWS] row sums
+  double* sink = acc + BS_LROWS;                              // [2] stores of absent rows
+  unsigned* tl = reinterpret_cast<unsigned*>(sink + 2);       // [nl + 2]
+  unsigned* tu = tl + nl + 2;                                 // [nu + 2]
+  // gathers: every load of a chunk issued before any is waited for
+  for (int k0 = 0; k0 < m; k0 += 64 * G) {
+    int g[G];
+    double v[G];
+#pragma unroll
+    for (int j = 0; j < G; j++) g[j] = gidx[rb + min(k0 + lane + 64 * j, m - 1)];
+#pragma unroll
+    for (int j = 0; j < G; j++) v[j] = rhs[g[j]];
+#pragma unroll
+    for (int j = 0; j < G; j++)
+      if (k0 + lane + 64 * j < m) x[k0 + lane + 64 * j] = v[j];
+  }
+  const int nt = nl + nu + 4;                                 // both tables, contiguous
+  for (int k0 = 0; k0 < nt; k0 += 64 * G) {
+    unsigned t[G];
+#pragma unroll
+    for (int j = 0; j < G; j++) t[j] = levt[lb + min(k0 + lane + 64 * j, nt - 1)];
+#pragma unroll
+    for (int j = 0; j < G; j++)
+      if (k0 + lane + 64 * j < nt) tl[k0 + lane + 64 * j] = t[j];
+  }
+  wave_sync();
+  block_levels<VT, true>(nl, tl, vlo + i1.z, mlo + i1.z, ulo + ub, wlo + 4 * (long long)ub,
+                         nullptr, x, acc, sink, lane, m);
+  block_levels<VT, false>(nu, tu, vup + i1.w, mup + i1.w, uup + ub, wup + 4 * (long long)ub,
+                          rdg + ub, x, acc, sink, lane, m);
+  wave_sync();
+  for (int k0 = 0; k0 < m; k0 += 64 * G) {
+    int e[G];
+    double a[G];
+#pragma unroll
+    for (int j = 0; j < G; j++) e[j] = gidx[rb + min(k0 + lane + 64 * j, m - 1)];
+    if (aidx) {
+      int ai[G];
+#pragma unroll
+      for (int j = 0; j < G; j++) ai[j] = aidx[e[j]];
+#pragma unroll
+      for (int j = 0; j < G; j++) a[j] = aval[ai[j]];
+    }
+#pragma unroll
+    for (int j = 0; j < G; j++) {
+      const int k = k0 + lane + 64 * j;
+      if (k < m) out[e[j]] = aidx ? x[k] + a[j] : x[k];
+    }
+  }
+}
+
+// ---------------------------------------------------------------------------------------
 // host set-up
 
 // Level sets of the lower or upper dependency graph of rows [a, e), entries with a column
@@ -1136,6 +1379,244 @@ static void build_sweep(SweepSet& S, const Ilu& I, bool lower, std::vector<int>&
   S.prs.upload(prs, s); S.prow.upload(prow, s); S.pcol.upload(pcol, s); S.psrc.upload(psrc, s);
   S.xp.alloc(n);
   order = prow;
+}
+
+// Block sweep layout.  A block is the rows of one strip in one segment; a unit is a run of
+// rows of one node (group), at most 3 (every row its own unit without `group`).  Unit
+// levels: a unit's lower level is one more than the highest of the units its rows couple to
+// below them in the block (upper: above); units of a level are independent.  Local
+// positions list the units in lower level order, a unit's rows together.  Levels are split
+// into parts of at most BS_LROWS rows and BS_LPAIRS entry pairs.  Entries keep the strip
+// sweeps' rule (the strict triangle inside the row's block), minus the couplings inside a
+// unit, which go to the unit's record.  Off (the strip sweeps run) when a block has more than
+// BS_ROWS rows or its entries do not fit 21 bits.
+static void build_blocks(Ilu& I, const std::vector<int>* group, hipStream_t s)
+{
+  BlockSweep& B = I.bs;
+  B.on = false;
+  const int n = I.n, nstrip = I.nblk, nseg = std::max(1, I.nseg);
+  const std::vector<int>& rp = I.h_rp;
+  const std::vector<int>& cl = I.h_cols;
+  const bool sg = !I.seg.empty();
+  auto segof = [&](int i) { return sg ? I.seg[i] : 0; };
+  if (nseg == 1)
+    for (int b = 0; b < nstrip; b++)
+      if (I.bstart[b + 1] - I.bstart[b] > BS_ROWS) return;
+  // tighter level caps (tests of the level split): RMX_BLOCK_LEVEL_ROWS, RMX_BLOCK_LEVEL_PAIRS
+  int cap_rows = BS_LROWS, cap_pairs = BS_LPAIRS;
+  if (const char* e = getenv("RMX_BLOCK_LEVEL_ROWS")) cap_rows = std::max(3, std::min(BS_LROWS, atoi(e)));
+  if (const char* e = getenv("RMX_BLOCK_LEVEL_PAIRS")) cap_pairs = std::max(1, std::min(BS_LPAIRS, atoi(e)));
+  // CSR position of entry (r, c), -1 if the pattern has none
+  auto find = [&](int r, int c) {
+    for (int q = rp[r]; q < rp[r + 1]; q++)
+      if (cl[q] == c) return q;
+    return -1;
+  };
+  struct Part {
+    std::vector<int> rows;               // equation at every local position
+    std::vector<unsigned> tab[2];        // level tables (lower, upper)
+    std::vector<unsigned short> meta[2];
+    std::vector<int> src[2];
+    std::vector<unsigned> unit[2];       // unit records
+    std::vector<int> usrc[2];            // unit coupling sources (4 per unit)
+    std::vector<int> dsrc;               // unit diagonal sources (4 per unit)
+    bool fits = true;
+  };
+  std::vector<Part> parts((size_t)nstrip * nseg);
+  parallel_for(nstrip, [&](long long b0, long long b1) {
+    for (long long b = b0; b < b1; b++) {
+      const int a = I.bstart[b], e = I.bstart[b + 1];
+      // units of the strip (runs of one node's rows in one segment, at most 3)
+      std::vector<int> ustart, uof(e - a);
+      for (int i = a; i < e; i++) {
+        const bool same = i > a && group && (*group)[i] == (*group)[i - 1] &&
+                          segof(i) == segof(i - 1) && i - ustart.back() < 3;
+        if (!same) ustart.push_back(i);
+        uof[i - a] = (int)ustart.size() - 1;
+      }
+      auto inblock = [&](int i, int c) { return c >= a && c < e && segof(c) == segof(i); };
+      // entry pairs of a row (at least one) when its unit starts at u0 and ends before u1
+      auto row_pairs = [&](int i, int u0, int u1, bool lower) {
+        int w = 0;
+        for (int q = rp[i] + 1; q < rp[i + 1]; q++) {
+          const int c = cl[q];
+          w += (lower ? c < i : c > i) && inblock(i, c) && (c < u0 || c >= u1);
+        }
+        return std::max(1, (w + 1) / 2);
+      };
+      // a unit whose rows need more pairs than a level holds becomes single-row units (a
+      // row has at most 127 entries, 64 pairs)
+      {
+        std::vector<int> us2;
+        for (size_t U = 0; U < ustart.size(); U++) {
+          const int u0 = ustart[U], u1 = U + 1 < ustart.size() ? ustart[U + 1] : e;
+          int pl = 0, pu = 0;
+          for (int i = u0; i < u1; i++) { pl += row_pairs(i, u0, u1, true); pu += row_pairs(i, u0, u1, false); }
+          if (std::max(pl, pu) <= BS_LPAIRS) us2.push_back(u0);
+          else for (int i = u0; i < u1; i++) us2.push_back(i);
+        }
+        ustart.swap(us2);
+        for (size_t U = 0; U < ustart.size(); U++) {
+          const int u1 = U + 1 < ustart.size() ? ustart[U + 1] : e;
+          for (int i = ustart[U]; i < u1; i++) uof[i - a] = (int)U;
+        }
+      }
+      const int nu = (int)ustart.size();
+      ustart.push_back(e);
+      // unit levels, lower and upper
+      std::vector<int> ulev[2] = { std::vector<int>(nu, 0), std::vector<int>(nu, 0) };
+      for (int U = 0; U < nu; U++)
+        for (int i = ustart[U]; i < ustart[U + 1]; i++)
+          for (int q = rp[i] + 1; q < rp[i + 1]; q++) {
+            const int c = cl[q];
+            if (c < i && inblock(i, c) && uof[c - a] != U)
+              ulev[0][U] = std::max(ulev[0][U], ulev[0][uof[c - a]] + 1);
+          }
+      for (int U = nu - 1; U >= 0; U--)
+        for (int i = ustart[U]; i < ustart[U + 1]; i++)
+          for (int q = rp[i] + 1; q < rp[i + 1]; q++) {
+            const int c = cl[q];
+            if (c > i && inblock(i, c) && uof[c - a] != U)
+              ulev[1][U] = std::max(ulev[1][U], ulev[1][uof[c - a]] + 1);
+          }
+      // units of every segment
+      std::vector<std::vector<int>> segu(nseg);
+      for (int U = 0; U < nu; U++) segu[segof(ustart[U])].push_back(U);
+      for (int g = 0; g < nseg; g++) {
+        Part& P = parts[(size_t)b * nseg + g];
+        std::vector<int>& us = segu[g];
+        if (us.empty()) continue;
+        int m = 0;
+        for (int U : us) m += ustart[U + 1] - ustart[U];
+        if (m > BS_ROWS) { P.fits = false; continue; }
+        // lower order: by lower level, then unit index; local positions follow it
+        std::vector<int> ord[2] = { us, us };
+        for (int t = 0; t < 2; t++)
+          std::stable_sort(ord[t].begin(), ord[t].end(),
+                           [&](int x, int y) { return ulev[t][x] < ulev[t][y]; });
+        std::vector<int> upos(nu, -1);
+        for (int U : ord[0]) {
+          upos[U] = (int)P.rows.size();
+          for (int i = ustart[U]; i < ustart[U + 1]; i++) P.rows.push_back(i);
+        }
+        auto pos = [&](int c) { return upos[uof[c - a]] + (c - ustart[uof[c - a]]); };
+        for (int t = 0; t < 2; t++) {
+          const bool lower = t == 0;
+          std::vector<unsigned>& tab = P.tab[t];
+          std::vector<unsigned short>& meta = P.meta[t];
+          std::vector<int>& src = P.src[t];
+          int cur = -1, nrow = 0, npair = 0;
+          for (int k = 0; k < (int)ord[t].size(); k++) {
+            const int U = ord[t][k], r0 = ustart[U], nr = ustart[U + 1] - r0;
+            int pairs = 0;                      // the unit's entry pairs (a row at least one)
+            for (int i = r0; i < r0 + nr; i++) pairs += row_pairs(i, r0, r0 + nr, lower);
+            if (ulev[t][U] != cur || nrow + nr > cap_rows || npair + pairs > cap_pairs) {
+              tab.push_back(((unsigned)k << 21) | (unsigned)meta.size());
+              cur = ulev[t][U]; nrow = 0; npair = 0;
+            }
+            for (int j = 0; j < nr; j++) {
+              const int i = r0 + j;
+              const size_t start = meta.size();
+              for (int q = rp[i] + 1; q < rp[i + 1]; q++) {
+                const int c = cl[q];
+                if ((lower ? c < i : c > i) && inblock(i, c) && uof[c - a] != U) {
+                  meta.push_back((unsigned short)(pos(c) | ((nrow + j) << 11)));
+                  src.push_back(q);
+                }
+              }
+              while (meta.size() - start < 2 || ((meta.size() - start) & 1)) {   // zero entries
+                meta.push_back((unsigned short)(((nrow + j) << 11)));
+                src.push_back(-1);
+              }
+            }
+            P.unit[t].push_back((unsigned)upos[U] | ((unsigned)nr << 11) | ((unsigned)nrow << 13));
+            const int r1 = r0 + std::min(1, nr - 1), r2 = r0 + std::min(2, nr - 1);
+            if (lower) {
+              const int l10 = nr > 1 ? find(r1, r0) : -1, l20 = nr > 2 ? find(r2, r0) : -1;
+              const int l21 = nr > 2 ? find(r2, r1) : -1;
+              for (int v : { l10, l20, l21, -1 }) P.usrc[t].push_back(v);
+            } else {
+              const int u01 = nr > 1 ? find(r0, r1) : -1, u02 = nr > 2 ? find(r0, r2) : -1;
+              const int u12 = nr > 2 ? find(r1, r2) : -1;
+              for (int v : { u01, u02, u12, -1 }) P.usrc[t].push_back(v);
+              for (int v : { rp[r0], nr > 1 ? rp[r1] : -1, nr > 2 ? rp[r2] : -1, -1 })
+                P.dsrc.push_back(v);
+            }
+            nrow += nr;
+            npair += pairs;
+          }
+          tab.push_back(((unsigned)ord[t].size() << 21) | (unsigned)meta.size());
+          tab.push_back(tab.back());         // read by the prefetch past the last level
+          for (int z = 0; z < 2; z++) {      // a zero pair, loaded by the levels past the last
+            meta.push_back(0);
+            src.push_back(-1);
+          }
+          if (meta.size() > BS_EM) P.fits = false;
+        }
+      }
+    }
+  });
+  long long nb = 0, ent[2] = {0, 0}, nlev = 0, nunit = 0;
+  size_t lds = 0;
+  for (const Part& P : parts) {
+    if (!P.fits) return;
+    if (P.rows.empty()) continue;
+    nb++;
+    for (int t = 0; t < 2; t++) ent[t] += P.meta[t].size();
+    nlev += P.tab[0].size() + P.tab[1].size();
+    nunit += P.unit[0].size();
+    const size_t m = P.rows.size();
+    lds = std::max(lds, 8 * (m + BS_LROWS + 2) + 4 * (P.tab[0].size() + P.tab[1].size()));
+  }
+  if (nb == 0 || ent[0] + BS_PAD > INT_MAX || ent[1] + BS_PAD > INT_MAX) return;
+  std::vector<int4> info(2 * nb);
+  std::vector<int> gidx(n);
+  std::vector<int> src[2], usrc[2];
+  std::vector<unsigned short> meta[2];
+  std::vector<unsigned> unit[2];
+  for (int t = 0; t < 2; t++) {
+    src[t].assign(ent[t] + BS_PAD, -1);
+    meta[t].assign(ent[t] + BS_PAD, 0);
+    unit[t].assign(nunit + 64, 1u << 11);
+    usrc[t].assign((size_t)4 * (nunit + 64), -1);
+  }
+  std::vector<int> dsrc((size_t)4 * (nunit + 64), -1);
+  std::vector<unsigned> lev(nlev);
+  long long k = 0, rb = 0, lb = 0, eo[2] = {0, 0}, ub = 0;
+  for (const Part& P : parts) {
+    if (P.rows.empty()) continue;
+    const int m = (int)P.rows.size();
+    const int nl = (int)P.tab[0].size() - 2, nu = (int)P.tab[1].size() - 2;
+    info[2 * k] = int4{ (int)rb, m, (int)lb, nl };
+    info[2 * k + 1] = int4{ nu, (int)ub, (int)eo[0], (int)eo[1] };
+    std::copy(P.tab[0].begin(), P.tab[0].end(), lev.begin() + lb);
+    std::copy(P.tab[1].begin(), P.tab[1].end(), lev.begin() + lb + nl + 2);
+    std::copy(P.rows.begin(), P.rows.end(), gidx.begin() + rb);
+    for (int t = 0; t < 2; t++) {
+      std::copy(P.meta[t].begin(), P.meta[t].end(), meta[t].begin() + eo[t]);
+      std::copy(P.src[t].begin(), P.src[t].end(), src[t].begin() + eo[t]);
+      std::copy(P.unit[t].begin(), P.unit[t].end(), unit[t].begin() + ub);
+      std::copy(P.usrc[t].begin(), P.usrc[t].end(), usrc[t].begin() + 4 * ub);
+      eo[t] += P.meta[t].size();
+    }
+    std::copy(P.dsrc.begin(), P.dsrc.end(), dsrc.begin() + 4 * ub);
+    k++; rb += m; lb += nl + nu + 4; ub += P.unit[0].size();
+  }
+  B.nb = (int)nb; B.lds = (lds + 15) & ~(size_t)15;
+  B.ne_lo = std::count_if(src[0].begin(), src[0].end(), [](int q) { return q >= 0; });
+  B.ne_up = std::count_if(src[1].begin(), src[1].end(), [](int q) { return q >= 0; });
+  B.nlev = nlev; B.nunit = nunit;
+  B.info.upload(info, s); B.gidx.upload(gidx, s); B.lev.upload(lev, s);
+  B.mlo.upload(meta[0], s); B.mup.upload(meta[1], s);
+  B.slo.upload(src[0], s); B.sup.upload(src[1], s);
+  B.ulo.upload(unit[0], s); B.uup.upload(unit[1], s);
+  B.sulo.upload(usrc[0], s); B.suup.upload(usrc[1], s); B.sdg.upload(dsrc, s);
+  HIPCHECK(hipStreamSynchronize(s));
+  B.on = true;
+  if (getenv("RMX_DEBUG"))
+    fprintf(stderr, "[rmx] block sweeps: %lld blocks, %lld nodes, %lld + %lld entries (%lld + "
+            "%lld padded), %.1f levels per block, %zu bytes LDS\n", nb, nunit, B.ne_lo, B.ne_up,
+            ent[0], ent[1], (double)nlev / nb - 4.0, B.lds);
 }
 
 // Segment of every row: each strip [bstart[b], bstart[b+1]) is cut into nseg parts of
@@ -1278,6 +1759,7 @@ void ilu_setup(Ilu& I, int n, const std::vector<int>& rp, const std::vector<int>
   std::vector<int> olo, oup;
   build_sweep(I.lo, I, true, olo, s, cap);
   build_sweep(I.up, I, false, oup, s, cap);
+  build_blocks(I, group, s);
   // forward position (global, concatenated) of the row at each backward position
   std::vector<int> fpos(n), f2b(n);
   for (int r = 0; r < n; r++) fpos[olo[r]] = r;
@@ -1299,6 +1781,20 @@ template <> DBuf<float>& PLbuf<float>(Ilu& I) { return I.plo32; }
 template <class VT> static DBuf<VT>& PUbuf(Ilu& I);
 template <> DBuf<double>& PUbuf<double>(Ilu& I) { return I.pup64; }
 template <> DBuf<float>& PUbuf<float>(Ilu& I) { return I.pup32; }
+// block sweep values of one precision: entries (lower, upper), unit couplings (lower, upper)
+template <class VT> struct BSvals {
+  DBuf<VT> *vlo, *vup, *wlo, *wup;
+  DBuf<double>* rdg;
+};
+template <class VT> static BSvals<VT> bsvals(BlockSweep& B);
+template <> BSvals<double> bsvals<double>(BlockSweep& B)
+{
+  return { &B.vlo64, &B.vup64, &B.wlo64, &B.wup64, &B.rdg64 };
+}
+template <> BSvals<float> bsvals<float>(BlockSweep& B)
+{
+  return { &B.vlo32, &B.vup32, &B.wlo32, &B.wup32, &B.rdg32 };
+}
 template <class VT> static DBuf<VT>& PDbuf(Ilu& I);
 template <> DBuf<double>& PDbuf<double>(Ilu& I) { return I.pdg64; }
 template <> DBuf<float>& PDbuf<float>(Ilu& I) { return I.pdg32; }
@@ -1378,6 +1874,19 @@ static void factor_t(Ctx& c, Ilu& I, const VT* A)
   if (I.lo.ne) k_pack<VT><<<4096, 256, 0, c.stream>>>(I.lo.ne, I.lo.psrc.p, LU.p, pl.p);
   if (I.up.ne) k_pack<VT><<<4096, 256, 0, c.stream>>>(I.up.ne, I.up.psrc.p, LU.p, pu.p);
   k_pack_diag<VT><<<(I.n + 255) / 256, 256, 0, c.stream>>>(I.n, I.up.prow.p, I.rp.p, LU.p, pd.p);
+  if (I.bs.on) {                                // block sweeps: values in their entry order
+    BlockSweep& B = I.bs;
+    const BSvals<VT> V = bsvals<VT>(B);
+    const std::pair<DBuf<VT>*, const DBuf<int>*> packs[4] = {
+      { V.vlo, &B.slo }, { V.vup, &B.sup }, { V.wlo, &B.sulo }, { V.wup, &B.suup } };
+    for (const auto& pk : packs) {
+      pk.first->alloc(pk.second->n);
+      k_pack<VT><<<4096, 256, 0, c.stream>>>((long long)pk.second->n, pk.second->p, LU.p,
+                                             pk.first->p);
+    }
+    V.rdg->alloc(B.sdg.n);
+    k_pack_rcp<VT><<<4096, 256, 0, c.stream>>>((long long)B.sdg.n, B.sdg.p, LU.p, V.rdg->p);
+  }
   HIPCHECK(hipGetLastError());
   int h_err = 0;
   HIPCHECK(hipMemcpyAsync(&h_err, err, sizeof(int), hipMemcpyDeviceToHost, c.stream));
@@ -1435,18 +1944,47 @@ static void launch_sweeps(Ctx& c, Ilu& I, const double* b, double* x, hipEvent_t
   HIPCHECK(hipGetLastError());
 }
 
-// RMX_SWEEP=0 selects the one-deep sweep kernel (tuning and comparison); default staged.
+// Both sweeps of every block in one launch (one wave per block).
+template <class VT>
+static void launch_blocks(Ctx& c, Ilu& I, const double* b, double* x, hipEvent_t* ev,
+                          const int* aidx, const double* aval)
+{
+  BlockSweep& B = I.bs;
+  const BSvals<VT> V = bsvals<VT>(B);
+  if (ev) HIPCHECK(hipEventRecord(ev[0], c.stream));
+  // algorithmic bytes: entries (value + 2-byte column), per node its two records, 8 coupling
+  // values and 3 FP64 reciprocal diagonals (stored as 4), per row the gather index,
+  // right-hand side and result, per level its table entry, and the folded prolongation
+  const double v = (double)sizeof(VT);
+  const double bytes = (double)(B.ne_lo + B.ne_up) * (v + 2.0) + (double)B.nunit * (32.0 + 8.0 * v) +
+                       20.0 * I.n + 4.0 * B.nlev + (aidx ? 12.0 * I.n : 0.0);
+  const int kt = kt_begin(c, KT_SWEEP_BLK, bytes);
+  k_block_sweep<VT><<<B.nb, 64, B.lds, c.stream>>>(B.info.p, B.gidx.p, B.lev.p,
+      V.vlo->p, B.mlo.p, B.ulo.p, V.wlo->p, V.vup->p, B.mup.p, B.uup.p, V.wup->p,
+      reinterpret_cast<const double4*>(V.rdg->p), b, x, aidx, aval);
+  kt_end(c, kt);
+  if (ev) {
+    HIPCHECK(hipEventRecord(ev[1], c.stream));
+    HIPCHECK(hipEventRecord(ev[2], c.stream));
+  }
+  HIPCHECK(hipGetLastError());
+}
+
+// RMX_SWEEP selects the sweep kernels (tuning and comparison): 0 the one-deep strip sweep,
+// 1 the staged strip sweep, 2 (default) the block sweeps where the layout exists, else 1.
 static int sweep_variant()
 {
   const char* e = getenv("RMX_SWEEP");
-  return (e && atoi(e) == 0) ? 0 : 1;
+  return e ? atoi(e) : 2;
 }
 
 template <class VT>
 static bool apply_t(Ctx& c, Ilu& I, const double* b, double* x, hipEvent_t* ev, const int* aidx,
                     const double* aval)
 {
-  if (sweep_variant() == 1) { launch_sweeps<VT, 1>(c, I, b, x, ev, aidx, aval); return aidx != nullptr; }
+  const int v = sweep_variant();
+  if (v >= 2 && I.bs.on) { launch_blocks<VT>(c, I, b, x, ev, aidx, aval); return aidx != nullptr; }
+  if (v != 0) { launch_sweeps<VT, 1>(c, I, b, x, ev, aidx, aval); return aidx != nullptr; }
   launch_sweeps<VT, 0>(c, I, b, x, ev, nullptr, nullptr);
   return false;
 }

@@ -125,6 +125,32 @@ struct SweepSet {
   DBuf<double> xp;
 };
 
+// Block sweeps of the strip ILU(0): one wave per block (a segment of a strip), the lower
+// and the upper sweep of the block in one launch with the block's values in LDS.  The unit
+// of a sweep level is a node (the run of its rows in the block, up to 3): the entries that
+// couple a node's rows are solved by one lane in registers, so a node adds one level, not
+// three.  Local positions are the rows in lower-sweep order.  A level holds at most 32 rows
+// and 64 entry pairs (a row's entries are padded to an even count, so no lane holds two
+// rows' entries).
+struct BlockSweep {
+  bool on = false;
+  int nb = 0;                      // blocks
+  size_t lds = 0;                  // dynamic LDS bytes of the largest block
+  long long ne_lo = 0, ne_up = 0, nlev = 0, nunit = 0;
+  DBuf<int4> info;                 // [2 nb] {row base, rows, level table, lower levels},
+                                   //        {upper levels, unit base, lower / upper entry base}
+  DBuf<int> gidx;                  // [n] equation at every local position
+  DBuf<unsigned> lev;              // per level: first unit (<< 21) | first entry in the block
+  DBuf<unsigned short> mlo, mup;   // per entry: local column | row in its level << 11
+  DBuf<unsigned> ulo, uup;         // per unit: position | rows << 11 | first row in level << 13
+  DBuf<int> slo, sup;              // CSR position of every entry value (-1: zero)
+  DBuf<int> sulo, suup, sdg;       // per unit: CSR positions of its lower / upper couplings
+                                   // and of its diagonals (4 each; -1: none)
+  DBuf<float> vlo32, vup32, wlo32, wup32;       // packed values (entries, unit couplings)
+  DBuf<double> vlo64, vup64, wlo64, wup64;
+  DBuf<double> rdg32, rdg64;       // reciprocal diagonals of the FP32 / FP64 factors
+};
+
 // One ILU(0) system: pattern, factorisation schedule, sweeps and factors.
 struct Ilu {
   bool ready = false;
@@ -139,6 +165,7 @@ struct Ilu {
   DBuf<long long> fpbase;       // per level position: offset of its plan entries
   DBuf<unsigned short> plan;    // per plan entry: pivot index | target position << 8
   SweepSet lo, up;
+  BlockSweep bs;
   DBuf<double> LU64, plo64, pup64, pdg64;
   DBuf<float> LU32, plo32, pup32, pdg32;
   bool fac64 = false, fac32 = false;
@@ -221,7 +248,8 @@ struct Mpnn {
 
 // Launch timing per kernel class (opt-in): a pair of HIP events on the context stream
 // around every launch of the class, read back when the stream has synchronised.
-enum { KT_FACTOR = 0, KT_SWEEP_LO, KT_SWEEP_UP, KT_SPMV, KT_COARSE, KT_GS, KT_RESID64, KT_N };
+enum { KT_FACTOR = 0, KT_SWEEP_LO, KT_SWEEP_UP, KT_SPMV, KT_COARSE, KT_GS, KT_RESID64, KT_SWEEP_BLK,
+       KT_N };
 struct KTimer {
   bool on = false;
   std::vector<hipEvent_t> ev;                 // event pool: pair k = ev[2k], ev[2k+1]

@@ -104,7 +104,7 @@ class KernelStats(C.Structure):
 
 # kernel classes of rmx_kernel_stats
 KERNEL_CLASSES = ("ilu_factor", "sweep_lower", "sweep_upper", "spmv", "coarse", "gram_schmidt",
-                  "resid_fp64")
+                  "resid_fp64", "sweeps_block")
 
 _LIB = None
 

@@ -241,12 +241,20 @@ def _check_blocks(block, n, strips, segments, group):
     assert (block[1:][same] == block[:-1][same]).all()
 
 
-@pytest.mark.parametrize("strips,segments", [(1, 1), (4, 1), (17, 1), (4, 8), (1, 16)])
-def test_strip_ilu_matches_oracle(ctx, strips, segments):
+@pytest.mark.parametrize("sweep", ["block", "staged"])
+@pytest.mark.parametrize("strips,segments", [(1, 1), (4, 1), (17, 1), (4, 8), (1, 16), (17, 4),
+                                             (4, 32)])
+def test_strip_ilu_matches_oracle(ctx, strips, segments, sweep, monkeypatch):
     """Strip ILU(0) = the oracle's ILU(0) factors of the whole matrix, swept with the
     entries that couple two blocks left out: FP64 and FP32 apply agree with the oracle's
-    triangular solves on that restricted pattern to 1e-12 of max|result|."""
-    print(f"[gpu] strip ILU, {strips} strips x {segments} segments", flush=True)
+    triangular solves on that restricted pattern to 1e-12 of max|result|.  Blocks of at
+    most 2047 rows run the block sweeps (one wave per block, both triangles in one launch)
+    unless RMX_SWEEP=1 selects the staged strip sweeps; larger blocks run the staged ones."""
+    print(f"[gpu] strip ILU, {strips} strips x {segments} segments, {sweep} sweeps", flush=True)
+    if sweep == "staged":
+        monkeypatch.setenv("RMX_SWEEP", "1")
+    else:
+        monkeypatch.delenv("RMX_SWEEP", raising=False)
     g = load_golden("channel200x20")
     ctx.upload_mesh(golden_deck(g))
     n = ctx.prepare()
@@ -272,12 +280,47 @@ def test_strip_ilu_matches_oracle(ctx, strips, segments):
         if nb * segments > 1:
             assert len(L2) < len(L)
         ref = po.ilu0_solve(rp2, c2, L2, B)
+        ctx.kernel_stats(True)
         y = ctx.silu0_apply(B, prec)
+        ran = ctx.kernel_stats(False)
+        blocks = sweep == "block" and max(np.bincount(block)) <= 2047
+        assert (ran["sweeps_block"][2] == 1) == blocks and (ran["sweep_lower"][2] == 1) != blocks
         assert np.abs(y - ref).max() <= 1e-12 * np.abs(ref).max(), prec
     # one block is the reference-order ILU(0)
     if nb * segments == 1:
         ctx.ilu0_factor(rmx.F32)
         np.testing.assert_array_equal(ctx.ilu0_apply(B, rmx.F32), y)
+
+
+@pytest.mark.parametrize("rows,pairs", [(3, 64), (7, 64), (32, 5), (3, 1)])
+def test_block_sweeps_split_levels(ctx, rows, pairs, monkeypatch):
+    """Block sweeps with levels split to at most `rows` rows and `pairs` entry pairs (a node
+    needing more sits alone in its level): FP64 and FP32 apply within 1e-12 of the oracle's
+    triangular solves of the block-restricted factors."""
+    print(f"[gpu] block sweeps, levels of at most {rows} rows / {pairs} entry pairs", flush=True)
+    monkeypatch.delenv("RMX_SWEEP", raising=False)
+    monkeypatch.setenv("RMX_BLOCK_LEVEL_ROWS", str(rows))
+    monkeypatch.setenv("RMX_BLOCK_LEVEL_PAIRS", str(pairs))
+    g = load_golden("channel200x20")
+    ctx.upload_mesh(golden_deck(g))
+    ctx.prepare()
+    ctx.set_strips(17, 4)
+    ctx.assemble(rmx.F64)
+    ctx.scale_l2()
+    rp, cols = ctx.pattern()
+    A, B = ctx.matrix(rmx.F64)
+    block = ctx.blocks()
+    for prec, dt in ((rmx.F64, np.float64), (rmx.F32, np.float32)):
+        if prec == rmx.F32:
+            ctx.convert(rmx.F32)
+        ctx.silu0_factor(prec)
+        L, _ = po.ilu0_factor(rp, cols, A.astype(dt))
+        rp2, c2, L2 = _restrict(rp, cols, L, block)
+        ref = po.ilu0_solve(rp2, c2, L2, B)
+        ctx.kernel_stats(True)
+        y = ctx.silu0_apply(B, prec)
+        assert ctx.kernel_stats(False)["sweeps_block"][2] == 1
+        assert np.abs(y - ref).max() <= 1e-12 * np.abs(ref).max(), prec
 
 
 @pytest.mark.parametrize("coarse_solve", ["device", "host"])
