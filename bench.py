@@ -74,12 +74,6 @@ def log(msg):
     print(f"[bench] {msg}", file=sys.stderr, flush=True)
 
 
-def sweep_bytes(entries, rows, vbytes):
-    """Algorithmic bytes of one triangular sweep: every kept factor entry (value + column)
-    once, per row its entry offset, diagonal (upper sweep), right-hand side and result."""
-    return entries * (vbytes + 4) + rows * (4 + 8 + 8)
-
-
 def reach_deck(n_along, n_across=CROSS):
     """A reach of the benchmark's meander generator at the benchmark's resolution."""
     from rismo2d_amd import deck as dk
@@ -158,30 +152,32 @@ def cpu_baseline_port(na):
 
 def kernels_only(c, n, a):
     """The system of the first Newton step (FP64 assembly, L2 scaling, FP32 copy), its strip
-    ILU(0) in FP32, five timed SpMV / sweep pairs, then three plain copies of an n-vector
-    (the identity preconditioner's device copy: 8n bytes read, 8n written), the reference
-    size for calibrating HBM byte counters."""
+    ILU(0) and coarse space in FP32, six timed FP32 SpMVs and five applications of the
+    two-level preconditioner (its block sweeps with the coarse prolongation folded, as in the
+    Newton step), then three plain copies of an n-vector (the identity preconditioner's
+    device copy: 8n bytes read, 8n written), the reference size for calibrating HBM byte
+    counters."""
     from rismo2d_amd import rmx as R
     c.assemble(R.F64)
     c.scale_l2()
     c.convert(R.F32)
     c.silu0_factor(R.F32)
-    c.kernel_stats(True)
-    kt = c.profile_kernels(R.F32, 5)
-    ks = c.kernel_stats(False)
+    c.coarse_factor(R.F32)
     x = np.random.default_rng(0).standard_normal(n)
+    c.spmv(x, R.F32)                          # brings the row-group values up to date
+    c.kernel_stats(True)
+    for _ in range(6):
+        c.spmv(x, R.F32)
+    for _ in range(5):
+        c.precond_apply(R.PREC_SILU0C, x, R.F32)
+    ks = c.kernel_stats(False)
     for _ in range(3):
         c.precond_apply(R.PREC_NONE, x, R.F32)
-    rec = {"n": n, "nnz": c.nnz, "spmv_ms": kt.spmv_s * 1e3, "spmv_bytes": kt.spmv_bytes,
-           "copy_bytes_each_way": 8 * n}
-    sec, byt, cnt = ks["sweeps_block"]
-    if cnt:                                   # both triangles in one launch per apply
-        rec.update(block_sweep_ms=sec / cnt * 1e3, block_sweep_bytes=byt / cnt)
-    else:
-        lo_e, up_e = c.sweep_entries()
-        rec.update(sweep_fwd_ms=kt.silu_fwd_s * 1e3, sweep_bwd_ms=kt.silu_bwd_s * 1e3,
-                   sweep_fwd_bytes=sweep_bytes(lo_e, n, 4),
-                   sweep_bwd_bytes=sweep_bytes(up_e, n, 4) + 4 * n)
+    rec = {"n": n, "nnz": c.nnz, "copy_bytes_each_way": 8 * n}
+    for name in ("spmv", "sweeps_block", "sweep_lower", "sweep_upper"):
+        sec, byt, cnt = ks[name]
+        if cnt:
+            rec[name] = {"launches": cnt, "avg_ms": sec / cnt * 1e3, "bytes": byt / cnt}
     print(json.dumps(rec), flush=True)
     c.close()
 
