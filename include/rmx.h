@@ -195,8 +195,10 @@ rmx_status rmx_ilu0_apply(rmx_ctx* ctx, int prec, const double* b, double* x);
  * reference's ILU(0) factors (PRECO_ILU0::Factor) applied with block-Jacobi triangular
  * sweeps (PRECO_ILU0::Solve, Preco_ilu0.cpp:497, without the factor entries that couple two
  * blocks).  A block is one of `segments` parts of a strip of consecutive equations, cut
- * along the principal axis of the strip's node coordinates; every strip is swept by one
- * workgroup.  rmx_set_strips chooses the number of strips (0 = two per compute unit,
+ * along the principal axis of the strip's node coordinates.  Blocks of at most 2047 rows are
+ * swept by one wave each, both triangles in one launch (a node's rows solved together);
+ * larger blocks by one workgroup per strip and triangle.  rmx_set_strips chooses the
+ * number of strips (0 = two per compute unit,
  * the default, and at most one per 2048 rows) and segments (0 = the default: blocks of
  * about 560 rows, at most 32 segments per strip; 1 for a system loaded with
  * rmx_csr_upload, which has no coordinates) and rebuilds the schedule; rmx_get_blocks
@@ -219,15 +221,19 @@ rmx_status rmx_coarse_factor(rmx_ctx* ctx, int prec);
 rmx_status rmx_get_aggregates(rmx_ctx* ctx, int* agg /* [neq] */, int* nc);
 
 /* Learned preconditioner (RMX_PREC_MPNN).  No reference counterpart.  A message-passing
- * network on the mesh node graph (an edge joins two nodes of one element): node features
- * (u, v, depth/4, bed/4, 100 cf, residual u/v/h components / RMS(r)), an input layer
- * 8 -> 64, 6 layers h = h + relu([h, mean of neighbours' h] W + b) (BF16 MFMA, FP32
- * accumulate), an output layer 64 -> 3; n(r) = RMS(r) * network(r) as an equation vector.
- * As a preconditioner it corrects the two-level preconditioner M2 (RMX_PREC_SILU0C, which
- * must be factored): z2 = M2^-1 r, r2 = r - A z2, z = z2 + n(r2), where n(r2) is kept only
- * if ||r2 - A n(r2)|| < ||r2|| (decided on the device, per application).
+ * network on the mesh node graph (an edge joins two nodes of one element) that sees the
+ * state, not the residual, so the preconditioner it defines is linear in the residual:
+ * node features (u, v, depth/4, bed/4, 100 cf, speed, open-boundary flag, rotated-node
+ * flag), an input layer 8 -> 64, 6 layers h = h + relu([h, mean of neighbours' h] W + b)
+ * (BF16 MFMA, FP32 accumulate), an output layer 64 -> 18: per node two 3x3 blocks D_i, E_i.
+ * The correction of a residual r: q = r / max_j |a_ij| (row-scaled), q_i its (u, v, h)
+ * components at node i, qbar_i their neighbour mean, n(r)_i = D_i q_i + E_i qbar_i.  As a
+ * preconditioner it corrects the two-level preconditioner M2 (RMX_PREC_SILU0C, which must
+ * be factored): z2 = M2^-1 r, r2 = r - A z2, z = z2 + n(r2).  The network runs once per
+ * Newton system; at the system's first application the correction is tested on that
+ * vector and dropped for the whole system if it does not lower the residual.
  * rmx_mpnn_n_params: number of FP32 weights (layout: W_in [8][64], b_in, then per layer
- * W [128][64], b [64], then W_out [64][3], b_out [3]).
+ * W [128][64], b [64], then W_out [64][18], b_out [18]).
  * rmx_mpnn_set_weights / get_weights: the weights (host, FP32); setting also resets Adam.
  * rmx_mpnn_forward: z = n(r) alone, and the node features [np][8] (nullable).
  * rmx_mpnn_add_sample: store a training sample: the features of the current state and r,
