@@ -53,7 +53,7 @@ def parse():
     ap.add_argument("--n-across", type=int, default=CROSS)
     ap.add_argument("--strips", type=int, default=0)
     ap.add_argument("--segments", type=int, default=0, help="segments per strip (0: blocks of about 560 rows)")
-    ap.add_argument("--coarse", type=int, default=8, help="coarse segments per strip (0: none)")
+    ap.add_argument("--coarse", type=int, default=24, help="coarse segments per strip (0: none)")
     ap.add_argument("--max-iter", type=int, default=1000, help="Krylov iterations per refinement sweep")
     ap.add_argument("--mkyrl", type=int, default=15, help="FGMRES restart length")
     ap.add_argument("--eps", type=float, default=1e-2, help="inner relative tolerance per sweep")

@@ -73,7 +73,14 @@ typedef struct {
 } rmx_mesh_desc;
 
 typedef struct {
-  int relax_method;           /* 0: none, 2: cap by max_delta_uv / max_delta_s */
+  int relax_method;           /* 0: none, 2: cap by max_delta_uv / max_delta_s, 3 and 4 (the
+                                 reference's default): time-relaxed Newton steps, a relaxed
+                                 time step in the Jacobian that starts from relax_time at every
+                                 rmx_set_params and rmx_time_step and grows by the ratio of the
+                                 maximum change to the largest correction up to the time step
+                                 (rmx_time_step's dt, or interval for stationary flow); the
+                                 Newton test fails until it gets there (EqsUVS2D.cpp:97-147,
+                                 540-627); the correction is capped as by method 2 */
   int dry_method;             /* 0,2,3: marsh bed lowering; else depth clip */
   double dry_limit, max_delta_uv, max_delta_s, conv_uv, conv_s, max_us;
   int turb;                   /* turbulence model, the $TM_TURBULENCE code (Timeint.cpp:1333):
@@ -85,6 +92,11 @@ typedef struct {
                                  step (kVtIterat), else at the first step after a mesh,
                                  state or parameter upload or rmx_turbulence */
   double cm, cd;              /* k-epsilon constants ($KDCONST), <= 0: 0.09, 1.0 */
+  double relax_time;          /* methods 3 and 4: relaxTimeFlow, the relaxed time step a cycle
+                                 starts from ($TM_INTERVAL's second value; <= 0 or above the
+                                 time step: the time step) */
+  double interval;            /* methods 3 and 4, stationary flow: the cycle's time step
+                                 ($TM_INTERVAL's first value) */
 } rmx_params;
 
 /* Mesh, boundary lines and material data (MODEL::Input, MODEL::Initialize,
@@ -212,7 +224,7 @@ rmx_status rmx_get_blocks(rmx_ctx* ctx, int* block /* [neq] */);
  * per (strip, coarse segment, variable U/V/H), coarse segments cut like the sweep
  * segments; P is piecewise constant on aggregates and R = P^T.  Apply:
  * z1 = P (R A P)^-1 R r, z = z1 + M^-1 (r - A z1) with M the strip ILU(0).
- * rmx_set_coarse chooses the coarse segments per strip (default 8; 0: no coarse space; needs a mesh,
+ * rmx_set_coarse chooses the coarse segments per strip (default 24; 0: no coarse space; needs a mesh,
  * i.e. not after rmx_csr_upload); rmx_coarse_factor sums R A P from the FP32 or FP64
  * matrix and factors it; rmx_get_aggregates writes the aggregate of every equation and
  * returns the number of aggregates in *nc. */
@@ -326,10 +338,13 @@ typedef struct {
   double maxAbs[3];           /* max |dU|, |dV|, |dh| of the correction (EQS::Update) */
   double avAbs[3];
   double relaxUs, relaxS;
-  int nrconv;                 /* max corrections below conv_uv / conv_s */
+  int nrconv;                 /* max corrections below conv_uv / conv_s (methods 3, 4: and the
+                                 relaxed time step at the time step) */
   double scale;               /* ||B||_2 before scaling */
   rmx_krylov_out solve;
   double t_prepare, t_assemble, t_factor, t_solve, t_update;   /* seconds */
+  double relax_time;          /* methods 3, 4: the relaxed time step after this update */
+  double relax_time_last;     /* and the one this step's Jacobian used (last_dt) */
 } rmx_step_stats;
 
 /* Relaxation, correction (rotated back for rotated nodes), velocity cap, depth / marsh

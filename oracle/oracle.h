@@ -58,11 +58,17 @@ void   orc_get_turbulence(const orc_model* m, double* vt, double* uu);
 /* the nodal Elder tensor exx, exy, eyy (global axes) the last prepare left */
 void   orc_get_elder(const orc_model* m, double* exx, double* exy, double* eyy);
 void   orc_destroy(orc_model* m);
-/* relaxMethod (0|2), dry method (0|2|3 marsh, else clip), dryLimit, maxDeltaUV, maxDeltaS,
- * convUV, convS, maxUs */
+/* relaxMethod (0|2|3|4), dry method (0|2|3 marsh, else clip), dryLimit, maxDeltaUV,
+ * maxDeltaS, convUV, convS, maxUs */
 void   orc_set_params(orc_model* m, int relaxMethod, int dryMethod, double dryLimit,
                       double maxDeltaUV, double maxDeltaS, double convUV, double convS,
                       double maxUs);
+/* relaxation methods 3 and 4: relaxTimeFlow (the relaxed time step a cycle starts from;
+ * <= 0 or above the time step: the time step) and the stationary cycle's time interval
+ * ($TM_INTERVAL); both restart the cycle's relaxed time.  get: the relaxed time steps of
+ * U, V and of h, and the one before the latest update */
+void   orc_set_relax_time(orc_model* m, double relaxTimeFlow, double interval);
+void   orc_get_relax_time(const orc_model* m, double* dt_UV, double* dt_H, double* last_dt);
 void   orc_set_state(orc_model* m, const double* U, const double* V, const double* S);
 void   orc_get_state(const orc_model* m, double* U, double* V, double* S, double* z);
 void   orc_get_nodal(const orc_model* m, double* cf, int* bckind, int* rot,

@@ -60,6 +60,8 @@ def lib():
         L.orc_lindner.argtypes = [C.c_double] * 7
         L.orc_lindner.restype = C.c_double
         L.orc_set_params.argtypes = [C.c_void_p, C.c_int, C.c_int] + [C.c_double] * 6
+        L.orc_set_relax_time.argtypes = [C.c_void_p, C.c_double, C.c_double]
+        L.orc_get_relax_time.argtypes = [C.c_void_p, _d, _d, _d]
         L.orc_set_state.argtypes = [C.c_void_p, _d, _d, _d]
         L.orc_get_state.argtypes = [C.c_void_p, _d, _d, _d, _d]
         L.orc_get_nodal.argtypes = [C.c_void_p, _d, _i, _i, _d, _d, _d]
@@ -220,9 +222,20 @@ class OracleModel:
             self.h = None
 
     def set_params(self, relax_method=2, dry_method=2, dry_limit=1e-3, max_delta_uv=10.0,
-                   max_delta_s=10.0, conv_uv=1e-7, conv_s=1e-7, max_us=100.0):
+                   max_delta_s=10.0, conv_uv=1e-7, conv_s=1e-7, max_us=100.0, relax_time=None,
+                   interval=None):
+        """Project parameters; relax_time (relaxTimeFlow) and interval (the stationary
+        cycle's time step) for relaxation methods 3 and 4."""
         lib().orc_set_params(self.h, relax_method, dry_method, dry_limit, max_delta_uv,
                              max_delta_s, conv_uv, conv_s, max_us)
+        if relax_time is not None or interval is not None:
+            lib().orc_set_relax_time(self.h, relax_time or 0.0, interval or 0.0)
+
+    def relax_time(self):
+        """The relaxed time steps of U, V and of h, and the one before the last update."""
+        v = [np.zeros(1) for _ in range(3)]
+        lib().orc_get_relax_time(self.h, *[_p(a, _d) for a in v])
+        return tuple(float(a[0]) for a in v)
 
     def set_state(self, U, V, S):
         U = np.ascontiguousarray(U, np.float64); V = np.ascontiguousarray(V, np.float64)

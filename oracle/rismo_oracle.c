@@ -248,6 +248,10 @@ struct orc_model {
   double g, vk, hmin, dryLimit, maxUs, maxDeltaUV, maxDeltaS, convUV, convS;
   double relaxThdt_UV, relaxThdt_H;
   int relaxMethod, dryMethod;
+  /* relaxation methods 3 and 4: the minimum (initial) relaxed time step relaxTimeFlow and
+   * the stationary cycle's time interval; the relaxed time steps of the cycle and the last
+   * one before the latest update (EqsUVS2D.cpp:97-101, 186, 540-627) */
+  double relaxDt, interval, dt_UV, dt_H, last_dt;
   /* drying and rewetting (DRYREW, Grid.h): per node the count down and whether it was dry
    * before the last call, per region element its flags and whether it was dry before */
   double rewetLimit;
@@ -407,6 +411,25 @@ void orc_destroy(orc_model* m)
   free(m);
 }
 
+/* The time terms' diagonal at the start of a flow cycle (EQS_UVS2D::Execute, EqsUVS2D.cpp:
+ * 97-147): relaxation methods 3 and 4 start from the relaxed time step relaxTimeFlow (the
+ * stationary cycle: 1 / dt_r; unsteady: 1 / dt_r / theta), the others use no time term
+ * (stationary) or the time step (1 / dt / theta). */
+static void cycle_start(orc_model* m)
+{
+  const double dt = m->unsteady ? m->dt : m->interval;
+  m->dt_UV = m->dt_H = m->last_dt = m->relaxDt > 0.0 && m->relaxDt <= dt ? m->relaxDt : dt;
+  if (m->relaxMethod >= 3) {
+    const double th = m->unsteady ? m->theta : 1.0;
+    m->relaxThdt_UV = 1.0 / m->dt_UV / th;
+    m->relaxThdt_H = 1.0 / m->dt_H / th;
+  } else if (m->unsteady) {
+    m->relaxThdt_UV = m->relaxThdt_H = 1.0 / m->dt / m->theta;
+  } else {
+    m->relaxThdt_UV = m->relaxThdt_H = 0.0;
+  }
+}
+
 void orc_set_params(orc_model* m, int relaxMethod, int dryMethod, double dryLimit,
                     double maxDeltaUV, double maxDeltaS, double convUV, double convS,
                     double maxUs)
@@ -415,9 +438,21 @@ void orc_set_params(orc_model* m, int relaxMethod, int dryMethod, double dryLimi
   m->hmin = dryLimit / 2.0;                     /* Project.cpp:1125 */
   m->maxDeltaUV = maxDeltaUV; m->maxDeltaS = maxDeltaS;
   m->convUV = convUV; m->convS = convS; m->maxUs = maxUs;
-  if (!m->unsteady) {           /* steady, relaxMethod < 3 (EqsUVS2D.cpp:124) */
-    m->relaxThdt_UV = 0.0; m->relaxThdt_H = 0.0;
-  }
+  cycle_start(m);
+}
+
+void orc_set_relax_time(orc_model* m, double relaxTimeFlow, double interval)
+{
+  m->relaxDt = relaxTimeFlow;
+  m->interval = interval;
+  cycle_start(m);
+}
+
+void orc_get_relax_time(const orc_model* m, double* dt_UV, double* dt_H, double* last_dt)
+{
+  if (dt_UV) *dt_UV = m->dt_UV;
+  if (dt_H) *dt_H = m->dt_H;
+  if (last_dt) *last_dt = m->last_dt;
 }
 
 void orc_set_state(orc_model* m, const double* U, const double* V, const double* S)
@@ -476,13 +511,12 @@ int orc_time_step(orc_model* m, double dt, double theta, int first)
   m->turb_stale = 1;
   if (dt <= 0.0) {
     m->unsteady = 0;
-    m->relaxThdt_UV = m->relaxThdt_H = 0.0;
+    cycle_start(m);
     return 0;
   }
   if (fabs(theta) < 1.0e-10 || fabs(dt) < 1.0e-10) return -1;
   m->unsteady = 1; m->dt = dt; m->theta = theta;
-  m->relaxThdt_H = 1.0 / dt / theta;
-  m->relaxThdt_UV = 1.0 / dt / theta;
+  cycle_start(m);
   for (int n = 0; n < m->np; n++) m->dUdt[n] = m->dVdt[n] = m->dSdt[n] = 0.0;
   for (int n = 0; n < m->np; n++) {
     if (m->So[n] < m->zor[n] + m->dryLimit) {
@@ -1788,9 +1822,42 @@ int orc_newton_step(orc_model* m, int prec, int solver, int maxIter, double eps,
     s.maxAbs[i] = mx; s.avAbs[i] = cnt ? av / cnt : 0.0;
     if (fabs(mx) > (i < 2 ? m->convUV : m->convS)) s.nrconv = 0;
   }
-  /* relaxation (EqsUVS2D.cpp:455-520) */
+  /* relaxation (EqsUVS2D.cpp:455-627) */
   double relaxUs = 1.0, relaxS = 1.0;
-  if (m->relaxMethod == 2) {
+  if (m->relaxMethod >= 3) {
+    /* the relaxed time steps: no convergence before they reach the time step; each grows
+     * by the ratio of the maximum change to the largest correction, within [relaxTimeFlow,
+     * dt], the two kept equal (:540-627) */
+    const double dt = m->unsteady ? m->dt : m->interval;
+    if (m->dt_H < dt || m->dt_UV < dt) s.nrconv = 0;
+    double maxUs = 0.0, maxS = 0.0;
+    for (int nd = 0; nd < m->np; nd++) {
+      double dU = 0.0, dV = 0.0, dS = 0.0;
+      int eU = m->eqno[3 * nd], eV = m->eqno[3 * nd + 1], eS = m->eqno[3 * nd + 2];
+      if (eU >= 0) dU = fabs(X[eU]);
+      if (eV >= 0) dV = fabs(X[eV]);
+      if (eS >= 0) dS = fabs(X[eS]);
+      double dUs = sqrt(dU * dU + dV * dV);
+      if (dUs > maxUs) maxUs = dUs;
+      if (dS > maxS) maxS = dS;
+    }
+    relaxUs = m->maxDeltaUV / maxUs;
+    relaxS = m->maxDeltaS / maxS;
+    const double rdt = m->relaxDt > 0.0 && m->relaxDt <= dt ? m->relaxDt : dt;
+    m->last_dt = m->dt_H;
+    m->dt_UV *= relaxUs;
+    m->dt_H *= relaxS;
+    if (m->dt_UV > dt) m->dt_UV = dt;
+    if (m->dt_UV < rdt) m->dt_UV = rdt;
+    if (m->dt_H > dt) m->dt_H = dt;
+    if (m->dt_H < rdt) m->dt_H = rdt;
+    if (m->dt_H < m->dt_UV) m->dt_UV = m->dt_H; else m->dt_H = m->dt_UV;
+    const double th = m->unsteady ? m->theta : 1.0;
+    m->relaxThdt_UV = 1.0 / m->dt_UV / th;
+    m->relaxThdt_H = 1.0 / m->dt_H / th;
+    if (relaxUs > 1.0) relaxUs = 1.0;
+    if (relaxS > 1.0) relaxS = 1.0;
+  } else if (m->relaxMethod == 2) {
     double maxUs = 0.0, maxS = 0.0;
     for (int nd = 0; nd < m->np; nd++) {
       double dU = 0.0, dV = 0.0, dS = 0.0;

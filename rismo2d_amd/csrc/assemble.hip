@@ -777,9 +777,10 @@ static AsmParams asm_params(Ctx& c)
 {
   AsmParams p;
   p.g = c.par.g; p.vk = c.par.vk; p.hmin = c.par.hmin();
-  // 1 / (dt theta) on the time terms' diagonal for unsteady flow (EqsUVS2D.cpp:130-147)
-  p.relaxUV = c.unsteady ? 1.0 / c.dt / c.theta : 0.0;
-  p.relaxH = c.unsteady ? 1.0 / c.dt / c.theta : 0.0;
+  // the time terms' diagonal: 1 / (dt theta) for unsteady flow, 1 / (dt_r theta) of the
+  // relaxed time steps with relaxation methods 3 and 4 (EqsUVS2D.cpp:97-147, 613-622)
+  p.relaxUV = c.thdt_uv;
+  p.relaxH = c.thdt_h;
   p.turbAlg = c.par.turb_alg() ? 1 : 0; p.turbMin = c.par.turb_min() ? 1 : 0;
   p.turbAniso = c.par.turb_aniso() ? 1 : 0;
   return p;

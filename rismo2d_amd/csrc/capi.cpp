@@ -149,6 +149,7 @@ rmx_status rmx_mesh_upload(rmx_ctx* p, const rmx_mesh_desc* d)
     b->alloc(np);
   dev_time_init(c);
   c.unsteady = false;
+  cycle_start(c);
   c.d_ste.upload(hm.ste, s); c.vtn.alloc(np); c.uun.alloc(np);
   c.d_vte.upload(hm.vt, s); c.d_exi.upload(hm.exi, s); c.d_eyi.upload(hm.eyi, s);
   c.exx.alloc(np); c.exy.alloc(np); c.eyy.alloc(np);
@@ -198,9 +199,13 @@ rmx_status rmx_set_params(rmx_ctx* p, const rmx_params* q)
   API_BEGIN
   Ctx& c = *need(p);
   if (!q) fail(RMX_ERR_ARG, "null params");
-  if (q->relax_method != 0 && q->relax_method != 2)
+  if (q->relax_method != 0 && (q->relax_method < 2 || q->relax_method > 4))
     fail(RMX_ERR_ARG, "params: relaxation method " + std::to_string(q->relax_method) +
-                      " not supported (0 none, 2 cap by max_delta_uv / max_delta_s)");
+                      " not supported (0 none, 2 cap by max_delta_uv / max_delta_s, 3 and 4 "
+                      "time-relaxed)");
+  if (q->relax_method >= 3 && !(q->interval > 0.0) && !c.unsteady)
+    fail(RMX_ERR_ARG, "params: relaxation methods 3 and 4 of stationary flow need the time "
+                      "interval (> 0)");
   if (!(q->dry_limit > 0.0))
     fail(RMX_ERR_ARG, "params: dry_limit must be positive (the minimum depth is dry_limit / 2)");
   c.par.relax_method = q->relax_method; c.par.dry_method = q->dry_method;
@@ -218,6 +223,9 @@ rmx_status rmx_set_params(rmx_ctx* p, const rmx_params* q)
   }
   c.par.cm = q->cm > 0.0 ? q->cm : 0.09;
   c.par.cd = q->cd > 0.0 ? q->cd : 1.00;
+  c.par.relax_time = q->relax_time;
+  c.par.interval = q->interval;
+  cycle_start(c);
   c.turb_stale = true;
   API_END
 }
@@ -335,11 +343,16 @@ rmx_status rmx_time_step(rmx_ctx* p, double dt, double theta, int first)
   if (!c.mesh_ready) fail(RMX_ERR_ARG, "time step: upload a mesh first");
   c.turb_stale = true;
   if (dt <= 0.0) {
+    if (c.par.relax_method >= 3 && !(c.par.interval > 0.0))
+      fail(RMX_ERR_ARG, "time step: relaxation methods 3 and 4 of stationary flow need the "
+                        "time interval (params.interval > 0)");
     c.unsteady = false;
+    cycle_start(c);
   } else {
     if (std::fabs(theta) < 1.0e-10 || std::fabs(dt) < 1.0e-10)
       fail(RMX_ERR_ARG, "time step: theta or time interval too small");
     c.unsteady = true; c.dt = dt; c.theta = theta;
+    cycle_start(c);
     dev_time_step(c, first ? 1 : 0);
     HIPCHECK(hipStreamSynchronize(c.stream));
   }

@@ -181,11 +181,15 @@ __global__ void k_cr_out(int nb, int bs, const int* __restrict__ blk_start,
   if (r < m) zc[a + r] = w[(long long)i * bs + r];
 }
 
-// one wave per kept block k (i = 2 s k) of level s: w_i -= P_i w_{i-s} + Q_i w_{i+s}
-__global__ void __launch_bounds__(64) k_cr_fwd(int nb, int bs, int s, const double* __restrict__ M,
-                                               double* __restrict__ w)
+constexpr int CR_MAX = 128;       // rows of a coarse block (a strip's aggregates)
+
+// one workgroup (a thread per block row) per kept block k (i = 2 s k) of level s:
+// w_i -= P_i w_{i-s} + Q_i w_{i+s}
+__global__ void __launch_bounds__(CR_MAX) k_cr_fwd(int nb, int bs, int s,
+                                                   const double* __restrict__ M,
+                                                   double* __restrict__ w)
 {
-  __shared__ double lo[64], hi[64];
+  __shared__ double lo[CR_MAX], hi[CR_MAX];
   const int k = blockIdx.x, r = threadIdx.x;
   const int i = 2 * s * k;
   if (i >= nb) return;
@@ -204,12 +208,13 @@ __global__ void __launch_bounds__(64) k_cr_fwd(int nb, int bs, int s, const doub
   w[(long long)i * bs + r] -= acc;
 }
 
-// one wave per eliminated block k (j = s + 2 s k):
+// one workgroup per eliminated block k (j = s + 2 s k):
 // w_j <- A_j^-1 (w_j - B_j w_{j-s} - C_j w_{j+s}); with s = 0 the top block, w_0 <- A^-1 w_0
-__global__ void __launch_bounds__(64) k_cr_bwd(int nb, int bs, int s, const double* __restrict__ M,
-                                               double* __restrict__ w)
+__global__ void __launch_bounds__(CR_MAX) k_cr_bwd(int nb, int bs, int s,
+                                                   const double* __restrict__ M,
+                                                   double* __restrict__ w)
 {
-  __shared__ double lo[64], hi[64], t[64];
+  __shared__ double lo[CR_MAX], hi[CR_MAX], t[CR_MAX];
   const int k = blockIdx.x, r = threadIdx.x;
   const int j = s + 2 * s * k;
   if (j >= nb) return;
@@ -294,7 +299,7 @@ static void dense_mm(int n, const double* a, const double* b, double alpha, bool
 }
 
 // Block structure of the coarse matrix: usable when every slot couples aggregates of one
-// strip or of neighbouring strips and no strip holds more than 64 aggregates.
+// strip or of neighbouring strips and no strip holds more than CR_MAX aggregates.
 static void cr_setup(Coarse& C, const std::vector<int>& agg_strip, int nstrips)
 {
   C.cr_ok = C.cr_fac = false;
@@ -303,7 +308,7 @@ static void cr_setup(Coarse& C, const std::vector<int>& agg_strip, int nstrips)
   for (int v : agg_strip) cnt[v]++;
   int bs = 0;
   for (int v : cnt) bs = std::max(bs, v);
-  if (bs < 1 || bs > 64) return;
+  if (bs < 1 || bs > CR_MAX) return;
   for (long long k = 0; k < C.nslot; k++)
     if (std::abs(agg_strip[C.slot_row[k]] - agg_strip[C.slot_col[k]]) > 1) return;
   const int nb = nstrips;
@@ -408,20 +413,21 @@ static bool cr_factor(Coarse& C, const double* cval)
 static void cr_apply(Ctx& c, Coarse& C)
 {
   const int nb = C.nb, bs = C.bs;
+  const int nt = (bs + 63) / 64 * 64;           // a thread per block row, whole waves
   hipStream_t st = c.stream;
-  k_cr_in<<<nb, 64, 0, st>>>(nb, bs, C.d_blk_start.p, C.rc.p, C.w.p);
+  k_cr_in<<<nb, nt, 0, st>>>(nb, bs, C.d_blk_start.p, C.rc.p, C.w.p);
   int lev = 0;
   for (int s = 1; s < nb; s *= 2, lev++) {
     const int nk = (nb + 2 * s - 1) / (2 * s);
-    k_cr_fwd<<<nk, 64, 0, st>>>(nb, bs, s, C.mats.p + C.lev_keep[lev], C.w.p);
+    k_cr_fwd<<<nk, nt, 0, st>>>(nb, bs, s, C.mats.p + C.lev_keep[lev], C.w.p);
   }
-  k_cr_bwd<<<1, 64, 0, st>>>(nb, bs, 0, C.mats.p + C.top_off, C.w.p);
+  k_cr_bwd<<<1, nt, 0, st>>>(nb, bs, 0, C.mats.p + C.top_off, C.w.p);
   for (int l = C.levels - 1; l >= 0; l--) {
     const int s = 1 << l;
     const int ne = (nb - s + 2 * s - 1) / (2 * s);
-    k_cr_bwd<<<ne, 64, 0, st>>>(nb, bs, s, C.mats.p + C.lev_elim[l], C.w.p);
+    k_cr_bwd<<<ne, nt, 0, st>>>(nb, bs, s, C.mats.p + C.lev_elim[l], C.w.p);
   }
-  k_cr_out<<<nb, 64, 0, st>>>(nb, bs, C.d_blk_start.p, C.w.p, C.zc.p);
+  k_cr_out<<<nb, nt, 0, st>>>(nb, bs, C.d_blk_start.p, C.w.p, C.zc.p);
   HIPCHECK(hipGetLastError());
 }
 

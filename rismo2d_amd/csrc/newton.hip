@@ -193,6 +193,24 @@ void dev_time_commit(Ctx& c)
   HIPCHECK(hipMemcpyAsync(c.dSdto.p, c.dSdt.p, b, hipMemcpyDeviceToDevice, c.stream));
 }
 
+// Start of a flow cycle (EQS_UVS2D::Execute, EqsUVS2D.cpp:97-147): relaxation methods 3 and
+// 4 start the relaxed time steps at relaxTimeFlow (at most the time step); the Jacobian's
+// time terms take 1 / dt_r (stationary) or 1 / dt_r / theta, else 1 / dt / theta (unsteady)
+// or none.
+void cycle_start(Ctx& c)
+{
+  const double dt = c.unsteady ? c.dt : c.par.interval;
+  const double rt = c.par.relax_time;
+  c.dt_uv = c.dt_h = c.last_dt = rt > 0.0 && rt <= dt ? rt : dt;
+  if (c.par.relax_method >= 3) {
+    const double th = c.unsteady ? c.theta : 1.0;
+    c.thdt_uv = 1.0 / c.dt_uv / th;
+    c.thdt_h = 1.0 / c.dt_h / th;
+  } else {
+    c.thdt_uv = c.thdt_h = c.unsteady ? 1.0 / c.dt / c.theta : 0.0;
+  }
+}
+
 void dev_newton_update(Ctx& c, rmx_step_stats& st)
 {
   const int np = c.hm.np;
@@ -218,13 +236,37 @@ void dev_newton_update(Ctx& c, rmx_step_stats& st)
     if (std::fabs(v[q]) > (q < 2 ? c.par.conv_uv : c.par.conv_s)) st.nrconv = 0;
   }
   double relaxUs = 1.0, relaxS = 1.0;
-  if (c.par.relax_method == 2) {
+  st.relax_time_last = c.dt_h;
+  if (c.par.relax_method >= 3) {
+    // the relaxed time steps (EqsUVS2D.cpp:540-627): no convergence before they reach the
+    // time step; each grows by the ratio of the maximum change to the largest correction,
+    // within [relaxTimeFlow, dt], the two kept equal
+    const double dt = c.unsteady ? c.dt : c.par.interval;
+    const double rt = c.par.relax_time > 0.0 && c.par.relax_time <= dt ? c.par.relax_time : dt;
+    if (c.dt_h < dt || c.dt_uv < dt) st.nrconv = 0;
+    relaxUs = c.par.max_delta_uv / v[9];
+    relaxS = c.par.max_delta_s / v[10];
+    c.last_dt = c.dt_h;
+    c.dt_uv *= relaxUs;
+    c.dt_h *= relaxS;
+    if (c.dt_uv > dt) c.dt_uv = dt;
+    if (c.dt_uv < rt) c.dt_uv = rt;
+    if (c.dt_h > dt) c.dt_h = dt;
+    if (c.dt_h < rt) c.dt_h = rt;
+    if (c.dt_h < c.dt_uv) c.dt_uv = c.dt_h; else c.dt_h = c.dt_uv;
+    const double th = c.unsteady ? c.theta : 1.0;
+    c.thdt_uv = 1.0 / c.dt_uv / th;
+    c.thdt_h = 1.0 / c.dt_h / th;
+    if (relaxUs > 1.0) relaxUs = 1.0;
+    if (relaxS > 1.0) relaxS = 1.0;
+  } else if (c.par.relax_method == 2) {
     relaxUs = c.par.max_delta_uv / v[9];
     relaxS = c.par.max_delta_s / v[10];
     if (relaxUs > 1.0) relaxUs = 1.0;
     if (relaxS > 1.0) relaxS = 1.0;
   }
   st.relaxUs = relaxUs; st.relaxS = relaxS;
+  st.relax_time = c.dt_h;
   UpdParams P;
   P.relaxUs = relaxUs; P.relaxS = relaxS; P.maxUs = c.par.max_us; P.dryLimit = c.par.dry_limit;
   P.hmin = c.par.hmin(); P.dryMethod = c.par.dry_method;

@@ -106,6 +106,7 @@ struct Params {
   double vk = 1.78e-6 / (1.0 + 0.0337 * 10.0 + 0.00022 * 10.0 * 10.0);
   int turb = 1;                               // $TM_TURBULENCE code
   double cm = 0.09, cd = 1.00;                // KDCONST
+  double relax_time = 0.0, interval = 0.0;    // methods 3, 4: relaxTimeFlow, stationary dt
   double hmin() const { return dry_limit / 2.0; }
   bool turb_alg() const { return turb % 10 == 2; }
   bool turb_aniso() const { return (turb / 10) % 10 == 1; }
@@ -273,6 +274,9 @@ struct Ctx {
   DBuf<double> dUdt, dVdt, dSdt, Uo, Vo, So, dUdto, dVdto, dSdto;
   bool unsteady = false;
   double dt = 0.0, theta = 1.0;
+  // time terms' diagonal of the Jacobian (1 / dt / theta; methods 3, 4: of the relaxed time
+  // steps dt_uv, dt_h, restarted by cycle_start; last_dt: the one before the latest update)
+  double thdt_uv = 0.0, thdt_h = 0.0, dt_uv = 0.0, dt_h = 0.0, last_dt = 0.0;
   DBuf<uint32_t> dflag;
   DBuf<int> eqno;                             // [3*np]
   // friction gather: node -> incident region elements (ascending), areaFact*cf summed
@@ -312,7 +316,7 @@ struct Ctx {
   int segments = 0;                           // cross-strip cuts of the strip sweeps (0: auto)
   Coarse coarse;
   Mpnn mpnn;
-  int coarse_segments = 8;                    // cross-strip cuts of the coarse aggregates
+  int coarse_segments = 24;                   // cross-strip cuts of the coarse aggregates
   // Krylov work vectors
   DBuf<double> w[10];
   DBuf<double> red;                           // reduction scratch
@@ -357,6 +361,7 @@ void dev_ilu0_factor(Ctx& c, int prec);
 void dev_ilu0_apply(Ctx& c, int prec, const double* b, double* x);   // device pointers
 void dev_spmv(Ctx& c, int prec, const double* x, double* r);        // device pointers
 void dev_solve(Ctx& c, const rmx_krylov_opts& o, rmx_krylov_out& out);
+void cycle_start(Ctx& c);
 void dev_newton_update(Ctx& c, rmx_step_stats& st);
 void dev_time_init(Ctx& c);                    // previous state := state, derivatives 0
 void dev_slip_flow(Ctx& c);                    // GRID::SetSlipFlow

@@ -336,14 +336,15 @@ def write_reference_deck(deck: Deck, outdir: str, solver: str = "1 6 1 0 60 1000
                          relax: str = "2 0.001 0.1 10.0 10.0 0.1",
                          dryrew: str = "2 1000 0.001 0.005 100 5", tm_steps: int = 1,
                          dt: float = 1000000.0, theta: float = 1.0,
-                         stationary: int = 1) -> None:
+                         stationary: int = 1, relax_time: float = 0.0) -> None:
     """Write the reference's keyword input files for ``deck`` into ``outdir``.
 
     One material per distinct (Manning n, vt, stem diameter, stem spacing, st, KDest, estx,
     esty).
     Values are written with %.17g so that the reference reads back the exact doubles used
     by the oracle.  ``tm_steps`` time steps of ``dt`` seconds with weight ``theta``;
-    ``stationary`` 0 selects the unsteady flow cycle.
+    ``stationary`` 0 selects the unsteady flow cycle; ``relax_time`` the relaxed time step
+    of relaxation methods 3 and 4 (relaxTimeFlow; 0: the time step).
     """
     os.makedirs(outdir, exist_ok=True)
     ne = deck.n_elems
@@ -389,7 +390,8 @@ def write_reference_deck(deck: Deck, outdir: str, solver: str = "1 6 1 0 60 1000
     else:
         inlet_lines = f"$TM_LINE QINLET {INLET_LINE} {deck.Q_inlet:.17g} {deck.S_inlet:.17g}\n"
     with open(os.path.join(outdir, "tm.dat"), "w") as f:
-        f.write(f"$RISMO2D 40100\n$TM_STEPS 1 {tm_steps} 0\n$TM_INTERVAL {dt:.17g} 0.0 0.0\n"
+        f.write(f"$RISMO2D 40100\n$TM_STEPS 1 {tm_steps} 0\n"
+                f"$TM_INTERVAL {dt:.17g} {relax_time:.17g} 0.0\n"
                 f"$TM_WEIGHT {theta:.17g} 1.0 1.0\n$TM_STEP_NO 1\n$TM_CYCLE 1\n"
                 f"$TM_STATIONARY {stationary}\n"
                 f"$TM_TURBULENCE {deck.turb}\n$TM_MAXITER {maxiter}\n$TM_SOLVER {tm_solver}\n"
@@ -626,7 +628,11 @@ def _read_timesteps(path: str):
         if key == "TM_STEPS":
             run["steps"] = int(t[1]) - int(t[0]) + 1
         elif key == "TM_INTERVAL":
+            # deltaTm relaxTmFlow ...; relaxTmFlow <= 0 or above deltaTm: deltaTm
+            # (Timeint.cpp:697-719)
             run["dt"] = float(t[0])
+            rt = float(t[1]) if len(t) > 1 else 0.0
+            run["relax_time"] = rt if 0.0 < rt <= run["dt"] else run["dt"]
         elif key == "TM_WEIGHT":
             run["theta"] = float(t[0])
         elif key in ("TM_OUTPUT", "TM_RESET_STATIST"):
@@ -739,8 +745,9 @@ def read_reference_deck(input_file: str) -> tuple[Deck, dict]:
 
     Supported: Manning roughness, emergent vegetation, constant / algebraic / anisotropic
     eddy viscosity, one QINLET or QTINLET line and one OUTLET line with a single water
-    level, relaxation method 2, drying and rewetting not called inside the flow cycle
-    (``$DRYREW`` frequency not below ``$TM_MAXITER``).  Anything else raises
+    level, relaxation methods 0, 2, 3 and 4 (3 and 4: time-relaxed Newton steps, with
+    relaxTimeFlow from ``$TM_INTERVAL``), drying and rewetting not called inside the flow
+    cycle (``$DRYREW`` frequency not below ``$TM_MAXITER``).  Anything else raises
     DeckFormatError."""
     base = os.path.dirname(os.path.abspath(input_file))
     files, other = {}, []
@@ -816,8 +823,11 @@ def read_reference_deck(input_file: str) -> tuple[Deck, dict]:
     run["project"] = other
     conv, relax, dry, minmax = (proj[k] for k in ("CONVERGENCE", "RELAX", "DRYREW", "MINMAX"))
     dry[3] = max(dry[3], dry[2])                 # rewetLimit not below dryLimit
-    if relax[0] != 2:
-        raise DeckFormatError(f"$RELAX: relaxation method {relax[0]} is not supported (2)")
+    if relax[0] not in (0, 2, 3, 4):
+        raise DeckFormatError(f"$RELAX: relaxation method {relax[0]} is not supported "
+                              "(0, 2, 3, 4)")
+    if relax[0] >= 3 and not run.get("dt"):
+        raise DeckFormatError("$RELAX: relaxation methods 3 and 4 need $TM_INTERVAL")
     maxiter = run["maxiter"]                     # the flow cycle's $TM_MAXITER entry
     if dry[0] and dry[1] < maxiter:
         # EqsUVS2D.cpp:201: the flow cycle calls the drying and rewetting
@@ -826,6 +836,8 @@ def read_reference_deck(input_file: str) -> tuple[Deck, dict]:
     run["params"] = dict(relax_method=relax[0], max_delta_uv=relax[3], max_delta_s=relax[4],
                          conv_uv=conv[0], conv_s=conv[1], dry_method=dry[0], dry_limit=dry[2],
                          max_us=minmax[3])
+    if relax[0] >= 3:                            # time-relaxed Newton steps
+        run["params"].update(relax_time=run["relax_time"], interval=run["dt"])
     run["dryrew"] = dict(method=dry[0], freq=dry[1], dry_limit=dry[2], rewet_limit=dry[3],
                          rewet_passes=dry[4], count_down=dry[5])
     return deck, run

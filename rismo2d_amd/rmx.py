@@ -46,7 +46,8 @@ class Params(C.Structure):
     _fields_ = [("relax_method", C.c_int), ("dry_method", C.c_int), ("dry_limit", C.c_double),
                 ("max_delta_uv", C.c_double), ("max_delta_s", C.c_double),
                 ("conv_uv", C.c_double), ("conv_s", C.c_double), ("max_us", C.c_double),
-                ("turb", C.c_int), ("cm", C.c_double), ("cd", C.c_double)]
+                ("turb", C.c_int), ("cm", C.c_double), ("cd", C.c_double),
+                ("relax_time", C.c_double), ("interval", C.c_double)]
 
 
 class KrylovOpts(C.Structure):
@@ -82,7 +83,8 @@ class StepStats(C.Structure):
     _fields_ = [("maxAbs", C.c_double * 3), ("avAbs", C.c_double * 3), ("relaxUs", C.c_double),
                 ("relaxS", C.c_double), ("nrconv", C.c_int), ("scale", C.c_double),
                 ("solve", KrylovOut), ("t_prepare", C.c_double), ("t_assemble", C.c_double),
-                ("t_factor", C.c_double), ("t_solve", C.c_double), ("t_update", C.c_double)]
+                ("t_factor", C.c_double), ("t_solve", C.c_double), ("t_update", C.c_double),
+                ("relax_time", C.c_double), ("relax_time_last", C.c_double)]
 
 
 class SweepInfo(C.Structure):
@@ -238,18 +240,21 @@ class Context:
 
     def set_params(self, relax_method=2, dry_method=2, dry_limit=1e-3, max_delta_uv=10.0,
                    max_delta_s=10.0, conv_uv=1e-7, conv_s=1e-7, max_us=100.0, turb=None,
-                   cm=0.09, cd=1.0):
-        """turb: $TM_TURBULENCE code; None takes the uploaded deck's."""
+                   cm=0.09, cd=1.0, relax_time=0.0, interval=0.0):
+        """turb: $TM_TURBULENCE code; None takes the uploaded deck's.  relax_time
+        (relaxTimeFlow) and interval (the stationary cycle's time step): relaxation methods
+        3 and 4."""
         if turb is None:
             turb = getattr(self, "deck_turb", 1)
         p = Params(relax_method, dry_method, dry_limit, max_delta_uv, max_delta_s, conv_uv,
-                   conv_s, max_us, turb, cm, cd)
+                   conv_s, max_us, turb, cm, cd, relax_time, interval)
         _check(lib().rmx_set_params(self.h, C.byref(p)))
         # kept (and re-applied by upload_mesh) only once the library has accepted them
         self._params = dict(relax_method=relax_method, dry_method=dry_method,
                             dry_limit=dry_limit, max_delta_uv=max_delta_uv,
                             max_delta_s=max_delta_s, conv_uv=conv_uv, conv_s=conv_s,
-                            max_us=max_us, cm=cm, cd=cd)
+                            max_us=max_us, cm=cm, cd=cd, relax_time=relax_time,
+                            interval=interval)
 
     TIME_KEYS = ("Uo", "Vo", "So", "dUdto", "dVdto", "dSdto", "dUdt", "dVdt", "dSdt")
 

@@ -41,7 +41,7 @@ def run_reference_deck(input_file: str, ctx: rmx.Context | None = None, solve: d
             solve = FAST
         if solve.get("precond") == rmx.PREC_SILU0C:
             c.prepare()
-            c.set_coarse(8)
+            c.set_coarse(24)
         maxiter = run["maxiter"]
         unsteady = run["stationary"] == 0
         steps = run["steps"] if unsteady else 1

@@ -50,6 +50,9 @@ the reference's input format, runs the reference and stores what the harness rec
                     1500 m3/s over 6,000 s, its first 10 implicit time steps of 30 s (theta
                     0.7), anisotropic (Elder) eddy viscosity recomputed every Newton step: the
                     Newton steps per time step and the state at the end of every time step.
+* ``relax3_20x4.npz`` / ``relax3u20x4.npz``  the 20 x 4 triangle channel with relaxation
+  method 3 (time-relaxed Newton steps), stationary (time step 100 s, relaxed time step from
+  5 s) and unsteady (3 x 30 s, theta 0.7, relaxed time step from 2 s)
 * ``cap20x4.npz``   the 20 x 4 triangle channel with a binding relaxation cap ($RELAX 2
                     with maxDeltaUV 0.02, maxDeltaS 0.005): every Newton step's records.
 
@@ -111,6 +114,10 @@ DECKS = {
                     Q=10.0),
     "rewet20x6": dict(nx=20, ny=6, length=100.0, width=30.0, noise_sigma=0.05, noise_corr=5.0,
                       Q=10.0),
+    "relax3_20x4": dict(nx=20, ny=4, length=100.0, width=20.0, noise_sigma=0.05, noise_corr=5.0,
+                        Q=10.0),
+    "relax3u20x4": dict(nx=20, ny=4, length=100.0, width=20.0, noise_sigma=0.05, noise_corr=5.0,
+                        Q=10.0),
 }
 
 # a bank that rises out of the water: the bed goes up linearly across the width from y0 to
@@ -137,7 +144,7 @@ HYDRO = {
 # decks kept to compact records (no per-step matrices or element records); "steps": the
 # per-step corrections, "timesteps": the committed state of every time step instead
 COMPACT = {"channel200x20": "steps", "vegchannel200x20": "steps", "meander40x64": "steps",
-           "flood200x6": "timesteps"}
+           "flood200x6": "timesteps", "relax3_20x4": "steps", "relax3u20x4": "steps"}
 
 # time integration of the decks that are not one stationary flow step, and the Newton step
 # whose element matrices are recorded (one with non-zero time derivatives)
@@ -150,6 +157,11 @@ RUNS = {
     "meander40x64": dict(maxiter=2, solver="1 6 1 0 60 2000 1.0e-10"),
     "flood200x6": dict(tm_steps=10, dt=30.0, theta=0.7, stationary=0),
     "cap20x4": dict(relax="2 0.001 0.1 0.02 0.005 0.1", maxiter=40),
+    # relaxation method 3 (the reference's default): time-relaxed Newton steps whose relaxed
+    # time step grows from relaxTimeFlow to the time step; stationary, and unsteady
+    "relax3_20x4": dict(relax="3 0.001 0.1 0.05 0.02 0.1", maxiter=40, dt=100.0, relax_time=5.0),
+    "relax3u20x4": dict(relax="3 0.001 0.1 0.02 0.01 0.1", maxiter=60, tm_steps=3, dt=30.0,
+                        theta=0.7, stationary=0, relax_time=2.0, elem_step=1),
 }
 
 
@@ -221,7 +233,7 @@ def main():
             (steps, final), iters = run_reference(d, tmp, run)
         out = deck_arrays(d)
         out["nsteps"] = len(steps)
-        for key in ("tm_steps", "dt", "theta", "stationary", "elem_step", "maxiter"):
+        for key in ("tm_steps", "dt", "theta", "stationary", "elem_step", "maxiter", "relax_time"):
             if key in run:
                 out[key] = run[key]
         if "relax" in run:

@@ -22,6 +22,10 @@ program's golden records.
   (the device path does not run them).
 * A deck whose relaxation cap binds (maxDeltaUV 0.02, maxDeltaS 0.005), read from the
   reference's files: the reference's Newton step count and final state within 1e-8.
+* Relaxation method 3, the reference's default (time-relaxed Newton steps), stationary and
+  unsteady, read from the reference's files, on both paths: the reference's Newton step
+  count per time step and its final state within 1e-8, the relaxed time step growing from
+  relaxTimeFlow to the time step.
 """
 import hashlib
 import os
@@ -191,3 +195,36 @@ def test_binding_relaxation_cap_follows_reference(ctx, tmp_path):
     assert _relerr(U, g["final_U"], us) <= 1e-8
     assert _relerr(V, g["final_V"], us) <= 1e-8
     assert _relerr(S - z, Hr, np.abs(Hr).max()) <= 1e-8
+
+
+@pytest.mark.parametrize("path", ["plain", "fast"])
+@pytest.mark.parametrize("name", ["relax3_20x4", "relax3u20x4"])
+def test_relaxation_method_3_follows_reference(ctx, tmp_path, name, path):
+    print(f"[gpu] relaxation method 3, {name}, {path} path, from the reference's files",
+          flush=True)
+    g = load_golden(name)
+    relax = " ".join(f"{v:g}" for v in g["relax"])
+    unsteady = int(g.get("stationary", 1)) == 0
+    dk.write_reference_deck(golden_deck(g), str(tmp_path), relax=relax, maxiter=int(g["maxiter"]),
+                            dt=float(g["dt"]), relax_time=float(g["relax_time"]),
+                            tm_steps=int(g.get("tm_steps", 1)), theta=float(g.get("theta", 1.0)),
+                            stationary=int(g.get("stationary", 1)))
+    rts = []
+    d, run, counts, (U, V, S, z) = rr.run_reference_deck(
+        os.path.join(str(tmp_path), "input.dat"), ctx=ctx,
+        solve=PLAIN if path == "plain" else FAST,
+        on_step=lambda t, k, st: rts.append(st.relax_time))
+    assert run["params"]["relax_method"] == 3
+    assert run["params"]["relax_time"] == float(g["relax_time"])
+    tsteps = [int(g[f"s{k}_tstep"]) for k in range(int(g["nsteps"]))]
+    want = [tsteps.count(t) for t in range(1, max(tsteps) + 1)]
+    print(f"    Newton steps per time step {counts} (reference {want}), relaxed time step "
+          f"{min(rts):.3g} .. {max(rts):.3g} s")
+    assert counts == want
+    assert min(rts) >= float(g["relax_time"]) and max(rts) == float(g["dt"])
+    us = np.abs(g["final_U"]).max()
+    Hr = g["final_S"] - g["final_z"]
+    eu, ev = _relerr(U, g["final_U"], us), _relerr(V, g["final_V"], us)
+    eh = _relerr(S - z, Hr, np.abs(Hr).max())
+    print(f"    final state vs reference: u {eu:.1e}, v {ev:.1e}, h {eh:.1e}")
+    assert eu <= 1e-8 and ev <= 1e-8 and eh <= 1e-8

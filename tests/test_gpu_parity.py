@@ -324,7 +324,7 @@ def test_block_sweeps_split_levels(ctx, rows, pairs, monkeypatch):
 
 
 @pytest.mark.parametrize("coarse_solve", ["device", "host"])
-@pytest.mark.parametrize("strips,segments,cseg", [(4, 8, 2), (17, 4, 4), (1, 1, 1)])
+@pytest.mark.parametrize("strips,segments,cseg", [(4, 8, 2), (17, 4, 4), (1, 1, 1), (4, 8, 24)])
 def test_two_level_apply_matches_dense_reference(ctx, strips, segments, cseg, coarse_solve,
                                                  monkeypatch):
     """Two-level apply z = z1 + M^-1 (r - A z1), z1 = P (R A P)^-1 R r, against the same

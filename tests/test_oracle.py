@@ -236,6 +236,44 @@ def test_oracle_matches_reference_flood_time_steps():
         np.testing.assert_array_equal(S, g[pre + "S"], err_msg=f"time step {t}")
 
 
+@pytest.mark.parametrize("name", ["relax3_20x4", "relax3u20x4"])
+def test_oracle_matches_reference_relaxation_method_3(name):
+    """Relaxation method 3, the reference's default (time-relaxed Newton steps, EqsUVS2D.cpp:
+    97-147, 540-627): a relaxed time step in the Jacobian that starts from relaxTimeFlow and
+    grows by the ratio of the maximum change to the largest correction, capped by the time
+    step; no convergence before it reaches the time step.  Stationary (time step 100 s, from
+    5 s) and unsteady (3 x 30 s, theta 0.7, from 2 s): every Newton correction, its scale and
+    BiCGStab iteration count and the final state, bit for bit."""
+    g = load_golden(name)
+    om = po.OracleModel(golden_deck(g))
+    r = g["relax"]
+    dt, theta = float(g["dt"]), float(g.get("theta", 1.0))
+    unsteady = int(g.get("stationary", 1)) == 0
+    om.set_params(relax_method=int(r[0]), max_delta_uv=float(r[3]), max_delta_s=float(r[4]),
+                  relax_time=float(g["relax_time"]), interval=dt)
+    tsteps = [int(g[f"s{k}_tstep"]) for k in range(int(g["nsteps"]))]
+    k = 0
+    grew = []
+    for t in range(1, max(tsteps) + 1):
+        if unsteady:
+            if t > 1:
+                om.time_commit()
+            om.time_step(dt, theta, first=(t == 1))
+        assert om.relax_time()[0] == float(g["relax_time"])       # each cycle starts over
+        for _ in range(tsteps.count(t)):
+            om.prepare()
+            X, st = om.newton_step(prec=32, solver=1, max_iter=1000, eps=1e-10)
+            assert st.scale == g[f"s{k}_scale"], k
+            assert st.iters == g["bicgstab_iters"][k], k
+            np.testing.assert_array_equal(X, g[f"s{k}_xsol"], err_msg=f"step {k}")
+            grew.append(om.relax_time()[0])
+            k += 1
+        om.slip_flow()
+    _check_state(om, g, "final_")
+    # the relaxed time step grew from relaxTimeFlow to the time step
+    assert min(grew) >= float(g["relax_time"]) and max(grew) == dt
+
+
 # ---------------------------------------------------------------------------------------
 # closed forms and invariants
 

@@ -12,6 +12,7 @@
 * What the solve path does not support is rejected, not misread.
 """
 import os
+import re
 
 import numpy as np
 import pytest
@@ -223,8 +224,8 @@ def test_initial_file_fixed_columns_and_defaults(tmp_path):
     (("tm.dat", "$TM_LINE OUTLET 9", "$TM_LINE SLIP 9"), "not supported"),
     (("init.dat", "9 0\n", "8 0\n"), "wrong number of nodes"),
     (("input.dat", "$ASC_INITFILE", "$BIN_INITFILE"), "only ASCII"),
-    (("input.dat", "$RELAX 2\n", ""), "relaxation method 3"),
     (("input.dat", "$RELAX 2", "$RELAX 1"), "relaxation method 1"),
+    (("input.dat", "$RELAX 2", "$RELAX 5"), "relaxation method 5"),
     (("input.dat", "$RELAX 2\n", "$RELAX 2\n$DRYREW 2 5\n"), "drying and rewetting every 5"),
     # the time-step file: one boundary set with the flow cycle alone, and its Newton limit
     (("tm.dat", "$TM_CYCLE 1\n", "$TM_CYCLE 90 1\n"), "only the flow cycle"),
@@ -273,6 +274,24 @@ def test_project_keywords_as_the_reference_scans_them(tmp_path):
         files["input.dat"] = files["input.dat"].replace("$RELAX 2\n", "$RELAX 2\n" + extra)
         _, run = dk.read_reference_deck(_write(tmp_path, files))
         assert run["params"]["max_us"] == want, extra
+
+
+def test_relaxation_method_3_takes_relax_time_from_the_time_interval(tmp_path):
+    """Without $RELAX the reference's default method 3 (Project.cpp:86) runs time-relaxed
+    Newton steps; relaxTimeFlow is $TM_INTERVAL's second value, and the time interval when
+    that is not positive or exceeds it (Timeint.cpp:697-719)."""
+    init = "# 0.0 x 40000 U V S\n9 0\n" + "".join(f"{i} 0 0 1.0\n" for i in range(1, 10))
+    files = _files(init)
+    assert "$TM_INTERVAL" in files["tm.dat"]
+    for tmi, want in (("100.0 5.0 0.0", 5.0), ("100.0 0.0 0.0", 100.0), ("100.0 500 0", 100.0),
+                      ("100.0", 100.0)):
+        f = dict(files)
+        f["input.dat"] = f["input.dat"].replace("$RELAX 2\n", "")
+        f["tm.dat"] = re.sub(r"\$TM_INTERVAL[^\n]*", "$TM_INTERVAL " + tmi, f["tm.dat"])
+        _, run = dk.read_reference_deck(_write(tmp_path, f))
+        p = run["params"]
+        assert (p["relax_method"], p["relax_time"], p["interval"]) == (3, want, 100.0), tmi
+        assert (p["max_delta_uv"], p["max_delta_s"]) == (0.1, 0.1)     # the defaults
 
 
 def test_time_step_lists_are_read_per_cycle(tmp_path):
