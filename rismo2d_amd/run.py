@@ -6,6 +6,11 @@ EQS_UVS2D::Execute, EqsUVS2D.cpp:71-1096). For each time step of ``$TM_STEPS``:
   step's end (EqsUVS2D.cpp:318-320);
 - Newton steps until the reference's convergence test holds (``nrconv``) or
   ``$TM_MAXITER`` steps, then the slip condition at rotated nodes;
+- the cycle ends at ``$TM_MAXITER`` Newton steps whether or not they converged: the test
+  that ends it (EqsUVS2D.cpp:893) holds at it == actualCycit - 1 before the restart with a
+  shorter time step of relaxation methods 3 and 4 (:931-962) is examined, so that restart
+  never runs (a golden deck whose relaxed time step stays far below the time step shows the
+  reference stopping there);
 - the step is committed as the old state of the next one.
 
 A stationary deck solves its flow once.
@@ -25,6 +30,34 @@ PLAIN = dict(method=rmx.BICGSTAB, prec=rmx.F64, precond=rmx.PREC_ILU0, eps=1e-12
              max_iter=1000)
 
 
+def time_loop(m, steps: int, dt: float, theta: float, maxiter: int, newton, hydrograph=None,
+              stationary: bool = False, on_step=None):
+    """The flow cycle's time loop on a model ``m`` with the methods ``time_step(dt, theta,
+    first)``, ``time_commit()``, ``set_inflow(Q, S)`` and ``slip_flow()``; ``newton()`` runs
+    one Newton step and returns its statistics (``nrconv``).  Returns the Newton steps of
+    every time step."""
+    counts = []
+    for t in range(1, steps + 1):
+        if not stationary:
+            if t > 1:
+                m.time_commit()
+            m.time_step(dt, theta, first=(t == 1))
+        if hydrograph is not None and not stationary:
+            m.set_inflow(*rmx.hydrograph(hydrograph, t * dt))
+        k = 0
+        while True:
+            st = newton()
+            k += 1
+            if on_step:
+                on_step(t, k, st)
+            if st.nrconv or k >= maxiter:
+                break
+        # EQS_UVS2D::Execute ends with the slip condition (EqsUVS2D.cpp:1078)
+        m.slip_flow()
+        counts.append(k)
+    return counts
+
+
 def run_reference_deck(input_file: str, ctx: rmx.Context | None = None, solve: dict | None = None,
                        on_step=None):
     """Read ``input_file`` (read_reference_deck) and run it. ``solve``: the options of each
@@ -42,29 +75,10 @@ def run_reference_deck(input_file: str, ctx: rmx.Context | None = None, solve: d
         if solve.get("precond") == rmx.PREC_SILU0C:
             c.prepare()
             c.set_coarse(24)
-        maxiter = run["maxiter"]
         unsteady = run["stationary"] == 0
-        steps = run["steps"] if unsteady else 1
-        dt = run["dt"]
-        counts = []
-        for t in range(1, steps + 1):
-            if unsteady:
-                if t > 1:
-                    c.time_commit()
-                c.time_step(dt, run["theta"], first=(t == 1))
-                if d.hydrograph is not None:
-                    c.set_inflow(*rmx.hydrograph(d.hydrograph, t * dt))
-            k = 0
-            while True:
-                st = c.newton_step(**solve)
-                k += 1
-                if on_step:
-                    on_step(t, k, st)
-                if st.nrconv or k >= maxiter:
-                    break
-            # EQS_UVS2D::Execute ends with the slip condition (EqsUVS2D.cpp:1078)
-            c.slip_flow()
-            counts.append(k)
+        counts = time_loop(c, run["steps"] if unsteady else 1, run["dt"], run["theta"],
+                           run["maxiter"], lambda: c.newton_step(**solve),
+                           hydrograph=d.hydrograph, stationary=not unsteady, on_step=on_step)
         U, V, S, z = c.state()
         return d, run, counts, (np.asarray(U), np.asarray(V), np.asarray(S), np.asarray(z))
     finally:

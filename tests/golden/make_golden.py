@@ -118,6 +118,8 @@ DECKS = {
                         Q=10.0),
     "relax3u20x4": dict(nx=20, ny=4, length=100.0, width=20.0, noise_sigma=0.05, noise_corr=5.0,
                         Q=10.0),
+    "restart20x4": dict(nx=20, ny=4, length=100.0, width=20.0, noise_sigma=0.05, noise_corr=5.0,
+                        Q=10.0),
 }
 
 # a bank that rises out of the water: the bed goes up linearly across the width from y0 to
@@ -144,7 +146,8 @@ HYDRO = {
 # decks kept to compact records (no per-step matrices or element records); "steps": the
 # per-step corrections, "timesteps": the committed state of every time step instead
 COMPACT = {"channel200x20": "steps", "vegchannel200x20": "steps", "meander40x64": "steps",
-           "flood200x6": "timesteps", "relax3_20x4": "steps", "relax3u20x4": "steps"}
+           "flood200x6": "timesteps", "relax3_20x4": "steps", "relax3u20x4": "steps",
+           "restart20x4": "steps"}
 
 # time integration of the decks that are not one stationary flow step, and the Newton step
 # whose element matrices are recorded (one with non-zero time derivatives)
@@ -162,6 +165,10 @@ RUNS = {
     "relax3_20x4": dict(relax="3 0.001 0.1 0.05 0.02 0.1", maxiter=40, dt=100.0, relax_time=5.0),
     "relax3u20x4": dict(relax="3 0.001 0.1 0.02 0.01 0.1", maxiter=60, tm_steps=3, dt=30.0,
                         theta=0.7, stationary=0, relax_time=2.0, elem_step=1),
+    # a Newton limit the relaxed time step cannot reach the time step in: the cycle restarts
+    # with the relaxed time step as the step's length, the rest of the interval follows
+    "restart20x4": dict(relax="3 0.001 0.1 0.005 0.002 0.1", maxiter=4, tm_steps=2, dt=30.0,
+                        theta=0.7, stationary=0, relax_time=0.5),
 }
 
 

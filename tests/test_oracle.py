@@ -274,6 +274,52 @@ def test_oracle_matches_reference_relaxation_method_3(name):
     assert min(grew) >= float(g["relax_time"]) and max(grew) == dt
 
 
+def test_unconverged_cycle_stops_at_the_newton_limit():
+    """The restart of an unsteady cycle with a shorter time step (EqsUVS2D.cpp:931-962) is
+    never reached: the test before it ends the cycle at it == actualCycit - 1 (:893).  A
+    deck whose relaxed time step cannot grow (maxDeltaUV 0.005, maxDeltaS 0.002, from 0.5 s,
+    time step 30 s, 4 Newton steps): the reference stops every time step after 4
+    unconverged Newton steps, and the runner's time loop driving the oracle reproduces every
+    correction and the final state bit for bit."""
+    from rismo2d_amd import run as rr
+    g = load_golden("restart20x4")
+    om = po.OracleModel(golden_deck(g))
+    r = g["relax"]
+    om.set_params(relax_method=int(r[0]), max_delta_uv=float(r[3]), max_delta_s=float(r[4]),
+                  relax_time=float(g["relax_time"]), interval=float(g["dt"]))
+
+    class Cycle:                       # the oracle behind the runner's time loop
+        k = 0
+
+        def time_step(self, dt, theta, first):
+            om.time_step(dt, theta, first)
+
+        def time_commit(self):
+            om.time_commit()
+
+        def set_inflow(self, Q, S):
+            om.set_inflow(Q, S)
+
+        def slip_flow(self):
+            om.slip_flow()
+
+    cyc = Cycle()
+
+    def newton():
+        om.prepare()
+        X, st = om.newton_step(prec=32, solver=1, max_iter=1000, eps=1e-10)
+        np.testing.assert_array_equal(X, g[f"s{cyc.k}_xsol"], err_msg=f"step {cyc.k}")
+        cyc.k += 1
+        assert om.relax_time()[0] < float(g["dt"])      # the relaxed time step stays short
+        return st
+
+    counts = rr.time_loop(cyc, int(g["tm_steps"]), float(g["dt"]), float(g["theta"]),
+                          int(g["maxiter"]), newton)
+    tsteps = [int(g[f"s{k}_tstep"]) for k in range(int(g["nsteps"]))]
+    assert counts == [tsteps.count(t) for t in (1, 2)] == [int(g["maxiter"])] * 2
+    _check_state(om, g, "final_")
+
+
 # ---------------------------------------------------------------------------------------
 # closed forms and invariants
 
