@@ -245,6 +245,166 @@ __global__ void __launch_bounds__(CR_MAX) k_cr_bwd(int nb, int bs, int s,
   w[(long long)j * bs + r] = x;
 }
 
+// ---- factorisation of the reduction on the device (column-major bs x bs blocks) ----
+
+// blocks[slot_dst[k]] = cval[k]: the level-0 blocks B (i, i-1), A (i, i), C (i, i+1)
+__global__ void k_cr_scatter(long long nslot, const long long* __restrict__ dst,
+                             const double* __restrict__ cval, double* __restrict__ blocks)
+{
+  const long long k = (long long)blockIdx.x * blockDim.x + threadIdx.x;
+  if (k < nslot) blocks[dst[k]] = cval[k];
+}
+
+// rows past a block's aggregates: identity in A
+__global__ void k_cr_pad(int nb, int bs, const int* __restrict__ blk_start, double* __restrict__ Am)
+{
+  const int i = blockIdx.x, r = threadIdx.x;
+  if (i >= nb || r >= bs) return;
+  if (r >= blk_start[i + 1] - blk_start[i]) Am[(long long)i * bs * bs + (long long)r * bs + r] = 1.0;
+}
+
+// One workgroup inverts an n x n block held in LDS (column-major) by Gauss-Jordan with
+// partial pivoting; returns false (same in every thread) for a pivot below 1e-300.
+__device__ bool lds_inverse(double* m, int n, int* perm, double* red_v, int* red_i, double* fcol)
+{
+  const int t = threadIdx.x, nt = blockDim.x;
+  for (int c = 0; c < n; c++) {
+    // pivot: the largest |m(r, c)|, r >= c (first one on ties)
+    double best = -1.0;
+    int bi = c;
+    for (int r = c + t; r < n; r += nt) {
+      const double v = fabs(m[c * n + r]);
+      if (v > best) { best = v; bi = r; }
+    }
+    red_v[t] = best; red_i[t] = bi;
+    __syncthreads();
+    for (int o = nt / 2; o > 0; o >>= 1) {
+      if (t < o) {
+        const double v = red_v[t + o];
+        const int j = red_i[t + o];
+        if (v > red_v[t] || (v == red_v[t] && j < red_i[t])) { red_v[t] = v; red_i[t] = j; }
+      }
+      __syncthreads();
+    }
+    const int p = red_i[0];
+    const bool ok = red_v[0] > 1.0e-300;
+    __syncthreads();
+    if (!ok) return false;
+    if (t == 0) perm[c] = p;
+    if (p != c)
+      for (int k = t; k < n; k += nt) {
+        const double a = m[k * n + c];
+        m[k * n + c] = m[k * n + p];
+        m[k * n + p] = a;
+      }
+    __syncthreads();
+    const double piv = m[c * n + c];
+    __syncthreads();
+    // row c scaled (its pivot column becomes 1 / piv), then every other row eliminated with
+    // the factors of column c read before any is overwritten
+    for (int k = t; k < n; k += nt) m[k * n + c] = (k == c ? 1.0 : m[k * n + c]) / piv;
+    for (int r = t; r < n; r += nt) fcol[r] = m[c * n + r];
+    __syncthreads();
+    for (int idx = t; idx < n * n; idx += nt) {
+      const int r = idx % n, k = idx / n;
+      if (r == c) continue;
+      const double v = k == c ? 0.0 : m[k * n + r];
+      m[k * n + r] = v - fcol[r] * m[k * n + c];
+    }
+    __syncthreads();
+  }
+  // undo the row interchanges as column interchanges, in reverse order
+  for (int c = n - 1; c >= 0; c--) {
+    const int p = perm[c];
+    if (p != c)
+      for (int r = t; r < n; r += nt) {
+        const double a = m[c * n + r];
+        m[c * n + r] = m[p * n + r];
+        m[p * n + r] = a;
+      }
+    __syncthreads();
+  }
+  return true;
+}
+
+// Eliminated blocks of level s (j = s + 2 s k; s = 0: the top block): A_j^-1 (also kept in
+// inv for the level's kept blocks), B_j and C_j into the level's storage E.
+__global__ void __launch_bounds__(256) k_cr_inv(int nb, int bs, int s, const double* __restrict__ Am,
+                                                const double* __restrict__ Bm,
+                                                const double* __restrict__ Cm,
+                                                double* __restrict__ E, double* __restrict__ inv,
+                                                int* __restrict__ bad)
+{
+  extern __shared__ double sm[];
+  __shared__ double red_v[256], fcol[CR_MAX];
+  __shared__ int red_i[256], perm[CR_MAX];
+  const int k = blockIdx.x, t = threadIdx.x, nt = blockDim.x;
+  const int j = s ? s + 2 * s * k : 0;
+  if (j >= nb) return;
+  const long long b2 = (long long)bs * bs;
+  for (int e = t; e < b2; e += nt) sm[e] = Am[j * b2 + e];
+  __syncthreads();
+  const bool ok = lds_inverse(sm, bs, perm, red_v, red_i, fcol);
+  if (!ok) {
+    if (t == 0) atomicOr(bad, 1);
+    return;
+  }
+  double* Ai = E + (s ? 3 * b2 * k : 0);
+  for (int e = t; e < b2; e += nt) {
+    Ai[e] = sm[e];
+    inv[j * b2 + e] = sm[e];
+    if (s) {
+      Ai[b2 + e] = Bm[j * b2 + e];
+      Ai[2 * b2 + e] = Cm[j * b2 + e];
+    }
+  }
+}
+
+// out (+)= alpha a b, column-major bs x bs, by one workgroup
+__device__ void wg_mm(int n, const double* a, const double* b, double alpha, bool acc, double* out)
+{
+  for (int idx = threadIdx.x; idx < n * n; idx += blockDim.x) {
+    const int r = idx % n, c = idx / n;
+    double v = 0.0;
+    for (int k = 0; k < n; k++) v += a[k * n + r] * b[c * n + k];
+    out[idx] = (acc ? out[idx] : 0.0) + alpha * v;
+  }
+  __syncthreads();
+}
+
+// Kept blocks of level s (i = 2 s k): P = B_i A_{i-s}^-1, Q = C_i A_{i+s}^-1 into the
+// level's storage K; A_i -= P C_{i-s} + Q B_{i+s}; B_i = -P B_{i-s}; C_i = -Q C_{i+s}.
+// tmp: per kept block two scratch blocks.
+__global__ void __launch_bounds__(256) k_cr_keep(int nb, int bs, int s, double* __restrict__ Am,
+                                                 double* __restrict__ Bm, double* __restrict__ Cm,
+                                                 const double* __restrict__ inv,
+                                                 double* __restrict__ K, double* __restrict__ tmp)
+{
+  const int k = blockIdx.x, i = 2 * s * k;
+  if (i >= nb) return;
+  const long long b2 = (long long)bs * bs;
+  double* P = K + 2 * b2 * k;
+  double* Q = P + b2;
+  double* nB = tmp + 2 * b2 * k;
+  double* nC = nB + b2;
+  for (int e = threadIdx.x; e < b2; e += blockDim.x) { P[e] = 0.0; Q[e] = 0.0; nB[e] = 0.0; nC[e] = 0.0; }
+  __syncthreads();
+  if (i - s >= 0) {
+    wg_mm(bs, Bm + i * b2, inv + (i - s) * b2, 1.0, false, P);
+    wg_mm(bs, P, Cm + (i - s) * b2, -1.0, true, Am + i * b2);
+    wg_mm(bs, P, Bm + (i - s) * b2, -1.0, false, nB);
+  }
+  if (i + s < nb) {
+    wg_mm(bs, Cm + i * b2, inv + (i + s) * b2, 1.0, false, Q);
+    wg_mm(bs, Q, Bm + (i + s) * b2, -1.0, true, Am + i * b2);
+    wg_mm(bs, Q, Cm + (i + s) * b2, -1.0, false, nC);
+  }
+  for (int e = threadIdx.x; e < b2; e += blockDim.x) {
+    Bm[i * b2 + e] = nB[e];
+    Cm[i * b2 + e] = nC[e];
+  }
+}
+
 // host dense helpers on column-major bs x bs blocks
 static bool dense_inverse(int n, const double* a, double* inv)
 {
@@ -343,6 +503,7 @@ static void cr_setup(Coarse& C, const std::vector<int>& agg_strip, int nstrips)
   C.mats.alloc(off);
   C.w.alloc((size_t)nb * bs);
   C.d_blk_start.upload(C.blk_start, nullptr);
+  C.d_slot_dst.upload(C.slot_dst, nullptr);
   HIPCHECK(hipDeviceSynchronize());
   C.cr_ok = true;
 }
@@ -408,6 +569,48 @@ static bool cr_factor(Coarse& C, const double* cval)
     });
   }
   return dense_inverse(bs, &Am[0], C.h_mats.data() + C.top_off);
+}
+
+// The factorisation on the device from the coarse entries on the device (C.cval): the same
+// steps as cr_factor, a workgroup per block and level.  Returns false for a singular block.
+static bool cr_factor_device(Ctx& c, Coarse& C)
+{
+  const int nb = C.nb, bs = C.bs;
+  const long long b2 = (long long)bs * bs;
+  hipStream_t st = c.stream;
+  C.d_blocks.alloc(3 * nb * b2);
+  C.d_inv.alloc(nb * b2);
+  C.d_tmp.alloc(nb * b2 + 2 * b2);
+  C.d_bad.alloc(1);
+  HIPCHECK(hipMemsetAsync(C.d_blocks.p, 0, sizeof(double) * 3 * nb * b2, st));
+  HIPCHECK(hipMemsetAsync(C.d_bad.p, 0, sizeof(int), st));
+  double* Bm = C.d_blocks.p;
+  double* Am = Bm + nb * b2;
+  double* Cm = Am + nb * b2;
+  k_cr_scatter<<<(int)((C.nslot + 255) / 256), 256, 0, st>>>(C.nslot, C.d_slot_dst.p, C.cval.p, Bm);
+  k_cr_pad<<<nb, (bs + 63) / 64 * 64, 0, st>>>(nb, bs, C.d_blk_start.p, Am);
+  const size_t lds = sizeof(double) * b2;
+  static bool attr = false;
+  if (!attr && lds > 65536) {
+    HIPCHECK(hipFuncSetAttribute((const void*)k_cr_inv, hipFuncAttributeMaxDynamicSharedMemorySize,
+                                 (int)(sizeof(double) * CR_MAX * CR_MAX)));
+    attr = true;
+  }
+  int lev = 0;
+  for (int s = 1; s < nb; s *= 2, lev++) {
+    const int ne = (nb - s + 2 * s - 1) / (2 * s);
+    const int nk = (nb + 2 * s - 1) / (2 * s);
+    k_cr_inv<<<ne, 256, lds, st>>>(nb, bs, s, Am, Bm, Cm, C.mats.p + C.lev_elim[lev], C.d_inv.p,
+                                   C.d_bad.p);
+    k_cr_keep<<<nk, 256, 0, st>>>(nb, bs, s, Am, Bm, Cm, C.d_inv.p, C.mats.p + C.lev_keep[lev],
+                                  C.d_tmp.p);
+  }
+  k_cr_inv<<<1, 256, lds, st>>>(nb, bs, 0, Am, Bm, Cm, C.mats.p + C.top_off, C.d_inv.p, C.d_bad.p);
+  HIPCHECK(hipGetLastError());
+  int bad = 0;
+  HIPCHECK(hipMemcpyAsync(&bad, C.d_bad.p, sizeof(int), hipMemcpyDeviceToHost, st));
+  HIPCHECK(hipStreamSynchronize(st));
+  return bad == 0;
 }
 
 static void cr_apply(Ctx& c, Coarse& C)
@@ -578,10 +781,17 @@ void coarse_factor(Ctx& c, int prec)
     k_ap_vals<double><<<gap, 256, 0, c.stream>>>(C.nap, C.ap_src_ptr.p, C.ap_src.p, c.A64.p, C.apv.p);
   }
   HIPCHECK(hipGetLastError());
-  HIPCHECK(hipMemcpyAsync(C.h_buf, C.cval.p, sizeof(double) * C.nslot, hipMemcpyDeviceToHost, c.stream));
-  HIPCHECK(hipStreamSynchronize(c.stream));
   C.fac = false;
   C.cr_fac = false;
+  // the block cyclic reduction factored on the device (RMX_COARSE_HOST_FACTOR: on the host)
+  if (C.cr_ok && !getenv("RMX_COARSE_HOST_FACTOR") && cr_factor_device(c, C)) {
+    C.cr_fac = true;
+    C.fac = true;
+    C.fac_prec = prec;
+    return;
+  }
+  HIPCHECK(hipMemcpyAsync(C.h_buf, C.cval.p, sizeof(double) * C.nslot, hipMemcpyDeviceToHost, c.stream));
+  HIPCHECK(hipStreamSynchronize(c.stream));
   if (C.cr_ok && cr_factor(C, C.h_buf)) {
     HIPCHECK(hipMemcpyAsync(C.mats.p, C.h_mats.data(), sizeof(double) * C.mats_n,
                             hipMemcpyHostToDevice, c.stream));

@@ -227,6 +227,9 @@ struct Coarse {
   std::vector<double> h_mats;
   DBuf<double> mats, w;                       // factor matrices; padded work vector
   DBuf<int> d_blk_start;
+  DBuf<long long> d_slot_dst;
+  DBuf<double> d_blocks, d_inv, d_tmp;        // device factorisation: B, A, C blocks; inverses
+  DBuf<int> d_bad;                            // a singular block
   ~Coarse() { if (h_buf) (void)hipHostFree(h_buf); }
 };
 

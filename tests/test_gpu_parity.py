@@ -323,7 +323,7 @@ def test_block_sweeps_split_levels(ctx, rows, pairs, monkeypatch):
         assert np.abs(y - ref).max() <= 1e-12 * np.abs(ref).max(), prec
 
 
-@pytest.mark.parametrize("coarse_solve", ["device", "host"])
+@pytest.mark.parametrize("coarse_solve", ["device", "device-host-factor", "host"])
 @pytest.mark.parametrize("strips,segments,cseg", [(4, 8, 2), (17, 4, 4), (1, 1, 1), (4, 8, 24)])
 def test_two_level_apply_matches_dense_reference(ctx, strips, segments, cseg, coarse_solve,
                                                  monkeypatch):
@@ -331,15 +331,18 @@ def test_two_level_apply_matches_dense_reference(ctx, strips, segments, cseg, co
     formula in numpy: R A P summed and solved densely, M^-1 by the oracle's triangular
     solves of the block-restricted full ILU(0) factors.  FP64 and FP32 (matrix, factors and
     coarse matrix from the FP32 values): 1e-9 of max|z|.  The coarse system is solved by
-    block cyclic reduction on the device (R A P is block tridiagonal in strips here) or,
-    with RMX_COARSE_HOST set when the coarse space is built, by the host band LU."""
+    block cyclic reduction on the device (R A P is block tridiagonal in strips here; its
+    blocks inverted and reduced on the device, or on the host with RMX_COARSE_HOST_FACTOR)
+    or, with RMX_COARSE_HOST set when the coarse space is built, by the host band LU."""
     import scipy.sparse as sp
     print(f"[gpu] two-level apply, {strips} strips x {segments} segments, {cseg} coarse, "
           f"{coarse_solve} coarse solve", flush=True)
+    monkeypatch.delenv("RMX_COARSE_HOST", raising=False)
+    monkeypatch.delenv("RMX_COARSE_HOST_FACTOR", raising=False)
     if coarse_solve == "host":
         monkeypatch.setenv("RMX_COARSE_HOST", "1")
-    else:
-        monkeypatch.delenv("RMX_COARSE_HOST", raising=False)
+    elif coarse_solve == "device-host-factor":
+        monkeypatch.setenv("RMX_COARSE_HOST_FACTOR", "1")
     g = load_golden("channel200x20")
     ctx.upload_mesh(golden_deck(g))
     n = ctx.prepare()
