@@ -81,7 +81,8 @@ typedef struct {
                                  (rmx_time_step's dt, or interval for stationary flow); the
                                  Newton test fails until it gets there (EqsUVS2D.cpp:97-147,
                                  540-627); the correction is capped as by method 2 */
-  int dry_method;             /* 0,2,3: marsh bed lowering; else depth clip */
+  int dry_method;             /* 0,2,3: marsh bed lowering; else depth clip; 2: the drying and
+                                 rewetting method of rmx_dry_rewet */
   double dry_limit, max_delta_uv, max_delta_s, conv_uv, conv_s, max_us;
   int turb;                   /* turbulence model, the $TM_TURBULENCE code (Timeint.cpp:1333):
                                  0 or 1 constant vt of the element, 2 algebraic model
@@ -97,6 +98,11 @@ typedef struct {
                                  time step: the time step) */
   double interval;            /* methods 3 and 4, stationary flow: the cycle's time step
                                  ($TM_INTERVAL's first value) */
+  double rewet_limit;         /* drying and rewetting ($DRYREW rewetLimit): the water level a
+                                 rewetted marsh node takes is capped at zor + rewet_limit (not
+                                 below dry_limit) */
+  int count_down;             /* $DRYREW countDown: calls a node stays dry after its depth
+                                 last fell below dry_limit (>= 0) */
 } rmx_params;
 
 /* Mesh, boundary lines and material data (MODEL::Input, MODEL::Initialize,
@@ -155,6 +161,29 @@ rmx_status rmx_time_commit(rmx_ctx* ctx);
  * inlet and outlet) the velocity and its time derivative keep only their component along
  * the first column of the node's rotation.  Needs rmx_prepare. */
 rmx_status rmx_slip_flow(rmx_ctx* ctx);
+/* Drying and rewetting, method 2 (MODEL::DoDryRewet, DoDryRew.cpp:50-238, with
+ * GRID::DryRewet, DryRewet.cpp:696-900), on the device: nodes shallower than dry_limit are
+ * dry (and stay so for count_down calls), elements with all corners dry are dry, with some
+ * marsh; marsh nodes that were dry take the level of their wet neighbours (capped at zor +
+ * rewet_limit) and a lowered bed; dry nodes are reset.  When elements got dry or wet the wet
+ * model is set up again as MODEL::Initialize, SetNormal and SetRotation do (Model.cpp:
+ * 121-215, Transform.cpp): boundary line elements on the wet/dry line, boundary conditions
+ * and rotations; the equations are numbered without the dry nodes and the pattern, assembly
+ * schedule, ILU plans and coarse space rebuilt, then the slip condition applied
+ * (GRID::SetSlipFlow).  The nodal eddy viscosity becomes due.  dried, wetted: the elements
+ * that got dry and wet (nullable).  Needs params.dry_method 2; the flow cycle calls it at
+ * its first Newton step and every dryRewFreq steps or after a converged one
+ * (EqsUVS2D.cpp:200-222).  rmx_time_commit keeps the flags as the previous time step's,
+ * which Predict and Timegrad read (Compute.cpp:773-783, EqsUVS2D.cpp:1153, 1193-1202). */
+rmx_status rmx_dry_rewet(rmx_ctx* ctx, int* dried, int* wetted);
+/* Per node the reference's dry (64) and marsh (256) bits (Node.h:100-102); per region element
+ * 1 dry, 2 marsh.  Either array nullable. */
+rmx_status rmx_get_wetdry(rmx_ctx* ctx, int* nflag /* [np] */, int* eflag /* [ne] */);
+/* The relaxed time steps dt_UV, dt_H of the running cycle and its time step dt (below
+ * relaxation method 3 all three are dt), for the flow cycle's test of the drying and
+ * rewetting (EqsUVS2D.cpp:213-214).  Each nullable. */
+rmx_status rmx_get_relax_time(rmx_ctx* ctx, double* dt_uv, double* dt_h, double* dt);
+
 /* Host copies of the time-integration state, 9 [np] arrays (NULL entries skipped):
  * Uo, Vo, So, dUdt_o, dVdt_o, dSdt_o (previous time step), dUdt, dVdt, dSdt (current).
  * set = 1 uploads them, 0 downloads. */
@@ -326,6 +355,9 @@ rmx_status rmx_get_matrix(rmx_ctx* ctx, int prec, void* vals /* nnz */, double* 
 rmx_status rmx_get_factors(rmx_ctx* ctx, int prec, void* vals /* nnz */);
 rmx_status rmx_get_solution(rmx_ctx* ctx, double* X /* neq */);
 rmx_status rmx_set_rhs(rmx_ctx* ctx, const double* B);
+/* The Newton correction rmx_newton_update applies: a caller that solved the system itself
+ * (as EQS_UVS2D::Execute gets X from EQS::Solve, EqsUVS2D.cpp:365) hands it over. */
+rmx_status rmx_set_solution(rmx_ctx* ctx, const double* X /* neq */);
 /* Load an arbitrary CSR system (diagonal first in every row): lets a caller that keeps
  * the reference's own EQS/CRSMAT host code hand its pattern and values over. */
 rmx_status rmx_csr_upload(rmx_ctx* ctx, int n, const int* rowptr, const int* cols,

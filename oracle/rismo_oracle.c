@@ -33,7 +33,7 @@ enum { kInlet = 1, kOutlet = 2, kOpenBnd = 4, kQInlet = 8, kQTInlet = 16, kSlip 
 enum { rotTangent = 1, rotNormal = 2 };
 /* node flags of this restatement (own bit values) */
 enum { fCorn = 1, fMids = 2, fBound = 4, fInlet = 8, fOutlet = 16, fOpen = 32, fRotat = 64,
-       fNoMoment = 128, fDry = 256, fMarsh = 512 };
+       fNoMoment = 128, fDry = 256, fMarsh = 512, fDryPrev = 1024, fMarshPrev = 2048 };
 /* boundary element flags */
 enum { eInlet = 1, eOutlet = 2, eOpen = 4 };
 /* region element flags */
@@ -503,8 +503,8 @@ void orc_get_time_state(const orc_model* m, double* Uo, double* Vo, double* So,
  * 105-127 with relaxMethod < 3).  dt > 0: the theta scheme, relaxThdt = 1 / dt / theta in
  * the Jacobian (:130-147), then EQS_UVS2D::Predict (:1101-1180): current time derivatives
  * zero; at nodes whose previous level lies below zor + dryLimit the previous time
- * derivatives are zero and, at the first time step, the previous state is the current
- * one.  The algebraic eddy viscosity becomes due (first Newton step of the cycle).
+ * derivatives are zero and, at the first time step or where the node was dry at the
+ * previous time step (:1153), the previous state is the current one.  The algebraic eddy viscosity becomes due (first Newton step of the cycle).
  * Returns -1 for theta or dt too small (:91-94). */
 int orc_time_step(orc_model* m, double dt, double theta, int first)
 {
@@ -520,20 +520,24 @@ int orc_time_step(orc_model* m, double dt, double theta, int first)
   for (int n = 0; n < m->np; n++) m->dUdt[n] = m->dVdt[n] = m->dSdt[n] = 0.0;
   for (int n = 0; n < m->np; n++) {
     if (m->So[n] < m->zor[n] + m->dryLimit) {
-      if (first) { m->Uo[n] = m->U[n]; m->Vo[n] = m->V[n]; m->So[n] = m->S[n]; }
+      if (first || (m->flag[n] & fDryPrev)) { m->Uo[n] = m->U[n]; m->Vo[n] = m->V[n]; m->So[n] = m->S[n]; }
       m->dUdto[n] = m->dVdto[n] = m->dSdto[n] = 0.0;
     }
   }
   return 0;
 }
 
-/* EQS_UVS2D::Timegrad (EqsUVS2D.cpp:1183-1230) for wet nodes (no node is dry or marsh
- * here): d/dt = (1 - 1/theta) d/dt_old + (v - v_old) / (dt theta) */
+/* EQS_UVS2D::Timegrad (EqsUVS2D.cpp:1183-1230): zero at nodes dry or marsh now or at the
+ * previous time step, else d/dt = (1 - 1/theta) d/dt_old + (v - v_old) / (dt theta) */
 static void timegrad(orc_model* m)
 {
   const double iTheta = 1.0 - 1.0 / m->theta;
   const double thdt = 1.0 / m->dt / m->theta;
   for (int n = 0; n < m->np; n++) {
+    if (m->flag[n] & (fDry | fDryPrev | fMarsh | fMarshPrev)) {
+      m->dUdt[n] = m->dVdt[n] = m->dSdt[n] = 0.0;
+      continue;
+    }
     m->dUdt[n] = iTheta * m->dUdto[n] + thdt * (m->U[n] - m->Uo[n]);
     m->dVdt[n] = iTheta * m->dVdto[n] + thdt * (m->V[n] - m->Vo[n]);
     m->dSdt[n] = iTheta * m->dSdto[n] + thdt * (m->S[n] - m->So[n]);
@@ -576,10 +580,16 @@ void orc_hydrograph(int n, const double* t, const double* Q, const double* S, do
   }
 }
 
-/* End of a time step: the state becomes the previous one (Compute.cpp:773-796) */
+/* End of a time step: the state and the dry and marsh flags become the previous ones
+ * (Compute.cpp:773-796) */
 void orc_time_commit(orc_model* m)
 {
   const size_t b = sizeof(double) * (size_t)m->np;
+  for (int n = 0; n < m->np; n++) {
+    m->flag[n] &= ~(unsigned)(fDryPrev | fMarshPrev);
+    if (m->flag[n] & fDry) m->flag[n] |= fDryPrev;
+    if (m->flag[n] & fMarsh) m->flag[n] |= fMarshPrev;
+  }
   memcpy(m->Uo, m->U, b); memcpy(m->Vo, m->V, b); memcpy(m->So, m->S, b);
   memcpy(m->dUdto, m->dUdt, b); memcpy(m->dVdto, m->dVdt, b); memcpy(m->dSdto, m->dSdt, b);
 }
@@ -1295,6 +1305,7 @@ int orc_dry_rewet(orc_model* m, int* dried)
         for (int i = 0; i < m->nnd[e]; i++) m->noel[m->elem[8 * e + i]]++;
   }
   m->firstDryRew = 0;
+  m->turb_stale = 1;                       /* eddy = true (EqsUVS2D.cpp:216) */
   if (dried) *dried = del;
   return wel;
 }

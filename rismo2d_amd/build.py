@@ -11,8 +11,8 @@ ROOT = os.path.dirname(HERE)
 CSRC = os.path.join(HERE, "csrc")
 BUILD = os.path.join(HERE, "_build")
 LIB = os.path.join(HERE, "librmx.so")
-SOURCES = ["capi.cpp", "model.cpp", "assemble.hip", "sparse.hip", "ilu.hip", "coarse.hip", "krylov.hip", "newton.hip",
-           "mpnn.hip"]
+SOURCES = ["capi.cpp", "model.cpp", "assemble.hip", "sparse.hip", "ilu.hip", "coarse.hip",
+           "krylov.hip", "newton.hip", "wetdry.hip", "mpnn.hip"]
 HIPCC = os.environ.get("HIPCC", "/opt/rocm/bin/hipcc")
 FLAGS = ["--offload-arch=gfx950", "-O3", "-fPIC", "-std=c++17", "-Wall", "-Wno-unused-function",
          "-Wno-unused-variable", "-Xclang", "-target-feature", "-Xclang", "-packed-fp32-ops",

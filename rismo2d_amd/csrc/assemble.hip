@@ -719,8 +719,10 @@ __global__ void k_turb_elder(int np, const int* __restrict__ n2e_ptr, const int*
 }
 
 // ---- inlet discharge distribution (BCONLINE::Inlet, QINLET, quadratic line rule) ----
+// Dry and marsh nodes of the line take no discharge.
 __global__ void k_inlet(int nseg, const int* __restrict__ segs, int nin,
-                        const int* __restrict__ inodes, const double* __restrict__ x,
+                        const int* __restrict__ inodes, const uint8_t* __restrict__ nwd,
+                        const double* __restrict__ x,
                         const double* __restrict__ y, const double* __restrict__ z,
                         const double* __restrict__ S, const double* __restrict__ cf,
                         double Qspec, double Sspec, double* __restrict__ valU)
@@ -735,7 +737,7 @@ __global__ void k_inlet(int nseg, const int* __restrict__ segs, int nin,
       double q[3];
       for (int k = 0; k < 3; k++) {
         const int n = nd[k];
-        if (S[n] - z[n] < 1.0e-9) { q[k] = 0.0; continue; }
+        if (S[n] - z[n] < 1.0e-9 || (nwd[n] & (WD_DRY | WD_MARSH))) { q[k] = 0.0; continue; }
         const double h = Sspec - z[n];
         q[k] = h > 0.0 ? sqrt(h * h * h) : 0.0;
         if (cf[n] > 0.0) q[k] /= sqrt(cf[n]);
@@ -757,6 +759,7 @@ __global__ void k_inlet(int nseg, const int* __restrict__ segs, int nin,
     const double h = Sspec - z[n];
     double q = h > 0.0 ? sqrt(h * h * h) : 0.0;
     if (cf[n] > 0.0) q /= sqrt(cf[n]);
+    if (nwd[n] & (WD_DRY | WD_MARSH)) q = 0.0;
     valU[n] = q * s_ratio;
   }
 }
@@ -827,8 +830,9 @@ void dev_prepare_step(Ctx& c)
     launch_turbulence(c);
   const int nseg = (int)(c.hm.in_segs.size() / 3), nin = (int)c.hm.inlet_nodes.size();
   if (nseg > 0) {
-    k_inlet<<<1, 256, 0, c.stream>>>(nseg, c.d_in_segs.p, nin, c.d_inlet_nodes.p, c.x.p, c.y.p,
-                                     c.z.p, c.S.p, c.cf.p, c.hm.Q, c.hm.S_in, c.valU.p);
+    k_inlet<<<1, 256, 0, c.stream>>>(nseg, c.d_in_segs.p, nin, c.d_inlet_nodes.p, c.d_nwd.p,
+                                     c.x.p, c.y.p, c.z.p, c.S.p, c.cf.p, c.hm.Q, c.hm.S_in,
+                                     c.valU.p);
     HIPCHECK(hipGetLastError());
   }
 }

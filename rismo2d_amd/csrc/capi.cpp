@@ -115,6 +115,29 @@ rmx_status rmx_destroy(rmx_ctx* p)
   API_END
 }
 
+// The model element list of the assembly: nodes, type, eddy viscosity.
+static void upload_model_elements(Ctx& c)
+{
+  const HostModel& hm = c.hm;
+  const int nme = (int)hm.me.size();
+  std::vector<int> men(8LL * nme, -1), met(nme);
+  std::vector<double> mvt(nme, 0.0);
+  for (int k = 0; k < nme; k++) {
+    const int v = hm.me[k];
+    if (v >= 0) {
+      for (int i = 0; i < 8; i++) men[8LL * k + i] = hm.elem[8LL * v + i];
+      met[k] = hm.nnd[v] == 6 ? ET_TRI : ET_QUAD;
+      mvt[k] = hm.vt[v];
+    } else {
+      const int b = -v - 1;
+      for (int i = 0; i < 3; i++) men[8LL * k + i] = hm.bnd[3LL * b + i];
+      met[k] = ET_LINE;
+    }
+  }
+  c.nme = nme;
+  c.me_nodes.upload(men, c.stream); c.me_type.upload(met, c.stream); c.me_vt.upload(mvt, c.stream);
+}
+
 rmx_status rmx_mesh_upload(rmx_ctx* p, const rmx_mesh_desc* d)
 {
   API_BEGIN
@@ -168,25 +191,14 @@ rmx_status rmx_mesh_upload(rmx_ctx* p, const rmx_mesh_desc* d)
   for (int e = 0; e < hm.ne; e++)
     for (int i = 0; i < hm.nnd[e]; i++) list[fill[hm.elem[8LL * e + i]]++] = e;
   c.n2e_ptr.upload(ptr, s); c.n2e.upload(list, s);
-  // model element list: nodes, type, eddy viscosity
-  const int nme = (int)hm.me.size();
-  std::vector<int> men(8LL * nme, -1), met(nme);
-  std::vector<double> mvt(nme, 0.0);
-  for (int k = 0; k < nme; k++) {
-    const int v = hm.me[k];
-    if (v >= 0) {
-      for (int i = 0; i < 8; i++) men[8LL * k + i] = hm.elem[8LL * v + i];
-      met[k] = hm.nnd[v] == 6 ? ET_TRI : ET_QUAD;
-      mvt[k] = hm.vt[v];
-    } else {
-      const int b = -v - 1;
-      for (int i = 0; i < 3; i++) men[8LL * k + i] = hm.bnd[3LL * b + i];
-      met[k] = ET_LINE;
-    }
-  }
-  c.nme = nme;
-  c.me_nodes.upload(men, s); c.me_type.upload(met, s); c.me_vt.upload(mvt, s);
+  upload_model_elements(c);
   c.d_in_segs.upload(hm.in_segs, s);
+  // drying and rewetting: every node and element wet, no count down
+  c.d_nwd.upload(hm.nwd, s); c.d_ewd.upload(hm.ewd, s);
+  c.d_cdown.alloc(np);
+  HIPCHECK(hipMemsetAsync(c.d_cdown.p, 0, sizeof(int) * np, s));
+  c.first_dry = true;
+  c.wet_sets = 0;
   c.mesh_ready = true;
   c.prepared = false;
   c.csr_only = false;
@@ -225,6 +237,9 @@ rmx_status rmx_set_params(rmx_ctx* p, const rmx_params* q)
   c.par.cd = q->cd > 0.0 ? q->cd : 1.00;
   c.par.relax_time = q->relax_time;
   c.par.interval = q->interval;
+  if (q->count_down < 0) fail(RMX_ERR_ARG, "params: count_down must not be negative");
+  c.par.rewet_limit = q->rewet_limit;
+  c.par.count_down = q->count_down;
   cycle_start(c);
   c.turb_stale = true;
   API_END
@@ -318,6 +333,67 @@ rmx_status rmx_prepare(rmx_ctx* p, int* neq, long long* nnz)
   HIPCHECK(hipStreamSynchronize(c.stream));
   if (neq) *neq = c.n;
   if (nnz) *nnz = c.nnz;
+  API_END
+}
+
+rmx_status rmx_dry_rewet(rmx_ctx* p, int* dried, int* wetted)
+{
+  API_BEGIN
+  Ctx& c = *need(p);
+  if (!c.mesh_ready || c.csr_only) fail(RMX_ERR_ARG, "drying and rewetting: upload a mesh first");
+  if (c.par.dry_method != 2)
+    fail(RMX_ERR_ARG, "drying and rewetting: params.dry_method " +
+                      std::to_string(c.par.dry_method) + " (only method 2 is supported)");
+  dev_dry_rewet(c);
+  int del = 0, wel = 0;
+  for (uint8_t f : c.hm.ewd) {
+    if ((f & EL_DRY) && !(f & EL_WASDRY)) del++;
+    if (!(f & EL_DRY) && (f & EL_WASDRY)) wel++;
+  }
+  if (del + wel) {
+    // the wet model set up again: boundary line elements on the wet/dry line, boundary
+    // conditions and rotations, numbering without the dry nodes, pattern, assembly schedule,
+    // ILU plans and coarse space; then the slip condition at the rotated nodes
+    host_topology(c.hm);
+    upload_model_elements(c);
+    c.wet_sets++;
+    c.topo_hash = (c.topo_hash ^ (unsigned long long)c.wet_sets) * 1099511628211ULL;
+    first_prepare(c);
+    dev_slip_flow(c);
+  }
+  c.turb_stale = true;                       // eddy viscosity due at the next Newton step
+  HIPCHECK(hipStreamSynchronize(c.stream));
+  if (dried) *dried = del;
+  if (wetted) *wetted = wel;
+  API_END
+}
+
+rmx_status rmx_get_wetdry(rmx_ctx* p, int* nflag, int* eflag)
+{
+  API_BEGIN
+  Ctx& c = *need(p);
+  if (!c.mesh_ready) fail(RMX_ERR_ARG, "wet/dry flags: upload a mesh first");
+  std::vector<uint8_t> nw(c.hm.np);
+  c.d_nwd.download(nw.data(), nw.size(), c.stream);
+  HIPCHECK(hipStreamSynchronize(c.stream));
+  if (nflag)
+    for (int n = 0; n < c.hm.np; n++)
+      nflag[n] = ((nw[n] & WD_DRY) ? 64 : 0) | ((nw[n] & WD_MARSH) ? 256 : 0);
+  if (eflag)
+    for (int e = 0; e < c.hm.ne; e++) eflag[e] = c.hm.ewd[e] & (EL_DRY | EL_MARSH);
+  API_END
+}
+
+rmx_status rmx_get_relax_time(rmx_ctx* p, double* dt_uv, double* dt_h, double* dt)
+{
+  API_BEGIN
+  Ctx& c = *need(p);
+  // below method 3 the relaxed time steps are the time step itself (EqsUVS2D.cpp:115-143)
+  const double t = c.unsteady ? c.dt : c.par.interval;
+  const bool relaxed = c.par.relax_method >= 3;
+  if (dt_uv) *dt_uv = relaxed ? c.dt_uv : t;
+  if (dt_h) *dt_h = relaxed ? c.dt_h : t;
+  if (dt) *dt = t;
   API_END
 }
 
@@ -423,6 +499,7 @@ rmx_status rmx_time_commit(rmx_ctx* p)
   Ctx& c = *need(p);
   if (!c.mesh_ready) fail(RMX_ERR_ARG, "time commit: upload a mesh first");
   dev_time_commit(c);
+  dev_wetdry_commit(c);
   HIPCHECK(hipStreamSynchronize(c.stream));
   API_END
 }
@@ -803,6 +880,17 @@ rmx_status rmx_get_solution(rmx_ctx* p, double* X)
   API_BEGIN
   Ctx& c = *need(p);
   c.X.download(X, c.n, c.stream);
+  HIPCHECK(hipStreamSynchronize(c.stream));
+  API_END
+}
+
+rmx_status rmx_set_solution(rmx_ctx* p, const double* X)
+{
+  API_BEGIN
+  Ctx& c = *need(p);
+  if (!c.prepared) fail(RMX_ERR_ARG, "no system");
+  if (!X) fail(RMX_ERR_ARG, "null solution");
+  c.X.upload(X, c.n, c.stream);
   HIPCHECK(hipStreamSynchronize(c.stream));
   API_END
 }

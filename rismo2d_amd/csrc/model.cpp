@@ -99,24 +99,39 @@ void host_mesh(HostModel& hm, const rmx_mesh_desc& d)
       hm.areaFact[e] = ae / ap;
     }
   });
-  // connectivity: corner / midside flags, number of connected elements
-  hm.noel.assign(d.np, 0);
   hm.flag.assign(d.np, 0);
-  for (int e = 0; e < d.ne; e++) {
+  hm.nwd.assign(d.np, 0);
+  hm.ewd.assign(d.ne, 0);
+  host_topology(hm);
+}
+
+// GRID::Connection over the wet elements and MODEL::Initialize / Boundary (Model.cpp:121-215,
+// Bound.cpp:50-170): corner / midside flags, the wet elements connected to each node, a
+// boundary line element for every midside node of one wet element, and the model element
+// list (each wet region element followed by its boundary line elements).  The boundary
+// flags of the last set-up are cleared first (Bound.cpp:64-80); dry elements (EL_DRY) are
+// left out, so the wet/dry line becomes a boundary.
+void host_topology(HostModel& hm)
+{
+  const int np = hm.np, ne = hm.ne;
+  hm.noel.assign(np, 0);
+  for (int n = 0; n < np; n++) hm.flag[n] &= ~nBound;
+  for (int e = 0; e < ne; e++) {
     int k = hm.nnd[e], ncn = k == 6 ? 3 : 4;
+    const bool wet = !(hm.ewd[e] & EL_DRY);
     for (int i = 0; i < k; i++) {
       int n = hm.elem[8LL * e + i];
-      hm.noel[n]++;
+      if (wet) hm.noel[n]++;
       hm.flag[n] |= (i < ncn) ? nCorn : nMids;
     }
   }
-  // boundary line element for every midside node with one connected element
-  for (int n = 0; n < d.np; n++)
+  for (int n = 0; n < np; n++)
     if (hm.noel[n] == 1 && (hm.flag[n] & nMids)) hm.flag[n] |= nBound;
   hm.bnd.clear();
   hm.me.clear();
-  hm.me.reserve(d.ne + d.ne / 8);
-  for (int e = 0; e < d.ne; e++) {
+  hm.me.reserve(ne + ne / 8);
+  for (int e = 0; e < ne; e++) {
+    if (hm.ewd[e] & EL_DRY) continue;
     int k = hm.nnd[e], ncn = k == 6 ? 3 : 4;
     hm.me.push_back(e);
     for (int i = ncn; i < k; i++) {
@@ -244,8 +259,8 @@ static int me_nodes(const HostModel& hm, int k, const int** nd)
 }
 
 // EQS::SetEqno / LastEquation / ResetEqOrder (SetEqno.cpp:52-487): provisional numbers
-// in node order without fixed dofs, then renumbering in order of last occurrence over
-// the model element list.
+// in node order without fixed dofs and dry nodes, then renumbering in order of last
+// occurrence over the model element list.
 void host_numbering(HostModel& hm)
 {
   static const uint32_t fix[3] = { kFixU, kFixV, kSetS };
@@ -253,6 +268,7 @@ void host_numbering(HostModel& hm)
   hm.eqno.assign(3LL * np, -1);
   int neq = 0;
   for (int n = 0; n < np; n++) {
+    if (hm.nwd[n] & WD_DRY) continue;
     int ndf = (hm.flag[n] & nCorn) ? 3 : 2;
     for (int j = 0; j < ndf; j++) {
       if (hm.kind[n] & fix[j]) continue;

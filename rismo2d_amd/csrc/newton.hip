@@ -102,9 +102,10 @@ __global__ void k_midside(int ne, const int* __restrict__ elem, const int* __res
 // ---- theta scheme (EQS_UVS2D::Predict / Timegrad, EqsUVS2D.cpp:1101-1230) ----
 
 // Predict: current time derivatives zero; where the previous level lies below
-// zor + dryLimit the previous derivatives are zero and, at the first time step, the
-// previous state is the current one.
-__global__ void k_predict(int np, int first, double dryLimit, const double* __restrict__ zor,
+// zor + dryLimit the previous derivatives are zero and, at the first time step or where the
+// node was dry at the previous time step, the previous state is the current one.
+__global__ void k_predict(int np, int first, double dryLimit, const uint8_t* __restrict__ nwd,
+                          const double* __restrict__ zor,
                           const double* __restrict__ U, const double* __restrict__ V,
                           const double* __restrict__ S, double* __restrict__ dUdt,
                           double* __restrict__ dVdt, double* __restrict__ dSdt,
@@ -116,13 +117,15 @@ __global__ void k_predict(int np, int first, double dryLimit, const double* __re
   if (n >= np) return;
   dUdt[n] = 0.0; dVdt[n] = 0.0; dSdt[n] = 0.0;
   if (So[n] < zor[n] + dryLimit) {
-    if (first) { Uo[n] = U[n]; Vo[n] = V[n]; So[n] = S[n]; }
+    if (first || (nwd[n] & WD_DRYPREV)) { Uo[n] = U[n]; Vo[n] = V[n]; So[n] = S[n]; }
     dUdto[n] = 0.0; dVdto[n] = 0.0; dSdto[n] = 0.0;
   }
 }
 
-// Timegrad for wet nodes: d/dt = (1 - 1/theta) d/dt_old + (v - v_old) / (dt theta)
-__global__ void k_timegrad(int np, double iTheta, double thdt, const double* __restrict__ U,
+// Timegrad: d/dt = (1 - 1/theta) d/dt_old + (v - v_old) / (dt theta); zero at nodes dry or
+// marsh now or at the previous time step
+__global__ void k_timegrad(int np, double iTheta, double thdt, const uint8_t* __restrict__ nwd,
+                           const double* __restrict__ U,
                            const double* __restrict__ V, const double* __restrict__ S,
                            const double* __restrict__ Uo, const double* __restrict__ Vo,
                            const double* __restrict__ So, const double* __restrict__ dUdto,
@@ -132,6 +135,10 @@ __global__ void k_timegrad(int np, double iTheta, double thdt, const double* __r
 {
   const int n = blockIdx.x * blockDim.x + threadIdx.x;
   if (n >= np) return;
+  if (nwd[n] & (WD_DRY | WD_MARSH | WD_DRYPREV | WD_MARSHPREV)) {
+    dUdt[n] = 0.0; dVdt[n] = 0.0; dSdt[n] = 0.0;
+    return;
+  }
   dUdt[n] = iTheta * dUdto[n] + thdt * (U[n] - Uo[n]);
   dVdt[n] = iTheta * dVdto[n] + thdt * (V[n] - Vo[n]);
   dSdt[n] = iTheta * dSdto[n] + thdt * (S[n] - So[n]);
@@ -176,9 +183,9 @@ void dev_time_init(Ctx& c)
 void dev_time_step(Ctx& c, int first)
 {
   const int np = c.hm.np;
-  k_predict<<<(np + 255) / 256, 256, 0, c.stream>>>(np, first, c.par.dry_limit, c.zor.p, c.U.p,
-      c.V.p, c.S.p, c.dUdt.p, c.dVdt.p, c.dSdt.p, c.Uo.p, c.Vo.p, c.So.p, c.dUdto.p, c.dVdto.p,
-      c.dSdto.p);
+  k_predict<<<(np + 255) / 256, 256, 0, c.stream>>>(np, first, c.par.dry_limit, c.d_nwd.p,
+      c.zor.p, c.U.p, c.V.p, c.S.p, c.dUdt.p, c.dVdt.p, c.dSdt.p, c.Uo.p, c.Vo.p, c.So.p,
+      c.dUdto.p, c.dVdto.p, c.dSdto.p);
   HIPCHECK(hipGetLastError());
 }
 
@@ -277,7 +284,7 @@ void dev_newton_update(Ctx& c, rmx_step_stats& st)
   HIPCHECK(hipGetLastError());
   if (c.unsteady) {                       // time derivatives of the corrected state
     k_timegrad<<<(np + 255) / 256, 256, 0, c.stream>>>(np, 1.0 - 1.0 / c.theta,
-        1.0 / c.dt / c.theta, c.U.p, c.V.p, c.S.p, c.Uo.p, c.Vo.p, c.So.p, c.dUdto.p,
+        1.0 / c.dt / c.theta, c.d_nwd.p, c.U.p, c.V.p, c.S.p, c.Uo.p, c.Vo.p, c.So.p, c.dUdto.p,
         c.dVdto.p, c.dSdto.p, c.dUdt.p, c.dVdt.p, c.dSdt.p);
     HIPCHECK(hipGetLastError());
   }

@@ -22,6 +22,11 @@ enum { rotTangent = 1, rotNormal = 2 };
 enum : uint32_t { nCorn = 1, nMids = 2, nBound = 4, nRotat = 64, nNoMoment = 128 };
 // device node flag bits (packed for the kernels)
 enum : uint32_t { dRotat = 1, dInlet = 2, dOutlet = 4 };
+// drying and rewetting, per node: dry, marsh, the same at the previous time step, dry when
+// the current drying and rewetting call started; per region element: dry, marsh, dry when
+// the call started
+enum : uint8_t { WD_DRY = 1, WD_MARSH = 2, WD_DRYPREV = 4, WD_MARSHPREV = 8, WD_WASDRY = 16 };
+enum : uint8_t { EL_DRY = 1, EL_MARSH = 2, EL_WASDRY = 4 };
 
 enum ElemType : int { ET_TRI = 0, ET_QUAD = 1, ET_LINE = 2 };
 
@@ -79,7 +84,9 @@ struct HostModel {
   std::vector<int> in_segs, out_segs;
   double Q = 0, S_in = 0, S_out = 0;
   bool inflow_qt = false;                     // the inlet line is a hydrograph (QTINLET)
-  // topology
+  // topology: wet/dry state of nodes and region elements (WD_*, EL_*), connected wet
+  // elements per node, node flags
+  std::vector<uint8_t> nwd, ewd;
   std::vector<int> noel;
   std::vector<uint32_t> flag;
   int nb = 0;
@@ -107,6 +114,8 @@ struct Params {
   int turb = 1;                               // $TM_TURBULENCE code
   double cm = 0.09, cd = 1.00;                // KDCONST
   double relax_time = 0.0, interval = 0.0;    // methods 3, 4: relaxTimeFlow, stationary dt
+  double rewet_limit = 0.005;                 // drying and rewetting ($DRYREW)
+  int count_down = 5;
   double hmin() const { return dry_limit / 2.0; }
   bool turb_alg() const { return turb % 10 == 2; }
   bool turb_aniso() const { return (turb / 10) % 10 == 1; }
@@ -291,6 +300,11 @@ struct Ctx {
   DBuf<double> exx, exy, eyy;                 // Elder: nodal tensor in global axes [np]
   bool turb_stale = true;                     // nodal vt due at the next Newton step
   DBuf<int> d_in_segs, d_inlet_nodes;
+  // drying and rewetting: node and element states (WD_*, EL_*), count down of dry nodes
+  DBuf<uint8_t> d_nwd, d_ewd;
+  DBuf<int> d_cdown;
+  bool first_dry = true;                      // no drying and rewetting call yet
+  int wet_sets = 0;                           // wet element sets set up since the mesh upload
   // ---- model elements for assembly ----
   int nme = 0;
   DBuf<int> me_nodes;                         // [nme*8]
@@ -342,6 +356,7 @@ void kt_collect(Ctx& c);
 
 // host model set-up (model.cpp)
 void host_mesh(HostModel& hm, const rmx_mesh_desc& d);
+void host_topology(HostModel& hm);
 void host_boundary_conditions(HostModel& hm);
 void host_numbering(HostModel& hm);
 void host_pattern(HostModel& hm);
@@ -368,6 +383,8 @@ void cycle_start(Ctx& c);
 void dev_newton_update(Ctx& c, rmx_step_stats& st);
 void dev_time_init(Ctx& c);                    // previous state := state, derivatives 0
 void dev_slip_flow(Ctx& c);                    // GRID::SetSlipFlow
+void dev_dry_rewet(Ctx& c);                    // GRID::DryRewet, method 2 (device pass)
+void dev_wetdry_commit(Ctx& c);                // previous dry / marsh flags := current
 void dev_time_step(Ctx& c, int first);         // EQS_UVS2D::Predict
 void dev_time_commit(Ctx& c);                  // previous state := state
 void dev_element_coefs(Ctx& c, int first, int count, double* K, double* F);

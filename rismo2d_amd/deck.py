@@ -746,9 +746,8 @@ def read_reference_deck(input_file: str) -> tuple[Deck, dict]:
     Supported: Manning roughness, emergent vegetation, constant / algebraic / anisotropic
     eddy viscosity, one QINLET or QTINLET line and one OUTLET line with a single water
     level, relaxation methods 0, 2, 3 and 4 (3 and 4: time-relaxed Newton steps, with
-    relaxTimeFlow from ``$TM_INTERVAL``), drying and rewetting not called inside the flow
-    cycle (``$DRYREW`` frequency not below ``$TM_MAXITER``).  Anything else raises
-    DeckFormatError."""
+    relaxTimeFlow from ``$TM_INTERVAL``), drying and rewetting by method 2 or none
+    (``$DRYREW``).  Anything else raises DeckFormatError."""
     base = os.path.dirname(os.path.abspath(input_file))
     files, other = {}, []
     proj = {k: list(v[0]) for k, v in _PROJECT_KEYS.items()}
@@ -829,13 +828,13 @@ def read_reference_deck(input_file: str) -> tuple[Deck, dict]:
     if relax[0] >= 3 and not run.get("dt"):
         raise DeckFormatError("$RELAX: relaxation methods 3 and 4 need $TM_INTERVAL")
     maxiter = run["maxiter"]                     # the flow cycle's $TM_MAXITER entry
-    if dry[0] and dry[1] < maxiter:
+    if dry[0] not in (0, 2) and dry[1] < maxiter:
         # EqsUVS2D.cpp:201: the flow cycle calls the drying and rewetting
-        raise DeckFormatError(f"$DRYREW: drying and rewetting every {dry[1]} Newton steps "
-                              f"(below $TM_MAXITER {maxiter}) is not supported on the device")
+        raise DeckFormatError(f"$DRYREW: drying and rewetting method {dry[0]} is not "
+                              "supported (0, 2)")
     run["params"] = dict(relax_method=relax[0], max_delta_uv=relax[3], max_delta_s=relax[4],
                          conv_uv=conv[0], conv_s=conv[1], dry_method=dry[0], dry_limit=dry[2],
-                         max_us=minmax[3])
+                         max_us=minmax[3], rewet_limit=dry[3], count_down=dry[5])
     if relax[0] >= 3:                            # time-relaxed Newton steps
         run["params"].update(relax_time=run["relax_time"], interval=run["dt"])
     run["dryrew"] = dict(method=dry[0], freq=dry[1], dry_limit=dry[2], rewet_limit=dry[3],

@@ -47,7 +47,8 @@ class Params(C.Structure):
                 ("max_delta_uv", C.c_double), ("max_delta_s", C.c_double),
                 ("conv_uv", C.c_double), ("conv_s", C.c_double), ("max_us", C.c_double),
                 ("turb", C.c_int), ("cm", C.c_double), ("cd", C.c_double),
-                ("relax_time", C.c_double), ("interval", C.c_double)]
+                ("relax_time", C.c_double), ("interval", C.c_double),
+                ("rewet_limit", C.c_double), ("count_down", C.c_int)]
 
 
 class KrylovOpts(C.Structure):
@@ -133,7 +134,8 @@ def lib():
             "rmx_n_model_elems": [_v, _i], "rmx_element_coefs": [_v, C.c_int, C.c_int, _d, _d],
             "rmx_turbulence": [_v, _d, _d],
             "rmx_time_step": [_v, C.c_double, C.c_double, C.c_int], "rmx_time_commit": [_v],
-            "rmx_slip_flow": [_v],
+            "rmx_slip_flow": [_v], "rmx_dry_rewet": [_v, _i, _i],
+            "rmx_get_wetdry": [_v, _i, _i], "rmx_get_relax_time": [_v, _d, _d, _d],
             "rmx_time_state": [_v, C.c_int, C.POINTER(_d)],
             "rmx_set_inflow": [_v, C.c_double, C.c_double, C.c_int],
             "rmx_hydrograph": [C.c_int, _d, _d, _d, C.c_double, _d, _d],
@@ -152,7 +154,7 @@ def lib():
             "rmx_get_sweep_info": [_v, C.c_int, C.POINTER(SweepInfo)],
             "rmx_solve": [_v, C.POINTER(KrylovOpts), C.POINTER(KrylovOut)],
             "rmx_get_matrix": [_v, C.c_int, _v, _d], "rmx_get_factors": [_v, C.c_int, _v],
-            "rmx_get_solution": [_v, _d], "rmx_set_rhs": [_v, _d],
+            "rmx_get_solution": [_v, _d], "rmx_set_rhs": [_v, _d], "rmx_set_solution": [_v, _d],
             "rmx_csr_upload": [_v, C.c_int, _i, _i, _d, _d],
             "rmx_newton_update": [_v, C.POINTER(StepStats)],
             "rmx_newton_step": [_v, C.POINTER(KrylovOpts), C.POINTER(StepStats)],
@@ -188,6 +190,7 @@ class Context:
         self.h = _v()
         _check(lib().rmx_create(device, C.byref(self.h)))
         self.np = 0
+        self.ne = 0
         self.n = 0
         self.nnz = 0
         self._keep = {}
@@ -231,6 +234,7 @@ class Context:
                 setattr(d, name, _p(k[key], _d))
         _check(lib().rmx_mesh_upload(self.h, C.byref(d)))
         self.np = deck.n_nodes
+        self.ne = deck.n_elems
         # the deck's turbulence model becomes active with the other parameters kept
         self.deck_turb = int(getattr(deck, "turb", 1))
         kw = dict(getattr(self, "_params", {}))
@@ -240,21 +244,23 @@ class Context:
 
     def set_params(self, relax_method=2, dry_method=2, dry_limit=1e-3, max_delta_uv=10.0,
                    max_delta_s=10.0, conv_uv=1e-7, conv_s=1e-7, max_us=100.0, turb=None,
-                   cm=0.09, cd=1.0, relax_time=0.0, interval=0.0):
+                   cm=0.09, cd=1.0, relax_time=0.0, interval=0.0, rewet_limit=0.005,
+                   count_down=5):
         """turb: $TM_TURBULENCE code; None takes the uploaded deck's.  relax_time
         (relaxTimeFlow) and interval (the stationary cycle's time step): relaxation methods
-        3 and 4."""
+        3 and 4.  rewet_limit, count_down: drying and rewetting ($DRYREW)."""
         if turb is None:
             turb = getattr(self, "deck_turb", 1)
         p = Params(relax_method, dry_method, dry_limit, max_delta_uv, max_delta_s, conv_uv,
-                   conv_s, max_us, turb, cm, cd, relax_time, interval)
+                   conv_s, max_us, turb, cm, cd, relax_time, interval, rewet_limit,
+                   count_down)
         _check(lib().rmx_set_params(self.h, C.byref(p)))
         # kept (and re-applied by upload_mesh) only once the library has accepted them
         self._params = dict(relax_method=relax_method, dry_method=dry_method,
                             dry_limit=dry_limit, max_delta_uv=max_delta_uv,
                             max_delta_s=max_delta_s, conv_uv=conv_uv, conv_s=conv_s,
                             max_us=max_us, cm=cm, cd=cd, relax_time=relax_time,
-                            interval=interval)
+                            interval=interval, rewet_limit=rewet_limit, count_down=count_down)
 
     TIME_KEYS = ("Uo", "Vo", "So", "dUdto", "dVdto", "dSdto", "dUdt", "dVdt", "dSdt")
 
@@ -268,6 +274,29 @@ class Context:
 
     def time_commit(self):
         _check(lib().rmx_time_commit(self.h))
+
+    def dry_rewet(self):
+        """One call of the drying and rewetting (method 2, MODEL::DoDryRewet): (elements
+        that got dry, that got wet).  A change of the wet elements renumbers the system."""
+        d, w = C.c_int(0), C.c_int(0)
+        _check(lib().rmx_dry_rewet(self.h, C.byref(d), C.byref(w)))
+        if d.value + w.value:
+            self.prepare()
+        return d.value, w.value
+
+    def wetdry(self):
+        """Per node the reference's dry (64) and marsh (256) bits, per element 1 dry,
+        2 marsh."""
+        nf = np.zeros(self.np, np.int32)
+        ef = np.zeros(self.ne, np.int32)
+        _check(lib().rmx_get_wetdry(self.h, _p(nf, _i), _p(ef, _i)))
+        return nf, ef
+
+    def relax_times(self):
+        """(dt_UV, dt_H, dt) of the running flow cycle."""
+        a, b, t = C.c_double(0), C.c_double(0), C.c_double(0)
+        _check(lib().rmx_get_relax_time(self.h, C.byref(a), C.byref(b), C.byref(t)))
+        return a.value, b.value, t.value
 
     def set_inflow(self, Q, S_in, hydrograph=True):
         """QINLET discharge and reference level for the following Newton steps (from a
@@ -473,6 +502,12 @@ class Context:
         X = np.zeros(self.n)
         _check(lib().rmx_get_solution(self.h, _p(X, _d)))
         return X
+
+    def set_solution(self, X):
+        """The correction the next newton_update applies."""
+        X = _c(X, np.float64)
+        assert len(X) == self.n
+        _check(lib().rmx_set_solution(self.h, _p(X, _d)))
 
     def set_rhs(self, B):
         B = _c(B, np.float64)

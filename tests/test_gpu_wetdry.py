@@ -1,0 +1,126 @@
+"""Drying and rewetting on the GPU (method 2, rmx_dry_rewet) against the reference program's
+records, through the C ABI.
+
+The decks: ``wetdry20x6`` (a bank rising out of the water: the far bank dries at the first
+call, the strip beside it becomes marsh, the wet/dry line a slip boundary) and
+``rewet20x6`` (the water rises from below the outlet level: elements of the bank get wet at
+a later call and the equations grow from 937 to 1115).  The runner's time loop calls the
+drying and rewetting on the reference's schedule (EqsUVS2D.cpp:200-222); every Newton step
+applies the reference's own correction (rmx_set_solution), so that the device's state
+follows the reference's trajectory and each drying pass sees the reference's input.
+
+Bars: dry and marsh flags, boundary kinds, numbering and CSR pattern bit-exact; nodal state,
+lowered bed, friction, inlet discharge and rotations within 1e-13 of their scale (the
+update's arithmetic is the reference's, the rotations come from normals summed in another
+order); the FP64 assembly within 1e-6 of the row maximum of the reference's FP32-stored
+matrix, the right-hand side within 1e-12 of its maximum.
+"""
+import numpy as np
+import pytest
+
+from helpers import golden_deck, load_golden
+from rismo2d_amd import rmx
+from rismo2d_amd import run as rr
+
+pytestmark = pytest.mark.gpu
+
+
+@pytest.fixture(scope="module")
+def ctx():
+    c = rmx.Context(0)
+    yield c
+    c.close()
+
+
+def _rel(a, b, scale):
+    return float(np.abs(np.asarray(a) - np.asarray(b)).max() / max(scale, 1e-300))
+
+
+@pytest.mark.parametrize("name", ["wetdry20x6", "rewet20x6"])
+def test_device_drying_follows_reference_records(ctx, name):
+    print(f"[gpu] drying and rewetting on the device, {name}", flush=True)
+    g = load_golden(name)
+    method, freq, dry_limit, rewet_limit, _passes, count_down = g["dryrew"]
+    ctx.upload_mesh(golden_deck(g))
+    ctx.set_params(dry_method=int(method), dry_limit=float(dry_limit),
+                   rewet_limit=float(rewet_limit), count_down=int(count_down))
+    calls = []
+
+    class Cycle:                       # the device behind the runner's time loop
+        k = 0
+
+        def time_step(self, dt, theta, first):
+            ctx.time_step(dt, theta, first)
+
+        def slip_flow(self):
+            ctx.slip_flow()
+
+        def dry_rewet(self):
+            calls.append((self.k,) + ctx.dry_rewet())
+            return calls[-1][1:]
+
+        def relax_times(self):
+            return ctx.relax_times()
+
+    cyc = Cycle()
+
+    def newton():
+        k = cyc.k
+        p = f"s{k}_"
+        n = ctx.prepare()
+        nf, ef = ctx.wetdry()
+        np.testing.assert_array_equal(nf, g[p + "flag"].astype(np.int64) & (64 | 256),
+                                      err_msg=f"dry/marsh flags at step {k}")
+        eq = ctx.numbering()
+        np.testing.assert_array_equal(eq, np.stack([g[p + "eqU"], g[p + "eqV"], g[p + "eqS"]], 1),
+                                      err_msg=f"numbering at step {k}")
+        assert n == len(g[p + "xsol"])
+        rp, cols = ctx.pattern()
+        np.testing.assert_array_equal(np.diff(rp), g[p + "width"])
+        np.testing.assert_array_equal(cols, g[p + "cols"])
+        U, V, S, z = ctx.state()
+        us = np.abs(g[p + "U"]).max() + np.abs(g[p + "V"]).max()
+        hs = np.abs(g[p + "S"]).max()
+        errs = dict(U=_rel(U, g[p + "U"], us), V=_rel(V, g[p + "V"], us),
+                    S=_rel(S, g[p + "S"], hs), z=_rel(z, g[p + "z"], hs))
+        nod = ctx.nodal()
+        cf, bcU, rotm = nod["cf"], nod["bcU"], nod["rotm"]
+        np.testing.assert_array_equal(nod["kind"], g[p + "kind"].astype(np.int32))
+        errs["cf"] = _rel(cf, g[p + "cf"], np.abs(g[p + "cf"]).max())
+        errs["bcU"] = _rel(bcU, g[p + "bcU"], max(np.abs(g[p + "bcU"]).max(), 1e-12))
+        ref_rot = np.stack([g[p + "r00"], g[p + "r01"], g[p + "r10"], g[p + "r11"]], 1)
+        errs["rot"] = _rel(rotm, ref_rot, 1.0)
+        ctx.assemble(rmx.F64)
+        A, B = ctx.matrix(rmx.F64)
+        aref = g[p + "aval"]
+        rows = np.repeat(np.arange(n), np.diff(rp))
+        rmax = np.zeros(n)
+        np.maximum.at(rmax, rows, np.abs(aref))
+        errs["A"] = float((np.abs(A - aref) / rmax[rows]).max())
+        errs["B"] = _rel(B, g[p + "bvec"], np.abs(g[p + "bvec"]).max())
+        print(f"    step {k}: {n} equations, dry {int((ef == 1).sum())}, marsh "
+              f"{int((ef == 2).sum())} elements; " +
+              ", ".join(f"{key} {v:.1e}" for key, v in errs.items()), flush=True)
+        for key in ("U", "V", "S", "z", "cf", "bcU", "rot"):
+            assert errs[key] <= 1e-13, (k, key, errs[key])
+        assert errs["A"] <= 1e-6 and errs["B"] <= 1e-12, (k, errs["A"], errs["B"])
+        ctx.set_solution(g[p + "xsol"])
+        st = ctx.newton_update()
+        cyc.k += 1
+        return st
+
+    maxit = int(g["maxiter"]) if "maxiter" in g else 12
+    counts = rr.time_loop(cyc, 1, 0.0, 1.0, maxit, newton, stationary=True, dry_freq=int(freq))
+    print(f"    calls (step, dried, wetted): {calls}")
+    assert counts == [int(g["nsteps"])]
+    U, V, S, z = ctx.state()
+    us = np.abs(g["final_U"]).max() + np.abs(g["final_V"]).max()
+    assert _rel(U, g["final_U"], us) <= 1e-13 and _rel(V, g["final_V"], us) <= 1e-13
+    assert _rel(S, g["final_S"], np.abs(g["final_S"]).max()) <= 1e-13
+    nf, ef = ctx.wetdry()
+    np.testing.assert_array_equal(nf, g["final_flag"].astype(np.int64) & (64 | 256))
+    assert calls[0][0] == 0 and calls[0][1] > 0 and (ef == 2).any()
+    if name == "rewet20x6":
+        assert any(k > 0 and w > 0 for k, _, w in calls)
+    else:
+        assert (ef == 1).sum() == calls[0][1]

@@ -320,6 +320,56 @@ def test_unconverged_cycle_stops_at_the_newton_limit():
     _check_state(om, g, "final_")
 
 
+@pytest.mark.parametrize("name", ["wetdry20x6", "rewet20x6"])
+def test_runner_time_loop_dries_and_rewets_as_the_reference(name):
+    """The runner's time loop calls the drying and rewetting where the reference's flow
+    cycle does (EqsUVS2D.cpp:200-222: before the first Newton step, then every dryRewFreq
+    steps or after a converged step, a wetting call holding the convergence): driving the
+    oracle it takes the reference's Newton steps and reproduces every correction and the
+    final state bit for bit."""
+    from rismo2d_amd import run as rr
+    g = load_golden(name)
+    method, freq, dry_limit, rewet_limit, _passes, count_down = g["dryrew"]
+    om = po.OracleModel(golden_deck(g))
+    om.set_params(dry_method=int(method), dry_limit=float(dry_limit))
+    om.set_dryrew(float(rewet_limit), int(count_down))
+    calls = []
+
+    class Cycle:                       # the oracle behind the runner's time loop
+        k = 0
+
+        def time_step(self, dt, theta, first):
+            om.time_step(dt, theta, first)
+
+        def slip_flow(self):
+            om.slip_flow()
+
+        def dry_rewet(self):
+            calls.append((self.k,) + om.dry_rewet())
+            return calls[-1][1:]
+
+        def relax_times(self):
+            return 0.0, 0.0, 0.0           # relaxation method 2: the time step itself
+
+    cyc = Cycle()
+
+    def newton():
+        om.prepare()
+        X, st = om.newton_step(prec=32, solver=1, max_iter=1000, eps=1e-10)
+        np.testing.assert_array_equal(X, g[f"s{cyc.k}_xsol"], err_msg=f"step {cyc.k}")
+        cyc.k += 1
+        return st
+
+    maxit = int(g["maxiter"]) if "maxiter" in g else 12
+    counts = rr.time_loop(cyc, 1, 0.0, 1.0, maxit, newton, stationary=True,
+                          dry_freq=int(freq))
+    assert counts == [int(g["nsteps"])]
+    assert calls[0][0] == 0 and len(calls) > 1
+    _check_state(om, g, "final_")
+    nf, _ = om.wetdry()
+    np.testing.assert_array_equal(nf, g["final_flag"].astype(np.int64) & (64 | 256))
+
+
 # ---------------------------------------------------------------------------------------
 # closed forms and invariants
 

@@ -226,7 +226,8 @@ def test_initial_file_fixed_columns_and_defaults(tmp_path):
     (("input.dat", "$ASC_INITFILE", "$BIN_INITFILE"), "only ASCII"),
     (("input.dat", "$RELAX 2", "$RELAX 1"), "relaxation method 1"),
     (("input.dat", "$RELAX 2", "$RELAX 5"), "relaxation method 5"),
-    (("input.dat", "$RELAX 2\n", "$RELAX 2\n$DRYREW 2 5\n"), "drying and rewetting every 5"),
+    (("input.dat", "$RELAX 2\n", "$RELAX 2\n$DRYREW 1 5\n"), "drying and rewetting method 1"),
+    (("input.dat", "$RELAX 2\n", "$RELAX 2\n$DRYREW 3 5\n"), "drying and rewetting method 3"),
     # the time-step file: one boundary set with the flow cycle alone, and its Newton limit
     (("tm.dat", "$TM_CYCLE 1\n", "$TM_CYCLE 90 1\n"), "only the flow cycle"),
     (("tm.dat", "$TM_CYCLE 1\n", ""), "no \\$TM_CYCLE"),
@@ -236,8 +237,6 @@ def test_initial_file_fixed_columns_and_defaults(tmp_path):
     (("tm.dat", "$TM_MAXITER 6\n", "$TM_MAXITER 6\n$TM_NODE QT 3 1.0 2.0\n"), "TM_NODE is not"),
     (("tm.dat", "$TM_MAXITER 6\n", "$TM_MAXITER 6\n$TM_BOUND_LINE 4 8\n"), "TM_BOUND_LINE is not"),
     (("tm.dat", "$TM_MAXITER 6\n", "$TM_MAXITER 6\n$TM_DISPERSION 1\n"), "DISPERSION 1"),
-    # the drying check takes the flow cycle's limit: 20 Newton steps, drying every 10
-    (("tm.dat", "$TM_MAXITER 6\n", "$TM_MAXITER 20 1\n"), "drying and rewetting every 10"),
 ])
 def test_reader_rejects_what_the_path_does_not_support(tmp_path, edit, msg):
     init = "# 0.0 x 40000 U V S\n9 0\n" + "".join(f"{i} 0 0 1.0\n" for i in range(1, 10))
@@ -247,6 +246,25 @@ def test_reader_rejects_what_the_path_does_not_support(tmp_path, edit, msg):
     files[name] = files[name].replace(old, new)
     with pytest.raises(dk.DeckFormatError, match=msg):
         dk.read_reference_deck(_write(tmp_path, files))
+
+
+def test_drying_method_check_takes_the_flow_cycles_newton_limit(tmp_path):
+    """Drying and rewetting by a method other than 2 is refused only where the flow cycle
+    would call it (its frequency below the flow cycle's $TM_MAXITER entry, EqsUVS2D.cpp:201);
+    method 2 inside the cycle is read with its parameters."""
+    init = "# 0.0 x 40000 U V S\n9 0\n" + "".join(f"{i} 0 0 1.0\n" for i in range(1, 10))
+    for dry, maxiter, ok in (("3 10", "6", True), ("3 10", "20 1", False), ("2 3", "20 1", True)):
+        files = _files(init)
+        files["input.dat"] = files["input.dat"].replace("$RELAX 2\n", f"$RELAX 2\n$DRYREW {dry}\n")
+        files["tm.dat"] = files["tm.dat"].replace("$TM_MAXITER 6\n", f"$TM_MAXITER {maxiter}\n")
+        path = _write(tmp_path, files)
+        if ok:
+            _, run = dk.read_reference_deck(path)
+            assert run["dryrew"]["method"] == int(dry[0])
+            assert run["params"]["dry_method"] == int(dry[0])
+        else:
+            with pytest.raises(dk.DeckFormatError, match="method 3"):
+                dk.read_reference_deck(path)
 
 
 def test_project_keywords_as_the_reference_scans_them(tmp_path):
@@ -262,7 +280,7 @@ def test_project_keywords_as_the_reference_scans_them(tmp_path):
     _, run = dk.read_reference_deck(_write(tmp_path, files))
     assert run["params"] == dict(relax_method=2, max_delta_uv=5.0, max_delta_s=0.1,
                                  conv_uv=1e-6, conv_s=1e-4, dry_method=0, dry_limit=0.01,
-                                 max_us=50.0)
+                                 max_us=50.0, rewet_limit=0.01, count_down=5)
     assert run["dryrew"] == dict(method=0, freq=3, dry_limit=0.01, rewet_limit=0.01,
                                  rewet_passes=100, count_down=5)
     # $MAX_Us sets maxUs as $MINMAX's fourth field does; the later line wins
@@ -313,7 +331,7 @@ def test_golden_deck_project_keywords_are_the_solve_defaults(tmp_path):
     _, run = dk.read_reference_deck(str(tmp_path / "input.dat"))
     assert run["params"] == dict(relax_method=2, max_delta_uv=10.0, max_delta_s=10.0,
                                  conv_uv=1e-7, conv_s=1e-7, dry_method=2, dry_limit=0.001,
-                                 max_us=100.0)
+                                 max_us=100.0, rewet_limit=0.005, count_down=5)
 
 
 @pytest.mark.parametrize("exact", [True, False])
