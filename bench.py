@@ -53,7 +53,8 @@ def parse():
     ap.add_argument("--n-across", type=int, default=CROSS)
     ap.add_argument("--strips", type=int, default=0)
     ap.add_argument("--segments", type=int, default=0, help="segments per strip (0: blocks of about 560 rows)")
-    ap.add_argument("--coarse", type=int, default=24, help="coarse segments per strip (0: none)")
+    ap.add_argument("--coarse", type=int, default=-1,
+                    help="coarse segments per strip (-1: automatic, 0: none)")
     ap.add_argument("--max-iter", type=int, default=1000, help="Krylov iterations per refinement sweep")
     ap.add_argument("--mkyrl", type=int, default=15, help="FGMRES restart length")
     ap.add_argument("--eps", type=float, default=1e-2, help="inner relative tolerance per sweep")
@@ -183,7 +184,7 @@ def kernels_only(c, n, a):
 
 
 def fast_opts(a, R, refine_tol=1e-8):
-    pk = R.PREC_SILU0C if a.coarse > 0 else R.PREC_SILU0
+    pk = R.PREC_SILU0C if a.coarse != 0 else R.PREC_SILU0
     return dict(method=R.FGMRES, prec=R.F32, precond=pk, max_iter=a.max_iter, mkyrl=a.mkyrl,
                 eps=a.eps, refine=True, refine_tol=refine_tol, refine_max=a.refine_max)
 
@@ -303,6 +304,7 @@ def main():
     T = sum(times) / len(times)
     last = stats[-1]
     its = max(last.solve.iters, 1)
+    n_aggregates = c.aggregates()[1] if a.coarse != 0 else 0
     t_solve = sum(s.t_solve for s in stats) / len(stats)
 
     # kernel table: per class, launches and time per step, share of the step, algorithmic
@@ -383,8 +385,8 @@ def main():
                                f"{d.n_elems} Taylor-Hood triangles, {n} equations, {c.nnz} nnz; "
                                "Newton step 1 from the initial state",
                    "solver": f"FGMRES({a.mkyrl}), FP32 matrix, coarse correction "
-                             f"({a.coarse} segments/strip x U,V,H aggregates, block cyclic "
-                             f"reduction on the device) + strip ILU(0) ({a.strips or '2 per CU'} "
+                             f"({n_aggregates} U,V,H aggregates over the strips' coarse "
+                             f"segments, block cyclic reduction on the device) + strip ILU(0) ({a.strips or '2 per CU'} "
                              f"strips x {a.segments or 'about 560-row'} segments, dataflow "
                              f"factorisation), FP64 "
                              f"iterative refinement to 1e-8 (inner tolerance {a.eps:g})",

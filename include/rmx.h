@@ -253,8 +253,8 @@ rmx_status rmx_get_blocks(rmx_ctx* ctx, int* block /* [neq] */);
  * per (strip, coarse segment, variable U/V/H), coarse segments cut like the sweep
  * segments; P is piecewise constant on aggregates and R = P^T.  Apply:
  * z1 = P (R A P)^-1 R r, z = z1 + M^-1 (r - A z1) with M the strip ILU(0).
- * rmx_set_coarse chooses the coarse segments per strip (default 24; 0: no coarse space; needs a mesh,
- * i.e. not after rmx_csr_upload); rmx_coarse_factor sums R A P from the FP32 or FP64
+ * rmx_set_coarse chooses the coarse segments per strip (-1, the default: one per 256 rows of
+ * a strip, at most 24; 0: no coarse space; needs a mesh, i.e. not after rmx_csr_upload); rmx_coarse_factor sums R A P from the FP32 or FP64
  * matrix and factors it; rmx_get_aggregates writes the aggregate of every equation and
  * returns the number of aggregates in *nc. */
 rmx_status rmx_set_coarse(rmx_ctx* ctx, int segments);

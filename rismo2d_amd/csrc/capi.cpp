@@ -668,7 +668,8 @@ rmx_status rmx_set_coarse(rmx_ctx* p, int segments)
 {
   API_BEGIN
   Ctx& c = *need(p);
-  if (segments < 0 || segments > 1024) fail(RMX_ERR_ARG, "coarse segments must be 0..1024");
+  if (segments < -1 || segments > 1024)
+    fail(RMX_ERR_ARG, "coarse segments must be -1 (automatic) or 0..1024");
   c.coarse_segments = segments;
   if (c.prepared) setup_ilu_schedule(c);
   API_END

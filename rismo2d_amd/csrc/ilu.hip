@@ -2040,7 +2040,8 @@ void setup_ilu_schedule(Ctx& c)
   ilu_setup(c.silu, c.n, rp, cl, nblk, c.csr_only ? nullptr : &group, c.stream,
             c.csr_only ? nullptr : &c.hm, strip_segments(c, nblk), STAGE_ROWS);
   c.coarse.ready = c.coarse.fac = false;
-  if (!c.csr_only && c.coarse_segments > 0) coarse_setup(c, rp, cl, group);
+  const int cseg = coarse_segments_of(c);
+  if (!c.csr_only && cseg > 0) coarse_setup(c, rp, cl, group, cseg);
 }
 
 Ilu& ilu_of(Ctx& c, int kind)

@@ -333,7 +333,8 @@ struct Ctx {
   int segments = 0;                           // cross-strip cuts of the strip sweeps (0: auto)
   Coarse coarse;
   Mpnn mpnn;
-  int coarse_segments = 24;                   // cross-strip cuts of the coarse aggregates
+  int coarse_segments = -1;                   // cross-strip cuts of the coarse aggregates
+                                              // (-1: automatic, 0: no coarse space)
   // Krylov work vectors
   DBuf<double> w[10];
   DBuf<double> red;                           // reduction scratch
@@ -403,7 +404,8 @@ void upload_shapes(hipStream_t s);
 std::vector<int> cut_segments(const std::vector<int>& bstart, const std::vector<int>& group,
                               const HostModel& hm, int nseg);
 void coarse_setup(Ctx& c, const std::vector<int>& rp, const std::vector<int>& cl,
-                  const std::vector<int>& group);
+                  const std::vector<int>& group, int nseg);
+int coarse_segments_of(const Ctx& c);          // coarse segments per strip in effect
 void coarse_factor(Ctx& c, int prec);
 void coarse_apply(Ctx& c, int prec, const double* r, double* z);   // device pointers
 void dev_precond(Ctx& c, int prec, int kind, const double* in, double* out);

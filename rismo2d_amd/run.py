@@ -90,9 +90,6 @@ def run_reference_deck(input_file: str, ctx: rmx.Context | None = None, solve: d
         c.set_params(**run["params"])
         if solve is None:
             solve = FAST
-        if solve.get("precond") == rmx.PREC_SILU0C:
-            c.prepare()
-            c.set_coarse(24)
         unsteady = run["stationary"] == 0
         dw = run["dryrew"]
         counts = time_loop(c, run["steps"] if unsteady else 1, run["dt"], run["theta"],
