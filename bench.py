@@ -1,9 +1,14 @@
 """Benchmark: time per Newton step to FP64 residual 1e-8 on the 2M-triangle meandering
 channel (one MI355X).
 
+The deck's floodplains rise away from the main channel, so that their outer parts lie above
+the initial water level: drying and rewetting (method 2, on the device) takes them out of
+the system before the first Newton step, as the flow cycle's call at its first step does
+(the set-up of the wet model after that call is reported, not timed).
 One step = one Newton step of the UVS2D system from the deck's initial state (the state is
 uploaded again before every step, outside the timed region, so every step solves the same
-system): nodal friction and inlet set-up, FP64 assembly, L2 scaling, FP32 copy of the
+system): the drying and rewetting call (no element changes: the device pass and the flags'
+download), nodal friction and inlet set-up, FP64 assembly, L2 scaling, FP32 copy of the
 matrix, ILU(0) factorisation, coarse-space factorisation, the Krylov solve in FP32 inside
 FP64 iterative refinement to ||b - J x|| / ||b|| <= 1e-8 (FP64 J), Newton update.  The
 inputs (mesh, pattern, schedules) are resident on the device before the timed region.
@@ -61,6 +66,9 @@ def parse():
     ap.add_argument("--refine-max", type=int, default=8)
     ap.add_argument("--plain-iters", type=int, default=10,
                     help="BiCGStab iterations of the bounded plain FP64 run on the 2M system")
+    ap.add_argument("--plain-rise", type=float, default=0.012,
+                    help="floodplain rise (m/m) away from the main channel; > 0: the outer "
+                         "floodplains are dry and drying and rewetting is active")
     ap.add_argument("--cmp-along", type=int, default=80,
                     help="cross-sections of the comparison reach (0: no comparison)")
     ap.add_argument("--no-cpu", action="store_true")
@@ -75,23 +83,42 @@ def log(msg):
     print(f"[bench] {msg}", file=sys.stderr, flush=True)
 
 
-def reach_deck(n_along, n_across=CROSS):
+def reach_deck(n_along, plain_rise, n_across=CROSS):
     """A reach of the benchmark's meander generator at the benchmark's resolution."""
     from rismo2d_amd import deck as dk
-    return dk.meander(n_along=n_along, n_across=n_across, reach=10000.0 * n_along / 3200)
+    return dk.meander(n_along=n_along, n_across=n_across, reach=10000.0 * n_along / 3200,
+                      plain_rise=plain_rise)
 
 
-def cpu_baseline(na):
+def dry_setup(c, rise):
+    """The flow cycle's first drying and rewetting call and the set-up of the wet model it
+    causes; returns its record (None without a floodplain rise)."""
+    if rise <= 0.0:
+        return None
+    n_all = c.n
+    t0 = time.perf_counter()
+    dried, wetted = c.dry_rewet()
+    t = time.perf_counter() - t0
+    _, ef = c.wetdry()
+    rec = {"plain_rise": rise, "dried_elements": int(dried), "wetted_elements": int(wetted),
+           "dry_elements": int((ef == 1).sum()), "marsh_elements": int((ef == 2).sum()),
+           "equations_all_wet": n_all, "equations": c.n, "first_call_and_set_up_s": t}
+    log(f"drying: {dried} elements dry, {rec['marsh_elements']} marsh, {n_all} -> {c.n} "
+        f"equations; call and set-up of the wet model {t:.2f}s")
+    return rec
+
+
+def cpu_baseline(na, rise):
     """The reference program itself when its build (oracle/_ref/rismo_harness, compiled from
     the reference's sources) is present, else the oracle; both on one core, on the
     comparison reach."""
     exe = os.path.join(HERE, "oracle", "_ref", "rismo_harness")
     if os.path.exists(exe):
         try:
-            return cpu_baseline_reference(exe, na)
+            return cpu_baseline_reference(exe, na, rise)
         except Exception as e:      # noqa: BLE001 - a failed baseline falls back, never stops
             log(f"reference baseline failed ({e}); timing the oracle instead")
-    return cpu_baseline_port(na)
+    return cpu_baseline_port(na, rise)
 
 
 def _cpu_seconds(field):
@@ -100,18 +127,27 @@ def _cpu_seconds(field):
     return 3600 * h + 60 * m + sec + cc / 100.0
 
 
-def cpu_baseline_reference(exe, na):
-    """One Newton step of the reference program on the comparison reach: its own
-    numbering, assembly, ILU(0) and BiCGStab to 1e-8 (at most 3000 iterations), written as
-    its input files; the value is the CPU time its report gives for the time step (the
-    report's last CPU line)."""
+CPU_ITERS = 400            # BiCGStab iterations of the bounded CPU sample
+
+
+def cpu_baseline_reference(exe, na, rise):
+    """One Newton step of the reference program on the comparison reach: its own drying
+    and rewetting at the step's start (with a floodplain rise) and set-up of the wet model,
+    numbering, assembly, ILU(0) and BiCGStab to 1e-8 bounded at CPU_ITERS iterations (its
+    FP32-stored matrix stalls above 1e-8 on the dried reach), written as its input files;
+    the value is the CPU time its report gives for the time step (the report's last CPU
+    line), with the iterations run and the accuracy reached."""
     import re
     import subprocess
     import tempfile
     from rismo2d_amd import deck as dk
-    d = reach_deck(na)
+    d = reach_deck(na, rise)
+    # drying frequency 0: the flow cycle of one Newton step calls it before that step
+    # (EqsUVS2D.cpp:201-206); 1000: never
+    dry = "2 0 0.001 0.005 100 5" if rise > 0 else "2 1000 0.001 0.005 100 5"
     with tempfile.TemporaryDirectory() as wd:
-        dk.write_reference_deck(d, wd, solver="1 6 1 0 60 3000 1.0e-8", maxiter=1)
+        dk.write_reference_deck(d, wd, solver=f"1 6 1 0 60 {CPU_ITERS} 1.0e-8", maxiter=1,
+                                dryrew=dry)
         subprocess.run([exe, "input.dat"], cwd=wd, check=True, timeout=300,
                        stdout=subprocess.DEVNULL, stderr=subprocess.DEVNULL)
         rep = open(os.path.join(wd, "report.txt")).read()
@@ -124,31 +160,38 @@ def cpu_baseline_reference(exe, na):
     return dict(value=_cpu_seconds(cpu[-1]), unit="s per Newton step", cores=1,
                 kind="reference", iters=its, accuracy=acc,
                 sample=(f"meander {na}x{CROSS} reach of the benchmark's generator "
-                        f"({d.n_elems} triangles), one Newton step of the reference program "
-                        f"built from its sources: numbering, assembly, ILU(0), BiCGStab "
-                        f"{its} it to {acc:.1e}; CPU time of the step from its report, 1 core "
-                        f"(the reference is single-threaded)"))
+                        f"({d.n_elems} triangles" +
+                        (f", floodplains rising {rise:g} m/m" if rise > 0 else "") +
+                        "), one Newton step of the reference program built from its sources: " +
+                        ("drying and rewetting and the wet model's set-up, " if rise > 0 else "") +
+                        f"numbering, assembly, ILU(0), BiCGStab {its} it (bounded at "
+                        f"{CPU_ITERS}) to accuracy {acc:.1e}; CPU time of the step from its "
+                        "report, 1 core (the reference is single-threaded)"))
 
 
-def cpu_baseline_port(na):
+def cpu_baseline_port(na, rise):
     """The oracle (C restatement of the reference's path, one core) on the comparison
-    reach: one Newton step, FP64 assembly + ILU(0) + BiCGStab to 1e-8."""
+    reach: one Newton step, drying and rewetting (with a floodplain rise), FP64 assembly +
+    ILU(0) + BiCGStab to 1e-8."""
     from oracle import pyoracle as po
-    d = reach_deck(na)
+    d = reach_deck(na, rise)
     m = po.OracleModel(d)
     m.set_params()
     t0 = time.perf_counter()
+    if rise > 0:
+        m.dry_rewet()
     m.prepare()
     rp, cols = m.pattern()
     A, B = m.assemble(64)
     po.scale(rp, A, B)
     L, _ = po.ilu0_factor(rp, cols, A)
-    X, conv, its, acc = po.bicgstab(rp, cols, A, L, B, max_iter=3000, eps=1e-8)
+    X, conv, its, acc = po.bicgstab(rp, cols, A, L, B, max_iter=CPU_ITERS, eps=1e-8)
     t = time.perf_counter() - t0
     return dict(value=t, unit="s per Newton step", cores=1, kind="port", iters=its, accuracy=acc,
                 sample=(f"meander {na}x{CROSS} reach of the benchmark's generator "
                         f"({d.n_elems} triangles), one Newton step with the oracle: FP64 "
-                        f"assembly, ILU(0), BiCGStab {its} it to {acc:.1e}, 1 core"))
+                        f"assembly, ILU(0), BiCGStab {its} it (bounded at {CPU_ITERS}) to "
+                        f"{acc:.1e}, 1 core"))
 
 
 def kernels_only(c, n, a):
@@ -199,20 +242,27 @@ def plain_opts(R, max_iter):
 def comparison(R, a):
     """Both paths on the comparison reach, each timed over one Newton step from the initial
     state to a true FP64 residual of 1e-8 (the step's factorisations included)."""
-    d = reach_deck(a.cmp_along)
+    d = reach_deck(a.cmp_along, a.plain_rise)
     c = R.Context(0)
     c.upload_mesh(d)
     c.set_params()
-    n = c.prepare()
     c.set_coarse(a.coarse)
     c.set_strips(a.strips, a.segments)
+    c.prepare()
+    wd = dry_setup(c, a.plain_rise)
+    n = c.n
     out = {"workload": f"meander {a.cmp_along}x{CROSS} reach of the same generator, "
-                       f"{d.n_elems} triangles, {n} equations, {c.nnz} nnz"}
+                       f"{d.n_elems} triangles, {n} equations, {c.nnz} nnz",
+           "drying": wd}
     for name, opts in (("fast", fast_opts(a, R)), ("plain_fp64", plain_opts(R, 6000))):
         c.state_upload(d.U0, d.V0, d.S0)
+        if wd:
+            c.dry_rewet()
         c.newton_step(**opts)                    # warm-up (schedules, first-touch)
         c.state_upload(d.U0, d.V0, d.S0)
         t0 = time.perf_counter()
+        if wd:
+            assert c.dry_rewet() == (0, 0)
         st = c.newton_step(**opts)
         t = time.perf_counter() - t0
         its = max(st.solve.iters, 1)
@@ -254,7 +304,7 @@ def main():
     cpu = None
     if not a.no_cpu and not a.kernels_only and a.cmp_along > 0:
         try:
-            cpu = cpu_baseline(a.cmp_along)
+            cpu = cpu_baseline(a.cmp_along, a.plain_rise)
             log(f"cpu baseline: {cpu['value']:.1f}s ({cpu['sample']})")
         except Exception as e:       # the oracle library is test infrastructure: report its absence
             log(f"cpu baseline unavailable: {e}")
@@ -262,17 +312,20 @@ def main():
     from rismo2d_amd import rmx as R
 
     t = time.perf_counter()
-    d = dk.meander(n_along=a.n_along, n_across=a.n_across, reach=10000.0 * a.n_along / 3200)
+    d = dk.meander(n_along=a.n_along, n_across=a.n_across, reach=10000.0 * a.n_along / 3200,
+                   plain_rise=a.plain_rise)
     log(f"deck {a.n_along}x{a.n_across}: {d.n_elems} triangles, {d.n_nodes} nodes "
         f"({time.perf_counter() - t:.1f}s)")
     c = R.Context(0)
     c.upload_mesh(d)
     c.set_params()
-    t = time.perf_counter()
-    n = c.prepare()
     c.set_coarse(a.coarse)
     c.set_strips(a.strips, a.segments)
+    t = time.perf_counter()
+    n = c.prepare()
     log(f"prepare: {n} equations, {c.nnz} non-zeros ({time.perf_counter() - t:.1f}s)")
+    wet = dry_setup(c, a.plain_rise)
+    n = c.n
 
     if a.kernels_only:
         kernels_only(c, n, a)
@@ -284,16 +337,25 @@ def main():
         # every step solves the same system
         c.state_upload(d.U0, d.V0, d.S0)
 
+    def dry_call():
+        # the flow cycle's call before its first Newton step: no element changes state on the
+        # same initial state, so it is the device pass and the flags' download
+        if wet:
+            assert c.dry_rewet() == (0, 0)
+
     for w in range(a.warmup):
         reset()
+        dry_call()
         st = c.newton_step(**opts)
         log(f"warmup {w}: {st.solve.iters} it, residual {st.solve.true_resid:.2e}, "
             f"{st.t_prepare + st.t_assemble + st.t_factor + st.t_solve + st.t_update:.2f}s")
-    times, stats = [], []
+    times, stats, t_dry = [], [], []
     c.kernel_stats(on=True)                     # HIP events around the kernel classes
     for k in range(a.steps):
         reset()
         t0 = time.perf_counter()
+        dry_call()
+        t_dry.append(time.perf_counter() - t0)
         st = c.newton_step(**opts)
         times.append(time.perf_counter() - t0)
         stats.append(st)
@@ -346,6 +408,7 @@ def main():
     # plain FP64 path on the 2M system (state reset, friction and inlet recomputed):
     # reference-order ILU(0), BiCGStab, bounded
     c.state_upload(d.U0, d.V0, d.S0)
+    dry_call()
     c.prepare()
     c.assemble(R.F64)
     c.scale_l2()
@@ -366,6 +429,7 @@ def main():
             cpu["dof_iters_per_s"] = cmp["n_eq"] * cpu["iters"] / cpu["value"]
             cmp["reference_cpu"] = {"s_per_newton_step": cpu["value"], "cores": 1,
                                     "krylov_iters": cpu["iters"],
+                                    "accuracy_reached": cpu.get("accuracy"),
                                     "dof_iters_per_s": cpu["dof_iters_per_s"]}
 
     line = {
@@ -383,7 +447,12 @@ def main():
         "data": "synthetic",
         "config": {"workload": f"meander channel {a.n_along}x{a.n_across} cells, "
                                f"{d.n_elems} Taylor-Hood triangles, {n} equations, {c.nnz} nnz; "
-                               "Newton step 1 from the initial state",
+                               "Newton step 1 from the initial state" +
+                               (f"; floodplains rising {a.plain_rise:g} m/m, "
+                                f"{wet['dry_elements']} elements dry and "
+                                f"{wet['marsh_elements']} marsh after drying and rewetting "
+                                f"(method 2, {wet['equations_all_wet']} equations when all wet)"
+                                if wet else ""),
                    "solver": f"FGMRES({a.mkyrl}), FP32 matrix, coarse correction "
                              f"({n_aggregates} U,V,H aggregates over the strips' coarse "
                              f"segments, block cyclic reduction on the device) + strip ILU(0) ({a.strips or '2 per CU'} "
@@ -404,6 +473,7 @@ def main():
                      "peak": HBM_PEAK / 1e9, "unit": "GB/s", "frac": D["frac_hbm_peak"],
                      "traffic": traffic, "algorithmic_bytes": byt, "avg_launch_ms": dur * 1e3,
                      "share_of_step": D["share_of_step"]},
+        "drying": wet and dict(wet, dry_call_s=sum(t_dry) / len(t_dry)),
         "comparison": cmp,
         "plain_fp64_2M": {"factor_s": t_fac64, "s_per_bicgstab_iter": plain_it,
                           "iters_run": out.iters, "residual_reached": out.accuracy,

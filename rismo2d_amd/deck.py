@@ -211,13 +211,17 @@ def meander(n_along: int = 3200, n_across: int = 320, reach: float = 10000.0,
             slope: float = 2.0e-4, noise_sigma: float = 0.5, noise_corr: float = 50.0,
             jitter: float = 0.2, seed: int = 0, n_range=(0.025, 0.045), vt: float = 0.1,
             Q: float = 600.0, S_out: float = 4.2, U_init: float = 0.4,
-            name: str | None = None) -> Deck:
+            plain_rise: float = 0.0, name: str | None = None) -> Deck:
     """Meandering channel with floodplains on a curvilinear, jittered Taylor-Hood mesh.
 
     Centreline ``y_c = amplitude*sin(2*pi*s/wavelength)`` over ``reach`` metres.  The
     cross-section is a trapezoidal main channel (``main_width`` wide at the bed,
     ``main_depth`` deep, banks of slope ``bank_slope``) cut into floodplains that span
-    ``width`` in total.  The lattice follows the centreline and interior corner nodes
+    ``width`` in total, rising by ``plain_rise`` m per m away from the main channel (with
+    a rise, the outer floodplains lie above the initial water level and dry, and the initial
+    velocity ``U_init`` is scaled by the depth over ``main_depth``, at most 1, so that the
+    shallow fringe does not start at full speed).  The lattice
+    follows the centreline and interior corner nodes
     are jittered by ``jitter`` of the local spacing, so element
     shapes and sizes vary (unstructured connectivity is not needed for the solver's cost
     model: row widths and the frontal ordering depend on the element adjacency, which is
@@ -260,7 +264,8 @@ def meander(n_along: int = 3200, n_across: int = 320, reach: float = 10000.0,
     half_bed = 0.5 * main_width
     dist = np.abs(tc)
     bank = np.clip((dist - half_bed) * bank_slope, 0.0, main_depth)
-    zsec = -main_depth + bank
+    plain = plain_rise * np.clip(dist - half_bed - main_depth / bank_slope, 0.0, None)
+    zsec = -main_depth + bank + plain
     grf = gaussian_random_field(np.arange(NX) * ds, np.arange(NY) * dt_, noise_sigma,
                                 noise_corr, seed)
     Zc = slope * (reach - sc) + zsec + grf
@@ -317,11 +322,14 @@ def meander(n_along: int = 3200, n_across: int = 320, reach: float = 10000.0,
     npn = NX * NY
     lo, hi = n_range
     man = np.random.default_rng(seed + 2).uniform(lo, hi, ne)
+    U0 = np.full(npn, U_init)
+    if plain_rise > 0.0:
+        U0 *= np.clip((S_out - Z.ravel()) / main_depth, 0.0, 1.0)
     return Deck(x=X.ravel().astype(np.float64), y=Y.ravel().astype(np.float64),
                 z=Z.ravel().astype(np.float64), elems=elems.astype(np.int32),
                 nnd=np.full(ne, 6, np.int32), manning=man, vt=np.full(ne, vt),
                 lines={INLET_LINE: inlet.astype(np.int32), OUTLET_LINE: outlet.astype(np.int32)},
-                U0=np.full(npn, U_init), V0=np.zeros(npn), S0=np.full(npn, S_out),
+                U0=U0, V0=np.zeros(npn), S0=np.full(npn, S_out),
                 Q_inlet=Q, S_inlet=S_out + slope * reach, S_outlet=S_out,
                 name=name or f"meander_{nx}x{ny}")
 

@@ -37,15 +37,15 @@ PLAIN = dict(method=rmx.BICGSTAB, prec=rmx.F64, precond=rmx.PREC_ILU0, eps=1e-12
 
 
 def time_loop(m, steps: int, dt: float, theta: float, maxiter: int, newton, hydrograph=None,
-              stationary: bool = False, on_step=None, dry_freq: int = 0):
+              stationary: bool = False, on_step=None, dry_freq: int | None = None):
     """The flow cycle's time loop on a model ``m`` with the methods ``time_step(dt, theta,
     first)``, ``time_commit()``, ``set_inflow(Q, S)`` and ``slip_flow()``; ``newton()`` runs
     one Newton step and returns its statistics (``nrconv``).  ``dry_freq``: the drying and
-    rewetting frequency (0: none); below ``maxiter`` ``m`` needs ``dry_rewet()`` (elements
-    got dry, got wet) and ``relax_times()`` (dt_UV, dt_H, dt).  Returns the Newton steps of
-    every time step."""
+    rewetting frequency (None: no drying); below ``maxiter`` ``m`` needs ``dry_rewet()``
+    (elements got dry, got wet) and ``relax_times()`` (dt_UV, dt_H, dt).  Returns the
+    Newton steps of every time step."""
     counts = []
-    in_cycle = 0 < dry_freq < maxiter
+    in_cycle = dry_freq is not None and dry_freq < maxiter
     for t in range(1, steps + 1):
         if not stationary:
             if t > 1:
@@ -95,7 +95,7 @@ def run_reference_deck(input_file: str, ctx: rmx.Context | None = None, solve: d
         counts = time_loop(c, run["steps"] if unsteady else 1, run["dt"], run["theta"],
                            run["maxiter"], lambda: c.newton_step(**solve),
                            hydrograph=d.hydrograph, stationary=not unsteady, on_step=on_step,
-                           dry_freq=dw["freq"] if dw["method"] else 0)
+                           dry_freq=dw["freq"] if dw["method"] else None)
         U, V, S, z = c.state()
         return d, run, counts, (np.asarray(U), np.asarray(V), np.asarray(S), np.asarray(z))
     finally:

@@ -97,17 +97,32 @@ def test_meander_reach_first_newton_steps(ctx, path):
     assert n == len(g["s0_xsol"])
 
 
-def test_full_size_meander_newton_step_reaches_fp64_residual(ctx):
+@pytest.mark.parametrize("rise", [0.0, 0.012])
+def test_full_size_meander_newton_step_reaches_fp64_residual(ctx, rise):
     """The benchmark's own system: 3200 x 320 cells, one Newton step from the initial state
     on the fast path (FGMRES(15), two-level preconditioner, FP32 matrix) refined to a true
-    FP64 relative residual of 1e-8 of the FP64 Jacobian."""
-    print("[gpu] meander 3200x320 (2,048,000 triangles): one fast-path Newton step", flush=True)
-    d = dk.meander(n_along=3200, n_across=320, reach=10000.0)
+    FP64 relative residual of 1e-8 of the FP64 Jacobian.  With floodplains rising 0.012 m/m
+    the drying and rewetting on the device first takes their dry outer parts out: the dry
+    nodes carry no equations and every node of a wet element is wet."""
+    print(f"[gpu] meander 3200x320 (2,048,000 triangles), floodplain rise {rise}: one "
+          "fast-path Newton step", flush=True)
+    d = dk.meander(n_along=3200, n_across=320, reach=10000.0, plain_rise=rise)
     ctx.upload_mesh(d)
     ctx.set_params()
     n = ctx.prepare()
     assert n == 9220162
     _solver_setup(ctx, FAST)
+    if rise > 0:
+        dried, wetted = ctx.dry_rewet()
+        nf, ef = ctx.wetdry()
+        n = ctx.n
+        print(f"    drying: {dried} elements dry, {(ef == 2).sum()} marsh, {n} equations")
+        assert dried == (ef == 1).sum() > 100000 and wetted == 0 and n < 9220162
+        eq = ctx.numbering()
+        dry = (nf & 64) != 0
+        assert (eq[dry] < 0).all() and (eq[~dry, :2] >= -1).all()
+        wet_nodes = np.unique(d.elems[ef != 1][:, :6])
+        assert not dry[wet_nodes].any()
     st = ctx.newton_step(**dict(FAST, refine_tol=1e-8, refine_max=8, max_iter=1000))
     print(f"    {st.solve.iters} FGMRES iterations in {st.solve.sweeps} sweeps, true FP64 "
           f"residual {st.solve.true_resid:.2e}, {st.t_factor + st.t_solve:.2f} s")
