@@ -176,8 +176,8 @@ rmx_status rmx_slip_flow(rmx_ctx* ctx);
  * (EqsUVS2D.cpp:200-222).  rmx_time_commit keeps the flags as the previous time step's,
  * which Predict and Timegrad read (Compute.cpp:773-783, EqsUVS2D.cpp:1153, 1193-1202). */
 rmx_status rmx_dry_rewet(rmx_ctx* ctx, int* dried, int* wetted);
-/* Per node the reference's dry (64) and marsh (256) bits (Node.h:100-102); per region element
- * 1 dry, 2 marsh.  Either array nullable. */
+/* Per node the reference's dry (64) and marsh (256) bits and those of the previous time step
+ * (128, 512; Node.h:100-103); per region element 1 dry, 2 marsh.  Either array nullable. */
 rmx_status rmx_get_wetdry(rmx_ctx* ctx, int* nflag /* [np] */, int* eflag /* [ne] */);
 /* The relaxed time steps dt_UV, dt_H of the running cycle and its time step dt (below
  * relaxation method 3 all three are dt), for the flow cycle's test of the drying and

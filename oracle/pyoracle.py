@@ -197,8 +197,8 @@ class OracleModel:
         lib().orc_slip_flow(self.h)
 
     def wetdry(self):
-        """Per node the reference's dry (64) / marsh (256) flag bits, per element 1 dry /
-        2 marsh."""
+        """Per node the reference's dry (64) / marsh (256) flag bits and those of the previous
+        time step (128 / 512), per element 1 dry / 2 marsh."""
         nf = np.zeros(self.np, np.int32)
         ef = np.zeros(self.ne, np.int32)
         lib().orc_get_wetdry(self.h, _p(nf, _i), _p(ef, _i))

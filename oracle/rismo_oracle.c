@@ -39,7 +39,7 @@ enum { eInlet = 1, eOutlet = 2, eOpen = 4 };
 /* region element flags */
 enum { elDry = 1, elMarsh = 2 };
 /* the reference's node flag bits for dry and marsh nodes (Node.h:100-102) */
-enum { refNodeDry = 64, refNodeMarsh = 256 };
+enum { refNodeDry = 64, refNodeDryPrev = 128, refNodeMarsh = 256, refNodeMarshPrev = 512 };
 
 #define MAXEQ 24
 
@@ -1315,12 +1315,14 @@ void orc_set_dryrew(orc_model* m, double rewetLimit, int countDown)
   m->rewetLimit = rewetLimit; m->countDown = countDown;
 }
 
-/* per node the reference's dry and marsh bits (Node.h:100-102), per region element
- * 1 dry, 2 marsh */
+/* per node the reference's dry and marsh bits and those of the previous time step
+ * (Node.h:100-103), per region element 1 dry, 2 marsh */
 void orc_get_wetdry(const orc_model* m, int* nflag, int* eflag)
 {
   for (int n = 0; n < m->np; n++)
-    nflag[n] = ((m->flag[n] & fDry) ? refNodeDry : 0) | ((m->flag[n] & fMarsh) ? refNodeMarsh : 0);
+    nflag[n] = ((m->flag[n] & fDry) ? refNodeDry : 0) | ((m->flag[n] & fMarsh) ? refNodeMarsh : 0) |
+               ((m->flag[n] & fDryPrev) ? refNodeDryPrev : 0) |
+               ((m->flag[n] & fMarshPrev) ? refNodeMarshPrev : 0);
   for (int e = 0; e < m->ne; e++) eflag[e] = (int)m->eflag[e];
 }
 

@@ -378,7 +378,8 @@ rmx_status rmx_get_wetdry(rmx_ctx* p, int* nflag, int* eflag)
   HIPCHECK(hipStreamSynchronize(c.stream));
   if (nflag)
     for (int n = 0; n < c.hm.np; n++)
-      nflag[n] = ((nw[n] & WD_DRY) ? 64 : 0) | ((nw[n] & WD_MARSH) ? 256 : 0);
+      nflag[n] = ((nw[n] & WD_DRY) ? 64 : 0) | ((nw[n] & WD_DRYPREV) ? 128 : 0) |
+                 ((nw[n] & WD_MARSH) ? 256 : 0) | ((nw[n] & WD_MARSHPREV) ? 512 : 0);
   if (eflag)
     for (int e = 0; e < c.hm.ne; e++) eflag[e] = c.hm.ewd[e] & (EL_DRY | EL_MARSH);
   API_END

@@ -285,8 +285,8 @@ class Context:
         return d.value, w.value
 
     def wetdry(self):
-        """Per node the reference's dry (64) and marsh (256) bits, per element 1 dry,
-        2 marsh."""
+        """Per node the reference's dry (64) and marsh (256) bits and those of the previous
+        time step (128, 512), per element 1 dry, 2 marsh."""
         nf = np.zeros(self.np, np.int32)
         ef = np.zeros(self.ne, np.int32)
         _check(lib().rmx_get_wetdry(self.h, _p(nf, _i), _p(ef, _i)))

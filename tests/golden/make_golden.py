@@ -53,6 +53,11 @@ the reference's input format, runs the reference and stores what the harness rec
 * ``relax3_20x4.npz`` / ``relax3u20x4.npz``  the 20 x 4 triangle channel with relaxation
   method 3 (time-relaxed Newton steps), stationary (time step 100 s, relaxed time step from
   5 s) and unsteady (3 x 30 s, theta 0.7, relaxed time step from 2 s)
+* ``wetdryu20x6.npz`` the bank of ``wetdry20x6`` as an unsteady flow under a rising inlet
+                    hydrograph (4 steps of 30 s, theta 0.7), drying and rewetting every 3
+                    Newton steps of every time step: every Newton step's records with the
+                    previous time step's dry and marsh flags, time derivatives and previous
+                    state.
 * ``cap20x4.npz``   the 20 x 4 triangle channel with a binding relaxation cap ($RELAX 2
                     with maxDeltaUV 0.02, maxDeltaS 0.005): every Newton step's records.
 
@@ -114,6 +119,8 @@ DECKS = {
                     Q=10.0),
     "rewet20x6": dict(nx=20, ny=6, length=100.0, width=30.0, noise_sigma=0.05, noise_corr=5.0,
                       Q=10.0),
+    "wetdryu20x6": dict(nx=20, ny=6, length=100.0, width=30.0, noise_sigma=0.05, noise_corr=5.0,
+                        Q=10.0),
     "relax3_20x4": dict(nx=20, ny=4, length=100.0, width=20.0, noise_sigma=0.05, noise_corr=5.0,
                         Q=10.0),
     "relax3u20x4": dict(nx=20, ny=4, length=100.0, width=20.0, noise_sigma=0.05, noise_corr=5.0,
@@ -129,6 +136,7 @@ DECKS = {
 BANKS = {
     "wetdry20x6": (15.0, 30.0, 4.5),
     "rewet20x6": (15.0, 30.0, 2.4),
+    "wetdryu20x6": (15.0, 30.0, 4.5),
 }
 
 # initial water level below the outlet's: the level rises over the Newton steps and the
@@ -141,6 +149,7 @@ INIT_S = {
 HYDRO = {
     "hydro20x4": [[0.0, 10.0, 2.10], [60.0, 16.0, 2.16], [120.0, 8.0, 2.04]],
     "flood200x6": [[0.0, 100.0, 2.2], [6000.0, 1500.0, 2.2]],
+    "wetdryu20x6": [[0.0, 10.0, 2.10], [120.0, 30.0, 2.40]],
 }
 
 # decks kept to compact records (no per-step matrices or element records); "steps": the
@@ -157,6 +166,11 @@ RUNS = {
     "hydro20x4": dict(tm_steps=4, dt=30.0, theta=0.7, stationary=0, elem_step=5),
     "wetdry20x6": dict(dryrew="2 3 0.001 0.005 100 5", elem_step=1),
     "rewet20x6": dict(dryrew="2 2 0.001 0.005 100 1", maxiter=5, elem_step=3),
+    # the bank of wetdry20x6 under a rising hydrograph as an unsteady flow: drying and
+    # rewetting inside every time step's cycle, nodes dry or marsh at the previous time
+    # step (Predict, Timegrad)
+    "wetdryu20x6": dict(dryrew="2 3 0.001 0.005 100 1", maxiter=8, tm_steps=4, dt=30.0,
+                        theta=0.7, stationary=0, elem_step=6),
     "meander40x64": dict(maxiter=2, solver="1 6 1 0 60 2000 1.0e-10"),
     "flood200x6": dict(tm_steps=10, dt=30.0, theta=0.7, stationary=0),
     "cap20x4": dict(relax="2 0.001 0.1 0.02 0.005 0.1", maxiter=40),

@@ -370,6 +370,69 @@ def test_runner_time_loop_dries_and_rewets_as_the_reference(name):
     np.testing.assert_array_equal(nf, g["final_flag"].astype(np.int64) & (64 | 256))
 
 
+def test_unsteady_drying_follows_reference():
+    """The bank of wetdry20x6 as an unsteady flow under a rising hydrograph (4 x 30 s,
+    theta 0.7), drying and rewetting every 3 Newton steps of each time step's cycle: the
+    runner's time loop on the oracle reproduces every correction and the final state bit for
+    bit, with the dry and marsh flags of the previous time step (Compute.cpp:773-783) that
+    Predict (EqsUVS2D.cpp:1153) and Timegrad (:1193-1202) read, and the time derivatives."""
+    from rismo2d_amd import run as rr
+    g = load_golden("wetdryu20x6")
+    method, freq, dry_limit, rewet_limit, _passes, count_down = g["dryrew"]
+    om = po.OracleModel(golden_deck(g))
+    om.set_params(dry_method=int(method), dry_limit=float(dry_limit))
+    om.set_dryrew(float(rewet_limit), int(count_down))
+    dt, th = float(g["dt"]), float(g["theta"])
+
+    class Cycle:                       # the oracle behind the runner's time loop
+        k = 0
+
+        def time_step(self, dt, theta, first):
+            om.time_step(dt, theta, first)
+
+        def time_commit(self):
+            om.time_commit()
+
+        def set_inflow(self, Q, S):
+            om.set_inflow(Q, S, True)
+
+        def slip_flow(self):
+            om.slip_flow()
+
+        def dry_rewet(self):
+            return om.dry_rewet()
+
+        def relax_times(self):
+            return dt, dt, dt                # relaxation method 2: the time step itself
+
+    cyc = Cycle()
+    prev_seen = 0
+
+    def newton():
+        nonlocal prev_seen
+        k = cyc.k
+        p = f"s{k}_"
+        om.prepare()
+        nf, _ = om.wetdry()
+        np.testing.assert_array_equal(nf, g[p + "flag"].astype(np.int64) & (64 | 128 | 256 | 512),
+                                      err_msg=f"flags at step {k}")
+        prev_seen += int((nf & (128 | 512)).any())
+        ts = om.time_state()
+        for key in ("dUdt", "dVdt", "dSdt", "Uo", "Vo", "So", "dUdto", "dVdto", "dSdto"):
+            np.testing.assert_array_equal(ts[key], g[p + key], err_msg=f"{key} at step {k}")
+        X, st = om.newton_step(prec=32, solver=1, max_iter=1000, eps=1e-10)
+        np.testing.assert_array_equal(X, g[p + "xsol"], err_msg=f"step {k}")
+        cyc.k += 1
+        return st
+
+    counts = rr.time_loop(cyc, int(g["tm_steps"]), dt, th, int(g["maxiter"]), newton,
+                          hydrograph=g["hydrograph"], dry_freq=int(freq))
+    tsteps = [int(g[f"s{k}_tstep"]) for k in range(int(g["nsteps"]))]
+    assert counts == [tsteps.count(t) for t in range(1, int(g["tm_steps"]) + 1)]
+    assert prev_seen > 0                  # the previous time step's flags were exercised
+    _check_state(om, g, "final_")
+
+
 # ---------------------------------------------------------------------------------------
 # closed forms and invariants
 
