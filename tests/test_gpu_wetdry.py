@@ -141,3 +141,79 @@ def test_device_drying_follows_reference_records(ctx, name):
         assert any(k > 0 and w > 0 for k, _, w in calls)
     else:
         assert (ef == 1).sum() == calls[0][1]
+
+
+def test_device_unsteady_drying_follows_reference_records(ctx):
+    """The bank under a rising hydrograph as an unsteady flow (``wetdryu20x6``: 4 x 30 s,
+    theta 0.7, drying every 3 Newton steps of each cycle), driven by the reference's
+    corrections: at every Newton step the dry / marsh flags and those of the previous time
+    step bit-exact, the state bit-exact to 1e-13 of its scale, the time derivatives and the
+    previous state within 1e-12 of theirs, the numbering bit-exact."""
+    print("[gpu] unsteady drying and rewetting on the device, wetdryu20x6", flush=True)
+    g = load_golden("wetdryu20x6")
+    method, freq, dry_limit, rewet_limit, _passes, count_down = g["dryrew"]
+    dt, th = float(g["dt"]), float(g["theta"])
+    ctx.upload_mesh(golden_deck(g))
+    ctx.set_params(dry_method=int(method), dry_limit=float(dry_limit),
+                   rewet_limit=float(rewet_limit), count_down=int(count_down))
+
+    class Cycle:
+        k = 0
+
+        def time_step(self, dt, theta, first):
+            ctx.time_step(dt, theta, first)
+
+        def time_commit(self):
+            ctx.time_commit()
+
+        def set_inflow(self, Q, S):
+            ctx.set_inflow(Q, S)
+
+        def slip_flow(self):
+            ctx.slip_flow()
+
+        def dry_rewet(self):
+            return ctx.dry_rewet()
+
+        def relax_times(self):
+            return ctx.relax_times()
+
+    cyc = Cycle()
+    worst = {}
+
+    def newton():
+        k = cyc.k
+        p = f"s{k}_"
+        ctx.prepare()
+        nf, _ = ctx.wetdry()
+        np.testing.assert_array_equal(nf, g[p + "flag"].astype(np.int64) & (64 | 128 | 256 | 512),
+                                      err_msg=f"flags at step {k}")
+        eq = ctx.numbering()
+        np.testing.assert_array_equal(eq, np.stack([g[p + "eqU"], g[p + "eqV"], g[p + "eqS"]], 1))
+        U, V, S, z = ctx.state()
+        us = np.abs(g[p + "U"]).max() + np.abs(g[p + "V"]).max()
+        errs = dict(U=_rel(U, g[p + "U"], us), V=_rel(V, g[p + "V"], us),
+                    S=_rel(S, g[p + "S"], np.abs(g[p + "S"]).max()),
+                    z=_rel(z, g[p + "z"], np.abs(g[p + "S"]).max()))
+        ts = ctx.time_state()
+        for key in ("dUdt", "dVdt", "dSdt", "Uo", "Vo", "So", "dUdto", "dVdto", "dSdto"):
+            errs[key] = _rel(ts[key], g[p + key], max(np.abs(g[p + key]).max(), 1e-12))
+        for key, v in errs.items():
+            worst[key] = max(worst.get(key, 0.0), v)
+        for key in ("U", "V", "S", "z"):
+            assert errs[key] <= 1e-13, (k, key, errs[key])
+        for key in ("dUdt", "dVdt", "dSdt", "Uo", "Vo", "So", "dUdto", "dVdto", "dSdto"):
+            assert errs[key] <= 1e-12, (k, key, errs[key])
+        ctx.set_solution(g[p + "xsol"])
+        st = ctx.newton_update()
+        cyc.k += 1
+        return st
+
+    counts = rr.time_loop(cyc, int(g["tm_steps"]), dt, th, int(g["maxiter"]), newton,
+                          hydrograph=g["hydrograph"], dry_freq=int(freq))
+    print("    worst over the steps: " + ", ".join(f"{k} {v:.1e}" for k, v in worst.items()))
+    tsteps = [int(g[f"s{k}_tstep"]) for k in range(int(g["nsteps"]))]
+    assert counts == [tsteps.count(t) for t in range(1, int(g["tm_steps"]) + 1)]
+    U, V, S, z = ctx.state()
+    us = np.abs(g["final_U"]).max() + np.abs(g["final_V"]).max()
+    assert _rel(U, g["final_U"], us) <= 1e-13 and _rel(S, g["final_S"], np.abs(g["final_S"]).max()) <= 1e-13

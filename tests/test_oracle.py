@@ -373,9 +373,13 @@ def test_runner_time_loop_dries_and_rewets_as_the_reference(name):
 def test_unsteady_drying_follows_reference():
     """The bank of wetdry20x6 as an unsteady flow under a rising hydrograph (4 x 30 s,
     theta 0.7), drying and rewetting every 3 Newton steps of each time step's cycle: the
-    runner's time loop on the oracle reproduces every correction and the final state bit for
-    bit, with the dry and marsh flags of the previous time step (Compute.cpp:773-783) that
-    Predict (EqsUVS2D.cpp:1153) and Timegrad (:1193-1202) read, and the time derivatives."""
+    runner's time loop on the oracle reproduces every correction, the time derivatives and
+    previous state, and the final state bit for bit, and the dry and marsh flags of the
+    previous time step (Compute.cpp:773-783) at every step.  The set of wet elements stays
+    the same between time steps here, so the branches of Predict (EqsUVS2D.cpp:1153) and
+    Timegrad (:1193-1202) that those flags select are not reached: planting a mistake in
+    either passes this test (every deck tried whose wet set changes between time steps
+    diverges in the reference itself)."""
     from rismo2d_amd import run as rr
     g = load_golden("wetdryu20x6")
     method, freq, dry_limit, rewet_limit, _passes, count_down = g["dryrew"]
