@@ -27,7 +27,8 @@ enum {
   RMX_OK = 0,
   RMX_ERR_ARG = -1,       /* invalid argument or call order */
   RMX_ERR_HIP = -2,       /* HIP runtime error */
-  RMX_ERR_PIVOT = -3,     /* ILU(0) met a pivot below 1e-20 (Preco_ilu0.cpp:95) */
+  RMX_ERR_PIVOT = -3,     /* ILU(0) met a pivot below 1e-20 that no later row can repair
+                             (Preco_ilu0.cpp:95-135) */
   RMX_ERR_NODEVICE = -4,  /* no gfx950 device */
   RMX_ERR_RANGE = -5      /* size outside what the kernels support (row width > 128) */
 };
@@ -224,8 +225,10 @@ rmx_status rmx_scale_l2_noalloc(int device, int n, const int* rowptr, int prec, 
 /* Copy the FP32 matrix to or from the FP64 matrix (rounding to nearest). */
 rmx_status rmx_convert_matrix(rmx_ctx* ctx, int to_prec);
 
-/* ILU(0) on the matrix pattern: replaces PRECO_ILU0::Factor (Preco_ilu0.cpp:64).
- * prec selects which matrix (and factor) storage to use. */
+/* ILU(0) on the matrix pattern: replaces PRECO_ILU0::Factor (Preco_ilu0.cpp:64), with its
+ * pivot repair (:95-135: a pivot below 1e-20 gets the later row of its pattern with the
+ * largest entry in its column added).  prec selects which matrix (and factor) storage to
+ * use. */
 rmx_status rmx_ilu0_factor(rmx_ctx* ctx, int prec);
 
 /* x = U^-1 L^-1 b with the ILU(0) factors: replaces PRECO_ILU0::Solve

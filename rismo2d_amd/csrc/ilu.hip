@@ -376,7 +376,7 @@ __device__ __forceinline__ void st_sc1(__amdgpu_buffer_rsrc_t r, unsigned e, dou
   __builtin_amdgcn_raw_buffer_store_b64(__builtin_bit_cast(u32x2, v), r, e * 8u, 0, 16);
 }
 
-enum { FLOW_COUNTER = 0, FLOW_ERR = 1, FLOW_ABORT = 2 };
+enum { FLOW_COUNTER = 0, FLOW_ERR = 1, FLOW_ABORT = 2, FLOW_BADROW = 3 };
 
 // Ordering of LDS accesses inside one wave: a one-wave workgroup needs no barrier (a wave's
 // LDS operations complete in order); the fences keep the compiler from moving accesses
@@ -412,7 +412,7 @@ __global__ void __launch_bounds__(64) k_ilu_flow(const int4* __restrict__ desc, 
                                                  const unsigned short* __restrict__ plan,
                                                  VT* __restrict__ LU, int* __restrict__ done,
                                                  int epoch, int* __restrict__ state,
-                                                 int spin_limit)
+                                                 int spin_limit, int skip_upto)
 {
 #pragma clang fp contract(off)
   __shared__ VT s_val[FW];
@@ -446,6 +446,7 @@ __global__ void __launch_bounds__(64) k_ilu_flow(const int4* __restrict__ desc, 
     if (w >= n) break;
     const int4 d = desc[w];
     const int i = d.x, p0 = d.y, wi = d.z, nk = d.w;
+    if (i <= skip_upto) continue;                  // final from an earlier pass
     for (int t = lane; t < wi; t += 64) {
       s_val[t] = LU[p0 + t];
       s_col[t] = srt_cols[p0 + t];
@@ -613,10 +614,103 @@ __global__ void __launch_bounds__(64) k_ilu_flow(const int4* __restrict__ desc, 
     __atomic_signal_fence(__ATOMIC_SEQ_CST);
     if (lane == 0) {
       st_flag(done + i, epoch);
-      if (bad) atomicOr(state + FLOW_ERR, 1);
+      if (bad) {
+        atomicOr(state + FLOW_ERR, 1);
+        atomicMin(state + FLOW_BADROW, i);
+      }
     }
     wave_sync();                                   // LDS reused by the next row
   }
+}
+
+// Pivot repair of row i (PRECO_ILU0::Factor, Preco_ilu0.cpp:95-135), every row before i
+// final: among the rows eq > i of row i's pattern that hold column i, the one whose entry
+// in column i is largest in magnitude (the first in row i's order on a tie) after its
+// elimination by its pivot rows before i, is added to row i on row i's pattern.  The
+// candidates' partial rows start from the matrix A (their factor rows may hold values of
+// the failed pass).  One wave; state[FLOW_ERR] = 2 when no row qualifies.
+__device__ __forceinline__ int row_pos(const int* __restrict__ srt_cols,
+                                       const unsigned char* __restrict__ srt_pos, int q0, int w,
+                                       int c)
+{
+  int lo = 0, hi = w - 1;
+  while (lo <= hi) {
+    const int mid = (lo + hi) >> 1;
+    const int v = srt_cols[q0 + mid];
+    if (v == c) return srt_pos[q0 + mid];
+    if (v < c) lo = mid + 1; else hi = mid - 1;
+  }
+  return -1;
+}
+
+template <class VT>
+__global__ void __launch_bounds__(64) k_ilu_repair(int i, const int* __restrict__ rp,
+                                                   const int* __restrict__ cols,
+                                                   const int* __restrict__ srt_cols,
+                                                   const unsigned char* __restrict__ srt_pos,
+                                                   const VT* __restrict__ A, VT* __restrict__ LU,
+                                                   int* __restrict__ state)
+{
+#pragma clang fp contract(off)
+  __shared__ VT s_v[FW], s_best[FW];
+  const int lane = threadIdx.x;
+  const int p0 = rp[i], wi = rp[i + 1] - p0;
+  double bmax = 0.0;
+  int best = -1, bq0 = 0, bw = 0;
+  for (int j = 1; j < wi; j++) {
+    const int eq = cols[p0 + j];
+    if (eq <= i) continue;
+    const int q0 = rp[eq], wq = rp[eq + 1] - q0;
+    const int pos_i = row_pos(srt_cols, srt_pos, q0, wq, i);
+    if (pos_i < 0) continue;
+    for (int t = lane; t < wq; t += 64) s_v[t] = A[q0 + t];
+    __syncthreads();
+    // elimination by the pivot rows p < i of row eq, ascending
+    for (int sidx = 0; sidx < wq; sidx++) {
+      const int p = srt_cols[q0 + sidx];
+      if (p >= i) break;
+      const int pp = srt_pos[q0 + sidx];
+      const int k0 = rp[p], wk = rp[p + 1] - k0;
+      const VT f = -s_v[pp] / LU[k0];
+      __syncthreads();
+      for (int e = lane; e < wk - 1; e += 64) {
+        const int q = k0 + 1 + e;
+        const int c = cols[q];
+        if (c > p) {
+          const int pos = row_pos(srt_cols, srt_pos, q0, wq, c);
+          if (pos >= 0) {
+            const VT prod = f * LU[q];
+            s_v[pos] = s_v[pos] + prod;
+          }
+        }
+      }
+      __syncthreads();
+      if (lane == 0) s_v[pp] = f;
+      __syncthreads();
+    }
+    const double m = fabs((double)s_v[pos_i]);
+    if (m > bmax) {
+      bmax = m;
+      best = eq; bq0 = q0; bw = wq;
+      for (int t = lane; t < wq; t += 64) s_best[t] = s_v[t];
+    }
+    __syncthreads();
+  }
+  if (best < 0) {
+    if (lane == 0) state[FLOW_ERR] = 2;
+    return;
+  }
+  for (int t = lane; t < wi; t += 64) {
+    const int pos = row_pos(srt_cols, srt_pos, bq0, bw, cols[p0 + t]);
+    if (pos >= 0) LU[p0 + t] = LU[p0 + t] + s_best[pos];
+  }
+}
+
+// Completion flags of the rows after `from` cleared (their factorisation runs again).
+__global__ void k_flags_clear(int from, int n, int* __restrict__ done)
+{
+  const int r = from + blockIdx.x * blockDim.x + threadIdx.x;
+  if (r < n) done[r] = 0;
 }
 
 template <class VT>
@@ -1809,7 +1903,6 @@ static void factor_t(Ctx& c, Ilu& I, const VT* A)
   HIPCHECK(hipMemsetAsync(err, 0, sizeof(int), c.stream));
   const long long plan_n = (long long)I.plan.n;
   const char* lv = getenv("RMX_ILU_LEVELS");
-  int flow_err = 0;
   if (!(lv && atoi(lv) == 1)) {
     // one persistent launch: rows in level order, pivot rows awaited by completion flags
     if (I.done.n != (size_t)I.n) {
@@ -1818,7 +1911,6 @@ static void factor_t(Ctx& c, Ilu& I, const VT* A)
       I.epoch = 0;
     }
     I.flow_state.alloc(4);
-    HIPCHECK(hipMemsetAsync(I.flow_state.p, 0, sizeof(int) * 4, c.stream));
     if (++I.epoch == INT_MAX) {                   // epochs exhausted: clear the flags
       HIPCHECK(hipMemsetAsync(I.done.p, 0, sizeof(int) * I.n, c.stream));
       I.epoch = 1;
@@ -1837,19 +1929,51 @@ static void factor_t(Ctx& c, Ilu& I, const VT* A)
     if (const char* e = getenv("RMX_ILU_WAVES")) nwg = std::max(1, std::min(nwg, atoi(e)));
     int spin = 1 << 22;
     if (const char* e = getenv("RMX_ILU_SPIN")) spin = std::max(32, atoi(e));
-    const int kt = kt_begin(c, KT_FACTOR, 3.0 * (double)I.nnz * (sizeof(VT) + 4));
-    k_ilu_flow<VT><<<nwg, 64, 0, c.stream>>>(I.fdesc.p, I.n, I.cols.p, I.srt_cols.p, I.srt_pos.p,
-                                             I.fpiv.p, I.fpbase.p, I.plan.p, LU.p, I.done.p,
-                                             I.epoch, I.flow_state.p, spin);
-    kt_end(c, kt);
-    HIPCHECK(hipGetLastError());
-    int st[4] = {0, 0, 0, 0};
-    HIPCHECK(hipMemcpyAsync(st, I.flow_state.p, sizeof(int) * 4, hipMemcpyDeviceToHost, c.stream));
-    HIPCHECK(hipStreamSynchronize(c.stream));
-    if (st[FLOW_ABORT])
-      fail(RMX_ERR_HIP, "ILU(0): a row's wait for its pivot rows exceeded its bound (dataflow "
-                        "factorisation aborted)");
-    flow_err = st[FLOW_ERR];
+    // a pass factors every row after skip; a pivot below 1e-20 is repaired as the
+    // reference does (k_ilu_repair, on the first such row: every row before it is final),
+    // and the rows after it are factored again from A
+    int skip = -1;
+    I.repaired = 0;
+    for (;;) {
+      int st[4] = {0, 0, 0, INT_MAX};
+      HIPCHECK(hipMemcpyAsync(I.flow_state.p, st, sizeof(int) * 4, hipMemcpyHostToDevice,
+                              c.stream));
+      const int kt = kt_begin(c, KT_FACTOR, 3.0 * (double)I.nnz * (sizeof(VT) + 4));
+      k_ilu_flow<VT><<<nwg, 64, 0, c.stream>>>(I.fdesc.p, I.n, I.cols.p, I.srt_cols.p,
+                                               I.srt_pos.p, I.fpiv.p, I.fpbase.p, I.plan.p, LU.p,
+                                               I.done.p, I.epoch, I.flow_state.p, spin, skip);
+      kt_end(c, kt);
+      HIPCHECK(hipGetLastError());
+      HIPCHECK(hipMemcpyAsync(st, I.flow_state.p, sizeof(int) * 4, hipMemcpyDeviceToHost,
+                              c.stream));
+      HIPCHECK(hipStreamSynchronize(c.stream));
+      if (st[FLOW_ABORT])
+        fail(RMX_ERR_HIP, "ILU(0): a row's wait for its pivot rows exceeded its bound (dataflow "
+                          "factorisation aborted)");
+      if (!st[FLOW_ERR]) break;
+      const int bad = st[FLOW_BADROW];
+      if (bad <= skip || bad >= I.n) fail(RMX_ERR_HIP, "ILU(0): inconsistent pivot record");
+      k_ilu_repair<VT><<<1, 64, 0, c.stream>>>(bad, I.rp.p, I.cols.p, I.srt_cols.p, I.srt_pos.p,
+                                               A, LU.p, I.flow_state.p);
+      HIPCHECK(hipGetLastError());
+      HIPCHECK(hipMemcpyAsync(st, I.flow_state.p, sizeof(int) * 4, hipMemcpyDeviceToHost,
+                              c.stream));
+      HIPCHECK(hipStreamSynchronize(c.stream));
+      if (st[FLOW_ERR] == 2)
+        fail(RMX_ERR_PIVOT, "ILU(0) cannot be applied to the matrix: pivot of row " +
+                            std::to_string(bad) + " below 1e-20 and no later row of its pattern "
+                            "holds its column (PRECO_ILU0::Factor - 1)");
+      I.repaired++;
+      // the rows after it: the matrix again, not finished
+      const long long q1 = I.h_rp[bad + 1];
+      if (I.nnz > q1)
+        HIPCHECK(hipMemcpyAsync(LU.p + q1, A + q1, sizeof(VT) * (I.nnz - q1),
+                                hipMemcpyDeviceToDevice, c.stream));
+      if (bad + 1 < I.n)
+        k_flags_clear<<<(I.n - bad - 1 + 255) / 256, 256, 0, c.stream>>>(bad + 1, I.n, I.done.p);
+      HIPCHECK(hipGetLastError());
+      skip = bad;
+    }
   } else {
   const int kt = kt_begin(c, KT_FACTOR, 3.0 * (double)I.nnz * (sizeof(VT) + 4));
   for (size_t l = 0; l + 1 < I.flev.size(); l++) {
@@ -1891,7 +2015,9 @@ static void factor_t(Ctx& c, Ilu& I, const VT* A)
   int h_err = 0;
   HIPCHECK(hipMemcpyAsync(&h_err, err, sizeof(int), hipMemcpyDeviceToHost, c.stream));
   HIPCHECK(hipStreamSynchronize(c.stream));
-  if (h_err || flow_err) fail(RMX_ERR_PIVOT, "ILU(0): pivot below 1e-20 (row repair is not supported)");
+  if (h_err)
+    fail(RMX_ERR_PIVOT, "ILU(0): pivot below 1e-20 (the per-level factorisation, "
+                        "RMX_ILU_LEVELS=1, does not repair pivots; the dataflow one does)");
 }
 
 void ilu_factor(Ctx& c, Ilu& I, int prec)

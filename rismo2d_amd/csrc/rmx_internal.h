@@ -183,6 +183,7 @@ struct Ilu {
   // {row counter, error bits, abort flag, unused}
   DBuf<int> done, flow_state;
   int epoch = 0;
+  int repaired = 0;             // pivots repaired by the last factorisation
 };
 
 // FP32 SpMV layout.  Consecutive rows with one column set (the U, V and H rows of a node)

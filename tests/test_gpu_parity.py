@@ -576,6 +576,44 @@ def test_factorisation_of_rows_with_long_pivot_rows(ctx):
     np.testing.assert_array_equal(ctx.factors(rmx.F32), L32)
 
 
+def test_pivot_repair_matches_oracle(ctx):
+    """Pivots below 1e-20 are repaired as PRECO_ILU0::Factor does (Preco_ilu0.cpp:95-135):
+    the hand-worked 3 x 3 case of the oracle's test bit for bit, and a 300-row system whose
+    diagonal is zeroed in 6 rows (the repair runs in the order of the rows, each pass of the
+    dataflow factorisation stopping at the first; their lower parts are zeroed as well, so the
+    pivots stay exactly zero): FP32 factors bit-identical to the oracle's,
+    FP64 within 1e-12, the FP64 apply a solve with those factors."""
+    print("[gpu] ILU(0) pivot repair", flush=True)
+    rp = np.array([0, 3, 6, 9], np.int32)
+    cols = np.array([0, 1, 2, 1, 0, 2, 2, 0, 1], np.int32)
+    A = np.array([0, 1, 0, 1, 1, 1, 2, 0, 1], np.float64)
+    ctx.csr_upload(rp, cols, A, np.ones(3))
+    ctx.ilu0_factor(rmx.F64)
+    want = np.array([1, 2, 1, -1, -1, 0, 2, -0.0, 1])
+    np.testing.assert_array_equal(ctx.factors(rmx.F64), want)
+    ctx.convert(rmx.F32)
+    ctx.ilu0_factor(rmx.F32)
+    np.testing.assert_array_equal(ctx.factors(rmx.F32), want.astype(np.float32))
+    M, rp, cols, vals = _random_csr(300, 21, 0.03)
+    z = vals.copy()
+    for r in (5, 50, 51, 120, 200, 298):        # zero diagonal and lower part: a zero pivot
+        q = np.arange(rp[r], rp[r + 1])
+        z[q[cols[q] <= r]] = 0.0
+    b = np.random.default_rng(22).standard_normal(300)
+    L64, rep64 = po.ilu0_factor(rp, cols, z)
+    L32, rep32 = po.ilu0_factor(rp, cols, z.astype(np.float32))
+    assert rep64 >= 6 and rep32 >= 6
+    ctx.csr_upload(rp, cols, z, b)
+    ctx.ilu0_factor(rmx.F64)
+    np.testing.assert_allclose(ctx.factors(rmx.F64), L64, rtol=1e-12, atol=1e-14)
+    np.testing.assert_allclose(ctx.ilu0_apply(b), po.ilu0_solve(rp, cols, L64, b), rtol=1e-10,
+                               atol=1e-12)
+    ctx.convert(rmx.F32)
+    ctx.ilu0_factor(rmx.F32)
+    np.testing.assert_array_equal(ctx.factors(rmx.F32), L32)
+    print(f"    {rep32} pivots repaired", flush=True)
+
+
 def test_error_paths(ctx):
     M, rp, cols, vals = _random_csr(20, 5, 0.2)
     bad = cols.copy()
@@ -587,9 +625,8 @@ def test_error_paths(ctx):
         ctx.solve(rmx.BICGSTAB, rmx.F32, rmx.PREC_ILU0)   # no FP32 matrix / factors
     with pytest.raises(rmx.RmxError):
         ctx.solve(rmx.FGMRES, rmx.F64, rmx.PREC_NONE, mkyrl=64)
-    z = vals.copy()
-    z[rp[:-1]] = 0.0                    # zero diagonal everywhere: pivot failure
-    ctx.csr_upload(rp, cols, z, np.ones(20))
+    # a zero pivot with no later row to repair it from (PRECO_ILU0::Factor - 1)
+    ctx.csr_upload(np.array([0, 1], np.int32), np.array([0], np.int32), np.zeros(1), np.ones(1))
     with pytest.raises(rmx.RmxError) as e:
         ctx.ilu0_factor(rmx.F64)
     assert e.value.code == -3

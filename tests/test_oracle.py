@@ -440,6 +440,27 @@ def test_unsteady_drying_follows_reference():
 # ---------------------------------------------------------------------------------------
 # closed forms and invariants
 
+@pytest.mark.parametrize("dt", [np.float64, np.float32])
+def test_oracle_pivot_repair_hand_computed(dt):
+    """PRECO_ILU0::Factor's pivot repair (Preco_ilu0.cpp:95-135) worked by hand on a 3 x 3
+    full pattern (diagonal first): row 0 has a zero pivot; of the later rows holding column
+    0, row 1 has the largest entry there (1 against 0), so row 0 becomes row 0 + row 1 =
+    [1, 2, 1], and the elimination goes on from it.  A 1 x 1 zero matrix has no row to
+    repair from: -1."""
+    rp = np.array([0, 3, 6, 9], np.int32)
+    cols = np.array([0, 1, 2, 1, 0, 2, 2, 0, 1], np.int32)
+    A = np.array([0, 1, 0, 1, 1, 1, 2, 0, 1], dt)
+    L, rep = po.ilu0_factor(rp, cols, A)
+    assert rep == 1
+    # row 0: [1, 2, 1]; row 1 (diagonal first): u11 = 1 - 2 = -1, l10 = -1, u12 = 1 - 1 = 0;
+    # row 2: l20 = -0/1, l21 = -(1 + 0 * 2) / -1 = 1, u22 = 2 + 0 * 1 + 1 * 0 = 2
+    want = np.array([1, 2, 1, -1, -1, 0, 2, -0.0, 1], dt)
+    np.testing.assert_array_equal(L, want)
+    assert np.signbit(L[7])                      # the multiplier -0/1 keeps its sign
+    _, rep = po.ilu0_factor(np.array([0, 1], np.int32), np.array([0], np.int32),
+                            np.array([0.0], dt))
+    assert rep == -1
+
 @pytest.mark.parametrize("kind,wsum", [(0, 2.0), (1, 0.5), (2, 4.0)])
 def test_shape_identities(kind, wsum):
     s = po.shape(kind)
