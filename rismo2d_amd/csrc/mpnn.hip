@@ -8,7 +8,7 @@
 //
 //     h0 = relu(f W_in + b_in)                  (FP32, stored BF16)
 //     6 x:  a = mean of h over the neighbours   (FP32 sums of BF16, stored BF16)
-//           p = [h, a] W_l + b_l                (BF16 MFMA 16x16x16, FP32 accumulate)
+//           p = [h, a] W_l + b_l                (BF16 MFMA 16x16x32, FP32 accumulate)
 //           h = h + relu(p)                     (stored BF16)
 //     out = h W_out + b_out                     (FP32, 18 per node): two 3x3 blocks D_i, E_i
 //
@@ -32,8 +32,17 @@
 
 namespace rmx {
 
-typedef short s4 __attribute__((ext_vector_type(4)));
+typedef short s8 __attribute__((ext_vector_type(8)));
+typedef __bf16 b8 __attribute__((ext_vector_type(8)));
 typedef float f4 __attribute__((ext_vector_type(4)));
+
+// gfx950's BF16 MFMA with K = 32 per instruction (16 x 16 output tile, FP32 accumulation):
+// lane l holds 8 of the K values of row (A) or column (B) l % 16, the same 8 in both
+__device__ __forceinline__ f4 mfma_k32(s8 a, s8 b, f4 c)
+{
+  return __builtin_amdgcn_mfma_f32_16x16x32_bf16(__builtin_bit_cast(b8, a),
+                                                 __builtin_bit_cast(b8, b), c, 0, 0, 0);
+}
 
 constexpr int MC = Mpnn::C, ML = Mpnn::L, MF = Mpnn::F, MO = Mpnn::O;
 constexpr int OFF_WIN = 0, OFF_BIN = MF * MC;
@@ -42,7 +51,7 @@ constexpr int OFF_L0 = OFF_BIN + MC;
 constexpr int OFF_WOUT = OFF_L0 + ML * LAYER_SZ, OFF_BOUT = OFF_WOUT + MC * MO;
 constexpr int NPARAM = OFF_BOUT + MO;
 static_assert(NPARAM == 51282, "parameter layout");
-constexpr int CHUNK = 1024;        // nodes per block of the small reductions
+constexpr int CHUNK = 64;          // nodes per block of the small reductions
 constexpr int WCHUNK = 256;        // nodes per block of the layer weight gradients
 
 __device__ __forceinline__ uint16_t f2bf(float f)
@@ -201,7 +210,7 @@ __global__ void __launch_bounds__(256) k_agg(int np, const int* __restrict__ ptr
 }
 
 // h_out = h + relu([h, a] W + b): 64 nodes per block, 16 per wave, 4 output tiles of 16
-// channels per wave, K = 128 in 8 MFMA steps.  Wt = W^T (BF16, [64][128]).
+// channels per wave, K = 128 in 4 MFMA steps.  Wt = W^T (BF16, [64][128]).
 __global__ void __launch_bounds__(256) k_layer_fwd(int np, const uint16_t* __restrict__ H,
                                                    const uint16_t* __restrict__ A,
                                                    const uint16_t* __restrict__ Wt,
@@ -211,22 +220,22 @@ __global__ void __launch_bounds__(256) k_layer_fwd(int np, const uint16_t* __res
 {
   const int wave = threadIdx.x >> 6, lane = threadIdx.x & 63;
   const int nb = blockIdx.x * 64 + wave * 16;
-  const int r16 = lane & 15, kq = (lane >> 4) * 4;
+  const int r16 = lane & 15, kq = (lane >> 4) * 8;
   const int node = nb + r16;
   f4 acc[4];
 #pragma unroll
   for (int t = 0; t < 4; t++) acc[t] = f4{0.0f, 0.0f, 0.0f, 0.0f};
 #pragma unroll
-  for (int ks = 0; ks < 8; ks++) {
-    const int k0 = ks * 16 + kq;
-    s4 a = s4{0, 0, 0, 0};
+  for (int ks = 0; ks < 4; ks++) {
+    const int k0 = ks * 32 + kq;
+    s8 a = s8{0, 0, 0, 0, 0, 0, 0, 0};
     if (node < np)
-      a = *reinterpret_cast<const s4*>(k0 < MC ? H + (size_t)node * MC + k0
+      a = *reinterpret_cast<const s8*>(k0 < MC ? H + (size_t)node * MC + k0
                                                : A + (size_t)node * MC + (k0 - MC));
 #pragma unroll
     for (int t = 0; t < 4; t++) {
-      const s4 b = *reinterpret_cast<const s4*>(Wt + (size_t)(t * 16 + r16) * (2 * MC) + k0);
-      acc[t] = __builtin_amdgcn_mfma_f32_16x16x16bf16_1k(a, b, acc[t], 0, 0, 0);
+      const s8 b = *reinterpret_cast<const s8*>(Wt + (size_t)(t * 16 + r16) * (2 * MC) + k0);
+      acc[t] = mfma_k32(a, b, acc[t]);
     }
   }
 #pragma unroll
@@ -300,20 +309,20 @@ __global__ void __launch_bounds__(256) k_layer_bwd(int np, const uint16_t* __res
 {
   const int wave = threadIdx.x >> 6, lane = threadIdx.x & 63;
   const int nb = blockIdx.x * 64 + wave * 16;
-  const int r16 = lane & 15, kq = (lane >> 4) * 4;
+  const int r16 = lane & 15, kq = (lane >> 4) * 8;
   const int node = nb + r16;
   f4 acc[8];
 #pragma unroll
   for (int t = 0; t < 8; t++) acc[t] = f4{0.0f, 0.0f, 0.0f, 0.0f};
 #pragma unroll
-  for (int ks = 0; ks < 4; ks++) {
-    const int k0 = ks * 16 + kq;
-    s4 a = s4{0, 0, 0, 0};
-    if (node < np) a = *reinterpret_cast<const s4*>(gp + (size_t)node * MC + k0);
+  for (int ks = 0; ks < 2; ks++) {
+    const int k0 = ks * 32 + kq;
+    s8 a = s8{0, 0, 0, 0, 0, 0, 0, 0};
+    if (node < np) a = *reinterpret_cast<const s8*>(gp + (size_t)node * MC + k0);
 #pragma unroll
     for (int t = 0; t < 8; t++) {
-      const s4 b = *reinterpret_cast<const s4*>(Wb + (size_t)(t * 16 + r16) * MC + k0);
-      acc[t] = __builtin_amdgcn_mfma_f32_16x16x16bf16_1k(a, b, acc[t], 0, 0, 0);
+      const s8 b = *reinterpret_cast<const s8*>(Wb + (size_t)(t * 16 + r16) * MC + k0);
+      acc[t] = mfma_k32(a, b, acc[t]);
     }
   }
 #pragma unroll
@@ -356,22 +365,22 @@ __global__ void __launch_bounds__(256) k_wgrad(int np, const uint16_t* __restric
                                                float* __restrict__ partial)
 {
   const int wave = threadIdx.x >> 6, lane = threadIdx.x & 63;
-  const int r16 = lane & 15, kq = (lane >> 4) * 4;
+  const int r16 = lane & 15, kq = (lane >> 4) * 8;
   const int n0 = blockIdx.x * WCHUNK;
   f4 acc[2][4];
 #pragma unroll
   for (int u = 0; u < 2; u++)
 #pragma unroll
     for (int t = 0; t < 4; t++) acc[u][t] = f4{0.0f, 0.0f, 0.0f, 0.0f};
-  for (int kk = 0; kk < WCHUNK; kk += 16) {
-    s4 a[2], b[4];
+  for (int kk = 0; kk < WCHUNK; kk += 32) {
+    s8 a[2], b[4];
 #pragma unroll
     for (int u = 0; u < 2; u++) {
       const int m = (wave * 2 + u) * 16 + r16;
       const uint16_t* X = m < MC ? H : A;
       const int mc = m < MC ? m : m - MC;
 #pragma unroll
-      for (int q = 0; q < 4; q++) {
+      for (int q = 0; q < 8; q++) {
         const int nd = n0 + kk + kq + q;
         a[u][q] = nd < np ? (short)X[(size_t)nd * MC + mc] : (short)0;
       }
@@ -380,7 +389,7 @@ __global__ void __launch_bounds__(256) k_wgrad(int np, const uint16_t* __restric
     for (int t = 0; t < 4; t++) {
       const int n = t * 16 + r16;
 #pragma unroll
-      for (int q = 0; q < 4; q++) {
+      for (int q = 0; q < 8; q++) {
         const int nd = n0 + kk + kq + q;
         b[t][q] = nd < np ? (short)gp[(size_t)nd * MC + n] : (short)0;
       }
@@ -389,7 +398,7 @@ __global__ void __launch_bounds__(256) k_wgrad(int np, const uint16_t* __restric
     for (int u = 0; u < 2; u++)
 #pragma unroll
       for (int t = 0; t < 4; t++)
-        acc[u][t] = __builtin_amdgcn_mfma_f32_16x16x16bf16_1k(a[u], b[t], acc[u][t], 0, 0, 0);
+        acc[u][t] = mfma_k32(a[u], b[t], acc[u][t]);
   }
   float* out = partial + (size_t)blockIdx.x * (2 * MC * MC);
 #pragma unroll
@@ -412,20 +421,37 @@ template <> __device__ __forceinline__ float ld<float>(const float* p, size_t i)
 template <> __device__ __forceinline__ float ld<uint16_t>(const uint16_t* p, size_t i) { return bf2f(p[i]); }
 
 template <class XT, class GT>
-__global__ void k_outer(int np, const XT* __restrict__ X, int mx, const GT* __restrict__ G, int ng,
-                        float* __restrict__ partial)
+__global__ void __launch_bounds__(256) k_outer(int np, const XT* __restrict__ X, int mx,
+                                               const GT* __restrict__ G, int ng,
+                                               float* __restrict__ partial)
 {
+  // tp threads share a pair, each summing every tp-th node of the block's CHUNK; their sums
+  // are added in a fixed order
+  __shared__ float s_part[256];
   const int pairs = (mx + 1) * ng;
-  const int t = threadIdx.x;
+  int tp = 1;
+  while (tp * 2 * pairs <= 256) tp *= 2;
+  const int t = threadIdx.x, sub = t % tp;
   const int n0 = blockIdx.x * CHUNK, n1 = min(np, n0 + CHUNK);
-  for (int pr = t; pr < pairs; pr += blockDim.x) {
-    const int m = pr / ng, o = pr % ng;
+  for (int base = 0; base < pairs; base += 256 / tp) {
+    const int pr = base + t / tp;
     float s = 0.0f;
-    if (m < mx)
-      for (int i = n0; i < n1; i++) s += ld<XT>(X, (size_t)i * mx + m) * ld<GT>(G, (size_t)i * ng + o);
-    else
-      for (int i = n0; i < n1; i++) s += ld<GT>(G, (size_t)i * ng + o);
-    partial[(size_t)blockIdx.x * pairs + pr] = s;
+    if (pr < pairs) {
+      const int m = pr / ng, o = pr % ng;
+      if (m < mx)
+        for (int i = n0 + sub; i < n1; i += tp)
+          s += ld<XT>(X, (size_t)i * mx + m) * ld<GT>(G, (size_t)i * ng + o);
+      else
+        for (int i = n0 + sub; i < n1; i += tp) s += ld<GT>(G, (size_t)i * ng + o);
+    }
+    s_part[t] = s;
+    __syncthreads();
+    if (sub == 0 && pr < pairs) {
+      float a = 0.0f;
+      for (int q = 0; q < tp; q++) a += s_part[t + q];
+      partial[(size_t)blockIdx.x * pairs + pr] = a;
+    }
+    __syncthreads();
   }
 }
 
