@@ -58,6 +58,11 @@ the reference's input format, runs the reference and stores what the harness rec
                     Newton steps of every time step: every Newton step's records with the
                     previous time step's dry and marsh flags, time derivatives and previous
                     state.
+* ``flooddry200x6.npz`` ``flood200x6`` with a bank rising 2.6 m across the outer 40 m of
+                    the width, drying and rewetting every 3 Newton steps of every time step
+                    (5 time steps; the Newton limit of 12 ends each, the marsh strip keeps
+                    the iteration from converging): every Newton step's correction, the
+                    final state and the final dry / marsh flags.
 * ``cap20x4.npz``   the 20 x 4 triangle channel with a binding relaxation cap ($RELAX 2
                     with maxDeltaUV 0.02, maxDeltaS 0.005): every Newton step's records.
 
@@ -113,6 +118,8 @@ DECKS = {
                        Q=10.0),
     "vegchannel200x20": dict(nx=200, ny=20, veg_frac=0.3, veg_band=0.3),
     "meander40x64": dict(gen="meander", n_along=40, n_across=64, reach=125.0),
+    "flooddry200x6": dict(nx=200, ny=6, length=2000.0, turb=211, turb_estx=(0.4, 1.2),
+                          turb_esty=(0.05, 0.3)),
     "flood200x6": dict(nx=200, ny=6, length=2000.0, turb=211, turb_estx=(0.4, 1.2),
                        turb_esty=(0.05, 0.3)),
     "cap20x4": dict(nx=20, ny=4, length=100.0, width=20.0, noise_sigma=0.05, noise_corr=5.0,
@@ -137,6 +144,7 @@ BANKS = {
     "wetdry20x6": (15.0, 30.0, 4.5),
     "rewet20x6": (15.0, 30.0, 2.4),
     "wetdryu20x6": (15.0, 30.0, 4.5),
+    "flooddry200x6": (60.0, 100.0, 2.6),
 }
 
 # initial water level below the outlet's: the level rises over the Newton steps and the
@@ -149,13 +157,14 @@ INIT_S = {
 HYDRO = {
     "hydro20x4": [[0.0, 10.0, 2.10], [60.0, 16.0, 2.16], [120.0, 8.0, 2.04]],
     "flood200x6": [[0.0, 100.0, 2.2], [6000.0, 1500.0, 2.2]],
+    "flooddry200x6": [[0.0, 100.0, 2.2], [6000.0, 1500.0, 2.2]],
     "wetdryu20x6": [[0.0, 10.0, 2.10], [120.0, 30.0, 2.40]],
 }
 
 # decks kept to compact records (no per-step matrices or element records); "steps": the
 # per-step corrections, "timesteps": the committed state of every time step instead
 COMPACT = {"channel200x20": "steps", "vegchannel200x20": "steps", "meander40x64": "steps",
-           "flood200x6": "timesteps", "relax3_20x4": "steps", "relax3u20x4": "steps",
+           "flood200x6": "timesteps", "flooddry200x6": "steps", "relax3_20x4": "steps", "relax3u20x4": "steps",
            "restart20x4": "steps"}
 
 # time integration of the decks that are not one stationary flow step, and the Newton step
@@ -173,6 +182,10 @@ RUNS = {
                         theta=0.7, stationary=0, elem_step=6),
     "meander40x64": dict(maxiter=2, solver="1 6 1 0 60 2000 1.0e-10"),
     "flood200x6": dict(tm_steps=10, dt=30.0, theta=0.7, stationary=0),
+    # the flood with a bank rising out of the water along one side: drying and rewetting
+    # every 3 Newton steps of every time step, with the hydrograph and Elder's viscosity
+    "flooddry200x6": dict(tm_steps=5, dt=30.0, theta=0.7, stationary=0,
+                          dryrew="2 3 0.001 0.005 100 5"),
     "cap20x4": dict(relax="2 0.001 0.1 0.02 0.005 0.1", maxiter=40),
     # relaxation method 3 (the reference's default): time-relaxed Newton steps whose relaxed
     # time step grows from relaxTimeFlow to the time step; stationary, and unsteady

@@ -217,3 +217,67 @@ def test_device_unsteady_drying_follows_reference_records(ctx):
     U, V, S, z = ctx.state()
     us = np.abs(g["final_U"]).max() + np.abs(g["final_V"]).max()
     assert _rel(U, g["final_U"], us) <= 1e-13 and _rel(S, g["final_S"], np.abs(g["final_S"]).max()) <= 1e-13
+
+
+def test_device_flood_with_drying_follows_reference(ctx):
+    """The 2 km flood channel (``flooddry200x6``: hydrograph, Elder's viscosity recomputed
+    every Newton step, 5 x 30 s, theta 0.7) with a bank rising out of the water along one
+    side and drying every 3 Newton steps, driven by the reference's corrections: the
+    reference's Newton steps per time step, its final state within 1e-13 of its scale and its
+    final dry / marsh flags bit-exact."""
+    print("[gpu] flood with drying on the device, flooddry200x6", flush=True)
+    g = load_golden("flooddry200x6")
+    method, freq, dry_limit, rewet_limit, _passes, count_down = g["dryrew"]
+    dt, th = float(g["dt"]), float(g["theta"])
+    ctx.upload_mesh(golden_deck(g))
+    ctx.set_params(dry_method=int(method), dry_limit=float(dry_limit),
+                   rewet_limit=float(rewet_limit), count_down=int(count_down))
+
+    class Cycle:
+        k = 0
+
+        def time_step(self, dt, theta, first):
+            ctx.time_step(dt, theta, first)
+
+        def time_commit(self):
+            ctx.time_commit()
+
+        def set_inflow(self, Q, S):
+            ctx.set_inflow(Q, S)
+
+        def slip_flow(self):
+            ctx.slip_flow()
+
+        def dry_rewet(self):
+            return ctx.dry_rewet()
+
+        def relax_times(self):
+            return ctx.relax_times()
+
+    cyc = Cycle()
+
+    def newton():
+        n = ctx.prepare()
+        X = g[f"s{cyc.k}_xsol"]
+        assert n == len(X), cyc.k
+        ctx.set_solution(X)
+        st = ctx.newton_update()
+        cyc.k += 1
+        return st
+
+    counts = rr.time_loop(cyc, int(g["tm_steps"]), dt, th, 12, newton,
+                          hydrograph=g["hydrograph"], dry_freq=int(freq))
+    tsteps = [int(g[f"s{k}_tstep"]) for k in range(int(g["nsteps"]))]
+    assert counts == [tsteps.count(t) for t in range(1, int(g["tm_steps"]) + 1)]
+    U, V, S, z = ctx.state()
+    us = np.abs(g["final_U"]).max() + np.abs(g["final_V"]).max()
+    eu, ev = _rel(U, g["final_U"], us), _rel(V, g["final_V"], us)
+    es = _rel(S, g["final_S"], np.abs(g["final_S"]).max())
+    ez = _rel(z, g["final_z"], np.abs(g["final_S"]).max())
+    print(f"    {sum(counts)} Newton steps; final state vs reference: u {eu:.1e}, v {ev:.1e}, "
+          f"S {es:.1e}, z {ez:.1e}")
+    assert max(eu, ev, es, ez) <= 1e-13
+    nf, _ = ctx.wetdry()
+    np.testing.assert_array_equal(nf & (64 | 256), g["final_flag"].astype(np.int64) & (64 | 256))
+    np.testing.assert_array_equal(ctx.numbering(),
+                                  np.stack([g["final_eqU"], g["final_eqV"], g["final_eqS"]], 1))

@@ -440,6 +440,60 @@ def test_unsteady_drying_follows_reference():
 # ---------------------------------------------------------------------------------------
 # closed forms and invariants
 
+def test_flood_with_drying_follows_reference():
+    """The 2 km flood channel under its hydrograph with Elder's viscosity recomputed every
+    Newton step, and a bank rising out of the water along one side, drying and rewetting
+    every 3 Newton steps of every time step (5 x 30 s, theta 0.7): the runner's time loop
+    on the oracle reproduces every Newton correction, the final state and the final dry and
+    marsh flags bit for bit."""
+    from rismo2d_amd import run as rr
+    g = load_golden("flooddry200x6")
+    method, freq, dry_limit, rewet_limit, _passes, count_down = g["dryrew"]
+    om = po.OracleModel(golden_deck(g))
+    om.set_params(dry_method=int(method), dry_limit=float(dry_limit))
+    om.set_dryrew(float(rewet_limit), int(count_down))
+    dt = float(g["dt"])
+
+    class Cycle:
+        k = 0
+
+        def time_step(self, dt, theta, first):
+            om.time_step(dt, theta, first)
+
+        def time_commit(self):
+            om.time_commit()
+
+        def set_inflow(self, Q, S):
+            om.set_inflow(Q, S, True)
+
+        def slip_flow(self):
+            om.slip_flow()
+
+        def dry_rewet(self):
+            return om.dry_rewet()
+
+        def relax_times(self):
+            return dt, dt, dt
+
+    cyc = Cycle()
+
+    def newton():
+        om.prepare()
+        X, st = om.newton_step(prec=32, solver=1, max_iter=1000, eps=1e-10)
+        np.testing.assert_array_equal(X, g[f"s{cyc.k}_xsol"], err_msg=f"step {cyc.k}")
+        cyc.k += 1
+        return st
+
+    counts = rr.time_loop(cyc, int(g["tm_steps"]), dt, float(g["theta"]), 12, newton,
+                          hydrograph=g["hydrograph"], dry_freq=int(freq))
+    tsteps = [int(g[f"s{k}_tstep"]) for k in range(int(g["nsteps"]))]
+    assert counts == [tsteps.count(t) for t in range(1, int(g["tm_steps"]) + 1)]
+    _check_state(om, g, "final_")
+    nf, _ = om.wetdry()
+    np.testing.assert_array_equal(nf & (64 | 256), g["final_flag"].astype(np.int64) & (64 | 256))
+    assert (nf & 256).any()
+
+
 @pytest.mark.parametrize("dt", [np.float64, np.float32])
 def test_oracle_pivot_repair_hand_computed(dt):
     """PRECO_ILU0::Factor's pivot repair (Preco_ilu0.cpp:95-135) worked by hand on a 3 x 3
