@@ -25,7 +25,10 @@ Printed (one JSON line):
   residual of 1e-8, and the reference program's own CPU step on the same reach;
 * plain_fp64_2M: the plain path on the 2M system run for a bounded number of iterations
   (its time to 1e-8 there exceeds a benchmark run; elapsed / value bounds the speedup);
-* learned: the learned preconditioner's leg, or null with the reason;
+* learned: the same Newton step with the learned network correcting the two-level
+  preconditioner (weights rismo2d_amd/mpnn_meander.npy, trained by
+  `python -m rismo2d_amd.mpnn` on 200k-element reaches of other seeds), or null with the
+  reason;
 * cpu_baseline: the reference program (built from its sources) on the comparison reach, one
   core, with its DOF-iterations/s beside the GPU paths' on the same reach.
 
@@ -71,6 +74,9 @@ def parse():
                          "floodplains are dry and drying and rewetting is active")
     ap.add_argument("--cmp-along", type=int, default=80,
                     help="cross-sections of the comparison reach (0: no comparison)")
+    ap.add_argument("--learned", default=os.path.join(HERE, "rismo2d_amd", "mpnn_meander.npy"),
+                    help="weights of the learned preconditioner (numpy float32); the learned "
+                         "leg runs when the file exists")
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--kernels-only", action="store_true",
                     help="profiling run: build and factor the system, time the FP32 SpMV and "
@@ -405,6 +411,31 @@ def main():
                 f"{v['achieved_gb_s']:.0f} GB/s ({v['frac_hbm_peak']:.3f} of peak)")
     log(f"other: {rest * 1e3:.0f} ms/step")
 
+    # learned leg: the same Newton step with the network correcting the two-level
+    # preconditioner (same Krylov method, restart and refinement), one untimed and one timed
+    learned, learned_reason = None, None
+    if os.path.exists(a.learned):
+        theta = np.load(a.learned)
+        c.mpnn_set_weights(theta)
+        lopts = dict(opts, precond=R.PREC_MPNN)
+        for k in range(2):
+            reset()
+            t0 = time.perf_counter()
+            dry_call()
+            st = c.newton_step(**lopts)
+            tl = time.perf_counter() - t0
+            log(f"learned {'warmup' if k == 0 else 'step'}: {tl:.2f}s ({st.solve.iters} it, "
+                f"{st.solve.sweeps} sweeps, fallback {st.solve.fallback}, residual "
+                f"{st.solve.true_resid:.2e})")
+        learned = {"weights": os.path.relpath(a.learned, HERE), "s_per_newton_step": tl,
+                   "converged": bool(st.solve.converged), "true_resid": st.solve.true_resid,
+                   "krylov_iters": st.solve.iters, "refine_sweeps": st.solve.sweeps,
+                   "fallback_to_bicgstab": bool(st.solve.fallback), "t_solve_s": st.t_solve,
+                   "vs_two_level_step": tl / T,
+                   "krylov_iters_two_level": last.solve.iters}
+    else:
+        learned_reason = f"no trained weights at {a.learned}"
+
     # plain FP64 path on the 2M system (state reset, friction and inlet recomputed):
     # reference-order ILU(0), BiCGStab, bounded
     c.state_upload(d.U0, d.V0, d.S0)
@@ -481,10 +512,9 @@ def main():
                           # not at 1e-8 when stopped: its time to 1e-8 exceeds elapsed_s
                           "speedup_lower_bound": (plain_elapsed / T) if (last.solve.converged and
                                                                           not out.converged) else None},
-        "learned": None,
-        "learned_reason": ("the learned preconditioner does not beat the two-level one inside "
-                           "FGMRES (tests/test_gpu_mpnn.py measures both on a held-out seed), so "
-                           "the timed path is the FP32 two-level one"),
+        # value is the two-level path's step; the learned leg is timed beside it
+        "learned": learned,
+        "learned_reason": learned_reason,
         "cpu_baseline": cpu and {k: cpu[k] for k in ("value", "unit", "cores", "kind", "sample",
                                                       "dof_iters_per_s") if k in cpu},
     }

@@ -280,12 +280,16 @@ rmx_status rmx_get_aggregates(rmx_ctx* ctx, int* agg /* [neq] */, int* nc);
  * W [128][64], b [64], then W_out [64][18], b_out [18]).
  * rmx_mpnn_set_weights / get_weights: the weights (host, FP32); setting also resets Adam.
  * rmx_mpnn_forward: z = n(r) alone, and the node features [np][8] (nullable).
- * rmx_mpnn_add_sample: store a training sample: the features of the current state and r,
- * r itself and the current FP64 matrix A; capacity > 0 (re)allocates and clears the set.
+ * rmx_mpnn_add_sample: store a training sample r (host).  The first sample of a Newton
+ * system (the current FP64 matrix A, not stored yet) also stores the system: A, its row
+ * maxima and the node features of the current state, shared by the system's later samples.
+ * capacity > 0 empties the set and sizes it for `capacity` samples; a mesh of another
+ * topology or wet set empties it too, and the next call must then pass a capacity.
+ * RMX_ERR_RANGE when the set is full.
  * rmx_mpnn_loss_grad: the preconditioned residual ||r - A n(r)||^2 / ||r||^2 of one
  * stored sample and its gradient (nullable).
- * rmx_mpnn_train: Adam (beta 0.9 / 0.999) over the stored samples, one step per sample,
- * `epochs` passes; *loss = mean loss of the last pass. */
+ * rmx_mpnn_train: Adam (beta 0.9 / 0.999) over the stored systems, one step per system on
+ * the mean loss of its samples, `epochs` passes; *loss = mean sample loss of the last pass. */
 rmx_status rmx_mpnn_n_params(int* n);
 rmx_status rmx_mpnn_set_weights(rmx_ctx* ctx, const float* theta, int n);
 rmx_status rmx_mpnn_get_weights(rmx_ctx* ctx, float* theta, int n);

@@ -124,10 +124,11 @@ __global__ void k_lin_apply(int np, const int* __restrict__ eqno, const int* __r
   }
 }
 
-// gout[i] = d loss / d (D_i, E_i) from g = d loss / d n (equation vector)
+// gout[i] = d loss / d (D_i, E_i) from g = d loss / d n (equation vector); with `add` the
+// gradient is added to what gout holds
 __global__ void k_lin_grad(int np, const int* __restrict__ eqno, const int* __restrict__ ptr,
                            const int* __restrict__ adj, const double* __restrict__ q,
-                           const double* __restrict__ g, float* __restrict__ gout)
+                           const double* __restrict__ g, int add, float* __restrict__ gout)
 {
   const int i = blockIdx.x * blockDim.x + threadIdx.x;
   if (i >= np) return;
@@ -138,8 +139,9 @@ __global__ void k_lin_grad(int np, const int* __restrict__ eqno, const int* __re
     const int e = eqno[3 * i + v];
     const double gv = e >= 0 ? g[e] : 0.0;
     for (int w = 0; w < 3; w++) {
-      o[3 * v + w] = (float)(gv * qi[w]);
-      o[9 + 3 * v + w] = (float)(gv * qb[w]);
+      const float a = (float)(gv * qi[w]), b = (float)(gv * qb[w]);
+      o[3 * v + w] = add ? o[3 * v + w] + a : a;
+      o[9 + 3 * v + w] = add ? o[9 + 3 * v + w] + b : b;
     }
   }
 }
@@ -550,8 +552,17 @@ static void build_graph(Ctx& c)
   M.graph = true;
 }
 
+static void clear_samples(Mpnn& M)
+{
+  M.nsamp = 0;
+  M.s_rr.clear();
+  M.s_sys.clear();
+  M.sys.clear();
+  M.sys_version = -1;
+}
+
 // The graph follows the mesh topology; a mesh of another topology rebuilds it and drops
-// the stored training samples (they belong to the old graph).
+// the stored training samples (they belong to the old graph and its numbering).
 static void need_graph(Ctx& c)
 {
   if (!c.prepared || c.csr_only) fail(RMX_ERR_ARG, "learned preconditioner: needs a prepared mesh");
@@ -559,8 +570,9 @@ static void need_graph(Ctx& c)
   if (!M.graph || M.topo != c.topo_hash) {
     build_graph(c);
     M.topo = c.topo_hash;
-    M.nsamp = 0;
-    M.s_rr.clear();
+    clear_samples(M);
+    M.cap = 0;                                 // the next sample sizes the set again
+    M.s_r.release();
     M.tn = 0;
   }
 }
@@ -639,34 +651,39 @@ static void forward(Ctx& c, const float* feat, bool train)
   HIPCHECK(hipGetLastError());
 }
 
-// loss ||r - A n||^2 / ||r||^2 and its gradient (into M.grad) for one stored sample
-static double backward(Ctx& c, int s)
+// Training loss of one stored sample s, ||r - A n(r)||^2 / ||r||^2 with A, the row scaling
+// and the network outputs (M.out, from a training forward pass of the sample's system) of its
+// system: ||r - A n||^2 goes to the device double *loss, and w * d loss / d (D, E) is written
+// to M.gout (add = false) or added to it (add = true).
+static void sample_out_grad(Ctx& c, int s, double w, bool add, double* loss)
 {
   Mpnn& M = c.mpnn;
   const int np = M.np, n = c.n;
-  const size_t nh = (size_t)np * MC;
-  const float* feat = M.s_feat.p + (size_t)s * np * MF;
+  const MpnnSys& S = *M.sys[M.s_sys[s]];
   const double* r = M.s_r.p + (size_t)s * n;
-  const double* A = M.s_A.p + (size_t)s * c.nnz;
-  forward(c, feat, true);
-  M.gh.alloc(nh); M.ga.alloc(nh); M.gp.alloc(nh); M.gout.alloc((size_t)np * MO);
-  M.ev.alloc(4 * (size_t)n);
+  M.ev.alloc(3 * (size_t)n);
   double* nv = M.ev.p;
   double* res = M.ev.p + n;
   double* qs = M.ev.p + 2 * (size_t)n;
-  double* ri = M.ev.p + 3 * (size_t)n;
-  k_rowinv<<<gb(n), 256, 0, c.stream>>>(n, c.rowptr.p, A, ri);
-  k_mul<<<gb(n), 256, 0, c.stream>>>(n, r, ri, qs);
+  k_mul<<<gb(n), 256, 0, c.stream>>>(n, r, S.rinv.p, qs);
   k_lin_apply<<<gb(np), 256, 0, c.stream>>>(np, c.eqno.p, M.adj_ptr.p, M.adj.p, M.out.p, qs, nv);
-  k_resid_s<<<gb((long long)n * 8), 256, 0, c.stream>>>(n, c.rowptr.p, c.cols.p, A, nv, r, res);
-  M.lpart.alloc(1);
+  k_resid_s<<<gb((long long)n * 8), 256, 0, c.stream>>>(n, c.rowptr.p, c.cols.p, S.A.p, nv, r, res);
   const double* xs[1] = { res };
-  dev_dots(c, 1, xs, xs, M.lpart.p);
-  const double rr = M.s_rr[s];
+  dev_dots(c, 1, xs, xs, loss);
   // d loss / d n = -2 A^T res / ||r||^2, into nv; then d loss / d (D, E)
-  k_spmv_t<<<gb((long long)n * 8), 256, 0, c.stream>>>(n, M.tptr.p, M.trow.p, M.tperm.p, A, res,
-                                                       -2.0 / rr, nv);
-  k_lin_grad<<<gb(np), 256, 0, c.stream>>>(np, c.eqno.p, M.adj_ptr.p, M.adj.p, qs, nv, M.gout.p);
+  k_spmv_t<<<gb((long long)n * 8), 256, 0, c.stream>>>(n, M.tptr.p, M.trow.p, M.tperm.p, S.A.p, res,
+                                                       -2.0 * w / M.s_rr[s], nv);
+  k_lin_grad<<<gb(np), 256, 0, c.stream>>>(np, c.eqno.p, M.adj_ptr.p, M.adj.p, qs, nv, add ? 1 : 0,
+                                           M.gout.p);
+}
+
+// Gradient of the weights (into M.grad) from d loss / d (D, E) in M.gout, back through the
+// network of the last training forward pass (node features feat).
+static void net_backward(Ctx& c, const float* feat)
+{
+  Mpnn& M = c.mpnn;
+  const int np = M.np;
+  const size_t nh = (size_t)np * MC;
   const int nchunk = gb(np, CHUNK), nw = gb(np, WCHUNK);
   M.partial.alloc(std::max<size_t>(std::max<size_t>((size_t)nchunk * (MF + 1) * MC,
                                                      (size_t)nchunk * (MC + 1) * MO),
@@ -693,10 +710,15 @@ static double backward(Ctx& c, int s)
   k_reduce<<<gb((MF + 1) * MC), 256, 0, c.stream>>>(nchunk, (MF + 1) * MC, M.partial.p, MF, MC,
                                                     OFF_WIN, OFF_BIN, M.grad.p);
   HIPCHECK(hipGetLastError());
-  double loss = 0.0;
-  HIPCHECK(hipMemcpyAsync(&loss, M.lpart.p, sizeof(double), hipMemcpyDeviceToHost, c.stream));
-  HIPCHECK(hipStreamSynchronize(c.stream));
-  return loss / rr;
+}
+
+// training forward pass of stored system k (network outputs into M.out)
+static void sys_forward(Ctx& c, int k)
+{
+  Mpnn& M = c.mpnn;
+  const size_t nh = (size_t)M.np * MC;
+  forward(c, M.sys[k]->feat.p, true);
+  M.gh.alloc(nh); M.ga.alloc(nh); M.gp.alloc(nh); M.gout.alloc((size_t)M.np * MO);
 }
 
 bool mpnn_ready(Ctx& c) { return c.mpnn.weights; }
@@ -725,7 +747,8 @@ void mpnn_get_weights(Ctx& c, float* th, int n)
   HIPCHECK(hipStreamSynchronize(c.stream));
 }
 
-// z = M2^-1 r + RMS(r) * network(r)   (device pointers)
+// z = z2 + n(r - A z2) with z2 = M2^-1 r, the two-level preconditioner's output, and n the
+// network's correction, linear in its argument (device pointers)
 void mpnn_apply(Ctx& c, int prec, const double* r, double* z)
 {
   if (!mpnn_ready(c)) fail(RMX_ERR_ARG, "learned preconditioner not loaded");
@@ -795,8 +818,11 @@ static void build_transpose(Ctx& c)
   M.tn = n;
 }
 
-// store a training sample: features of (state, r), r itself and the current FP64 matrix
-// (device pointer r)
+// Store a training sample: the residual r (device pointer).  The first sample of a Newton
+// system (a matrix version not stored yet) also stores the system: the current FP64 matrix,
+// its inverse row maxima and the node features of the current state; the later samples of
+// the system share them.  capacity > 0 empties the set (samples and systems) and sizes it
+// for `capacity` samples.
 int mpnn_add_sample(Ctx& c, const double* r, int capacity)
 {
   need_graph(c);
@@ -804,23 +830,31 @@ int mpnn_add_sample(Ctx& c, const double* r, int capacity)
   Mpnn& M = c.mpnn;
   const int np = M.np, n = c.n;
   if (M.tn != n) build_transpose(c);
-  if (capacity > 0 && capacity != M.cap) {
+  if (capacity > 0) {
+    clear_samples(M);
     M.cap = capacity;
-    M.s_feat.alloc((size_t)capacity * np * MF);
     M.s_r.alloc((size_t)capacity * n);
-    M.s_A.alloc((size_t)capacity * c.nnz);
   }
-  if (capacity > 0) { M.nsamp = 0; M.s_rr.clear(); }
-  M.lpart.alloc(1);
   if (M.nsamp >= M.cap) fail(RMX_ERR_RANGE, "learned preconditioner: training set full");
-  const int s = M.nsamp;
-  features(c);
+  if (M.s_r.n != (size_t)M.cap * n)
+    fail(RMX_ERR_ARG, "learned preconditioner: the training set holds systems of another size");
   hipStream_t st = c.stream;
-  HIPCHECK(hipMemcpyAsync(M.s_feat.p + (size_t)s * np * MF, M.feat.p, sizeof(float) * np * MF,
-                          hipMemcpyDeviceToDevice, st));
+  if (M.sys.empty() || M.sys_version != c.mat_version) {
+    auto S = std::make_unique<MpnnSys>();
+    features(c);
+    S->feat.alloc((size_t)np * MF);
+    S->A.alloc(c.nnz);
+    S->rinv.alloc(n);
+    HIPCHECK(hipMemcpyAsync(S->feat.p, M.feat.p, sizeof(float) * np * MF, hipMemcpyDeviceToDevice, st));
+    HIPCHECK(hipMemcpyAsync(S->A.p, c.A64.p, sizeof(double) * c.nnz, hipMemcpyDeviceToDevice, st));
+    k_rowinv<<<gb(n), 256, 0, st>>>(n, c.rowptr.p, c.A64.p, S->rinv.p);
+    HIPCHECK(hipGetLastError());
+    M.sys.push_back(std::move(S));
+    M.sys_version = c.mat_version;
+  }
+  const int s = M.nsamp;
+  M.lpart.alloc(1);
   HIPCHECK(hipMemcpyAsync(M.s_r.p + (size_t)s * n, r, sizeof(double) * n, hipMemcpyDeviceToDevice, st));
-  HIPCHECK(hipMemcpyAsync(M.s_A.p + (size_t)s * c.nnz, c.A64.p, sizeof(double) * c.nnz,
-                          hipMemcpyDeviceToDevice, st));
   double rr = 0.0;
   const double* xs[1] = { r };
   dev_dots(c, 1, xs, xs, M.lpart.p);
@@ -828,48 +862,71 @@ int mpnn_add_sample(Ctx& c, const double* r, int capacity)
   HIPCHECK(hipStreamSynchronize(st));
   if (!(rr > 0.0)) fail(RMX_ERR_ARG, "learned preconditioner: zero residual sample");
   M.s_rr.push_back(rr);
+  M.s_sys.push_back((int)M.sys.size() - 1);
   return ++M.nsamp;
 }
 
+// loss of stored sample s and its gradient (host grad, optional)
 double mpnn_loss_grad(Ctx& c, int s, float* grad)
 {
   Mpnn& M = c.mpnn;
   if (!M.weights) fail(RMX_ERR_ARG, "learned preconditioner: no weights");
   if (s < 0 || s >= M.nsamp) fail(RMX_ERR_ARG, "learned preconditioner: no such sample");
   need_graph(c);
-  const double loss = backward(c, s);
-  if (grad) {
-    M.grad.download(grad, NPARAM, c.stream);
-    HIPCHECK(hipStreamSynchronize(c.stream));
-  }
-  return loss;
+  const int k = M.s_sys[s];
+  sys_forward(c, k);
+  M.lpart.alloc(1);
+  sample_out_grad(c, s, 1.0, false, M.lpart.p);
+  net_backward(c, M.sys[k]->feat.p);
+  double loss = 0.0;
+  HIPCHECK(hipMemcpyAsync(&loss, M.lpart.p, sizeof(double), hipMemcpyDeviceToHost, c.stream));
+  if (grad) M.grad.download(grad, NPARAM, c.stream);
+  HIPCHECK(hipStreamSynchronize(c.stream));
+  M.trunk_version = -1;
+  return loss / M.s_rr[s];
 }
 
-// Adam over the stored samples, one sample per step in order, `epochs` passes; returns
-// the mean loss of the last pass
+// Adam over the stored systems, `epochs` passes over them in order: one step per system, on
+// the mean loss of its samples (the network sees only the system's state, so one forward and
+// one backward pass serve all its samples).  Returns the mean sample loss of the last pass.
 double mpnn_train(Ctx& c, int epochs, double lr)
 {
   Mpnn& M = c.mpnn;
   if (!M.weights) fail(RMX_ERR_ARG, "learned preconditioner: no weights");
   if (M.nsamp == 0) fail(RMX_ERR_ARG, "learned preconditioner: no training samples");
   need_graph(c);
+  const int nsys = (int)M.sys.size();
+  std::vector<std::vector<int>> of(nsys);
+  size_t most = 1;
+  for (int s = 0; s < M.nsamp; s++) of[M.s_sys[s]].push_back(s);
+  for (const auto& v : of) most = std::max(most, v.size());
+  M.lpart.alloc(most);
+  std::vector<double> lh(most);
   const float b1 = 0.9f, b2 = 0.999f, eps = 1e-8f;
   double last = 0.0;
   for (int ep = 0; ep < epochs; ep++) {
     double sum = 0.0;
-    for (int s = 0; s < M.nsamp; s++) {
-      sum += backward(c, s);
+    for (int k = 0; k < nsys; k++) {
+      const std::vector<int>& ss = of[k];
+      if (ss.empty()) continue;
+      sys_forward(c, k);
+      for (size_t j = 0; j < ss.size(); j++)
+        sample_out_grad(c, ss[j], 1.0 / (double)ss.size(), j > 0, M.lpart.p + j);
+      net_backward(c, M.sys[k]->feat.p);
       M.step++;
       const float c1 = 1.0f - std::pow(b1, (float)M.step), c2 = 1.0f - std::pow(b2, (float)M.step);
       k_adam<<<gb(NPARAM), 256, 0, c.stream>>>(NPARAM, M.theta.p, M.grad.p, M.m.p, M.v.p, (float)lr,
                                                b1, b2, c1, c2, eps);
       k_refresh<<<gb(ML * 2 * MC * MC), 256, 0, c.stream>>>(M.theta.p, M.wt.p, M.wb.p);
+      HIPCHECK(hipGetLastError());
+      HIPCHECK(hipMemcpyAsync(lh.data(), M.lpart.p, sizeof(double) * ss.size(), hipMemcpyDeviceToHost,
+                              c.stream));
+      HIPCHECK(hipStreamSynchronize(c.stream));
+      for (size_t j = 0; j < ss.size(); j++) sum += lh[j] / M.s_rr[ss[j]];
     }
     last = sum / M.nsamp;
   }
   M.trunk_version = -1;
-  HIPCHECK(hipGetLastError());
-  HIPCHECK(hipStreamSynchronize(c.stream));
   return last;
 }
 

@@ -5,6 +5,7 @@
 
 #include <cstdint>
 #include <functional>
+#include <memory>
 #include <string>
 #include <vector>
 
@@ -243,19 +244,30 @@ struct Coarse {
   ~Coarse() { if (h_buf) (void)hipHostFree(h_buf); }
 };
 
+// One stored Newton system of the training set: its FP64 matrix, the inverse row maxima of
+// that matrix and the node features of its state, shared by the samples taken from it.
+struct MpnnSys {
+  DBuf<double> A, rinv;
+  DBuf<float> feat;
+};
+
 // Learned preconditioner: mesh node graph, weights (FP32 master + BF16 layer copies),
-// activations, gradients, Adam moments, stored training samples.
+// activations, gradients, Adam moments, stored training samples (residual vectors, each
+// naming the stored system it was taken from).
 struct Mpnn {
   static constexpr int C = 64, L = 6, F = 8, O = 18;
   bool graph = false, weights = false;
   int np = 0, step = 0, cap = 0, nsamp = 0, tn = 0;
   unsigned long long topo = 0;                // topology fingerprint of the graph's mesh
   std::vector<double> s_rr;                   // ||r||^2 of every stored sample
+  std::vector<int> s_sys;                     // stored system of every sample
+  std::vector<std::unique_ptr<MpnnSys>> sys;  // stored systems, in the order they arrived
+  long long sys_version = -1;                 // matrix version of the last stored system
   DBuf<int> adj_ptr, adj, tptr, trow, tperm;  // node graph; transpose of the system pattern
   DBuf<float> theta, grad, m, v;
   DBuf<uint16_t> wt, wb, H, A, gp;
-  DBuf<float> P, Q, feat, out, gh, ga, gout, partial, s_feat;
-  DBuf<double> lpart, ev, rv, s_r, s_A, rinv, q, acc;
+  DBuf<float> P, Q, feat, out, gh, ga, gout, partial;
+  DBuf<double> lpart, ev, rv, s_r, rinv, q, acc;
   long long rinv_version = -1, trunk_version = -1;
   int use_net = -1;                           // per system: -1 undecided, 0 dropped, 1 used
 };

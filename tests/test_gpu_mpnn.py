@@ -114,6 +114,66 @@ def test_mpnn_gradient_matches_oracle(ctx):
         k0 += m
 
 
+def test_training_set_stores_systems_and_steps_per_system(ctx):
+    """Samples of one Newton system share its stored matrix and features: a pass over a
+    system is one Adam step on the mean gradient of its samples; a sample keeps its own
+    system's loss after another system arrives, and a sample of the second system matches
+    the oracle on that system's matrix and state."""
+    print("[gpu] MPNN training set: stored systems", flush=True)
+    deck, B = _setup(ctx)
+    # from the zero output layer of the initialisation, the gradient of each sample is the
+    # output layer's direction that best reduces that sample's residual, which differs in
+    # sign between unlike residuals
+    theta = mpnn.init_weights(5)
+    ctx.mpnn_set_weights(theta)
+    rng = np.random.default_rng(3)
+    B2 = np.abs(B).max() * rng.standard_normal(len(B))
+    assert ctx.mpnn_add_sample(B, capacity=3) == 1
+    assert ctx.mpnn_add_sample(B2) == 2
+    l0, g0 = ctx.mpnn_loss_grad(0)
+    l1, g1 = ctx.mpnn_loss_grad(1)
+    # the first Adam step moves every weight by lr * g / (|g| + 1e-8), g the mean gradient
+    lr = 1e-3
+    loss = ctx.mpnn_train(1, lr)
+    assert abs(loss - 0.5 * (l0 + l1)) <= 1e-9 * loss
+    gm = 0.5 * (g0.astype(np.float64) + g1)
+    d = ctx.mpnn_get_weights().astype(np.float64) - theta
+    big = np.abs(gm) > 3e-2 * np.abs(gm).max()
+    agree = np.mean(np.sign(d[big]) == -np.sign(gm[big]))
+    only0 = np.mean(np.sign(g0[big]) != np.sign(gm[big]))     # what a step on one sample misses
+    only1 = np.mean(np.sign(g1[big]) != np.sign(gm[big]))
+    print(f"    {big.sum()} weights: step along the mean gradient on {agree:.4f}; a single "
+          f"sample's gradient differs in sign on {only0:.4f} / {only1:.4f}")
+    assert big.sum() >= 100 and max(only0, only1) > 0.005
+    assert agree >= 0.995
+    np.testing.assert_allclose(np.abs(d[big]), lr, rtol=1e-2)
+    assert np.abs(d).max() <= lr * 1.001
+    # a second system (the next Newton state), with weights that give every block gradient
+    theta = _random_weights(5)
+    ctx.mpnn_set_weights(theta)
+    l0, g0 = ctx.mpnn_loss_grad(0)
+    ctx.newton_step(method=rmx.FGMRES, prec=rmx.F32, precond=rmx.PREC_SILU0C, eps=1e-3,
+                    refine=True, refine_tol=1e-8, refine_max=20)
+    ctx.prepare()
+    ctx.assemble(rmx.F64)
+    ctx.scale_l2()
+    _, Bn = ctx.matrix(rmx.F64)
+    assert ctx.mpnn_add_sample(Bn) == 3
+    l0b, g0b = ctx.mpnn_loss_grad(0)
+    assert l0b == l0 and np.array_equal(g0b, g0)
+    loss, g = ctx.mpnn_loss_grad(2)
+    _, feat = ctx.mpnn_forward(Bn)
+    ptr, adj = _graph(deck)
+    rp, cols = ctx.pattern()
+    A, _ = ctx.matrix(rmx.F64)
+    lo, go = mo.resid_loss_grad(theta.astype(np.float64), feat.astype(np.float64), Bn, rp, cols,
+                                A, ctx.numbering(), ptr, adj, bf=True)
+    gerr = np.linalg.norm(g - go) / np.linalg.norm(go)
+    print(f"    second system: loss {loss:.6e} vs {lo:.6e}; gradient rel L2 err {gerr:.2e}")
+    assert abs(loss - lo) <= 1e-3 * lo and gerr <= 2e-2
+    assert abs(loss - l0) > 1e-3 * l0
+
+
 def test_training_on_held_out_seed_against_two_level(ctx):
     """Train on residual samples harvested from Newton steps of seeded bathymetries of
     the channel generator; on a held-out seed, compare the two-level preconditioner with
@@ -132,7 +192,7 @@ def test_training_on_held_out_seed_against_two_level(ctx):
     assert n == 6 * len(seeds)
     ctx.mpnn_set_weights(mpnn.init_weights(0))
     l0 = np.mean([ctx.mpnn_loss_grad(k)[0] for k in range(n)])
-    loss = mpnn.train(ctx, epochs=30, lr=1e-3)
+    loss = mpnn.train(ctx, epochs=90, lr=1e-3)       # 12 systems: 1,080 Adam steps
     print(f"    training loss {l0:.4e} -> {loss:.4e}")
     assert loss < 0.9 * l0
     # held-out seed
