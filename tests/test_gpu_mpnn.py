@@ -226,3 +226,44 @@ def test_training_on_held_out_seed_against_two_level(ctx):
     for pk, out in runs.items():
         assert out.converged and out.true_resid <= 1e-8, pk
     assert runs[rmx.PREC_SILU0C].fallback == 0
+
+
+def test_shipped_weights_on_held_out_meander_reach(ctx):
+    """The weights shipped with the package (trained on 204,800-triangle reaches of the
+    meander generator, seeds 1-64) on a held-out seed's 80 x 320-cell reach after the first
+    drying and rewetting call: one application with the network leaves a smaller residual
+    than the two-level preconditioner alone (the training loss of a seed's first samples
+    before training on it was about 0.8), and both FGMRES(15) solves reach the FP64
+    residual 1e-8.  The iteration counts and times are printed: the learned solve is not
+    faster."""
+    import os
+    print("[gpu] MPNN shipped weights, held-out meander reach", flush=True)
+    w = os.path.join(os.path.dirname(mpnn.__file__), "mpnn_meander.npy")
+    theta = np.load(w)
+    assert theta.dtype == np.float32 and len(theta) == ctx.mpnn_n_params()
+    d = mpnn.meander_deck(0, 80, 0.012)
+    ctx.upload_mesh(d)
+    ctx.set_params()
+    ctx.prepare()
+    dried, _ = ctx.dry_rewet()
+    assert dried > 0
+    ctx.mpnn_set_weights(theta)
+    ctx.assemble(rmx.F64)
+    ctx.scale_l2()
+    ctx.convert(rmx.F32)
+    ctx.silu0_factor(rmx.F32)
+    ctx.coarse_factor(rmx.F32)
+    _, B = ctx.matrix(rmx.F64)
+    z2 = ctx.precond_apply(rmx.PREC_SILU0C, B, rmx.F32)
+    zl = ctx.precond_apply(rmx.PREC_MPNN, B, rmx.F32)
+    r2 = np.linalg.norm(B - ctx.spmv(z2, rmx.F64)) / np.linalg.norm(B)
+    rl = np.linalg.norm(B - ctx.spmv(zl, rmx.F64)) / np.linalg.norm(B)
+    print(f"    {ctx.n} equations; residual left by one application {r2:.4e} (two-level) -> "
+          f"{rl:.4e} (with network)", flush=True)
+    assert rl < r2
+    for pk in (rmx.PREC_SILU0C, rmx.PREC_MPNN):
+        out = ctx.solve(rmx.FGMRES, rmx.F32, pk, max_iter=1000, mkyrl=15, eps=1e-2, refine=True,
+                        refine_tol=1e-8, refine_max=8)
+        print(f"    precond {pk}: FGMRES {out.iters} it, {out.sweeps} sweeps, fallback "
+              f"{out.fallback}, FP64 residual {out.true_resid:.2e}, {out.seconds:.3f}s", flush=True)
+        assert out.converged and out.true_resid <= 1e-8, pk
