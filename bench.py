@@ -430,7 +430,7 @@ def main():
         learned = {"weights": os.path.relpath(a.learned, HERE), "s_per_newton_step": tl,
                    "converged": bool(st.solve.converged), "true_resid": st.solve.true_resid,
                    "krylov_iters": st.solve.iters, "refine_sweeps": st.solve.sweeps,
-                   "fallback_to_bicgstab": bool(st.solve.fallback), "t_solve_s": st.t_solve,
+                   "fallback": st.solve.fallback, "t_solve_s": st.t_solve,
                    "vs_two_level_step": tl / T,
                    "krylov_iters_two_level": last.solve.iters}
     else:

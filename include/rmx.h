@@ -340,9 +340,9 @@ typedef struct {
   double accuracy;            /* last Krylov relative residual estimate */
   double true_resid;          /* ||B - A64 X||_2 / ||B||_2 in FP64 */
   double seconds;             /* wall time of the solve on the device stream */
-  int fallback;               /* 1: a refinement sweep stalled and the remaining sweeps ran
-                                 BiCGStab (with the two-level preconditioner after the
-                                 learned one) */
+  int fallback;               /* refinement sweeps that stalled stepped down: 1 to the
+                                 two-level preconditioner in place of the learned one, or
+                                 to FGMRES restart 30; 2 to BiCGStab */
   int solvers;                /* solvers of the proceed chain that ran (1 without it) */
   int interrupted;            /* 1: not converged with proceed = -1 (X is zero) */
 } rmx_krylov_out;
@@ -351,10 +351,11 @@ typedef struct {
  * (P_bcgstabd.cpp:98) or P_FGMRESD (P_fgmresd.cpp:90) and the ILU(0) preconditioner.
  * X starts at zero.  With refine = 1 the solve runs in the requested storage precision
  * inside an FP64 iterative refinement on the FP64 matrix; a refinement sweep that lowers
- * the true FP64 residual less than tenfold switches the remaining sweeps to BiCGStab (with
- * the two-level preconditioner in place of the learned one; out->fallback = 1).  FGMRES
- * ends a solve whose restart cycle made no progress (restarted GMRES can stagnate on these
- * indefinite systems).  A solve that does not converge follows opt->proceed. */
+ * the true FP64 residual less than tenfold steps the remaining sweeps down one rung: the
+ * two-level preconditioner in place of the learned one, then FGMRES with restart 30 in
+ * place of a shorter restart (out->fallback = 1), then BiCGStab (out->fallback = 2).
+ * FGMRES ends a solve whose restart cycle made no progress (restarted GMRES can stagnate on
+ * these indefinite systems).  A solve that does not converge follows opt->proceed. */
 rmx_status rmx_solve(rmx_ctx* ctx, const rmx_krylov_opts* opt, rmx_krylov_out* out);
 
 /* Host access to the system (tests, drop-in callers that keep host arrays). */

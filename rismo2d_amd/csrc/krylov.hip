@@ -561,6 +561,7 @@ static void solve_one(Ctx& c, const rmx_krylov_opts& o, bool from_zero, rmx_kryl
   double* xr = c.X.p;
   double* r = c.w[7].p;
   double* d = c.w[8].p;
+  double* xkeep = c.w[9].p;                    // X before the running sweep
   double tr = 1.0;
   if (from_zero) {
     HIPCHECK(hipMemsetAsync(xr, 0, sizeof(double) * c.n, c.stream));
@@ -571,7 +572,7 @@ static void solve_one(Ctx& c, const rmx_krylov_opts& o, bool from_zero, rmx_kryl
   int total = 0;
   const int sweeps_max = o.refine_max > 0 ? o.refine_max : 10;
   int sw = 0;
-  int method = o.method, kind = pk;
+  int method = o.method, kind = pk, mk = o.mkyrl;
   if (tr > o.refine_tol) {
     for (; sw < sweeps_max; sw++) {
       int it = 0; double acc = 1.0;
@@ -579,22 +580,44 @@ static void solve_one(Ctx& c, const rmx_krylov_opts& o, bool from_zero, rmx_kryl
       // a sweep solves for the remaining factor only: the inner tolerance is relaxed to half
       // of what takes the true residual from tr to refine_tol, never tightened below eps
       const double eps_k = std::max(o.eps, 0.5 * o.refine_tol / tr);
-      krylov(c, method, prec, kind, r, d, o.max_iter, std::min(eps_k, 0.5), o.mkyrl, it, acc);
+      krylov(c, method, prec, kind, r, d, o.max_iter, std::min(eps_k, 0.5), mk, it, acc);
       total += it;
       out.accuracy = acc;
+      k_copy<<<vblocks(c.n), 256, 0, c.stream>>>(c.n, xr, xkeep);
       k_axpy<<<vblocks(c.n), 256, 0, c.stream>>>(c.n, 1.0, d, xr);
       tr = true_resid(c, xr, r);
       if (getenv("RMX_DEBUG"))
         fprintf(stderr, "[rmx] refinement sweep %d: %d iterations (inner %.2e), FP64 residual %.3e\n",
                 sw + 1, it, acc, tr);
-      if (tr <= o.refine_tol || !std::isfinite(tr)) { sw++; break; }
+      if (tr <= o.refine_tol) { sw++; break; }
+      if (!(tr < tr_prev)) {
+        // the sweep raised the true residual (a diverging inner solve): undo it, so that X
+        // is the best iterate; the same solver from the same residual would repeat it, so
+        // the solve ends unless a rung below is left
+        k_copy<<<vblocks(c.n), 256, 0, c.stream>>>(c.n, xkeep, xr);
+        tr = true_resid(c, xr, r);
+        if (getenv("RMX_DEBUG")) fprintf(stderr, "[rmx] refinement sweep %d undone\n", sw + 1);
+        if (method != RMX_FGMRES && kind != RMX_PREC_MPNN) { sw++; break; }
+      }
       if (!(tr < 0.1 * tr_prev) && (method == RMX_FGMRES || kind == RMX_PREC_MPNN)) {
-        // the sweep stalled (FGMRES stagnating, or the learned preconditioner): the remaining
-        // sweeps run BiCGStab, with the two-level preconditioner in place of the learned one
-        method = RMX_BICGSTAB;
-        if (kind == RMX_PREC_MPNN) kind = RMX_PREC_SILU0C;
-        out.fallback = 1;
-        if (getenv("RMX_DEBUG")) fprintf(stderr, "[rmx] refinement sweep stalled: BiCGStab\n");
+        // the sweep stalled (the learned preconditioner, or FGMRES stagnating): the
+        // remaining sweeps step down one rung: the two-level preconditioner in place of the
+        // learned one; then a short FGMRES restart raised to the longest (restarted GMRES
+        // stagnates on these indefinite systems where a longer restart still converges);
+        // then BiCGStab
+        if (kind == RMX_PREC_MPNN) {
+          kind = RMX_PREC_SILU0C;
+          out.fallback = std::max(out.fallback, 1);
+        } else if (mk < MDOT_MAX - 2) {
+          mk = MDOT_MAX - 2;
+          out.fallback = std::max(out.fallback, 1);
+        } else {
+          method = RMX_BICGSTAB;
+          out.fallback = 2;
+        }
+        if (getenv("RMX_DEBUG"))
+          fprintf(stderr, "[rmx] refinement sweep stalled: %s(%d), preconditioner %d\n",
+                  method == RMX_FGMRES ? "FGMRES" : "BiCGStab", mk, kind);
       }
     }
   }
@@ -618,7 +641,7 @@ void dev_solve(Ctx& c, const rmx_krylov_opts& o0, rmx_krylov_out& out)
     solvers++;
     iters += one.iters;
     sweeps += one.sweeps;
-    fallback |= one.fallback;
+    fallback = std::max(fallback, one.fallback);
     if (one.converged || o->proceed == 0) break;
     if (o->proceed < 0) {
       // the reference sets X to zero and interrupts (Solve.cpp:345-353)

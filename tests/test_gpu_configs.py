@@ -132,6 +132,39 @@ def test_full_size_meander_newton_step_reaches_fp64_residual(ctx, rise):
     assert 0.0 < st.maxAbs[0] < 10.0
 
 
+@pytest.mark.parametrize("seed", [1, 4])
+def test_stalled_sweeps_step_down_and_keep_the_best_iterate(seed):
+    """The second Newton system of a dried meander reach (80 x 320 cells, floodplains rising
+    0.012 m/m, no relaxation): FGMRES(15) with the two-level preconditioner stagnates on it.
+    The stalled sweep steps the remaining sweeps down: seed 1 then converges with restart
+    30 (fallback 1).  On seed 4 every rung fails and BiCGStab diverges: that sweep is undone,
+    so the solve ends with its best iterate (finite, true residual below the start) and the
+    Newton update stays finite."""
+    print(f"[gpu] meander 80x320 seed {seed}: second Newton step, stalled sweeps", flush=True)
+    d = dk.meander(n_along=80, n_across=320, reach=250.0, seed=seed, plain_rise=0.012)
+    ctx = rmx.Context(0)                       # default strips and coarse space
+    ctx.upload_mesh(d)
+    ctx.set_params()
+    ctx.prepare()
+    ctx.dry_rewet()
+    opts = dict(FAST, refine_tol=1e-8, refine_max=20, max_iter=1000)
+    st = ctx.newton_step(**opts)
+    assert st.solve.converged and st.solve.fallback == 0
+    st = ctx.newton_step(**opts)
+    print(f"    {st.solve.iters} iterations in {st.solve.sweeps} sweeps, fallback "
+          f"{st.solve.fallback}, converged {st.solve.converged}, true FP64 residual "
+          f"{st.solve.true_resid:.2e}", flush=True)
+    U, V, S, z = ctx.state()
+    assert np.isfinite(U).all() and np.isfinite(V).all() and np.isfinite(S).all()
+    assert np.isfinite(st.maxAbs).all()
+    if seed == 1:
+        assert st.solve.converged and st.solve.true_resid <= 1e-8 and st.solve.fallback == 1
+    else:
+        assert not st.solve.converged and st.solve.fallback == 2
+        assert st.solve.true_resid < 1.0
+    ctx.close()
+
+
 def test_vegetated_channel_fp32_refinement_reaches_reference(ctx):
     print("[gpu] channel 200x20 with vegetation, FP32 inside FP64 refinement", flush=True)
     g = load_golden("vegchannel200x20")
