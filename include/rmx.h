@@ -353,8 +353,9 @@ typedef struct {
  * inside an FP64 iterative refinement on the FP64 matrix; a refinement sweep that lowers
  * the true FP64 residual less than tenfold steps the remaining sweeps down one rung: the
  * two-level preconditioner in place of the learned one, then FGMRES with restart 30 in
- * place of a shorter restart (out->fallback = 1), then BiCGStab (out->fallback = 2).
- * FGMRES ends a solve whose restart cycle made no progress (restarted GMRES can stagnate on
+ * place of a shorter restart (out->fallback = 1), then BiCGStab (out->fallback = 2).  X is
+ * the refinement's iterate of the smallest true residual; a sweep that blows the residual
+ * up (non-finite, or 1e4 times the smallest) ends the refinement.  FGMRES ends a solve whose restart cycle made no progress (restarted GMRES can stagnate on
  * these indefinite systems).  A solve that does not converge follows opt->proceed. */
 rmx_status rmx_solve(rmx_ctx* ctx, const rmx_krylov_opts* opt, rmx_krylov_out* out);
 

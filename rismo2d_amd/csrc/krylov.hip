@@ -561,7 +561,7 @@ static void solve_one(Ctx& c, const rmx_krylov_opts& o, bool from_zero, rmx_kryl
   double* xr = c.X.p;
   double* r = c.w[7].p;
   double* d = c.w[8].p;
-  double* xkeep = c.w[9].p;                    // X before the running sweep
+  double* xbest = c.w[9].p;                    // the iterate of the smallest true residual
   double tr = 1.0;
   if (from_zero) {
     HIPCHECK(hipMemsetAsync(xr, 0, sizeof(double) * c.n, c.stream));
@@ -573,7 +573,9 @@ static void solve_one(Ctx& c, const rmx_krylov_opts& o, bool from_zero, rmx_kryl
   const int sweeps_max = o.refine_max > 0 ? o.refine_max : 10;
   int sw = 0;
   int method = o.method, kind = pk, mk = o.mkyrl;
+  double best = tr;
   if (tr > o.refine_tol) {
+    k_copy<<<vblocks(c.n), 256, 0, c.stream>>>(c.n, xr, xbest);
     for (; sw < sweeps_max; sw++) {
       int it = 0; double acc = 1.0;
       const double tr_prev = tr;
@@ -583,22 +585,19 @@ static void solve_one(Ctx& c, const rmx_krylov_opts& o, bool from_zero, rmx_kryl
       krylov(c, method, prec, kind, r, d, o.max_iter, std::min(eps_k, 0.5), mk, it, acc);
       total += it;
       out.accuracy = acc;
-      k_copy<<<vblocks(c.n), 256, 0, c.stream>>>(c.n, xr, xkeep);
       k_axpy<<<vblocks(c.n), 256, 0, c.stream>>>(c.n, 1.0, d, xr);
       tr = true_resid(c, xr, r);
       if (getenv("RMX_DEBUG"))
         fprintf(stderr, "[rmx] refinement sweep %d: %d iterations (inner %.2e), FP64 residual %.3e\n",
                 sw + 1, it, acc, tr);
-      if (tr <= o.refine_tol) { sw++; break; }
-      if (!(tr < tr_prev)) {
-        // the sweep raised the true residual (a diverging inner solve): undo it, so that X
-        // is the best iterate; the same solver from the same residual would repeat it, so
-        // the solve ends unless a rung below is left
-        k_copy<<<vblocks(c.n), 256, 0, c.stream>>>(c.n, xkeep, xr);
-        tr = true_resid(c, xr, r);
-        if (getenv("RMX_DEBUG")) fprintf(stderr, "[rmx] refinement sweep %d undone\n", sw + 1);
-        if (method != RMX_FGMRES && kind != RMX_PREC_MPNN) { sw++; break; }
+      if (tr < best) {
+        best = tr;
+        k_copy<<<vblocks(c.n), 256, 0, c.stream>>>(c.n, xr, xbest);
       }
+      if (tr <= o.refine_tol) { sw++; break; }
+      // an inner solve that blew up (non-finite, or ten thousand times the best residual):
+      // nothing to refine from
+      if (!(tr < 1.0e4 * best)) { sw++; break; }
       if (!(tr < 0.1 * tr_prev) && (method == RMX_FGMRES || kind == RMX_PREC_MPNN)) {
         // the sweep stalled (the learned preconditioner, or FGMRES stagnating): the
         // remaining sweeps step down one rung: the two-level preconditioner in place of the
@@ -619,6 +618,11 @@ static void solve_one(Ctx& c, const rmx_krylov_opts& o, bool from_zero, rmx_kryl
           fprintf(stderr, "[rmx] refinement sweep stalled: %s(%d), preconditioner %d\n",
                   method == RMX_FGMRES ? "FGMRES" : "BiCGStab", mk, kind);
       }
+    }
+    if (best < tr || !std::isfinite(tr)) {
+      // later sweeps raised the residual: X is the best iterate
+      k_copy<<<vblocks(c.n), 256, 0, c.stream>>>(c.n, xbest, xr);
+      tr = best;
     }
   }
   out.iters = total; out.sweeps = sw; out.true_resid = tr;

@@ -137,9 +137,9 @@ def test_stalled_sweeps_step_down_and_keep_the_best_iterate(seed):
     """The second Newton system of a dried meander reach (80 x 320 cells, floodplains rising
     0.012 m/m, no relaxation): FGMRES(15) with the two-level preconditioner stagnates on it.
     The stalled sweep steps the remaining sweeps down: seed 1 then converges with restart
-    30 (fallback 1).  On seed 4 every rung fails and BiCGStab diverges: that sweep is undone,
-    so the solve ends with its best iterate (finite, true residual below the start) and the
-    Newton update stays finite."""
+    30 (fallback 1).  On seed 4 every rung fails and BiCGStab diverges: the solve returns
+    its best iterate (finite, true residual below the start), so the Newton update stays
+    finite."""
     print(f"[gpu] meander 80x320 seed {seed}: second Newton step, stalled sweeps", flush=True)
     d = dk.meander(n_along=80, n_across=320, reach=250.0, seed=seed, plain_rise=0.012)
     ctx = rmx.Context(0)                       # default strips and coarse space
