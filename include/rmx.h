@@ -340,9 +340,10 @@ typedef struct {
   double accuracy;            /* last Krylov relative residual estimate */
   double true_resid;          /* ||B - A64 X||_2 / ||B||_2 in FP64 */
   double seconds;             /* wall time of the solve on the device stream */
-  int fallback;               /* refinement sweeps that stalled stepped down: 1 to the
-                                 two-level preconditioner in place of the learned one, or
-                                 to FGMRES restart 30; 2 to BiCGStab */
+  int fallback;               /* 1: a refinement sweep stalled and the remaining sweeps ran
+                                 the two-level preconditioner in place of the learned one,
+                                 or BiCGStab in place of FGMRES; 2: a sweep raised the
+                                 residual and the rest ran FGMRES(30) from the best iterate */
   int solvers;                /* solvers of the proceed chain that ran (1 without it) */
   int interrupted;            /* 1: not converged with proceed = -1 (X is zero) */
 } rmx_krylov_out;
@@ -351,11 +352,11 @@ typedef struct {
  * (P_bcgstabd.cpp:98) or P_FGMRESD (P_fgmresd.cpp:90) and the ILU(0) preconditioner.
  * X starts at zero.  With refine = 1 the solve runs in the requested storage precision
  * inside an FP64 iterative refinement on the FP64 matrix; a refinement sweep that lowers
- * the true FP64 residual less than tenfold steps the remaining sweeps down one rung: the
- * two-level preconditioner in place of the learned one, then FGMRES with restart 30 in
- * place of a shorter restart (out->fallback = 1), then BiCGStab (out->fallback = 2).  X is
- * the refinement's iterate of the smallest true residual; a sweep that blows the residual
- * up (non-finite, or 1e4 times the smallest) ends the refinement.  FGMRES ends a solve whose restart cycle made no progress (restarted GMRES can stagnate on
+ * the true FP64 residual less than tenfold switches the remaining sweeps to the two-level
+ * preconditioner in place of the learned one, or from FGMRES to BiCGStab
+ * (out->fallback = 1).  A sweep that raises it tenfold or more is undone and the remaining
+ * sweeps run FGMRES with restart 30 (out->fallback = 2); if that has run, the refinement
+ * ends.  X is the refinement's iterate of the smallest true residual.  FGMRES ends a solve whose restart cycle made no progress (restarted GMRES can stagnate on
  * these indefinite systems).  A solve that does not converge follows opt->proceed. */
 rmx_status rmx_solve(rmx_ctx* ctx, const rmx_krylov_opts* opt, rmx_krylov_out* out);
 

@@ -573,12 +573,14 @@ static void solve_one(Ctx& c, const rmx_krylov_opts& o, bool from_zero, rmx_kryl
   const int sweeps_max = o.refine_max > 0 ? o.refine_max : 10;
   int sw = 0;
   int method = o.method, kind = pk, mk = o.mkyrl;
+  bool bicg_tried = method == RMX_BICGSTAB;
   double best = tr;
   if (tr > o.refine_tol) {
     k_copy<<<vblocks(c.n), 256, 0, c.stream>>>(c.n, xr, xbest);
     for (; sw < sweeps_max; sw++) {
       int it = 0; double acc = 1.0;
       const double tr_prev = tr;
+      const int method_prev = method, kind_prev = kind;
       // a sweep solves for the remaining factor only: the inner tolerance is relaxed to half
       // of what takes the true residual from tr to refine_tol, never tightened below eps
       const double eps_k = std::max(o.eps, 0.5 * o.refine_tol / tr);
@@ -595,29 +597,40 @@ static void solve_one(Ctx& c, const rmx_krylov_opts& o, bool from_zero, rmx_kryl
         k_copy<<<vblocks(c.n), 256, 0, c.stream>>>(c.n, xr, xbest);
       }
       if (tr <= o.refine_tol) { sw++; break; }
-      // an inner solve that blew up (non-finite, or ten thousand times the best residual):
-      // nothing to refine from
-      if (!(tr < 1.0e4 * best)) { sw++; break; }
-      if (!(tr < 0.1 * tr_prev) && (method == RMX_FGMRES || kind == RMX_PREC_MPNN)) {
-        // the sweep stalled (the learned preconditioner, or FGMRES stagnating): the
-        // remaining sweeps step down one rung: the two-level preconditioner in place of the
-        // learned one; then a short FGMRES restart raised to the longest (restarted GMRES
-        // stagnates on these indefinite systems where a longer restart still converges);
-        // then BiCGStab
+      const bool rose = !(tr < 10.0 * tr_prev);
+      if (rose) {
+        // the sweep raised the true residual tenfold or more (a diverging inner solve): back
+        // to the best iterate, and the rest of the solve on FGMRES with the longest restart,
+        // which minimises the residual over its Krylov space; when that has run, nothing is
+        // left (a smaller rise is refined from, as any other sweep)
+        k_copy<<<vblocks(c.n), 256, 0, c.stream>>>(c.n, xbest, xr);
+        tr = true_resid(c, xr, r);
         if (kind == RMX_PREC_MPNN) {
           kind = RMX_PREC_SILU0C;
           out.fallback = std::max(out.fallback, 1);
-        } else if (mk < MDOT_MAX - 2) {
+        } else if (!(method == RMX_FGMRES && mk == MDOT_MAX - 2)) {
+          method = RMX_FGMRES;
           mk = MDOT_MAX - 2;
-          out.fallback = std::max(out.fallback, 1);
+          out.fallback = 2;
+        } else {
+          sw++;
+          break;
+        }
+      } else if (!(tr < 0.1 * tr_prev) && (kind == RMX_PREC_MPNN || (method == RMX_FGMRES && !bicg_tried))) {
+        // the sweep stalled (the learned preconditioner, or FGMRES stagnating): the two-level
+        // preconditioner in place of the learned one, then BiCGStab in place of FGMRES
+        if (kind == RMX_PREC_MPNN) {
+          kind = RMX_PREC_SILU0C;
         } else {
           method = RMX_BICGSTAB;
-          out.fallback = 2;
+          bicg_tried = true;
         }
-        if (getenv("RMX_DEBUG"))
-          fprintf(stderr, "[rmx] refinement sweep stalled: %s(%d), preconditioner %d\n",
-                  method == RMX_FGMRES ? "FGMRES" : "BiCGStab", mk, kind);
+        out.fallback = std::max(out.fallback, 1);
       }
+      if (getenv("RMX_DEBUG") && (rose || method != method_prev || kind != kind_prev))
+        fprintf(stderr, "[rmx] refinement sweep %s: %s(%d), preconditioner %d from here\n",
+                rose ? "raised the residual" : "stalled", method == RMX_FGMRES ? "FGMRES" : "BiCGStab",
+                mk, kind);
     }
     if (best < tr || !std::isfinite(tr)) {
       // later sweeps raised the residual: X is the best iterate

@@ -135,11 +135,11 @@ def test_full_size_meander_newton_step_reaches_fp64_residual(ctx, rise):
 @pytest.mark.parametrize("seed", [1, 4])
 def test_stalled_sweeps_step_down_and_keep_the_best_iterate(seed):
     """The second Newton system of a dried meander reach (80 x 320 cells, floodplains rising
-    0.012 m/m, no relaxation): FGMRES(15) with the two-level preconditioner stagnates on it.
-    The stalled sweep steps the remaining sweeps down: seed 1 then converges with restart
-    30 (fallback 1).  On seed 4 every rung fails and BiCGStab diverges: the solve returns
-    its best iterate (finite, true residual below the start), so the Newton update stays
-    finite."""
+    0.012 m/m, no relaxation): FGMRES(15) with the two-level preconditioner stagnates on it
+    and the remaining sweeps run BiCGStab (fallback 1); seed 1 converges.  On seed 4
+    BiCGStab diverges: the solve goes back to its best iterate and runs FGMRES(30) from there
+    (fallback 2); the Newton update stays finite and the solve's residual is the best one
+    reached, below the start."""
     print(f"[gpu] meander 80x320 seed {seed}: second Newton step, stalled sweeps", flush=True)
     d = dk.meander(n_along=80, n_across=320, reach=250.0, seed=seed, plain_rise=0.012)
     ctx = rmx.Context(0)                       # default strips and coarse space
@@ -158,10 +158,9 @@ def test_stalled_sweeps_step_down_and_keep_the_best_iterate(seed):
     assert np.isfinite(U).all() and np.isfinite(V).all() and np.isfinite(S).all()
     assert np.isfinite(st.maxAbs).all()
     if seed == 1:
-        assert st.solve.converged and st.solve.true_resid <= 1e-8 and st.solve.fallback == 1
+        assert st.solve.converged and st.solve.true_resid <= 1e-8 and st.solve.fallback >= 1
     else:
-        assert not st.solve.converged and st.solve.fallback == 2
-        assert st.solve.true_resid < 1.0
+        assert st.solve.fallback == 2 and st.solve.true_resid < 1.0
     ctx.close()
 
 
