@@ -433,6 +433,22 @@ def main():
                    "fallback": st.solve.fallback, "t_solve_s": st.t_solve,
                    "vs_two_level_step": tl / T,
                    "krylov_iters_two_level": last.solve.iters}
+        if a.mkyrl != 30:
+            # both preconditioners inside FGMRES(30) as well (the first step allocates the
+            # longer basis and is not timed)
+            f30 = {}
+            for name, pk, reps in (("two_level", opts["precond"], 2), ("learned", R.PREC_MPNN, 1)):
+                for k in range(reps):
+                    reset()
+                    t0 = time.perf_counter()
+                    dry_call()
+                    st = c.newton_step(**dict(opts, precond=pk, mkyrl=30))
+                    t30 = time.perf_counter() - t0
+                f30[name] = {"s_per_newton_step": t30, "krylov_iters": st.solve.iters,
+                             "converged": bool(st.solve.converged), "fallback": st.solve.fallback}
+                log(f"FGMRES(30) {name}: {t30:.2f}s ({st.solve.iters} it, fallback "
+                    f"{st.solve.fallback}, residual {st.solve.true_resid:.2e})")
+            learned["fgmres30"] = f30
     else:
         learned_reason = f"no trained weights at {a.learned}"
 
