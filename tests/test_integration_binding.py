@@ -44,7 +44,7 @@ int main(int argc, char**)
   GPU_KRYLOV* k = new GPU_KRYLOV; k->ctx = ctx; k->method = RMX_FGMRES; k->prec = RMX_F32;
   SOLVER* s = k;
   void* fns[] = { (void*)&gpu_newton_cycle, (void*)&gpu_solve_with_fallback,
-                  (void*)&gpu_time_loop };
+                  (void*)&gpu_time_loop, (void*)&gpu_load_learned };
   delete p; delete s;
   rmx_destroy(ctx);
   return fns[0] == nullptr;
@@ -60,7 +60,7 @@ def cpp_blocks():
 def test_integration_blocks_are_found():
     names = " ".join(cpp_blocks())
     for sym in ("class GPU_ILU0", "class GPU_KRYLOV", "struct GpuCrs", "gpu_newton_cycle",
-                "gpu_solve_with_fallback", "gpu_time_loop"):
+                "gpu_solve_with_fallback", "gpu_time_loop", "gpu_load_learned"):
         assert sym in names, sym
 
 
