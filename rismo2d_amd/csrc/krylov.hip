@@ -476,6 +476,9 @@ static bool fgmres(Ctx& c, int prec, int pk, const double* b, double* x, int m, 
     HIPCHECK(hipMemcpyAsync(d_rs.p, rs.data(), sizeof(double) * (i + 1), hipMemcpyHostToDevice, c.stream));
     k_comb<<<vb, 256, 0, c.stream>>>(n, i + 1, d_rs.p, Zb, n, x);
     HIPCHECK(hipStreamSynchronize(c.stream));
+    if (getenv("RMX_TRACE"))
+      fprintf(stderr, "[rmx] fgmres(%d) %d: cycle end, residual estimate %.6e of the start\n", m, iters,
+              ro / resid);
     if (ro <= eps1) { conv = true; break; }
     if (iters >= maxIter || !std::isfinite(ro)) { conv = false; break; }
     // a restart cycle that lowered the residual by less than a thousandth has stagnated
