@@ -470,6 +470,25 @@ __global__ void k_reduce(int nblk, int pairs, const float* __restrict__ partial,
   grad[m < mx ? offW + m * ng + o : offB + o] = s;
 }
 
+// first stage of the same sum for many blocks: part2[y][p] = sum of partial[b][p] over the
+// `per` consecutive blocks of slice y (four waves over interleaved blocks, combined in LDS in
+// a fixed order); 64 entries per workgroup
+__global__ void __launch_bounds__(256) k_reduce_slices(int nblk, int pairs, int per,
+                                                       const float* __restrict__ partial,
+                                                       float* __restrict__ part2)
+{
+  __shared__ float sh[4][64];
+  const int l = threadIdx.x & 63, w = threadIdx.x >> 6;
+  const int p = blockIdx.x * 64 + l;
+  const int b0 = blockIdx.y * per, b1 = min(nblk, b0 + per);
+  float s = 0.0f;
+  if (p < pairs)
+    for (int b = b0 + w; b < b1; b += 4) s += partial[(size_t)b * pairs + p];
+  sh[w][l] = s;
+  __syncthreads();
+  if (w == 0 && p < pairs) part2[(size_t)blockIdx.y * pairs + p] = ((sh[0][l] + sh[1][l]) + sh[2][l]) + sh[3][l];
+}
+
 __global__ void k_adam(int n, float* __restrict__ th, const float* __restrict__ g,
                        float* __restrict__ m, float* __restrict__ v, float lr, float b1, float b2,
                        float c1, float c2, float eps)
@@ -505,6 +524,23 @@ __global__ void k_add_v(int n, const double* __restrict__ nv, double* __restrict
 // host side
 
 static int gb(long long n, int b = 256) { return (int)((n + b - 1) / b); }
+
+// grad[map(p)] = sum over nblk blocks of partial[b][p], in a fixed order: slices of up to
+// 48 blocks summed first (k_reduce_slices), then the slices (k_reduce)
+static void reduce_blocks(Ctx& c, int nblk, int pairs, const float* partial, int mx, int ng,
+                          int offW, int offB, float* grad)
+{
+  constexpr int PER = 48;
+  const int ns = gb(nblk, PER);
+  if (ns <= 1) {
+    k_reduce<<<gb(pairs), 256, 0, c.stream>>>(nblk, pairs, partial, mx, ng, offW, offB, grad);
+    return;
+  }
+  DBuf<float>& P2 = c.mpnn.part2;
+  if (P2.n < (size_t)ns * pairs) P2.alloc((size_t)ns * pairs);
+  k_reduce_slices<<<dim3(gb(pairs, 64), ns), 256, 0, c.stream>>>(nblk, pairs, PER, partial, P2.p);
+  k_reduce<<<gb(pairs), 256, 0, c.stream>>>(ns, pairs, P2.p, mx, ng, offW, offB, grad);
+}
 
 // node graph: nodes sharing an element, sorted, no self loops
 static void build_graph(Ctx& c)
@@ -690,25 +726,22 @@ static void net_backward(Ctx& c, const float* feat)
                                     (size_t)nw * 2 * MC * MC));
   // output layer
   k_outer<uint16_t, float><<<nchunk, 256, 0, c.stream>>>(np, M.H.p + ML * nh, MC, M.gout.p, MO, M.partial.p);
-  k_reduce<<<gb((MC + 1) * MO), 256, 0, c.stream>>>(nchunk, (MC + 1) * MO, M.partial.p, MC, MO,
-                                                    OFF_WOUT, OFF_BOUT, M.grad.p);
+  reduce_blocks(c, nchunk, (MC + 1) * MO, M.partial.p, MC, MO, OFF_WOUT, OFF_BOUT, M.grad.p);
   k_gh_out<<<gb((long long)np * MC), 256, 0, c.stream>>>(np, M.gout.p, M.theta.p, M.gh.p);
   for (int l = ML - 1; l >= 0; l--) {
     k_gp<<<2048, 256, 0, c.stream>>>((long long)nh, M.gh.p, M.P.p + l * nh, M.gp.p);
     const int off = OFF_L0 + l * LAYER_SZ;
     k_wgrad<<<nw, 256, 0, c.stream>>>(np, M.H.p + l * nh, M.A.p + l * nh, M.gp.p, M.partial.p);
-    k_reduce<<<gb(2 * MC * MC), 256, 0, c.stream>>>(nw, 2 * MC * MC, M.partial.p, 2 * MC * MC, 1,
-                                                    off, off, M.grad.p);
+    reduce_blocks(c, nw, 2 * MC * MC, M.partial.p, 2 * MC * MC, 1, off, off, M.grad.p);
     k_outer<float, uint16_t><<<nchunk, 256, 0, c.stream>>>(np, nullptr, 0, M.gp.p, MC, M.partial.p);
-    k_reduce<<<1, 256, 0, c.stream>>>(nchunk, MC, M.partial.p, 0, MC, off, off + 2 * MC * MC, M.grad.p);
+    reduce_blocks(c, nchunk, MC, M.partial.p, 0, MC, off, off + 2 * MC * MC, M.grad.p);
     k_layer_bwd<<<gb(np, 64), 256, 0, c.stream>>>(np, M.gp.p, M.wb.p + (size_t)l * 2 * MC * MC,
                                                   M.gh.p, M.ga.p);
     k_aggT<<<gb((long long)np * 64), 256, 0, c.stream>>>(np, M.adj_ptr.p, M.adj.p, M.ga.p, M.gh.p);
   }
   k_gq<<<2048, 256, 0, c.stream>>>((long long)nh, M.gh.p, M.Q.p);
   k_outer<float, float><<<nchunk, 256, 0, c.stream>>>(np, feat, MF, M.gh.p, MC, M.partial.p);
-  k_reduce<<<gb((MF + 1) * MC), 256, 0, c.stream>>>(nchunk, (MF + 1) * MC, M.partial.p, MF, MC,
-                                                    OFF_WIN, OFF_BIN, M.grad.p);
+  reduce_blocks(c, nchunk, (MF + 1) * MC, M.partial.p, MF, MC, OFF_WIN, OFF_BIN, M.grad.p);
   HIPCHECK(hipGetLastError());
 }
 

@@ -266,7 +266,7 @@ struct Mpnn {
   DBuf<int> adj_ptr, adj, tptr, trow, tperm;  // node graph; transpose of the system pattern
   DBuf<float> theta, grad, m, v;
   DBuf<uint16_t> wt, wb, H, A, gp;
-  DBuf<float> P, Q, feat, out, gh, ga, gout, partial;
+  DBuf<float> P, Q, feat, out, gh, ga, gout, partial, part2;
   DBuf<double> lpart, ev, rv, s_r, rinv, q, acc;
   long long rinv_version = -1, trunk_version = -1;
   int use_net = -1;                           // per system: -1 undecided, 0 dropped, 1 used
