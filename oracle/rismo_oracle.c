@@ -16,9 +16,8 @@
  * (Elder) eddy viscosity, drying and rewetting by method 2, GPdeg 5 Gauss rules.
  *
  * Parity unpinned: in dry_rewet_method2, the count down of a dry node whose depth has
- * recovered (unreachable in a serial run, a dry node's level is reset to its bed) and the
- * marsh flag of a midside node lowered by less than dryLimit; the golden decks reach
- * neither branch.
+ * recovered (unreachable in a serial run, a dry node's level is reset to its bed); no
+ * golden deck reaches it.
  */
 #include "oracle.h"
 

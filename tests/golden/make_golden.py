@@ -65,6 +65,13 @@ the reference's input format, runs the reference and stores what the harness rec
                     final state and the final dry / marsh flags.
 * ``cap20x4.npz``   the 20 x 4 triangle channel with a binding relaxation cap ($RELAX 2
                     with maxDeltaUV 0.02, maxDeltaS 0.005): every Newton step's records.
+* ``marshmid20x6.npz`` a flat channel (20 x 6 cells, no bed noise) whose corner row at
+                    y = 20 m lies half the dry limit (0.05 m) below the still water level and
+                    whose rows above are dry land: the first drying call lowers that row's
+                    marsh corners by half the dry limit, so the midside nodes beside them
+                    sit a quarter or half of the dry limit below their original bed, inside
+                    the band (dryLimit / 10, dryLimit) where the midside marsh flag is set
+                    only by the reference's dryLimit / 10 test: every Newton step's records.
 
 The reference ships no test vectors of its own; these are the only golden data.
 Usage:  python tests/golden/make_golden.py [deck names]
@@ -134,6 +141,14 @@ DECKS = {
                         Q=10.0),
     "restart20x4": dict(nx=20, ny=4, length=100.0, width=20.0, noise_sigma=0.05, noise_corr=5.0,
                         Q=10.0),
+    "marshmid20x6": dict(nx=20, ny=6, length=100.0, width=30.0, noise_sigma=0.0, Q=10.0),
+}
+
+# corner rows set to a bed level relative to the still water level S0: (y of the row, bed
+# above S0); rows beyond the last listed one keep the last offset, rows before the first
+# keep the generator's bed.  Midside nodes are the mean of their corners.
+SHELF = {
+    "marshmid20x6": [(20.0, -0.025), (25.0, 0.5)],
 }
 
 # a bank that rises out of the water: the bed goes up linearly across the width from y0 to
@@ -196,6 +211,7 @@ RUNS = {
     # with the relaxed time step as the step's length, the rest of the interval follows
     "restart20x4": dict(relax="3 0.001 0.1 0.005 0.002 0.1", maxiter=4, tm_steps=2, dt=30.0,
                         theta=0.7, stationary=0, relax_time=0.5),
+    "marshmid20x6": dict(dryrew="2 3 0.05 0.1 100 5", elem_step=1),
 }
 
 
@@ -254,6 +270,20 @@ def main():
             y0, y1, rise = BANKS[name]
             z = d.z + rise * np.clip((d.y - y0) / (y1 - y0), 0.0, 1.0)
             # midside nodes: the mean of their corners, as the region reader sets them
+            for e in range(d.n_elems):
+                nnd = int(d.nnd[e])
+                ncn = nnd // 2
+                for i in range(ncn, nnd):
+                    left, right = d.elems[e, i - ncn], d.elems[e, (i - ncn + 1) % ncn]
+                    z[d.elems[e, i]] = 0.5 * (z[left] + z[right])
+            d.z = z
+        if name in SHELF:
+            z = d.z.copy()
+            rows = SHELF[name]
+            for k, (y_row, dz) in enumerate(rows):
+                y_next = rows[k + 1][0] if k + 1 < len(rows) else np.inf
+                sel = (d.y >= y_row - 1e-9) & (d.y < y_next - 1e-9)
+                z[sel] = d.S0[sel] + dz
             for e in range(d.n_elems):
                 nnd = int(d.nnd[e])
                 ncn = nnd // 2

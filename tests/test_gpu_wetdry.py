@@ -4,7 +4,9 @@ records, through the C ABI.
 The decks: ``wetdry20x6`` (a bank rising out of the water: the far bank dries at the first
 call, the strip beside it becomes marsh, the wet/dry line a slip boundary) and
 ``rewet20x6`` (the water rises from below the outlet level: elements of the bank get wet at
-a later call and the equations grow from 937 to 1115).  The runner's time loop calls the
+a later call and the equations grow from 937 to 1115) and ``marshmid20x6`` (marsh corners
+lowered by half the dry limit: 61 midside nodes between dryLimit / 10 and dryLimit below
+their original bed take the marsh flag).  The runner's time loop calls the
 drying and rewetting on the reference's schedule (EqsUVS2D.cpp:200-222); every Newton step
 applies the reference's own correction (rmx_set_solution), so that the device's state
 follows the reference's trajectory and each drying pass sees the reference's input.
@@ -41,7 +43,7 @@ def _rel(a, b, scale):
     return float(np.abs(np.asarray(a) - np.asarray(b)).max() / max(scale, 1e-300))
 
 
-@pytest.mark.parametrize("name", ["wetdry20x6", "rewet20x6"])
+@pytest.mark.parametrize("name", ["wetdry20x6", "rewet20x6", "marshmid20x6"])
 def test_device_drying_follows_reference_records(ctx, name):
     print(f"[gpu] drying and rewetting on the device, {name}", flush=True)
     g = load_golden(name)
