@@ -72,6 +72,12 @@ the reference's input format, runs the reference and stores what the harness rec
                     sit a quarter or half of the dry limit below their original bed, inside
                     the band (dryLimit / 10, dryLimit) where the midside marsh flag is set
                     only by the reference's dryLimit / 10 test: every Newton step's records.
+* ``shelfu20x6.npz`` the flat channel with a shelf 0.1 m below the outlet level (dry land
+                    above it), filling from 1.86 m as an unsteady flow (4 x 30 s, theta 0.7,
+                    drying every 3 Newton steps of every time step): the shelf, dry at the
+                    end of the first time step, gets wet during the second, where Timegrad
+                    zeroes the time derivatives of nodes dry or marsh at the previous time
+                    step: every Newton step's records.
 
 The reference ships no test vectors of its own; these are the only golden data.
 Usage:  python tests/golden/make_golden.py [deck names]
@@ -142,6 +148,7 @@ DECKS = {
     "restart20x4": dict(nx=20, ny=4, length=100.0, width=20.0, noise_sigma=0.05, noise_corr=5.0,
                         Q=10.0),
     "marshmid20x6": dict(nx=20, ny=6, length=100.0, width=30.0, noise_sigma=0.0, Q=10.0),
+    "shelfu20x6": dict(nx=20, ny=6, length=100.0, width=30.0, noise_sigma=0.0, Q=10.0),
 }
 
 # corner rows set to a bed level relative to the still water level S0: (y of the row, bed
@@ -149,6 +156,7 @@ DECKS = {
 # keep the generator's bed.  Midside nodes are the mean of their corners.
 SHELF = {
     "marshmid20x6": [(20.0, -0.025), (25.0, 0.5)],
+    "shelfu20x6": [(20.0, -0.1), (25.0, 0.5)],
 }
 
 # a bank that rises out of the water: the bed goes up linearly across the width from y0 to
@@ -166,6 +174,7 @@ BANKS = {
 # marsh strip and the dry bank above it get wet
 INIT_S = {
     "rewet20x6": 1.5,
+    "shelfu20x6": 1.86,
 }
 
 # inlet hydrographs (QTINLET), rows (t s, Q m3/s, S m)
@@ -212,6 +221,8 @@ RUNS = {
     "restart20x4": dict(relax="3 0.001 0.1 0.005 0.002 0.1", maxiter=4, tm_steps=2, dt=30.0,
                         theta=0.7, stationary=0, relax_time=0.5),
     "marshmid20x6": dict(dryrew="2 3 0.05 0.1 100 5", elem_step=1),
+    "shelfu20x6": dict(dryrew="2 3 0.05 0.1 100 1", maxiter=8, tm_steps=4, dt=30.0, theta=0.7,
+                       stationary=0, elem_step=6),
 }
 
 
