@@ -61,3 +61,25 @@ def test_hydrograph_lookup_matches_oracle():
                   [300.0, 12.0, 2.2]])
     for t in np.concatenate([np.linspace(-10.0, 320.0, 331), [60.0, 120.0, 120.06, 300.2]]):
         assert rmx.hydrograph(h, t) == po.hydrograph(h, t), t
+
+
+def test_shipped_learned_weights_fit_the_network():
+    """The weights shipped with the package (rismo2d_amd/mpnn_meander.npy) load without
+    pickle, are float32 with the library's parameter count in the layout of mpnn.shapes(),
+    finite, with a trained (non-zero) output layer; the record beside them lists the 64
+    training seeds and the samples harvested."""
+    import json
+    import os
+    from rismo2d_amd import mpnn
+    here = os.path.dirname(mpnn.__file__)
+    theta = np.load(os.path.join(here, "mpnn_meander.npy"))
+    n = rmx.Context.mpnn_n_params()
+    assert theta.dtype == np.float32 and theta.shape == (n,)
+    assert sum(int(np.prod(sh)) for _, sh in mpnn.shapes()) == n
+    assert np.isfinite(theta).all()
+    k0 = sum(int(np.prod(sh)) for name, sh in mpnn.shapes() if name not in ("W_out", "b_out"))
+    assert np.abs(theta[k0:]).max() > 0.0          # the initialisation's output layer is zero
+    rec = json.load(open(os.path.join(here, "mpnn_meander.json")))
+    seeds = [r["seed"] for r in rec["seeds"]]
+    assert seeds == list(range(1, 65))
+    assert sum(r.get("samples", 0) for r in rec["seeds"]) == 2640
