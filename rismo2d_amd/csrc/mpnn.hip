@@ -868,6 +868,9 @@ int mpnn_add_sample(Ctx& c, const double* r, int capacity)
     M.cap = capacity;
     M.s_r.alloc((size_t)capacity * n);
   }
+  if (M.cap == 0)
+    fail(RMX_ERR_ARG, "learned preconditioner: no training set (the first sample, and the first "
+                      "after a change of mesh topology or wet set, passes a capacity)");
   if (M.nsamp >= M.cap) fail(RMX_ERR_RANGE, "learned preconditioner: training set full");
   if (M.s_r.n != (size_t)M.cap * n)
     fail(RMX_ERR_ARG, "learned preconditioner: the training set holds systems of another size");
