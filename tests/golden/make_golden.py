@@ -72,6 +72,9 @@ the reference's input format, runs the reference and stores what the harness rec
                     sit a quarter or half of the dry limit below their original bed, inside
                     the band (dryLimit / 10, dryLimit) where the midside marsh flag is set
                     only by the reference's dryLimit / 10 test: every Newton step's records.
+* ``wetdryq10x3.npz`` the bank of ``wetdry20x6`` on 10 x 3 eight-node quadrilaterals (the
+                    bank rising 4.5 m from y = 10 m): the far third dry, the middle third
+                    marsh, drying every 3 Newton steps: every Newton step's records.
 * ``shelfu20x6.npz`` the flat channel with a shelf 0.1 m below the outlet level (dry land
                     above it), filling from 1.86 m as an unsteady flow (4 x 30 s, theta 0.7,
                     drying every 3 Newton steps of every time step): the shelf, dry at the
@@ -149,6 +152,8 @@ DECKS = {
                         Q=10.0),
     "marshmid20x6": dict(nx=20, ny=6, length=100.0, width=30.0, noise_sigma=0.0, Q=10.0),
     "shelfu20x6": dict(nx=20, ny=6, length=100.0, width=30.0, noise_sigma=0.0, Q=10.0),
+    "wetdryq10x3": dict(nx=10, ny=3, length=100.0, width=30.0, noise_sigma=0.05, noise_corr=5.0,
+                        Q=10.0, quads=True, vt=0.04),
 }
 
 # corner rows set to a bed level relative to the still water level S0: (y of the row, bed
@@ -168,6 +173,7 @@ BANKS = {
     "rewet20x6": (15.0, 30.0, 2.4),
     "wetdryu20x6": (15.0, 30.0, 4.5),
     "flooddry200x6": (60.0, 100.0, 2.6),
+    "wetdryq10x3": (10.0, 30.0, 4.5),
 }
 
 # initial water level below the outlet's: the level rises over the Newton steps and the
@@ -223,6 +229,7 @@ RUNS = {
     "marshmid20x6": dict(dryrew="2 3 0.05 0.1 100 5", elem_step=1),
     "shelfu20x6": dict(dryrew="2 3 0.05 0.1 100 1", maxiter=8, tm_steps=4, dt=30.0, theta=0.7,
                        stationary=0, elem_step=6),
+    "wetdryq10x3": dict(dryrew="2 3 0.001 0.005 100 5", elem_step=1),
 }
 
 

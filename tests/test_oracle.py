@@ -112,7 +112,7 @@ def test_oracle_matches_reference_every_record(name):
     _check_state(om, g, "final_")
 
 
-@pytest.mark.parametrize("name", ["wetdry20x6", "rewet20x6", "marshmid20x6"])
+@pytest.mark.parametrize("name", ["wetdry20x6", "rewet20x6", "marshmid20x6", "wetdryq10x3"])
 def test_oracle_wet_dry_matches_reference_every_record(name):
     """A bank that rises out of the water, drying and rewetting (method 2) at Newton step 0
     and then every few steps: the reference's dry and marsh node flags, lowered bed,
@@ -121,7 +121,9 @@ def test_oracle_wet_dry_matches_reference_every_record(name):
     dry, the strip beside it marsh; rewet20x6: the water rises from below the outlet level
     and the dry bank gets wet; marshmid20x6: marsh corners lowered by half the dry limit,
     so 61 midside nodes lie between dryLimit / 10 and dryLimit below their original bed
-    and take the marsh flag by the reference's dryLimit / 10 test."""
+    and take the marsh flag by the reference's dryLimit / 10 test; wetdryq10x3: the bank
+    on eight-node quadrilaterals (four corners per element in the element and midside
+    passes)."""
     g = load_golden(name)
     method, freq, dry_limit, rewet_limit, _passes, count_down = g["dryrew"]
     om = po.OracleModel(golden_deck(g))
@@ -322,7 +324,7 @@ def test_unconverged_cycle_stops_at_the_newton_limit():
     _check_state(om, g, "final_")
 
 
-@pytest.mark.parametrize("name", ["wetdry20x6", "rewet20x6", "marshmid20x6"])
+@pytest.mark.parametrize("name", ["wetdry20x6", "rewet20x6", "marshmid20x6", "wetdryq10x3"])
 def test_runner_time_loop_dries_and_rewets_as_the_reference(name):
     """The runner's time loop calls the drying and rewetting where the reference's flow
     cycle does (EqsUVS2D.cpp:200-222: before the first Newton step, then every dryRewFreq
