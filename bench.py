@@ -505,12 +505,14 @@ def main():
                              f"segments, block cyclic reduction on the device) + strip ILU(0) ({a.strips or '2 per CU'} "
                              f"strips x {a.segments or 'about 560-row'} segments, dataflow "
                              f"factorisation), FP64 "
-                             f"iterative refinement to 1e-8 (inner tolerance {a.eps:g})",
+                             f"iterative refinement to 1e-8 (inner tolerance {a.eps:g}; a "
+                             f"sweep that stalls hands the remaining ones to BiCGStab)",
                    "parallelism": "1 GPU"},
         "converged": bool(last.solve.converged),
         "true_resid": last.solve.true_resid,
         "krylov_iters": last.solve.iters,
         "refine_sweeps": last.solve.sweeps,
+        "fallback": last.solve.fallback,
         "t_factor_s": last.t_factor,
         "t_solve_s": last.t_solve,
         "newton_steps_per_s": 1.0 / T,
